@@ -1,0 +1,158 @@
+"""ctypes binding of libfluxhip.so (declarations in include/fluxhip.h).
+
+The library is the only compute path: if it is missing or cannot be loaded the
+import of any solver raises, there is no CPU fallback.
+"""
+import ctypes as C
+from pathlib import Path
+
+LIB_PATH = Path(__file__).resolve().parent / "libfluxhip.so"
+
+FX_OK, FX_EINVAL, FX_EDEVICE, FX_ESTATE = 0, -1, -2, -3
+FX_F32, FX_BF16, FX_F16 = 0, 1, 2
+FX_OP_BOX, FX_OP_FLAGS, FX_OP_EDGES = 0, 1, 2
+FX_RESTRICT_MEAN, FX_RESTRICT_FULL = 0, 1
+FX_PROLONG_TRILINEAR, FX_PROLONG_RENORM, FX_PROLONG_CONSTANT = 0, 1, 2
+MAX_LEVELS = 16
+
+
+class GPUUnavailable(RuntimeError):
+    """Device failure (the reference's backend/device.py:11-12 name)."""
+
+
+class fx_grid(C.Structure):
+    _fields_ = [("nx", C.c_int32), ("ny", C.c_int32), ("nz", C.c_int32),
+                ("ex", C.c_float), ("ey", C.c_float), ("ez", C.c_float)]
+
+
+class fx_sources(C.Structure):
+    _fields_ = [("count", C.c_int32), ("multi", C.c_int32), ("table", C.c_float * 128)]
+
+
+class fx_mg_level(C.Structure):
+    _fields_ = [("nx", C.c_int32), ("ny", C.c_int32), ("nz", C.c_int32), ("op", C.c_int32),
+                ("w", C.c_float * 3), ("pad_", C.c_float), ("op_data", C.c_void_p),
+                ("p", C.c_void_p), ("q", C.c_void_p), ("rhs", C.c_void_p)]
+
+
+class fx_mg_config(C.Structure):
+    _fields_ = [("levels", C.c_int32), ("storage", C.c_int32), ("pre", C.c_int32), ("post", C.c_int32),
+                ("omega", C.c_float), ("restriction", C.c_int32), ("prolongation", C.c_int32),
+                ("coarse_sweeps", C.c_int32), ("coarse_matrix", C.c_void_p), ("coarse_scale", C.c_float),
+                ("pad_", C.c_int32), ("level", fx_mg_level * MAX_LEVELS)]
+
+
+class fx_solve_info(C.Structure):
+    _fields_ = [("iterations", C.c_int32), ("converged", C.c_int32), ("fallback", C.c_int32),
+                ("pad_", C.c_int32), ("bnorm", C.c_double), ("residual0", C.c_double),
+                ("residual", C.c_double)]
+
+
+class fx_velocity_args(C.Structure):
+    _fields_ = [("g", fx_grid), ("dt", C.c_float), ("maccormack", C.c_int32),
+                ("u", C.c_void_p), ("v", C.c_void_p), ("w", C.c_void_p),
+                ("pu", C.c_void_p), ("pv", C.c_void_p), ("pw", C.c_void_p),
+                ("ou", C.c_void_p), ("ov", C.c_void_p), ("ow", C.c_void_p),
+                ("density", C.c_void_p), ("temperature", C.c_void_p),
+                ("thermal_lift", C.c_float), ("density_weight", C.c_float),
+                ("solid", C.c_void_p), ("sources", C.POINTER(fx_sources))]
+
+
+class fx_scalar_args(C.Structure):
+    _fields_ = [("g", fx_grid), ("dt", C.c_float), ("maccormack", C.c_int32), ("channels", C.c_int32),
+                ("boundary", C.c_int32), ("u", C.c_void_p), ("v", C.c_void_p), ("w", C.c_void_p),
+                ("field", C.c_void_p * 3), ("pred", C.c_void_p * 3), ("out", C.c_void_p * 3),
+                ("decay", C.c_float * 3), ("channel_id", C.c_int32 * 3), ("solid", C.c_void_p),
+                ("sources", C.POINTER(fx_sources))]
+
+
+P = C.c_void_p
+I32, I64, F32 = C.c_int32, C.c_int64, C.c_float
+
+SIGNATURES = {
+    "fx_last_error": ([], C.c_char_p),
+    "fx_version": ([], C.c_int),
+    "fx_solid_raster": ([fx_grid, I32, P, P, P], C.c_int),
+    "fx_build_flags": ([fx_grid, P, P, P], C.c_int),
+    "fx_seed": ([fx_grid, I32, P, P, P], C.c_int),
+    "fx_solid_clear": ([fx_grid, P, P, P, P], C.c_int),
+    "fx_fill": ([P, I64, F32, P], C.c_int),
+    "fx_scale": ([P, P, I64, F32, P], C.c_int),
+    "fx_enforce_walls": ([fx_grid, P, P, P, P, P, P, P, P], C.c_int),
+    "fx_divergence": ([fx_grid, P, P, P, P, P, P], C.c_int),
+    "fx_jacobi_sweep": ([fx_grid, P, P, P, P, F32, F32, P], C.c_int),
+    "fx_gradient": ([fx_grid, P, P, P, P, P, P, P, P, F32, P], C.c_int),
+    "fx_mg_vcycle": ([C.POINTER(fx_mg_config), F32, P], C.c_int),
+    "fx_mg_smooth": ([C.POINTER(fx_mg_config), I32, I32, F32, P], C.c_int),
+    "fx_mg_residual": ([C.POINTER(fx_mg_config), I32, F32, P, P], C.c_int),
+    "fx_mg_residual_restrict": ([C.POINTER(fx_mg_config), I32, F32, P], C.c_int),
+    "fx_mg_restrict": ([C.POINTER(fx_mg_config), I32, P, P, P], C.c_int),
+    "fx_mg_prolong": ([C.POINTER(fx_mg_config), I32, P, P, P, P], C.c_int),
+    "fx_mg_direct": ([C.POINTER(fx_mg_config), P], C.c_int),
+    "fx_project_mean": ([fx_grid, P, P, P, P], C.c_int),
+    "fx_residual_norm": ([fx_grid, P, P, P, C.POINTER(C.c_double), C.POINTER(C.c_double), P, P], C.c_int),
+    "fx_mg_solve": ([C.POINTER(fx_mg_config), F32, I32, P, C.POINTER(fx_solve_info), P], C.c_int),
+    "fx_face_predict": ([fx_grid, F32, P, P, P, P, P, P, P, P], C.c_int),
+    "fx_velocity_advect": ([C.POINTER(fx_velocity_args), P], C.c_int),
+    "fx_scalar_predict": ([fx_grid, F32, P, P, P, I32, C.POINTER(P), C.POINTER(P), P, P], C.c_int),
+    "fx_scalar_advect": ([C.POINTER(fx_scalar_args), P], C.c_int),
+    "fx_combustion": ([I64, F32, F32, F32, F32, F32, P, P, P, P, P, P, P, P], C.c_int),
+    "fx_max_abs": ([I32, C.POINTER(P), C.POINTER(I64), P, P], C.c_int),
+}
+
+_lib = None
+
+
+def load(path=None):
+    """Load libfluxhip.so once and attach the signatures."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    path = Path(path or LIB_PATH)
+    if not path.exists():
+        raise GPUUnavailable(f"{path} is not built; run fluxfx_amd.build.build() first")
+    lib = C.CDLL(str(path))
+    for name, (args, res) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.argtypes = args
+        fn.restype = res
+    _lib = lib
+    return lib
+
+
+def exported_names():
+    return list(SIGNATURES)
+
+
+def check(code):
+    """Map a C-ABI status to the reference's exception types."""
+    if code == FX_OK:
+        return
+    msg = _lib.fx_last_error().decode(errors="replace") if _lib is not None else "libfluxhip error"
+    if code == FX_EINVAL:
+        raise ValueError(msg)
+    if code == FX_ESTATE:
+        raise RuntimeError(msg)
+    raise GPUUnavailable(msg)
+
+
+def call(name, *args):
+    lib = load()
+    check(getattr(lib, name)(*args))
+
+
+def make_grid(shape, extent):
+    return fx_grid(int(shape[0]), int(shape[1]), int(shape[2]),
+                   float(extent[0]), float(extent[1]), float(extent[2]))
+
+
+def make_sources(rows, multi):
+    s = fx_sources()
+    s.count = len(rows)
+    s.multi = 1 if multi else 0
+    flat = [float(v) for row in rows for v in row]
+    if len(rows) > 8:
+        raise ValueError("At most eight sources are supported")
+    for i, v in enumerate(flat):
+        s.table[i] = v
+    return s

@@ -1,0 +1,232 @@
+"""Closed-box smoke/fire step with pressure projection.
+
+Drop-in for DenseProjectedSmoke (backend/projected.py:13-229): same
+constructor (grid, settings, device, projector_class, colliders), reset/upload/
+read, update_settings, measure_projection, step(dt, emission, sources), fault
+flag.  Step order (backend/projected.py:149-202):
+
+  1. face predictors (velocity MacCormack)
+  2. fused velocity transport + jet relaxation + buoyancy from the
+     pre-transport density/temperature            -> provisional U, V, W
+  3. projection into the old velocity storage
+  4. fused multi-channel scalar predictor (scalar MacCormack)
+  5. fused scalar transport: reverse trace, correction, clamp, decay, sources
+  6. fused combustion
+"""
+import ctypes as C
+
+from .. import _lib
+from ..physics.config import GridSpec, validate_dt
+from ..physics.emission import Emission, pack_sources, MAX_SOURCES
+from ..physics.interaction import validate_live_update
+from ..physics.pressure import PressureSettings
+from .collision import SolidMask
+from .combustion import CombustionPasses
+from .detail import DetailPasses
+from .device import HipDevice, ptr
+from .mac import DenseThermalAdvection, source_row
+from .multigrid import MultigridPressureProjector, use_multigrid
+from .pressure import PressureProjector, SolidPressureProjector
+from .solid_multigrid import SolidMultigridProjector
+from .tolerance import ToleranceProjector
+
+
+def select_projector(grid, settings, solids):
+    """Projector choice of backend/projected.py:14-43, plus the tolerance mode."""
+    if settings.pressure_tolerance > 0:
+        return ToleranceProjector
+    if solids is not None:
+        return SolidMultigridProjector if use_multigrid(grid, settings) else SolidPressureProjector
+    return MultigridPressureProjector if use_multigrid(grid, settings) else PressureProjector
+
+
+class DenseProjectedSmoke(DenseThermalAdvection):
+    def __init__(self, grid=None, settings=None, device=None, projector_class=None, *, colliders=(),
+                 solid_mask=None):
+        self.combustion = None
+        self.solids = None
+        self.projector = None
+        self.detail = None
+        self.faulted = False
+        self._source_data = None
+        self._multi_sources = None
+        settings = settings or PressureSettings()
+        grid = grid or GridSpec()
+        device = device or HipDevice()
+        if settings.moving_colliders:
+            raise ValueError("Moving colliders are not part of this build")
+        if colliders or solid_mask is not None:
+            self.solids = SolidMask(grid, colliders, device, mask=solid_mask)
+            if settings.pressure_tolerance <= 0:
+                projector_class = None
+        super().__init__(grid, settings, device)
+        if projector_class is None:
+            projector_class = select_projector(self.grid, self.settings, self.solids)
+        self.projector = projector_class(self.grid, self.settings, self.device, solids=self.solids)
+        self.detail = DetailPasses(self.grid, self.device, self.solids)
+        if self.settings.combustion_enabled:
+            self.combustion = CombustionPasses(self.grid, self.device)
+        self.reset()
+
+    @property
+    def allocated_bytes(self):
+        return (super().allocated_bytes + self.projector.allocated_bytes + self.detail.allocated_bytes +
+                (self.combustion.allocated_bytes if self.combustion else 0) +
+                (self.solids.allocated_bytes if self.solids else 0))
+
+    def reset(self, seed=True):
+        super().reset(seed)
+        self.faulted = False
+        if self.combustion:
+            self.combustion.reset()
+        if self.solids is not None:
+            self.solids.clear_solid(self._front, self._back)
+            self._front, self._back = self._back, self._front
+            if self._temperature is not None:
+                self.solids.clear_solid(self._temperature, self._temperature_back)
+                self._temperature, self._temperature_back = self._temperature_back, self._temperature
+        if self.projector is not None:
+            self.projector.reset()
+            self.projector.enforce_walls(self._velocity, self._velocity_back)
+            self._velocity, self._velocity_back = self._velocity_back, self._velocity
+
+    def upload_velocity(self, components):
+        super().upload_velocity(components)
+        if self.solids is not None:
+            self.projector.enforce_walls(self._velocity, self._velocity_back)
+            self._velocity, self._velocity_back = self._velocity_back, self._velocity
+        self.projector.reset()
+
+    def upload(self, values):
+        super().upload(values)
+        if self.solids is not None:
+            self.solids.clear_solid(self._front, self._back)
+            self._front, self._back = self._back, self._front
+        self.projector.reset()
+
+    def upload_temperature(self, values):
+        super().upload_temperature(values)
+        if self.solids is not None:
+            self.solids.clear_solid(self._temperature, self._temperature_back)
+            self._temperature, self._temperature_back = self._temperature_back, self._temperature
+        self.projector.reset()
+
+    def upload_fuel(self, values):
+        if not self.combustion:
+            raise ValueError("Enable combustion and Reset for fuel")
+        self.combustion.fuel = self.device.texture(self.grid.shape, values)
+
+    def read_fuel(self):
+        if not self.combustion:
+            raise ValueError("Enable combustion and Reset for fuel")
+        return self.device.read(self.combustion.fuel)
+
+    def update_settings(self, settings):
+        """Apply live parameters between steps, keeping fields and pressure."""
+        validate_live_update(self.settings, settings)
+        self.settings = settings
+        self.projector.settings = settings
+
+    def move_colliders(self, colliders, dt):
+        raise ValueError("Enable Moving colliders and Reset first")
+
+    def measure_projection(self):
+        return self.projector.metrics() | {"step": self.steps, "simulation_time": self.time}
+
+    def step(self, dt=1.0 / 30.0, emission=None, sources=None):
+        validate_dt(dt)
+        if sources is not None:
+            self._prepare_sources(sources)
+        if self.faulted:
+            raise RuntimeError("Reset required after a failed GPU step")
+        try:
+            self._step(dt, emission or Emission(self.settings.source_center), sources)
+        except Exception:
+            self.faulted = True
+            self.projector.ready = False
+            raise
+
+    def _prepare_sources(self, sources):
+        data = pack_sources(sources)
+        if data != self._source_data:
+            rows = [data[16 * i:16 * i + 16] for i in range(len(sources))]
+            self._multi_sources = _lib.make_sources(rows, multi=True)
+            self._source_data = data
+
+    def _single_source(self, emission):
+        s = self.settings
+        row = source_row(s.source_center, s.source_radius, s.source_rate, s.heat_source_rate, emission.start,
+                         emission.velocity, emission.coupling, s.density_mode == "TARGET",
+                         s.source_profile == "SOLID", s.fuel_source_rate)
+        return _lib.make_sources([row], multi=False)
+
+    def _step(self, dt, emission, emitters=None):
+        s = self.settings
+        if s.turbulence_strength > 0 or s.vorticity_strength > 0:
+            raise NotImplementedError("turbulence and vorticity confinement are not part of this build")
+        table = self._multi_sources if emitters is not None else self._single_source(emission)
+        solid = self.solids.solid if self.solids is not None else None
+        velocity_detail = s.velocity_advection == "MACCORMACK"
+        preds = self.detail.transport_velocity(self._velocity, dt) if velocity_detail else None
+        provisional = [self._velocity_back[0], self._velocity_back[1], self._buoyancy_scratch]
+        self._advect_velocity(self._velocity, provisional, dt, maccormack=velocity_detail, predictors=preds,
+                              sources=table, density=self._front, temperature=self._temperature,
+                              lift=s.thermal_lift, weight=s.density_weight, solid=solid)
+        # the old velocity is consumed: its storage receives the projected field
+        self.projector.project(provisional, self._velocity, dt)
+        fields = [self._front, self._temperature]
+        outs = [self._back, self._temperature_back]
+        decays = [s.dissipation, s.cooling]
+        if self.combustion:
+            fields.append(self.combustion.fuel)
+            outs.append(self.combustion.spare)
+            decays.append(0.0)
+        corrected = s.scalar_advection == "MACCORMACK"
+        preds = self.detail.scalar_predictors(fields, self._velocity, dt) if corrected else None
+        args = _lib.fx_scalar_args()
+        args.g = self._g
+        args.dt = dt
+        args.maccormack = 1 if corrected else 0
+        args.channels = len(fields)
+        args.boundary = 0
+        args.u, args.v, args.w = (ptr(f) for f in self._velocity)
+        for k, (f, o, d) in enumerate(zip(fields, outs, decays)):
+            args.field[k] = f.ptr
+            args.out[k] = o.ptr
+            args.pred[k] = preds[k].ptr if corrected else None
+            args.decay[k] = d
+            args.channel_id[k] = k
+        args.solid = ptr(solid)
+        args.sources = C.pointer(table)
+        _lib.call("fx_scalar_advect", C.byref(args), self.device.stream)
+        self._front, self._back = self._back, self._front
+        self._temperature, self._temperature_back = self._temperature_back, self._temperature
+        if self.combustion:
+            self.combustion.fuel, self.combustion.spare = self.combustion.spare, self.combustion.fuel
+            self.combustion.react(self, dt)
+        self.time += dt
+        self.steps += 1
+
+    def preview_field(self, channel):
+        if channel == "COLLISION":
+            if self.solids is None:
+                raise ValueError("Add a collider and Reset for collision display")
+            return self.solids.mask
+        if channel in {"FUEL", "FLAME"}:
+            if not self.combustion:
+                raise ValueError("Enable combustion and Reset for fuel/flame display")
+            return self.combustion.fuel if channel == "FUEL" else self.combustion.flame
+        return self.temperature if channel == "TEMPERATURE" else self.density
+
+    def close(self):
+        if self.combustion:
+            self.combustion.close()
+        if self.projector is not None:
+            self.projector.close()
+        if self.detail is not None:
+            self.detail.close()
+        self.projector = self.detail = None
+        super().close()
+        if self.solids is not None:
+            self.solids.close()
+        self.combustion = self.solids = None

@@ -1,0 +1,136 @@
+"""Tolerance-driven projection (no reference counterpart; the configs ask for it).
+
+Solves the impulse equation L q = div to ||div - L q||_2 <= tol*||div||_2
+(div mean-projected over fluid cells, norms in FP64 of the FP32 residual),
+warm-started from the previous step's impulse rescaled by dt/dt_prev.
+
+V-cycle: ``mg_pre``/``mg_post`` damped-Jacobi sweeps (omega 2/3),
+8-child-mean or full-weighting restriction, trilinear prolongation
+(renormalised over active coarse cells next to solids), FP64 pseudo-inverse on
+the coarsest level.  Solid masks coarsen by conductance fractions: a coarse
+face conducts the mean of its four fine faces, a coarse cell is active when any
+child is fluid.
+
+Modes (``pressure_precision`` x ``pressure_krylov``):
+  FP32 + NONE   stationary V-cycles on the FP32 impulse
+  FP32 + PCG    flexible CG, FP32 V-cycle preconditioner
+  BF16/FP16 + NONE  iterative refinement: FP32 residual, V-cycle on the error
+                in 16-bit storage with FP32 arithmetic, FP32 update
+  BF16/FP16 + PCG   flexible CG, 16-bit V-cycle preconditioner
+  LEARNED       flexible CG with the U-Net preconditioner; plain FP32 MG
+                finishes the solve whenever the true residual misses tol
+"""
+import numpy as np
+
+from .. import _lib
+from .engine import (MGLevels, coarsen_to, conductance_hierarchy, edges_matrix, STORAGE)
+from .pressure import PressureProjector
+
+
+def box_edges(shape, extent):
+    nx, ny, nz = shape
+    e = np.zeros((nz, ny, nx, 4))
+    idx = [np.arange(n) for n in (nx, ny, nz)]
+    for a in range(3):
+        inside = np.ones((nz, ny, nx), bool)
+        sl = [slice(None)] * 3
+        sl[(2, 1, 0)[a]] = -1
+        inside[tuple(sl)] = False
+        e[..., a] = inside * (shape[a] / extent[a]) ** 2
+    e[..., 3] = 1
+    return e
+
+
+class ToleranceProjector(PressureProjector):
+    solver_name = "TOLERANCE_MULTIGRID"
+
+    def __init__(self, grid, settings=None, device=None, *, solids=None):
+        super().__init__(grid, settings, device, solids=solids)
+        s = self.settings
+        if s.pressure_tolerance <= 0:
+            raise ValueError("ToleranceProjector needs pressure_tolerance > 0")
+        if s.mg_restriction == "FULL_WEIGHTING" and solids is not None:
+            raise ValueError("Full-weighting restriction is defined for unmasked grids only")
+        self.shapes = coarsen_to(grid.shape, s.mg_coarsest)
+        if len(self.shapes) < 2:
+            raise ValueError("Tolerance multigrid needs at least two levels")
+        self.levels = [{"shape": sh} for sh in self.shapes]
+        if solids is not None:
+            coarse_edges = conductance_hierarchy(self.shapes, grid.extent, solids.numpy())
+            self._edges = [self.device.torch.from_numpy(np.ascontiguousarray(e, np.float32).reshape(-1)).to(
+                self.device.device) for e in coarse_edges]
+            ops = [(_lib.FX_OP_FLAGS, solids.flags)] + [(_lib.FX_OP_EDGES, t) for t in self._edges]
+            last = coarse_edges[-1]
+            self.prolongation = _lib.FX_PROLONG_RENORM
+        else:
+            self._edges = []
+            ops = [(_lib.FX_OP_BOX, None)] * len(self.shapes)
+            last = box_edges(self.shapes[-1], grid.extent)
+            self.prolongation = _lib.FX_PROLONG_TRILINEAR
+        self.ops = ops
+        self.coarse_matrix = np.linalg.pinv(edges_matrix(last), rcond=1e-12, hermitian=True)
+        self.restriction = _lib.FX_RESTRICT_FULL if s.mg_restriction == "FULL_WEIGHTING" else _lib.FX_RESTRICT_MEAN
+        self.engine = self._engine(_lib.FX_F32, own_level0=False)
+        self.ws = self.device.zeros(16, "float64")
+        self.last_cycles = 0
+        self.last_info = None
+
+    def _engine(self, storage, own_level0):
+        s = self.settings
+        return MGLevels(self.device, self.shapes, self.grid.extent, storage, self.ops, pre=s.mg_pre, post=s.mg_post,
+                        omega=2 / 3, restriction=self.restriction, prolongation=self.prolongation,
+                        coarse_sweeps=0, coarse_matrix=self.coarse_matrix, coarse_scale=1.0,
+                        own_level0=own_level0)
+
+    @property
+    def uses_impulse(self):
+        return True
+
+    @property
+    def allocated_bytes(self):
+        return super().allocated_bytes + self.engine.allocated_bytes
+
+    def reset(self):
+        super().reset()
+        self.last_cycles = 0
+        for arr in self.engine.arrays[1:]:
+            for t in arr.values():
+                t.zero_()
+
+    def solve_pressure(self, dt, previous_dt):
+        if previous_dt != dt:
+            self.pressure.clear(format="FLOAT", value=(0.0,))
+        s = self.settings
+        _lib.call("fx_project_mean", self._g, self.flags_ptr, self.before.ptr, self.ws.data_ptr(),
+                  self.device.stream)
+        if s.pressure_precision == "FP32" and s.pressure_krylov == "NONE":
+            eng = self.engine
+            eng.bind_level0(self.pressure.tensor, self.spare.tensor, self.before.tensor)
+            info = eng.solve(s.pressure_tolerance, s.pressure_max_cycles, self.ws)
+            if eng.tensor(0, "p") is not self.pressure.tensor:
+                self.pressure, self.spare = self.spare, self.pressure
+        else:
+            raise NotImplementedError(f"{s.pressure_precision}/{s.pressure_krylov} solve")
+        self.last_info = info
+        self.last_cycles = info.iterations
+        self.last_iterations = info.iterations * (len(self.shapes) - 1) * (s.mg_pre + s.mg_post)
+        if not info.converged:
+            raise RuntimeError(f"pressure solve missed tolerance {s.pressure_tolerance:g}: "
+                               f"residual {info.residual / max(info.bnorm, 1e-300):.3e} after {info.iterations}")
+
+    def solve_metrics(self):
+        i = self.last_info
+        if i is None:
+            return {}
+        rel = i.residual / i.bnorm if i.bnorm > 0 else 0.0
+        return {"v_cycles": i.iterations, "converged": bool(i.converged), "relative_residual": rel,
+                "bnorm": i.bnorm, "residual0": i.residual0, "fallback": bool(i.fallback),
+                "precision": self.settings.pressure_precision, "krylov": self.settings.pressure_krylov,
+                "tolerance": self.settings.pressure_tolerance, "levels": [v["shape"] for v in self.levels]}
+
+    def metrics(self):
+        return super().metrics() | {"solver": self.solver_name} | self.solve_metrics()
+
+    def close(self):
+        self.engine = None
+        super().close()

@@ -1,0 +1,659 @@
+// Transport kernels: face predictor, fused velocity transport + jet relaxation
+// + buoyancy, multi-channel scalar predictor and fused reverse/correct/source
+// kernel, combustion and max-abs reduction.  One thread per face or cell;
+// trilinear gathers stay in L1/L2 (neighbouring lanes read neighbouring data).
+#include "common.hpp"
+
+namespace fx {
+
+struct GridDev {
+  int nx, ny, nz;
+  float ex, ey, ez;  // extent (m)
+};
+
+struct V3 {
+  float x, y, z;
+};
+__device__ __forceinline__ V3 v3(float x, float y, float z) { return V3{x, y, z}; }
+
+// ---------------------------------------------------------------------------
+// face sampling: q clamped to [0,size-1], corner index min(b+o,size-1)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ float sample_face(const float* __restrict__ F, int sx, int sy, int sz, float qx, float qy,
+                                             float qz) {
+  qx = fminf(fmaxf(qx, 0.0f), float(sx - 1));
+  qy = fminf(fmaxf(qy, 0.0f), float(sy - 1));
+  qz = fminf(fmaxf(qz, 0.0f), float(sz - 1));
+  int bx = int(floorf(qx)), by = int(floorf(qy)), bz = int(floorf(qz));
+  float fx = qx - bx, fy = qy - by, fz = qz - bz;
+  int x1 = min(bx + 1, sx - 1), y1 = min(by + 1, sy - 1), z1 = min(bz + 1, sz - 1);
+  int64_t r00 = int64_t(sx) * (by + int64_t(sy) * bz), r10 = int64_t(sx) * (y1 + int64_t(sy) * bz);
+  int64_t r01 = int64_t(sx) * (by + int64_t(sy) * z1), r11 = int64_t(sx) * (y1 + int64_t(sy) * z1);
+  float gx = 1.0f - fx, gy = 1.0f - fy, gz = 1.0f - fz;
+  float v = 0.0f;
+  v += gx * gy * gz * F[r00 + bx];
+  v += fx * gy * gz * F[r00 + x1];
+  v += gx * fy * gz * F[r10 + bx];
+  v += fx * fy * gz * F[r10 + x1];
+  v += gx * gy * fz * F[r01 + bx];
+  v += fx * gy * fz * F[r01 + x1];
+  v += gx * fy * fz * F[r11 + bx];
+  v += fx * fy * fz * F[r11 + x1];
+  return v;
+}
+
+struct Vel {
+  const float* u;
+  const float* v;
+  const float* w;
+};
+
+// velocityAt: q = p*cellCount/extent, three staggered samples
+__device__ __forceinline__ V3 velocity_at(const GridDev& g, const Vel& V, V3 p) {
+  float qx = p.x * g.nx / g.ex, qy = p.y * g.ny / g.ey, qz = p.z * g.nz / g.ez;
+  return v3(sample_face(V.u, g.nx + 1, g.ny, g.nz, qx, qy - 0.5f, qz - 0.5f),
+            sample_face(V.v, g.nx, g.ny + 1, g.nz, qx - 0.5f, qy, qz - 0.5f),
+            sample_face(V.w, g.nx, g.ny, g.nz + 1, qx - 0.5f, qy - 0.5f, qz));
+}
+
+// ---------------------------------------------------------------------------
+// solids
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ bool solid_cell(const GridDev& g, const uint8_t* __restrict__ s, int x, int y, int z) {
+  if (x < 0 || y < 0 || z < 0 || x >= g.nx || y >= g.ny || z >= g.nz) return true;
+  return s[idx3(x, y, z, g.nx, g.ny)] != 0;
+}
+
+// 3-D DDA clip of origin->target at the first solid cell.  A trace that is
+// neither clamped to the domain nor blocked returns `target` itself.
+__device__ V3 clip_trace(const GridDev& g, const uint8_t* __restrict__ s, V3 o, V3 t) {
+  const float cx = float(g.nx), cy = float(g.ny), cz = float(g.nz);
+  float ax = fminf(fmaxf(o.x / g.ex * cx, 1e-4f), cx - 1e-4f);
+  float ay = fminf(fmaxf(o.y / g.ey * cy, 1e-4f), cy - 1e-4f);
+  float az = fminf(fmaxf(o.z / g.ez * cz, 1e-4f), cz - 1e-4f);
+  float rbx = t.x / g.ex * cx, rby = t.y / g.ey * cy, rbz = t.z / g.ez * cz;
+  float bx = fminf(fmaxf(rbx, 1e-4f), cx - 1e-4f);
+  float by = fminf(fmaxf(rby, 1e-4f), cy - 1e-4f);
+  float bz = fminf(fmaxf(rbz, 1e-4f), cz - 1e-4f);
+  bool clamped = bx != rbx || by != rby || bz != rbz;
+  float dx = bx - ax, dy = by - ay, dz = bz - az;
+  int ix = int(floorf(ax)), iy = int(floorf(ay)), iz = int(floorf(az));
+  if (solid_cell(g, s, ix, iy, iz)) return o;
+  int sx = dx > 0.0f ? 1 : (dx < 0.0f ? -1 : 0);
+  int sy = dy > 0.0f ? 1 : (dy < 0.0f ? -1 : 0);
+  int sz = dz > 0.0f ? 1 : (dz < 0.0f ? -1 : 0);
+  float crx = 1e20f, cry = 1e20f, crz = 1e20f, stx = 1e20f, sty = 1e20f, stz = 1e20f;
+  if (fabsf(dx) > 1e-10f) { crx = (float(ix + (sx > 0 ? 1 : 0)) - ax) / dx; stx = 1.0f / fabsf(dx); }
+  if (fabsf(dy) > 1e-10f) { cry = (float(iy + (sy > 0 ? 1 : 0)) - ay) / dy; sty = 1.0f / fabsf(dy); }
+  if (fabsf(dz) > 1e-10f) { crz = (float(iz + (sz > 0 ? 1 : 0)) - az) / dz; stz = 1.0f / fabsf(dz); }
+  float len = sqrtf(dx * dx + dy * dy + dz * dz);
+  for (int i = 0; i < 384; ++i) {
+    int axis = (crx <= cry && crx <= crz) ? 0 : (cry <= crz ? 1 : 2);
+    float tt = axis == 0 ? crx : (axis == 1 ? cry : crz);
+    if (tt >= 1.0f) {
+      if (!clamped) return t;
+      return v3(bx / cx * g.ex, by / cy * g.ey, bz / cz * g.ez);
+    }
+    if (axis == 0) ix += sx; else if (axis == 1) iy += sy; else iz += sz;
+    if (solid_cell(g, s, ix, iy, iz)) {
+      float f = fmaxf(0.0f, tt - 1e-4f / fmaxf(len, 1.0f));
+      return v3((ax + dx * f) / cx * g.ex, (ay + dy * f) / cy * g.ey, (az + dz * f) / cz * g.ez);
+    }
+    if (axis == 0) crx += stx; else if (axis == 1) cry += sty; else crz += stz;
+  }
+  return o;
+}
+
+__device__ __forceinline__ bool solid_halo(const GridDev& g, const uint8_t* __restrict__ s, V3 p) {
+  int cx = int(floorf(p.x / g.ex * g.nx)), cy = int(floorf(p.y / g.ey * g.ny)), cz = int(floorf(p.z / g.ez * g.nz));
+  for (int z = -1; z <= 1; ++z)
+    for (int y = -1; y <= 1; ++y)
+      for (int x = -1; x <= 1; ++x)
+        if (solid_cell(g, s, cx + x, cy + y, cz + z)) return true;
+  return false;
+}
+
+__device__ __forceinline__ float dist3(V3 a, V3 b) {
+  float dx = a.x - b.x, dy = a.y - b.y, dz = a.z - b.z;
+  return sqrtf(dx * dx + dy * dy + dz * dz);
+}
+
+__device__ bool safe_correction(const GridDev& g, const uint8_t* __restrict__ s, V3 p, V3 dep, V3 mid, V3 arr) {
+  if (solid_halo(g, s, p) || solid_halo(g, s, dep) || solid_halo(g, s, arr)) return false;
+  if (dist3(clip_trace(g, s, p, dep), dep) >= 1e-7f) return false;
+  if (dist3(clip_trace(g, s, p, mid), mid) >= 1e-7f) return false;
+  return dist3(clip_trace(g, s, p, arr), arr) < 1e-7f;
+}
+
+// ---------------------------------------------------------------------------
+// emission: exact time-average of the compact sphere profile along the sweep
+// ---------------------------------------------------------------------------
+__device__ float emission_weight(V3 p, const float* row) {
+  V3 c = v3(row[0], row[1], row[2]);
+  float radius = row[3];
+  V3 st = v3(row[4], row[5], row[6]);
+  bool solid_profile = row[14] > 0.5f;
+  float sgx = c.x - st.x, sgy = c.y - st.y, sgz = c.z - st.z;
+  float l2 = sgx * sgx + sgy * sgy + sgz * sgz;
+  float r2 = radius * radius;
+  if (l2 < 1e-12f) {
+    float dx = p.x - c.x, dy = p.y - c.y, dz = p.z - c.z;
+    float d2 = dx * dx + dy * dy + dz * dz;
+    float w = fmaxf(1.0f - d2 / r2, 0.0f);
+    return solid_profile ? (d2 < r2 ? 1.0f : 0.0f) : w * w;
+  }
+  float lp = sqrtf(l2);
+  float axx = sgx / lp, axy = sgy / lp, axz = sgz / lp;
+  float ox = p.x - st.x, oy = p.y - st.y, oz = p.z - st.z;
+  float along = ox * axx + oy * axy + oz * axz;
+  float px = ox - along * axx, py = oy - along * axy, pz = oz - along * axz;
+  float a = 1.0f - (px * px + py * py + pz * pz) / r2;
+  if (a <= 0.0f) return 0.0f;
+  float hw = radius * sqrtf(a);
+  float lo = fmaxf(-along, -hw), hi = fminf(lp - along, hw);
+  if (hi <= lo) return 0.0f;
+  if (solid_profile) return (hi - lo) / lp;
+  float u = lo / hw, v = hi / hw;
+  float u3 = u * u * u, v3_ = v * v * v;
+  float integral = (v - u) - 2.0f / 3.0f * (v3_ - u3) + 0.2f * (v3_ * v * v - u3 * u * u);
+  return fminf(fmaxf(a * a * hw / lp * integral, 0.0f), 1.0f);
+}
+
+// ---------------------------------------------------------------------------
+// face predictor (transport_face)
+// ---------------------------------------------------------------------------
+template <int AXIS, bool SOLID>
+__global__ void k_face_predict(GridDev g, Vel V, float dt, const float* __restrict__ in, float* __restrict__ out,
+                               const uint8_t* __restrict__ solid) {
+  const int sx = g.nx + (AXIS == 0), sy = g.ny + (AXIS == 1), sz = g.nz + (AXIS == 2);
+  int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y * blockDim.y + threadIdx.y, z = blockIdx.z;
+  if (x >= sx || y >= sy) return;
+  int64_t i = idx3(x, y, z, sx, sy);
+  if (SOLID) {
+    int ox = AXIS == 0, oy = AXIS == 1, oz = AXIS == 2;
+    if (solid_cell(g, solid, x, y, z) || solid_cell(g, solid, x - ox, y - oy, z - oz)) {
+      out[i] = 0.0f;
+      return;
+    }
+  }
+  const float offx = AXIS == 0 ? 0.0f : 0.5f, offy = AXIS == 1 ? 0.0f : 0.5f, offz = AXIS == 2 ? 0.0f : 0.5f;
+  float hx = g.ex / g.nx, hy = g.ey / g.ny, hz = g.ez / g.nz;
+  V3 p = v3((x + offx) * hx, (y + offy) * hy, (z + offz) * hz);
+  V3 vp = velocity_at(g, V, p);
+  V3 mid = v3(p.x - 0.5f * dt * vp.x, p.y - 0.5f * dt * vp.y, p.z - 0.5f * dt * vp.z);
+  if (SOLID) mid = clip_trace(g, solid, p, mid);
+  V3 vm = velocity_at(g, V, mid);
+  V3 dep = v3(p.x - dt * vm.x, p.y - dt * vm.y, p.z - dt * vm.z);
+  if (SOLID) dep = clip_trace(g, solid, p, dep);
+  out[i] = sample_face(in, sx, sy, sz, dep.x / hx - offx, dep.y / hy - offy, dep.z / hz - offz);
+}
+
+// ---------------------------------------------------------------------------
+// fused velocity transport + jet relaxation + buoyancy (W)
+// ---------------------------------------------------------------------------
+struct VelArgs {
+  GridDev g;
+  Vel V;
+  float dt;
+  const float* pred;
+  float* out;
+  const float* density;
+  const float* temperature;
+  float lift, weight;
+  const uint8_t* solid;
+};
+
+template <int AXIS, bool SOLID, bool MC>
+__global__ void k_velocity(VelArgs A, fx_sources src) {
+  const GridDev& g = A.g;
+  const int sx = g.nx + (AXIS == 0), sy = g.ny + (AXIS == 1), sz = g.nz + (AXIS == 2);
+  int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y * blockDim.y + threadIdx.y, z = blockIdx.z;
+  if (x >= sx || y >= sy) return;
+  int64_t i = idx3(x, y, z, sx, sy);
+  const float* orig = AXIS == 0 ? A.V.u : (AXIS == 1 ? A.V.v : A.V.w);
+  const float offx = AXIS == 0 ? 0.0f : 0.5f, offy = AXIS == 1 ? 0.0f : 0.5f, offz = AXIS == 2 ? 0.0f : 0.5f;
+  float hx = g.ex / g.nx, hy = g.ey / g.ny, hz = g.ez / g.nz;
+  const float dt = A.dt;
+  float value = 0.0f;
+  bool blocked = false;
+  if (SOLID) {
+    int ox = AXIS == 0, oy = AXIS == 1, oz = AXIS == 2;
+    blocked = solid_cell(g, A.solid, x, y, z) || solid_cell(g, A.solid, x - ox, y - oy, z - oz);
+  }
+  V3 p = v3((x + offx) * g.ex / g.nx, (y + offy) * g.ey / g.ny, (z + offz) * g.ez / g.nz);
+  if (!blocked) {
+    V3 vp = velocity_at(g, A.V, p);
+    V3 mid = v3(p.x - 0.5f * dt * vp.x, p.y - 0.5f * dt * vp.y, p.z - 0.5f * dt * vp.z);
+    if (SOLID) mid = clip_trace(g, A.solid, p, mid);
+    V3 vm = velocity_at(g, A.V, mid);
+    V3 dep = v3(p.x - dt * vm.x, p.y - dt * vm.y, p.z - dt * vm.z);
+    if (SOLID) dep = clip_trace(g, A.solid, p, dep);
+    float qx = dep.x / hx - offx, qy = dep.y / hy - offy, qz = dep.z / hz - offz;
+    float low = sample_face(orig, sx, sy, sz, qx, qy, qz);
+    value = low;
+    if (MC) {
+      float lbx = offx * hx, lby = offy * hy, lbz = offz * hz;
+      float hbx = (float(sx - 1) + offx) * hx, hby = (float(sy - 1) + offy) * hy, hbz = (float(sz - 1) + offz) * hz;
+      V3 rmid = v3(p.x + 0.5f * dt * vp.x, p.y + 0.5f * dt * vp.y, p.z + 0.5f * dt * vp.z);
+      V3 va = velocity_at(g, A.V, rmid);
+      V3 arr = v3(p.x + dt * va.x, p.y + dt * va.y, p.z + dt * va.z);
+      bool ok = true;
+      if (SOLID) ok = safe_correction(g, A.solid, p, dep, rmid, arr);
+      ok = ok && !(dep.x < lbx || dep.y < lby || dep.z < lbz || dep.x > hbx || dep.y > hby || dep.z > hbz);
+      ok = ok && !(arr.x < lbx || arr.y < lby || arr.z < lbz || arr.x > hbx || arr.y > hby || arr.z > hbz);
+      if (ok) {
+        float rev = sample_face(A.pred, sx, sy, sz, arr.x / hx - offx, arr.y / hy - offy, arr.z / hz - offz);
+        float corr = A.pred[i] + 0.5f * (orig[i] - rev);
+        int bx = int(floorf(qx)), by = int(floorf(qy)), bz = int(floorf(qz));
+        float lo = 3.4e38f, hi = -3.4e38f;
+        for (int dz = 0; dz < 2; ++dz)
+          for (int dy = 0; dy < 2; ++dy)
+            for (int dx = 0; dx < 2; ++dx) {
+              int cx = min(max(bx + dx, 0), sx - 1), cy = min(max(by + dy, 0), sy - 1), cz = min(max(bz + dz, 0), sz - 1);
+              float d = orig[idx3(cx, cy, cz, sx, sy)];
+              lo = fminf(lo, d);
+              hi = fmaxf(hi, d);
+            }
+        value = 0.5f * low + 0.5f * fminf(fmaxf(corr, lo), hi);
+      }
+    }
+    // jet relaxation: coupling rates summed first (order independent)
+    float total = 0.0f, weighted = 0.0f;
+    for (int s = 0; s < src.count; ++s) {
+      const float* row = src.table + 16 * s;
+      float rate = row[7] * emission_weight(p, row);
+      total += rate;
+      weighted += rate * row[8 + AXIS];
+    }
+    if (total > 0.0f) {
+      float blend = -expm1f(-dt * total);
+      value = value + blend * (weighted / total - value);
+    }
+  }
+  if (AXIS == 2 && A.density) {
+    int zb = min(max(z - 1, 0), g.nz - 1), za = min(max(z, 0), g.nz - 1);
+    int64_t ib = idx3(x, y, zb, g.nx, g.ny), ia = idx3(x, y, za, g.nx, g.ny);
+    float rho = 0.5f * (A.density[ib] + A.density[ia]);
+    float theta = 0.5f * (A.temperature[ib] + A.temperature[ia]);
+    value += dt * (A.lift * theta - A.weight * rho);
+  }
+  A.out[i] = value;
+}
+
+// ---------------------------------------------------------------------------
+// scalar sampling (closed box: integer indices clamped)
+// ---------------------------------------------------------------------------
+struct Corners {
+  int64_t idx[8];
+  float w[8];
+};
+
+// zero_ext: corners outside the grid contribute 0 (shaders/mac_advect_density.glsl
+// fetchZero); otherwise integer indices are clamped (shaders/closed_sample.glsl).
+__device__ __forceinline__ void closed_corners(const GridDev& g, float qx, float qy, float qz, Corners& c,
+                                               bool zero_ext = false) {
+  int bx = int(floorf(qx)), by = int(floorf(qy)), bz = int(floorf(qz));
+  float fx = qx - bx, fy = qy - by, fz = qz - bz;
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    int dx = k & 1, dy = (k >> 1) & 1, dz = k >> 2;
+    int ux = bx + dx, uy = by + dy, uz = bz + dz;
+    int cx = min(max(ux, 0), g.nx - 1), cy = min(max(uy, 0), g.ny - 1), cz = min(max(uz, 0), g.nz - 1);
+    c.idx[k] = idx3(cx, cy, cz, g.nx, g.ny);
+    c.w[k] = (dx ? fx : 1.0f - fx) * (dy ? fy : 1.0f - fy) * (dz ? fz : 1.0f - fz);
+    if (zero_ext && (ux != cx || uy != cy || uz != cz)) c.w[k] = 0.0f;
+  }
+}
+
+template <bool SOLID>
+__device__ __forceinline__ float closed_sample(const float* __restrict__ f, const Corners& c,
+                                               const uint8_t* __restrict__ solid) {
+  float v = 0.0f, t = 0.0f;
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    if (SOLID && solid[c.idx[k]]) continue;
+    v += c.w[k] * f[c.idx[k]];
+    t += c.w[k];
+  }
+  if (!SOLID) return v;
+  return t > 1e-8f ? v / t : 0.0f;
+}
+
+struct ScalarSet {
+  const float* f[3];
+  float* o[3];
+};
+
+template <bool SOLID>
+__global__ void k_scalar_predict(GridDev g, Vel V, float dt, int channels, ScalarSet S_,
+                                 const uint8_t* __restrict__ solid) {
+  int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y * blockDim.y + threadIdx.y, z = blockIdx.z;
+  if (x >= g.nx || y >= g.ny) return;
+  int64_t i = idx3(x, y, z, g.nx, g.ny);
+  if (SOLID && solid[i]) {
+    for (int k = 0; k < channels; ++k) S_.o[k][i] = 0.0f;
+    return;
+  }
+  float hx = g.ex / g.nx, hy = g.ey / g.ny, hz = g.ez / g.nz;
+  V3 p = v3((x + 0.5f) * hx, (y + 0.5f) * hy, (z + 0.5f) * hz);
+  V3 vp = velocity_at(g, V, p);
+  V3 mid = v3(p.x - 0.5f * dt * vp.x, p.y - 0.5f * dt * vp.y, p.z - 0.5f * dt * vp.z);
+  if (SOLID) mid = clip_trace(g, solid, p, mid);
+  V3 vm = velocity_at(g, V, mid);
+  V3 dep = v3(p.x - dt * vm.x, p.y - dt * vm.y, p.z - dt * vm.z);
+  if (SOLID) dep = clip_trace(g, solid, p, dep);
+  Corners c;
+  closed_corners(g, dep.x / hx - 0.5f, dep.y / hy - 0.5f, dep.z / hz - 0.5f, c);
+  for (int k = 0; k < channels; ++k) S_.o[k][i] = closed_sample<SOLID>(S_.f[k], c, solid);
+}
+
+struct ScalarArgs {
+  GridDev g;
+  Vel V;
+  float dt;
+  int channels;
+  const float* f[3];
+  const float* pred[3];
+  float* out[3];
+  float keep[3];  // exp(-decay*dt)
+  int chan[3];
+  int zero_ext;
+  const uint8_t* solid;
+};
+
+template <bool SOLID, bool MC>
+__global__ void k_scalar_advect(ScalarArgs A, fx_sources src) {
+  const GridDev& g = A.g;
+  int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y * blockDim.y + threadIdx.y, z = blockIdx.z;
+  if (x >= g.nx || y >= g.ny) return;
+  int64_t i = idx3(x, y, z, g.nx, g.ny);
+  if (SOLID && A.solid[i]) {
+    for (int k = 0; k < A.channels; ++k) A.out[k][i] = 0.0f;
+    return;
+  }
+  const float dt = A.dt;
+  float hx = g.ex / g.nx, hy = g.ey / g.ny, hz = g.ez / g.nz;
+  V3 p = v3((x + 0.5f) * hx, (y + 0.5f) * hy, (z + 0.5f) * hz);
+  V3 vp = velocity_at(g, A.V, p);
+  V3 mid = v3(p.x - 0.5f * dt * vp.x, p.y - 0.5f * dt * vp.y, p.z - 0.5f * dt * vp.z);
+  if (SOLID) mid = clip_trace(g, A.solid, p, mid);
+  V3 vm = velocity_at(g, A.V, mid);
+  V3 dep = v3(p.x - dt * vm.x, p.y - dt * vm.y, p.z - dt * vm.z);
+  if (SOLID) dep = clip_trace(g, A.solid, p, dep);
+  Corners c;
+  closed_corners(g, dep.x / hx - 0.5f, dep.y / hy - 0.5f, dep.z / hz - 0.5f, c, A.zero_ext != 0);
+  bool ok = false;
+  Corners ca;
+  if (MC) {
+    V3 rmid = v3(p.x + 0.5f * dt * vp.x, p.y + 0.5f * dt * vp.y, p.z + 0.5f * dt * vp.z);
+    V3 va = velocity_at(g, A.V, rmid);
+    V3 arr = v3(p.x + dt * va.x, p.y + dt * va.y, p.z + dt * va.z);
+    ok = true;
+    if (SOLID) ok = safe_correction(g, A.solid, p, dep, rmid, arr);
+    float lx = 0.5f * hx, ly = 0.5f * hy, lz = 0.5f * hz;
+    float ux = g.ex - 0.5f * hx, uy = g.ey - 0.5f * hy, uz = g.ez - 0.5f * hz;
+    ok = ok && !(dep.x < lx || dep.y < ly || dep.z < lz || dep.x > ux || dep.y > uy || dep.z > uz);
+    ok = ok && !(arr.x < lx || arr.y < ly || arr.z < lz || arr.x > ux || arr.y > uy || arr.z > uz);
+    closed_corners(g, arr.x / hx - 0.5f, arr.y / hy - 0.5f, arr.z / hz - 0.5f, ca);
+  }
+  float wsrc[8];
+  for (int s = 0; s < src.count; ++s) wsrc[s] = emission_weight(p, src.table + 16 * s);
+  for (int k = 0; k < A.channels; ++k) {
+    const float* f = A.f[k];
+    float value = closed_sample<SOLID>(f, c, A.solid);
+    if (MC && ok) {
+      float rev = closed_sample<SOLID>(A.pred[k], ca, A.solid);
+      float corr = A.pred[k][i] + 0.5f * (f[i] - rev);
+      float lo = 3.4e38f, hi = -3.4e38f;
+#pragma unroll
+      for (int q = 0; q < 8; ++q) {
+        float d = f[c.idx[q]];
+        lo = fminf(lo, d);
+        hi = fmaxf(hi, d);
+      }
+      value = fminf(fmaxf(corr, lo), hi);
+    }
+    value *= A.keep[k];
+    int ch = A.chan[k];
+    float target = 0.0f;
+    bool any_target = false;
+    for (int s = 0; s < src.count; ++s) {
+      const float* row = src.table + 16 * s;
+      float rate = ch == 0 ? row[11] : (ch == 1 ? row[12] : row[15]);
+      if (ch == 0 && row[13] > 0.5f) {
+        target = fmaxf(target, rate * wsrc[s]);
+        any_target = true;
+      } else {
+        value += dt * rate * wsrc[s];
+      }
+    }
+    if (ch == 0 && (src.multi || any_target)) value = fmaxf(value, target);
+    A.out[k][i] = value;
+  }
+}
+
+// ---------------------------------------------------------------------------
+__global__ void k_combustion(int64_t n, float dt, float ign, float k, float heat, float smoke,
+                             const float* __restrict__ fuel, const float* __restrict__ T,
+                             const float* __restrict__ rho, float* __restrict__ fo, float* __restrict__ To,
+                             float* __restrict__ ro, float* __restrict__ flame) {
+  int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  float f = fuel[i], t = T[i];
+  float burned = t >= ign ? fmaxf(f, 0.0f) * -expm1f(-k * dt) : 0.0f;
+  float fl = burned / dt;
+  fo[i] = fmaxf(f - dt * fl, 0.0f);
+  To[i] = t + dt * heat * fl;
+  ro[i] = rho[i] + dt * smoke * fl;
+  flame[i] = fl;
+}
+
+struct MaxArgs {
+  const float* a[8];
+  int64_t n[8];
+};
+
+__global__ void k_max_init(float* out, int count) {
+  if (int(threadIdx.x) < count) out[threadIdx.x] = 0.0f;
+}
+
+__global__ void k_max_abs(MaxArgs A, float* out) {
+  int k = blockIdx.y;
+  const float* a = A.a[k];
+  int64_t n = A.n[k];
+  float m = 0.0f;
+  for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += int64_t(gridDim.x) * blockDim.x) {
+    float v = a[i];
+    m = fmaxf(m, isfinite(v) ? fabsf(v) : 3.4e38f);
+  }
+  m = wave_max(m);
+  __shared__ float part[4];
+  int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  if (lane == 0) part[w] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    float r = fmaxf(fmaxf(part[0], part[1]), fmaxf(part[2], part[3]));
+    atomicMax(reinterpret_cast<unsigned int*>(out + k), __float_as_uint(r));
+  }
+}
+
+static const dim3 kBlock(64, 4, 1);
+
+GridDev dev_grid(const fx_grid& g) { return GridDev{g.nx, g.ny, g.nz, g.ex, g.ey, g.ez}; }
+
+int check_sources(const fx_sources* s) {
+  FX_REQUIRE(s, FX_EINVAL, "null source table");
+  FX_REQUIRE(s->count >= 0 && s->count <= 8, FX_EINVAL, "At most eight sources are supported");
+  for (int i = 0; i < s->count; ++i) {
+    FX_REQUIRE(s->table[16 * i + 3] > 0.0f, FX_EINVAL, "source %d radius must be positive", i);
+    for (int j = 0; j < 16; ++j)
+      FX_REQUIRE(std::isfinite(s->table[16 * i + j]), FX_EINVAL, "source %d has a non-finite entry", i);
+  }
+  return FX_OK;
+}
+
+}  // namespace fx
+
+using namespace fx;
+
+extern "C" {
+
+int fx_face_predict(fx_grid g, float dt, const float* u, const float* v, const float* w, float* pu, float* pv,
+                    float* pw, const uint8_t* solid, void* stream) {
+  if (int e = check_grid(g)) return e;
+  FX_REQUIRE(u && v && w && pu && pv && pw, FX_EINVAL, "null face array");
+  FX_REQUIRE(pu != u && pv != v && pw != w && pu != v && pu != w && pv != u && pv != w && pw != u && pw != v,
+             FX_EINVAL, "Advection input/output textures must be distinct");
+  FX_REQUIRE(std::isfinite(dt), FX_EINVAL, "dt must be finite");
+  GridDev G = dev_grid(g);
+  Vel V{u, v, w};
+  hipStream_t s = S(stream);
+  dim3 gx = grid3(g.nx + 1, g.ny, g.nz, kBlock), gy = grid3(g.nx, g.ny + 1, g.nz, kBlock),
+       gz = grid3(g.nx, g.ny, g.nz + 1, kBlock);
+  if (solid) {
+    hipLaunchKernelGGL((k_face_predict<0, true>), gx, kBlock, 0, s, G, V, dt, u, pu, solid);
+    hipLaunchKernelGGL((k_face_predict<1, true>), gy, kBlock, 0, s, G, V, dt, v, pv, solid);
+    hipLaunchKernelGGL((k_face_predict<2, true>), gz, kBlock, 0, s, G, V, dt, w, pw, solid);
+  } else {
+    hipLaunchKernelGGL((k_face_predict<0, false>), gx, kBlock, 0, s, G, V, dt, u, pu, solid);
+    hipLaunchKernelGGL((k_face_predict<1, false>), gy, kBlock, 0, s, G, V, dt, v, pv, solid);
+    hipLaunchKernelGGL((k_face_predict<2, false>), gz, kBlock, 0, s, G, V, dt, w, pw, solid);
+  }
+  FX_LAUNCHED("fx_face_predict");
+  return FX_OK;
+}
+
+int fx_velocity_advect(const fx_velocity_args* a, void* stream) {
+  FX_REQUIRE(a, FX_EINVAL, "null arguments");
+  if (int e = check_grid(a->g)) return e;
+  if (int e = check_sources(a->sources)) return e;
+  FX_REQUIRE(a->u && a->v && a->w && a->ou && a->ov && a->ow, FX_EINVAL, "null face array");
+  const float* ins[6] = {a->u, a->v, a->w, a->pu, a->pv, a->pw};
+  const float* outs[3] = {a->ou, a->ov, a->ow};
+  for (int o = 0; o < 3; ++o)
+    for (int k = 0; k < 6; ++k)
+      FX_REQUIRE(outs[o] != ins[k], FX_EINVAL, "Advection input/output textures must be distinct");
+  FX_REQUIRE(!a->maccormack || (a->pu && a->pv && a->pw), FX_EINVAL, "MacCormack needs face predictors");
+  FX_REQUIRE(!a->density == !a->temperature, FX_EINVAL, "buoyancy needs both density and temperature");
+  GridDev G = dev_grid(a->g);
+  Vel V{a->u, a->v, a->w};
+  hipStream_t s = S(stream);
+  const fx_grid& g = a->g;
+  dim3 grids[3] = {grid3(g.nx + 1, g.ny, g.nz, kBlock), grid3(g.nx, g.ny + 1, g.nz, kBlock),
+                   grid3(g.nx, g.ny, g.nz + 1, kBlock)};
+  const float* preds[3] = {a->pu, a->pv, a->pw};
+  float* o3[3] = {a->ou, a->ov, a->ow};
+  for (int axis = 0; axis < 3; ++axis) {
+    VelArgs A{G, V, a->dt, preds[axis], o3[axis], a->density, a->temperature, a->thermal_lift, a->density_weight,
+              a->solid};
+#define FX_VEL(AX, SO, MC) hipLaunchKernelGGL((k_velocity<AX, SO, MC>), grids[AX], kBlock, 0, s, A, *a->sources)
+#define FX_VEL_AX(AX)                                             \
+  if (a->solid) {                                                 \
+    if (a->maccormack) FX_VEL(AX, true, true); else FX_VEL(AX, true, false);   \
+  } else {                                                        \
+    if (a->maccormack) FX_VEL(AX, false, true); else FX_VEL(AX, false, false); \
+  }
+    if (axis == 0) { FX_VEL_AX(0) } else if (axis == 1) { FX_VEL_AX(1) } else { FX_VEL_AX(2) }
+#undef FX_VEL_AX
+#undef FX_VEL
+  }
+  FX_LAUNCHED("fx_velocity_advect");
+  return FX_OK;
+}
+
+int fx_scalar_predict(fx_grid g, float dt, const float* u, const float* v, const float* w, int32_t channels,
+                      const float* const* fields, float* const* preds, const uint8_t* solid, void* stream) {
+  if (int e = check_grid(g)) return e;
+  FX_REQUIRE(channels >= 1 && channels <= 3, FX_EINVAL, "channels must be 1..3");
+  FX_REQUIRE(u && v && w && fields && preds, FX_EINVAL, "null array");
+  ScalarSet set{};
+  for (int k = 0; k < channels; ++k) {
+    FX_REQUIRE(fields[k] && preds[k] && fields[k] != preds[k], FX_EINVAL,
+               "Advection input/output textures must be distinct");
+    set.f[k] = fields[k];
+    set.o[k] = preds[k];
+  }
+  dim3 gr = grid3(g.nx, g.ny, g.nz, kBlock);
+  Vel V{u, v, w};
+  if (solid)
+    hipLaunchKernelGGL((k_scalar_predict<true>), gr, kBlock, 0, S(stream), dev_grid(g), V, dt, channels, set, solid);
+  else
+    hipLaunchKernelGGL((k_scalar_predict<false>), gr, kBlock, 0, S(stream), dev_grid(g), V, dt, channels, set, solid);
+  FX_LAUNCHED("fx_scalar_predict");
+  return FX_OK;
+}
+
+int fx_scalar_advect(const fx_scalar_args* a, void* stream) {
+  FX_REQUIRE(a, FX_EINVAL, "null arguments");
+  if (int e = check_grid(a->g)) return e;
+  if (int e = check_sources(a->sources)) return e;
+  FX_REQUIRE(a->channels >= 1 && a->channels <= 3, FX_EINVAL, "channels must be 1..3");
+  FX_REQUIRE(a->u && a->v && a->w, FX_EINVAL, "null velocity");
+  ScalarArgs A{};
+  A.g = dev_grid(a->g);
+  A.V = Vel{a->u, a->v, a->w};
+  A.dt = a->dt;
+  A.channels = a->channels;
+  A.solid = a->solid;
+  A.zero_ext = a->boundary;
+  FX_REQUIRE(a->boundary == 0 || a->boundary == 1, FX_EINVAL, "boundary must be 0 (closed) or 1 (zero)");
+  FX_REQUIRE(!(a->boundary == 1 && (a->maccormack || a->solid)), FX_EINVAL,
+             "zero-extension transport is semi-Lagrangian without solids");
+  for (int k = 0; k < a->channels; ++k) {
+    FX_REQUIRE(a->field[k] && a->out[k], FX_EINVAL, "null channel %d", k);
+    FX_REQUIRE(!a->maccormack || a->pred[k], FX_EINVAL, "MacCormack needs predictors");
+    FX_REQUIRE(a->channel_id[k] >= 0 && a->channel_id[k] <= 2, FX_EINVAL, "channel id must be 0..2");
+    for (int j = 0; j < a->channels; ++j)
+      FX_REQUIRE(a->out[k] != a->field[j] && a->out[k] != (a->maccormack ? a->pred[j] : nullptr), FX_EINVAL,
+                 "Advection input/output textures must be distinct");
+    A.f[k] = a->field[k];
+    A.pred[k] = a->pred[k];
+    A.out[k] = a->out[k];
+    A.keep[k] = expf(-a->decay[k] * a->dt);
+    A.chan[k] = a->channel_id[k];
+  }
+  dim3 gr = grid3(a->g.nx, a->g.ny, a->g.nz, kBlock);
+  hipStream_t s = S(stream);
+  if (a->solid) {
+    if (a->maccormack) hipLaunchKernelGGL((k_scalar_advect<true, true>), gr, kBlock, 0, s, A, *a->sources);
+    else hipLaunchKernelGGL((k_scalar_advect<true, false>), gr, kBlock, 0, s, A, *a->sources);
+  } else {
+    if (a->maccormack) hipLaunchKernelGGL((k_scalar_advect<false, true>), gr, kBlock, 0, s, A, *a->sources);
+    else hipLaunchKernelGGL((k_scalar_advect<false, false>), gr, kBlock, 0, s, A, *a->sources);
+  }
+  FX_LAUNCHED("fx_scalar_advect");
+  return FX_OK;
+}
+
+int fx_combustion(int64_t n, float dt, float ignition, float burn_rate, float heat_yield, float smoke_yield,
+                  const float* fuel, const float* temperature, const float* density, float* fuel_out,
+                  float* temperature_out, float* density_out, float* flame, void* stream) {
+  FX_REQUIRE(n >= 0 && fuel && temperature && density && fuel_out && temperature_out && density_out && flame,
+             FX_EINVAL, "combustion: null array");
+  FX_REQUIRE(dt > 0.0f, FX_EINVAL, "dt must be positive");
+  FX_REQUIRE(fuel_out != fuel && temperature_out != temperature && density_out != density, FX_EINVAL,
+             "Advection input/output textures must be distinct");
+  if (n == 0) return FX_OK;
+  hipLaunchKernelGGL(k_combustion, dim3((n + 255) / 256), dim3(256), 0, S(stream), n, dt, ignition, burn_rate,
+                     heat_yield, smoke_yield, fuel, temperature, density, fuel_out, temperature_out, density_out,
+                     flame);
+  FX_LAUNCHED("fx_combustion");
+  return FX_OK;
+}
+
+int fx_max_abs(int32_t count, const float* const* arrays, const int64_t* sizes, float* out, void* stream) {
+  FX_REQUIRE(count >= 1 && count <= 8 && arrays && sizes && out, FX_EINVAL, "max_abs: 1..8 arrays");
+  MaxArgs A{};
+  for (int k = 0; k < count; ++k) {
+    FX_REQUIRE(arrays[k] && sizes[k] >= 0, FX_EINVAL, "max_abs: bad array %d", k);
+    A.a[k] = arrays[k];
+    A.n[k] = sizes[k];
+  }
+  hipStream_t s = S(stream);
+  hipLaunchKernelGGL(k_max_init, dim3(1), dim3(64), 0, s, out, count);
+  hipLaunchKernelGGL(k_max_abs, dim3(512, count), dim3(256), 0, s, A, out);
+  FX_LAUNCHED("fx_max_abs");
+  return FX_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,127 @@
+// Shared device/host helpers for libfluxhip (gfx950).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <string>
+
+#include "../../include/fluxhip.h"
+
+namespace fx {
+
+// ---------------------------------------------------------------------------
+// error plumbing: every C-ABI entry returns 0 or a negative code, message kept
+// per thread and read back with fx_last_error().
+// ---------------------------------------------------------------------------
+void set_error(const char* fmt, ...);
+int fail_hip(hipError_t e, const char* what);
+
+#define FX_HIP(call)                                                     \
+  do {                                                                   \
+    hipError_t _e = (call);                                              \
+    if (_e != hipSuccess) return ::fx::fail_hip(_e, #call);              \
+  } while (0)
+
+#define FX_LAUNCHED(name)                                                \
+  do {                                                                   \
+    hipError_t _e = hipGetLastError();                                   \
+    if (_e != hipSuccess) return ::fx::fail_hip(_e, name);               \
+  } while (0)
+
+#define FX_REQUIRE(cond, code, ...)                                      \
+  do {                                                                   \
+    if (!(cond)) {                                                       \
+      ::fx::set_error(__VA_ARGS__);                                      \
+      return code;                                                       \
+    }                                                                    \
+  } while (0)
+
+inline hipStream_t S(void* s) { return reinterpret_cast<hipStream_t>(s); }
+
+int check_grid(const fx_grid& g);
+
+// ---------------------------------------------------------------------------
+// storage types for multigrid levels: fp32, bf16 (raw u16) and fp16
+// ---------------------------------------------------------------------------
+struct bf16 {
+  uint16_t bits;
+};
+
+__device__ __forceinline__ float to_f(float v) { return v; }
+__device__ __forceinline__ float to_f(bf16 v) { return __uint_as_float(uint32_t(v.bits) << 16); }
+__device__ __forceinline__ float to_f(_Float16 v) { return float(v); }
+
+template <typename T>
+__device__ __forceinline__ T from_f(float v);
+template <>
+__device__ __forceinline__ float from_f<float>(float v) { return v; }
+template <>
+__device__ __forceinline__ bf16 from_f<bf16>(float v) {
+  // round to nearest even; NaN stays NaN
+  uint32_t u = __float_as_uint(v);
+  if ((u & 0x7f800000u) == 0x7f800000u && (u & 0x007fffffu)) return bf16{uint16_t((u >> 16) | 0x40)};
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return bf16{uint16_t(u >> 16)};
+}
+template <>
+__device__ __forceinline__ _Float16 from_f<_Float16>(float v) { return _Float16(v); }
+
+// ---------------------------------------------------------------------------
+// indexing helpers (flat x-fast: x + nx*(y + ny*z))
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int64_t idx3(int x, int y, int z, int nx, int ny) {
+  return int64_t(x) + int64_t(nx) * (int64_t(y) + int64_t(ny) * int64_t(z));
+}
+
+// Per-cell open-face bits of the level-0 operator.
+enum : uint8_t {
+  FLAG_XLO = 1, FLAG_XHI = 2, FLAG_YLO = 4, FLAG_YHI = 8, FLAG_ZLO = 16, FLAG_ZHI = 32, FLAG_FLUID = 64
+};
+
+// Box (unmasked) flags computed from coordinates.
+__device__ __forceinline__ uint8_t box_flags(int x, int y, int z, int nx, int ny, int nz) {
+  uint8_t f = FLAG_FLUID;
+  if (x > 0) f |= FLAG_XLO;
+  if (x + 1 < nx) f |= FLAG_XHI;
+  if (y > 0) f |= FLAG_YLO;
+  if (y + 1 < ny) f |= FLAG_YHI;
+  if (z > 0) f |= FLAG_ZLO;
+  if (z + 1 < nz) f |= FLAG_ZHI;
+  return f;
+}
+
+// FP64 block reduction helpers (wavefront = 64 lanes)
+__device__ __forceinline__ double wave_sum(double v) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+__device__ __forceinline__ float wave_max(float v) {
+  for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, 64));
+  return v;
+}
+
+// Sum `v` over a block of BLOCK threads (any 1-3D shape) and add it to *out
+// with one atomic per block.  Every thread of the block must call it.
+template <int BLOCK>
+__device__ __forceinline__ void block_sum_atomic(double v, double* out) {
+  __shared__ double part[BLOCK / 64];
+  v = wave_sum(v);
+  int tid = threadIdx.x + blockDim.x * (threadIdx.y + blockDim.y * threadIdx.z);
+  int lane = tid & 63, w = tid >> 6;
+  if (lane == 0) part[w] = v;
+  __syncthreads();
+  if (w == 0) {
+    double s = lane < BLOCK / 64 ? part[lane] : 0.0;
+    s = wave_sum(s);
+    if (lane == 0) atomicAdd(out, s);
+  }
+  __syncthreads();  // `part` is reused by the next call in the same kernel
+}
+
+inline dim3 grid3(int nx, int ny, int nz, dim3 block) {
+  return dim3((nx + block.x - 1) / block.x, (ny + block.y - 1) / block.y, (nz + block.z - 1) / block.z);
+}
+
+}  // namespace fx

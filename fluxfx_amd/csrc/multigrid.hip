@@ -1,0 +1,593 @@
+// Geometric multigrid engine: smoothing, fused residual+restriction,
+// prolongation, FP64 coarsest direct solve, FP64 norms and the
+// tolerance-driven stationary solve.  Templated on the storage type of the
+// level arrays (fp32 / bf16 / fp16, arithmetic always fp32) and on the level
+// operator (closed box, open-face flags, or edge conductances).
+#include "common.hpp"
+
+namespace fx {
+
+struct LevelDev {
+  int nx, ny, nz;
+  float wx, wy, wz;
+  const void* op;
+};
+
+static LevelDev dev_level(const fx_mg_level& L) {
+  return LevelDev{L.nx, L.ny, L.nz, L.w[0], L.w[1], L.w[2], L.op_data};
+}
+
+struct Stencil {
+  float s;       // sum_n w_n p_n
+  float d;       // sum_n w_n
+  bool active;   // fluid / active cell
+};
+
+// Gather the 7-point operator at cell i for the given level operator.
+template <int OP, typename T>
+__device__ __forceinline__ Stencil stencil(const LevelDev& L, int x, int y, int z, int64_t i,
+                                           const T* __restrict__ p, bool read_p) {
+  const int64_t sy = L.nx, sz = int64_t(L.nx) * L.ny;
+  Stencil st{0.0f, 0.0f, true};
+  if (OP == FX_OP_EDGES) {
+    const float4* e = reinterpret_cast<const float4*>(L.op);
+    float4 c = e[i];
+    float wxl = x > 0 ? e[i - 1].x : 0.0f;
+    float wyl = y > 0 ? e[i - sy].y : 0.0f;
+    float wzl = z > 0 ? e[i - sz].z : 0.0f;
+    st.d = c.x + c.y + c.z + wxl + wyl + wzl;
+    st.active = c.w > 0.0f;
+    if (read_p) {
+      float s = 0.0f;
+      if (c.x != 0.0f) s += c.x * to_f(p[i + 1]);
+      if (wxl != 0.0f) s += wxl * to_f(p[i - 1]);
+      if (c.y != 0.0f) s += c.y * to_f(p[i + sy]);
+      if (wyl != 0.0f) s += wyl * to_f(p[i - sy]);
+      if (c.z != 0.0f) s += c.z * to_f(p[i + sz]);
+      if (wzl != 0.0f) s += wzl * to_f(p[i - sz]);
+      st.s = s;
+    }
+  } else {
+    uint8_t f = OP == FX_OP_FLAGS ? reinterpret_cast<const uint8_t*>(L.op)[i] : box_flags(x, y, z, L.nx, L.ny, L.nz);
+    st.active = f & FLAG_FLUID;
+    float s = 0.0f, d = 0.0f;
+    if (f & FLAG_XLO) { d += L.wx; if (read_p) s += L.wx * to_f(p[i - 1]); }
+    if (f & FLAG_XHI) { d += L.wx; if (read_p) s += L.wx * to_f(p[i + 1]); }
+    if (f & FLAG_YLO) { d += L.wy; if (read_p) s += L.wy * to_f(p[i - sy]); }
+    if (f & FLAG_YHI) { d += L.wy; if (read_p) s += L.wy * to_f(p[i + sy]); }
+    if (f & FLAG_ZLO) { d += L.wz; if (read_p) s += L.wz * to_f(p[i - sz]); }
+    if (f & FLAG_ZHI) { d += L.wz; if (read_p) s += L.wz * to_f(p[i + sz]); }
+    st.s = s;
+    st.d = d;
+  }
+  return st;
+}
+
+template <int OP>
+__device__ __forceinline__ bool cell_active(const LevelDev& L, int x, int y, int z, int64_t i) {
+  if (OP == FX_OP_EDGES) return reinterpret_cast<const float4*>(L.op)[i].w > 0.0f;
+  if (OP == FX_OP_FLAGS) return reinterpret_cast<const uint8_t*>(L.op)[i] & FLAG_FLUID;
+  return true;
+}
+
+// ---------------------------------------------------------------------------
+// smoothing: out = active && d>0 ? p + omega*((s - rhs*inv_dt)/d - p) : 0
+// zero_guess: p is known to be 0 (fresh coarse level or error equation)
+// ---------------------------------------------------------------------------
+template <typename T, int OP>
+__global__ void k_smooth(LevelDev L, const T* __restrict__ p, const T* __restrict__ rhs, T* __restrict__ out,
+                         float inv_dt, float omega, int zero_guess) {
+  int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y * blockDim.y + threadIdx.y, z = blockIdx.z;
+  if (x >= L.nx || y >= L.ny) return;
+  int64_t i = idx3(x, y, z, L.nx, L.ny);
+  Stencil st = stencil<OP>(L, x, y, z, i, p, !zero_guess);
+  float r = 0.0f;
+  if (st.active && st.d > 0.0f) {
+    float cand = (st.s - to_f(rhs[i]) * inv_dt) / st.d;
+    float old = zero_guess ? 0.0f : to_f(p[i]);
+    r = old + omega * (cand - old);
+  }
+  out[i] = from_f<T>(r);
+}
+
+// residual r = rhs*inv_dt - (s - d p) on active cells, 0 elsewhere
+template <typename T, int OP>
+__device__ __forceinline__ float residual_at(const LevelDev& L, int x, int y, int z, const T* __restrict__ p,
+                                             const T* __restrict__ rhs, float inv_dt) {
+  int64_t i = idx3(x, y, z, L.nx, L.ny);
+  Stencil st = stencil<OP>(L, x, y, z, i, p, true);
+  if (!st.active) return 0.0f;
+  if (OP == FX_OP_EDGES && !(st.d > 0.0f)) return 0.0f;
+  return to_f(rhs[i]) * inv_dt - (st.s - st.d * to_f(p[i]));
+}
+
+template <typename T, int OP>
+__global__ void k_residual(LevelDev L, const T* __restrict__ p, const T* __restrict__ rhs, T* __restrict__ out,
+                           float inv_dt) {
+  int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y * blockDim.y + threadIdx.y, z = blockIdx.z;
+  if (x >= L.nx || y >= L.ny) return;
+  out[idx3(x, y, z, L.nx, L.ny)] = from_f<T>(residual_at<T, OP>(L, x, y, z, p, rhs, inv_dt));
+}
+
+// coarse rhs[C] = mean of the 8 child residuals (shaders/mg_restrict.glsl semantics)
+template <typename T, int OP>
+__global__ void k_residual_restrict(LevelDev L, int cnx, int cny, const T* __restrict__ p,
+                                    const T* __restrict__ rhs, T* __restrict__ crhs, float inv_dt) {
+  int X = blockIdx.x * blockDim.x + threadIdx.x, Y = blockIdx.y * blockDim.y + threadIdx.y, Z = blockIdx.z;
+  if (X >= cnx || Y >= cny) return;
+  float acc = 0.0f;
+#pragma unroll
+  for (int dz = 0; dz < 2; ++dz)
+#pragma unroll
+    for (int dy = 0; dy < 2; ++dy)
+#pragma unroll
+      for (int dx = 0; dx < 2; ++dx) acc += residual_at<T, OP>(L, 2 * X + dx, 2 * Y + dy, 2 * Z + dz, p, rhs, inv_dt);
+  crhs[idx3(X, Y, Z, cnx, cny)] = from_f<T>(acc * 0.125f);
+}
+
+template <typename T>
+__global__ void k_restrict_mean(int cnx, int cny, int nx, int ny, const T* __restrict__ fine, T* __restrict__ coarse) {
+  int X = blockIdx.x * blockDim.x + threadIdx.x, Y = blockIdx.y * blockDim.y + threadIdx.y, Z = blockIdx.z;
+  if (X >= cnx || Y >= cny) return;
+  float acc = 0.0f;
+  for (int dz = 0; dz < 2; ++dz)
+    for (int dy = 0; dy < 2; ++dy)
+      for (int dx = 0; dx < 2; ++dx) acc += to_f(fine[idx3(2 * X + dx, 2 * Y + dy, 2 * Z + dz, nx, ny)]);
+  coarse[idx3(X, Y, Z, cnx, cny)] = from_f<T>(acc * 0.125f);
+}
+
+// Adjoint of clamped trilinear prolongation with unit row sums: per axis
+// fine (2i-1, 2i, 2i+1, 2i+2) weighted (1,3,3,1)/8, out-of-range folded onto the edge.
+template <typename T>
+__global__ void k_restrict_full(int cnx, int cny, int cnz, int nx, int ny, int nz, const T* __restrict__ fine,
+                                T* __restrict__ coarse) {
+  int X = blockIdx.x * blockDim.x + threadIdx.x, Y = blockIdx.y * blockDim.y + threadIdx.y, Z = blockIdx.z;
+  if (X >= cnx || Y >= cny) return;
+  const float wt[4] = {0.125f, 0.375f, 0.375f, 0.125f};
+  float acc = 0.0f;
+  for (int kz = 0; kz < 4; ++kz) {
+    int fz = min(max(2 * Z - 1 + kz, 0), nz - 1);
+    for (int ky = 0; ky < 4; ++ky) {
+      int fy = min(max(2 * Y - 1 + ky, 0), ny - 1);
+      float wzy = wt[kz] * wt[ky];
+      for (int kx = 0; kx < 4; ++kx) {
+        int fx = min(max(2 * X - 1 + kx, 0), nx - 1);
+        acc += wzy * wt[kx] * to_f(fine[idx3(fx, fy, fz, nx, ny)]);
+      }
+    }
+  }
+  coarse[idx3(X, Y, Z, cnx, cny)] = from_f<T>(acc);
+}
+
+// prolongation; out may alias in (each fine cell reads only its own value)
+template <typename T, int FOP, int COP, int MODE>
+__global__ void k_prolong(LevelDev F, LevelDev C, const T* in, const T* __restrict__ coarse, T* out) {
+  int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y * blockDim.y + threadIdx.y, z = blockIdx.z;
+  if (x >= F.nx || y >= F.ny) return;
+  int64_t i = idx3(x, y, z, F.nx, F.ny);
+  if (MODE == FX_PROLONG_CONSTANT) {
+    float r = 0.0f;
+    if (cell_active<FOP>(F, x, y, z, i)) r = to_f(in[i]) + to_f(coarse[idx3(x >> 1, y >> 1, z >> 1, C.nx, C.ny)]);
+    out[i] = from_f<T>(r);
+    return;
+  }
+  // cell-centred trilinear: q = (c+0.5)*0.5-0.5, base = floor(q)
+  int bx = (x >> 1) - ((x & 1) ? 0 : 1), by = (y >> 1) - ((y & 1) ? 0 : 1), bz = (z >> 1) - ((z & 1) ? 0 : 1);
+  float fx = (x & 1) ? 0.25f : 0.75f, fy = (y & 1) ? 0.25f : 0.75f, fz = (z & 1) ? 0.25f : 0.75f;
+  float num = 0.0f, den = 0.0f;
+#pragma unroll
+  for (int dz = 0; dz < 2; ++dz) {
+    int cz = min(max(bz + dz, 0), C.nz - 1);
+    float wz = dz ? fz : 1.0f - fz;
+#pragma unroll
+    for (int dy = 0; dy < 2; ++dy) {
+      int cy = min(max(by + dy, 0), C.ny - 1);
+      float wy = dy ? fy : 1.0f - fy;
+#pragma unroll
+      for (int dx = 0; dx < 2; ++dx) {
+        int cx = min(max(bx + dx, 0), C.nx - 1);
+        float wgt = wz * wy * (dx ? fx : 1.0f - fx);
+        int64_t ci = idx3(cx, cy, cz, C.nx, C.ny);
+        if (MODE == FX_PROLONG_RENORM) {
+          if (!cell_active<COP>(C, cx, cy, cz, ci)) continue;
+          den += wgt;
+        }
+        num += wgt * to_f(coarse[ci]);
+      }
+    }
+  }
+  float corr = num;
+  if (MODE == FX_PROLONG_RENORM) corr = den > 1e-12f ? num / den : 0.0f;
+  float r = to_f(in[i]) + corr;
+  if (FOP != FX_OP_BOX && !cell_active<FOP>(F, x, y, z, i)) r = 0.0f;
+  out[i] = from_f<T>(r);
+}
+
+// p = scale * M rhs, one wavefront per row, FP64 accumulation
+template <typename T>
+__global__ void k_direct(int m, const double* __restrict__ M, const T* __restrict__ rhs, T* __restrict__ out,
+                         float scale) {
+  int row = blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6);
+  int lane = threadIdx.x & 63;
+  if (row >= m) return;
+  double acc = 0.0;
+  const double* Mr = M + int64_t(row) * m;
+  for (int j = lane; j < m; j += 64) acc += Mr[j] * double(to_f(rhs[j]));
+  acc = wave_sum(acc);
+  if (lane == 0) out[row] = from_f<T>(float(acc * double(scale)));
+}
+
+// ---------------------------------------------------------------------------
+// FP64 reductions on the level-0 fp32 arrays
+// ---------------------------------------------------------------------------
+__global__ void k_fill_d(double* ws, int n) {
+  if (threadIdx.x < n) ws[threadIdx.x] = 0.0;
+}
+
+template <int OP>
+__global__ void k_resnorm(LevelDev L, const float* __restrict__ x, const float* __restrict__ b, double* ws) {
+  int X = blockIdx.x * blockDim.x + threadIdx.x, Y = blockIdx.y * blockDim.y + threadIdx.y, Z = blockIdx.z;
+  double rr = 0.0, bb = 0.0;
+  if (X < L.nx && Y < L.ny) {
+    int64_t i = idx3(X, Y, Z, L.nx, L.ny);
+    float r = residual_at<float, OP>(L, X, Y, Z, x, b, 1.0f);
+    float bv = b[i];
+    rr = double(r) * double(r);
+    bb = double(bv) * double(bv);
+  }
+  block_sum_atomic<256>(rr, ws + 0);
+  block_sum_atomic<256>(bb, ws + 1);
+}
+
+template <int OP>
+__global__ void k_mean_sum(LevelDev L, const float* __restrict__ b, double* ws) {
+  int X = blockIdx.x * blockDim.x + threadIdx.x, Y = blockIdx.y * blockDim.y + threadIdx.y, Z = blockIdx.z;
+  double s = 0.0, c = 0.0;
+  if (X < L.nx && Y < L.ny) {
+    int64_t i = idx3(X, Y, Z, L.nx, L.ny);
+    if (cell_active<OP>(L, X, Y, Z, i)) {
+      s = b[i];
+      c = 1.0;
+    }
+  }
+  block_sum_atomic<256>(s, ws + 2);
+  block_sum_atomic<256>(c, ws + 3);
+}
+
+template <int OP>
+__global__ void k_sub_mean(LevelDev L, float* __restrict__ b, const double* __restrict__ ws) {
+  int X = blockIdx.x * blockDim.x + threadIdx.x, Y = blockIdx.y * blockDim.y + threadIdx.y, Z = blockIdx.z;
+  if (X >= L.nx || Y >= L.ny) return;
+  int64_t i = idx3(X, Y, Z, L.nx, L.ny);
+  double cnt = ws[3];
+  float mean = cnt > 0 ? float(ws[2] / cnt) : 0.0f;
+  b[i] = cell_active<OP>(L, X, Y, Z, i) ? b[i] - mean : 0.0f;
+}
+
+static const dim3 kBlock(64, 4, 1);
+
+static dim3 level_grid(const fx_mg_level& L) { return grid3(L.nx, L.ny, L.nz, kBlock); }
+
+// ---------------------------------------------------------------------------
+// host-side dispatch over (storage, operator)
+// ---------------------------------------------------------------------------
+#define FX_DISPATCH_OP(op, ...)                                             \
+  switch (op) {                                                             \
+    case FX_OP_BOX: { constexpr int OPV = FX_OP_BOX; __VA_ARGS__; break; }  \
+    case FX_OP_FLAGS: { constexpr int OPV = FX_OP_FLAGS; __VA_ARGS__; break; } \
+    case FX_OP_EDGES: { constexpr int OPV = FX_OP_EDGES; __VA_ARGS__; break; } \
+    default: set_error("unknown level operator %d", op); return FX_EINVAL; \
+  }
+
+#define FX_DISPATCH_T(st, ...)                                              \
+  switch (st) {                                                             \
+    case FX_F32: { using TV = float; __VA_ARGS__; break; }                  \
+    case FX_BF16: { using TV = bf16; __VA_ARGS__; break; }                  \
+    case FX_F16: { using TV = _Float16; __VA_ARGS__; break; }               \
+    default: set_error("unknown storage type %d", st); return FX_EINVAL;  \
+  }
+
+int check_config(const fx_mg_config* c) {
+  FX_REQUIRE(c, FX_EINVAL, "null config");
+  FX_REQUIRE(c->levels >= 1 && c->levels <= 16, FX_EINVAL, "levels must be in [1,16]");
+  FX_REQUIRE(c->storage >= 0 && c->storage <= 2, FX_EINVAL, "bad storage type");
+  for (int l = 0; l < c->levels; ++l) {
+    const fx_mg_level& L = c->level[l];
+    FX_REQUIRE(L.nx >= 1 && L.ny >= 1 && L.nz >= 1, FX_EINVAL, "level %d has an empty shape", l);
+    FX_REQUIRE(L.p && L.q && L.rhs && L.p != L.q, FX_EINVAL, "level %d needs distinct p, q and rhs", l);
+    FX_REQUIRE(L.op == FX_OP_BOX || L.op_data, FX_EINVAL, "level %d operator data missing", l);
+    if (l > 0) {
+      const fx_mg_level& F = c->level[l - 1];
+      FX_REQUIRE(F.nx == 2 * L.nx && F.ny == 2 * L.ny && F.nz == 2 * L.nz, FX_EINVAL,
+                 "level %d is not an exact factor-two coarsening", l);
+    }
+  }
+  const fx_mg_level& C = c->level[c->levels - 1];
+  if (c->coarse_sweeps <= 0) {
+    FX_REQUIRE(c->coarse_matrix, FX_EINVAL, "direct coarse solve needs a matrix");
+  }
+  (void)C;
+  return FX_OK;
+}
+
+static int launch_smooth(fx_mg_config* c, int l, float inv_dt, bool zero_guess, hipStream_t s) {
+  fx_mg_level& L = c->level[l];
+  LevelDev D = dev_level(L);
+  FX_DISPATCH_T(c->storage, FX_DISPATCH_OP(L.op, {
+    hipLaunchKernelGGL((k_smooth<TV, OPV>), level_grid(L), kBlock, 0, s, D, (const TV*)L.p, (const TV*)L.rhs,
+                       (TV*)L.q, inv_dt, c->omega, zero_guess ? 1 : 0);
+  }));
+  FX_LAUNCHED("k_smooth");
+  void* t = L.p;
+  L.p = L.q;
+  L.q = t;
+  return FX_OK;
+}
+
+static int launch_fill_zero(fx_mg_config* c, void* ptr, int64_t n, hipStream_t s);
+
+static int smooth_n(fx_mg_config* c, int l, int count, float inv_dt, bool zero_guess, hipStream_t s) {
+  for (int k = 0; k < count; ++k) {
+    if (int e = launch_smooth(c, l, inv_dt, zero_guess && k == 0, s)) return e;
+  }
+  return FX_OK;
+}
+
+static int launch_residual_restrict(fx_mg_config* c, int l, float inv_dt, hipStream_t s) {
+  fx_mg_level& F = c->level[l];
+  fx_mg_level& C = c->level[l + 1];
+  LevelDev D = dev_level(F);
+  if (c->restriction == FX_RESTRICT_MEAN) {
+    FX_DISPATCH_T(c->storage, FX_DISPATCH_OP(F.op, {
+      hipLaunchKernelGGL((k_residual_restrict<TV, OPV>), level_grid(C), kBlock, 0, s, D, C.nx, C.ny,
+                         (const TV*)F.p, (const TV*)F.rhs, (TV*)C.rhs, inv_dt);
+    }));
+    FX_LAUNCHED("k_residual_restrict");
+  } else {
+    // residual into the free ping-pong partner, then the 64-point restriction
+    FX_DISPATCH_T(c->storage, FX_DISPATCH_OP(F.op, {
+      hipLaunchKernelGGL((k_residual<TV, OPV>), level_grid(F), kBlock, 0, s, D, (const TV*)F.p,
+                         (const TV*)F.rhs, (TV*)F.q, inv_dt);
+    }));
+    FX_LAUNCHED("k_residual");
+    FX_DISPATCH_T(c->storage, {
+      hipLaunchKernelGGL((k_restrict_full<TV>), level_grid(C), kBlock, 0, s, C.nx, C.ny, C.nz, F.nx, F.ny, F.nz,
+                         (const TV*)F.q, (TV*)C.rhs);
+    });
+    FX_LAUNCHED("k_restrict_full");
+  }
+  return FX_OK;
+}
+
+static int launch_prolong(const fx_mg_config* c, int l, const void* in, const void* coarse, void* out,
+                          hipStream_t s) {
+  const fx_mg_level& F = c->level[l];
+  const fx_mg_level& C = c->level[l + 1];
+  LevelDev DF = dev_level(F), DC = dev_level(C);
+  int mode = c->prolongation;
+  FX_REQUIRE(mode >= 0 && mode <= 2, FX_EINVAL, "bad prolongation mode");
+  FX_REQUIRE(F.op != FX_OP_FLAGS || C.op != FX_OP_FLAGS, FX_EINVAL, "flags operators are level-0 only");
+#define FX_PRO(FO, CO, MO)                                                                             \
+  FX_DISPATCH_T(c->storage, {                                                                          \
+    hipLaunchKernelGGL((k_prolong<TV, FO, CO, MO>), level_grid(F), kBlock, 0, s, DF, DC, (const TV*)in, \
+                       (const TV*)coarse, (TV*)out);                                                   \
+  })
+  if (mode == FX_PROLONG_CONSTANT) {
+    if (F.op == FX_OP_EDGES) { FX_PRO(FX_OP_EDGES, FX_OP_EDGES, FX_PROLONG_CONSTANT); }
+    else if (F.op == FX_OP_FLAGS) { FX_PRO(FX_OP_FLAGS, FX_OP_EDGES, FX_PROLONG_CONSTANT); }
+    else { FX_PRO(FX_OP_BOX, FX_OP_BOX, FX_PROLONG_CONSTANT); }
+  } else if (mode == FX_PROLONG_TRILINEAR) {
+    if (F.op == FX_OP_BOX) { FX_PRO(FX_OP_BOX, FX_OP_BOX, FX_PROLONG_TRILINEAR); }
+    else if (F.op == FX_OP_FLAGS) { FX_PRO(FX_OP_FLAGS, FX_OP_BOX, FX_PROLONG_TRILINEAR); }
+    else { FX_PRO(FX_OP_EDGES, FX_OP_BOX, FX_PROLONG_TRILINEAR); }
+  } else {
+    FX_REQUIRE(C.op != FX_OP_FLAGS, FX_EINVAL, "renormalised prolongation needs a coarse edge or box level");
+    if (F.op == FX_OP_FLAGS) {
+      if (C.op == FX_OP_EDGES) { FX_PRO(FX_OP_FLAGS, FX_OP_EDGES, FX_PROLONG_RENORM); }
+      else { FX_PRO(FX_OP_FLAGS, FX_OP_BOX, FX_PROLONG_RENORM); }
+    } else if (F.op == FX_OP_EDGES) {
+      if (C.op == FX_OP_EDGES) { FX_PRO(FX_OP_EDGES, FX_OP_EDGES, FX_PROLONG_RENORM); }
+      else { FX_PRO(FX_OP_EDGES, FX_OP_BOX, FX_PROLONG_RENORM); }
+    } else {
+      if (C.op == FX_OP_EDGES) { FX_PRO(FX_OP_BOX, FX_OP_EDGES, FX_PROLONG_RENORM); }
+      else { FX_PRO(FX_OP_BOX, FX_OP_BOX, FX_PROLONG_RENORM); }
+    }
+  }
+#undef FX_PRO
+  FX_LAUNCHED("k_prolong");
+  return FX_OK;
+}
+
+static int launch_direct(const fx_mg_config* c, hipStream_t s) {
+  const fx_mg_level& C = c->level[c->levels - 1];
+  int m = C.nx * C.ny * C.nz;
+  FX_REQUIRE(m <= 4096, FX_EINVAL, "direct coarse solve limited to 4096 cells, got %d", m);
+  dim3 grid((m + 3) / 4);
+  FX_DISPATCH_T(c->storage, {
+    hipLaunchKernelGGL((k_direct<TV>), grid, dim3(256), 0, s, m, c->coarse_matrix, (const TV*)C.rhs, (TV*)C.p,
+                       c->coarse_scale);
+  });
+  FX_LAUNCHED("k_direct");
+  return FX_OK;
+}
+
+template <typename T>
+__global__ void k_zero(int64_t n, T* out) {
+  int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = from_f<T>(0.0f);
+}
+
+static int launch_fill_zero(fx_mg_config* c, void* ptr, int64_t n, hipStream_t s) {
+  FX_DISPATCH_T(c->storage, {
+    hipLaunchKernelGGL((k_zero<TV>), dim3((n + 255) / 256), dim3(256), 0, s, n, (TV*)ptr);
+  });
+  FX_LAUNCHED("k_zero");
+  return FX_OK;
+}
+
+// Recursive V-cycle.  zero_guess: the level's current iterate is 0.
+int vcycle(fx_mg_config* c, int l, float inv_dt, bool zero_guess, hipStream_t s) {
+  fx_mg_level& L = c->level[l];
+  if (l == c->levels - 1) {
+    if (c->coarse_sweeps > 0) return smooth_n(c, l, c->coarse_sweeps, inv_dt, zero_guess, s);
+    return launch_direct(c, s);
+  }
+  if (c->pre > 0) {
+    if (int e = smooth_n(c, l, c->pre, inv_dt, zero_guess, s)) return e;
+  } else if (zero_guess) {
+    if (int e = launch_fill_zero(c, L.p, int64_t(L.nx) * L.ny * L.nz, s)) return e;
+  }
+  if (int e = launch_residual_restrict(c, l, inv_dt, s)) return e;
+  if (int e = vcycle(c, l + 1, 1.0f, true, s)) return e;
+  if (int e = launch_prolong(c, l, L.p, c->level[l + 1].p, L.p, s)) return e;
+  return smooth_n(c, l, c->post, inv_dt, false, s);
+}
+
+int resnorm(const fx_mg_level& L0, const float* x, const float* b, double* ws, double* rn, double* bn,
+            hipStream_t s) {
+  LevelDev D = dev_level(L0);
+  hipLaunchKernelGGL(k_fill_d, dim3(1), dim3(64), 0, s, ws, 2);
+  FX_DISPATCH_OP(L0.op, {
+    hipLaunchKernelGGL((k_resnorm<OPV>), level_grid(L0), kBlock, 0, s, D, x, b, ws);
+  });
+  FX_LAUNCHED("k_resnorm");
+  double h[2];
+  FX_HIP(hipMemcpyAsync(h, ws, sizeof(h), hipMemcpyDeviceToHost, s));
+  FX_HIP(hipStreamSynchronize(s));
+  *rn = sqrt(h[0]);
+  *bn = sqrt(h[1]);
+  return FX_OK;
+}
+
+int project_mean_level(const fx_mg_level& L0, float* b, double* ws, hipStream_t s) {
+  LevelDev D = dev_level(L0);
+  hipLaunchKernelGGL(k_fill_d, dim3(1), dim3(64), 0, s, ws, 4);
+  FX_DISPATCH_OP(L0.op, {
+    hipLaunchKernelGGL((k_mean_sum<OPV>), level_grid(L0), kBlock, 0, s, D, b, ws);
+    hipLaunchKernelGGL((k_sub_mean<OPV>), level_grid(L0), kBlock, 0, s, D, b, ws);
+  });
+  FX_LAUNCHED("k_project_mean");
+  return FX_OK;
+}
+
+fx_mg_level level0_of(fx_grid g, const uint8_t* flags) {
+  fx_mg_level L{};
+  L.nx = g.nx; L.ny = g.ny; L.nz = g.nz;
+  L.op = flags ? FX_OP_FLAGS : FX_OP_BOX;
+  float ix = g.nx / g.ex, iy = g.ny / g.ey, iz = g.nz / g.ez;
+  L.w[0] = ix * ix; L.w[1] = iy * iy; L.w[2] = iz * iz;
+  L.op_data = flags;
+  return L;
+}
+
+}  // namespace fx
+
+using namespace fx;
+
+extern "C" {
+
+int fx_mg_vcycle(fx_mg_config* cfg, float inv_dt, void* stream) {
+  if (int e = check_config(cfg)) return e;
+  return vcycle(cfg, 0, inv_dt, false, S(stream));
+}
+
+int fx_mg_smooth(fx_mg_config* cfg, int32_t index, int32_t count, float inv_dt, void* stream) {
+  if (int e = check_config(cfg)) return e;
+  FX_REQUIRE(index >= 0 && index < cfg->levels && count >= 0, FX_EINVAL, "bad level or count");
+  return smooth_n(cfg, index, count, inv_dt, false, S(stream));
+}
+
+int fx_mg_residual(const fx_mg_config* cfg, int32_t index, float inv_dt, void* out, void* stream) {
+  if (int e = check_config(cfg)) return e;
+  FX_REQUIRE(index >= 0 && index < cfg->levels && out, FX_EINVAL, "bad level or output");
+  const fx_mg_level& L = cfg->level[index];
+  LevelDev D = dev_level(L);
+  FX_DISPATCH_T(cfg->storage, FX_DISPATCH_OP(L.op, {
+    hipLaunchKernelGGL((k_residual<TV, OPV>), level_grid(L), kBlock, 0, S(stream), D, (const TV*)L.p,
+                       (const TV*)L.rhs, (TV*)out, inv_dt);
+  }));
+  FX_LAUNCHED("fx_mg_residual");
+  return FX_OK;
+}
+
+int fx_mg_residual_restrict(fx_mg_config* cfg, int32_t index, float inv_dt, void* stream) {
+  if (int e = check_config(cfg)) return e;
+  FX_REQUIRE(index >= 0 && index + 1 < cfg->levels, FX_EINVAL, "bad level");
+  return launch_residual_restrict(cfg, index, inv_dt, S(stream));
+}
+
+int fx_mg_restrict(const fx_mg_config* cfg, int32_t index, const void* fine, void* coarse, void* stream) {
+  if (int e = check_config(cfg)) return e;
+  FX_REQUIRE(index >= 0 && index + 1 < cfg->levels && fine && coarse, FX_EINVAL, "bad level or arrays");
+  const fx_mg_level& F = cfg->level[index];
+  const fx_mg_level& C = cfg->level[index + 1];
+  if (cfg->restriction == FX_RESTRICT_MEAN) {
+    FX_DISPATCH_T(cfg->storage, {
+      hipLaunchKernelGGL((k_restrict_mean<TV>), level_grid(C), kBlock, 0, S(stream), C.nx, C.ny, F.nx, F.ny,
+                         (const TV*)fine, (TV*)coarse);
+    });
+  } else {
+    FX_DISPATCH_T(cfg->storage, {
+      hipLaunchKernelGGL((k_restrict_full<TV>), level_grid(C), kBlock, 0, S(stream), C.nx, C.ny, C.nz, F.nx, F.ny,
+                         F.nz, (const TV*)fine, (TV*)coarse);
+    });
+  }
+  FX_LAUNCHED("fx_mg_restrict");
+  return FX_OK;
+}
+
+int fx_mg_prolong(const fx_mg_config* cfg, int32_t index, const void* fine_in, const void* coarse, void* fine_out,
+                  void* stream) {
+  if (int e = check_config(cfg)) return e;
+  FX_REQUIRE(index >= 0 && index + 1 < cfg->levels && fine_in && coarse && fine_out, FX_EINVAL,
+             "bad level or arrays");
+  return launch_prolong(cfg, index, fine_in, coarse, fine_out, S(stream));
+}
+
+int fx_mg_direct(const fx_mg_config* cfg, void* stream) {
+  if (int e = check_config(cfg)) return e;
+  FX_REQUIRE(cfg->coarse_matrix, FX_EINVAL, "no coarse matrix");
+  return launch_direct(cfg, S(stream));
+}
+
+int fx_project_mean(fx_grid g, const uint8_t* flags, float* rhs, double* ws, void* stream) {
+  if (int e = check_grid(g)) return e;
+  FX_REQUIRE(rhs && ws, FX_EINVAL, "null array");
+  return project_mean_level(level0_of(g, flags), rhs, ws, S(stream));
+}
+
+int fx_residual_norm(fx_grid g, const uint8_t* flags, const float* x, const float* b, double* rnorm, double* bnorm,
+                     double* ws, void* stream) {
+  if (int e = check_grid(g)) return e;
+  FX_REQUIRE(x && b && ws && rnorm && bnorm, FX_EINVAL, "null array");
+  return resnorm(level0_of(g, flags), x, b, ws, rnorm, bnorm, S(stream));
+}
+
+int fx_mg_solve(fx_mg_config* cfg, float tol, int32_t max_cycles, double* ws, fx_solve_info* info, void* stream) {
+  if (int e = check_config(cfg)) return e;
+  FX_REQUIRE(cfg->storage == FX_F32, FX_EINVAL, "fx_mg_solve runs on fp32 levels");
+  FX_REQUIRE(ws && info && tol > 0 && max_cycles >= 0, FX_EINVAL, "bad solve arguments");
+  FX_REQUIRE(cfg->level[0].op != FX_OP_EDGES, FX_EINVAL, "level 0 must be a box or flags operator");
+  hipStream_t s = S(stream);
+  fx_solve_info out{};
+  double rn, bn;
+  fx_mg_level& L0 = cfg->level[0];
+  if (int e = resnorm(L0, (const float*)L0.p, (const float*)L0.rhs, ws, &rn, &bn, s)) return e;
+  out.bnorm = bn;
+  out.residual0 = rn;
+  out.residual = rn;
+  int it = 0;
+  while (!(rn <= double(tol) * bn) && it < max_cycles) {
+    if (int e = vcycle(cfg, 0, 1.0f, false, s)) return e;
+    ++it;
+    if (int e = resnorm(L0, (const float*)L0.p, (const float*)L0.rhs, ws, &rn, &bn, s)) return e;
+    FX_REQUIRE(std::isfinite(rn), FX_ESTATE, "non-finite residual after %d cycles", it);
+  }
+  out.iterations = it;
+  out.residual = rn;
+  out.converged = rn <= double(tol) * bn ? 1 : 0;
+  *info = out;
+  return FX_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,326 @@
+// Projection kernels: masks, walls, divergence, Jacobi, gradient.
+// One thread per cell or face, lanes of a wavefront walk the contiguous x axis.
+#include <cstdarg>
+#include <cstdio>
+
+#include "common.hpp"
+
+namespace fx {
+
+static thread_local char g_error[512] = "";
+
+void set_error(const char* fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_error, sizeof(g_error), fmt, ap);
+  va_end(ap);
+}
+
+int fail_hip(hipError_t e, const char* what) {
+  set_error("%s: %s", what, hipGetErrorString(e));
+  return FX_EDEVICE;
+}
+
+int check_grid(const fx_grid& g) {
+  FX_REQUIRE(g.nx >= 1 && g.ny >= 1 && g.nz >= 1 && g.nx <= 4096 && g.ny <= 4096 && g.nz <= 4096, FX_EINVAL,
+             "grid dimensions must be in [1, 4096], got (%d, %d, %d)", g.nx, g.ny, g.nz);
+  FX_REQUIRE(g.ex > 0 && g.ey > 0 && g.ez > 0, FX_EINVAL, "grid extent must be positive");
+  return FX_OK;
+}
+
+static const dim3 kBlock(64, 4, 1);
+
+// ---------------------------------------------------------------------------
+__global__ void k_solid_raster(int nx, int ny, int nz, float hx, float hy, float hz, int kind, float r00,
+                               float r01, float r02, float r03, float r10, float r11, float r12, float r13,
+                               float r20, float r21, float r22, float r23, uint8_t* solid) {
+  int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y * blockDim.y + threadIdx.y, z = blockIdx.z;
+  if (x >= nx || y >= ny) return;
+  float px = (x + 0.5f) * hx, py = (y + 0.5f) * hy, pz = (z + 0.5f) * hz;
+  float qx = r00 * px + r01 * py + r02 * pz + r03;
+  float qy = r10 * px + r11 * py + r12 * pz + r13;
+  float qz = r20 * px + r21 * py + r22 * pz + r23;
+  bool inside = kind == 0 ? (qx * qx + qy * qy + qz * qz <= 1.0f)
+                          : (fabsf(qx) <= 1.0f && fabsf(qy) <= 1.0f && fabsf(qz) <= 1.0f);
+  if (inside) solid[idx3(x, y, z, nx, ny)] = 1;
+}
+
+__global__ void k_build_flags(int nx, int ny, int nz, const uint8_t* __restrict__ solid, uint8_t* flags) {
+  int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y * blockDim.y + threadIdx.y, z = blockIdx.z;
+  if (x >= nx || y >= ny) return;
+  int64_t i = idx3(x, y, z, nx, ny);
+  uint8_t f = box_flags(x, y, z, nx, ny, nz);
+  if (solid) {
+    if (solid[i]) {
+      f = 0;
+    } else {
+      int64_t sx = 1, sy = nx, sz = int64_t(nx) * ny;
+      if ((f & FLAG_XLO) && solid[i - sx]) f &= ~FLAG_XLO;
+      if ((f & FLAG_XHI) && solid[i + sx]) f &= ~FLAG_XHI;
+      if ((f & FLAG_YLO) && solid[i - sy]) f &= ~FLAG_YLO;
+      if ((f & FLAG_YHI) && solid[i + sy]) f &= ~FLAG_YHI;
+      if ((f & FLAG_ZLO) && solid[i - sz]) f &= ~FLAG_ZLO;
+      if ((f & FLAG_ZHI) && solid[i + sz]) f &= ~FLAG_ZHI;
+    }
+  }
+  flags[i] = f;
+}
+
+__global__ void k_solid_clear(int64_t n, const uint8_t* __restrict__ solid, const float* __restrict__ in,
+                              float* __restrict__ out) {
+  int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = solid[i] ? 0.0f : in[i];
+}
+
+__global__ void k_fill(int64_t n, float v, float* __restrict__ out) {
+  int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = v;
+}
+
+__global__ void k_scale(int64_t n, float f, const float* __restrict__ in, float* __restrict__ out) {
+  int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = in[i] * f;
+}
+
+// Flags of the cell (x,y,z); box flags when no mask is given.
+__device__ __forceinline__ uint8_t cell_flags(const uint8_t* flags, int x, int y, int z, int nx, int ny,
+                                              int nz) {
+  return flags ? flags[idx3(x, y, z, nx, ny)] : box_flags(x, y, z, nx, ny, nz);
+}
+
+// A face along axis a at face index (x,y,z) is open iff it is interior and both
+// neighbouring cells are fluid: equivalently the high-side cell's low-face bit.
+__device__ __forceinline__ bool face_open(const uint8_t* flags, int a, int x, int y, int z, int nx, int ny,
+                                          int nz) {
+  int c = a == 0 ? x : (a == 1 ? y : z);
+  int n = a == 0 ? nx : (a == 1 ? ny : nz);
+  if (c <= 0 || c >= n) return false;
+  uint8_t f = cell_flags(flags, x, y, z, nx, ny, nz);
+  return (f >> (2 * a)) & 1;
+}
+
+// Threads cover (nx+1, ny+1, nz+1); each writes the U, V, W face it owns.
+__global__ void k_enforce_walls(int nx, int ny, int nz, const uint8_t* __restrict__ flags,
+                                const float* __restrict__ u, const float* __restrict__ v,
+                                const float* __restrict__ w, float* __restrict__ uo, float* __restrict__ vo,
+                                float* __restrict__ wo) {
+  int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y * blockDim.y + threadIdx.y, z = blockIdx.z;
+  if (x > nx || y > ny) return;
+  if (y < ny && z < nz) {
+    int64_t i = idx3(x, y, z, nx + 1, ny);
+    uo[i] = face_open(flags, 0, x, y, z, nx, ny, nz) ? u[i] : 0.0f;
+  }
+  if (x < nx && z < nz) {
+    int64_t i = idx3(x, y, z, nx, ny + 1);
+    vo[i] = face_open(flags, 1, x, y, z, nx, ny, nz) ? v[i] : 0.0f;
+  }
+  if (x < nx && y < ny) {
+    int64_t i = idx3(x, y, z, nx, ny);
+    wo[i] = face_open(flags, 2, x, y, z, nx, ny, nz) ? w[i] : 0.0f;
+  }
+}
+
+__global__ void k_divergence(int nx, int ny, int nz, float ix, float iy, float iz,
+                             const uint8_t* __restrict__ flags, const float* __restrict__ u,
+                             const float* __restrict__ v, const float* __restrict__ w, float* __restrict__ div) {
+  int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y * blockDim.y + threadIdx.y, z = blockIdx.z;
+  if (x >= nx || y >= ny) return;
+  uint8_t f = cell_flags(flags, x, y, z, nx, ny, nz);
+  float d = 0.0f;
+  if (f & FLAG_FLUID) {
+    int64_t iu = idx3(x, y, z, nx + 1, ny), iv = idx3(x, y, z, nx, ny + 1), iw = idx3(x, y, z, nx, ny);
+    float u0 = (f & FLAG_XLO) ? u[iu] : 0.0f, u1 = (f & FLAG_XHI) ? u[iu + 1] : 0.0f;
+    float v0 = (f & FLAG_YLO) ? v[iv] : 0.0f, v1 = (f & FLAG_YHI) ? v[iv + nx] : 0.0f;
+    float w0 = (f & FLAG_ZLO) ? w[iw] : 0.0f, w1 = (f & FLAG_ZHI) ? w[iw + int64_t(nx) * ny] : 0.0f;
+    d = (u1 - u0) * ix + (v1 - v0) * iy + (w1 - w0) * iz;
+  }
+  div[idx3(x, y, z, nx, ny)] = d;
+}
+
+__global__ void k_jacobi(int nx, int ny, int nz, float wx, float wy, float wz, float inv_dt, float omega,
+                         const uint8_t* __restrict__ flags, const float* __restrict__ p,
+                         const float* __restrict__ rhs, float* __restrict__ out) {
+  int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y * blockDim.y + threadIdx.y, z = blockIdx.z;
+  if (x >= nx || y >= ny) return;
+  int64_t i = idx3(x, y, z, nx, ny);
+  uint8_t f = cell_flags(flags, x, y, z, nx, ny, nz);
+  int64_t sy = nx, sz = int64_t(nx) * ny;
+  float s = 0.0f, d = 0.0f;
+  if (f & FLAG_XLO) { s += wx * p[i - 1]; d += wx; }
+  if (f & FLAG_XHI) { s += wx * p[i + 1]; d += wx; }
+  if (f & FLAG_YLO) { s += wy * p[i - sy]; d += wy; }
+  if (f & FLAG_YHI) { s += wy * p[i + sy]; d += wy; }
+  if (f & FLAG_ZLO) { s += wz * p[i - sz]; d += wz; }
+  if (f & FLAG_ZHI) { s += wz * p[i + sz]; d += wz; }
+  float r = 0.0f;
+  if ((f & FLAG_FLUID) && d != 0.0f) {
+    float cand = (s - rhs[i] * inv_dt) / d;
+    float old = p[i];
+    r = old + omega * (cand - old);
+  }
+  out[i] = r;
+}
+
+__global__ void k_gradient(int nx, int ny, int nz, float ix, float iy, float iz, float scale,
+                           const uint8_t* __restrict__ flags, const float* __restrict__ p,
+                           const float* __restrict__ u, const float* __restrict__ v, const float* __restrict__ w,
+                           float* __restrict__ uo, float* __restrict__ vo, float* __restrict__ wo) {
+  int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y * blockDim.y + threadIdx.y, z = blockIdx.z;
+  if (x > nx || y > ny) return;
+  int64_t sy = nx, sz = int64_t(nx) * ny;
+  int64_t c = idx3(x, y, z, nx, ny);
+  if (y < ny && z < nz) {
+    int64_t i = idx3(x, y, z, nx + 1, ny);
+    float r = 0.0f;
+    if (face_open(flags, 0, x, y, z, nx, ny, nz)) r = u[i] - scale * (p[c] - p[c - 1]) * ix;
+    uo[i] = r;
+  }
+  if (x < nx && z < nz) {
+    int64_t i = idx3(x, y, z, nx, ny + 1);
+    float r = 0.0f;
+    if (face_open(flags, 1, x, y, z, nx, ny, nz)) r = v[i] - scale * (p[c] - p[c - sy]) * iy;
+    vo[i] = r;
+  }
+  if (x < nx && y < ny) {
+    float r = 0.0f;
+    if (face_open(flags, 2, x, y, z, nx, ny, nz)) r = w[c] - scale * (p[c] - p[c - sz]) * iz;
+    wo[c] = r;
+  }
+}
+
+// Initial fields of reset(): kind 0 = scalar scale*w^2 of the soft sphere
+// (shaders/seed.glsl, temperature_seed.glsl); kind 1..3 = face component a of
+// base + rotation about the vertical axis (shaders/mac_seed.glsl).
+__global__ void k_seed(int nx, int ny, int nz, float ex, float ey, float ez, int kind, float cx, float cy, float cz,
+                       float radius, float scale, float bx, float by, float bz, float omega, float* out) {
+  int a = kind - 1;
+  int sx = nx + (a == 0), sy = ny + (a == 1), sz = nz + (a == 2);
+  int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y * blockDim.y + threadIdx.y, z = blockIdx.z;
+  if (x >= sx || y >= sy || z >= sz) return;
+  if (kind == 0) {
+    float px = (x + 0.5f) * ex / nx, py = (y + 0.5f) * ey / ny, pz = (z + 0.5f) * ez / nz;
+    float dx = px - cx, dy = py - cy, dz = pz - cz;
+    float w = fmaxf(1.0f - (dx * dx + dy * dy + dz * dz) / (radius * radius), 0.0f);
+    out[idx3(x, y, z, nx, ny)] = scale * w * w;
+    return;
+  }
+  float px = (x + (a == 0 ? 0.0f : 0.5f)) * ex / nx, py = (y + (a == 1 ? 0.0f : 0.5f)) * ey / ny;
+  float qx = px - 0.5f * ex, qy = py - 0.5f * ey;
+  float v = a == 0 ? bx - omega * qy : (a == 1 ? by + omega * qx : bz);
+  out[idx3(x, y, z, sx, sy)] = v;
+}
+
+}  // namespace fx
+
+using namespace fx;
+
+extern "C" {
+
+const char* fx_last_error(void) { return g_error; }
+int fx_version(void) { return 1; }
+
+int fx_solid_raster(fx_grid g, int32_t kind, const float* r, uint8_t* solid, void* stream) {
+  if (int e = check_grid(g)) return e;
+  FX_REQUIRE(kind == 0 || kind == 1, FX_EINVAL, "collider kind must be 0 (sphere) or 1 (box)");
+  FX_REQUIRE(r && solid, FX_EINVAL, "null pointer");
+  dim3 gr = grid3(g.nx, g.ny, g.nz, kBlock);
+  hipLaunchKernelGGL(k_solid_raster, gr, kBlock, 0, S(stream), g.nx, g.ny, g.nz, g.ex / g.nx, g.ey / g.ny,
+                     g.ez / g.nz, kind, r[0], r[1], r[2], r[3], r[4], r[5], r[6], r[7], r[8], r[9], r[10], r[11],
+                     solid);
+  FX_LAUNCHED("fx_solid_raster");
+  return FX_OK;
+}
+
+int fx_seed(fx_grid g, int32_t kind, const float* params9, float* out, void* stream) {
+  if (int e = check_grid(g)) return e;
+  FX_REQUIRE(kind >= 0 && kind <= 3 && params9 && out, FX_EINVAL, "seed: bad arguments");
+  const float* q = params9;
+  dim3 gr = grid3(g.nx + 1, g.ny + 1, g.nz + 1, kBlock);
+  hipLaunchKernelGGL(k_seed, gr, kBlock, 0, S(stream), g.nx, g.ny, g.nz, g.ex, g.ey, g.ez, kind, q[0], q[1], q[2],
+                     q[3], q[4], q[5], q[6], q[7], q[8], out);
+  FX_LAUNCHED("fx_seed");
+  return FX_OK;
+}
+
+int fx_build_flags(fx_grid g, const uint8_t* solid, uint8_t* flags, void* stream) {
+  if (int e = check_grid(g)) return e;
+  FX_REQUIRE(flags, FX_EINVAL, "null flags");
+  dim3 gr = grid3(g.nx, g.ny, g.nz, kBlock);
+  hipLaunchKernelGGL(k_build_flags, gr, kBlock, 0, S(stream), g.nx, g.ny, g.nz, solid, flags);
+  FX_LAUNCHED("fx_build_flags");
+  return FX_OK;
+}
+
+int fx_solid_clear(fx_grid g, const uint8_t* solid, const float* in, float* out, void* stream) {
+  if (int e = check_grid(g)) return e;
+  FX_REQUIRE(solid && in && out && in != out, FX_EINVAL, "solid_clear needs distinct input and output");
+  int64_t n = int64_t(g.nx) * g.ny * g.nz;
+  hipLaunchKernelGGL(k_solid_clear, dim3((n + 255) / 256), dim3(256), 0, S(stream), n, solid, in, out);
+  FX_LAUNCHED("fx_solid_clear");
+  return FX_OK;
+}
+
+int fx_fill(float* out, int64_t n, float value, void* stream) {
+  FX_REQUIRE(out && n >= 0, FX_EINVAL, "fill: bad arguments");
+  if (n == 0) return FX_OK;
+  hipLaunchKernelGGL(k_fill, dim3((n + 255) / 256), dim3(256), 0, S(stream), n, value, out);
+  FX_LAUNCHED("fx_fill");
+  return FX_OK;
+}
+
+int fx_scale(const float* in, float* out, int64_t n, float factor, void* stream) {
+  FX_REQUIRE(in && out && n >= 0, FX_EINVAL, "scale: bad arguments");
+  if (n == 0) return FX_OK;
+  hipLaunchKernelGGL(k_scale, dim3((n + 255) / 256), dim3(256), 0, S(stream), n, factor, in, out);
+  FX_LAUNCHED("fx_scale");
+  return FX_OK;
+}
+
+int fx_enforce_walls(fx_grid g, const uint8_t* flags, const float* u, const float* v, const float* w, float* uo,
+                     float* vo, float* wo, void* stream) {
+  if (int e = check_grid(g)) return e;
+  FX_REQUIRE(u && v && w && uo && vo && wo, FX_EINVAL, "null face array");
+  FX_REQUIRE(uo != u && vo != v && wo != w, FX_EINVAL, "Advection input/output textures must be distinct");
+  dim3 gr = grid3(g.nx + 1, g.ny + 1, g.nz + 1, kBlock);
+  hipLaunchKernelGGL(k_enforce_walls, gr, kBlock, 0, S(stream), g.nx, g.ny, g.nz, flags, u, v, w, uo, vo, wo);
+  FX_LAUNCHED("fx_enforce_walls");
+  return FX_OK;
+}
+
+int fx_divergence(fx_grid g, const uint8_t* flags, const float* u, const float* v, const float* w, float* div,
+                  void* stream) {
+  if (int e = check_grid(g)) return e;
+  FX_REQUIRE(u && v && w && div, FX_EINVAL, "null array");
+  dim3 gr = grid3(g.nx, g.ny, g.nz, kBlock);
+  hipLaunchKernelGGL(k_divergence, gr, kBlock, 0, S(stream), g.nx, g.ny, g.nz, g.nx / g.ex, g.ny / g.ey,
+                     g.nz / g.ez, flags, u, v, w, div);
+  FX_LAUNCHED("fx_divergence");
+  return FX_OK;
+}
+
+int fx_jacobi_sweep(fx_grid g, const uint8_t* flags, const float* p, const float* rhs, float* out, float inv_dt,
+                    float omega, void* stream) {
+  if (int e = check_grid(g)) return e;
+  FX_REQUIRE(p && rhs && out && out != p && out != rhs, FX_EINVAL,
+             "Advection input/output textures must be distinct");
+  float ix = g.nx / g.ex, iy = g.ny / g.ey, iz = g.nz / g.ez;
+  dim3 gr = grid3(g.nx, g.ny, g.nz, kBlock);
+  hipLaunchKernelGGL(k_jacobi, gr, kBlock, 0, S(stream), g.nx, g.ny, g.nz, ix * ix, iy * iy, iz * iz, inv_dt,
+                     omega, flags, p, rhs, out);
+  FX_LAUNCHED("fx_jacobi_sweep");
+  return FX_OK;
+}
+
+int fx_gradient(fx_grid g, const uint8_t* flags, const float* p, const float* u, const float* v, const float* w,
+                float* uo, float* vo, float* wo, float scale, void* stream) {
+  if (int e = check_grid(g)) return e;
+  FX_REQUIRE(p && u && v && w && uo && vo && wo, FX_EINVAL, "null array");
+  FX_REQUIRE(uo != u && vo != v && wo != w, FX_EINVAL, "Advection input/output textures must be distinct");
+  dim3 gr = grid3(g.nx + 1, g.ny + 1, g.nz + 1, kBlock);
+  hipLaunchKernelGGL(k_gradient, gr, kBlock, 0, S(stream), g.nx, g.ny, g.nz, g.nx / g.ex, g.ny / g.ey,
+                     g.nz / g.ez, scale, flags, p, u, v, w, uo, vo, wo);
+  FX_LAUNCHED("fx_gradient");
+  return FX_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,277 @@
+/*
+ * libfluxhip: C-ABI of the MI355X (gfx950) dense smoke/fire step.
+ *
+ * Every entry point takes plain integers, floats and pointers, returns 0 on
+ * success or a negative FX_E* code, and leaves a message for fx_last_error().
+ * Device pointers come from the caller (torch tensors own the memory); no entry
+ * point allocates device memory.  `stream` is a hipStream_t (NULL = default).
+ *
+ * Layout everywhere is the reference's flat x-fast index x + nx*(y + ny*z)
+ * (physics/reference.py:3).  Face arrays are U (nx+1,ny,nz), V (nx,ny+1,nz),
+ * W (nx,ny,nz+1) (physics/config.py:21-23).  The vertical (buoyancy) axis is z.
+ *
+ * Each declaration cites the reference interface it replaces.  In the
+ * reference the boundary is a GLSL dispatch (backend/device.py:57-74) issued by
+ * the Python solver classes; here it is one C call per fused pass.
+ */
+#ifndef FLUXHIP_H
+#define FLUXHIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  FX_OK = 0,
+  FX_EINVAL = -1,   /* invalid argument  -> Python ValueError   */
+  FX_EDEVICE = -2,  /* HIP runtime error -> GPUUnavailable      */
+  FX_ESTATE = -3    /* misuse of a handle -> RuntimeError       */
+};
+
+/* Grid descriptor: cell counts and domain extent in metres
+ * (physics/config.py:6-23 GridSpec.shape/extent). */
+typedef struct fx_grid {
+  int32_t nx, ny, nz;
+  float ex, ey, ez;
+} fx_grid;
+
+/* Packed source table: `count` rows of 16 floats in the row order of
+ * physics/emission.py:49-58 (centre xyz, radius, start xyz, coupling,
+ * velocity xyz, density rate, heat rate, target flag, solid-profile flag,
+ * fuel rate).  `multi` selects the source-table semantics of
+ * shaders/multi_advect_*.glsl (density clamped to >= the max target);
+ * 0 selects the single-emitter uniforms of shaders/closed_advect_scalar.glsl and
+ * shaders/emitting_advect_velocity.glsl. */
+typedef struct fx_sources {
+  int32_t count;
+  int32_t multi;
+  float table[128];
+} fx_sources;
+
+const char* fx_last_error(void);
+int fx_version(void);
+
+/* --------------------------------------------------------------------------
+ * masks and flags
+ * ------------------------------------------------------------------------ */
+
+/* Union one analytic collider into a uint8 solid mask (1 = solid).
+ * kind 0 = sphere, 1 = box; rows = 3x4 inverse affine rows.
+ * Replaces SolidDevice mask construction (backend/collision.py:24-36,
+ * shaders/solid_mask.glsl:1-9). */
+int fx_solid_raster(fx_grid g, int32_t kind, const float* rows12, uint8_t* solid, void* stream);
+
+/* Per-cell open-face bits of the level-0 pressure operator (bit 2a: low face of
+ * axis a open, bit 2a+1: high face open, bit 6: fluid cell).  solid may be NULL
+ * (closed box).  Encodes shaders/solid_boundary.glsl:2-10 solidCell/solidFace. */
+int fx_build_flags(fx_grid g, const uint8_t* solid, uint8_t* flags, void* stream);
+
+/* out = solid ? 0 : in (shaders/solid_clear.glsl:1-6; backend/collision.py:65-66). */
+int fx_solid_clear(fx_grid g, const uint8_t* solid, const float* in, float* out, void* stream);
+
+/* Initial fields of reset(): kind 0 writes the cell field scale*w^2 of the soft
+ * sphere (shaders/seed.glsl, temperature_seed.glsl); kind 1..3 writes face
+ * component kind-1 of base + rotation about z (shaders/mac_seed.glsl).
+ * params9 = centre xyz, radius, scale, base velocity xyz, angular speed. */
+int fx_seed(fx_grid g, int32_t kind, const float* params9, float* out, void* stream);
+
+/* out[i] = value (graph-safe clear; replaces texture.clear). */
+int fx_fill(float* out, int64_t n, float value, void* stream);
+
+/* out[i] = factor * in[i] (shaders/pressure_scale.glsl:1-6). */
+int fx_scale(const float* in, float* out, int64_t n, float factor, void* stream);
+
+/* --------------------------------------------------------------------------
+ * projection (backend/pressure.py:49-98)
+ * ------------------------------------------------------------------------ */
+
+/* Zero normal velocity on wall and solid faces, copy the rest
+ * (PressureProjector.enforce_walls, backend/pressure.py:49-51,
+ * shaders/pressure_walls.glsl:1-15).  flags NULL = closed box. */
+int fx_enforce_walls(fx_grid g, const uint8_t* flags, const float* u, const float* v, const float* w,
+                     float* uo, float* vo, float* wo, void* stream);
+
+/* div = sum_a (u_a[c+e_a]-u_a[c])/h_a with wall/solid faces read as 0; 0 in
+ * solid cells (compute_divergence, backend/pressure.py:53-55,
+ * shaders/pressure_divergence.glsl:1-12, applied to enforce_walls' output). */
+int fx_divergence(fx_grid g, const uint8_t* flags, const float* u, const float* v, const float* w,
+                  float* div, void* stream);
+
+/* One weighted-Jacobi sweep of L p = rhs*inv_dt with Neumann walls and solids
+ * (shaders/pressure_jacobi.glsl:2-34; backend/pressure.py:95-98). */
+int fx_jacobi_sweep(fx_grid g, const uint8_t* flags, const float* p, const float* rhs, float* out,
+                    float inv_dt, float omega, void* stream);
+
+/* Open faces: out = in - scale*(p[c]-p[c-e])/h; wall/solid faces 0
+ * (backend/pressure.py:76-80, shaders/pressure_gradient.glsl:1-21 fused with
+ * pressure_walls.glsl so the bounded copies need not exist). */
+int fx_gradient(fx_grid g, const uint8_t* flags, const float* p, const float* u, const float* v,
+                const float* w, float* uo, float* vo, float* wo, float scale, void* stream);
+
+/* --------------------------------------------------------------------------
+ * multigrid engine (backend/multigrid.py:20-95, backend/solid_multigrid.py:7-76)
+ * ------------------------------------------------------------------------ */
+enum { FX_F32 = 0, FX_BF16 = 1, FX_F16 = 2 };
+enum { FX_OP_BOX = 0, FX_OP_FLAGS = 1, FX_OP_EDGES = 2 };
+enum { FX_RESTRICT_MEAN = 0, FX_RESTRICT_FULL = 1 };
+enum { FX_PROLONG_TRILINEAR = 0, FX_PROLONG_RENORM = 1, FX_PROLONG_CONSTANT = 2 };
+
+/* One level.  op BOX: Neumann box with weights w[a] = (n_a/extent_a)^2;
+ * FLAGS: op_data = uint8 open-face bits (fx_build_flags), weights w[a];
+ * EDGES: op_data = float4 (w+x, w+y, w+z, active) absolute conductances to the
+ * positive neighbour (the RGBA32F edge records of physics/solid_hierarchy.py:11-47).
+ * p/q are the ping-pong solution arrays, rhs the right-hand side, all in the
+ * config's storage type. */
+typedef struct fx_mg_level {
+  int32_t nx, ny, nz, op;
+  float w[3];
+  float pad_;
+  const void* op_data;
+  void* p;
+  void* q;
+  void* rhs;
+} fx_mg_level;
+
+typedef struct fx_mg_config {
+  int32_t levels;
+  int32_t storage;        /* FX_F32 / FX_BF16 / FX_F16 */
+  int32_t pre, post;      /* smoothing sweeps per level */
+  float omega;            /* Jacobi relaxation */
+  int32_t restriction;    /* FX_RESTRICT_* */
+  int32_t prolongation;   /* FX_PROLONG_* */
+  int32_t coarse_sweeps;  /* > 0: smooth the coarsest level this many times; 0: direct */
+  const double* coarse_matrix; /* device, m x m FP64 pseudo-inverse, m = cells of coarsest */
+  float coarse_scale;     /* direct: p = coarse_scale * M rhs */
+  int32_t pad_;
+  fx_mg_level level[16];
+} fx_mg_config;
+
+/* One V-cycle on level 0 (MultigridPressureProjector.cycle,
+ * backend/multigrid.py:57-81; SolidMultigridProjector.cycle,
+ * backend/solid_multigrid.py:46-64).  inv_dt scales the level-0 rhs (coarse
+ * levels use 1, as the reference recurses with dt=1).  Ping-pong swaps are
+ * written back into cfg->level[i].p/q. */
+int fx_mg_vcycle(fx_mg_config* cfg, float inv_dt, void* stream);
+
+/* `count` smoothing sweeps of one level (MultigridPressureProjector.smooth,
+ * backend/multigrid.py:50-55).  Swaps are written back into the level. */
+int fx_mg_smooth(fx_mg_config* cfg, int32_t index, int32_t count, float inv_dt, void* stream);
+
+/* r = rhs*inv_dt - L p on one level (shaders/mg_residual.glsl:1-13). */
+int fx_mg_residual(const fx_mg_config* cfg, int32_t index, float inv_dt, void* out, void* stream);
+
+/* coarse rhs = restriction of fine residual, fused (mg_residual + mg_restrict). */
+int fx_mg_residual_restrict(fx_mg_config* cfg, int32_t index, float inv_dt, void* stream);
+
+/* coarse rhs = restriction of an arbitrary fine array (shaders/mg_restrict.glsl:2-9). */
+int fx_mg_restrict(const fx_mg_config* cfg, int32_t index, const void* fine, void* coarse, void* stream);
+
+/* fine out = fine in + prolongation(coarse) (shaders/mg_prolong.glsl:2-15,
+ * shaders/solid_mg_prolong.glsl:1-7). */
+int fx_mg_prolong(const fx_mg_config* cfg, int32_t index, const void* fine_in, const void* coarse,
+                  void* fine_out, void* stream);
+
+/* Coarsest direct solve p = scale * M rhs (shaders/coarse_pressure.glsl:1-12,
+ * shaders/solid_mg_direct.glsl:1-12), FP64 accumulation. */
+int fx_mg_direct(const fx_mg_config* cfg, void* stream);
+
+/* Solve info returned by the tolerance-driven solvers (no reference
+ * counterpart: the reference runs fixed cycle budgets). */
+typedef struct fx_solve_info {
+  int32_t iterations;   /* V-cycles or Krylov iterations taken */
+  int32_t converged;    /* 1 if residual <= tol * bnorm */
+  int32_t fallback;     /* learned FPCG: 1 if plain MG finished the solve */
+  int32_t pad_;
+  double bnorm;         /* ||b||_2 after mean projection */
+  double residual0;     /* ||b - A x0||_2 */
+  double residual;      /* final true ||b - A x||_2 (FP64 norm of the FP32 residual) */
+} fx_solve_info;
+
+/* Remove the fluid-cell mean from rhs (Neumann compatibility), FP64 sums.
+ * ws: >= 4 doubles of device scratch. */
+int fx_project_mean(fx_grid g, const uint8_t* flags, float* rhs, double* ws, void* stream);
+
+/* ||b - A x||_2 and ||b||_2 in FP64 for the level-0 operator (flags NULL = box).
+ * Synchronises the stream; writes host doubles. ws: >= 4 doubles. */
+int fx_residual_norm(fx_grid g, const uint8_t* flags, const float* x, const float* b,
+                     double* rnorm, double* bnorm, double* ws, void* stream);
+
+/* Stationary V-cycles on level 0 of cfg (storage FX_F32) until
+ * ||b - A x|| <= tol*||b|| or max_cycles.  x = level[0].p, b = level[0].rhs.
+ * ws: >= 8 doubles.  Synchronises once per cycle. */
+int fx_mg_solve(fx_mg_config* cfg, float tol, int32_t max_cycles, double* ws, fx_solve_info* info,
+                void* stream);
+
+/* --------------------------------------------------------------------------
+ * transport (backend/detail.py:16-37, backend/projected.py:150-195)
+ * ------------------------------------------------------------------------ */
+
+/* Face predictor for U, V and W (DetailPasses.transport_velocity,
+ * backend/detail.py:27-37, shaders/transport_face.glsl:1-20). */
+int fx_face_predict(fx_grid g, float dt, const float* u, const float* v, const float* w, float* pu,
+                    float* pv, float* pw, const uint8_t* solid, void* stream);
+
+typedef struct fx_velocity_args {
+  fx_grid g;
+  float dt;
+  int32_t maccormack;
+  const float* u; const float* v; const float* w;      /* old velocity */
+  const float* pu; const float* pv; const float* pw;   /* face predictors (maccormack) */
+  float* ou; float* ov; float* ow;                     /* new velocity (provisional W) */
+  const float* density; const float* temperature;      /* pre-transport, for buoyancy; may be NULL */
+  float thermal_lift; float density_weight;
+  const uint8_t* solid;
+  const fx_sources* sources;                           /* host pointer */
+} fx_velocity_args;
+
+/* Fused velocity SL/MacCormack transport + jet relaxation + buoyancy on W
+ * (backend/projected.py:155-167: emitting_advect_velocity.glsl /
+ * multi_advect_velocity.glsl with correct_velocity_fused.glsl, then
+ * buoyancy.glsl). */
+int fx_velocity_advect(const fx_velocity_args* a, void* stream);
+
+/* Scalar predictor for up to 3 channels (DetailPasses.transport with +dt,
+ * backend/detail.py:16-25, shaders/transport_scalar.glsl:1-19). */
+int fx_scalar_predict(fx_grid g, float dt, const float* u, const float* v, const float* w, int32_t channels,
+                      const float* const* fields, float* const* preds, const uint8_t* solid, void* stream);
+
+typedef struct fx_scalar_args {
+  fx_grid g;
+  float dt;
+  int32_t maccormack;
+  int32_t channels;                /* 1..3 */
+  int32_t boundary;                /* 0 closed box (clamped indices), 1 zero extension */
+  const float* u; const float* v; const float* w;     /* projected velocity */
+  const float* field[3];
+  const float* pred[3];            /* maccormack predictors */
+  float* out[3];
+  float decay[3];                  /* exp(-decay*dt) factor per channel */
+  int32_t channel_id[3];           /* 0 density, 1 temperature, 2 fuel */
+  const uint8_t* solid;
+  const fx_sources* sources;       /* host pointer */
+} fx_scalar_args;
+
+/* Fused closed-box scalar transport (reverse sampling, MacCormack correction,
+ * clamp, decay, sources) for up to 3 channels (backend/projected.py:178-195,
+ * shaders/closed_advect_scalar.glsl, multi_advect_scalar.glsl,
+ * correct_scalar.glsl, closed_sample.glsl).  boundary = 1 gives the
+ * zero-extension sampling of shaders/mac_advect_density.glsl (backend/mac.py:60-76). */
+int fx_scalar_advect(const fx_scalar_args* a, void* stream);
+
+/* Fused combustion: flame = burned/dt, fuel/density/temperature updated in one
+ * pass (CombustionPasses.react, backend/combustion.py:21-31,
+ * shaders/combustion_burn.glsl, combustion_apply.glsl). */
+int fx_combustion(int64_t n, float dt, float ignition, float burn_rate, float heat_yield, float smoke_yield,
+                  const float* fuel, const float* temperature, const float* density, float* fuel_out,
+                  float* temperature_out, float* density_out, float* flame, void* stream);
+
+/* max |x| over `count` arrays into out[i] (device floats), NaN/Inf -> 3.4e38
+ * (AdaptiveTimestep.bounds, backend/timestep.py:24-33, shaders/max_reduce.glsl). */
+int fx_max_abs(int32_t count, const float* const* arrays, const int64_t* sizes, float* out, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,20 @@
+import os
+import sys
+from pathlib import Path
+
+import pytest
+
+ROOT = Path(__file__).resolve().parents[1]
+if str(ROOT) not in sys.path:
+    sys.path.insert(0, str(ROOT))
+
+
+def pytest_configure(config):
+    config.addinivalue_line("markers", "gpu: needs an MI355X (HIP device) and the built libfluxhip.so")
+
+
+@pytest.fixture(scope="session")
+def device():
+    """HipDevice on cuda:0; GPU tests fail loudly if the library or device is missing."""
+    from fluxfx_amd.backend.device import HipDevice
+    return HipDevice(0)

@@ -1,0 +1,131 @@
+"""Whole-step parity: DenseProjectedSmoke on libfluxhip vs the FP64 oracle step
+over several steps (fields compared with a relative tolerance that allows for
+FP32 accumulation over the steps and for MacCormack clamp decisions)."""
+import numpy as np
+import pytest
+
+from oracle import mac, transport as otr, step as ostep
+from tests.helpers import faces_np, cells_np, rel_err, sphere_mask
+
+pytestmark = pytest.mark.gpu
+
+
+def _case(device, shape, settings, solid=None, steps=3, dt=1 / 128, seed=0, multi_sources=None):
+    from fluxfx_amd.physics.config import GridSpec
+    from fluxfx_amd.backend.projected import DenseProjectedSmoke
+    grid = GridSpec(shape, (1., 1., 1.), max_dim=512)
+    og = mac.Grid(shape, (1., 1., 1.))
+    sim = DenseProjectedSmoke(grid, settings, device, solid_mask=None if solid is None else solid.ravel())
+    rng = np.random.default_rng(seed)
+    vel = tuple(rng.uniform(-.05, .05, og.face_array_shape(a)).astype(np.float32).astype(np.float64)
+                for a in range(3))
+    sim.upload_velocity([v.ravel() for v in vel])
+    sim.upload(np.zeros(og.array_shape).ravel())
+    sim.upload_temperature(np.zeros(og.array_shape).ravel())
+    state = ostep.State(og, mac.walls(vel, og, solid) if solid is not None else vel,
+                        np.zeros(og.array_shape), np.zeros(og.array_shape), solid=solid)
+    if settings.combustion_enabled:
+        fuel = rng.uniform(0, 1, og.array_shape).astype(np.float32).astype(np.float64)
+        sim.upload_fuel(fuel.ravel())
+        state.fuel = fuel
+    s = settings
+    if multi_sources is None:
+        srcs = [otr.Src(s.source_center, s.source_radius, s.source_rate, s.heat_source_rate,
+                        target=s.density_mode == "TARGET", solid_profile=s.source_profile == "SOLID",
+                        fuel_rate=s.fuel_source_rate if s.combustion_enabled else 0.0)]
+    else:
+        srcs = multi_sources[1]
+    if s.pressure_tolerance > 0:
+        cfg = ostep.PressureConfig("TOLERANCE", tol=s.pressure_tolerance, pre=s.mg_pre, post=s.mg_post,
+                                   coarsest=s.mg_coarsest)
+    elif s.pressure_solver == "JACOBI":
+        cfg = ostep.PressureConfig("SOLID_JACOBI" if solid is not None else "JACOBI",
+                                   iterations=s.pressure_iterations, cold_multiplier=s.pressure_cold_multiplier)
+    else:
+        cfg = ostep.PressureConfig("MULTIGRID", cycles=s.pressure_cycles)
+    cache = {}
+    for _ in range(steps):
+        sim.step(dt, sources=multi_sources[0] if multi_sources else None)
+        ostep.step(state, s, dt, srcs, cfg, multi=multi_sources is not None, mg_cache=cache)
+    return sim, state, og
+
+
+def _compare(sim, state, og, tol):
+    errs = {"density": rel_err(cells_np(og, sim.density), state.density),
+            "temperature": rel_err(cells_np(og, sim.temperature), state.temperature)}
+    for a, (g, o) in enumerate(zip(faces_np(og, sim._velocity), state.velocity)):
+        errs["uvw"[a]] = rel_err(g, o)
+    bad = {k: v for k, v in errs.items() if not v < tol}
+    assert not bad, errs
+    return errs
+
+
+def _settings(**kw):
+    from fluxfx_amd.physics.pressure import PressureSettings
+    base = dict(source_center=(.5, .5, .16), source_radius=.1, source_rate=1.0, heat_source_rate=1.0,
+                dissipation=0.0, cooling=0.0, thermal_lift=1.0, density_weight=0.0, source_profile="SOLID",
+                initial_temperature=0.0)
+    base.update(kw)
+    return PressureSettings(**base)
+
+
+def test_step_jacobi_semi_lagrangian(device):
+    s = _settings(pressure_solver="JACOBI", pressure_iterations=64, pressure_cold_multiplier=1)
+    sim, st, og = _case(device, (24, 24, 24), s, steps=3)
+    _compare(sim, st, og, 1e-4)
+
+
+def test_step_jacobi_maccormack(device):
+    s = _settings(pressure_solver="JACOBI", pressure_iterations=64, pressure_cold_multiplier=1,
+                  velocity_advection="MACCORMACK", scalar_advection="MACCORMACK")
+    sim, st, og = _case(device, (24, 24, 24), s, steps=3)
+    _compare(sim, st, og, 2e-4)
+
+
+def test_step_reference_multigrid(device):
+    s = _settings(pressure_solver="MULTIGRID", pressure_cycles=2, velocity_advection="MACCORMACK",
+                  scalar_advection="MACCORMACK")
+    sim, st, og = _case(device, (32, 32, 32), s, steps=3)
+    _compare(sim, st, og, 2e-4)
+    assert sim.measure_projection()["v_cycles"] == 2
+
+
+def test_step_tolerance_with_solids_and_combustion(device):
+    og = mac.Grid((32, 32, 32))
+    solid = sphere_mask(og, (16, 16, 18), 5)
+    s = _settings(pressure_tolerance=1e-5, mg_pre=2, mg_post=2, mg_coarsest=8, velocity_advection="MACCORMACK",
+                  scalar_advection="MACCORMACK", combustion_enabled=True, ignition_temperature=.3, burn_rate=15.,
+                  heat_yield=1., smoke_yield=.5, fuel_source_rate=1.0)
+    sim, st, og = _case(device, (32, 32, 32), s, solid=solid, steps=3)
+    # the pressure solve stops at 1e-5 relative residual in both: looser field bound
+    _compare(sim, st, og, 2e-3)
+    assert rel_err(cells_np(og, sim.combustion.fuel), st.fuel) < 2e-3
+    m = sim.measure_projection()
+    assert m["converged"] and m["relative_residual"] <= 1e-5
+
+
+def test_step_multi_sources_order_independent(device):
+    from fluxfx_amd.physics.emission import Source, Emission
+    s = _settings(pressure_solver="JACOBI", pressure_iterations=32, pressure_cold_multiplier=1)
+    recs = [Source((.5, .5, .3), .15, 1, .5, Emission((.5, .5, .3)), "TARGET", "SOLID"),
+            Source((.45, .5, .3), .12, 2, 1.0, Emission((.4, .5, .25), (0, 0, 1), 4.0), "RATE", "SOFT")]
+    osrc = [otr.Src(r.center, r.radius, r.density_rate, r.heat_rate, r.motion.start, r.motion.velocity,
+                    r.motion.coupling, r.density_mode == "TARGET", r.profile == "SOLID", r.fuel_rate) for r in recs]
+    a, st, og = _case(device, (20, 20, 20), s, steps=2, multi_sources=(recs, osrc))
+    _compare(a, st, og, 1e-4)
+    b, _, _ = _case(device, (20, 20, 20), s, steps=2, multi_sources=(recs[::-1], osrc[::-1]))
+    assert np.abs(cells_np(og, a.density) - cells_np(og, b.density)).max() < 1e-6
+
+
+def test_fault_flag_and_errors(device):
+    from fluxfx_amd.physics.config import GridSpec
+    from fluxfx_amd.backend.projected import DenseProjectedSmoke
+    sim = DenseProjectedSmoke(GridSpec((16, 16, 16)), _settings(pressure_solver="JACOBI"), device)
+    with pytest.raises(ValueError):
+        sim.step(0.2)
+    with pytest.raises(ValueError):
+        sim.upload([-1.0] * 16 ** 3)
+    with pytest.raises(RuntimeError):
+        sim.projector.metrics()
+    sim.step(1 / 60)
+    assert sim.projector.ready and sim.steps == 1
