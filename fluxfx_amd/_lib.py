@@ -14,6 +14,7 @@ FX_OP_BOX, FX_OP_FLAGS, FX_OP_EDGES = 0, 1, 2
 FX_RESTRICT_MEAN, FX_RESTRICT_FULL = 0, 1
 FX_PROLONG_TRILINEAR, FX_PROLONG_RENORM, FX_PROLONG_CONSTANT = 0, 1, 2
 MAX_LEVELS = 16
+FX_WS_DOUBLES = 16400  # FP64 reduction workspace (16 scalars + 16 x 1024 block partials)
 
 
 class GPUUnavailable(RuntimeError):
@@ -48,6 +49,11 @@ class fx_solve_info(C.Structure):
                 ("residual", C.c_double)]
 
 
+class fx_solve_params(C.Structure):
+    _fields_ = [("tol", C.c_float), ("max_iters", C.c_int32), ("check_from", C.c_int32), ("ref_q", C.c_int32),
+                ("x_is_zero", C.c_int32), ("pad_", C.c_int32)]
+
+
 class fx_velocity_args(C.Structure):
     _fields_ = [("g", fx_grid), ("dt", C.c_float), ("maccormack", C.c_int32),
                 ("u", C.c_void_p), ("v", C.c_void_p), ("w", C.c_void_p),
@@ -55,7 +61,7 @@ class fx_velocity_args(C.Structure):
                 ("ou", C.c_void_p), ("ov", C.c_void_p), ("ow", C.c_void_p),
                 ("density", C.c_void_p), ("temperature", C.c_void_p),
                 ("thermal_lift", C.c_float), ("density_weight", C.c_float),
-                ("solid", C.c_void_p), ("sources", C.POINTER(fx_sources))]
+                ("solid", C.c_void_p), ("clearance", C.c_void_p), ("sources", C.POINTER(fx_sources))]
 
 
 class fx_scalar_args(C.Structure):
@@ -63,7 +69,7 @@ class fx_scalar_args(C.Structure):
                 ("boundary", C.c_int32), ("u", C.c_void_p), ("v", C.c_void_p), ("w", C.c_void_p),
                 ("field", C.c_void_p * 3), ("pred", C.c_void_p * 3), ("out", C.c_void_p * 3),
                 ("decay", C.c_float * 3), ("channel_id", C.c_int32 * 3), ("solid", C.c_void_p),
-                ("sources", C.POINTER(fx_sources))]
+                ("clearance", C.c_void_p), ("sources", C.POINTER(fx_sources))]
 
 
 P = C.c_void_p
@@ -82,6 +88,9 @@ SIGNATURES = {
     "fx_divergence": ([fx_grid, P, P, P, P, P, P], C.c_int),
     "fx_jacobi_sweep": ([fx_grid, P, P, P, P, F32, F32, P], C.c_int),
     "fx_gradient": ([fx_grid, P, P, P, P, P, P, P, P, F32, P], C.c_int),
+    "fx_gradient_split": ([fx_grid, P, P, P, P, P, P, P, P, P, F32, P], C.c_int),
+    "fx_increment_rhs": ([fx_grid, P, P, P, P, P, P], C.c_int),
+    "fx_combine": ([P, P, P, I64, P], C.c_int),
     "fx_mg_vcycle": ([C.POINTER(fx_mg_config), F32, P], C.c_int),
     "fx_mg_smooth": ([C.POINTER(fx_mg_config), I32, I32, F32, P], C.c_int),
     "fx_mg_residual": ([C.POINTER(fx_mg_config), I32, F32, P, P], C.c_int),
@@ -91,10 +100,16 @@ SIGNATURES = {
     "fx_mg_direct": ([C.POINTER(fx_mg_config), P], C.c_int),
     "fx_project_mean": ([fx_grid, P, P, P, P], C.c_int),
     "fx_residual_norm": ([fx_grid, P, P, P, C.POINTER(C.c_double), C.POINTER(C.c_double), P, P], C.c_int),
-    "fx_mg_solve": ([C.POINTER(fx_mg_config), F32, I32, P, C.POINTER(fx_solve_info), P], C.c_int),
-    "fx_face_predict": ([fx_grid, F32, P, P, P, P, P, P, P, P], C.c_int),
+    "fx_mg_solve": ([C.POINTER(fx_mg_config), C.POINTER(fx_solve_params), P, C.POINTER(fx_solve_info), P],
+                    C.c_int),
+    "fx_mixed_solve": ([C.POINTER(fx_mg_config), P, P, P, C.POINTER(fx_solve_params), P,
+                        C.POINTER(fx_solve_info), C.POINTER(C.POINTER(C.c_float)), P], C.c_int),
+    "fx_pcg_solve": ([C.POINTER(fx_mg_config), P, P, P, P, P, P, C.POINTER(fx_solve_params), P,
+                      C.POINTER(fx_solve_info), P], C.c_int),
+    "fx_face_predict": ([fx_grid, F32, P, P, P, P, P, P, P, P, P], C.c_int),
+    "fx_build_clearance": ([fx_grid, P, P, P, P], C.c_int),
     "fx_velocity_advect": ([C.POINTER(fx_velocity_args), P], C.c_int),
-    "fx_scalar_predict": ([fx_grid, F32, P, P, P, I32, C.POINTER(P), C.POINTER(P), P, P], C.c_int),
+    "fx_scalar_predict": ([fx_grid, F32, P, P, P, I32, C.POINTER(P), C.POINTER(P), P, P, P], C.c_int),
     "fx_scalar_advect": ([C.POINTER(fx_scalar_args), P], C.c_int),
     "fx_combustion": ([I64, F32, F32, F32, F32, F32, P, P, P, P, P, P, P, P], C.c_int),
     "fx_max_abs": ([I32, C.POINTER(P), C.POINTER(I64), P, P], C.c_int),

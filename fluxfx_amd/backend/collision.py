@@ -39,6 +39,14 @@ class SolidMask:
                           self.solid.data_ptr(), self.device.stream)
         self.flags = self.device.zeros(n, "uint8")
         _lib.call("fx_build_flags", self._g, self.solid.data_ptr(), self.flags.data_ptr(), self.device.stream)
+        # Chebyshev clearance to the nearest solid/wall cell: lets the transport
+        # kernels skip trace clipping far from solids
+        self.clearance = self.device.zeros(n, "uint8")
+        scratch = self.device.zeros(n, "uint8")
+        _lib.call("fx_build_clearance", self._g, self.solid.data_ptr(), self.clearance.data_ptr(),
+                  scratch.data_ptr(), self.device.stream)
+        self.device.synchronize()
+        del scratch
 
     @property
     def mask(self):
@@ -46,7 +54,7 @@ class SolidMask:
 
     @property
     def allocated_bytes(self):
-        return 2 * self.grid.cells
+        return 3 * self.grid.cells
 
     def numpy(self):
         nx, ny, nz = self.grid.shape
@@ -57,4 +65,4 @@ class SolidMask:
         _lib.call("fx_solid_clear", self._g, self.solid.data_ptr(), source.ptr, target.ptr, self.device.stream)
 
     def close(self):
-        self.solid = self.flags = None
+        self.solid = self.flags = self.clearance = None

@@ -32,12 +32,16 @@ class DetailPasses:
     def _solid(self):
         return self.solids.solid if self.solids is not None else None
 
+    @property
+    def _clearance(self):
+        return self.solids.clearance if self.solids is not None else None
+
     def _predict(self, fields, outs, velocities, dt):
         n = len(fields)
         fa = (C.c_void_p * n)(*[ptr(f) for f in fields])
         oa = (C.c_void_p * n)(*[ptr(f) for f in outs])
         _lib.call("fx_scalar_predict", self._g, float(dt), *(ptr(v) for v in velocities), n, fa, oa,
-                  ptr(self._solid), self.device.stream)
+                  ptr(self._solid), ptr(self._clearance), self.device.stream)
 
     def transport(self, field, velocities, dt):
         """Predictor (dt) and reverse (-dt) semi-Lagrangian passes of one scalar."""
@@ -63,7 +67,8 @@ class DetailPasses:
         if self.face_predictor is None:
             self.face_predictor = [self.device.texture(s) for s in self.grid.face_shapes]
         _lib.call("fx_face_predict", self._g, float(dt), *(ptr(f) for f in fields),
-                  *(ptr(f) for f in self.face_predictor), ptr(self._solid), self.device.stream)
+                  *(ptr(f) for f in self.face_predictor), ptr(self._solid), ptr(self._clearance),
+                  self.device.stream)
         return self.face_predictor
 
     def confine(self, fields, outputs, dt, strength, limit):

@@ -222,9 +222,9 @@ class MGLevels:
     def smooth(self, index, count, inv_dt=1.0):
         _lib.call("fx_mg_smooth", C.byref(self.cfg), int(index), int(count), float(inv_dt), self.device.stream)
 
-    def solve(self, tol, max_cycles, ws):
+    def solve(self, params, ws):
         info = _lib.fx_solve_info()
-        _lib.call("fx_mg_solve", C.byref(self.cfg), float(tol), int(max_cycles), ws.data_ptr(), C.byref(info),
+        _lib.call("fx_mg_solve", C.byref(self.cfg), C.byref(params), ws.data_ptr(), C.byref(info),
                   self.device.stream)
         return info
 

@@ -127,10 +127,12 @@ class MultigridPressureProjector(PressureProjector):
         _lib.call("fx_mg_residual", C.byref(sub), 0, float(1.0 / dt), out.data_ptr(), self.device.stream)
 
     def restrict(self, index, fine, coarse):
+        self._bind()
         _lib.call("fx_mg_restrict", C.byref(self.engine.cfg), int(index), fine.data_ptr(), coarse.data_ptr(),
                   self.device.stream)
 
     def prolong(self, index, fine_in, coarse, fine_out):
+        self._bind()
         _lib.call("fx_mg_prolong", C.byref(self.engine.cfg), int(index), fine_in.data_ptr(), coarse.data_ptr(),
                   fine_out.data_ptr(), self.device.stream)
 

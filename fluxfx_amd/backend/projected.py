@@ -166,12 +166,13 @@ class DenseProjectedSmoke(DenseThermalAdvection):
             raise NotImplementedError("turbulence and vorticity confinement are not part of this build")
         table = self._multi_sources if emitters is not None else self._single_source(emission)
         solid = self.solids.solid if self.solids is not None else None
+        clearance = self.solids.clearance if self.solids is not None else None
         velocity_detail = s.velocity_advection == "MACCORMACK"
         preds = self.detail.transport_velocity(self._velocity, dt) if velocity_detail else None
         provisional = [self._velocity_back[0], self._velocity_back[1], self._buoyancy_scratch]
         self._advect_velocity(self._velocity, provisional, dt, maccormack=velocity_detail, predictors=preds,
                               sources=table, density=self._front, temperature=self._temperature,
-                              lift=s.thermal_lift, weight=s.density_weight, solid=solid)
+                              lift=s.thermal_lift, weight=s.density_weight, solid=solid, clearance=clearance)
         # the old velocity is consumed: its storage receives the projected field
         self.projector.project(provisional, self._velocity, dt)
         fields = [self._front, self._temperature]
@@ -197,6 +198,7 @@ class DenseProjectedSmoke(DenseThermalAdvection):
             args.decay[k] = d
             args.channel_id[k] = k
         args.solid = ptr(solid)
+        args.clearance = ptr(clearance)
         args.sources = C.pointer(table)
         _lib.call("fx_scalar_advect", C.byref(args), self.device.stream)
         self._front, self._back = self._back, self._front

@@ -20,6 +20,8 @@ Modes (``pressure_precision`` x ``pressure_krylov``):
   LEARNED       flexible CG with the U-Net preconditioner; plain FP32 MG
                 finishes the solve whenever the true residual misses tol
 """
+import ctypes as C
+
 import numpy as np
 
 from .. import _lib
@@ -71,9 +73,13 @@ class ToleranceProjector(PressureProjector):
         self.coarse_matrix = np.linalg.pinv(edges_matrix(last), rcond=1e-12, hermitian=True)
         self.restriction = _lib.FX_RESTRICT_FULL if s.mg_restriction == "FULL_WEIGHTING" else _lib.FX_RESTRICT_MEAN
         self.engine = self._engine(_lib.FX_F32, own_level0=False)
-        self.ws = self.device.zeros(16, "float64")
+        self.ws = self.device.zeros(_lib.FX_WS_DOUBLES, "float64")
         self.last_cycles = 0
         self.last_info = None
+        self._pengine = None
+        self._kvec = None
+        self._inc = None
+        self._check_from = 0
 
     def _engine(self, storage, own_level0):
         s = self.settings
@@ -93,30 +99,118 @@ class ToleranceProjector(PressureProjector):
     def reset(self):
         super().reset()
         self.last_cycles = 0
+        self._check_from = 0
         for arr in self.engine.arrays[1:]:
             for t in arr.values():
                 t.zero_()
 
     def solve_pressure(self, dt, previous_dt):
+        """Solve for the increment d over the warm start x0 = pressure:
+        L d = r0 = b - L x0, stopping at ||r0 - L d|| <= tol*||b||.  The
+        gradient then uses x0 and d separately (fx_gradient_split), so the
+        impulse keeps more than fp32 precision where it matters."""
         if previous_dt != dt:
             self.pressure.clear(format="FLOAT", value=(0.0,))
         s = self.settings
-        _lib.call("fx_project_mean", self._g, self.flags_ptr, self.before.ptr, self.ws.data_ptr(),
-                  self.device.stream)
+        stream = self.device.stream
+        _lib.call("fx_project_mean", self._g, self.flags_ptr, self.before.ptr, self.ws.data_ptr(), stream)
+        inc = self._increment_fields()
+        # r0 = b - L x0 with ws[6] = ||r0||^2, ws[7] = ||b||^2 (no host sync)
+        _lib.call("fx_increment_rhs", self._g, self.flags_ptr, self.pressure.ptr, self.before.ptr, inc["r0"].ptr,
+                  self.ws.data_ptr(), stream)
+        inc["d"].clear()
+        params = _lib.fx_solve_params(float(s.pressure_tolerance), int(s.pressure_max_cycles),
+                                      int(self._check_from), 7, 1, 0)
         if s.pressure_precision == "FP32" and s.pressure_krylov == "NONE":
-            eng = self.engine
-            eng.bind_level0(self.pressure.tensor, self.spare.tensor, self.before.tensor)
-            info = eng.solve(s.pressure_tolerance, s.pressure_max_cycles, self.ws)
-            if eng.tensor(0, "p") is not self.pressure.tensor:
-                self.pressure, self.spare = self.spare, self.pressure
+            info = self._solve_stationary(params)
+        elif s.pressure_krylov == "NONE":
+            info = self._solve_refinement(params)
+        elif s.pressure_krylov == "PCG":
+            info = self._solve_pcg(params)
         else:
-            raise NotImplementedError(f"{s.pressure_precision}/{s.pressure_krylov} solve")
+            info = self._solve_learned(params)
+        # the next solve checks convergence from one iteration before this one's count
+        self._check_from = max(0, info.iterations - 1)
+        target = s.pressure_tolerance * info.bnorm
+        info.converged = int(info.residual <= target)
         self.last_info = info
         self.last_cycles = info.iterations
         self.last_iterations = info.iterations * (len(self.shapes) - 1) * (s.mg_pre + s.mg_post)
         if not info.converged:
             raise RuntimeError(f"pressure solve missed tolerance {s.pressure_tolerance:g}: "
                                f"residual {info.residual / max(info.bnorm, 1e-300):.3e} after {info.iterations}")
+
+    def apply_gradient(self, fields, outputs, scale):
+        inc = self._increment_fields()
+        _lib.call("fx_gradient_split", self._g, self.flags_ptr, self.pressure.ptr, inc["d"].ptr,
+                  *(f.ptr for f in fields), *(f.ptr for f in outputs), float(scale), self.device.stream)
+        # impulse for the next warm start and for diagnostics: x0 + d, fluid mean removed
+        _lib.call("fx_combine", self.pressure.ptr, inc["d"].ptr, self.spare.ptr, self.pressure.size, self.device.stream)
+        self.pressure, self.spare = self.spare, self.pressure
+        _lib.call("fx_project_mean", self._g, self.flags_ptr, self.pressure.ptr, self.ws.data_ptr(), self.device.stream)
+
+    def _increment_fields(self):
+        if self._inc is None:
+            self._inc = {k: self.device.texture(self.grid.shape) for k in ("r0", "d", "d2")}
+        return self._inc
+
+    # -- solvers ---------------------------------------------------------------------
+    def _swap_increment(self, tensor):
+        inc = self._inc
+        if tensor is not inc["d"].tensor:
+            inc["d"], inc["d2"] = inc["d2"], inc["d"]
+
+    def _solve_stationary(self, params):
+        eng = self.engine
+        inc = self._inc
+        eng.bind_level0(inc["d"].tensor, inc["d2"].tensor, inc["r0"].tensor)
+        info = eng.solve(params, self.ws)
+        self._swap_increment(eng.tensor(0, "p"))
+        return info
+
+    def _precond_engine(self):
+        """V-cycle engine with its own level-0 arrays in the configured storage."""
+        if self._pengine is None:
+            storage = STORAGE[self.settings.pressure_precision]
+            self._pengine = self._engine(storage, own_level0=True)
+            n = self.grid.cells
+            self._pengine.arrays[0]["z"] = self.device.zeros(n, self._pengine.dtype)
+        return self._pengine
+
+    def _krylov_vectors(self):
+        if self._kvec is None:
+            n = self.grid.cells
+            self._kvec = {k: self.device.zeros(n) for k in ("r", "p", "Ap")}
+        return self._kvec
+
+    def _solve_refinement(self, params):
+        eng = self._precond_engine()
+        inc = self._inc
+        info = _lib.fx_solve_info()
+        xout = C.POINTER(C.c_float)()
+        _lib.call("fx_mixed_solve", C.byref(eng.cfg), inc["d"].ptr, inc["d2"].ptr, inc["r0"].ptr, C.byref(params),
+                  self.ws.data_ptr(), C.byref(info), C.byref(xout), self.device.stream)
+        if C.cast(xout, C.c_void_p).value == inc["d2"].ptr:
+            inc["d"], inc["d2"] = inc["d2"], inc["d"]
+        return info
+
+    def _solve_pcg(self, params):
+        eng = self._precond_engine()
+        v = self._krylov_vectors()
+        inc = self._inc
+        info = _lib.fx_solve_info()
+        # the C side swaps the preconditioned residual with level-0 p: the free
+        # buffer of the three is the one level 0 does not point at
+        arr = eng.arrays[0]
+        used = {eng.cfg.level[0].p, eng.cfg.level[0].q}
+        zold = next(t for t in (arr["z"], arr["p"], arr["q"]) if t.data_ptr() not in used)
+        _lib.call("fx_pcg_solve", C.byref(eng.cfg), inc["d"].ptr, inc["r0"].ptr, v["r"].data_ptr(),
+                  v["p"].data_ptr(), v["Ap"].data_ptr(), zold.data_ptr(), C.byref(params), self.ws.data_ptr(),
+                  C.byref(info), self.device.stream)
+        return info
+
+    def _solve_learned(self, params):
+        raise NotImplementedError("learned preconditioner is not built into this projector")
 
     def solve_metrics(self):
         i = self.last_info

@@ -125,6 +125,69 @@ __device__ bool safe_correction(const GridDev& g, const uint8_t* __restrict__ s,
   return dist3(clip_trace(g, s, p, arr), arr) < 1e-7f;
 }
 
+// Clearance: Chebyshev distance (cells, capped at 255) from a cell to the
+// nearest solid or out-of-range cell.  A segment whose cells all lie within
+// that distance of its origin cell crosses no solid, so the DDA can be skipped;
+// a 3x3x3 halo is solid-free exactly when the clearance exceeds 1.
+__device__ __forceinline__ int clearance_at(const GridDev& g, const uint8_t* __restrict__ cl, V3 p) {
+  int cx = int(floorf(p.x / g.ex * g.nx)), cy = int(floorf(p.y / g.ey * g.ny)), cz = int(floorf(p.z / g.ez * g.nz));
+  if (cx < 0 || cy < 0 || cz < 0 || cx >= g.nx || cy >= g.ny || cz >= g.nz) return 0;
+  return cl[idx3(cx, cy, cz, g.nx, g.ny)];
+}
+
+__device__ __forceinline__ V3 clip_fast(const GridDev& g, const uint8_t* __restrict__ s,
+                                        const uint8_t* __restrict__ cl, V3 o, V3 t) {
+  if (cl) {
+    float r = fmaxf(fmaxf(fabsf((t.x - o.x) / g.ex * g.nx), fabsf((t.y - o.y) / g.ey * g.ny)),
+                    fabsf((t.z - o.z) / g.ez * g.nz));
+    if (float(clearance_at(g, cl, o)) > ceilf(r) + 1.0f) return t;
+  }
+  return clip_trace(g, s, o, t);
+}
+
+__device__ __forceinline__ bool halo_fast(const GridDev& g, const uint8_t* __restrict__ s,
+                                          const uint8_t* __restrict__ cl, V3 p) {
+  if (cl) return clearance_at(g, cl, p) <= 1;
+  return solid_halo(g, s, p);
+}
+
+__device__ bool safe_fast(const GridDev& g, const uint8_t* __restrict__ s, const uint8_t* __restrict__ cl, V3 p,
+                          V3 dep, V3 mid, V3 arr) {
+  if (halo_fast(g, s, cl, p) || halo_fast(g, s, cl, dep) || halo_fast(g, s, cl, arr)) return false;
+  if (dist3(clip_fast(g, s, cl, p, dep), dep) >= 1e-7f) return false;
+  if (dist3(clip_fast(g, s, cl, p, mid), mid) >= 1e-7f) return false;
+  return dist3(clip_fast(g, s, cl, p, arr), arr) < 1e-7f;
+}
+
+// separable L-infinity distance transform, one axis per pass:
+// pass 0: d = min |k| over solid/out-of-range cells c+k*e_x (|k| <= 254)
+// pass 1,2: d = min over |k| of max(|k|, d_prev[c+k*e_a]), out-of-range -> |k|
+__global__ void k_clearance(int nx, int ny, int nz, int axis, const uint8_t* __restrict__ in, uint8_t* __restrict__ out,
+                            int radius) {
+  int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y * blockDim.y + threadIdx.y, z = blockIdx.z;
+  if (x >= nx || y >= ny) return;
+  int n = axis == 0 ? nx : (axis == 1 ? ny : nz);
+  int c = axis == 0 ? x : (axis == 1 ? y : z);
+  int64_t stride = axis == 0 ? 1 : (axis == 1 ? int64_t(nx) : int64_t(nx) * ny);
+  int64_t i = idx3(x, y, z, nx, ny);
+  int best = 255;
+  for (int k = -radius; k <= radius; ++k) {
+    int ak = k < 0 ? -k : k;
+    if (ak >= best) continue;
+    int cc = c + k;
+    int v;
+    if (cc < 0 || cc >= n) {
+      v = ak;
+    } else {
+      uint8_t src = in[i + k * stride];
+      v = axis == 0 ? (src ? 0 : 255) : src;
+      if (v < 255) v = max(v, ak);
+    }
+    best = min(best, v);
+  }
+  out[i] = uint8_t(best);
+}
+
 // ---------------------------------------------------------------------------
 // emission: exact time-average of the compact sphere profile along the sweep
 // ---------------------------------------------------------------------------
@@ -164,7 +227,7 @@ __device__ float emission_weight(V3 p, const float* row) {
 // ---------------------------------------------------------------------------
 template <int AXIS, bool SOLID>
 __global__ void k_face_predict(GridDev g, Vel V, float dt, const float* __restrict__ in, float* __restrict__ out,
-                               const uint8_t* __restrict__ solid) {
+                               const uint8_t* __restrict__ solid, const uint8_t* __restrict__ clear) {
   const int sx = g.nx + (AXIS == 0), sy = g.ny + (AXIS == 1), sz = g.nz + (AXIS == 2);
   int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y * blockDim.y + threadIdx.y, z = blockIdx.z;
   if (x >= sx || y >= sy) return;
@@ -181,10 +244,10 @@ __global__ void k_face_predict(GridDev g, Vel V, float dt, const float* __restri
   V3 p = v3((x + offx) * hx, (y + offy) * hy, (z + offz) * hz);
   V3 vp = velocity_at(g, V, p);
   V3 mid = v3(p.x - 0.5f * dt * vp.x, p.y - 0.5f * dt * vp.y, p.z - 0.5f * dt * vp.z);
-  if (SOLID) mid = clip_trace(g, solid, p, mid);
+  if (SOLID) mid = clip_fast(g, solid, clear, p, mid);
   V3 vm = velocity_at(g, V, mid);
   V3 dep = v3(p.x - dt * vm.x, p.y - dt * vm.y, p.z - dt * vm.z);
-  if (SOLID) dep = clip_trace(g, solid, p, dep);
+  if (SOLID) dep = clip_fast(g, solid, clear, p, dep);
   out[i] = sample_face(in, sx, sy, sz, dep.x / hx - offx, dep.y / hy - offy, dep.z / hz - offz);
 }
 
@@ -201,6 +264,7 @@ struct VelArgs {
   const float* temperature;
   float lift, weight;
   const uint8_t* solid;
+  const uint8_t* clear;
 };
 
 template <int AXIS, bool SOLID, bool MC>
@@ -224,10 +288,10 @@ __global__ void k_velocity(VelArgs A, fx_sources src) {
   if (!blocked) {
     V3 vp = velocity_at(g, A.V, p);
     V3 mid = v3(p.x - 0.5f * dt * vp.x, p.y - 0.5f * dt * vp.y, p.z - 0.5f * dt * vp.z);
-    if (SOLID) mid = clip_trace(g, A.solid, p, mid);
+    if (SOLID) mid = clip_fast(g, A.solid, A.clear, p, mid);
     V3 vm = velocity_at(g, A.V, mid);
     V3 dep = v3(p.x - dt * vm.x, p.y - dt * vm.y, p.z - dt * vm.z);
-    if (SOLID) dep = clip_trace(g, A.solid, p, dep);
+    if (SOLID) dep = clip_fast(g, A.solid, A.clear, p, dep);
     float qx = dep.x / hx - offx, qy = dep.y / hy - offy, qz = dep.z / hz - offz;
     float low = sample_face(orig, sx, sy, sz, qx, qy, qz);
     value = low;
@@ -238,7 +302,7 @@ __global__ void k_velocity(VelArgs A, fx_sources src) {
       V3 va = velocity_at(g, A.V, rmid);
       V3 arr = v3(p.x + dt * va.x, p.y + dt * va.y, p.z + dt * va.z);
       bool ok = true;
-      if (SOLID) ok = safe_correction(g, A.solid, p, dep, rmid, arr);
+      if (SOLID) ok = safe_fast(g, A.solid, A.clear, p, dep, rmid, arr);
       ok = ok && !(dep.x < lbx || dep.y < lby || dep.z < lbz || dep.x > hbx || dep.y > hby || dep.z > hbz);
       ok = ok && !(arr.x < lbx || arr.y < lby || arr.z < lbz || arr.x > hbx || arr.y > hby || arr.z > hbz);
       if (ok) {
@@ -326,7 +390,7 @@ struct ScalarSet {
 
 template <bool SOLID>
 __global__ void k_scalar_predict(GridDev g, Vel V, float dt, int channels, ScalarSet S_,
-                                 const uint8_t* __restrict__ solid) {
+                                 const uint8_t* __restrict__ solid, const uint8_t* __restrict__ clear) {
   int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y * blockDim.y + threadIdx.y, z = blockIdx.z;
   if (x >= g.nx || y >= g.ny) return;
   int64_t i = idx3(x, y, z, g.nx, g.ny);
@@ -338,10 +402,10 @@ __global__ void k_scalar_predict(GridDev g, Vel V, float dt, int channels, Scala
   V3 p = v3((x + 0.5f) * hx, (y + 0.5f) * hy, (z + 0.5f) * hz);
   V3 vp = velocity_at(g, V, p);
   V3 mid = v3(p.x - 0.5f * dt * vp.x, p.y - 0.5f * dt * vp.y, p.z - 0.5f * dt * vp.z);
-  if (SOLID) mid = clip_trace(g, solid, p, mid);
+  if (SOLID) mid = clip_fast(g, solid, clear, p, mid);
   V3 vm = velocity_at(g, V, mid);
   V3 dep = v3(p.x - dt * vm.x, p.y - dt * vm.y, p.z - dt * vm.z);
-  if (SOLID) dep = clip_trace(g, solid, p, dep);
+  if (SOLID) dep = clip_fast(g, solid, clear, p, dep);
   Corners c;
   closed_corners(g, dep.x / hx - 0.5f, dep.y / hy - 0.5f, dep.z / hz - 0.5f, c);
   for (int k = 0; k < channels; ++k) S_.o[k][i] = closed_sample<SOLID>(S_.f[k], c, solid);
@@ -359,6 +423,7 @@ struct ScalarArgs {
   int chan[3];
   int zero_ext;
   const uint8_t* solid;
+  const uint8_t* clear;
 };
 
 template <bool SOLID, bool MC>
@@ -376,10 +441,10 @@ __global__ void k_scalar_advect(ScalarArgs A, fx_sources src) {
   V3 p = v3((x + 0.5f) * hx, (y + 0.5f) * hy, (z + 0.5f) * hz);
   V3 vp = velocity_at(g, A.V, p);
   V3 mid = v3(p.x - 0.5f * dt * vp.x, p.y - 0.5f * dt * vp.y, p.z - 0.5f * dt * vp.z);
-  if (SOLID) mid = clip_trace(g, A.solid, p, mid);
+  if (SOLID) mid = clip_fast(g, A.solid, A.clear, p, mid);
   V3 vm = velocity_at(g, A.V, mid);
   V3 dep = v3(p.x - dt * vm.x, p.y - dt * vm.y, p.z - dt * vm.z);
-  if (SOLID) dep = clip_trace(g, A.solid, p, dep);
+  if (SOLID) dep = clip_fast(g, A.solid, A.clear, p, dep);
   Corners c;
   closed_corners(g, dep.x / hx - 0.5f, dep.y / hy - 0.5f, dep.z / hz - 0.5f, c, A.zero_ext != 0);
   bool ok = false;
@@ -389,7 +454,7 @@ __global__ void k_scalar_advect(ScalarArgs A, fx_sources src) {
     V3 va = velocity_at(g, A.V, rmid);
     V3 arr = v3(p.x + dt * va.x, p.y + dt * va.y, p.z + dt * va.z);
     ok = true;
-    if (SOLID) ok = safe_correction(g, A.solid, p, dep, rmid, arr);
+    if (SOLID) ok = safe_fast(g, A.solid, A.clear, p, dep, rmid, arr);
     float lx = 0.5f * hx, ly = 0.5f * hy, lz = 0.5f * hz;
     float ux = g.ex - 0.5f * hx, uy = g.ey - 0.5f * hy, uz = g.ez - 0.5f * hz;
     ok = ok && !(dep.x < lx || dep.y < ly || dep.z < lz || dep.x > ux || dep.y > uy || dep.z > uz);
@@ -499,7 +564,7 @@ using namespace fx;
 extern "C" {
 
 int fx_face_predict(fx_grid g, float dt, const float* u, const float* v, const float* w, float* pu, float* pv,
-                    float* pw, const uint8_t* solid, void* stream) {
+                    float* pw, const uint8_t* solid, const uint8_t* clearance, void* stream) {
   if (int e = check_grid(g)) return e;
   FX_REQUIRE(u && v && w && pu && pv && pw, FX_EINVAL, "null face array");
   FX_REQUIRE(pu != u && pv != v && pw != w && pu != v && pu != w && pv != u && pv != w && pw != u && pw != v,
@@ -511,13 +576,13 @@ int fx_face_predict(fx_grid g, float dt, const float* u, const float* v, const f
   dim3 gx = grid3(g.nx + 1, g.ny, g.nz, kBlock), gy = grid3(g.nx, g.ny + 1, g.nz, kBlock),
        gz = grid3(g.nx, g.ny, g.nz + 1, kBlock);
   if (solid) {
-    hipLaunchKernelGGL((k_face_predict<0, true>), gx, kBlock, 0, s, G, V, dt, u, pu, solid);
-    hipLaunchKernelGGL((k_face_predict<1, true>), gy, kBlock, 0, s, G, V, dt, v, pv, solid);
-    hipLaunchKernelGGL((k_face_predict<2, true>), gz, kBlock, 0, s, G, V, dt, w, pw, solid);
+    hipLaunchKernelGGL((k_face_predict<0, true>), gx, kBlock, 0, s, G, V, dt, u, pu, solid, clearance);
+    hipLaunchKernelGGL((k_face_predict<1, true>), gy, kBlock, 0, s, G, V, dt, v, pv, solid, clearance);
+    hipLaunchKernelGGL((k_face_predict<2, true>), gz, kBlock, 0, s, G, V, dt, w, pw, solid, clearance);
   } else {
-    hipLaunchKernelGGL((k_face_predict<0, false>), gx, kBlock, 0, s, G, V, dt, u, pu, solid);
-    hipLaunchKernelGGL((k_face_predict<1, false>), gy, kBlock, 0, s, G, V, dt, v, pv, solid);
-    hipLaunchKernelGGL((k_face_predict<2, false>), gz, kBlock, 0, s, G, V, dt, w, pw, solid);
+    hipLaunchKernelGGL((k_face_predict<0, false>), gx, kBlock, 0, s, G, V, dt, u, pu, solid, clearance);
+    hipLaunchKernelGGL((k_face_predict<1, false>), gy, kBlock, 0, s, G, V, dt, v, pv, solid, clearance);
+    hipLaunchKernelGGL((k_face_predict<2, false>), gz, kBlock, 0, s, G, V, dt, w, pw, solid, clearance);
   }
   FX_LAUNCHED("fx_face_predict");
   return FX_OK;
@@ -545,7 +610,7 @@ int fx_velocity_advect(const fx_velocity_args* a, void* stream) {
   float* o3[3] = {a->ou, a->ov, a->ow};
   for (int axis = 0; axis < 3; ++axis) {
     VelArgs A{G, V, a->dt, preds[axis], o3[axis], a->density, a->temperature, a->thermal_lift, a->density_weight,
-              a->solid};
+              a->solid, a->solid ? a->clearance : nullptr};
 #define FX_VEL(AX, SO, MC) hipLaunchKernelGGL((k_velocity<AX, SO, MC>), grids[AX], kBlock, 0, s, A, *a->sources)
 #define FX_VEL_AX(AX)                                             \
   if (a->solid) {                                                 \
@@ -562,7 +627,8 @@ int fx_velocity_advect(const fx_velocity_args* a, void* stream) {
 }
 
 int fx_scalar_predict(fx_grid g, float dt, const float* u, const float* v, const float* w, int32_t channels,
-                      const float* const* fields, float* const* preds, const uint8_t* solid, void* stream) {
+                      const float* const* fields, float* const* preds, const uint8_t* solid,
+                      const uint8_t* clearance, void* stream) {
   if (int e = check_grid(g)) return e;
   FX_REQUIRE(channels >= 1 && channels <= 3, FX_EINVAL, "channels must be 1..3");
   FX_REQUIRE(u && v && w && fields && preds, FX_EINVAL, "null array");
@@ -576,9 +642,11 @@ int fx_scalar_predict(fx_grid g, float dt, const float* u, const float* v, const
   dim3 gr = grid3(g.nx, g.ny, g.nz, kBlock);
   Vel V{u, v, w};
   if (solid)
-    hipLaunchKernelGGL((k_scalar_predict<true>), gr, kBlock, 0, S(stream), dev_grid(g), V, dt, channels, set, solid);
+    hipLaunchKernelGGL((k_scalar_predict<true>), gr, kBlock, 0, S(stream), dev_grid(g), V, dt, channels, set, solid,
+                       clearance);
   else
-    hipLaunchKernelGGL((k_scalar_predict<false>), gr, kBlock, 0, S(stream), dev_grid(g), V, dt, channels, set, solid);
+    hipLaunchKernelGGL((k_scalar_predict<false>), gr, kBlock, 0, S(stream), dev_grid(g), V, dt, channels, set, solid,
+                       (const uint8_t*)nullptr);
   FX_LAUNCHED("fx_scalar_predict");
   return FX_OK;
 }
@@ -595,6 +663,7 @@ int fx_scalar_advect(const fx_scalar_args* a, void* stream) {
   A.dt = a->dt;
   A.channels = a->channels;
   A.solid = a->solid;
+  A.clear = a->solid ? a->clearance : nullptr;
   A.zero_ext = a->boundary;
   FX_REQUIRE(a->boundary == 0 || a->boundary == 1, FX_EINVAL, "boundary must be 0 (closed) or 1 (zero)");
   FX_REQUIRE(!(a->boundary == 1 && (a->maccormack || a->solid)), FX_EINVAL,
@@ -638,6 +707,21 @@ int fx_combustion(int64_t n, float dt, float ignition, float burn_rate, float he
                      heat_yield, smoke_yield, fuel, temperature, density, fuel_out, temperature_out, density_out,
                      flame);
   FX_LAUNCHED("fx_combustion");
+  return FX_OK;
+}
+
+int fx_build_clearance(fx_grid g, const uint8_t* solid, uint8_t* clearance, uint8_t* scratch, void* stream) {
+  if (int e = check_grid(g)) return e;
+  FX_REQUIRE(solid && clearance && scratch && clearance != scratch && clearance != solid && scratch != solid,
+             FX_EINVAL, "clearance: needs solid, output and scratch arrays");
+  dim3 gr = grid3(g.nx, g.ny, g.nz, kBlock);
+  hipStream_t s = S(stream);
+  const int radius = 15;  // clearances above 15 cells are stored as 255 (far)
+  hipLaunchKernelGGL(k_clearance, gr, kBlock, 0, s, g.nx, g.ny, g.nz, 0, solid, scratch, radius);
+  hipLaunchKernelGGL(k_clearance, gr, kBlock, 0, s, g.nx, g.ny, g.nz, 1, (const uint8_t*)scratch, clearance, radius);
+  hipLaunchKernelGGL(k_clearance, gr, kBlock, 0, s, g.nx, g.ny, g.nz, 2, (const uint8_t*)clearance, scratch, radius);
+  FX_HIP(hipMemcpyAsync(clearance, scratch, size_t(g.nx) * g.ny * g.nz, hipMemcpyDeviceToDevice, s));
+  FX_LAUNCHED("fx_build_clearance");
   return FX_OK;
 }
 

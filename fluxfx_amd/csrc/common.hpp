@@ -102,23 +102,35 @@ __device__ __forceinline__ float wave_max(float v) {
   return v;
 }
 
-// Sum `v` over a block of BLOCK threads (any 1-3D shape) and add it to *out
-// with one atomic per block.  Every thread of the block must call it.
-template <int BLOCK>
-__device__ __forceinline__ void block_sum_atomic(double v, double* out) {
-  __shared__ double part[BLOCK / 64];
+// FP64 reduction workspace: ws[q] holds finalised scalar q (q < 16).  Every
+// reduction kernel runs on a 1-D grid of exactly kRedBlocks blocks; block b
+// writes its partial of quantity q to ws[kWsBase + kRedBlocks*q + b] (no
+// atomics, deterministic) and ws_finalize folds the partials into ws[q].
+constexpr int kRedBlocks = 1024;
+constexpr int kRedThreads = 256;
+constexpr int kWsBase = 16;
+constexpr int kWsQuantities = 16;
+constexpr int kWsDoubles = kWsBase + kWsQuantities * kRedBlocks;
+
+// Sum `v` over the (1-D, kRedThreads) block and store the block's partial of
+// quantity q.  Every thread of the block must call it.
+__device__ __forceinline__ void block_partial(double v, double* ws, int q) {
+  __shared__ double part[kRedThreads / 64];
   v = wave_sum(v);
-  int tid = threadIdx.x + blockDim.x * (threadIdx.y + blockDim.y * threadIdx.z);
-  int lane = tid & 63, w = tid >> 6;
+  int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   if (lane == 0) part[w] = v;
   __syncthreads();
-  if (w == 0) {
-    double s = lane < BLOCK / 64 ? part[lane] : 0.0;
-    s = wave_sum(s);
-    if (lane == 0) atomicAdd(out, s);
+  if (threadIdx.x == 0) {
+    double s = 0.0;
+    for (int k = 0; k < kRedThreads / 64; ++k) s += part[k];
+    ws[kWsBase + kRedBlocks * q + blockIdx.x] = s;
   }
   __syncthreads();  // `part` is reused by the next call in the same kernel
 }
+
+// host helpers (reduce.hip)
+void ws_finalize(double* ws, int q0, int nq, hipStream_t s);
+int ws_read(const double* ws, int q0, int nq, double* host, hipStream_t s);
 
 inline dim3 grid3(int nx, int ny, int nz, dim3 block) {
   return dim3((nx + block.x - 1) / block.x, (ny + block.y - 1) / block.y, (nz + block.z - 1) / block.z);

@@ -1,0 +1,114 @@
+// Level operators shared by the multigrid engine and the Krylov/IR solvers.
+#pragma once
+#include "common.hpp"
+
+namespace fx {
+
+struct LevelDev {
+  int nx, ny, nz;
+  float wx, wy, wz;
+  const void* op;
+};
+
+static LevelDev dev_level(const fx_mg_level& L) {
+  return LevelDev{L.nx, L.ny, L.nz, L.w[0], L.w[1], L.w[2], L.op_data};
+}
+
+struct Stencil {
+  float s;       // sum_n w_n (p_n - p_c): neighbour differences first, so an
+                 // offset in p does not cancel catastrophically in fp32
+  float d;       // sum_n w_n
+  float c;       // p_c
+  bool active;   // fluid / active cell
+};
+
+// Gather the 7-point operator at cell i for the given level operator.
+// read_p = false: p is known to be zero (s = c = 0).
+template <int OP, typename T>
+__device__ __forceinline__ Stencil stencil(const LevelDev& L, int x, int y, int z, int64_t i,
+                                           const T* __restrict__ p, bool read_p) {
+  const int64_t sy = L.nx, sz = int64_t(L.nx) * L.ny;
+  Stencil st{0.0f, 0.0f, 0.0f, true};
+  const float c0 = read_p ? to_f(p[i]) : 0.0f;
+  st.c = c0;
+  if (OP == FX_OP_EDGES) {
+    const float4* e = reinterpret_cast<const float4*>(L.op);
+    float4 c = e[i];
+    float wxl = x > 0 ? e[i - 1].x : 0.0f;
+    float wyl = y > 0 ? e[i - sy].y : 0.0f;
+    float wzl = z > 0 ? e[i - sz].z : 0.0f;
+    st.d = c.x + c.y + c.z + wxl + wyl + wzl;
+    st.active = c.w > 0.0f;
+    if (read_p) {
+      float s = 0.0f;
+      if (c.x != 0.0f) s += c.x * (to_f(p[i + 1]) - c0);
+      if (wxl != 0.0f) s += wxl * (to_f(p[i - 1]) - c0);
+      if (c.y != 0.0f) s += c.y * (to_f(p[i + sy]) - c0);
+      if (wyl != 0.0f) s += wyl * (to_f(p[i - sy]) - c0);
+      if (c.z != 0.0f) s += c.z * (to_f(p[i + sz]) - c0);
+      if (wzl != 0.0f) s += wzl * (to_f(p[i - sz]) - c0);
+      st.s = s;
+    }
+  } else {
+    uint8_t f = OP == FX_OP_FLAGS ? reinterpret_cast<const uint8_t*>(L.op)[i] : box_flags(x, y, z, L.nx, L.ny, L.nz);
+    st.active = f & FLAG_FLUID;
+    float s = 0.0f, d = 0.0f;
+    if (f & FLAG_XLO) { d += L.wx; if (read_p) s += L.wx * (to_f(p[i - 1]) - c0); }
+    if (f & FLAG_XHI) { d += L.wx; if (read_p) s += L.wx * (to_f(p[i + 1]) - c0); }
+    if (f & FLAG_YLO) { d += L.wy; if (read_p) s += L.wy * (to_f(p[i - sy]) - c0); }
+    if (f & FLAG_YHI) { d += L.wy; if (read_p) s += L.wy * (to_f(p[i + sy]) - c0); }
+    if (f & FLAG_ZLO) { d += L.wz; if (read_p) s += L.wz * (to_f(p[i - sz]) - c0); }
+    if (f & FLAG_ZHI) { d += L.wz; if (read_p) s += L.wz * (to_f(p[i + sz]) - c0); }
+    st.s = s;
+    st.d = d;
+  }
+  return st;
+}
+
+template <int OP>
+__device__ __forceinline__ bool cell_active(const LevelDev& L, int x, int y, int z, int64_t i) {
+  if (OP == FX_OP_EDGES) return reinterpret_cast<const float4*>(L.op)[i].w > 0.0f;
+  if (OP == FX_OP_FLAGS) return reinterpret_cast<const uint8_t*>(L.op)[i] & FLAG_FLUID;
+  return true;
+}
+
+
+// residual r = rhs*inv_dt - (s - d p) on active cells, 0 elsewhere
+template <typename T, int OP>
+__device__ __forceinline__ float residual_at(const LevelDev& L, int x, int y, int z, const T* __restrict__ p,
+                                             const T* __restrict__ rhs, float inv_dt) {
+  int64_t i = idx3(x, y, z, L.nx, L.ny);
+  Stencil st = stencil<OP>(L, x, y, z, i, p, true);
+  if (!st.active) return 0.0f;
+  if (OP == FX_OP_EDGES && !(st.d > 0.0f)) return 0.0f;
+  return to_f(rhs[i]) * inv_dt - st.s;
+}
+
+static const dim3 kMgBlock(64, 4, 1);
+inline dim3 level_grid(const fx_mg_level& L) { return grid3(L.nx, L.ny, L.nz, kMgBlock); }
+
+#define FX_DISPATCH_OP(op, ...)                                             \
+  switch (op) {                                                             \
+    case FX_OP_BOX: { constexpr int OPV = FX_OP_BOX; __VA_ARGS__; break; }  \
+    case FX_OP_FLAGS: { constexpr int OPV = FX_OP_FLAGS; __VA_ARGS__; break; } \
+    case FX_OP_EDGES: { constexpr int OPV = FX_OP_EDGES; __VA_ARGS__; break; } \
+    default: set_error("unknown level operator %d", op); return FX_EINVAL; \
+  }
+
+#define FX_DISPATCH_T(st, ...)                                              \
+  switch (st) {                                                             \
+    case FX_F32: { using TV = float; __VA_ARGS__; break; }                  \
+    case FX_BF16: { using TV = bf16; __VA_ARGS__; break; }                  \
+    case FX_F16: { using TV = _Float16; __VA_ARGS__; break; }               \
+    default: set_error("unknown storage type %d", st); return FX_EINVAL;  \
+  }
+
+// host-side engine pieces (multigrid.hip)
+int check_config(const fx_mg_config* c);
+int vcycle(fx_mg_config* c, int l, float inv_dt, bool zero_guess, hipStream_t s);
+int resnorm(const fx_mg_level& L0, const float* x, const float* b, double* ws, double* rn, double* bn,
+            hipStream_t s);
+int project_mean_level(const fx_mg_level& L0, float* b, double* ws, hipStream_t s);
+fx_mg_level level0_of(fx_grid g, const uint8_t* flags);
+
+}  // namespace fx

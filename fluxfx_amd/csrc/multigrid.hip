@@ -3,72 +3,9 @@
 // tolerance-driven stationary solve.  Templated on the storage type of the
 // level arrays (fp32 / bf16 / fp16, arithmetic always fp32) and on the level
 // operator (closed box, open-face flags, or edge conductances).
-#include "common.hpp"
+#include "mg_internal.hpp"
 
 namespace fx {
-
-struct LevelDev {
-  int nx, ny, nz;
-  float wx, wy, wz;
-  const void* op;
-};
-
-static LevelDev dev_level(const fx_mg_level& L) {
-  return LevelDev{L.nx, L.ny, L.nz, L.w[0], L.w[1], L.w[2], L.op_data};
-}
-
-struct Stencil {
-  float s;       // sum_n w_n p_n
-  float d;       // sum_n w_n
-  bool active;   // fluid / active cell
-};
-
-// Gather the 7-point operator at cell i for the given level operator.
-template <int OP, typename T>
-__device__ __forceinline__ Stencil stencil(const LevelDev& L, int x, int y, int z, int64_t i,
-                                           const T* __restrict__ p, bool read_p) {
-  const int64_t sy = L.nx, sz = int64_t(L.nx) * L.ny;
-  Stencil st{0.0f, 0.0f, true};
-  if (OP == FX_OP_EDGES) {
-    const float4* e = reinterpret_cast<const float4*>(L.op);
-    float4 c = e[i];
-    float wxl = x > 0 ? e[i - 1].x : 0.0f;
-    float wyl = y > 0 ? e[i - sy].y : 0.0f;
-    float wzl = z > 0 ? e[i - sz].z : 0.0f;
-    st.d = c.x + c.y + c.z + wxl + wyl + wzl;
-    st.active = c.w > 0.0f;
-    if (read_p) {
-      float s = 0.0f;
-      if (c.x != 0.0f) s += c.x * to_f(p[i + 1]);
-      if (wxl != 0.0f) s += wxl * to_f(p[i - 1]);
-      if (c.y != 0.0f) s += c.y * to_f(p[i + sy]);
-      if (wyl != 0.0f) s += wyl * to_f(p[i - sy]);
-      if (c.z != 0.0f) s += c.z * to_f(p[i + sz]);
-      if (wzl != 0.0f) s += wzl * to_f(p[i - sz]);
-      st.s = s;
-    }
-  } else {
-    uint8_t f = OP == FX_OP_FLAGS ? reinterpret_cast<const uint8_t*>(L.op)[i] : box_flags(x, y, z, L.nx, L.ny, L.nz);
-    st.active = f & FLAG_FLUID;
-    float s = 0.0f, d = 0.0f;
-    if (f & FLAG_XLO) { d += L.wx; if (read_p) s += L.wx * to_f(p[i - 1]); }
-    if (f & FLAG_XHI) { d += L.wx; if (read_p) s += L.wx * to_f(p[i + 1]); }
-    if (f & FLAG_YLO) { d += L.wy; if (read_p) s += L.wy * to_f(p[i - sy]); }
-    if (f & FLAG_YHI) { d += L.wy; if (read_p) s += L.wy * to_f(p[i + sy]); }
-    if (f & FLAG_ZLO) { d += L.wz; if (read_p) s += L.wz * to_f(p[i - sz]); }
-    if (f & FLAG_ZHI) { d += L.wz; if (read_p) s += L.wz * to_f(p[i + sz]); }
-    st.s = s;
-    st.d = d;
-  }
-  return st;
-}
-
-template <int OP>
-__device__ __forceinline__ bool cell_active(const LevelDev& L, int x, int y, int z, int64_t i) {
-  if (OP == FX_OP_EDGES) return reinterpret_cast<const float4*>(L.op)[i].w > 0.0f;
-  if (OP == FX_OP_FLAGS) return reinterpret_cast<const uint8_t*>(L.op)[i] & FLAG_FLUID;
-  return true;
-}
 
 // ---------------------------------------------------------------------------
 // smoothing: out = active && d>0 ? p + omega*((s - rhs*inv_dt)/d - p) : 0
@@ -82,23 +19,51 @@ __global__ void k_smooth(LevelDev L, const T* __restrict__ p, const T* __restric
   int64_t i = idx3(x, y, z, L.nx, L.ny);
   Stencil st = stencil<OP>(L, x, y, z, i, p, !zero_guess);
   float r = 0.0f;
-  if (st.active && st.d > 0.0f) {
-    float cand = (st.s - to_f(rhs[i]) * inv_dt) / st.d;
-    float old = zero_guess ? 0.0f : to_f(p[i]);
-    r = old + omega * (cand - old);
-  }
+  // correction form of p + omega*((sum w p_n - rhs)/d - p)
+  if (st.active && st.d > 0.0f) r = st.c + omega * (st.s - to_f(rhs[i]) * inv_dt) / st.d;
   out[i] = from_f<T>(r);
 }
 
-// residual r = rhs*inv_dt - (s - d p) on active cells, 0 elsewhere
+// Level-0 smoother for box/flags operators: each thread marches a column of
+// `zc` cells along z keeping p[z-1], p[z], p[z+1] in registers; blocks are
+// assigned so that each XCD (workgroup id mod 8) sweeps one contiguous slab,
+// keeping the y/z neighbour rows in its own L2.
+__device__ __forceinline__ int xcd_tile(int b, int nb) {
+  return (nb % 8 == 0) ? (b % 8) * (nb / 8) + b / 8 : b;
+}
+
 template <typename T, int OP>
-__device__ __forceinline__ float residual_at(const LevelDev& L, int x, int y, int z, const T* __restrict__ p,
-                                             const T* __restrict__ rhs, float inv_dt) {
-  int64_t i = idx3(x, y, z, L.nx, L.ny);
-  Stencil st = stencil<OP>(L, x, y, z, i, p, true);
-  if (!st.active) return 0.0f;
-  if (OP == FX_OP_EDGES && !(st.d > 0.0f)) return 0.0f;
-  return to_f(rhs[i]) * inv_dt - (st.s - st.d * to_f(p[i]));
+__global__ __launch_bounds__(256) void k_smooth_zm(LevelDev L, const T* __restrict__ p, const T* __restrict__ rhs,
+                                                   T* __restrict__ out, float inv_dt, float omega, int zero_guess,
+                                                   int zc, int tiles_x, int tiles_y) {
+  int t = xcd_tile(blockIdx.x, gridDim.x);
+  int tx = t % tiles_x, rest = t / tiles_x, ty = rest % tiles_y, tz = rest / tiles_y;
+  int x = tx * 64 + threadIdx.x, y = ty * 4 + threadIdx.y;
+  int z0 = tz * zc, z1 = min(z0 + zc, L.nz);
+  if (x >= L.nx || y >= L.ny || z0 >= L.nz) return;
+  const int64_t sy = L.nx, sz = int64_t(L.nx) * L.ny;
+  int64_t i = idx3(x, y, z0, L.nx, L.ny);
+  float pm = 0.0f, pc = 0.0f;
+  if (!zero_guess) {
+    pc = to_f(p[i]);
+    pm = z0 > 0 ? to_f(p[i - sz]) : 0.0f;
+  }
+  for (int z = z0; z < z1; ++z, i += sz) {
+    float pn = (!zero_guess && z + 1 < L.nz) ? to_f(p[i + sz]) : 0.0f;
+    uint8_t f = OP == FX_OP_FLAGS ? reinterpret_cast<const uint8_t*>(L.op)[i] : box_flags(x, y, z, L.nx, L.ny, L.nz);
+    float sdiff = 0.0f, d = 0.0f;
+    if (f & FLAG_XLO) { d += L.wx; if (!zero_guess) sdiff += L.wx * (to_f(p[i - 1]) - pc); }
+    if (f & FLAG_XHI) { d += L.wx; if (!zero_guess) sdiff += L.wx * (to_f(p[i + 1]) - pc); }
+    if (f & FLAG_YLO) { d += L.wy; if (!zero_guess) sdiff += L.wy * (to_f(p[i - sy]) - pc); }
+    if (f & FLAG_YHI) { d += L.wy; if (!zero_guess) sdiff += L.wy * (to_f(p[i + sy]) - pc); }
+    if (f & FLAG_ZLO) { d += L.wz; sdiff += L.wz * (pm - pc); }
+    if (f & FLAG_ZHI) { d += L.wz; sdiff += L.wz * (pn - pc); }
+    float r = 0.0f;
+    if ((f & FLAG_FLUID) && d > 0.0f) r = pc + omega * (sdiff - to_f(rhs[i]) * inv_dt) / d;
+    out[i] = from_f<T>(r);
+    pm = pc;
+    pc = pn;
+  }
 }
 
 template <typename T, int OP>
@@ -217,76 +182,11 @@ __global__ void k_direct(int m, const double* __restrict__ M, const T* __restric
   if (lane == 0) out[row] = from_f<T>(float(acc * double(scale)));
 }
 
-// ---------------------------------------------------------------------------
-// FP64 reductions on the level-0 fp32 arrays
-// ---------------------------------------------------------------------------
-__global__ void k_fill_d(double* ws, int n) {
-  if (threadIdx.x < n) ws[threadIdx.x] = 0.0;
-}
-
-template <int OP>
-__global__ void k_resnorm(LevelDev L, const float* __restrict__ x, const float* __restrict__ b, double* ws) {
-  int X = blockIdx.x * blockDim.x + threadIdx.x, Y = blockIdx.y * blockDim.y + threadIdx.y, Z = blockIdx.z;
-  double rr = 0.0, bb = 0.0;
-  if (X < L.nx && Y < L.ny) {
-    int64_t i = idx3(X, Y, Z, L.nx, L.ny);
-    float r = residual_at<float, OP>(L, X, Y, Z, x, b, 1.0f);
-    float bv = b[i];
-    rr = double(r) * double(r);
-    bb = double(bv) * double(bv);
-  }
-  block_sum_atomic<256>(rr, ws + 0);
-  block_sum_atomic<256>(bb, ws + 1);
-}
-
-template <int OP>
-__global__ void k_mean_sum(LevelDev L, const float* __restrict__ b, double* ws) {
-  int X = blockIdx.x * blockDim.x + threadIdx.x, Y = blockIdx.y * blockDim.y + threadIdx.y, Z = blockIdx.z;
-  double s = 0.0, c = 0.0;
-  if (X < L.nx && Y < L.ny) {
-    int64_t i = idx3(X, Y, Z, L.nx, L.ny);
-    if (cell_active<OP>(L, X, Y, Z, i)) {
-      s = b[i];
-      c = 1.0;
-    }
-  }
-  block_sum_atomic<256>(s, ws + 2);
-  block_sum_atomic<256>(c, ws + 3);
-}
-
-template <int OP>
-__global__ void k_sub_mean(LevelDev L, float* __restrict__ b, const double* __restrict__ ws) {
-  int X = blockIdx.x * blockDim.x + threadIdx.x, Y = blockIdx.y * blockDim.y + threadIdx.y, Z = blockIdx.z;
-  if (X >= L.nx || Y >= L.ny) return;
-  int64_t i = idx3(X, Y, Z, L.nx, L.ny);
-  double cnt = ws[3];
-  float mean = cnt > 0 ? float(ws[2] / cnt) : 0.0f;
-  b[i] = cell_active<OP>(L, X, Y, Z, i) ? b[i] - mean : 0.0f;
-}
-
-static const dim3 kBlock(64, 4, 1);
-
-static dim3 level_grid(const fx_mg_level& L) { return grid3(L.nx, L.ny, L.nz, kBlock); }
+static const dim3 kBlock = kMgBlock;
 
 // ---------------------------------------------------------------------------
 // host-side dispatch over (storage, operator)
 // ---------------------------------------------------------------------------
-#define FX_DISPATCH_OP(op, ...)                                             \
-  switch (op) {                                                             \
-    case FX_OP_BOX: { constexpr int OPV = FX_OP_BOX; __VA_ARGS__; break; }  \
-    case FX_OP_FLAGS: { constexpr int OPV = FX_OP_FLAGS; __VA_ARGS__; break; } \
-    case FX_OP_EDGES: { constexpr int OPV = FX_OP_EDGES; __VA_ARGS__; break; } \
-    default: set_error("unknown level operator %d", op); return FX_EINVAL; \
-  }
-
-#define FX_DISPATCH_T(st, ...)                                              \
-  switch (st) {                                                             \
-    case FX_F32: { using TV = float; __VA_ARGS__; break; }                  \
-    case FX_BF16: { using TV = bf16; __VA_ARGS__; break; }                  \
-    case FX_F16: { using TV = _Float16; __VA_ARGS__; break; }               \
-    default: set_error("unknown storage type %d", st); return FX_EINVAL;  \
-  }
-
 int check_config(const fx_mg_config* c) {
   FX_REQUIRE(c, FX_EINVAL, "null config");
   FX_REQUIRE(c->levels >= 1 && c->levels <= 16, FX_EINVAL, "levels must be in [1,16]");
@@ -313,10 +213,25 @@ int check_config(const fx_mg_config* c) {
 static int launch_smooth(fx_mg_config* c, int l, float inv_dt, bool zero_guess, hipStream_t s) {
   fx_mg_level& L = c->level[l];
   LevelDev D = dev_level(L);
-  FX_DISPATCH_T(c->storage, FX_DISPATCH_OP(L.op, {
-    hipLaunchKernelGGL((k_smooth<TV, OPV>), level_grid(L), kBlock, 0, s, D, (const TV*)L.p, (const TV*)L.rhs,
-                       (TV*)L.q, inv_dt, c->omega, zero_guess ? 1 : 0);
-  }));
+  const int64_t cells = int64_t(L.nx) * L.ny * L.nz;
+  if (L.op != FX_OP_EDGES && cells >= (1 << 18)) {
+    const int zc = 16;
+    int tx = (L.nx + 63) / 64, ty = (L.ny + 3) / 4, tz = (L.nz + zc - 1) / zc;
+    dim3 grid(tx * ty * tz);
+    FX_DISPATCH_T(c->storage, {
+      if (L.op == FX_OP_FLAGS)
+        hipLaunchKernelGGL((k_smooth_zm<TV, FX_OP_FLAGS>), grid, kBlock, 0, s, D, (const TV*)L.p, (const TV*)L.rhs,
+                           (TV*)L.q, inv_dt, c->omega, zero_guess ? 1 : 0, zc, tx, ty);
+      else
+        hipLaunchKernelGGL((k_smooth_zm<TV, FX_OP_BOX>), grid, kBlock, 0, s, D, (const TV*)L.p, (const TV*)L.rhs,
+                           (TV*)L.q, inv_dt, c->omega, zero_guess ? 1 : 0, zc, tx, ty);
+    });
+  } else {
+    FX_DISPATCH_T(c->storage, FX_DISPATCH_OP(L.op, {
+      hipLaunchKernelGGL((k_smooth<TV, OPV>), level_grid(L), kBlock, 0, s, D, (const TV*)L.p, (const TV*)L.rhs,
+                         (TV*)L.q, inv_dt, c->omega, zero_guess ? 1 : 0);
+    }));
+  }
   FX_LAUNCHED("k_smooth");
   void* t = L.p;
   L.p = L.q;
@@ -443,33 +358,6 @@ int vcycle(fx_mg_config* c, int l, float inv_dt, bool zero_guess, hipStream_t s)
   return smooth_n(c, l, c->post, inv_dt, false, s);
 }
 
-int resnorm(const fx_mg_level& L0, const float* x, const float* b, double* ws, double* rn, double* bn,
-            hipStream_t s) {
-  LevelDev D = dev_level(L0);
-  hipLaunchKernelGGL(k_fill_d, dim3(1), dim3(64), 0, s, ws, 2);
-  FX_DISPATCH_OP(L0.op, {
-    hipLaunchKernelGGL((k_resnorm<OPV>), level_grid(L0), kBlock, 0, s, D, x, b, ws);
-  });
-  FX_LAUNCHED("k_resnorm");
-  double h[2];
-  FX_HIP(hipMemcpyAsync(h, ws, sizeof(h), hipMemcpyDeviceToHost, s));
-  FX_HIP(hipStreamSynchronize(s));
-  *rn = sqrt(h[0]);
-  *bn = sqrt(h[1]);
-  return FX_OK;
-}
-
-int project_mean_level(const fx_mg_level& L0, float* b, double* ws, hipStream_t s) {
-  LevelDev D = dev_level(L0);
-  hipLaunchKernelGGL(k_fill_d, dim3(1), dim3(64), 0, s, ws, 4);
-  FX_DISPATCH_OP(L0.op, {
-    hipLaunchKernelGGL((k_mean_sum<OPV>), level_grid(L0), kBlock, 0, s, D, b, ws);
-    hipLaunchKernelGGL((k_sub_mean<OPV>), level_grid(L0), kBlock, 0, s, D, b, ws);
-  });
-  FX_LAUNCHED("k_project_mean");
-  return FX_OK;
-}
-
 fx_mg_level level0_of(fx_grid g, const uint8_t* flags) {
   fx_mg_level L{};
   L.nx = g.nx; L.ny = g.ny; L.nz = g.nz;
@@ -548,46 +436,6 @@ int fx_mg_direct(const fx_mg_config* cfg, void* stream) {
   if (int e = check_config(cfg)) return e;
   FX_REQUIRE(cfg->coarse_matrix, FX_EINVAL, "no coarse matrix");
   return launch_direct(cfg, S(stream));
-}
-
-int fx_project_mean(fx_grid g, const uint8_t* flags, float* rhs, double* ws, void* stream) {
-  if (int e = check_grid(g)) return e;
-  FX_REQUIRE(rhs && ws, FX_EINVAL, "null array");
-  return project_mean_level(level0_of(g, flags), rhs, ws, S(stream));
-}
-
-int fx_residual_norm(fx_grid g, const uint8_t* flags, const float* x, const float* b, double* rnorm, double* bnorm,
-                     double* ws, void* stream) {
-  if (int e = check_grid(g)) return e;
-  FX_REQUIRE(x && b && ws && rnorm && bnorm, FX_EINVAL, "null array");
-  return resnorm(level0_of(g, flags), x, b, ws, rnorm, bnorm, S(stream));
-}
-
-int fx_mg_solve(fx_mg_config* cfg, float tol, int32_t max_cycles, double* ws, fx_solve_info* info, void* stream) {
-  if (int e = check_config(cfg)) return e;
-  FX_REQUIRE(cfg->storage == FX_F32, FX_EINVAL, "fx_mg_solve runs on fp32 levels");
-  FX_REQUIRE(ws && info && tol > 0 && max_cycles >= 0, FX_EINVAL, "bad solve arguments");
-  FX_REQUIRE(cfg->level[0].op != FX_OP_EDGES, FX_EINVAL, "level 0 must be a box or flags operator");
-  hipStream_t s = S(stream);
-  fx_solve_info out{};
-  double rn, bn;
-  fx_mg_level& L0 = cfg->level[0];
-  if (int e = resnorm(L0, (const float*)L0.p, (const float*)L0.rhs, ws, &rn, &bn, s)) return e;
-  out.bnorm = bn;
-  out.residual0 = rn;
-  out.residual = rn;
-  int it = 0;
-  while (!(rn <= double(tol) * bn) && it < max_cycles) {
-    if (int e = vcycle(cfg, 0, 1.0f, false, s)) return e;
-    ++it;
-    if (int e = resnorm(L0, (const float*)L0.p, (const float*)L0.rhs, ws, &rn, &bn, s)) return e;
-    FX_REQUIRE(std::isfinite(rn), FX_ESTATE, "non-finite residual after %d cycles", it);
-  }
-  out.iterations = it;
-  out.residual = rn;
-  out.converged = rn <= double(tol) * bn ? 1 : 0;
-  *info = out;
-  return FX_OK;
 }
 
 }  // extern "C"

@@ -145,25 +145,22 @@ __global__ void k_jacobi(int nx, int ny, int nz, float wx, float wy, float wz, f
   int64_t i = idx3(x, y, z, nx, ny);
   uint8_t f = cell_flags(flags, x, y, z, nx, ny, nz);
   int64_t sy = nx, sz = int64_t(nx) * ny;
-  float s = 0.0f, d = 0.0f;
-  if (f & FLAG_XLO) { s += wx * p[i - 1]; d += wx; }
-  if (f & FLAG_XHI) { s += wx * p[i + 1]; d += wx; }
-  if (f & FLAG_YLO) { s += wy * p[i - sy]; d += wy; }
-  if (f & FLAG_YHI) { s += wy * p[i + sy]; d += wy; }
-  if (f & FLAG_ZLO) { s += wz * p[i - sz]; d += wz; }
-  if (f & FLAG_ZHI) { s += wz * p[i + sz]; d += wz; }
+  const float c = p[i];
+  float s = 0.0f, d = 0.0f;  // s = sum w (p_n - p_c)
+  if (f & FLAG_XLO) { s += wx * (p[i - 1] - c); d += wx; }
+  if (f & FLAG_XHI) { s += wx * (p[i + 1] - c); d += wx; }
+  if (f & FLAG_YLO) { s += wy * (p[i - sy] - c); d += wy; }
+  if (f & FLAG_YHI) { s += wy * (p[i + sy] - c); d += wy; }
+  if (f & FLAG_ZLO) { s += wz * (p[i - sz] - c); d += wz; }
+  if (f & FLAG_ZHI) { s += wz * (p[i + sz] - c); d += wz; }
   float r = 0.0f;
-  if ((f & FLAG_FLUID) && d != 0.0f) {
-    float cand = (s - rhs[i] * inv_dt) / d;
-    float old = p[i];
-    r = old + omega * (cand - old);
-  }
+  if ((f & FLAG_FLUID) && d != 0.0f) r = c + omega * (s - rhs[i] * inv_dt) / d;
   out[i] = r;
 }
 
 __global__ void k_gradient(int nx, int ny, int nz, float ix, float iy, float iz, float scale,
                            const uint8_t* __restrict__ flags, const float* __restrict__ p,
-                           const float* __restrict__ u, const float* __restrict__ v, const float* __restrict__ w,
+                           const float* __restrict__ dp, const float* __restrict__ u, const float* __restrict__ v, const float* __restrict__ w,
                            float* __restrict__ uo, float* __restrict__ vo, float* __restrict__ wo) {
   int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y * blockDim.y + threadIdx.y, z = blockIdx.z;
   if (x > nx || y > ny) return;
@@ -172,20 +169,37 @@ __global__ void k_gradient(int nx, int ny, int nz, float ix, float iy, float iz,
   if (y < ny && z < nz) {
     int64_t i = idx3(x, y, z, nx + 1, ny);
     float r = 0.0f;
-    if (face_open(flags, 0, x, y, z, nx, ny, nz)) r = u[i] - scale * (p[c] - p[c - 1]) * ix;
+    if (face_open(flags, 0, x, y, z, nx, ny, nz)) {
+      float g = p[c] - p[c - 1];
+      if (dp) g += dp[c] - dp[c - 1];
+      r = u[i] - scale * g * ix;
+    }
     uo[i] = r;
   }
   if (x < nx && z < nz) {
     int64_t i = idx3(x, y, z, nx, ny + 1);
     float r = 0.0f;
-    if (face_open(flags, 1, x, y, z, nx, ny, nz)) r = v[i] - scale * (p[c] - p[c - sy]) * iy;
+    if (face_open(flags, 1, x, y, z, nx, ny, nz)) {
+      float g = p[c] - p[c - sy];
+      if (dp) g += dp[c] - dp[c - sy];
+      r = v[i] - scale * g * iy;
+    }
     vo[i] = r;
   }
   if (x < nx && y < ny) {
     float r = 0.0f;
-    if (face_open(flags, 2, x, y, z, nx, ny, nz)) r = w[c] - scale * (p[c] - p[c - sz]) * iz;
+    if (face_open(flags, 2, x, y, z, nx, ny, nz)) {
+      float g = p[c] - p[c - sz];
+      if (dp) g += dp[c] - dp[c - sz];
+      r = w[c] - scale * g * iz;
+    }
     wo[c] = r;
   }
+}
+
+__global__ void k_combine(int64_t n, const float* __restrict__ a, const float* __restrict__ b, float* __restrict__ out) {
+  int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = a[i] + b[i];
 }
 
 // Initial fields of reset(): kind 0 = scalar scale*w^2 of the soft sphere
@@ -318,8 +332,28 @@ int fx_gradient(fx_grid g, const uint8_t* flags, const float* p, const float* u,
   FX_REQUIRE(uo != u && vo != v && wo != w, FX_EINVAL, "Advection input/output textures must be distinct");
   dim3 gr = grid3(g.nx + 1, g.ny + 1, g.nz + 1, kBlock);
   hipLaunchKernelGGL(k_gradient, gr, kBlock, 0, S(stream), g.nx, g.ny, g.nz, g.nx / g.ex, g.ny / g.ey,
-                     g.nz / g.ez, scale, flags, p, u, v, w, uo, vo, wo);
+                     g.nz / g.ez, scale, flags, p, (const float*)nullptr, u, v, w, uo, vo, wo);
   FX_LAUNCHED("fx_gradient");
+  return FX_OK;
+}
+
+int fx_gradient_split(fx_grid g, const uint8_t* flags, const float* p, const float* dp, const float* u,
+                      const float* v, const float* w, float* uo, float* vo, float* wo, float scale, void* stream) {
+  if (int e = check_grid(g)) return e;
+  FX_REQUIRE(p && dp && u && v && w && uo && vo && wo, FX_EINVAL, "null array");
+  FX_REQUIRE(uo != u && vo != v && wo != w, FX_EINVAL, "Advection input/output textures must be distinct");
+  dim3 gr = grid3(g.nx + 1, g.ny + 1, g.nz + 1, kBlock);
+  hipLaunchKernelGGL(k_gradient, gr, kBlock, 0, S(stream), g.nx, g.ny, g.nz, g.nx / g.ex, g.ny / g.ey,
+                     g.nz / g.ez, scale, flags, p, dp, u, v, w, uo, vo, wo);
+  FX_LAUNCHED("fx_gradient_split");
+  return FX_OK;
+}
+
+int fx_combine(const float* a, const float* b, float* out, int64_t n, void* stream) {
+  FX_REQUIRE(a && b && out && n >= 0, FX_EINVAL, "combine: bad arrays");
+  if (n == 0) return FX_OK;
+  hipLaunchKernelGGL(k_combine, dim3((n + 255) / 256), dim3(256), 0, S(stream), n, a, b, out);
+  FX_LAUNCHED("fx_combine");
   return FX_OK;
 }
 

@@ -1,0 +1,415 @@
+// Tolerance-driven solvers on top of the multigrid engine (no reference
+// counterpart; the reference runs fixed cycle budgets):
+//   * mixed-precision iterative refinement: FP32 iterate and residual, V-cycle
+//     on the error equation in 16-bit storage with FP32 arithmetic;
+//   * flexible preconditioned CG (Polak-Ribiere beta) with a V-cycle
+//     preconditioner in any storage type.
+// Norms and dot products accumulate in FP64 per block and are folded by one
+// finalize block; alpha/beta are formed on the device.  The host reads one
+// residual norm per iteration from `check_from` on.
+#include "mg_internal.hpp"
+
+namespace fx {
+
+int resnorm_q(const fx_mg_level& L0, const float* x, const float* b, double* ws, int qr, int qb, hipStream_t s);
+int norm2_q(int64_t n, const float* a, double* ws, int q, hipStream_t s);
+int check_params(const fx_solve_params* p);
+
+__device__ __forceinline__ void cell_xyz2(int64_t i, int nx, int ny, int& x, int& y, int& z) {
+  int t = int(i / nx);
+  x = int(i - int64_t(t) * nx);
+  z = t / ny;
+  y = t - z * ny;
+}
+
+// 7-point operator of (x + s*e) at cell i, x fp32, e in storage T (or null).
+template <int OP, typename T>
+__device__ __forceinline__ float apply_shifted(const LevelDev& L, int X, int Y, int Z, int64_t i,
+                                               const float* __restrict__ x, const T* __restrict__ e, float s,
+                                               bool& active, float& center) {
+  const int64_t sy = L.nx, sz = int64_t(L.nx) * L.ny;
+  auto val = [&](int64_t j) { return x[j] + (e ? s * to_f(e[j]) : 0.0f); };
+  center = val(i);
+  float acc = 0.0f;
+  if (OP == FX_OP_EDGES) {
+    const float4* ed = reinterpret_cast<const float4*>(L.op);
+    float4 c = ed[i];
+    active = c.w > 0.0f;
+    float wxl = X > 0 ? ed[i - 1].x : 0.0f, wyl = Y > 0 ? ed[i - sy].y : 0.0f, wzl = Z > 0 ? ed[i - sz].z : 0.0f;
+    if (c.x != 0.0f) acc += c.x * (val(i + 1) - center);
+    if (wxl != 0.0f) acc += wxl * (val(i - 1) - center);
+    if (c.y != 0.0f) acc += c.y * (val(i + sy) - center);
+    if (wyl != 0.0f) acc += wyl * (val(i - sy) - center);
+    if (c.z != 0.0f) acc += c.z * (val(i + sz) - center);
+    if (wzl != 0.0f) acc += wzl * (val(i - sz) - center);
+  } else {
+    uint8_t f = OP == FX_OP_FLAGS ? reinterpret_cast<const uint8_t*>(L.op)[i] : box_flags(X, Y, Z, L.nx, L.ny, L.nz);
+    active = f & FLAG_FLUID;
+    if (f & FLAG_XLO) acc += L.wx * (val(i - 1) - center);
+    if (f & FLAG_XHI) acc += L.wx * (val(i + 1) - center);
+    if (f & FLAG_YLO) acc += L.wy * (val(i - sy) - center);
+    if (f & FLAG_YHI) acc += L.wy * (val(i + sy) - center);
+    if (f & FLAG_ZLO) acc += L.wz * (val(i - sz) - center);
+    if (f & FLAG_ZHI) acc += L.wz * (val(i + sz) - center);
+  }
+  return acc;
+}
+
+#define FX_FOR_CELLS_N(n)                                                                     \
+  for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < (n);                  \
+       i += int64_t(gridDim.x) * blockDim.x)
+
+// x' = x + s_e*e (e may be null), r = b - L x', rT = r*inv_s, ws[q] partials of ||r||^2
+template <int OP, typename T>
+__global__ void k_ir_update(LevelDev L, const float* __restrict__ x, const T* __restrict__ e, float s_e,
+                            const float* __restrict__ b, float* __restrict__ xo, T* __restrict__ rT, float inv_s,
+                            double* ws, int q) {
+  const int64_t n = int64_t(L.nx) * L.ny * L.nz;
+  double rr = 0.0;
+  FX_FOR_CELLS_N(n) {
+    int X, Y, Z;
+    cell_xyz2(i, L.nx, L.ny, X, Y, Z);
+    bool active;
+    float center;
+    float lap = apply_shifted<OP, T>(L, X, Y, Z, i, x, e, s_e, active, center);
+    float r = active ? b[i] - lap : 0.0f;
+    if (xo) xo[i] = active ? center : 0.0f;
+    rT[i] = from_f<T>(r * inv_s);
+    rr += double(r) * double(r);
+  }
+  block_partial(rr, ws, q);
+}
+
+template <typename T>
+__global__ void k_rescale(int64_t n, T* __restrict__ a, float f) {
+  int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) a[i] = from_f<T>(to_f(a[i]) * f);
+}
+
+// ---------------------------------------------------------------------------
+// Krylov kernels on A = -L (positive semi-definite on the fluid), c = -b
+// ---------------------------------------------------------------------------
+// r = L x - b, rT = -r*inv_s, partial ||r||^2 -> q
+template <int OP, typename T>
+__global__ void k_pcg_init(LevelDev L, const float* __restrict__ x, const float* __restrict__ b, float* __restrict__ r,
+                           T* __restrict__ rT, float inv_s, double* ws, int q) {
+  const int64_t n = int64_t(L.nx) * L.ny * L.nz;
+  double rr = 0.0;
+  FX_FOR_CELLS_N(n) {
+    int X, Y, Z;
+    cell_xyz2(i, L.nx, L.ny, X, Y, Z);
+    bool active;
+    float center;
+    float lap = apply_shifted<OP, float>(L, X, Y, Z, i, x, (const float*)nullptr, 0.0f, active, center);
+    float rv = active ? lap - b[i] : 0.0f;
+    r[i] = rv;
+    rT[i] = from_f<T>(-rv * inv_s);
+    rr += double(rv) * double(rv);
+  }
+  block_partial(rr, ws, q);
+}
+
+// Ap = -L p ; partial p.Ap -> q
+template <int OP>
+__global__ void k_pcg_apply(LevelDev L, const float* __restrict__ p, float* __restrict__ Ap, double* ws, int q) {
+  const int64_t n = int64_t(L.nx) * L.ny * L.nz;
+  double acc = 0.0;
+  FX_FOR_CELLS_N(n) {
+    int X, Y, Z;
+    cell_xyz2(i, L.nx, L.ny, X, Y, Z);
+    bool active;
+    float center;
+    float lap = apply_shifted<OP, float>(L, X, Y, Z, i, p, (const float*)nullptr, 0.0f, active, center);
+    float a = active ? -lap : 0.0f;
+    Ap[i] = a;
+    acc += double(center) * double(a);
+  }
+  block_partial(acc, ws, q);
+}
+
+// alpha = ws[q_rz]/ws[q_pap]; x += alpha p; r -= alpha Ap; rT = -r*inv_s;
+// partials ||r||^2 -> q_rr, r.(zold*zscale) -> q_cross
+template <typename T>
+__global__ void k_pcg_update(int64_t n, double* ws, int q_rz, int q_pap, float* __restrict__ x,
+                             const float* __restrict__ p, float* __restrict__ r, const float* __restrict__ Ap,
+                             const T* __restrict__ zold, float zscale, T* __restrict__ rT, float inv_s, int q_rr,
+                             int q_cross) {
+  double pap = ws[q_pap];
+  float alpha = pap > 0.0 ? float(ws[q_rz] / pap) : 0.0f;
+  double rr = 0.0, rzo = 0.0;
+  FX_FOR_CELLS_N(n) {
+    x[i] += alpha * p[i];
+    float rv = r[i] - alpha * Ap[i];
+    r[i] = rv;
+    rT[i] = from_f<T>(-rv * inv_s);
+    rr += double(rv) * double(rv);
+    rzo += double(rv) * double(to_f(zold[i]) * zscale);
+  }
+  block_partial(rr, ws, q_rr);
+  block_partial(rzo, ws, q_cross);
+}
+
+// partial r.(z*zscale) -> q
+template <typename T>
+__global__ void k_dot_rz(int64_t n, const float* __restrict__ r, const T* __restrict__ z, float zscale, double* ws,
+                         int q) {
+  double acc = 0.0;
+  FX_FOR_CELLS_N(n) acc += double(r[i]) * double(to_f(z[i]) * zscale);
+  block_partial(acc, ws, q);
+}
+
+// beta = max(0, (rz_new - rz_cross) / rz_old); p = z*zscale + beta p
+template <typename T>
+__global__ void k_pcg_direction(int64_t n, const double* __restrict__ ws, int q_new, int q_cross, int q_old,
+                                const T* __restrict__ z, float zscale, float* __restrict__ p, int first) {
+  float beta = 0.0f;
+  if (!first) {
+    double den = ws[q_old];
+    beta = den != 0.0 ? float((ws[q_new] - ws[q_cross]) / den) : 0.0f;
+    beta = fmaxf(beta, 0.0f);
+  }
+  FX_FOR_CELLS_N(n) p[i] = to_f(z[i]) * zscale + (first ? 0.0f : beta * p[i]);
+}
+
+static const dim3 kRedGrid(kRedBlocks), kRedBlock(kRedThreads);
+
+// r = b - L(x + s_e e) into rT (scaled) and xo; ws[q] = ||r||^2 (finalised)
+static int launch_residual(const fx_mg_level& L0, int storage, const float* x, const void* e, float s_e,
+                           const float* b, float* xo, void* rT, float inv_s, double* ws, int q, hipStream_t s) {
+  LevelDev D = dev_level(L0);
+  FX_DISPATCH_T(storage, FX_DISPATCH_OP(L0.op, {
+    hipLaunchKernelGGL((k_ir_update<OPV, TV>), kRedGrid, kRedBlock, 0, s, D, x, (const TV*)e, s_e, b, xo,
+                       (TV*)rT, inv_s, ws, q);
+  }));
+  FX_LAUNCHED("k_ir_update");
+  ws_finalize(ws, q, 1, s);
+  return FX_OK;
+}
+
+// reference norm^2 slot: the caller's, or ||b||^2 computed into slot 1
+static int reference_slot(const fx_solve_params* prm, int64_t n, const float* b, double* ws, hipStream_t s,
+                          int* ref_q) {
+  *ref_q = prm->ref_q;
+  if (*ref_q < 0) {
+    *ref_q = 1;
+    return norm2_q(n, b, ws, 1, s);
+  }
+  return FX_OK;
+}
+
+// read ws[0] (residual^2) and ws[ref] in one sync
+static int read_residual(double* ws, int ref_q, double* rr, double* ref2, hipStream_t s) {
+  double buf[kWsQuantities];
+  if (int e = ws_read(ws, 0, ref_q + 1, buf, s)) return e;
+  *rr = buf[0];
+  *ref2 = buf[ref_q];
+  return FX_OK;
+}
+
+// Flexible PCG on A = -L, c = -b.  The preconditioner fills `*z_slot`
+// (storage T) from `rT` = -r/s: the V-cycle solves L e = rT, so z_A = -s*e.
+struct PcgVectors {
+  float* x;
+  float* r;
+  float* p;
+  float* Ap;
+  void* zold;
+};
+
+template <typename Precond>
+int pcg_loop(const fx_mg_level& L0, int storage, void* rT, void** z_slot, const float* b, PcgVectors v,
+             const fx_solve_params* prm, double* ws, fx_solve_info* info, Precond&& precond, hipStream_t s) {
+  LevelDev D = dev_level(L0);
+  const int64_t n = int64_t(L0.nx) * L0.ny * L0.nz;
+  const double rms_div = 1.0 / sqrt(double(n));
+  const bool fp16 = storage == FX_F16;
+  int ref_q;
+  if (int e = reference_slot(prm, n, b, ws, s, &ref_q)) return e;
+  fx_solve_info out{};
+  double rr = 0.0, ref2 = 0.0;
+  const double tol2 = double(prm->tol) * double(prm->tol);
+  int it = 0;
+  bool done = false;
+  // slots: 0 ||r||^2, 2 r.z_old, 3 p.Ap, 4/5 alternating r.z
+  for (int restart = 0; restart < 4 && !done && it < prm->max_iters; ++restart) {
+    int q_rz = 4, q_next = 5;
+    // 16-bit residual scale: unit RMS (FP16 range); fp32/bf16 keep scale 1
+    double scale = 1.0;
+    if (fp16) {
+      if (int e = resnorm_q(L0, v.x, b, ws, 0, 15, s)) return e;
+      if (int e = read_residual(ws, ref_q, &rr, &ref2, s)) return e;
+      scale = fmax(sqrt(rr) * rms_div, 1e-30);
+    }
+    FX_DISPATCH_T(storage, FX_DISPATCH_OP(L0.op, {
+      hipLaunchKernelGGL((k_pcg_init<OPV, TV>), kRedGrid, kRedBlock, 0, s, D, v.x, b, v.r, (TV*)rT,
+                         float(1.0 / scale), ws, 0);
+    }));
+    FX_LAUNCHED("k_pcg_init");
+    ws_finalize(ws, 0, 1, s);
+    if (it >= prm->check_from || restart > 0) {
+      if (int e = read_residual(ws, ref_q, &rr, &ref2, s)) return e;
+      if (it == 0) out.residual0 = sqrt(rr);
+      if (rr <= tol2 * ref2) {
+        done = true;
+        break;
+      }
+    }
+    if (int e = precond(s)) return e;
+    float zscale = float(-scale);
+    FX_DISPATCH_T(storage, {
+      hipLaunchKernelGGL((k_dot_rz<TV>), kRedGrid, kRedBlock, 0, s, n, v.r, (const TV*)*z_slot, zscale, ws, q_rz);
+      ws_finalize(ws, q_rz, 1, s);
+      hipLaunchKernelGGL((k_pcg_direction<TV>), kRedGrid, kRedBlock, 0, s, n, ws, q_rz, q_rz, q_rz,
+                         (const TV*)*z_slot, zscale, v.p, 1);
+    });
+    FX_LAUNCHED("pcg start");
+    while (it < prm->max_iters) {
+      // the preconditioned residual becomes z_old; its buffer is reused next
+      void* t = v.zold;
+      v.zold = *z_slot;
+      *z_slot = t;
+      float zscale_old = zscale;
+      double next_scale = scale;
+      if (fp16) next_scale = fmax(sqrt(rr) * rms_div * 0.3, 1e-30);
+      FX_DISPATCH_OP(L0.op, {
+        hipLaunchKernelGGL((k_pcg_apply<OPV>), kRedGrid, kRedBlock, 0, s, D, v.p, v.Ap, ws, 3);
+      });
+      ws_finalize(ws, 3, 1, s);
+      FX_DISPATCH_T(storage, {
+        hipLaunchKernelGGL((k_pcg_update<TV>), kRedGrid, kRedBlock, 0, s, n, ws, q_rz, 3, v.x, v.p, v.r, v.Ap,
+                           (const TV*)v.zold, zscale_old, (TV*)rT, float(1.0 / next_scale), 0, 2);
+      });
+      FX_LAUNCHED("pcg update");
+      ws_finalize(ws, 0, 1, s);
+      ws_finalize(ws, 2, 1, s);
+      ++it;
+      if (fp16 || it >= prm->check_from) {
+        if (int e = read_residual(ws, ref_q, &rr, &ref2, s)) return e;
+        FX_REQUIRE(std::isfinite(rr), FX_ESTATE, "non-finite residual after %d PCG iterations", it);
+        if (rr <= tol2 * ref2) break;
+      }
+      if (it >= prm->max_iters) break;
+      scale = next_scale;
+      if (int e = precond(s)) return e;
+      zscale = float(-scale);
+      FX_DISPATCH_T(storage, {
+        hipLaunchKernelGGL((k_dot_rz<TV>), kRedGrid, kRedBlock, 0, s, n, v.r, (const TV*)*z_slot, zscale, ws,
+                           q_next);
+        ws_finalize(ws, q_next, 1, s);
+        hipLaunchKernelGGL((k_pcg_direction<TV>), kRedGrid, kRedBlock, 0, s, n, ws, q_next, 2, q_rz,
+                           (const TV*)*z_slot, zscale, v.p, 0);
+      });
+      FX_LAUNCHED("pcg direction");
+      int tmp = q_rz;
+      q_rz = q_next;
+      q_next = tmp;
+    }
+    // true residual of the final iterate (the recurrence can drift in fp32)
+    if (int e = resnorm_q(L0, v.x, b, ws, 0, 15, s)) return e;
+    if (int e = read_residual(ws, ref_q, &rr, &ref2, s)) return e;
+    done = rr <= tol2 * ref2;
+  }
+  out.iterations = it;
+  out.residual = sqrt(rr);
+  out.bnorm = sqrt(ref2);
+  out.converged = done ? 1 : 0;
+  *info = out;
+  return FX_OK;
+}
+
+}  // namespace fx
+
+using namespace fx;
+
+extern "C" {
+
+int fx_pcg_solve(fx_mg_config* pcfg, float* x, const float* b, float* r, float* p, float* Ap, void* zold,
+                 const fx_solve_params* prm, double* ws, fx_solve_info* info, void* stream) {
+  if (int e = check_config(pcfg)) return e;
+  if (int e = check_params(prm)) return e;
+  FX_REQUIRE(x && b && r && p && Ap && zold && ws && info, FX_EINVAL, "pcg: null array");
+  FX_REQUIRE(pcfg->level[0].op != FX_OP_EDGES, FX_EINVAL, "level 0 must be a box or flags operator");
+  hipStream_t s = S(stream);
+  fx_mg_level& L0 = pcfg->level[0];
+  PcgVectors v{x, r, p, Ap, zold};
+  auto precond = [&](hipStream_t st) { return vcycle(pcfg, 0, 1.0f, true, st); };
+  return pcg_loop(L0, pcfg->storage, L0.rhs, &L0.p, b, v, prm, ws, info, precond, s);
+}
+
+int fx_mixed_solve(fx_mg_config* ecfg, float* x, float* x_spare, const float* b, const fx_solve_params* prm,
+                   double* ws, fx_solve_info* info, float** x_out, void* stream) {
+  if (int e = check_config(ecfg)) return e;
+  if (int e = check_params(prm)) return e;
+  FX_REQUIRE(x && x_spare && b && ws && info && x_out && x != x_spare, FX_EINVAL, "mixed solve: bad arrays");
+  FX_REQUIRE(ecfg->level[0].op != FX_OP_EDGES, FX_EINVAL, "level 0 must be a box or flags operator");
+  hipStream_t s = S(stream);
+  fx_mg_level& L0 = ecfg->level[0];
+  const int64_t n = int64_t(L0.nx) * L0.ny * L0.nz;
+  const double rms_div = 1.0 / sqrt(double(n));
+  const bool fp16 = ecfg->storage == FX_F16;
+  int ref_q;
+  if (int e = reference_slot(prm, n, b, ws, s, &ref_q)) return e;
+  const double tol2 = double(prm->tol) * double(prm->tol);
+  fx_solve_info out{};
+  double rr = 0.0, ref2 = 0.0;
+  float* cur = x;
+  float* nxt = x_spare;
+  int it = 0;
+  bool done = false;
+  // residual of the start, written as the 16-bit right-hand side
+  double scale = 1.0;
+  if (fp16) {
+    if (int e = resnorm_q(L0, cur, b, ws, 0, 15, s)) return e;
+    if (int e = read_residual(ws, ref_q, &rr, &ref2, s)) return e;
+    scale = fmax(sqrt(rr) * rms_div, 1e-30);
+  }
+  if (int e = launch_residual(L0, ecfg->storage, cur, nullptr, 0.0f, b, nullptr, L0.rhs, float(1.0 / scale), ws, 0,
+                              s))
+    return e;
+  if (prm->check_from <= 0) {
+    if (int e = read_residual(ws, ref_q, &rr, &ref2, s)) return e;
+    out.residual0 = sqrt(rr);
+    done = rr <= tol2 * ref2;
+  }
+  while (!done && it < prm->max_iters) {
+    if (int e = vcycle(ecfg, 0, 1.0f, true, s)) return e;
+    double next_scale = fp16 ? fmax(sqrt(rr) * rms_div * 0.25, 1e-30) : 1.0;
+    if (int e = launch_residual(L0, ecfg->storage, cur, L0.p, float(scale), b, nxt, L0.rhs, float(1.0 / next_scale),
+                                ws, 0, s))
+      return e;
+    float* t = cur;
+    cur = nxt;
+    nxt = t;
+    ++it;
+    scale = next_scale;
+    if (fp16 || it >= prm->check_from) {
+      if (int e = read_residual(ws, ref_q, &rr, &ref2, s)) return e;
+      FX_REQUIRE(std::isfinite(rr), FX_ESTATE, "non-finite residual after %d refinement steps", it);
+      done = rr <= tol2 * ref2;
+      if (fp16 && !done) {
+        // keep the stored 16-bit residual near unit RMS
+        double actual = sqrt(rr) * rms_div;
+        double ratio = actual / scale;
+        if (ratio > 64.0 || ratio < 1.0 / 64.0) {
+          hipLaunchKernelGGL((k_rescale<_Float16>), dim3((n + 255) / 256), dim3(256), 0, s, n, (_Float16*)L0.rhs,
+                             float(1.0 / ratio));
+          FX_LAUNCHED("k_rescale");
+          scale = actual;
+        }
+      }
+    }
+  }
+  if (!done && it < prm->check_from) {
+    if (int e = read_residual(ws, ref_q, &rr, &ref2, s)) return e;
+    done = rr <= tol2 * ref2;
+  }
+  out.iterations = it;
+  out.residual = sqrt(rr);
+  out.bnorm = sqrt(ref2);
+  out.converged = done ? 1 : 0;
+  *info = out;
+  *x_out = cur;
+  return FX_OK;
+}
+
+}  // extern "C"

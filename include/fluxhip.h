@@ -23,6 +23,11 @@
 extern "C" {
 #endif
 
+/* Size in doubles of the FP64 reduction workspace `ws` taken by the norm and
+ * solver entry points: 16 finalised scalars followed by 1024 per-block
+ * partials for each of them (reductions use no atomics). */
+#define FX_WS_DOUBLES 16400
+
 enum {
   FX_OK = 0,
   FX_EINVAL = -1,   /* invalid argument  -> Python ValueError   */
@@ -110,6 +115,16 @@ int fx_jacobi_sweep(fx_grid g, const uint8_t* flags, const float* p, const float
 int fx_gradient(fx_grid g, const uint8_t* flags, const float* p, const float* u, const float* v,
                 const float* w, float* uo, float* vo, float* wo, float scale, void* stream);
 
+/* As fx_gradient with the pressure split p + dp: the neighbour differences of
+ * both parts are formed before they are added, so an impulse carried as
+ * warm start p plus solver increment dp keeps its full accuracy in fp32. */
+int fx_gradient_split(fx_grid g, const uint8_t* flags, const float* p, const float* dp, const float* u,
+                      const float* v, const float* w, float* uo, float* vo, float* wo, float scale, void* stream);
+
+
+/* out = a + b (n floats). */
+int fx_combine(const float* a, const float* b, float* out, int64_t n, void* stream);
+
 /* --------------------------------------------------------------------------
  * multigrid engine (backend/multigrid.py:20-95, backend/solid_multigrid.py:7-76)
  * ------------------------------------------------------------------------ */
@@ -189,29 +204,66 @@ typedef struct fx_solve_info {
   double residual;      /* final true ||b - A x||_2 (FP64 norm of the FP32 residual) */
 } fx_solve_info;
 
+/* Controls of the tolerance-driven solvers. */
+typedef struct fx_solve_params {
+  float tol;           /* stop when ||b - L x|| <= tol * reference norm */
+  int32_t max_iters;   /* V-cycles / refinement steps / CG iterations */
+  int32_t check_from;  /* iterations run before the first host-side convergence check */
+  int32_t ref_q;       /* ws[ref_q] holds the squared reference norm; -1: ||b|| of this solve */
+  int32_t x_is_zero;   /* the start iterate is zero (skips reading it in the first sweep) */
+  int32_t pad_;
+} fx_solve_params;
+
+/* r0 = b - L x0 (neighbour differences first): the right-hand side of the
+ * increment equation L d = r0 around a warm start x0.  With ws != NULL also
+ * ws[6] = ||r0||^2 and ws[7] = ||b||^2 (FP64, device side, no sync). */
+int fx_increment_rhs(fx_grid g, const uint8_t* flags, const float* x0, const float* b, float* r0, double* ws,
+                     void* stream);
+
 /* Remove the fluid-cell mean from rhs (Neumann compatibility), FP64 sums.
- * ws: >= 4 doubles of device scratch. */
+ * ws: FX_WS_DOUBLES doubles of device scratch. */
 int fx_project_mean(fx_grid g, const uint8_t* flags, float* rhs, double* ws, void* stream);
 
 /* ||b - A x||_2 and ||b||_2 in FP64 for the level-0 operator (flags NULL = box).
- * Synchronises the stream; writes host doubles. ws: >= 4 doubles. */
+ * Synchronises the stream; writes host doubles. ws: FX_WS_DOUBLES doubles. */
 int fx_residual_norm(fx_grid g, const uint8_t* flags, const float* x, const float* b,
                      double* rnorm, double* bnorm, double* ws, void* stream);
 
-/* Stationary V-cycles on level 0 of cfg (storage FX_F32) until
- * ||b - A x|| <= tol*||b|| or max_cycles.  x = level[0].p, b = level[0].rhs.
- * ws: >= 8 doubles.  Synchronises once per cycle. */
-int fx_mg_solve(fx_mg_config* cfg, float tol, int32_t max_cycles, double* ws, fx_solve_info* info,
-                void* stream);
+/* Stationary V-cycles on level 0 of cfg (storage FX_F32) until the true
+ * residual meets prm (x = level[0].p, b = level[0].rhs).  ws: FX_WS_DOUBLES
+ * doubles.  Synchronises once per cycle from prm->check_from on. */
+int fx_mg_solve(fx_mg_config* cfg, const fx_solve_params* prm, double* ws, fx_solve_info* info, void* stream);
+
+/* Mixed-precision iterative refinement on level 0 of ecfg (storage BF16/FP16,
+ * its own level-0 p/q/rhs hold the error and the 16-bit residual): x (fp32)
+ * is refined until ||b - L x|| <= tol*||b||; the fp32 residual is recomputed
+ * from x each step with the update fused in.  *x_out receives x or x_spare,
+ * whichever holds the result.  ws: FX_WS_DOUBLES doubles. */
+int fx_mixed_solve(fx_mg_config* ecfg, float* x, float* x_spare, const float* b, const fx_solve_params* prm,
+                   double* ws, fx_solve_info* info, float** x_out, void* stream);
+
+/* Flexible preconditioned CG (Polak-Ribiere beta, restarted from the true
+ * residual if the recurrence drifts) on L x = b with one V-cycle of pcfg as
+ * preconditioner.  r, p, Ap: fp32 scratch; zold: scratch in pcfg's storage.
+ * ws: FX_WS_DOUBLES doubles. */
+int fx_pcg_solve(fx_mg_config* pcfg, float* x, const float* b, float* r, float* p, float* Ap, void* zold,
+                 const fx_solve_params* prm, double* ws, fx_solve_info* info, void* stream);
 
 /* --------------------------------------------------------------------------
  * transport (backend/detail.py:16-37, backend/projected.py:150-195)
  * ------------------------------------------------------------------------ */
 
+/* Chebyshev distance (cells; 255 = more than 15) from every cell to the
+ * nearest solid or out-of-range cell, by a separable three-pass transform.  The
+ * transport kernels use it to skip the solid trace clip (3-D DDA of
+ * shaders/solid_boundary.glsl:13-36) and the 3x3x3 halo test (:39-44) where no
+ * solid is within reach; results are unchanged.  scratch: n bytes. */
+int fx_build_clearance(fx_grid g, const uint8_t* solid, uint8_t* clearance, uint8_t* scratch, void* stream);
+
 /* Face predictor for U, V and W (DetailPasses.transport_velocity,
  * backend/detail.py:27-37, shaders/transport_face.glsl:1-20). */
 int fx_face_predict(fx_grid g, float dt, const float* u, const float* v, const float* w, float* pu,
-                    float* pv, float* pw, const uint8_t* solid, void* stream);
+                    float* pv, float* pw, const uint8_t* solid, const uint8_t* clearance, void* stream);
 
 typedef struct fx_velocity_args {
   fx_grid g;
@@ -223,6 +275,7 @@ typedef struct fx_velocity_args {
   const float* density; const float* temperature;      /* pre-transport, for buoyancy; may be NULL */
   float thermal_lift; float density_weight;
   const uint8_t* solid;
+  const uint8_t* clearance;                            /* fx_build_clearance of solid, or NULL */
   const fx_sources* sources;                           /* host pointer */
 } fx_velocity_args;
 
@@ -235,7 +288,8 @@ int fx_velocity_advect(const fx_velocity_args* a, void* stream);
 /* Scalar predictor for up to 3 channels (DetailPasses.transport with +dt,
  * backend/detail.py:16-25, shaders/transport_scalar.glsl:1-19). */
 int fx_scalar_predict(fx_grid g, float dt, const float* u, const float* v, const float* w, int32_t channels,
-                      const float* const* fields, float* const* preds, const uint8_t* solid, void* stream);
+                      const float* const* fields, float* const* preds, const uint8_t* solid,
+                      const uint8_t* clearance, void* stream);
 
 typedef struct fx_scalar_args {
   fx_grid g;
@@ -250,6 +304,7 @@ typedef struct fx_scalar_args {
   float decay[3];                  /* exp(-decay*dt) factor per channel */
   int32_t channel_id[3];           /* 0 density, 1 temperature, 2 fuel */
   const uint8_t* solid;
+  const uint8_t* clearance;        /* fx_build_clearance of solid, or NULL */
   const fx_sources* sources;       /* host pointer */
 } fx_scalar_args;
 

@@ -80,7 +80,8 @@ def test_velocity_advect_matches_oracle(device, maccormack, masked):
     out = [device.texture(s) for s in grid.face_shapes]
     src = _fx_sources(SOURCES, multi=True)
     sv._advect_velocity(fields, out, dt, maccormack=maccormack, predictors=preds, sources=src, density=frho,
-                        temperature=ftemp, lift=0.7, weight=0.3, solid=solids.solid if masked else None)
+                        temperature=ftemp, lift=0.7, weight=0.3, solid=solids.solid if masked else None,
+                        clearance=solids.clearance if masked else None)
     pred_o = otr.face_predictor(vel, og, dt, solid) if maccormack else None
     if maccormack:
         for g, o in zip(faces_np(og, preds), pred_o):
@@ -138,6 +139,7 @@ def test_scalar_advect_matches_oracle(device, maccormack, masked, multi):
         args.decay[k] = decays[k]
         args.channel_id[k] = k
     args.solid = solids.solid.data_ptr() if masked else None
+    args.clearance = solids.clearance.data_ptr() if masked else None
     args.sources = C.pointer(table)
     _lib.call("fx_scalar_advect", C.byref(args), device.stream)
     ref = otr.advect_scalars(chans, vel, og, dt, maccormack, decays, srcs, multi, solid)
@@ -185,3 +187,24 @@ def test_max_abs_with_nonfinite(device):
     _lib.call("fx_max_abs", 2, arrays, sizes, out.data_ptr(), device.stream)
     res = out.cpu().numpy()
     assert res[0] == 5.5 and res[1] == np.float32(3.4e38)
+
+
+def test_clearance_matches_brute_force(device):
+    """fx_build_clearance = Chebyshev distance to the nearest solid or
+    out-of-range cell, 255 beyond 15 cells (brute force over the mask)."""
+    from fluxfx_amd.backend.collision import SolidMask
+    grid, og = _grid((40, 24, 20))
+    rng = np.random.default_rng(13)
+    solid = rng.uniform(size=og.array_shape) < .002
+    solid[5:9, 10:14, 20:26] = True
+    sm = SolidMask(grid, device=device, mask=solid.ravel())
+    got = sm.clearance.cpu().numpy().reshape(og.array_shape).astype(int)
+    nz, ny, nx = og.array_shape
+    pz, py, px = np.nonzero(solid)
+    exp = np.full(og.array_shape, 10 ** 6)
+    for (z, y, x) in np.ndindex(*og.array_shape):
+        d = min(x + 1, nx - x, y + 1, ny - y, z + 1, nz - z)
+        if len(px):
+            d = min(d, int(np.max(np.abs(np.stack([px - x, py - y, pz - z])), axis=0).min()))
+        exp[z, y, x] = d if d <= 15 else 255
+    assert np.array_equal(got, exp)
