@@ -146,7 +146,20 @@ def smoother_roofline(device, sim, reps=40):
             "bytes_per_launch": bytes_per, "achieved_gbs": gbs}
 
 
-def cpu_baseline(n_sample=64, steps=2):
+def copy_calibration(device, sim, reps=5):
+    """A plain copy kernel of known size (fx_scale with factor 1 over the cell
+    field: reads and writes 4 B per cell) that the FETCH_SIZE / WRITE_SIZE
+    counters of a profiled run are scaled by."""
+    from fluxfx_amd import _lib
+    pp = sim.projector
+    n = sim.grid.cells
+    for _ in range(reps):
+        _lib.call("fx_scale", pp.pressure.ptr, pp.spare.ptr, n, 1.0, device.stream)
+    device.torch.cuda.synchronize()
+    return {"kernel": "k_scale", "read_bytes": 4 * n, "write_bytes": 4 * n}
+
+
+def cpu_baseline(n_sample=64, steps=6):
     """FP64 NumPy oracle on the same scene resampled to n_sample^3 (1 core)."""
     from oracle import mac, transport as otr, step as ostep
     center_cells, radius_cells, boxes = scene(n_sample)
@@ -204,6 +217,7 @@ def main():
               f"cycles {np.mean(cycles):.2f}, residual {m.get('relative_residual')}", file=sys.stderr, flush=True)
         if mode == "plain":
             roof = smoother_roofline(device, sim)
+            copy_calibration(device, sim)
         del sim
         device.torch.cuda.empty_cache()
     ok = {k: v for k, v in results.items() if "steps_per_s" in v and v.get("converged")}

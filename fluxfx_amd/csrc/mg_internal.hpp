@@ -112,3 +112,11 @@ int project_mean_level(const fx_mg_level& L0, float* b, double* ws, hipStream_t 
 fx_mg_level level0_of(fx_grid g, const uint8_t* flags);
 
 }  // namespace fx
+
+// Cell loop of the 1-D reduction grids: one wave per (y, z) row with lanes
+// along x, so cell coordinates need no per-cell integer division.
+#define FX_FOR_ROW_CELLS(L, X, Y, Z, i)                                                              \
+  for (int row_ = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6), rows_ = (L).ny * (L).nz;       \
+       row_ < rows_; row_ += gridDim.x * (blockDim.x >> 6))                                          \
+    for (int Z = row_ / (L).ny, Y = row_ - Z * (L).ny, X = threadIdx.x & 63; X < (L).nx; X += 64)     \
+      if (const int64_t i = int64_t(row_) * (L).nx + X; true)

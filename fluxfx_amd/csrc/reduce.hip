@@ -50,26 +50,12 @@ int ws_read(const double* ws, int q0, int nq, double* host, hipStream_t s) {
   return FX_OK;
 }
 
-#define FX_FOR_CELLS(L, n)                                                                    \
-  for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < (n);                  \
-       i += int64_t(gridDim.x) * blockDim.x)
-
-__device__ __forceinline__ void cell_xyz(int64_t i, int nx, int ny, int& x, int& y, int& z) {
-  int t = int(i / nx);
-  x = int(i - int64_t(t) * nx);
-  z = t / ny;
-  y = t - z * ny;
-}
-
 // ws[qr] = ||b - L x||^2 (active cells), ws[qb] = ||b||^2
 template <int OP>
 __global__ void k_resnorm(LevelDev L, const float* __restrict__ x, const float* __restrict__ b, double* ws, int qr,
                           int qb) {
-  const int64_t n = int64_t(L.nx) * L.ny * L.nz;
   double rr = 0.0, bb = 0.0;
-  FX_FOR_CELLS(L, n) {
-    int X, Y, Z;
-    cell_xyz(i, L.nx, L.ny, X, Y, Z);
+  FX_FOR_ROW_CELLS(L, X, Y, Z, i) {
     float r = residual_at<float, OP>(L, X, Y, Z, x, b, 1.0f);
     float bv = b[i];
     rr += double(r) * double(r);
@@ -81,11 +67,8 @@ __global__ void k_resnorm(LevelDev L, const float* __restrict__ x, const float* 
 
 template <int OP>
 __global__ void k_mean_sum(LevelDev L, const float* __restrict__ b, double* ws) {
-  const int64_t n = int64_t(L.nx) * L.ny * L.nz;
   double s = 0.0, c = 0.0;
-  FX_FOR_CELLS(L, n) {
-    int X, Y, Z;
-    cell_xyz(i, L.nx, L.ny, X, Y, Z);
+  FX_FOR_ROW_CELLS(L, X, Y, Z, i) {
     if (cell_active<OP>(L, X, Y, Z, i)) {
       s += b[i];
       c += 1.0;
@@ -97,12 +80,9 @@ __global__ void k_mean_sum(LevelDev L, const float* __restrict__ b, double* ws) 
 
 template <int OP>
 __global__ void k_sub_mean(LevelDev L, float* __restrict__ b, const double* __restrict__ ws) {
-  const int64_t n = int64_t(L.nx) * L.ny * L.nz;
   double cnt = ws[3];
   float mean = cnt > 0 ? float(ws[2] / cnt) : 0.0f;
-  FX_FOR_CELLS(L, n) {
-    int X, Y, Z;
-    cell_xyz(i, L.nx, L.ny, X, Y, Z);
+  FX_FOR_ROW_CELLS(L, X, Y, Z, i) {
     b[i] = cell_active<OP>(L, X, Y, Z, i) ? b[i] - mean : 0.0f;
   }
 }
@@ -111,11 +91,8 @@ __global__ void k_sub_mean(LevelDev L, float* __restrict__ b, const double* __re
 template <int OP>
 __global__ void k_increment_rhs(LevelDev L, const float* __restrict__ x0, const float* __restrict__ b,
                                 float* __restrict__ r0, double* ws) {
-  const int64_t n = int64_t(L.nx) * L.ny * L.nz;
   double rr = 0.0, bb = 0.0;
-  FX_FOR_CELLS(L, n) {
-    int X, Y, Z;
-    cell_xyz(i, L.nx, L.ny, X, Y, Z);
+  FX_FOR_ROW_CELLS(L, X, Y, Z, i) {
     float r = residual_at<float, OP>(L, X, Y, Z, x0, b, 1.0f);
     r0[i] = r;
     float bv = b[i];
@@ -126,6 +103,22 @@ __global__ void k_increment_rhs(LevelDev L, const float* __restrict__ x0, const 
     block_partial(rr, ws, 6);
     block_partial(bb, ws, 7);
   }
+}
+
+// Warm-start guard: a start with ||b - L x0|| >= ||b|| is worse than zero, so it
+// is dropped (x0 = 0, r0 = b).  Every block reads the decision from ws[6], ws[7].
+__global__ void k_drop_start(int64_t n, float* __restrict__ x0, const float* __restrict__ b,
+                             float* __restrict__ r0, const double* ws) {
+  if (ws[6] < ws[7]) return;
+  for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += int64_t(gridDim.x) * blockDim.x) {
+    x0[i] = 0.0f;
+    r0[i] = b[i];
+  }
+}
+
+// Runs after k_drop_start so that no block still reads ws[6].
+__global__ void k_drop_start_norm(double* ws) {
+  if (!(ws[6] < ws[7])) ws[6] = ws[7];
 }
 
 // ws[q] = ||a||^2
@@ -205,7 +198,7 @@ int fx_residual_norm(fx_grid g, const uint8_t* flags, const float* x, const floa
   return resnorm(level0_of(g, flags), x, b, ws, rnorm, bnorm, S(stream));
 }
 
-int fx_increment_rhs(fx_grid g, const uint8_t* flags, const float* x0, const float* b, float* r0, double* ws,
+int fx_increment_rhs(fx_grid g, const uint8_t* flags, float* x0, const float* b, float* r0, double* ws,
                      void* stream) {
   if (int e = check_grid(g)) return e;
   FX_REQUIRE(x0 && b && r0 && r0 != x0 && r0 != b, FX_EINVAL, "increment rhs: bad arrays");
@@ -216,7 +209,13 @@ int fx_increment_rhs(fx_grid g, const uint8_t* flags, const float* x0, const flo
     hipLaunchKernelGGL((k_increment_rhs<OPV>), kRedGrid, kRedBlock, 0, s, D, x0, b, r0, ws);
   });
   FX_LAUNCHED("fx_increment_rhs");
-  if (ws) ws_finalize(ws, 6, 2, s);
+  if (ws) {
+    ws_finalize(ws, 6, 2, s);
+    const int64_t n = int64_t(g.nx) * g.ny * g.nz;
+    hipLaunchKernelGGL(k_drop_start, kRedGrid, kRedBlock, 0, s, n, x0, b, r0, (const double*)ws);
+    hipLaunchKernelGGL(k_drop_start_norm, dim3(1), dim3(1), 0, s, ws);
+    FX_LAUNCHED("fx_increment_rhs guard");
+  }
   return FX_OK;
 }
 

@@ -15,13 +15,6 @@ int resnorm_q(const fx_mg_level& L0, const float* x, const float* b, double* ws,
 int norm2_q(int64_t n, const float* a, double* ws, int q, hipStream_t s);
 int check_params(const fx_solve_params* p);
 
-__device__ __forceinline__ void cell_xyz2(int64_t i, int nx, int ny, int& x, int& y, int& z) {
-  int t = int(i / nx);
-  x = int(i - int64_t(t) * nx);
-  z = t / ny;
-  y = t - z * ny;
-}
-
 // 7-point operator of (x + s*e) at cell i, x fp32, e in storage T (or null).
 template <int OP, typename T>
 __device__ __forceinline__ float apply_shifted(const LevelDev& L, int X, int Y, int Z, int64_t i,
@@ -64,11 +57,8 @@ template <int OP, typename T>
 __global__ void k_ir_update(LevelDev L, const float* __restrict__ x, const T* __restrict__ e, float s_e,
                             const float* __restrict__ b, float* __restrict__ xo, T* __restrict__ rT, float inv_s,
                             double* ws, int q) {
-  const int64_t n = int64_t(L.nx) * L.ny * L.nz;
   double rr = 0.0;
-  FX_FOR_CELLS_N(n) {
-    int X, Y, Z;
-    cell_xyz2(i, L.nx, L.ny, X, Y, Z);
+  FX_FOR_ROW_CELLS(L, X, Y, Z, i) {
     bool active;
     float center;
     float lap = apply_shifted<OP, T>(L, X, Y, Z, i, x, e, s_e, active, center);
@@ -93,11 +83,8 @@ __global__ void k_rescale(int64_t n, T* __restrict__ a, float f) {
 template <int OP, typename T>
 __global__ void k_pcg_init(LevelDev L, const float* __restrict__ x, const float* __restrict__ b, float* __restrict__ r,
                            T* __restrict__ rT, float inv_s, double* ws, int q) {
-  const int64_t n = int64_t(L.nx) * L.ny * L.nz;
   double rr = 0.0;
-  FX_FOR_CELLS_N(n) {
-    int X, Y, Z;
-    cell_xyz2(i, L.nx, L.ny, X, Y, Z);
+  FX_FOR_ROW_CELLS(L, X, Y, Z, i) {
     bool active;
     float center;
     float lap = apply_shifted<OP, float>(L, X, Y, Z, i, x, (const float*)nullptr, 0.0f, active, center);
@@ -112,11 +99,8 @@ __global__ void k_pcg_init(LevelDev L, const float* __restrict__ x, const float*
 // Ap = -L p ; partial p.Ap -> q
 template <int OP>
 __global__ void k_pcg_apply(LevelDev L, const float* __restrict__ p, float* __restrict__ Ap, double* ws, int q) {
-  const int64_t n = int64_t(L.nx) * L.ny * L.nz;
   double acc = 0.0;
-  FX_FOR_CELLS_N(n) {
-    int X, Y, Z;
-    cell_xyz2(i, L.nx, L.ny, X, Y, Z);
+  FX_FOR_ROW_CELLS(L, X, Y, Z, i) {
     bool active;
     float center;
     float lap = apply_shifted<OP, float>(L, X, Y, Z, i, p, (const float*)nullptr, 0.0f, active, center);

@@ -216,8 +216,10 @@ typedef struct fx_solve_params {
 
 /* r0 = b - L x0 (neighbour differences first): the right-hand side of the
  * increment equation L d = r0 around a warm start x0.  With ws != NULL also
- * ws[6] = ||r0||^2 and ws[7] = ||b||^2 (FP64, device side, no sync). */
-int fx_increment_rhs(fx_grid g, const uint8_t* flags, const float* x0, const float* b, float* r0, double* ws,
+ * ws[6] = ||r0||^2 and ws[7] = ||b||^2 (FP64, device side, no sync), and a
+ * start no better than zero (||r0|| >= ||b||) is dropped on the device:
+ * x0 = 0, r0 = b, ws[6] = ws[7]. */
+int fx_increment_rhs(fx_grid g, const uint8_t* flags, float* x0, const float* b, float* r0, double* ws,
                      void* stream);
 
 /* Remove the fluid-cell mean from rhs (Neumann compatibility), FP64 sums.

@@ -134,6 +134,9 @@ def project(state, provisional, dt, cfg, mg_cache=None):
         x0 = None
         if cfg.warm_start and prev is not None and state.pressure is not None:
             x0 = state.pressure * (dt / prev)
+            # a start no better than zero is dropped (||b - A x0|| >= ||b||)
+            if mg.relative_residual(x0, mg.project_mean(before)) >= 1.0:
+                x0 = None
         p, hist = mg.solve(before, x0, cfg.tol, cfg.max_cycles)
         info["cycles"] = len(hist) - 1
         info["residual_history"] = hist
