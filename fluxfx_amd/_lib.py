@@ -33,7 +33,7 @@ class fx_sources(C.Structure):
 class fx_mg_level(C.Structure):
     _fields_ = [("nx", C.c_int32), ("ny", C.c_int32), ("nz", C.c_int32), ("op", C.c_int32),
                 ("w", C.c_float * 3), ("pad_", C.c_float), ("op_data", C.c_void_p),
-                ("p", C.c_void_p), ("q", C.c_void_p), ("rhs", C.c_void_p)]
+                ("p", C.c_void_p), ("q", C.c_void_p), ("rhs", C.c_void_p), ("active", C.c_void_p)]
 
 
 class fx_mg_config(C.Structure):

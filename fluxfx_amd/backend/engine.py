@@ -185,6 +185,7 @@ class MGLevels:
             self.coarse_matrix = device.torch.from_numpy(np.ascontiguousarray(coarse_matrix, np.float64)).to(device.device)
             cfg.coarse_matrix = self.coarse_matrix.data_ptr()
         self.arrays = []
+        self._active = []
         for i, shape in enumerate(shapes):
             lv = cfg.level[i]
             lv.nx, lv.ny, lv.nz = shape
@@ -193,6 +194,10 @@ class MGLevels:
             for a in range(3):
                 lv.w[a] = (shape[a] / extent[a]) ** 2
             lv.op_data = data.data_ptr() if data is not None else None
+            if op == _lib.FX_OP_EDGES and data is not None:
+                act = (data.view(-1, 4)[:, 3] > 0).to(device.torch.uint8).contiguous()
+                self._active.append(act)
+                lv.active = act.data_ptr()
             n = prod(shape)
             if i == 0 and not own_level0:
                 self.arrays.append({})

@@ -1,0 +1,187 @@
+// Four-cells-per-lane ("quad") access to x-fast level arrays.
+//
+// A lane owns four consecutive x cells (x0 % 4 == 0) of one row, so every
+// array is read with one 16-byte (fp32) or 8-byte (bf16/fp16) vector load per
+// quad instead of four scalar loads.  The x neighbours outside the quad come
+// from two scalar loads; y and z neighbours are quad loads of the adjacent
+// rows/planes (cache hits for the neighbouring lanes' data).  Usable when
+// nx % 4 == 0 and every array is 16-byte aligned; the 7-point sums keep the
+// order of the scalar stencil (x-, x+, y-, y+, z-, z+), so results match the
+// one-cell kernels bit for bit.
+#pragma once
+#include "mg_internal.hpp"
+
+namespace fx {
+
+typedef _Float16 half4_t __attribute__((ext_vector_type(4)));
+typedef _Float16 half2_t __attribute__((ext_vector_type(2)));
+
+struct F4 {
+  float v[4];
+};
+
+template <typename T>
+__device__ __forceinline__ F4 ld4(const T* __restrict__ p, int64_t i);
+template <>
+__device__ __forceinline__ F4 ld4<float>(const float* __restrict__ p, int64_t i) {
+  float4 a = *reinterpret_cast<const float4*>(p + i);
+  return F4{{a.x, a.y, a.z, a.w}};
+}
+template <>
+__device__ __forceinline__ F4 ld4<bf16>(const bf16* __restrict__ p, int64_t i) {
+  uint2 a = *reinterpret_cast<const uint2*>(p + i);
+  return F4{{__uint_as_float(a.x << 16), __uint_as_float(a.x & 0xffff0000u), __uint_as_float(a.y << 16),
+             __uint_as_float(a.y & 0xffff0000u)}};
+}
+template <>
+__device__ __forceinline__ F4 ld4<_Float16>(const _Float16* __restrict__ p, int64_t i) {
+  half4_t a = *reinterpret_cast<const half4_t*>(p + i);
+  return F4{{float(a.x), float(a.y), float(a.z), float(a.w)}};
+}
+
+template <typename T>
+__device__ __forceinline__ void st4(T* __restrict__ p, int64_t i, const F4& v);
+template <>
+__device__ __forceinline__ void st4<float>(float* __restrict__ p, int64_t i, const F4& v) {
+  *reinterpret_cast<float4*>(p + i) = make_float4(v.v[0], v.v[1], v.v[2], v.v[3]);
+}
+template <>
+__device__ __forceinline__ void st4<bf16>(bf16* __restrict__ p, int64_t i, const F4& v) {
+  uint2 a;
+  a.x = uint32_t(from_f<bf16>(v.v[0]).bits) | (uint32_t(from_f<bf16>(v.v[1]).bits) << 16);
+  a.y = uint32_t(from_f<bf16>(v.v[2]).bits) | (uint32_t(from_f<bf16>(v.v[3]).bits) << 16);
+  *reinterpret_cast<uint2*>(p + i) = a;
+}
+template <>
+__device__ __forceinline__ void st4<_Float16>(_Float16* __restrict__ p, int64_t i, const F4& v) {
+  half4_t a = {_Float16(v.v[0]), _Float16(v.v[1]), _Float16(v.v[2]), _Float16(v.v[3])};
+  *reinterpret_cast<half4_t*>(p + i) = a;
+}
+
+// two consecutive values at an even index
+template <typename T>
+__device__ __forceinline__ void st2(T* __restrict__ p, int64_t i, float a, float b);
+template <>
+__device__ __forceinline__ void st2<float>(float* __restrict__ p, int64_t i, float a, float b) {
+  *reinterpret_cast<float2*>(p + i) = make_float2(a, b);
+}
+template <>
+__device__ __forceinline__ void st2<bf16>(bf16* __restrict__ p, int64_t i, float a, float b) {
+  *reinterpret_cast<uint32_t*>(p + i) = uint32_t(from_f<bf16>(a).bits) | (uint32_t(from_f<bf16>(b).bits) << 16);
+}
+template <>
+__device__ __forceinline__ void st2<_Float16>(_Float16* __restrict__ p, int64_t i, float a, float b) {
+  half2_t h = {_Float16(a), _Float16(b)};
+  *reinterpret_cast<half2_t*>(p + i) = h;
+}
+
+// Reads of the operand p in storage T.
+template <typename T>
+struct PLoad {
+  const T* __restrict__ p;
+  __device__ __forceinline__ F4 quad(int64_t j) const { return ld4(p, j); }
+  __device__ __forceinline__ float one(int64_t j) const { return to_f(p[j]); }
+};
+
+// Reads of x + s*e (x fp32, e in storage T or null).
+template <typename T>
+struct ShiftLoad {
+  const float* __restrict__ x;
+  const T* __restrict__ e;
+  float s;
+  __device__ __forceinline__ F4 quad(int64_t j) const {
+    F4 a = ld4(x, j);
+    if (e) {
+      F4 b = ld4(e, j);
+#pragma unroll
+      for (int k = 0; k < 4; ++k) a.v[k] += s * b.v[k];
+    }
+    return a;
+  }
+  __device__ __forceinline__ float one(int64_t j) const { return x[j] + (e ? s * to_f(e[j]) : 0.0f); }
+};
+
+struct QuadStencil {
+  float s[4];     // sum_n w_n (p_n - p_c)
+  float d[4];     // sum_n w_n
+  float c[4];     // p_c
+  uint32_t f[4];  // open-face / fluid bits
+};
+
+// Quad coordinates of quad q of a level (x0 = 4*(q mod nx/4)).
+struct QuadPos {
+  int x, y, z;
+  int64_t i;
+};
+
+__device__ __forceinline__ QuadPos quad_pos(const LevelDev& L, int q) {
+  const int qx = L.nx >> 2;
+  const int row = q / qx;
+  QuadPos r;
+  r.x = (q - row * qx) << 2;
+  r.z = row / L.ny;
+  r.y = row - r.z * L.ny;
+  r.i = int64_t(row) * L.nx + r.x;
+  return r;
+}
+
+__device__ __forceinline__ uint32_t quad_flags(const LevelDev& L, int op, int x0, int y, int z, int64_t i) {
+  if (op == FX_OP_FLAGS) return *reinterpret_cast<const uint32_t*>(reinterpret_cast<const uint8_t*>(L.op) + i);
+  uint32_t w = 0;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) w |= uint32_t(box_flags(x0 + k, y, z, L.nx, L.ny, L.nz)) << (8 * k);
+  return w;
+}
+
+// 7-point operator over the quad at (x0, y, z) for a FLAGS or BOX level.
+template <int OP, class Ld>
+__device__ __forceinline__ QuadStencil quad_stencil(const LevelDev& L, int x0, int y, int z, int64_t i, const Ld& ld,
+                                                    bool read_p) {
+  static_assert(OP != FX_OP_EDGES, "quad stencils cover flag and box levels");
+  const int64_t sy = L.nx, sz = int64_t(L.nx) * L.ny;
+  const uint32_t fw = quad_flags(L, OP, x0, y, z, i);
+  F4 c{}, ym{}, yp{}, zm{}, zp{};
+  float xl = 0.0f, xr = 0.0f;
+  if (read_p) {
+    c = ld.quad(i);
+    if (y > 0) ym = ld.quad(i - sy);
+    if (y + 1 < L.ny) yp = ld.quad(i + sy);
+    if (z > 0) zm = ld.quad(i - sz);
+    if (z + 1 < L.nz) zp = ld.quad(i + sz);
+    if (x0 > 0) xl = ld.one(i - 1);
+    if (x0 + 4 < L.nx) xr = ld.one(i + 4);
+  }
+  QuadStencil st;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const uint32_t f = (fw >> (8 * k)) & 0xffu;
+    const float ck = c.v[k];
+    const float left = k > 0 ? c.v[k > 0 ? k - 1 : 0] : xl;
+    const float right = k < 3 ? c.v[k < 3 ? k + 1 : 3] : xr;
+    float s = 0.0f, d = 0.0f;
+    if (f & FLAG_XLO) { d += L.wx; s += L.wx * (left - ck); }
+    if (f & FLAG_XHI) { d += L.wx; s += L.wx * (right - ck); }
+    if (f & FLAG_YLO) { d += L.wy; s += L.wy * (ym.v[k] - ck); }
+    if (f & FLAG_YHI) { d += L.wy; s += L.wy * (yp.v[k] - ck); }
+    if (f & FLAG_ZLO) { d += L.wz; s += L.wz * (zm.v[k] - ck); }
+    if (f & FLAG_ZHI) { d += L.wz; s += L.wz * (zp.v[k] - ck); }
+    st.s[k] = s;
+    st.d[k] = d;
+    st.c[k] = ck;
+    st.f[k] = f;
+  }
+  return st;
+}
+
+// levels below this size keep the one-cell kernels (more lanes in flight)
+constexpr int64_t kQuadMinCells = int64_t(1) << 15;
+
+// host: can a level run the quad kernels?
+inline bool quad_ok(const fx_mg_level& L, const void* a = nullptr, const void* b = nullptr,
+                    const void* c = nullptr) {
+  auto al = [](const void* p) { return (reinterpret_cast<uintptr_t>(p) & 15u) == 0; };
+  return L.op != FX_OP_EDGES && L.nx % 4 == 0 && (L.op != FX_OP_FLAGS || al(L.op_data)) && al(a) && al(b) &&
+         al(c);
+}
+
+}  // namespace fx

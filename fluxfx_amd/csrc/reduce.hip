@@ -2,7 +2,7 @@
 // tolerance-driven V-cycle solve.  Reduction kernels run on a fixed 1-D grid of
 // kRedBlocks blocks with grid-stride loops; each block stores one partial per
 // quantity and a single finalize block folds them (no atomics).
-#include "mg_internal.hpp"
+#include "quad.hpp"
 
 namespace fx {
 
@@ -105,6 +105,85 @@ __global__ void k_increment_rhs(LevelDev L, const float* __restrict__ x0, const 
   }
 }
 
+// Quad versions of the level-0 reductions (nx % 4 == 0, aligned arrays).
+#define FX_FOR_QUADS(nq) \
+  for (int q_ = blockIdx.x * blockDim.x + threadIdx.x; q_ < (nq); q_ += gridDim.x * blockDim.x)
+
+template <int OP>
+__global__ void k_resnorm_q(LevelDev L, const float* __restrict__ x, const float* __restrict__ b, double* ws, int qr,
+                            int qb, int nq) {
+  double rr = 0.0, bb = 0.0;
+  FX_FOR_QUADS(nq) {
+    const QuadPos at = quad_pos(L, q_);
+    const QuadStencil st = quad_stencil<OP>(L, at.x, at.y, at.z, at.i, PLoad<float>{x}, true);
+    const F4 bv = ld4(b, at.i);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const float r = (st.f[k] & FLAG_FLUID) ? bv.v[k] - st.s[k] : 0.0f;
+      rr += double(r) * double(r);
+      bb += double(bv.v[k]) * double(bv.v[k]);
+    }
+  }
+  block_partial(rr, ws, qr);
+  block_partial(bb, ws, qb);
+}
+
+template <int OP>
+__global__ void k_increment_rhs_q(LevelDev L, const float* __restrict__ x0, const float* __restrict__ b,
+                                  float* __restrict__ r0, double* ws, int nq) {
+  double rr = 0.0, bb = 0.0;
+  FX_FOR_QUADS(nq) {
+    const QuadPos at = quad_pos(L, q_);
+    const QuadStencil st = quad_stencil<OP>(L, at.x, at.y, at.z, at.i, PLoad<float>{x0}, true);
+    const F4 bv = ld4(b, at.i);
+    F4 r;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      r.v[k] = (st.f[k] & FLAG_FLUID) ? bv.v[k] - st.s[k] : 0.0f;
+      rr += double(r.v[k]) * double(r.v[k]);
+      bb += double(bv.v[k]) * double(bv.v[k]);
+    }
+    st4(r0, at.i, r);
+  }
+  if (ws) {
+    block_partial(rr, ws, 6);
+    block_partial(bb, ws, 7);
+  }
+}
+
+template <int OP>
+__global__ void k_mean_sum_q(LevelDev L, const float* __restrict__ b, double* ws, int nq) {
+  double s = 0.0, c = 0.0;
+  FX_FOR_QUADS(nq) {
+    const QuadPos at = quad_pos(L, q_);
+    const uint32_t fw = quad_flags(L, OP, at.x, at.y, at.z, at.i);
+    const F4 bv = ld4(b, at.i);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      if ((fw >> (8 * k)) & FLAG_FLUID) {
+        s += bv.v[k];
+        c += 1.0;
+      }
+    }
+  }
+  block_partial(s, ws, 2);
+  block_partial(c, ws, 3);
+}
+
+template <int OP>
+__global__ void k_sub_mean_q(LevelDev L, float* __restrict__ b, const double* __restrict__ ws, int nq) {
+  double cnt = ws[3];
+  float mean = cnt > 0 ? float(ws[2] / cnt) : 0.0f;
+  FX_FOR_QUADS(nq) {
+    const QuadPos at = quad_pos(L, q_);
+    const uint32_t fw = quad_flags(L, OP, at.x, at.y, at.z, at.i);
+    F4 bv = ld4(b, at.i);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) bv.v[k] = ((fw >> (8 * k)) & FLAG_FLUID) ? bv.v[k] - mean : 0.0f;
+    st4(b, at.i, bv);
+  }
+}
+
 // Warm-start guard: a start with ||b - L x0|| >= ||b|| is worse than zero, so it
 // is dropped (x0 = 0, r0 = b).  Every block reads the decision from ws[6], ws[7].
 __global__ void k_drop_start(int64_t n, float* __restrict__ x0, const float* __restrict__ b,
@@ -135,9 +214,17 @@ static const dim3 kRedGrid(kRedBlocks), kRedBlock(kRedThreads);
 
 int resnorm_q(const fx_mg_level& L0, const float* x, const float* b, double* ws, int qr, int qb, hipStream_t s) {
   LevelDev D = dev_level(L0);
-  FX_DISPATCH_OP(L0.op, {
-    hipLaunchKernelGGL((k_resnorm<OPV>), kRedGrid, kRedBlock, 0, s, D, x, b, ws, qr, qb);
-  });
+  if (quad_ok(L0, x, b)) {
+    const int nq = int(int64_t(L0.nx) * L0.ny * L0.nz / 4);
+    if (L0.op == FX_OP_FLAGS)
+      hipLaunchKernelGGL((k_resnorm_q<FX_OP_FLAGS>), kRedGrid, kRedBlock, 0, s, D, x, b, ws, qr, qb, nq);
+    else
+      hipLaunchKernelGGL((k_resnorm_q<FX_OP_BOX>), kRedGrid, kRedBlock, 0, s, D, x, b, ws, qr, qb, nq);
+  } else {
+    FX_DISPATCH_OP(L0.op, {
+      hipLaunchKernelGGL((k_resnorm<OPV>), kRedGrid, kRedBlock, 0, s, D, x, b, ws, qr, qb);
+    });
+  }
   FX_LAUNCHED("k_resnorm");
   ws_finalize(ws, qr, 1, s);
   if (qb != qr) ws_finalize(ws, qb, 1, s);
@@ -163,11 +250,24 @@ int norm2_q(int64_t n, const float* a, double* ws, int q, hipStream_t s) {
 
 int project_mean_level(const fx_mg_level& L0, float* b, double* ws, hipStream_t s) {
   LevelDev D = dev_level(L0);
-  FX_DISPATCH_OP(L0.op, {
-    hipLaunchKernelGGL((k_mean_sum<OPV>), kRedGrid, kRedBlock, 0, s, D, b, ws);
-    ws_finalize(ws, 2, 2, s);
-    hipLaunchKernelGGL((k_sub_mean<OPV>), kRedGrid, kRedBlock, 0, s, D, b, ws);
-  });
+  if (quad_ok(L0, b)) {
+    const int nq = int(int64_t(L0.nx) * L0.ny * L0.nz / 4);
+    if (L0.op == FX_OP_FLAGS) {
+      hipLaunchKernelGGL((k_mean_sum_q<FX_OP_FLAGS>), kRedGrid, kRedBlock, 0, s, D, b, ws, nq);
+      ws_finalize(ws, 2, 2, s);
+      hipLaunchKernelGGL((k_sub_mean_q<FX_OP_FLAGS>), kRedGrid, kRedBlock, 0, s, D, b, ws, nq);
+    } else {
+      hipLaunchKernelGGL((k_mean_sum_q<FX_OP_BOX>), kRedGrid, kRedBlock, 0, s, D, b, ws, nq);
+      ws_finalize(ws, 2, 2, s);
+      hipLaunchKernelGGL((k_sub_mean_q<FX_OP_BOX>), kRedGrid, kRedBlock, 0, s, D, b, ws, nq);
+    }
+  } else {
+    FX_DISPATCH_OP(L0.op, {
+      hipLaunchKernelGGL((k_mean_sum<OPV>), kRedGrid, kRedBlock, 0, s, D, b, ws);
+      ws_finalize(ws, 2, 2, s);
+      hipLaunchKernelGGL((k_sub_mean<OPV>), kRedGrid, kRedBlock, 0, s, D, b, ws);
+    });
+  }
   FX_LAUNCHED("k_project_mean");
   return FX_OK;
 }
@@ -205,9 +305,17 @@ int fx_increment_rhs(fx_grid g, const uint8_t* flags, float* x0, const float* b,
   fx_mg_level L0 = level0_of(g, flags);
   LevelDev D = dev_level(L0);
   hipStream_t s = S(stream);
-  FX_DISPATCH_OP(L0.op, {
-    hipLaunchKernelGGL((k_increment_rhs<OPV>), kRedGrid, kRedBlock, 0, s, D, x0, b, r0, ws);
-  });
+  if (quad_ok(L0, x0, b, r0)) {
+    const int nq = int(int64_t(L0.nx) * L0.ny * L0.nz / 4);
+    if (L0.op == FX_OP_FLAGS)
+      hipLaunchKernelGGL((k_increment_rhs_q<FX_OP_FLAGS>), kRedGrid, kRedBlock, 0, s, D, x0, b, r0, ws, nq);
+    else
+      hipLaunchKernelGGL((k_increment_rhs_q<FX_OP_BOX>), kRedGrid, kRedBlock, 0, s, D, x0, b, r0, ws, nq);
+  } else {
+    FX_DISPATCH_OP(L0.op, {
+      hipLaunchKernelGGL((k_increment_rhs<OPV>), kRedGrid, kRedBlock, 0, s, D, x0, b, r0, ws);
+    });
+  }
   FX_LAUNCHED("fx_increment_rhs");
   if (ws) {
     ws_finalize(ws, 6, 2, s);

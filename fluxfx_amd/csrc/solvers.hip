@@ -7,7 +7,7 @@
 // Norms and dot products accumulate in FP64 per block and are folded by one
 // finalize block; alpha/beta are formed on the device.  The host reads one
 // residual norm per iteration from `check_from` on.
-#include "mg_internal.hpp"
+#include "quad.hpp"
 
 namespace fx {
 
@@ -155,16 +155,156 @@ __global__ void k_pcg_direction(int64_t n, const double* __restrict__ ws, int q_
   FX_FOR_CELLS_N(n) p[i] = to_f(z[i]) * zscale + (first ? 0.0f : beta * p[i]);
 }
 
+// ---------------------------------------------------------------------------
+// quad (four cells per lane) versions for nx % 4 == 0 and 16-byte aligned arrays
+// ---------------------------------------------------------------------------
+#define FX_FOR_QUADS(nq) \
+  for (int q_ = blockIdx.x * blockDim.x + threadIdx.x; q_ < (nq); q_ += gridDim.x * blockDim.x)
+
+template <int OP, typename T>
+__global__ void k_ir_update_q(LevelDev L, const float* __restrict__ x, const T* __restrict__ e, float s_e,
+                              const float* __restrict__ b, float* __restrict__ xo, T* __restrict__ rT, float inv_s,
+                              double* ws, int q, int nq) {
+  double rr = 0.0;
+  FX_FOR_QUADS(nq) {
+    const QuadPos at = quad_pos(L, q_);
+    const QuadStencil st = quad_stencil<OP>(L, at.x, at.y, at.z, at.i, ShiftLoad<T>{x, e, s_e}, true);
+    const F4 bv = ld4(b, at.i);
+    F4 r, xv, rs;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const bool active = st.f[k] & FLAG_FLUID;
+      r.v[k] = active ? bv.v[k] - st.s[k] : 0.0f;
+      xv.v[k] = active ? st.c[k] : 0.0f;
+      rs.v[k] = r.v[k] * inv_s;
+      rr += double(r.v[k]) * double(r.v[k]);
+    }
+    if (xo) st4(xo, at.i, xv);
+    st4(rT, at.i, rs);
+  }
+  block_partial(rr, ws, q);
+}
+
+template <int OP, typename T>
+__global__ void k_pcg_init_q(LevelDev L, const float* __restrict__ x, const float* __restrict__ b,
+                             float* __restrict__ r, T* __restrict__ rT, float inv_s, double* ws, int q, int nq) {
+  double rr = 0.0;
+  FX_FOR_QUADS(nq) {
+    const QuadPos at = quad_pos(L, q_);
+    const QuadStencil st = quad_stencil<OP>(L, at.x, at.y, at.z, at.i, PLoad<float>{x}, true);
+    const F4 bv = ld4(b, at.i);
+    F4 rv, rs;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      rv.v[k] = (st.f[k] & FLAG_FLUID) ? st.s[k] - bv.v[k] : 0.0f;
+      rs.v[k] = -rv.v[k] * inv_s;
+      rr += double(rv.v[k]) * double(rv.v[k]);
+    }
+    st4(r, at.i, rv);
+    st4(rT, at.i, rs);
+  }
+  block_partial(rr, ws, q);
+}
+
+template <int OP>
+__global__ void k_pcg_apply_q(LevelDev L, const float* __restrict__ p, float* __restrict__ Ap, double* ws, int q,
+                              int nq) {
+  double acc = 0.0;
+  FX_FOR_QUADS(nq) {
+    const QuadPos at = quad_pos(L, q_);
+    const QuadStencil st = quad_stencil<OP>(L, at.x, at.y, at.z, at.i, PLoad<float>{p}, true);
+    F4 a;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      a.v[k] = (st.f[k] & FLAG_FLUID) ? -st.s[k] : 0.0f;
+      acc += double(st.c[k]) * double(a.v[k]);
+    }
+    st4(Ap, at.i, a);
+  }
+  block_partial(acc, ws, q);
+}
+
+template <typename T>
+__global__ void k_pcg_update_q(int nq, double* ws, int q_rz, int q_pap, float* __restrict__ x,
+                               const float* __restrict__ p, float* __restrict__ r, const float* __restrict__ Ap,
+                               const T* __restrict__ zold, float zscale, T* __restrict__ rT, float inv_s, int q_rr,
+                               int q_cross) {
+  double pap = ws[q_pap];
+  float alpha = pap > 0.0 ? float(ws[q_rz] / pap) : 0.0f;
+  double rr = 0.0, rzo = 0.0;
+  FX_FOR_QUADS(nq) {
+    const int64_t i = int64_t(q_) * 4;
+    F4 xv = ld4(x, i), pv = ld4(p, i), rv = ld4(r, i), av = ld4(Ap, i), zv = ld4(zold, i), rs;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      xv.v[k] += alpha * pv.v[k];
+      rv.v[k] = rv.v[k] - alpha * av.v[k];
+      rs.v[k] = -rv.v[k] * inv_s;
+      rr += double(rv.v[k]) * double(rv.v[k]);
+      rzo += double(rv.v[k]) * double(zv.v[k] * zscale);
+    }
+    st4(x, i, xv);
+    st4(r, i, rv);
+    st4(rT, i, rs);
+  }
+  block_partial(rr, ws, q_rr);
+  block_partial(rzo, ws, q_cross);
+}
+
+template <typename T>
+__global__ void k_dot_rz_q(int nq, const float* __restrict__ r, const T* __restrict__ z, float zscale, double* ws,
+                           int q) {
+  double acc = 0.0;
+  FX_FOR_QUADS(nq) {
+    const int64_t i = int64_t(q_) * 4;
+    const F4 rv = ld4(r, i), zv = ld4(z, i);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) acc += double(rv.v[k]) * double(zv.v[k] * zscale);
+  }
+  block_partial(acc, ws, q);
+}
+
+template <typename T>
+__global__ void k_pcg_direction_q(int nq, const double* __restrict__ ws, int q_new, int q_cross, int q_old,
+                                  const T* __restrict__ z, float zscale, float* __restrict__ p, int first) {
+  float beta = 0.0f;
+  if (!first) {
+    double den = ws[q_old];
+    beta = den != 0.0 ? float((ws[q_new] - ws[q_cross]) / den) : 0.0f;
+    beta = fmaxf(beta, 0.0f);
+  }
+  FX_FOR_QUADS(nq) {
+    const int64_t i = int64_t(q_) * 4;
+    const F4 zv = ld4(z, i);
+    F4 pv = first ? F4{} : ld4(p, i);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) pv.v[k] = zv.v[k] * zscale + (first ? 0.0f : beta * pv.v[k]);
+    st4(p, i, pv);
+  }
+}
+
 static const dim3 kRedGrid(kRedBlocks), kRedBlock(kRedThreads);
 
 // r = b - L(x + s_e e) into rT (scaled) and xo; ws[q] = ||r||^2 (finalised)
 static int launch_residual(const fx_mg_level& L0, int storage, const float* x, const void* e, float s_e,
                            const float* b, float* xo, void* rT, float inv_s, double* ws, int q, hipStream_t s) {
   LevelDev D = dev_level(L0);
-  FX_DISPATCH_T(storage, FX_DISPATCH_OP(L0.op, {
-    hipLaunchKernelGGL((k_ir_update<OPV, TV>), kRedGrid, kRedBlock, 0, s, D, x, (const TV*)e, s_e, b, xo,
-                       (TV*)rT, inv_s, ws, q);
-  }));
+  if (quad_ok(L0, x, b, rT) && quad_ok(L0, e, xo)) {
+    const int nq = int(int64_t(L0.nx) * L0.ny * L0.nz / 4);
+    FX_DISPATCH_T(storage, {
+      if (L0.op == FX_OP_FLAGS)
+        hipLaunchKernelGGL((k_ir_update_q<FX_OP_FLAGS, TV>), kRedGrid, kRedBlock, 0, s, D, x, (const TV*)e, s_e, b,
+                           xo, (TV*)rT, inv_s, ws, q, nq);
+      else
+        hipLaunchKernelGGL((k_ir_update_q<FX_OP_BOX, TV>), kRedGrid, kRedBlock, 0, s, D, x, (const TV*)e, s_e, b,
+                           xo, (TV*)rT, inv_s, ws, q, nq);
+    });
+  } else {
+    FX_DISPATCH_T(storage, FX_DISPATCH_OP(L0.op, {
+      hipLaunchKernelGGL((k_ir_update<OPV, TV>), kRedGrid, kRedBlock, 0, s, D, x, (const TV*)e, s_e, b, xo,
+                         (TV*)rT, inv_s, ws, q);
+    }));
+  }
   FX_LAUNCHED("k_ir_update");
   ws_finalize(ws, q, 1, s);
   return FX_OK;
@@ -207,6 +347,9 @@ int pcg_loop(const fx_mg_level& L0, int storage, void* rT, void** z_slot, const 
   const int64_t n = int64_t(L0.nx) * L0.ny * L0.nz;
   const double rms_div = 1.0 / sqrt(double(n));
   const bool fp16 = storage == FX_F16;
+  // every vector (and both preconditioner slots) 16-byte aligned: quad kernels
+  const bool quad = quad_ok(L0, v.x, v.r, v.p) && quad_ok(L0, v.Ap, v.zold, b) && quad_ok(L0, rT, *z_slot);
+  const int nq = int(n / 4);
   int ref_q;
   if (int e = reference_slot(prm, n, b, ws, s, &ref_q)) return e;
   fx_solve_info out{};
@@ -224,10 +367,21 @@ int pcg_loop(const fx_mg_level& L0, int storage, void* rT, void** z_slot, const 
       if (int e = read_residual(ws, ref_q, &rr, &ref2, s)) return e;
       scale = fmax(sqrt(rr) * rms_div, 1e-30);
     }
-    FX_DISPATCH_T(storage, FX_DISPATCH_OP(L0.op, {
-      hipLaunchKernelGGL((k_pcg_init<OPV, TV>), kRedGrid, kRedBlock, 0, s, D, v.x, b, v.r, (TV*)rT,
-                         float(1.0 / scale), ws, 0);
-    }));
+    if (quad) {
+      FX_DISPATCH_T(storage, {
+        if (L0.op == FX_OP_FLAGS)
+          hipLaunchKernelGGL((k_pcg_init_q<FX_OP_FLAGS, TV>), kRedGrid, kRedBlock, 0, s, D, v.x, b, v.r, (TV*)rT,
+                             float(1.0 / scale), ws, 0, nq);
+        else
+          hipLaunchKernelGGL((k_pcg_init_q<FX_OP_BOX, TV>), kRedGrid, kRedBlock, 0, s, D, v.x, b, v.r, (TV*)rT,
+                             float(1.0 / scale), ws, 0, nq);
+      });
+    } else {
+      FX_DISPATCH_T(storage, FX_DISPATCH_OP(L0.op, {
+        hipLaunchKernelGGL((k_pcg_init<OPV, TV>), kRedGrid, kRedBlock, 0, s, D, v.x, b, v.r, (TV*)rT,
+                           float(1.0 / scale), ws, 0);
+      }));
+    }
     FX_LAUNCHED("k_pcg_init");
     ws_finalize(ws, 0, 1, s);
     if (it >= prm->check_from || restart > 0) {
@@ -241,10 +395,19 @@ int pcg_loop(const fx_mg_level& L0, int storage, void* rT, void** z_slot, const 
     if (int e = precond(s)) return e;
     float zscale = float(-scale);
     FX_DISPATCH_T(storage, {
-      hipLaunchKernelGGL((k_dot_rz<TV>), kRedGrid, kRedBlock, 0, s, n, v.r, (const TV*)*z_slot, zscale, ws, q_rz);
-      ws_finalize(ws, q_rz, 1, s);
-      hipLaunchKernelGGL((k_pcg_direction<TV>), kRedGrid, kRedBlock, 0, s, n, ws, q_rz, q_rz, q_rz,
-                         (const TV*)*z_slot, zscale, v.p, 1);
+      if (quad) {
+        hipLaunchKernelGGL((k_dot_rz_q<TV>), kRedGrid, kRedBlock, 0, s, nq, v.r, (const TV*)*z_slot, zscale, ws,
+                           q_rz);
+        ws_finalize(ws, q_rz, 1, s);
+        hipLaunchKernelGGL((k_pcg_direction_q<TV>), kRedGrid, kRedBlock, 0, s, nq, ws, q_rz, q_rz, q_rz,
+                           (const TV*)*z_slot, zscale, v.p, 1);
+      } else {
+        hipLaunchKernelGGL((k_dot_rz<TV>), kRedGrid, kRedBlock, 0, s, n, v.r, (const TV*)*z_slot, zscale, ws,
+                           q_rz);
+        ws_finalize(ws, q_rz, 1, s);
+        hipLaunchKernelGGL((k_pcg_direction<TV>), kRedGrid, kRedBlock, 0, s, n, ws, q_rz, q_rz, q_rz,
+                           (const TV*)*z_slot, zscale, v.p, 1);
+      }
     });
     FX_LAUNCHED("pcg start");
     while (it < prm->max_iters) {
@@ -255,13 +418,24 @@ int pcg_loop(const fx_mg_level& L0, int storage, void* rT, void** z_slot, const 
       float zscale_old = zscale;
       double next_scale = scale;
       if (fp16) next_scale = fmax(sqrt(rr) * rms_div * 0.3, 1e-30);
-      FX_DISPATCH_OP(L0.op, {
-        hipLaunchKernelGGL((k_pcg_apply<OPV>), kRedGrid, kRedBlock, 0, s, D, v.p, v.Ap, ws, 3);
-      });
+      if (quad) {
+        if (L0.op == FX_OP_FLAGS)
+          hipLaunchKernelGGL((k_pcg_apply_q<FX_OP_FLAGS>), kRedGrid, kRedBlock, 0, s, D, v.p, v.Ap, ws, 3, nq);
+        else
+          hipLaunchKernelGGL((k_pcg_apply_q<FX_OP_BOX>), kRedGrid, kRedBlock, 0, s, D, v.p, v.Ap, ws, 3, nq);
+      } else {
+        FX_DISPATCH_OP(L0.op, {
+          hipLaunchKernelGGL((k_pcg_apply<OPV>), kRedGrid, kRedBlock, 0, s, D, v.p, v.Ap, ws, 3);
+        });
+      }
       ws_finalize(ws, 3, 1, s);
       FX_DISPATCH_T(storage, {
-        hipLaunchKernelGGL((k_pcg_update<TV>), kRedGrid, kRedBlock, 0, s, n, ws, q_rz, 3, v.x, v.p, v.r, v.Ap,
-                           (const TV*)v.zold, zscale_old, (TV*)rT, float(1.0 / next_scale), 0, 2);
+        if (quad)
+          hipLaunchKernelGGL((k_pcg_update_q<TV>), kRedGrid, kRedBlock, 0, s, nq, ws, q_rz, 3, v.x, v.p, v.r, v.Ap,
+                             (const TV*)v.zold, zscale_old, (TV*)rT, float(1.0 / next_scale), 0, 2);
+        else
+          hipLaunchKernelGGL((k_pcg_update<TV>), kRedGrid, kRedBlock, 0, s, n, ws, q_rz, 3, v.x, v.p, v.r, v.Ap,
+                             (const TV*)v.zold, zscale_old, (TV*)rT, float(1.0 / next_scale), 0, 2);
       });
       FX_LAUNCHED("pcg update");
       ws_finalize(ws, 0, 1, s);
@@ -277,11 +451,19 @@ int pcg_loop(const fx_mg_level& L0, int storage, void* rT, void** z_slot, const 
       if (int e = precond(s)) return e;
       zscale = float(-scale);
       FX_DISPATCH_T(storage, {
-        hipLaunchKernelGGL((k_dot_rz<TV>), kRedGrid, kRedBlock, 0, s, n, v.r, (const TV*)*z_slot, zscale, ws,
-                           q_next);
-        ws_finalize(ws, q_next, 1, s);
-        hipLaunchKernelGGL((k_pcg_direction<TV>), kRedGrid, kRedBlock, 0, s, n, ws, q_next, 2, q_rz,
-                           (const TV*)*z_slot, zscale, v.p, 0);
+        if (quad) {
+          hipLaunchKernelGGL((k_dot_rz_q<TV>), kRedGrid, kRedBlock, 0, s, nq, v.r, (const TV*)*z_slot, zscale, ws,
+                             q_next);
+          ws_finalize(ws, q_next, 1, s);
+          hipLaunchKernelGGL((k_pcg_direction_q<TV>), kRedGrid, kRedBlock, 0, s, nq, ws, q_next, 2, q_rz,
+                             (const TV*)*z_slot, zscale, v.p, 0);
+        } else {
+          hipLaunchKernelGGL((k_dot_rz<TV>), kRedGrid, kRedBlock, 0, s, n, v.r, (const TV*)*z_slot, zscale, ws,
+                             q_next);
+          ws_finalize(ws, q_next, 1, s);
+          hipLaunchKernelGGL((k_pcg_direction<TV>), kRedGrid, kRedBlock, 0, s, n, ws, q_next, 2, q_rz,
+                             (const TV*)*z_slot, zscale, v.p, 0);
+        }
       });
       FX_LAUNCHED("pcg direction");
       int tmp = q_rz;

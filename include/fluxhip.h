@@ -147,6 +147,7 @@ typedef struct fx_mg_level {
   void* p;
   void* q;
   void* rhs;
+  const uint8_t* active;  /* optional, FX_OP_EDGES levels: 1 = active cell (fast renormalised prolongation) */
 } fx_mg_level;
 
 typedef struct fx_mg_config {

@@ -183,3 +183,35 @@ def test_tolerance_multigrid_reaches_1e5(device, masked, restriction):
     qa, xa = q[fluid] - q[fluid].mean(), x[fluid] - x[fluid].mean()
     # both iterates sit within 1e-5 relative residual of the same solution
     assert rel_err(qa, xa) < 5e-3
+
+
+@pytest.mark.parametrize("precision,krylov", [("FP32", "PCG"), ("BF16", "NONE"), ("BF16", "PCG"),
+                                              ("FP16", "NONE"), ("FP16", "PCG")])
+@pytest.mark.parametrize("masked", [False, True])
+def test_mixed_precision_solvers_reach_1e5(device, precision, krylov, masked):
+    """16-bit V-cycles inside FP32 iterative refinement, and V-cycle
+    preconditioned flexible CG, reach the FP32 tolerance; the downloaded impulse
+    is checked by an FP64 residual and against the FP64 oracle solution."""
+    from fluxfx_amd.physics.pressure import PressureSettings
+    from fluxfx_amd.backend.tolerance import ToleranceProjector
+    grid, og, rng, solid, solids = _setup(device, (64, 48, 40), (1., .75, .625), masked, seed=5)
+    fields, vel = _upload_faces(device, grid, random_faces(og, rng))
+    s = PressureSettings(pressure_tolerance=1e-5, mg_pre=2, mg_post=2, mg_coarsest=8,
+                         pressure_precision=precision, pressure_krylov=krylov)
+    pp = ToleranceProjector(grid, s, device, solids=solids)
+    outs = [device.texture(sh) for sh in grid.face_shapes]
+    pp.project(fields, outs, .04)
+    info = pp.solve_metrics()
+    assert info["converged"] and info["relative_residual"] <= 1e-5, info
+    mgo = omg.MaskedMG(og, solid, 2, 2, 8)
+    before = mac.divergence(mac.walls(vel, og, solid), og, solid)
+    b = mgo.project_mean(before)
+    q = cells_np(og, pp.pressure)
+    assert mgo.relative_residual(q, b) < 2e-5
+    x, _ = mgo.solve(before, None, 1e-9, max_cycles=200)
+    fluid = ~solid if masked else np.ones(og.array_shape, bool)
+    qa, xa = q[fluid] - q[fluid].mean(), x[fluid] - x[fluid].mean()
+    assert rel_err(qa, xa) < 5e-3
+    # the corrected velocity is divergence free to the same relative level
+    after = mac.divergence(faces_np(og, outs), og, solid)
+    assert np.sqrt((after ** 2).sum()) <= 2e-5 * np.sqrt((b ** 2).sum())
