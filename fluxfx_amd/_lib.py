@@ -43,6 +43,17 @@ class fx_mg_config(C.Structure):
                 ("pad_", C.c_int32), ("level", fx_mg_level * MAX_LEVELS)]
 
 
+class fx_conv_desc(C.Structure):
+    _fields_ = [("nx", C.c_int32), ("ny", C.c_int32), ("nz", C.c_int32), ("c0", C.c_int32), ("c1", C.c_int32),
+                ("cout", C.c_int32), ("up0", C.c_int32), ("relu", C.c_int32)]
+
+
+class fx_unet(C.Structure):
+    _fields_ = [("nx", C.c_int32), ("ny", C.c_int32), ("nz", C.c_int32), ("pad_", C.c_int32),
+                ("wt", C.c_void_p * 10), ("bias", C.c_void_p * 10), ("head", C.c_void_p), ("act", C.c_void_p * 10),
+                ("pool1", C.c_void_p), ("pool2", C.c_void_p), ("input", C.c_void_p)]
+
+
 class fx_solve_info(C.Structure):
     _fields_ = [("iterations", C.c_int32), ("converged", C.c_int32), ("fallback", C.c_int32),
                 ("pad_", C.c_int32), ("bnorm", C.c_double), ("residual0", C.c_double),
@@ -90,6 +101,23 @@ SIGNATURES = {
     "fx_gradient": ([fx_grid, P, P, P, P, P, P, P, P, F32, P], C.c_int),
     "fx_gradient_split": ([fx_grid, P, P, P, P, P, P, P, P, P, F32, P], C.c_int),
     "fx_increment_rhs": ([fx_grid, P, P, P, P, P, P], C.c_int),
+    "fx_conv3d_forward": ([C.POINTER(fx_conv_desc), P, P, P, P, P, P], C.c_int),
+    "fx_conv3d_weight_grad": ([C.POINTER(fx_conv_desc), P, P, P, P, P, I64, P, P, I32, P], C.c_int),
+    "fx_conv3d_weight_grad_workspace": ([C.POINTER(fx_conv_desc)], C.c_int64),
+    "fx_conv_pack_weights": ([I32, I32, I32, I32, P, P, P], C.c_int),
+    "fx_relu_mask": ([I64, P, P, P], C.c_int),
+    "fx_pool": ([I32, I32, I32, I32, P, P, P], C.c_int),
+    "fx_pool_backward_add": ([I32, I32, I32, I32, P, P, P, P], C.c_int),
+    "fx_concat_backward": ([I32, I32, I32, I32, I32, P, P, P, P], C.c_int),
+    "fx_unet_input": ([I64, P, P, P, P, P], C.c_int),
+    "fx_unet_head": ([I64, P, P, P, P, P, F32, P, P], C.c_int),
+    "fx_unet_head_backward": ([I64, P, P, P, P, P, P], C.c_int),
+    "fx_masked_mse": ([I64, P, P, P, F32, P, P, P], C.c_int),
+    "fx_adam": ([I64, P, P, P, P, I32, F32, F32, F32, F32, P], C.c_int),
+    "fx_to_bf16": ([I64, P, P, P], C.c_int),
+    "fx_unet_forward": ([C.POINTER(fx_unet), P, P, P, F32, I32, P, P], C.c_int),
+    "fx_learned_pcg_solve": ([C.POINTER(fx_unet), C.POINTER(fx_mg_config), P, P, P, P, P, P, P, P, P, P, P,
+                              F32, I32, I32, C.POINTER(fx_solve_params), P, C.POINTER(fx_solve_info), P], C.c_int),
     "fx_combine": ([P, P, P, I64, P], C.c_int),
     "fx_mg_vcycle": ([C.POINTER(fx_mg_config), F32, P], C.c_int),
     "fx_mg_smooth": ([C.POINTER(fx_mg_config), I32, I32, F32, P], C.c_int),

@@ -209,8 +209,44 @@ class ToleranceProjector(PressureProjector):
                   C.byref(info), self.device.stream)
         return info
 
+    def attach_network(self, network):
+        """Use a trained UNetPreconditioner for pressure_krylov = LEARNED."""
+        self.network = network
+
+    def _learned_network(self):
+        net = getattr(self, "network", None)
+        if net is None and self.settings.learned_weights:
+            from ..learned import UNetPreconditioner
+            net = UNetPreconditioner(self.device)
+            net.load(self.settings.learned_weights)
+            self.network = net
+        if net is None:
+            raise RuntimeError("LEARNED pressure mode needs a trained network: set learned_weights or "
+                               "call attach_network")
+        return net
+
     def _solve_learned(self, params):
-        raise NotImplementedError("learned preconditioner is not built into this projector")
+        net = self._learned_network()
+        eng = self._precond_engine()
+        if self._kvec is None or "rT" not in self._kvec:
+            n = self.grid.cells
+            self._kvec = {k: self.device.zeros(n) for k in ("r", "p", "Ap", "zold", "z", "rT", "ztmp")}
+        v = self._kvec
+        inc = self._inc
+        buf = net.buffers(self.grid.shape)
+        arr = eng.arrays[0]
+        used = {eng.cfg.level[0].p, eng.cfg.level[0].q}
+        mg_zold = next(t for t in (arr["z"], arr["p"], arr["q"]) if t.data_ptr() not in used)
+        w = [(n / e) ** 2 for n, e in zip(self.grid.shape, self.grid.extent)]
+        post = 3.0 / sum(w)
+        solid = self.solids.solid.data_ptr() if self.solids is not None else None
+        info = _lib.fx_solve_info()
+        _lib.call("fx_learned_pcg_solve", C.byref(buf["net"]), C.byref(eng.cfg), inc["d"].ptr, inc["r0"].ptr,
+                  v["r"].data_ptr(), v["p"].data_ptr(), v["Ap"].data_ptr(), v["zold"].data_ptr(),
+                  v["z"].data_ptr(), v["rT"].data_ptr(), v["ztmp"].data_ptr(), mg_zold.data_ptr(), solid,
+                  float(post), int(self.settings.learned_iterations), int(self.settings.learned_sweeps),
+                  C.byref(params), self.ws.data_ptr(), C.byref(info), self.device.stream)
+        return info
 
     def solve_metrics(self):
         i = self.last_info

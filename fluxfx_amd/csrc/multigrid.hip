@@ -543,6 +543,11 @@ int vcycle(fx_mg_config* c, int l, float inv_dt, bool zero_guess, hipStream_t s)
   return smooth_n(c, l, c->post, inv_dt, false, s);
 }
 
+// `count` sweeps on level l from its current p (p/q swap per sweep)
+int smooth_level(fx_mg_config* c, int l, int count, hipStream_t s) {
+  return smooth_n(c, l, count, 1.0f, false, s);
+}
+
 fx_mg_level level0_of(fx_grid g, const uint8_t* flags) {
   fx_mg_level L{};
   L.nx = g.nx; L.ny = g.ny; L.nz = g.nz;

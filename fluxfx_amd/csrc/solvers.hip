@@ -7,6 +7,8 @@
 // Norms and dot products accumulate in FP64 per block and are folded by one
 // finalize block; alpha/beta are formed on the device.  The host reads one
 // residual norm per iteration from `check_from` on.
+#include <algorithm>
+
 #include "quad.hpp"
 
 namespace fx {
@@ -14,6 +16,9 @@ namespace fx {
 int resnorm_q(const fx_mg_level& L0, const float* x, const float* b, double* ws, int qr, int qb, hipStream_t s);
 int norm2_q(int64_t n, const float* a, double* ws, int q, hipStream_t s);
 int check_params(const fx_solve_params* p);
+int unet_forward(const fx_unet* net, const float* r, const uint8_t* solid, double* ws, float post_scale,
+                 int output_rms, float* z, hipStream_t s);
+int smooth_level(fx_mg_config* c, int l, int count, hipStream_t s);
 
 // 7-point operator of (x + s*e) at cell i, x fp32, e in storage T (or null).
 template <int OP, typename T>
@@ -357,8 +362,10 @@ int pcg_loop(const fx_mg_level& L0, int storage, void* rT, void** z_slot, const 
   const double tol2 = double(prm->tol) * double(prm->tol);
   int it = 0;
   bool done = false;
+  bool measured = false;
   // slots: 0 ||r||^2, 2 r.z_old, 3 p.Ap, 4/5 alternating r.z
   for (int restart = 0; restart < 4 && !done && it < prm->max_iters; ++restart) {
+    measured = true;
     int q_rz = 4, q_next = 5;
     // 16-bit residual scale: unit RMS (FP16 range); fp32/bf16 keep scale 1
     double scale = 1.0;
@@ -475,6 +482,13 @@ int pcg_loop(const fx_mg_level& L0, int storage, void* rT, void** z_slot, const 
     if (int e = read_residual(ws, ref_q, &rr, &ref2, s)) return e;
     done = rr <= tol2 * ref2;
   }
+  if (!measured) {
+    // no iteration budget: report the start
+    if (int e = resnorm_q(L0, v.x, b, ws, 0, 15, s)) return e;
+    if (int e = read_residual(ws, ref_q, &rr, &ref2, s)) return e;
+    out.residual0 = sqrt(rr);
+    done = rr <= tol2 * ref2;
+  }
   out.iterations = it;
   out.residual = sqrt(rr);
   out.bnorm = sqrt(ref2);
@@ -500,6 +514,55 @@ int fx_pcg_solve(fx_mg_config* pcfg, float* x, const float* b, float* r, float* 
   PcgVectors v{x, r, p, Ap, zold};
   auto precond = [&](hipStream_t st) { return vcycle(pcfg, 0, 1.0f, true, st); };
   return pcg_loop(L0, pcfg->storage, L0.rhs, &L0.p, b, v, prm, ws, info, precond, s);
+}
+
+int fx_learned_pcg_solve(const fx_unet* net, fx_mg_config* mg, float* x, const float* b, float* r, float* p,
+                         float* Ap, float* zold, float* z, float* rT, float* ztmp, void* mg_zold,
+                         const uint8_t* solid, float post_scale, int32_t learned_iters, int32_t sweeps,
+                         const fx_solve_params* prm, double* ws, fx_solve_info* info, void* stream) {
+  if (int e = check_config(mg)) return e;
+  if (int e = check_params(prm)) return e;
+  FX_REQUIRE(net && x && b && r && p && Ap && zold && z && rT && ztmp && mg_zold && ws && info, FX_EINVAL,
+             "learned pcg: null array");
+  FX_REQUIRE(sweeps >= 0 && sweeps % 2 == 0, FX_EINVAL, "learned pcg: smoothing sweeps must be even");
+  fx_mg_level& L0 = mg->level[0];
+  FX_REQUIRE(L0.op != FX_OP_EDGES, FX_EINVAL, "level 0 must be a box or flags operator");
+  FX_REQUIRE(net->nx == L0.nx && net->ny == L0.ny && net->nz == L0.nz, FX_EINVAL, "network grid differs from level 0");
+  hipStream_t s = S(stream);
+  fx_solve_params lp = *prm;
+  lp.max_iters = std::min(prm->max_iters, std::max(0, learned_iters));
+  void* zs = z;
+  PcgVectors v{x, r, p, Ap, zold};
+  // z = U-Net(rT), then `sweeps` damped-Jacobi sweeps on L z = rT from it (the
+  // network carries the smooth part of the solution, the sweeps the rough part)
+  fx_mg_config sm = *mg;
+  sm.storage = FX_F32;
+  auto precond = [&](hipStream_t st) {
+    if (int e = unet_forward(net, rT, solid, ws, post_scale, 1, (float*)zs, st)) return e;
+    if (sweeps <= 0) return int(FX_OK);
+    sm.level[0].p = zs;
+    sm.level[0].q = ztmp;
+    sm.level[0].rhs = rT;
+    if (int e = smooth_level(&sm, 0, sweeps, st)) return e;
+    return int(FX_OK);
+  };
+  fx_solve_info first{};
+  if (int e = pcg_loop(L0, FX_F32, rT, &zs, b, v, &lp, ws, &first, precond, s)) return e;
+  *info = first;
+  if (first.converged) return FX_OK;
+  // fallback: V-cycle preconditioned FPCG from the learned iterate
+  fx_solve_params fp = *prm;
+  fp.max_iters = std::max(0, prm->max_iters - first.iterations);
+  fp.check_from = 0;
+  fx_solve_info second{};
+  PcgVectors v2{x, r, p, Ap, mg_zold};
+  auto mg_precond = [&](hipStream_t st) { return vcycle(mg, 0, 1.0f, true, st); };
+  if (int e = pcg_loop(L0, mg->storage, L0.rhs, &L0.p, b, v2, &fp, ws, &second, mg_precond, s)) return e;
+  second.iterations += first.iterations;
+  second.residual0 = first.residual0;
+  second.fallback = 1;
+  *info = second;
+  return FX_OK;
 }
 
 int fx_mixed_solve(fx_mg_config* ecfg, float* x, float* x_spare, const float* b, const fx_solve_params* prm,

@@ -11,6 +11,9 @@ tolerance-driven solvers:
 * ``pressure_krylov``: NONE (stationary V-cycles), PCG (flexible CG with a
   V-cycle preconditioner) or LEARNED (flexible CG with the U-Net
   preconditioner, MG fallback).
+* ``learned_weights``/``learned_iterations``: the U-Net's trained parameters
+  (.npz written by UNetPreconditioner.save) and how many U-Net preconditioned
+  iterations run before the V-cycle preconditioned fallback takes over.
 * ``mg_pre``/``mg_post``/``mg_coarsest``/``mg_restriction``/``mg_levels``:
   V-cycle shape.  ``mg_coarsest`` = 4 with 3/3 sweeps is the reference
   (backend/multigrid.py:57-81); the configs use 2/2 and an 8^3 FP64 direct solve.
@@ -94,6 +97,10 @@ class PressureSettings(ThermalSettings):
     mg_post: int = 3
     mg_coarsest: int = 4
     mg_restriction: str = "MEAN"
+    # learned preconditioner: trained weights (.npz) and the U-Net iteration budget before MG fallback
+    learned_weights: str = ""
+    learned_iterations: int = 2
+    learned_sweeps: int = 2
 
     def __post_init__(self):
         super().__post_init__()

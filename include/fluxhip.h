@@ -329,6 +329,105 @@ int fx_combustion(int64_t n, float dt, float ignition, float burn_rate, float he
  * (AdaptiveTimestep.bounds, backend/timestep.py:24-33, shaders/max_reduce.glsl). */
 int fx_max_abs(int32_t count, const float* const* arrays, const int64_t* sizes, float* out, void* stream);
 
+
+/* --------------------------------------------------------------------------
+ * learned preconditioner (no reference counterpart): 3-level 3D U-Net on
+ * channels-last bf16 activations act[v*C + c], v = x + nx*(y + ny*z), fp32
+ * accumulation on the matrix cores.  Weights: fp32 masters w[27][cin][cout]
+ * (tap = (dz+1)*9 + (dy+1)*3 + (dx+1)), packed to bf16 for the kernels.
+ * ------------------------------------------------------------------------ */
+typedef struct fx_conv_desc {
+  int32_t nx, ny, nz;  /* output grid */
+  int32_t c0;          /* channels of in0 (multiple of 8) */
+  int32_t c1;          /* channels of in1, concatenated after in0 (0: none) */
+  int32_t cout;        /* output channels (16..96, multiple of 16) */
+  int32_t up0;         /* in0 lives on the half grid and is nearest-upsampled */
+  int32_t relu;        /* ReLU after the bias */
+} fx_conv_desc;
+
+/* out[v][co] = act(bias[co] + sum_{tap,ci} in(v + off(tap), ci) wt[co][tap*(c0+c1) + ci]),
+ * zero padding; in = concat(upsample?(in0), in1).  wt: bf16 [cout][kp],
+ * kp = 27*(c0+c1) rounded up to 32 (fx_conv_pack_weights).  Also the data
+ * gradient, with weights packed with flip = 1 and c0 = cout of the layer. */
+int fx_conv3d_forward(const fx_conv_desc* conv, const void* in0, const void* in1, const void* wt, const float* bias,
+                      void* out, void* stream);
+/* dw[27][cin][cout] (+)= sum_v in(v + off, ci) g'(v, co), db[co] (+)= sum_v g'(v, co) with
+ * g' = g * (act > 0) when act != NULL.  `work` (fx_conv3d_weight_grad_workspace
+ * bytes) holds channel-major copies and per-chunk partial sums. */
+int fx_conv3d_weight_grad(const fx_conv_desc* conv, const void* in0, const void* in1, const void* g,
+                          const void* act, void* work, int64_t work_bytes, float* dw, float* db, int32_t accumulate,
+                          void* stream);
+int64_t fx_conv3d_weight_grad_workspace(const fx_conv_desc* conv);
+/* fp32 w[27][cin][cout] -> bf16 kernel layout: flip = 0 -> [cout][kp] with the
+ * input padded to cpack channels; flip = 1 -> data-gradient layout
+ * [round16(cin)][27*cout rounded to 32] with w[26 - tap]. */
+int fx_conv_pack_weights(int32_t cin, int32_t cpack, int32_t cout, int32_t flip, const float* w, void* wt,
+                         void* stream);
+/* g[i] = act[i] > 0 ? g[i] : 0 (bf16, in place) */
+int fx_relu_mask(int64_t n, void* g, const void* act, void* stream);
+/* 2x2x2 mean of a channels-last bf16 tensor on the fine grid (nx, ny, nz) */
+int fx_pool(int32_t nx, int32_t ny, int32_t nz, int32_t c, const void* in, void* out, void* stream);
+/* out = skip + (adjoint of the 2x2x2 mean)(gpool) on the fine grid; skip may be NULL */
+int fx_pool_backward_add(int32_t nx, int32_t ny, int32_t nz, int32_t c, const void* gpool, const void* skip,
+                         void* out, void* stream);
+/* gradient of concat(upsample(a), b) on the fine grid: gup = sum over the 8
+ * children of g[:, :c0] (half grid), gskip = g[:, c0:] */
+int fx_concat_backward(int32_t nx, int32_t ny, int32_t nz, int32_t c0, int32_t c1, const void* g, void* gup,
+                       void* gskip, void* stream);
+/* packed network input [n][8]: (r / rms_fluid(r), solid, 0...) with
+ * ws[8] = sum_fluid r^2, ws[9] = fluid count (device side) */
+int fx_unet_input(int64_t n, const float* r, const uint8_t* solid, double* ws, void* out, void* stream);
+/* z[v] = fluid ? (sum_c act[v][c] w[c] + b) * s : 0, s = rms_fluid(r) * post_scale
+ * from ws[8..9] (fx_unet_input), or s = post_scale when ws == NULL */
+int fx_unet_head(int64_t n, const void* act, const float* w, const float* b, const uint8_t* solid, const double* ws,
+                 float post_scale, float* z, void* stream);
+/* head backward: gact = gout (x) w; dwb[0..15] += sum gout*act, dwb[16] += sum gout */
+int fx_unet_head_backward(int64_t n, const void* act, const float* gout, const float* w, void* gact, float* dwb,
+                          void* stream);
+/* masked MSE: gout = fluid ? 2 (out - target) * inv_count : 0; ws[10] = sum_fluid
+ * (out - target)^2, ws[11] = fluid count */
+int fx_masked_mse(int64_t n, const float* out, const float* target, const uint8_t* solid, float inv_count,
+                  float* gout, double* ws, void* stream);
+/* Adam (bias-corrected, step t >= 1) on fp32 parameters */
+int fx_adam(int64_t n, float* p, const float* g, float* m, float* v, int32_t t, float lr, float beta1, float beta2,
+            float eps, void* stream);
+int fx_to_bf16(int64_t n, const float* a, void* b, void* stream);
+
+
+/* Network buffers for fx_unet_forward / fx_learned_pcg_solve.  Level-0 grid
+ * dims divisible by 4 (two poolings).  Layer order: e0a e0b e1a e1b b2a b2b
+ * d1a d1b d0a d0b; act[i] is the ReLU output of layer i. */
+typedef struct fx_unet {
+  int32_t nx, ny, nz;
+  int32_t pad_;
+  const void* wt[10];    /* packed bf16 forward weights (fx_conv_pack_weights, flip = 0) */
+  const float* bias[10]; /* fp32 biases */
+  const float* head;     /* fp32 [17]: 1x1 head weights (16) then bias */
+  void* act[10];         /* bf16 [V_level][cout] */
+  void* pool1;           /* bf16 [V/8][16] */
+  void* pool2;           /* bf16 [V/64][32] */
+  void* input;           /* bf16 [V][8] */
+} fx_unet;
+
+/* z = U-Net(r): input (r / rms_fluid(r), solid); output on fluid cells
+ * scaled by post_scale, times rms_fluid(r) when output_rms != 0.  Uses
+ * ws[8], ws[9]. */
+int fx_unet_forward(const fx_unet* net, const float* r, const uint8_t* solid, double* ws, float post_scale,
+                    int32_t output_rms, float* z, void* stream);
+
+/* FP32 flexible PCG on the level-0 operator of `mg` with the U-Net as the
+ * preconditioner: z = U-Net(r) * post_scale followed by `sweeps` (even)
+ * damped-Jacobi sweeps on L z = r (ztmp: fp32 ping-pong); when the true residual misses
+ * the tolerance it continues with V-cycle preconditioned FPCG on `mg`
+ * (info->fallback = 1).  Vectors: x (in/out), b, r, p, Ap, zold, z, rT fp32
+ * of the level-0 size; mg->level[0] p/q/rhs are the V-cycle's work arrays and
+ * mg_zold a fourth level-0 array, all in mg->storage (the fallback swaps
+ * level[0].p with mg_zold as fx_pcg_solve does). */
+int fx_learned_pcg_solve(const fx_unet* net, fx_mg_config* mg, float* x, const float* b, float* r, float* p,
+                         float* Ap, float* zold, float* z, float* rT, float* ztmp, void* mg_zold,
+                         const uint8_t* solid, float post_scale, int32_t learned_iters, int32_t sweeps,
+                         const fx_solve_params* prm, double* ws, fx_solve_info* info, void* stream);
+
 #ifdef __cplusplus
 }
 #endif
