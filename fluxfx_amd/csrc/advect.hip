@@ -9,6 +9,8 @@ namespace fx {
 struct GridDev {
   int nx, ny, nz;
   float ex, ey, ez;  // extent (m)
+  float hx, hy, hz;  // cell size (m)
+  float ix, iy, iz;  // cells per metre
 };
 
 struct V3 {
@@ -19,6 +21,7 @@ __device__ __forceinline__ V3 v3(float x, float y, float z) { return V3{x, y, z}
 // ---------------------------------------------------------------------------
 // face sampling: q clamped to [0,size-1], corner index min(b+o,size-1)
 // ---------------------------------------------------------------------------
+// 32-bit indices: every array here has fewer than 2^31 elements (checked by check_grid)
 __device__ __forceinline__ float sample_face(const float* __restrict__ F, int sx, int sy, int sz, float qx, float qy,
                                              float qz) {
   qx = fminf(fmaxf(qx, 0.0f), float(sx - 1));
@@ -27,8 +30,9 @@ __device__ __forceinline__ float sample_face(const float* __restrict__ F, int sx
   int bx = int(floorf(qx)), by = int(floorf(qy)), bz = int(floorf(qz));
   float fx = qx - bx, fy = qy - by, fz = qz - bz;
   int x1 = min(bx + 1, sx - 1), y1 = min(by + 1, sy - 1), z1 = min(bz + 1, sz - 1);
-  int64_t r00 = int64_t(sx) * (by + int64_t(sy) * bz), r10 = int64_t(sx) * (y1 + int64_t(sy) * bz);
-  int64_t r01 = int64_t(sx) * (by + int64_t(sy) * z1), r11 = int64_t(sx) * (y1 + int64_t(sy) * z1);
+  const int pl = sx * sy;
+  int r00 = by * sx + bz * pl, r10 = y1 * sx + bz * pl;
+  int r01 = by * sx + z1 * pl, r11 = y1 * sx + z1 * pl;
   float gx = 1.0f - fx, gy = 1.0f - fy, gz = 1.0f - fz;
   float v = 0.0f;
   v += gx * gy * gz * F[r00 + bx];
@@ -42,6 +46,36 @@ __device__ __forceinline__ float sample_face(const float* __restrict__ F, int sx
   return v;
 }
 
+// sample_face plus the min/max of its eight donors (MacCormack clamp; the
+// donor set equals the clamp set whenever q lies inside the array)
+__device__ __forceinline__ float sample_face_mm(const float* __restrict__ F, int sx, int sy, int sz, float qx, float qy,
+                                                float qz, float& lo, float& hi) {
+  qx = fminf(fmaxf(qx, 0.0f), float(sx - 1));
+  qy = fminf(fmaxf(qy, 0.0f), float(sy - 1));
+  qz = fminf(fmaxf(qz, 0.0f), float(sz - 1));
+  int bx = int(floorf(qx)), by = int(floorf(qy)), bz = int(floorf(qz));
+  float fx = qx - bx, fy = qy - by, fz = qz - bz;
+  int x1 = min(bx + 1, sx - 1), y1 = min(by + 1, sy - 1), z1 = min(bz + 1, sz - 1);
+  const int pl = sx * sy;
+  int r00 = by * sx + bz * pl, r10 = y1 * sx + bz * pl;
+  int r01 = by * sx + z1 * pl, r11 = y1 * sx + z1 * pl;
+  const float d0 = F[r00 + bx], d1 = F[r00 + x1], d2 = F[r10 + bx], d3 = F[r10 + x1];
+  const float d4 = F[r01 + bx], d5 = F[r01 + x1], d6 = F[r11 + bx], d7 = F[r11 + x1];
+  float gx = 1.0f - fx, gy = 1.0f - fy, gz = 1.0f - fz;
+  float v = 0.0f;
+  v += gx * gy * gz * d0;
+  v += fx * gy * gz * d1;
+  v += gx * fy * gz * d2;
+  v += fx * fy * gz * d3;
+  v += gx * gy * fz * d4;
+  v += fx * gy * fz * d5;
+  v += gx * fy * fz * d6;
+  v += fx * fy * fz * d7;
+  lo = fminf(fminf(fminf(d0, d1), fminf(d2, d3)), fminf(fminf(d4, d5), fminf(d6, d7)));
+  hi = fmaxf(fmaxf(fmaxf(d0, d1), fmaxf(d2, d3)), fmaxf(fmaxf(d4, d5), fmaxf(d6, d7)));
+  return v;
+}
+
 struct Vel {
   const float* u;
   const float* v;
@@ -50,10 +84,21 @@ struct Vel {
 
 // velocityAt: q = p*cellCount/extent, three staggered samples
 __device__ __forceinline__ V3 velocity_at(const GridDev& g, const Vel& V, V3 p) {
-  float qx = p.x * g.nx / g.ex, qy = p.y * g.ny / g.ey, qz = p.z * g.nz / g.ez;
+  float qx = p.x * g.ix, qy = p.y * g.iy, qz = p.z * g.iz;
   return v3(sample_face(V.u, g.nx + 1, g.ny, g.nz, qx, qy - 0.5f, qz - 0.5f),
             sample_face(V.v, g.nx, g.ny + 1, g.nz, qx - 0.5f, qy, qz - 0.5f),
             sample_face(V.w, g.nx, g.ny, g.nz + 1, qx - 0.5f, qy - 0.5f, qz));
+}
+
+// velocityAt at face i of axis AXIS: the own component is the face value
+// itself (the trilinear weights at a lattice node), the others are sampled
+template <int AXIS>
+__device__ __forceinline__ V3 velocity_at_face(const GridDev& g, const Vel& V, V3 p, int i) {
+  float qx = p.x * g.ix, qy = p.y * g.iy, qz = p.z * g.iz;
+  float u = AXIS == 0 ? V.u[i] : sample_face(V.u, g.nx + 1, g.ny, g.nz, qx, qy - 0.5f, qz - 0.5f);
+  float v = AXIS == 1 ? V.v[i] : sample_face(V.v, g.nx, g.ny + 1, g.nz, qx - 0.5f, qy, qz - 0.5f);
+  float w = AXIS == 2 ? V.w[i] : sample_face(V.w, g.nx, g.ny, g.nz + 1, qx - 0.5f, qy - 0.5f, qz);
+  return v3(u, v, w);
 }
 
 // ---------------------------------------------------------------------------
@@ -61,17 +106,17 @@ __device__ __forceinline__ V3 velocity_at(const GridDev& g, const Vel& V, V3 p) 
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ bool solid_cell(const GridDev& g, const uint8_t* __restrict__ s, int x, int y, int z) {
   if (x < 0 || y < 0 || z < 0 || x >= g.nx || y >= g.ny || z >= g.nz) return true;
-  return s[idx3(x, y, z, g.nx, g.ny)] != 0;
+  return s[x + g.nx * (y + g.ny * z)] != 0;
 }
 
 // 3-D DDA clip of origin->target at the first solid cell.  A trace that is
 // neither clamped to the domain nor blocked returns `target` itself.
 __device__ V3 clip_trace(const GridDev& g, const uint8_t* __restrict__ s, V3 o, V3 t) {
   const float cx = float(g.nx), cy = float(g.ny), cz = float(g.nz);
-  float ax = fminf(fmaxf(o.x / g.ex * cx, 1e-4f), cx - 1e-4f);
-  float ay = fminf(fmaxf(o.y / g.ey * cy, 1e-4f), cy - 1e-4f);
-  float az = fminf(fmaxf(o.z / g.ez * cz, 1e-4f), cz - 1e-4f);
-  float rbx = t.x / g.ex * cx, rby = t.y / g.ey * cy, rbz = t.z / g.ez * cz;
+  float ax = fminf(fmaxf(o.x * g.ix, 1e-4f), cx - 1e-4f);
+  float ay = fminf(fmaxf(o.y * g.iy, 1e-4f), cy - 1e-4f);
+  float az = fminf(fmaxf(o.z * g.iz, 1e-4f), cz - 1e-4f);
+  float rbx = t.x * g.ix, rby = t.y * g.iy, rbz = t.z * g.iz;
   float bx = fminf(fmaxf(rbx, 1e-4f), cx - 1e-4f);
   float by = fminf(fmaxf(rby, 1e-4f), cy - 1e-4f);
   float bz = fminf(fmaxf(rbz, 1e-4f), cz - 1e-4f);
@@ -92,12 +137,12 @@ __device__ V3 clip_trace(const GridDev& g, const uint8_t* __restrict__ s, V3 o, 
     float tt = axis == 0 ? crx : (axis == 1 ? cry : crz);
     if (tt >= 1.0f) {
       if (!clamped) return t;
-      return v3(bx / cx * g.ex, by / cy * g.ey, bz / cz * g.ez);
+      return v3(bx * g.hx, by * g.hy, bz * g.hz);
     }
     if (axis == 0) ix += sx; else if (axis == 1) iy += sy; else iz += sz;
     if (solid_cell(g, s, ix, iy, iz)) {
       float f = fmaxf(0.0f, tt - 1e-4f / fmaxf(len, 1.0f));
-      return v3((ax + dx * f) / cx * g.ex, (ay + dy * f) / cy * g.ey, (az + dz * f) / cz * g.ez);
+      return v3((ax + dx * f) * g.hx, (ay + dy * f) * g.hy, (az + dz * f) * g.hz);
     }
     if (axis == 0) crx += stx; else if (axis == 1) cry += sty; else crz += stz;
   }
@@ -105,7 +150,7 @@ __device__ V3 clip_trace(const GridDev& g, const uint8_t* __restrict__ s, V3 o, 
 }
 
 __device__ __forceinline__ bool solid_halo(const GridDev& g, const uint8_t* __restrict__ s, V3 p) {
-  int cx = int(floorf(p.x / g.ex * g.nx)), cy = int(floorf(p.y / g.ey * g.ny)), cz = int(floorf(p.z / g.ez * g.nz));
+  int cx = int(floorf(p.x * g.ix)), cy = int(floorf(p.y * g.iy)), cz = int(floorf(p.z * g.iz));
   for (int z = -1; z <= 1; ++z)
     for (int y = -1; y <= 1; ++y)
       for (int x = -1; x <= 1; ++x)
@@ -130,16 +175,16 @@ __device__ bool safe_correction(const GridDev& g, const uint8_t* __restrict__ s,
 // that distance of its origin cell crosses no solid, so the DDA can be skipped;
 // a 3x3x3 halo is solid-free exactly when the clearance exceeds 1.
 __device__ __forceinline__ int clearance_at(const GridDev& g, const uint8_t* __restrict__ cl, V3 p) {
-  int cx = int(floorf(p.x / g.ex * g.nx)), cy = int(floorf(p.y / g.ey * g.ny)), cz = int(floorf(p.z / g.ez * g.nz));
+  int cx = int(floorf(p.x * g.ix)), cy = int(floorf(p.y * g.iy)), cz = int(floorf(p.z * g.iz));
   if (cx < 0 || cy < 0 || cz < 0 || cx >= g.nx || cy >= g.ny || cz >= g.nz) return 0;
-  return cl[idx3(cx, cy, cz, g.nx, g.ny)];
+  return cl[cx + g.nx * (cy + g.ny * cz)];
 }
 
 __device__ __forceinline__ V3 clip_fast(const GridDev& g, const uint8_t* __restrict__ s,
                                         const uint8_t* __restrict__ cl, V3 o, V3 t) {
   if (cl) {
-    float r = fmaxf(fmaxf(fabsf((t.x - o.x) / g.ex * g.nx), fabsf((t.y - o.y) / g.ey * g.ny)),
-                    fabsf((t.z - o.z) / g.ez * g.nz));
+    float r = fmaxf(fmaxf(fabsf((t.x - o.x) * g.ix), fabsf((t.y - o.y) * g.iy)),
+                    fabsf((t.z - o.z) * g.iz));
     if (float(clearance_at(g, cl, o)) > ceilf(r) + 1.0f) return t;
   }
   return clip_trace(g, s, o, t);
@@ -240,15 +285,14 @@ __global__ void k_face_predict(GridDev g, Vel V, float dt, const float* __restri
     }
   }
   const float offx = AXIS == 0 ? 0.0f : 0.5f, offy = AXIS == 1 ? 0.0f : 0.5f, offz = AXIS == 2 ? 0.0f : 0.5f;
-  float hx = g.ex / g.nx, hy = g.ey / g.ny, hz = g.ez / g.nz;
-  V3 p = v3((x + offx) * hx, (y + offy) * hy, (z + offz) * hz);
-  V3 vp = velocity_at(g, V, p);
+  V3 p = v3((x + offx) * g.hx, (y + offy) * g.hy, (z + offz) * g.hz);
+  V3 vp = velocity_at_face<AXIS>(g, V, p, int(i));
   V3 mid = v3(p.x - 0.5f * dt * vp.x, p.y - 0.5f * dt * vp.y, p.z - 0.5f * dt * vp.z);
   if (SOLID) mid = clip_fast(g, solid, clear, p, mid);
   V3 vm = velocity_at(g, V, mid);
   V3 dep = v3(p.x - dt * vm.x, p.y - dt * vm.y, p.z - dt * vm.z);
   if (SOLID) dep = clip_fast(g, solid, clear, p, dep);
-  out[i] = sample_face(in, sx, sy, sz, dep.x / hx - offx, dep.y / hy - offy, dep.z / hz - offz);
+  out[i] = sample_face(in, sx, sy, sz, dep.x * g.ix - offx, dep.y * g.iy - offy, dep.z * g.iz - offz);
 }
 
 // ---------------------------------------------------------------------------
@@ -276,7 +320,7 @@ __global__ void k_velocity(VelArgs A, fx_sources src) {
   int64_t i = idx3(x, y, z, sx, sy);
   const float* orig = AXIS == 0 ? A.V.u : (AXIS == 1 ? A.V.v : A.V.w);
   const float offx = AXIS == 0 ? 0.0f : 0.5f, offy = AXIS == 1 ? 0.0f : 0.5f, offz = AXIS == 2 ? 0.0f : 0.5f;
-  float hx = g.ex / g.nx, hy = g.ey / g.ny, hz = g.ez / g.nz;
+  const float hx = g.hx, hy = g.hy, hz = g.hz;
   const float dt = A.dt;
   float value = 0.0f;
   bool blocked = false;
@@ -284,16 +328,17 @@ __global__ void k_velocity(VelArgs A, fx_sources src) {
     int ox = AXIS == 0, oy = AXIS == 1, oz = AXIS == 2;
     blocked = solid_cell(g, A.solid, x, y, z) || solid_cell(g, A.solid, x - ox, y - oy, z - oz);
   }
-  V3 p = v3((x + offx) * g.ex / g.nx, (y + offy) * g.ey / g.ny, (z + offz) * g.ez / g.nz);
+  V3 p = v3((x + offx) * hx, (y + offy) * hy, (z + offz) * hz);
   if (!blocked) {
-    V3 vp = velocity_at(g, A.V, p);
+    V3 vp = velocity_at_face<AXIS>(g, A.V, p, int(i));
     V3 mid = v3(p.x - 0.5f * dt * vp.x, p.y - 0.5f * dt * vp.y, p.z - 0.5f * dt * vp.z);
     if (SOLID) mid = clip_fast(g, A.solid, A.clear, p, mid);
     V3 vm = velocity_at(g, A.V, mid);
     V3 dep = v3(p.x - dt * vm.x, p.y - dt * vm.y, p.z - dt * vm.z);
     if (SOLID) dep = clip_fast(g, A.solid, A.clear, p, dep);
-    float qx = dep.x / hx - offx, qy = dep.y / hy - offy, qz = dep.z / hz - offz;
-    float low = sample_face(orig, sx, sy, sz, qx, qy, qz);
+    float qx = dep.x * g.ix - offx, qy = dep.y * g.iy - offy, qz = dep.z * g.iz - offz;
+    float lo = 0.0f, hi = 0.0f;
+    float low = MC ? sample_face_mm(orig, sx, sy, sz, qx, qy, qz, lo, hi) : sample_face(orig, sx, sy, sz, qx, qy, qz);
     value = low;
     if (MC) {
       float lbx = offx * hx, lby = offy * hy, lbz = offz * hz;
@@ -306,18 +351,9 @@ __global__ void k_velocity(VelArgs A, fx_sources src) {
       ok = ok && !(dep.x < lbx || dep.y < lby || dep.z < lbz || dep.x > hbx || dep.y > hby || dep.z > hbz);
       ok = ok && !(arr.x < lbx || arr.y < lby || arr.z < lbz || arr.x > hbx || arr.y > hby || arr.z > hbz);
       if (ok) {
-        float rev = sample_face(A.pred, sx, sy, sz, arr.x / hx - offx, arr.y / hy - offy, arr.z / hz - offz);
+        // dep lies inside the face lattice here, so the sampled donors are the clamp set
+        float rev = sample_face(A.pred, sx, sy, sz, arr.x * g.ix - offx, arr.y * g.iy - offy, arr.z * g.iz - offz);
         float corr = A.pred[i] + 0.5f * (orig[i] - rev);
-        int bx = int(floorf(qx)), by = int(floorf(qy)), bz = int(floorf(qz));
-        float lo = 3.4e38f, hi = -3.4e38f;
-        for (int dz = 0; dz < 2; ++dz)
-          for (int dy = 0; dy < 2; ++dy)
-            for (int dx = 0; dx < 2; ++dx) {
-              int cx = min(max(bx + dx, 0), sx - 1), cy = min(max(by + dy, 0), sy - 1), cz = min(max(bz + dz, 0), sz - 1);
-              float d = orig[idx3(cx, cy, cz, sx, sy)];
-              lo = fminf(lo, d);
-              hi = fmaxf(hi, d);
-            }
         value = 0.5f * low + 0.5f * fminf(fmaxf(corr, lo), hi);
       }
     }
@@ -348,8 +384,9 @@ __global__ void k_velocity(VelArgs A, fx_sources src) {
 // scalar sampling (closed box: integer indices clamped)
 // ---------------------------------------------------------------------------
 struct Corners {
-  int64_t idx[8];
+  int idx[8];
   float w[8];
+  float norm;  // 1 / sum of fluid weights (solid sampling)
 };
 
 // zero_ext: corners outside the grid contribute 0 (shaders/mac_advect_density.glsl
@@ -363,24 +400,33 @@ __device__ __forceinline__ void closed_corners(const GridDev& g, float qx, float
     int dx = k & 1, dy = (k >> 1) & 1, dz = k >> 2;
     int ux = bx + dx, uy = by + dy, uz = bz + dz;
     int cx = min(max(ux, 0), g.nx - 1), cy = min(max(uy, 0), g.ny - 1), cz = min(max(uz, 0), g.nz - 1);
-    c.idx[k] = idx3(cx, cy, cz, g.nx, g.ny);
+    c.idx[k] = cx + g.nx * (cy + g.ny * cz);
     c.w[k] = (dx ? fx : 1.0f - fx) * (dy ? fy : 1.0f - fy) * (dz ? fz : 1.0f - fz);
     if (zero_ext && (ux != cx || uy != cy || uz != cz)) c.w[k] = 0.0f;
   }
 }
 
+// Solid corners drop out and the rest renormalise (shaders/closed_sample.glsl:9-19):
+// done once per corner set, then every channel is a plain weighted sum.
 template <bool SOLID>
-__device__ __forceinline__ float closed_sample(const float* __restrict__ f, const Corners& c,
-                                               const uint8_t* __restrict__ solid) {
-  float v = 0.0f, t = 0.0f;
+__device__ __forceinline__ void fluid_weights(Corners& c, const uint8_t* __restrict__ solid) {
+  if (!SOLID) return;
+  float t = 0.0f;
 #pragma unroll
   for (int k = 0; k < 8; ++k) {
-    if (SOLID && solid[c.idx[k]]) continue;
-    v += c.w[k] * f[c.idx[k]];
+    if (solid[c.idx[k]]) c.w[k] = 0.0f;
     t += c.w[k];
   }
+  c.norm = t > 1e-8f ? 1.0f / t : 0.0f;
+}
+
+template <bool SOLID>
+__device__ __forceinline__ float closed_sample(const float* __restrict__ f, const Corners& c) {
+  float v = 0.0f;
+#pragma unroll
+  for (int k = 0; k < 8; ++k) v += c.w[k] * f[c.idx[k]];
   if (!SOLID) return v;
-  return t > 1e-8f ? v / t : 0.0f;
+  return c.norm > 0.0f ? v * c.norm : 0.0f;
 }
 
 struct ScalarSet {
@@ -398,7 +444,7 @@ __global__ void k_scalar_predict(GridDev g, Vel V, float dt, int channels, Scala
     for (int k = 0; k < channels; ++k) S_.o[k][i] = 0.0f;
     return;
   }
-  float hx = g.ex / g.nx, hy = g.ey / g.ny, hz = g.ez / g.nz;
+  const float hx = g.hx, hy = g.hy, hz = g.hz;
   V3 p = v3((x + 0.5f) * hx, (y + 0.5f) * hy, (z + 0.5f) * hz);
   V3 vp = velocity_at(g, V, p);
   V3 mid = v3(p.x - 0.5f * dt * vp.x, p.y - 0.5f * dt * vp.y, p.z - 0.5f * dt * vp.z);
@@ -407,8 +453,9 @@ __global__ void k_scalar_predict(GridDev g, Vel V, float dt, int channels, Scala
   V3 dep = v3(p.x - dt * vm.x, p.y - dt * vm.y, p.z - dt * vm.z);
   if (SOLID) dep = clip_fast(g, solid, clear, p, dep);
   Corners c;
-  closed_corners(g, dep.x / hx - 0.5f, dep.y / hy - 0.5f, dep.z / hz - 0.5f, c);
-  for (int k = 0; k < channels; ++k) S_.o[k][i] = closed_sample<SOLID>(S_.f[k], c, solid);
+  closed_corners(g, dep.x * g.ix - 0.5f, dep.y * g.iy - 0.5f, dep.z * g.iz - 0.5f, c);
+  fluid_weights<SOLID>(c, solid);
+  for (int k = 0; k < channels; ++k) S_.o[k][i] = closed_sample<SOLID>(S_.f[k], c);
 }
 
 struct ScalarArgs {
@@ -437,7 +484,7 @@ __global__ void k_scalar_advect(ScalarArgs A, fx_sources src) {
     return;
   }
   const float dt = A.dt;
-  float hx = g.ex / g.nx, hy = g.ey / g.ny, hz = g.ez / g.nz;
+  const float hx = g.hx, hy = g.hy, hz = g.hz;
   V3 p = v3((x + 0.5f) * hx, (y + 0.5f) * hy, (z + 0.5f) * hz);
   V3 vp = velocity_at(g, A.V, p);
   V3 mid = v3(p.x - 0.5f * dt * vp.x, p.y - 0.5f * dt * vp.y, p.z - 0.5f * dt * vp.z);
@@ -446,7 +493,12 @@ __global__ void k_scalar_advect(ScalarArgs A, fx_sources src) {
   V3 dep = v3(p.x - dt * vm.x, p.y - dt * vm.y, p.z - dt * vm.z);
   if (SOLID) dep = clip_fast(g, A.solid, A.clear, p, dep);
   Corners c;
-  closed_corners(g, dep.x / hx - 0.5f, dep.y / hy - 0.5f, dep.z / hz - 0.5f, c, A.zero_ext != 0);
+  closed_corners(g, dep.x * g.ix - 0.5f, dep.y * g.iy - 0.5f, dep.z * g.iz - 0.5f, c, A.zero_ext != 0);
+  // min/max donors: all eight corners of the departure cell, solid or not
+  int donors[8];
+#pragma unroll
+  for (int q = 0; q < 8; ++q) donors[q] = c.idx[q];
+  fluid_weights<SOLID>(c, A.solid);
   bool ok = false;
   Corners ca;
   if (MC) {
@@ -459,20 +511,21 @@ __global__ void k_scalar_advect(ScalarArgs A, fx_sources src) {
     float ux = g.ex - 0.5f * hx, uy = g.ey - 0.5f * hy, uz = g.ez - 0.5f * hz;
     ok = ok && !(dep.x < lx || dep.y < ly || dep.z < lz || dep.x > ux || dep.y > uy || dep.z > uz);
     ok = ok && !(arr.x < lx || arr.y < ly || arr.z < lz || arr.x > ux || arr.y > uy || arr.z > uz);
-    closed_corners(g, arr.x / hx - 0.5f, arr.y / hy - 0.5f, arr.z / hz - 0.5f, ca);
+    closed_corners(g, arr.x * g.ix - 0.5f, arr.y * g.iy - 0.5f, arr.z * g.iz - 0.5f, ca);
+    fluid_weights<SOLID>(ca, A.solid);
   }
   float wsrc[8];
   for (int s = 0; s < src.count; ++s) wsrc[s] = emission_weight(p, src.table + 16 * s);
   for (int k = 0; k < A.channels; ++k) {
     const float* f = A.f[k];
-    float value = closed_sample<SOLID>(f, c, A.solid);
+    float value = closed_sample<SOLID>(f, c);
     if (MC && ok) {
-      float rev = closed_sample<SOLID>(A.pred[k], ca, A.solid);
+      float rev = closed_sample<SOLID>(A.pred[k], ca);
       float corr = A.pred[k][i] + 0.5f * (f[i] - rev);
       float lo = 3.4e38f, hi = -3.4e38f;
 #pragma unroll
       for (int q = 0; q < 8; ++q) {
-        float d = f[c.idx[q]];
+        float d = f[donors[q]];
         lo = fminf(lo, d);
         hi = fmaxf(hi, d);
       }
@@ -544,7 +597,10 @@ __global__ void k_max_abs(MaxArgs A, float* out) {
 
 static const dim3 kBlock(64, 4, 1);
 
-GridDev dev_grid(const fx_grid& g) { return GridDev{g.nx, g.ny, g.nz, g.ex, g.ey, g.ez}; }
+GridDev dev_grid(const fx_grid& g) {
+  return GridDev{g.nx,        g.ny,        g.nz,        g.ex,        g.ey,        g.ez,
+                 g.ex / g.nx, g.ey / g.ny, g.ez / g.nz, g.nx / g.ex, g.ny / g.ey, g.nz / g.ez};
+}
 
 int check_sources(const fx_sources* s) {
   FX_REQUIRE(s, FX_EINVAL, "null source table");

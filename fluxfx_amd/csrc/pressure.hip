@@ -25,6 +25,9 @@ int check_grid(const fx_grid& g) {
   FX_REQUIRE(g.nx >= 1 && g.ny >= 1 && g.nz >= 1 && g.nx <= 4096 && g.ny <= 4096 && g.nz <= 4096, FX_EINVAL,
              "grid dimensions must be in [1, 4096], got (%d, %d, %d)", g.nx, g.ny, g.nz);
   FX_REQUIRE(g.ex > 0 && g.ey > 0 && g.ez > 0, FX_EINVAL, "grid extent must be positive");
+  // kernels index cells and faces with 32-bit offsets
+  FX_REQUIRE(int64_t(g.nx + 1) * (g.ny + 1) * (g.nz + 1) < (int64_t(1) << 31), FX_EINVAL,
+             "grid too large: (nx+1)(ny+1)(nz+1) must stay below 2^31");
   return FX_OK;
 }
 

@@ -549,7 +549,7 @@ int fx_learned_pcg_solve(const fx_unet* net, fx_mg_config* mg, float* x, const f
   fx_solve_info first{};
   if (int e = pcg_loop(L0, FX_F32, rT, &zs, b, v, &lp, ws, &first, precond, s)) return e;
   *info = first;
-  if (first.converged) return FX_OK;
+  if (first.converged || prm->max_iters <= first.iterations) return FX_OK;
   // fallback: V-cycle preconditioned FPCG from the learned iterate
   fx_solve_params fp = *prm;
   fp.max_iters = std::max(0, prm->max_iters - first.iterations);
