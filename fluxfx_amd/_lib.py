@@ -72,7 +72,9 @@ class fx_velocity_args(C.Structure):
                 ("ou", C.c_void_p), ("ov", C.c_void_p), ("ow", C.c_void_p),
                 ("density", C.c_void_p), ("temperature", C.c_void_p),
                 ("thermal_lift", C.c_float), ("density_weight", C.c_float),
-                ("solid", C.c_void_p), ("clearance", C.c_void_p), ("sources", C.POINTER(fx_sources))]
+                ("solid", C.c_void_p), ("clearance", C.c_void_p), ("sources", C.POINTER(fx_sources)),
+                ("lu", C.c_void_p), ("lv", C.c_void_p), ("lw", C.c_void_p),
+                ("hu", C.c_void_p), ("hv", C.c_void_p), ("hw", C.c_void_p)]
 
 
 class fx_scalar_args(C.Structure):
@@ -135,6 +137,7 @@ SIGNATURES = {
     "fx_pcg_solve": ([C.POINTER(fx_mg_config), P, P, P, P, P, P, C.POINTER(fx_solve_params), P,
                       C.POINTER(fx_solve_info), P], C.c_int),
     "fx_face_predict": ([fx_grid, F32, P, P, P, P, P, P, P, P, P], C.c_int),
+    "fx_face_predict_bounds": ([fx_grid, F32, P, P, P, P, P, P, P, P, P], C.c_int),
     "fx_build_clearance": ([fx_grid, P, P, P, P], C.c_int),
     "fx_velocity_advect": ([C.POINTER(fx_velocity_args), P], C.c_int),
     "fx_scalar_predict": ([fx_grid, F32, P, P, P, I32, C.POINTER(P), C.POINTER(P), P, P, P], C.c_int),

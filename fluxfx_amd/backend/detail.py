@@ -19,6 +19,7 @@ class DetailPasses:
         self._g = grid_struct(grid)
         self.predictor = self.reverse = None
         self.face_predictor = None
+        self.face_bounds = None
         self._scalar_pred = []
 
     @property
@@ -26,7 +27,8 @@ class DetailPasses:
         cells = prod(self.grid.shape)
         faces = sum(prod(s) for s in self.grid.face_shapes)
         return (8 * cells if self.predictor is not None else 0) + \
-            (4 * faces if self.face_predictor is not None else 0) + 4 * cells * len(self._scalar_pred)
+            (4 * faces if self.face_predictor is not None else 0) + \
+            (8 * faces if self.face_bounds is not None else 0) + 4 * cells * len(self._scalar_pred)
 
     @property
     def _solid(self):
@@ -71,9 +73,24 @@ class DetailPasses:
                   self.device.stream)
         return self.face_predictor
 
+    def transport_velocity_bounds(self, fields, dt):
+        """Face predictors plus the donor bounds of their departure samples
+        (fx_face_predict_bounds): the fused velocity pass then reuses the
+        predictor as its semi-Lagrangian value instead of tracing again."""
+        if self.face_predictor is None:
+            self.face_predictor = [self.device.texture(s) for s in self.grid.face_shapes]
+        if self.face_bounds is None:
+            self.face_bounds = ([self.device.texture(s, nonnegative=False) for s in self.grid.face_shapes],
+                                [self.device.texture(s, nonnegative=False) for s in self.grid.face_shapes])
+        arr = lambda fs: (C.c_void_p * 3)(*[ptr(f) for f in fs])
+        lo, hi = self.face_bounds
+        _lib.call("fx_face_predict_bounds", self._g, float(dt), *(ptr(f) for f in fields), arr(self.face_predictor),
+                  arr(lo), arr(hi), ptr(self._solid), ptr(self._clearance), self.device.stream)
+        return self.face_predictor, self.face_bounds
+
     def confine(self, fields, outputs, dt, strength, limit):
         raise NotImplementedError("vorticity confinement is not part of this build")
 
     def close(self):
-        self.predictor = self.reverse = self.face_predictor = None
+        self.predictor = self.reverse = self.face_predictor = self.face_bounds = None
         self._scalar_pred = []

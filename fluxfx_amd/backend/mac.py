@@ -144,7 +144,8 @@ class DenseMACAdvection(DenseAdvection):
         return tuple(self.device.read(f, s) for f, s in zip(self._velocity, self.grid.face_shapes))
 
     def _advect_velocity(self, old, out, dt, *, maccormack=False, predictors=None, sources=None,
-                         density=None, temperature=None, lift=0.0, weight=0.0, solid=None, clearance=None):
+                         density=None, temperature=None, lift=0.0, weight=0.0, solid=None, clearance=None,
+                         bounds=None):
         args = _lib.fx_velocity_args()
         args.g = self._g
         args.dt = dt
@@ -159,6 +160,8 @@ class DenseMACAdvection(DenseAdvection):
         args.density_weight = weight
         args.solid = ptr(solid)
         args.clearance = ptr(clearance)
+        if bounds is not None:
+            (args.lu, args.lv, args.lw), (args.hu, args.hv, args.hw) = ([ptr(f) for f in b] for b in bounds)
         src = sources if sources is not None else self._empty_sources
         args.sources = C.pointer(src)
         _lib.call("fx_velocity_advect", C.byref(args), self.device.stream)

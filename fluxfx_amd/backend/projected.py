@@ -5,7 +5,8 @@ constructor (grid, settings, device, projector_class, colliders), reset/upload/
 read, update_settings, measure_projection, step(dt, emission, sources), fault
 flag.  Step order (backend/projected.py:149-202):
 
-  1. face predictors (velocity MacCormack)
+  1. face predictors (velocity MacCormack) with the donor bounds of their
+     departure samples
   2. fused velocity transport + jet relaxation + buoyancy from the
      pre-transport density/temperature            -> provisional U, V, W
   3. projection into the old velocity storage
@@ -168,11 +169,14 @@ class DenseProjectedSmoke(DenseThermalAdvection):
         solid = self.solids.solid if self.solids is not None else None
         clearance = self.solids.clearance if self.solids is not None else None
         velocity_detail = s.velocity_advection == "MACCORMACK"
-        preds = self.detail.transport_velocity(self._velocity, dt) if velocity_detail else None
+        preds = bounds = None
+        if velocity_detail:
+            preds, bounds = self.detail.transport_velocity_bounds(self._velocity, dt)
         provisional = [self._velocity_back[0], self._velocity_back[1], self._buoyancy_scratch]
         self._advect_velocity(self._velocity, provisional, dt, maccormack=velocity_detail, predictors=preds,
                               sources=table, density=self._front, temperature=self._temperature,
-                              lift=s.thermal_lift, weight=s.density_weight, solid=solid, clearance=clearance)
+                              lift=s.thermal_lift, weight=s.density_weight, solid=solid, clearance=clearance,
+                              bounds=bounds)
         # the old velocity is consumed: its storage receives the projected field
         self.projector.project(provisional, self._velocity, dt)
         fields = [self._front, self._temperature]

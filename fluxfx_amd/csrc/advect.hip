@@ -270,8 +270,12 @@ __device__ float emission_weight(V3 p, const float* row) {
 // ---------------------------------------------------------------------------
 // face predictor (transport_face)
 // ---------------------------------------------------------------------------
-template <int AXIS, bool SOLID>
+// BOUNDS: also store the donor min/max of the departure sample when that
+// sample may be MacCormack-corrected (departure inside the face lattice, its
+// halo fluid and the trace to it unclipped), else lo = +inf, hi = -inf
+template <int AXIS, bool SOLID, bool BOUNDS>
 __global__ void k_face_predict(GridDev g, Vel V, float dt, const float* __restrict__ in, float* __restrict__ out,
+                               float* __restrict__ lo_out, float* __restrict__ hi_out,
                                const uint8_t* __restrict__ solid, const uint8_t* __restrict__ clear) {
   const int sx = g.nx + (AXIS == 0), sy = g.ny + (AXIS == 1), sz = g.nz + (AXIS == 2);
   int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y * blockDim.y + threadIdx.y, z = blockIdx.z;
@@ -281,6 +285,10 @@ __global__ void k_face_predict(GridDev g, Vel V, float dt, const float* __restri
     int ox = AXIS == 0, oy = AXIS == 1, oz = AXIS == 2;
     if (solid_cell(g, solid, x, y, z) || solid_cell(g, solid, x - ox, y - oy, z - oz)) {
       out[i] = 0.0f;
+      if (BOUNDS) {
+        lo_out[i] = __builtin_inff();
+        hi_out[i] = -__builtin_inff();
+      }
       return;
     }
   }
@@ -292,7 +300,19 @@ __global__ void k_face_predict(GridDev g, Vel V, float dt, const float* __restri
   V3 vm = velocity_at(g, V, mid);
   V3 dep = v3(p.x - dt * vm.x, p.y - dt * vm.y, p.z - dt * vm.z);
   if (SOLID) dep = clip_fast(g, solid, clear, p, dep);
-  out[i] = sample_face(in, sx, sy, sz, dep.x * g.ix - offx, dep.y * g.iy - offy, dep.z * g.iz - offz);
+  if (!BOUNDS) {
+    out[i] = sample_face(in, sx, sy, sz, dep.x * g.ix - offx, dep.y * g.iy - offy, dep.z * g.iz - offz);
+    return;
+  }
+  float lo, hi;
+  out[i] = sample_face_mm(in, sx, sy, sz, dep.x * g.ix - offx, dep.y * g.iy - offy, dep.z * g.iz - offz, lo, hi);
+  const float lbx = offx * g.hx, lby = offy * g.hy, lbz = offz * g.hz;
+  const float hbx = (float(sx - 1) + offx) * g.hx, hby = (float(sy - 1) + offy) * g.hy,
+              hbz = (float(sz - 1) + offz) * g.hz;
+  bool ok = !(dep.x < lbx || dep.y < lby || dep.z < lbz || dep.x > hbx || dep.y > hby || dep.z > hbz);
+  if (SOLID && ok) ok = !halo_fast(g, solid, clear, dep) && dist3(clip_fast(g, solid, clear, p, dep), dep) < 1e-7f;
+  lo_out[i] = ok ? lo : __builtin_inff();
+  hi_out[i] = ok ? hi : -__builtin_inff();
 }
 
 // ---------------------------------------------------------------------------
@@ -309,9 +329,14 @@ struct VelArgs {
   float lift, weight;
   const uint8_t* solid;
   const uint8_t* clear;
+  const float* lo;  // donor bounds from the predictor pass (BOUNDS variant)
+  const float* hi;
 };
 
-template <int AXIS, bool SOLID, bool MC>
+// BOUNDS (MacCormack only): the predictor pass already sampled the old field at
+// the departure point and stored the donor bounds, so only the forward trace
+// and the reverse sample remain here.
+template <int AXIS, bool SOLID, bool MC, bool BOUNDS = false>
 __global__ void k_velocity(VelArgs A, fx_sources src) {
   const GridDev& g = A.g;
   const int sx = g.nx + (AXIS == 0), sy = g.ny + (AXIS == 1), sz = g.nz + (AXIS == 2);
@@ -329,7 +354,30 @@ __global__ void k_velocity(VelArgs A, fx_sources src) {
     blocked = solid_cell(g, A.solid, x, y, z) || solid_cell(g, A.solid, x - ox, y - oy, z - oz);
   }
   V3 p = v3((x + offx) * hx, (y + offy) * hy, (z + offz) * hz);
-  if (!blocked) {
+  if (BOUNDS && !blocked) {
+    V3 vp = velocity_at_face<AXIS>(g, A.V, p, int(i));
+    const float low = A.pred[i], lo = A.lo[i], hi = A.hi[i];
+    value = low;
+    if (lo <= hi) {
+      const float hbx = (float(sx - 1) + offx) * hx, hby = (float(sy - 1) + offy) * hy,
+                  hbz = (float(sz - 1) + offz) * hz;
+      V3 rmid = v3(p.x + 0.5f * dt * vp.x, p.y + 0.5f * dt * vp.y, p.z + 0.5f * dt * vp.z);
+      V3 va = velocity_at(g, A.V, rmid);
+      V3 arr = v3(p.x + dt * va.x, p.y + dt * va.y, p.z + dt * va.z);
+      bool ok = !(arr.x < offx * hx || arr.y < offy * hy || arr.z < offz * hz || arr.x > hbx || arr.y > hby ||
+                  arr.z > hbz);
+      if (SOLID && ok)
+        ok = !halo_fast(g, A.solid, A.clear, p) && !halo_fast(g, A.solid, A.clear, arr) &&
+             dist3(clip_fast(g, A.solid, A.clear, p, rmid), rmid) < 1e-7f &&
+             dist3(clip_fast(g, A.solid, A.clear, p, arr), arr) < 1e-7f;
+      if (ok) {
+        float rev = sample_face(A.pred, sx, sy, sz, arr.x * g.ix - offx, arr.y * g.iy - offy, arr.z * g.iz - offz);
+        float corr = A.pred[i] + 0.5f * (orig[i] - rev);
+        value = 0.5f * low + 0.5f * fminf(fmaxf(corr, lo), hi);
+      }
+    }
+  }
+  if (!BOUNDS && !blocked) {
     V3 vp = velocity_at_face<AXIS>(g, A.V, p, int(i));
     V3 mid = v3(p.x - 0.5f * dt * vp.x, p.y - 0.5f * dt * vp.y, p.z - 0.5f * dt * vp.z);
     if (SOLID) mid = clip_fast(g, A.solid, A.clear, p, mid);
@@ -357,6 +405,8 @@ __global__ void k_velocity(VelArgs A, fx_sources src) {
         value = 0.5f * low + 0.5f * fminf(fmaxf(corr, lo), hi);
       }
     }
+  }
+  if (!blocked) {
     // jet relaxation: coupling rates summed first (order independent)
     float total = 0.0f, weighted = 0.0f;
     for (int s = 0; s < src.count; ++s) {
@@ -631,16 +681,58 @@ int fx_face_predict(fx_grid g, float dt, const float* u, const float* v, const f
   hipStream_t s = S(stream);
   dim3 gx = grid3(g.nx + 1, g.ny, g.nz, kBlock), gy = grid3(g.nx, g.ny + 1, g.nz, kBlock),
        gz = grid3(g.nx, g.ny, g.nz + 1, kBlock);
+  float* nul = nullptr;
   if (solid) {
-    hipLaunchKernelGGL((k_face_predict<0, true>), gx, kBlock, 0, s, G, V, dt, u, pu, solid, clearance);
-    hipLaunchKernelGGL((k_face_predict<1, true>), gy, kBlock, 0, s, G, V, dt, v, pv, solid, clearance);
-    hipLaunchKernelGGL((k_face_predict<2, true>), gz, kBlock, 0, s, G, V, dt, w, pw, solid, clearance);
+    hipLaunchKernelGGL((k_face_predict<0, true, false>), gx, kBlock, 0, s, G, V, dt, u, pu, nul, nul, solid, clearance);
+    hipLaunchKernelGGL((k_face_predict<1, true, false>), gy, kBlock, 0, s, G, V, dt, v, pv, nul, nul, solid, clearance);
+    hipLaunchKernelGGL((k_face_predict<2, true, false>), gz, kBlock, 0, s, G, V, dt, w, pw, nul, nul, solid, clearance);
   } else {
-    hipLaunchKernelGGL((k_face_predict<0, false>), gx, kBlock, 0, s, G, V, dt, u, pu, solid, clearance);
-    hipLaunchKernelGGL((k_face_predict<1, false>), gy, kBlock, 0, s, G, V, dt, v, pv, solid, clearance);
-    hipLaunchKernelGGL((k_face_predict<2, false>), gz, kBlock, 0, s, G, V, dt, w, pw, solid, clearance);
+    hipLaunchKernelGGL((k_face_predict<0, false, false>), gx, kBlock, 0, s, G, V, dt, u, pu, nul, nul, solid,
+                       clearance);
+    hipLaunchKernelGGL((k_face_predict<1, false, false>), gy, kBlock, 0, s, G, V, dt, v, pv, nul, nul, solid,
+                       clearance);
+    hipLaunchKernelGGL((k_face_predict<2, false, false>), gz, kBlock, 0, s, G, V, dt, w, pw, nul, nul, solid,
+                       clearance);
   }
   FX_LAUNCHED("fx_face_predict");
+  return FX_OK;
+}
+
+int fx_face_predict_bounds(fx_grid g, float dt, const float* u, const float* v, const float* w, float* const* pred,
+                           float* const* lo, float* const* hi, const uint8_t* solid, const uint8_t* clearance,
+                           void* stream) {
+  if (int e = check_grid(g)) return e;
+  FX_REQUIRE(u && v && w && pred && lo && hi, FX_EINVAL, "null face array");
+  const float* ins[3] = {u, v, w};
+  for (int a = 0; a < 3; ++a) {
+    FX_REQUIRE(pred[a] && lo[a] && hi[a], FX_EINVAL, "null face array");
+    for (int b = 0; b < 3; ++b)
+      FX_REQUIRE(pred[a] != ins[b] && lo[a] != ins[b] && hi[a] != ins[b], FX_EINVAL,
+                 "Advection input/output textures must be distinct");
+    FX_REQUIRE(pred[a] != lo[a] && pred[a] != hi[a] && lo[a] != hi[a], FX_EINVAL, "bounds arrays must be distinct");
+  }
+  FX_REQUIRE(std::isfinite(dt), FX_EINVAL, "dt must be finite");
+  GridDev G = dev_grid(g);
+  Vel V{u, v, w};
+  hipStream_t s = S(stream);
+  dim3 gx = grid3(g.nx + 1, g.ny, g.nz, kBlock), gy = grid3(g.nx, g.ny + 1, g.nz, kBlock),
+       gz = grid3(g.nx, g.ny, g.nz + 1, kBlock);
+  if (solid) {
+    hipLaunchKernelGGL((k_face_predict<0, true, true>), gx, kBlock, 0, s, G, V, dt, u, pred[0], lo[0], hi[0], solid,
+                       clearance);
+    hipLaunchKernelGGL((k_face_predict<1, true, true>), gy, kBlock, 0, s, G, V, dt, v, pred[1], lo[1], hi[1], solid,
+                       clearance);
+    hipLaunchKernelGGL((k_face_predict<2, true, true>), gz, kBlock, 0, s, G, V, dt, w, pred[2], lo[2], hi[2], solid,
+                       clearance);
+  } else {
+    hipLaunchKernelGGL((k_face_predict<0, false, true>), gx, kBlock, 0, s, G, V, dt, u, pred[0], lo[0], hi[0], solid,
+                       clearance);
+    hipLaunchKernelGGL((k_face_predict<1, false, true>), gy, kBlock, 0, s, G, V, dt, v, pred[1], lo[1], hi[1], solid,
+                       clearance);
+    hipLaunchKernelGGL((k_face_predict<2, false, true>), gz, kBlock, 0, s, G, V, dt, w, pred[2], lo[2], hi[2], solid,
+                       clearance);
+  }
+  FX_LAUNCHED("fx_face_predict_bounds");
   return FX_OK;
 }
 
@@ -664,15 +756,22 @@ int fx_velocity_advect(const fx_velocity_args* a, void* stream) {
                    grid3(g.nx, g.ny, g.nz + 1, kBlock)};
   const float* preds[3] = {a->pu, a->pv, a->pw};
   float* o3[3] = {a->ou, a->ov, a->ow};
+  const float* los[3] = {a->lu, a->lv, a->lw};
+  const float* his[3] = {a->hu, a->hv, a->hw};
+  const bool bounds = a->maccormack && a->lu && a->lv && a->lw && a->hu && a->hv && a->hw;
   for (int axis = 0; axis < 3; ++axis) {
     VelArgs A{G, V, a->dt, preds[axis], o3[axis], a->density, a->temperature, a->thermal_lift, a->density_weight,
-              a->solid, a->solid ? a->clearance : nullptr};
-#define FX_VEL(AX, SO, MC) hipLaunchKernelGGL((k_velocity<AX, SO, MC>), grids[AX], kBlock, 0, s, A, *a->sources)
-#define FX_VEL_AX(AX)                                             \
-  if (a->solid) {                                                 \
-    if (a->maccormack) FX_VEL(AX, true, true); else FX_VEL(AX, true, false);   \
-  } else {                                                        \
-    if (a->maccormack) FX_VEL(AX, false, true); else FX_VEL(AX, false, false); \
+              a->solid, a->solid ? a->clearance : nullptr, bounds ? los[axis] : nullptr,
+              bounds ? his[axis] : nullptr};
+#define FX_VEL(AX, SO, MC, BO) \
+  hipLaunchKernelGGL((k_velocity<AX, SO, MC, BO>), grids[AX], kBlock, 0, s, A, *a->sources)
+#define FX_VEL_AX(AX)                                                                             \
+  if (a->solid) {                                                                                 \
+    if (bounds) FX_VEL(AX, true, true, true);                                                     \
+    else if (a->maccormack) FX_VEL(AX, true, true, false); else FX_VEL(AX, true, false, false);   \
+  } else {                                                                                        \
+    if (bounds) FX_VEL(AX, false, true, true);                                                    \
+    else if (a->maccormack) FX_VEL(AX, false, true, false); else FX_VEL(AX, false, false, false); \
   }
     if (axis == 0) { FX_VEL_AX(0) } else if (axis == 1) { FX_VEL_AX(1) } else { FX_VEL_AX(2) }
 #undef FX_VEL_AX

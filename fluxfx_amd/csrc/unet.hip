@@ -57,70 +57,96 @@ __device__ __forceinline__ uint4 conv_load_a(const ConvDev& d, const uint16_t* _
 
 __device__ __forceinline__ bf16x8_t as_bf16x8(uint4 u) { return __builtin_bit_cast(bf16x8_t, u); }
 
+// Forward convolution, LDS-tiled: a block owns 16 (x) x 4 (y) x 4 (z) output
+// voxels; wave w owns plane w of the tile as four 16-voxel M tiles (one per
+// row) times all output channels.  Per input-channel chunk (8, 16 or 32
+// channels) the block stages the 18 x 6 x 6 halo tile in LDS (zero outside
+// the grid; upsampling and concatenation resolved while staging), then every
+// K step reads its A operands from LDS at a per-tap offset: no per-lane
+// bounds or index arithmetic in the inner loop.  A voxel occupies 16 bytes
+// more than its channels in LDS, which spreads the 16 lanes of a fragment
+// read over all 64 banks.
+constexpr int kTX = 16, kTY = 4, kTZ = 4;
+constexpr int kHX = kTX + 2, kHY = kTY + 2, kHZ = kTZ + 2;
+constexpr int kHalo = kHX * kHY * kHZ;  // 648 voxels
+constexpr int kChunk = 32;              // input channels per staged chunk
+constexpr int kVoxBytes = kChunk * 2 + 16;
+
 template <int NT>
 __global__ __launch_bounds__(256) void k_conv_fwd(ConvDev d, const uint16_t* __restrict__ in0,
                                                   const uint16_t* __restrict__ in1, const uint16_t* __restrict__ wt,
                                                   const float* __restrict__ bias, uint16_t* __restrict__ out,
-                                                  int nvox) {
+                                                  int tiles_x, int tiles_y) {
+  __shared__ __attribute__((aligned(16))) unsigned char tile[kHalo * kVoxBytes];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int li = lane & 15, lg = lane >> 4;
-  const int m0 = (blockIdx.x * 4 + wave) * 32;
-  if (m0 >= nvox) return;
-  int vx[2], vy[2], vz[2];
-  bool vok[2];
+  const int b = blockIdx.x;
+  const int tx = b % tiles_x, rest = b / tiles_x, ty = rest % tiles_y, tz = rest / tiles_y;
+  const int x0 = tx * kTX, y0 = ty * kTY, z0 = tz * kTZ;
+  floatx4_t acc[kTY][NT];
 #pragma unroll
-  for (int t = 0; t < 2; ++t) {
-    const int v = m0 + 16 * t + li;
-    vok[t] = v < nvox;
-    const int row = v / d.nx;
-    vx[t] = v - row * d.nx;
-    vz[t] = row / d.ny;
-    vy[t] = row - vz[t] * d.ny;
-  }
-  floatx4_t acc[2][NT];
-#pragma unroll
-  for (int t = 0; t < 2; ++t)
+  for (int t = 0; t < kTY; ++t)
 #pragma unroll
     for (int n = 0; n < NT; ++n) acc[t][n] = floatx4_t{0.0f, 0.0f, 0.0f, 0.0f};
-  const int ksteps = d.kp / 32;
-  int tap = (lg * 8) / d.cin, ci0 = lg * 8 - tap * d.cin;
-  const uint16_t* wrow = wt + int64_t(li) * d.kp + lg * 8;
-  for (int ks = 0; ks < ksteps; ++ks) {
-    uint4 a[2] = {make_uint4(0, 0, 0, 0), make_uint4(0, 0, 0, 0)};
-    if (tap < 27) {
-      const int dz = tap / 9 - 1, dy = (tap / 3) % 3 - 1, dx = tap % 3 - 1;
-#pragma unroll
-      for (int t = 0; t < 2; ++t)
-        if (vok[t]) a[t] = conv_load_a(d, in0, in1, vx[t] + dx, vy[t] + dy, vz[t] + dz, ci0);
+  // LDS voxel of output (li, row t, plane wave) at tap (0,0,0): halo coords (li+1, t+1, wave+1)
+  const int lbase = ((wave + 1) * kHY + 1) * kHX + li + 1;
+  for (int cbase = 0; cbase < d.cin; cbase += kChunk) {
+    const int cw = min(kChunk, d.cin - cbase);  // 8, 16 or 32
+    const int groups = cw >> 3;
+    __syncthreads();  // previous chunk fully consumed
+    for (int e = threadIdx.x; e < kHalo * groups; e += 256) {
+      const int hv = e / groups, g = e - hv * groups;
+      const int hx = hv % kHX, hr = hv / kHX, hy = hr % kHY, hz = hr / kHY;
+      const uint4 v = conv_load_a(d, in0, in1, x0 - 1 + hx, y0 - 1 + hy, z0 - 1 + hz, cbase + 8 * g);
+      *reinterpret_cast<uint4*>(tile + hv * kVoxBytes + g * 16) = v;
     }
-    uint4 b[NT];
+    __syncthreads();
+    const int shift = cw == 32 ? 5 : (cw == 16 ? 4 : 3);
+    const int ksteps = (27 * cw + 31) >> 5;
+    for (int ks = 0; ks < ksteps; ++ks) {
+      const int kk = ks * 32 + lg * 8;
+      const int tap = kk >> shift, cg = kk & (cw - 1);
+      uint4 a[kTY];
+      uint4 bw[NT];
+      if (tap < 27) {
+        const int dz = tap / 9 - 1, dy = (tap / 3) % 3 - 1, dx = tap % 3 - 1;
+        const int off = (lbase + (dz * kHY + dy) * kHX + dx) * kVoxBytes + cg * 2;
 #pragma unroll
-    for (int n = 0; n < NT; ++n) b[n] = *reinterpret_cast<const uint4*>(wrow + int64_t(n) * 16 * d.kp + ks * 32);
+        for (int t = 0; t < kTY; ++t) a[t] = *reinterpret_cast<const uint4*>(tile + off + t * kHX * kVoxBytes);
+        const uint16_t* wp = wt + int64_t(li) * d.kp + tap * d.cin + cbase + cg;
 #pragma unroll
-    for (int t = 0; t < 2; ++t)
+        for (int n = 0; n < NT; ++n) bw[n] = *reinterpret_cast<const uint4*>(wp + int64_t(n) * 16 * d.kp);
+      } else {
 #pragma unroll
-      for (int n = 0; n < NT; ++n)
-        acc[t][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(as_bf16x8(a[t]), as_bf16x8(b[n]), acc[t][n], 0, 0, 0);
-    ci0 += 32;
-    while (ci0 >= d.cin) {
-      ci0 -= d.cin;
-      ++tap;
+        for (int t = 0; t < kTY; ++t) a[t] = make_uint4(0, 0, 0, 0);
+#pragma unroll
+        for (int n = 0; n < NT; ++n) bw[n] = make_uint4(0, 0, 0, 0);
+      }
+#pragma unroll
+      for (int t = 0; t < kTY; ++t)
+#pragma unroll
+        for (int n = 0; n < NT; ++n)
+          acc[t][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(as_bf16x8(a[t]), as_bf16x8(bw[n]), acc[t][n], 0, 0, 0);
     }
   }
-  // lane (li, lg) holds rows 4*lg + j of each voxel tile, output channel 16*n + li
+  // lane (li, lg) holds x = x0 + 4*lg + j of row t, output channel 16*n + li
+  const int z = z0 + wave;
+  if (z >= d.nz) return;
 #pragma unroll
-  for (int t = 0; t < 2; ++t) {
+  for (int t = 0; t < kTY; ++t) {
+    const int y = y0 + t;
+    if (y >= d.ny) continue;
 #pragma unroll
     for (int n = 0; n < NT; ++n) {
       const int co = 16 * n + li;
       const float bv = bias ? bias[co] : 0.0f;
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
-        const int v = m0 + 16 * t + 4 * lg + j;
-        if (v < nvox) {
+        const int x = x0 + 4 * lg + j;
+        if (x < d.nx) {
           float r = acc[t][n][j] + bv;
           if (d.relu) r = fmaxf(r, 0.0f);
-          out[int64_t(v) * d.cout + co] = bf16_bits(r);
+          out[((int64_t(z) * d.ny + y) * d.nx + x) * d.cout + co] = bf16_bits(r);
         }
       }
     }
@@ -523,19 +549,19 @@ int conv_forward(const fx_conv_desc* c, const void* in0, const void* in1, const 
   FX_REQUIRE(in0 && wt && out && (c->c1 == 0 || in1), FX_EINVAL, "conv: null array");
   FX_REQUIRE(al16(in0) && al16(wt) && (!in1 || al16(in1)), FX_EINVAL, "conv arrays must be 16-byte aligned");
   ConvDev d = conv_dev(c);
-  const int nvox = c->nx * c->ny * c->nz;
-  dim3 grid((nvox + 127) / 128);
+  const int tiles_x = (c->nx + kTX - 1) / kTX, tiles_y = (c->ny + kTY - 1) / kTY, tiles_z = (c->nz + kTZ - 1) / kTZ;
+  dim3 grid(tiles_x * tiles_y * tiles_z);
   const uint16_t* a0 = (const uint16_t*)in0;
   const uint16_t* a1 = (const uint16_t*)in1;
   const uint16_t* w = (const uint16_t*)wt;
   uint16_t* o = (uint16_t*)out;
   switch (c->cout / 16) {
-    case 1: hipLaunchKernelGGL(k_conv_fwd<1>, grid, dim3(256), 0, s, d, a0, a1, w, bias, o, nvox); break;
-    case 2: hipLaunchKernelGGL(k_conv_fwd<2>, grid, dim3(256), 0, s, d, a0, a1, w, bias, o, nvox); break;
-    case 3: hipLaunchKernelGGL(k_conv_fwd<3>, grid, dim3(256), 0, s, d, a0, a1, w, bias, o, nvox); break;
-    case 4: hipLaunchKernelGGL(k_conv_fwd<4>, grid, dim3(256), 0, s, d, a0, a1, w, bias, o, nvox); break;
-    case 5: hipLaunchKernelGGL(k_conv_fwd<5>, grid, dim3(256), 0, s, d, a0, a1, w, bias, o, nvox); break;
-    default: hipLaunchKernelGGL(k_conv_fwd<6>, grid, dim3(256), 0, s, d, a0, a1, w, bias, o, nvox); break;
+    case 1: hipLaunchKernelGGL(k_conv_fwd<1>, grid, dim3(256), 0, s, d, a0, a1, w, bias, o, tiles_x, tiles_y); break;
+    case 2: hipLaunchKernelGGL(k_conv_fwd<2>, grid, dim3(256), 0, s, d, a0, a1, w, bias, o, tiles_x, tiles_y); break;
+    case 3: hipLaunchKernelGGL(k_conv_fwd<3>, grid, dim3(256), 0, s, d, a0, a1, w, bias, o, tiles_x, tiles_y); break;
+    case 4: hipLaunchKernelGGL(k_conv_fwd<4>, grid, dim3(256), 0, s, d, a0, a1, w, bias, o, tiles_x, tiles_y); break;
+    case 5: hipLaunchKernelGGL(k_conv_fwd<5>, grid, dim3(256), 0, s, d, a0, a1, w, bias, o, tiles_x, tiles_y); break;
+    default: hipLaunchKernelGGL(k_conv_fwd<6>, grid, dim3(256), 0, s, d, a0, a1, w, bias, o, tiles_x, tiles_y); break;
   }
   FX_LAUNCHED("k_conv_fwd");
   return FX_OK;

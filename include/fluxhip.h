@@ -267,6 +267,13 @@ int fx_build_clearance(fx_grid g, const uint8_t* solid, uint8_t* clearance, uint
  * backend/detail.py:27-37, shaders/transport_face.glsl:1-20). */
 int fx_face_predict(fx_grid g, float dt, const float* u, const float* v, const float* w, float* pu,
                     float* pv, float* pw, const uint8_t* solid, const uint8_t* clearance, void* stream);
+/* fx_face_predict plus, per face, the min (lo) and max (hi) of the eight donors
+ * of the departure sample when the departure point may be MacCormack-corrected
+ * (inside the face lattice, 3x3x3 halo fluid, trace unclipped), else
+ * lo = +inf, hi = -inf.  lo/hi: 3 face arrays each. */
+int fx_face_predict_bounds(fx_grid g, float dt, const float* u, const float* v, const float* w, float* const* pred,
+                           float* const* lo, float* const* hi, const uint8_t* solid, const uint8_t* clearance,
+                           void* stream);
 
 typedef struct fx_velocity_args {
   fx_grid g;
@@ -280,6 +287,11 @@ typedef struct fx_velocity_args {
   const uint8_t* solid;
   const uint8_t* clearance;                            /* fx_build_clearance of solid, or NULL */
   const fx_sources* sources;                           /* host pointer */
+  /* optional (maccormack): donor bounds from fx_face_predict_bounds; the
+   * predictor then doubles as the semi-Lagrangian value, so the back-trace
+   * is not repeated.  NULL: full recomputation. */
+  const float* lu; const float* lv; const float* lw;
+  const float* hu; const float* hv; const float* hw;
 } fx_velocity_args;
 
 /* Fused velocity SL/MacCormack transport + jet relaxation + buoyancy on W

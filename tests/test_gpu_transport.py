@@ -59,9 +59,11 @@ def _fx_sources(srcs, multi):
     return _lib.make_sources(rows, multi)
 
 
-@pytest.mark.parametrize("maccormack", [False, True])
+@pytest.mark.parametrize("maccormack,bounds", [(False, False), (True, False), (True, True)])
 @pytest.mark.parametrize("masked", [False, True])
-def test_velocity_advect_matches_oracle(device, maccormack, masked):
+def test_velocity_advect_matches_oracle(device, maccormack, bounds, masked):
+    """bounds: the predictor pass also stores the donor bounds and the fused
+    velocity pass reuses the predictor instead of tracing back again."""
     from fluxfx_amd.backend.detail import DetailPasses
     from fluxfx_amd.backend.collision import SolidMask
     grid, og = _grid((20, 18, 16))
@@ -76,12 +78,16 @@ def test_velocity_advect_matches_oracle(device, maccormack, masked):
     frho, ftemp = _upload_cells(device, grid, rho), _upload_cells(device, grid, temp)
     sv = _solver(device, grid)
     dp = DetailPasses(grid, device, solids)
-    preds = dp.transport_velocity(fields, dt) if maccormack else None
+    fb = None
+    if bounds:
+        preds, fb = dp.transport_velocity_bounds(fields, dt)
+    else:
+        preds = dp.transport_velocity(fields, dt) if maccormack else None
     out = [device.texture(s) for s in grid.face_shapes]
     src = _fx_sources(SOURCES, multi=True)
     sv._advect_velocity(fields, out, dt, maccormack=maccormack, predictors=preds, sources=src, density=frho,
                         temperature=ftemp, lift=0.7, weight=0.3, solid=solids.solid if masked else None,
-                        clearance=solids.clearance if masked else None)
+                        clearance=solids.clearance if masked else None, bounds=fb)
     pred_o = otr.face_predictor(vel, og, dt, solid) if maccormack else None
     if maccormack:
         for g, o in zip(faces_np(og, preds), pred_o):
