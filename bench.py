@@ -93,7 +93,7 @@ def settings_for(n, mode):
         if not WEIGHTS.exists():
             raise NotImplementedError(f"no trained U-Net weights at {WEIGHTS.relative_to(ROOT)}")
         kw.update(pressure_precision="BF16", pressure_krylov="LEARNED", learned_weights=str(WEIGHTS),
-                  learned_iterations=2, learned_sweeps=2)
+                  learned_iterations=1, learned_sweeps=2)
     return PressureSettings(**kw), dt
 
 

@@ -82,7 +82,8 @@ class fx_scalar_args(C.Structure):
                 ("boundary", C.c_int32), ("u", C.c_void_p), ("v", C.c_void_p), ("w", C.c_void_p),
                 ("field", C.c_void_p * 3), ("pred", C.c_void_p * 3), ("out", C.c_void_p * 3),
                 ("decay", C.c_float * 3), ("channel_id", C.c_int32 * 3), ("solid", C.c_void_p),
-                ("clearance", C.c_void_p), ("sources", C.POINTER(fx_sources))]
+                ("clearance", C.c_void_p), ("sources", C.POINTER(fx_sources)),
+                ("lo", C.c_void_p * 3), ("hi", C.c_void_p * 3)]
 
 
 P = C.c_void_p
@@ -141,6 +142,8 @@ SIGNATURES = {
     "fx_build_clearance": ([fx_grid, P, P, P, P], C.c_int),
     "fx_velocity_advect": ([C.POINTER(fx_velocity_args), P], C.c_int),
     "fx_scalar_predict": ([fx_grid, F32, P, P, P, I32, C.POINTER(P), C.POINTER(P), P, P, P], C.c_int),
+    "fx_scalar_predict_bounds": ([fx_grid, F32, P, P, P, I32, C.POINTER(P), C.POINTER(P), C.POINTER(P),
+                                  C.POINTER(P), P, P, P], C.c_int),
     "fx_scalar_advect": ([C.POINTER(fx_scalar_args), P], C.c_int),
     "fx_combustion": ([I64, F32, F32, F32, F32, F32, P, P, P, P, P, P, P, P], C.c_int),
     "fx_max_abs": ([I32, C.POINTER(P), C.POINTER(I64), P, P], C.c_int),

@@ -21,6 +21,7 @@ class DetailPasses:
         self.face_predictor = None
         self.face_bounds = None
         self._scalar_pred = []
+        self._scalar_bounds = []
 
     @property
     def allocated_bytes(self):
@@ -28,7 +29,8 @@ class DetailPasses:
         faces = sum(prod(s) for s in self.grid.face_shapes)
         return (8 * cells if self.predictor is not None else 0) + \
             (4 * faces if self.face_predictor is not None else 0) + \
-            (8 * faces if self.face_bounds is not None else 0) + 4 * cells * len(self._scalar_pred)
+            (8 * faces if self.face_bounds is not None else 0) + 4 * cells * len(self._scalar_pred) + \
+            8 * cells * len(self._scalar_bounds)
 
     @property
     def _solid(self):
@@ -64,6 +66,23 @@ class DetailPasses:
         self._predict(fields, preds, velocities, dt)
         return preds
 
+    def scalar_predictors_bounds(self, fields, velocities, dt):
+        """Fused predictor plus the donor bounds of every channel's departure
+        sample (fx_scalar_predict_bounds), for the fused correct pass."""
+        while len(self._scalar_pred) < len(fields):
+            self._scalar_pred.append(self.device.texture(self.grid.shape))
+        while len(self._scalar_bounds) < len(fields):
+            self._scalar_bounds.append((self.device.texture(self.grid.shape, nonnegative=False),
+                                        self.device.texture(self.grid.shape, nonnegative=False)))
+        n = len(fields)
+        preds = self._scalar_pred[:n]
+        bounds = self._scalar_bounds[:n]
+        arr = lambda fs: (C.c_void_p * n)(*[ptr(f) for f in fs])
+        _lib.call("fx_scalar_predict_bounds", self._g, float(dt), *(ptr(v) for v in velocities), n, arr(fields),
+                  arr(preds), arr([b[0] for b in bounds]), arr([b[1] for b in bounds]), ptr(self._solid),
+                  ptr(self._clearance), self.device.stream)
+        return preds, bounds
+
     def transport_velocity(self, fields, dt):
         """Face predictors of U, V, W from the same velocity."""
         if self.face_predictor is None:
@@ -94,3 +113,4 @@ class DetailPasses:
     def close(self):
         self.predictor = self.reverse = self.face_predictor = self.face_bounds = None
         self._scalar_pred = []
+        self._scalar_bounds = []

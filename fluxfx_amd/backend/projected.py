@@ -187,7 +187,9 @@ class DenseProjectedSmoke(DenseThermalAdvection):
             outs.append(self.combustion.spare)
             decays.append(0.0)
         corrected = s.scalar_advection == "MACCORMACK"
-        preds = self.detail.scalar_predictors(fields, self._velocity, dt) if corrected else None
+        preds = bounds = None
+        if corrected:
+            preds, bounds = self.detail.scalar_predictors_bounds(fields, self._velocity, dt)
         args = _lib.fx_scalar_args()
         args.g = self._g
         args.dt = dt
@@ -201,6 +203,8 @@ class DenseProjectedSmoke(DenseThermalAdvection):
             args.pred[k] = preds[k].ptr if corrected else None
             args.decay[k] = d
             args.channel_id[k] = k
+            if bounds is not None:
+                args.lo[k], args.hi[k] = bounds[k][0].ptr, bounds[k][1].ptr
         args.solid = ptr(solid)
         args.clearance = ptr(clearance)
         args.sources = C.pointer(table)

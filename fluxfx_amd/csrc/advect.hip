@@ -482,16 +482,27 @@ __device__ __forceinline__ float closed_sample(const float* __restrict__ f, cons
 struct ScalarSet {
   const float* f[3];
   float* o[3];
+  float* lo[3];  // donor bounds (BOUNDS variant)
+  float* hi[3];
 };
 
-template <bool SOLID>
+// BOUNDS: also store, per channel, the min/max of the eight donors of the
+// departure sample when the departure point may be MacCormack-corrected, else
+// lo = +inf, hi = -inf
+template <bool SOLID, bool BOUNDS = false>
 __global__ void k_scalar_predict(GridDev g, Vel V, float dt, int channels, ScalarSet S_,
                                  const uint8_t* __restrict__ solid, const uint8_t* __restrict__ clear) {
   int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y * blockDim.y + threadIdx.y, z = blockIdx.z;
   if (x >= g.nx || y >= g.ny) return;
   int64_t i = idx3(x, y, z, g.nx, g.ny);
   if (SOLID && solid[i]) {
-    for (int k = 0; k < channels; ++k) S_.o[k][i] = 0.0f;
+    for (int k = 0; k < channels; ++k) {
+      S_.o[k][i] = 0.0f;
+      if (BOUNDS) {
+        S_.lo[k][i] = __builtin_inff();
+        S_.hi[k][i] = -__builtin_inff();
+      }
+    }
     return;
   }
   const float hx = g.hx, hy = g.hy, hz = g.hz;
@@ -504,8 +515,34 @@ __global__ void k_scalar_predict(GridDev g, Vel V, float dt, int channels, Scala
   if (SOLID) dep = clip_fast(g, solid, clear, p, dep);
   Corners c;
   closed_corners(g, dep.x * g.ix - 0.5f, dep.y * g.iy - 0.5f, dep.z * g.iz - 0.5f, c);
+  int donors[8];
+#pragma unroll
+  for (int q = 0; q < 8; ++q) donors[q] = c.idx[q];
   fluid_weights<SOLID>(c, solid);
-  for (int k = 0; k < channels; ++k) S_.o[k][i] = closed_sample<SOLID>(S_.f[k], c);
+  bool ok = true;
+  if (BOUNDS) {
+    const float lx = 0.5f * hx, ly = 0.5f * hy, lz = 0.5f * hz;
+    const float ux = g.ex - 0.5f * hx, uy = g.ey - 0.5f * hy, uz = g.ez - 0.5f * hz;
+    ok = !(dep.x < lx || dep.y < ly || dep.z < lz || dep.x > ux || dep.y > uy || dep.z > uz);
+    if (SOLID && ok) ok = !halo_fast(g, solid, clear, dep) && dist3(clip_fast(g, solid, clear, p, dep), dep) < 1e-7f;
+  }
+  for (int k = 0; k < channels; ++k) {
+    const float* f = S_.f[k];
+    S_.o[k][i] = closed_sample<SOLID>(f, c);
+    if (BOUNDS) {
+      float lo = __builtin_inff(), hi = -__builtin_inff();
+      if (ok) {
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+          const float d = f[donors[q]];
+          lo = fminf(lo, d);
+          hi = fmaxf(hi, d);
+        }
+      }
+      S_.lo[k][i] = lo;
+      S_.hi[k][i] = hi;
+    }
+  }
 }
 
 struct ScalarArgs {
@@ -521,9 +558,14 @@ struct ScalarArgs {
   int zero_ext;
   const uint8_t* solid;
   const uint8_t* clear;
+  const float* lo[3];  // donor bounds from the predictor pass (BOUNDS variant)
+  const float* hi[3];
 };
 
-template <bool SOLID, bool MC>
+// BOUNDS (MacCormack only): the predictor pass already holds the
+// semi-Lagrangian value and the donor bounds of every channel, so only the
+// forward trace and the reverse samples remain here.
+template <bool SOLID, bool MC, bool BOUNDS = false>
 __global__ void k_scalar_advect(ScalarArgs A, fx_sources src) {
   const GridDev& g = A.g;
   int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y * blockDim.y + threadIdx.y, z = blockIdx.z;
@@ -537,6 +579,50 @@ __global__ void k_scalar_advect(ScalarArgs A, fx_sources src) {
   const float hx = g.hx, hy = g.hy, hz = g.hz;
   V3 p = v3((x + 0.5f) * hx, (y + 0.5f) * hy, (z + 0.5f) * hz);
   V3 vp = velocity_at(g, A.V, p);
+  float wsrc[8];
+  for (int s = 0; s < src.count; ++s) wsrc[s] = emission_weight(p, src.table + 16 * s);
+  if (BOUNDS) {
+    // forward trace and its fluid/clip conditions (the departure ones are in lo <= hi)
+    V3 rmid = v3(p.x + 0.5f * dt * vp.x, p.y + 0.5f * dt * vp.y, p.z + 0.5f * dt * vp.z);
+    V3 va = velocity_at(g, A.V, rmid);
+    V3 arr = v3(p.x + dt * va.x, p.y + dt * va.y, p.z + dt * va.z);
+    const float ux = g.ex - 0.5f * hx, uy = g.ey - 0.5f * hy, uz = g.ez - 0.5f * hz;
+    bool ok_arr = !(arr.x < 0.5f * hx || arr.y < 0.5f * hy || arr.z < 0.5f * hz || arr.x > ux || arr.y > uy ||
+                    arr.z > uz);
+    if (SOLID && ok_arr)
+      ok_arr = !halo_fast(g, A.solid, A.clear, p) && !halo_fast(g, A.solid, A.clear, arr) &&
+               dist3(clip_fast(g, A.solid, A.clear, p, rmid), rmid) < 1e-7f &&
+               dist3(clip_fast(g, A.solid, A.clear, p, arr), arr) < 1e-7f;
+    Corners ca;
+    closed_corners(g, arr.x * g.ix - 0.5f, arr.y * g.iy - 0.5f, arr.z * g.iz - 0.5f, ca);
+    fluid_weights<SOLID>(ca, A.solid);
+    for (int k = 0; k < A.channels; ++k) {
+      const float low = A.pred[k][i], lo = A.lo[k][i], hi = A.hi[k][i];
+      float value = low;
+      if (ok_arr && lo <= hi) {
+        float rev = closed_sample<SOLID>(A.pred[k], ca);
+        float corr = low + 0.5f * (A.f[k][i] - rev);
+        value = fminf(fmaxf(corr, lo), hi);
+      }
+      value *= A.keep[k];
+      int ch = A.chan[k];
+      float target = 0.0f;
+      bool any_target = false;
+      for (int s = 0; s < src.count; ++s) {
+        const float* row = src.table + 16 * s;
+        float rate = ch == 0 ? row[11] : (ch == 1 ? row[12] : row[15]);
+        if (ch == 0 && row[13] > 0.5f) {
+          target = fmaxf(target, rate * wsrc[s]);
+          any_target = true;
+        } else {
+          value += dt * rate * wsrc[s];
+        }
+      }
+      if (ch == 0 && (src.multi || any_target)) value = fmaxf(value, target);
+      A.out[k][i] = value;
+    }
+    return;
+  }
   V3 mid = v3(p.x - 0.5f * dt * vp.x, p.y - 0.5f * dt * vp.y, p.z - 0.5f * dt * vp.z);
   if (SOLID) mid = clip_fast(g, A.solid, A.clear, p, mid);
   V3 vm = velocity_at(g, A.V, mid);
@@ -564,8 +650,6 @@ __global__ void k_scalar_advect(ScalarArgs A, fx_sources src) {
     closed_corners(g, arr.x * g.ix - 0.5f, arr.y * g.iy - 0.5f, arr.z * g.iz - 0.5f, ca);
     fluid_weights<SOLID>(ca, A.solid);
   }
-  float wsrc[8];
-  for (int s = 0; s < src.count; ++s) wsrc[s] = emission_weight(p, src.table + 16 * s);
   for (int k = 0; k < A.channels; ++k) {
     const float* f = A.f[k];
     float value = closed_sample<SOLID>(f, c);
@@ -806,6 +890,36 @@ int fx_scalar_predict(fx_grid g, float dt, const float* u, const float* v, const
   return FX_OK;
 }
 
+int fx_scalar_predict_bounds(fx_grid g, float dt, const float* u, const float* v, const float* w, int32_t channels,
+                             const float* const* fields, float* const* preds, float* const* lo, float* const* hi,
+                             const uint8_t* solid, const uint8_t* clearance, void* stream) {
+  if (int e = check_grid(g)) return e;
+  FX_REQUIRE(channels >= 1 && channels <= 3, FX_EINVAL, "channels must be 1..3");
+  FX_REQUIRE(u && v && w && fields && preds && lo && hi, FX_EINVAL, "null array");
+  ScalarSet set{};
+  for (int k = 0; k < channels; ++k) {
+    FX_REQUIRE(fields[k] && preds[k] && lo[k] && hi[k], FX_EINVAL, "null channel %d", k);
+    for (int j = 0; j < channels; ++j)
+      FX_REQUIRE(preds[k] != fields[j] && lo[k] != fields[j] && hi[k] != fields[j], FX_EINVAL,
+                 "Advection input/output textures must be distinct");
+    FX_REQUIRE(preds[k] != lo[k] && preds[k] != hi[k] && lo[k] != hi[k], FX_EINVAL, "bounds arrays must be distinct");
+    set.f[k] = fields[k];
+    set.o[k] = preds[k];
+    set.lo[k] = lo[k];
+    set.hi[k] = hi[k];
+  }
+  dim3 gr = grid3(g.nx, g.ny, g.nz, kBlock);
+  Vel V{u, v, w};
+  if (solid)
+    hipLaunchKernelGGL((k_scalar_predict<true, true>), gr, kBlock, 0, S(stream), dev_grid(g), V, dt, channels, set,
+                       solid, clearance);
+  else
+    hipLaunchKernelGGL((k_scalar_predict<false, true>), gr, kBlock, 0, S(stream), dev_grid(g), V, dt, channels, set,
+                       solid, (const uint8_t*)nullptr);
+  FX_LAUNCHED("fx_scalar_predict_bounds");
+  return FX_OK;
+}
+
 int fx_scalar_advect(const fx_scalar_args* a, void* stream) {
   FX_REQUIRE(a, FX_EINVAL, "null arguments");
   if (int e = check_grid(a->g)) return e;
@@ -836,13 +950,24 @@ int fx_scalar_advect(const fx_scalar_args* a, void* stream) {
     A.keep[k] = expf(-a->decay[k] * a->dt);
     A.chan[k] = a->channel_id[k];
   }
+  bool bounds = a->maccormack != 0 && a->boundary == 0;
+  for (int k = 0; k < a->channels; ++k) bounds = bounds && a->lo[k] && a->hi[k];
+  for (int k = 0; bounds && k < a->channels; ++k) {
+    A.lo[k] = a->lo[k];
+    A.hi[k] = a->hi[k];
+    for (int j = 0; j < a->channels; ++j)
+      FX_REQUIRE(a->out[j] != a->lo[k] && a->out[j] != a->hi[k], FX_EINVAL,
+                 "Advection input/output textures must be distinct");
+  }
   dim3 gr = grid3(a->g.nx, a->g.ny, a->g.nz, kBlock);
   hipStream_t s = S(stream);
   if (a->solid) {
-    if (a->maccormack) hipLaunchKernelGGL((k_scalar_advect<true, true>), gr, kBlock, 0, s, A, *a->sources);
+    if (bounds) hipLaunchKernelGGL((k_scalar_advect<true, true, true>), gr, kBlock, 0, s, A, *a->sources);
+    else if (a->maccormack) hipLaunchKernelGGL((k_scalar_advect<true, true>), gr, kBlock, 0, s, A, *a->sources);
     else hipLaunchKernelGGL((k_scalar_advect<true, false>), gr, kBlock, 0, s, A, *a->sources);
   } else {
-    if (a->maccormack) hipLaunchKernelGGL((k_scalar_advect<false, true>), gr, kBlock, 0, s, A, *a->sources);
+    if (bounds) hipLaunchKernelGGL((k_scalar_advect<false, true, true>), gr, kBlock, 0, s, A, *a->sources);
+    else if (a->maccormack) hipLaunchKernelGGL((k_scalar_advect<false, true>), gr, kBlock, 0, s, A, *a->sources);
     else hipLaunchKernelGGL((k_scalar_advect<false, false>), gr, kBlock, 0, s, A, *a->sources);
   }
   FX_LAUNCHED("fx_scalar_advect");

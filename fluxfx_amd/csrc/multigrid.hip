@@ -66,79 +66,86 @@ __global__ __launch_bounds__(256) void k_smooth_zm(LevelDev L, const T* __restri
   }
 }
 
-// Quad smoother (level 0 and other large box/flags levels): one lane per four
-// x cells, quads enumerated x-fastest over the level; block ids are spread so
-// each XCD sweeps a contiguous range of rows.
-template <typename T, int OP>
+// Vector smoother (level 0 and other large box/flags levels): one lane per W
+// consecutive x cells (one 16-byte load per operand: W = 4 for fp32, 8 for
+// 16-bit storage), cells enumerated x-fastest over the level; block ids are
+// spread so each XCD sweeps a contiguous range of rows.
+template <typename T, int OP, int W>
 __global__ __launch_bounds__(256) void k_smooth_q(LevelDev L, const T* __restrict__ p, const T* __restrict__ rhs,
                                                   T* __restrict__ out, float inv_dt, float omega, int zero_guess,
                                                   int nq) {
   const int q = xcd_tile(blockIdx.x, gridDim.x) * 256 + threadIdx.x;
   if (q >= nq) return;
-  const QuadPos at = quad_pos(L, q);
-  const QuadStencil st = quad_stencil<OP>(L, at.x, at.y, at.z, at.i, PLoad<T>{p}, !zero_guess);
-  const F4 b = ld4(rhs, at.i);
-  F4 r;
+  const QuadPos at = vec_pos(L, q, W);
+  const VecStencil<W> st = vec_stencil<OP, W>(L, at.x, at.y, at.z, at.i, PLoad<T>{p}, !zero_guess);
+  const FV<W> b = VecIO<T, W>::ld(rhs, at.i);
+  FV<W> r;
 #pragma unroll
-  for (int k = 0; k < 4; ++k) {
+  for (int k = 0; k < W; ++k) {
     r.v[k] = 0.0f;
     if ((st.f[k] & FLAG_FLUID) && st.d[k] > 0.0f) r.v[k] = st.c[k] + omega * (st.s[k] - b.v[k] * inv_dt) / st.d[k];
   }
-  st4(out, at.i, r);
+  VecIO<T, W>::st(out, at.i, r);
 }
 
-// Quad residual + 8-child mean: lane u covers fine x 4u..4u+3 of rows 2Y, 2Y+1
-// and planes 2Z, 2Z+1 and writes coarse cells (2u, Y, Z) and (2u+1, Y, Z).
-template <typename T, int OP>
+// Vector residual + 8-child mean: lane u covers fine x W*u .. W*u+W-1 of rows
+// 2Y, 2Y+1 and planes 2Z, 2Z+1 and writes the W/2 coarse cells above them.
+template <typename T, int OP, int W>
 __global__ __launch_bounds__(256) void k_residual_restrict_q(LevelDev L, int cny, const T* __restrict__ p,
                                                              const T* __restrict__ rhs, T* __restrict__ crhs,
                                                              float inv_dt, int nu) {
   const int u = xcd_tile(blockIdx.x, gridDim.x) * 256 + threadIdx.x;
   if (u >= nu) return;
-  const int un = L.nx >> 2;
+  const int un = L.nx / W;
   const int crow = u / un, ux = u - crow * un;
   const int Z = crow / cny, Y = crow - Z * cny;
-  float a0 = 0.0f, a1 = 0.0f;
+  float acc[W / 2];
 #pragma unroll
+  for (int j = 0; j < W / 2; ++j) acc[j] = 0.0f;
+#pragma unroll 1
   for (int dz = 0; dz < 2; ++dz) {
 #pragma unroll
     for (int dy = 0; dy < 2; ++dy) {
-      const int y = 2 * Y + dy, z = 2 * Z + dz, x0 = 4 * ux;
+      const int y = 2 * Y + dy, z = 2 * Z + dz, x0 = W * ux;
       const int64_t i = idx3(x0, y, z, L.nx, L.ny);
-      const QuadStencil st = quad_stencil<OP>(L, x0, y, z, i, PLoad<T>{p}, true);
-      const F4 b = ld4(rhs, i);
-      float r[4];
+      const VecStencil<W> st = vec_stencil<OP, W>(L, x0, y, z, i, PLoad<T>{p}, true);
+      const FV<W> b = VecIO<T, W>::ld(rhs, i);
+      float r[W];
 #pragma unroll
-      for (int k = 0; k < 4; ++k) r[k] = (st.f[k] & FLAG_FLUID) ? b.v[k] * inv_dt - st.s[k] : 0.0f;
-      a0 += r[0] + r[1];
-      a1 += r[2] + r[3];
+      for (int k = 0; k < W; ++k) r[k] = (st.f[k] & FLAG_FLUID) ? b.v[k] * inv_dt - st.s[k] : 0.0f;
+#pragma unroll
+      for (int j = 0; j < W / 2; ++j) acc[j] += r[2 * j] + r[2 * j + 1];
     }
   }
-  st2(crhs, idx3(2 * ux, Y, Z, 2 * un, cny), a0 * 0.125f, a1 * 0.125f);
+#pragma unroll
+  for (int j = 0; j < W / 2; ++j) acc[j] *= 0.125f;
+  st_coarse<T, W / 2>(crhs, idx3((W / 2) * ux, Y, Z, (W / 2) * un, cny), acc);
 }
 
-// Quad prolongation (in place on p): lane u owns the fine 4x2x2 block above
-// coarse cells (2u, Y, Z), (2u+1, Y, Z) and reads the clamped 4x3x3 coarse
-// window once.  Per fine cell the eight trilinear terms are summed in the
-// same order as k_prolong.  cact: coarse active bytes (renormalised mode on an
-// edge level), else null.
-template <typename T, int FOP, int MODE>
+// Vector prolongation (in place on p): lane u owns the fine W x 2 x 2 block
+// above coarse cells (W/2)u .. (W/2)u + W/2 - 1 of (Y, Z) and reads the clamped
+// (W/2 + 2) x 3 x 3 coarse window once.  Per fine cell the eight trilinear
+// terms are summed in the same order as k_prolong.  cact: coarse active bytes
+// (renormalised mode on an edge level), else null.
+template <typename T, int FOP, int MODE, int W>
 __global__ __launch_bounds__(256) void k_prolong_q(LevelDev F, LevelDev C, const uint8_t* __restrict__ cact,
                                                    T* __restrict__ p, const T* __restrict__ coarse, int nu) {
+  constexpr int CW = W / 2 + 2;
   const int u = xcd_tile(blockIdx.x, gridDim.x) * 256 + threadIdx.x;
   if (u >= nu) return;
-  const int un = F.nx >> 2;
+  const int un = F.nx / W;
   const int crow = u / un, ux = u - crow * un;
   const int Z = crow / C.ny, Y = crow - Z * C.ny;
-  float cv[3][3][4];
-  float av[3][3][4];
+  const int cx0 = (W / 2) * ux;
+  float cv[3][3][CW];
+  float av[3][3][CW];
   if (MODE == FX_PROLONG_CONSTANT) {
 #pragma unroll
-    for (int k = 0; k < 2; ++k) cv[1][1][k] = to_f(coarse[idx3(2 * ux + k, Y, Z, C.nx, C.ny)]);
+    for (int k = 0; k < W / 2; ++k) cv[1][1][k] = to_f(coarse[idx3(cx0 + k, Y, Z, C.nx, C.ny)]);
   } else {
-    int xs[4];
+    int xs[CW];
 #pragma unroll
-    for (int k = 0; k < 4; ++k) xs[k] = min(max(2 * ux - 1 + k, 0), C.nx - 1);
+    for (int k = 0; k < CW; ++k) xs[k] = min(max(cx0 - 1 + k, 0), C.nx - 1);
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
       const int cz = min(max(Z - 1 + a, 0), C.nz - 1);
@@ -147,31 +154,31 @@ __global__ __launch_bounds__(256) void k_prolong_q(LevelDev F, LevelDev C, const
         const int cy = min(max(Y - 1 + b, 0), C.ny - 1);
         const int64_t row = (int64_t(cz) * C.ny + cy) * C.nx;
 #pragma unroll
-        for (int k = 0; k < 4; ++k) {
+        for (int k = 0; k < CW; ++k) {
           cv[a][b][k] = to_f(coarse[row + xs[k]]);
           av[a][b][k] = (MODE == FX_PROLONG_RENORM && cact) ? (cact[row + xs[k]] ? 1.0f : 0.0f) : 1.0f;
         }
       }
     }
   }
-  // window offsets and upper weights of the four fine x positions
-  const int xb[4] = {0, 1, 1, 2};
-  const float fxw[4] = {0.75f, 0.25f, 0.75f, 0.25f};
 #pragma unroll
   for (int dz = 0; dz < 2; ++dz) {
 #pragma unroll
     for (int dy = 0; dy < 2; ++dy) {
-      const int y = 2 * Y + dy, z = 2 * Z + dz, x0 = 4 * ux;
+      const int y = 2 * Y + dy, z = 2 * Z + dz, x0 = W * ux;
       const int64_t i = idx3(x0, y, z, F.nx, F.ny);
-      const uint32_t fw = quad_flags(F, FOP, x0, y, z, i);
-      F4 r = ld4(p, i);
+      uint32_t fl[W];
+      vec_flags<W>(F, FOP, x0, y, z, i, fl);
+      FV<W> r = VecIO<T, W>::ld(p, i);
 #pragma unroll
-      for (int k = 0; k < 4; ++k) {
+      for (int k = 0; k < W; ++k) {
         float corr;
         if (MODE == FX_PROLONG_CONSTANT) {
           corr = cv[1][1][k >> 1];
         } else {
-          const float fz = dz ? 0.25f : 0.75f, fy = dy ? 0.25f : 0.75f, fx = fxw[k];
+          // even fine x: coarse (x/2 - 1, x/2) with upper weight 3/4; odd: (x/2, x/2 + 1) with 1/4
+          const int xb = (k + 1) >> 1;
+          const float fz = dz ? 0.25f : 0.75f, fy = dy ? 0.25f : 0.75f, fx = (k & 1) ? 0.25f : 0.75f;
           float num = 0.0f, den = 0.0f;
 #pragma unroll
           for (int tz = 0; tz < 2; ++tz) {
@@ -182,8 +189,8 @@ __global__ __launch_bounds__(256) void k_prolong_q(LevelDev F, LevelDev C, const
 #pragma unroll
               for (int tx = 0; tx < 2; ++tx) {
                 const float wgt = wz * wy * (tx ? fx : 1.0f - fx);
-                const float a = av[dz + tz][dy + ty][xb[k] + tx];
-                const float v = cv[dz + tz][dy + ty][xb[k] + tx];
+                const float a = av[dz + tz][dy + ty][xb + tx];
+                const float v = cv[dz + tz][dy + ty][xb + tx];
                 if (MODE == FX_PROLONG_RENORM) {
                   if (a == 0.0f) continue;
                   den += wgt;
@@ -195,10 +202,10 @@ __global__ __launch_bounds__(256) void k_prolong_q(LevelDev F, LevelDev C, const
           corr = num;
           if (MODE == FX_PROLONG_RENORM) corr = den > 1e-12f ? num / den : 0.0f;
         }
-        const bool active = ((fw >> (8 * k)) & FLAG_FLUID) != 0;
+        const bool active = (fl[k] & FLAG_FLUID) != 0;
         r.v[k] = (FOP == FX_OP_BOX || active) ? r.v[k] + corr : 0.0f;
       }
-      st4(p, i, r);
+      VecIO<T, W>::st(p, i, r);
     }
   }
 }
@@ -351,15 +358,17 @@ static int launch_smooth(fx_mg_config* c, int l, float inv_dt, bool zero_guess, 
   fx_mg_level& L = c->level[l];
   LevelDev D = dev_level(L);
   const int64_t cells = int64_t(L.nx) * L.ny * L.nz;
-  if (cells >= kQuadMinCells && quad_ok(L, L.p, L.q, L.rhs)) {
-    const int nq = int(cells / 4);
+  const int W = storage_width(c->storage);
+  if (cells >= kQuadMinCells && quad_ok(L, L.p, L.q, L.rhs, W)) {
+    const int nq = int(cells / W);
     dim3 grid((nq + 255) / 256);
     FX_DISPATCH_T(c->storage, {
+      constexpr int WV = vec_width<TV>();
       if (L.op == FX_OP_FLAGS)
-        hipLaunchKernelGGL((k_smooth_q<TV, FX_OP_FLAGS>), grid, dim3(256), 0, s, D, (const TV*)L.p,
+        hipLaunchKernelGGL((k_smooth_q<TV, FX_OP_FLAGS, WV>), grid, dim3(256), 0, s, D, (const TV*)L.p,
                            (const TV*)L.rhs, (TV*)L.q, inv_dt, c->omega, zero_guess ? 1 : 0, nq);
       else
-        hipLaunchKernelGGL((k_smooth_q<TV, FX_OP_BOX>), grid, dim3(256), 0, s, D, (const TV*)L.p,
+        hipLaunchKernelGGL((k_smooth_q<TV, FX_OP_BOX, WV>), grid, dim3(256), 0, s, D, (const TV*)L.p,
                            (const TV*)L.rhs, (TV*)L.q, inv_dt, c->omega, zero_guess ? 1 : 0, nq);
     });
   } else if (L.op != FX_OP_EDGES && cells >= (1 << 18)) {
@@ -401,15 +410,17 @@ static int launch_residual_restrict(fx_mg_config* c, int l, float inv_dt, hipStr
   fx_mg_level& C = c->level[l + 1];
   LevelDev D = dev_level(F);
   const int64_t fcells = int64_t(F.nx) * F.ny * F.nz;
-  if (c->restriction == FX_RESTRICT_MEAN && fcells >= kQuadMinCells && quad_ok(F, F.p, F.rhs, C.rhs)) {
-    const int nu = int(int64_t(F.nx / 4) * C.ny * C.nz);
+  const int W = storage_width(c->storage);
+  if (c->restriction == FX_RESTRICT_MEAN && fcells >= kQuadMinCells && quad_ok(F, F.p, F.rhs, C.rhs, W)) {
+    const int nu = int(int64_t(F.nx / W) * C.ny * C.nz);
     dim3 grid((nu + 255) / 256);
     FX_DISPATCH_T(c->storage, {
+      constexpr int WV = vec_width<TV>();
       if (F.op == FX_OP_FLAGS)
-        hipLaunchKernelGGL((k_residual_restrict_q<TV, FX_OP_FLAGS>), grid, dim3(256), 0, s, D, C.ny,
+        hipLaunchKernelGGL((k_residual_restrict_q<TV, FX_OP_FLAGS, WV>), grid, dim3(256), 0, s, D, C.ny,
                            (const TV*)F.p, (const TV*)F.rhs, (TV*)C.rhs, inv_dt, nu);
       else
-        hipLaunchKernelGGL((k_residual_restrict_q<TV, FX_OP_BOX>), grid, dim3(256), 0, s, D, C.ny,
+        hipLaunchKernelGGL((k_residual_restrict_q<TV, FX_OP_BOX, WV>), grid, dim3(256), 0, s, D, C.ny,
                            (const TV*)F.p, (const TV*)F.rhs, (TV*)C.rhs, inv_dt, nu);
     });
     FX_LAUNCHED("k_residual_restrict_q");
@@ -445,14 +456,17 @@ static int launch_prolong(const fx_mg_config* c, int l, const void* in, const vo
   FX_REQUIRE(F.op != FX_OP_FLAGS || C.op != FX_OP_FLAGS, FX_EINVAL, "flags operators are level-0 only");
   const int64_t fcells = int64_t(F.nx) * F.ny * F.nz;
   const bool coarse_known = mode != FX_PROLONG_RENORM || C.op == FX_OP_BOX || (C.op == FX_OP_EDGES && C.active);
-  if (in == out && fcells >= kQuadMinCells && coarse_known && C.op != FX_OP_FLAGS && quad_ok(F, out)) {
-    const int nu = int(int64_t(F.nx / 4) * C.ny * C.nz);
+  const int W = storage_width(c->storage);
+  if (in == out && fcells >= kQuadMinCells && coarse_known && C.op != FX_OP_FLAGS && quad_ok(F, out, nullptr,
+                                                                                             nullptr, W)) {
+    const int nu = int(int64_t(F.nx / W) * C.ny * C.nz);
     dim3 grid((nu + 255) / 256);
     const uint8_t* cact = C.op == FX_OP_EDGES ? C.active : nullptr;
-#define FX_PROQ(FO, MO)                                                                              \
-  FX_DISPATCH_T(c->storage, {                                                                        \
-    hipLaunchKernelGGL((k_prolong_q<TV, FO, MO>), grid, dim3(256), 0, s, DF, DC, cact, (TV*)out,     \
-                       (const TV*)coarse, nu);                                                       \
+#define FX_PROQ(FO, MO)                                                                                 \
+  FX_DISPATCH_T(c->storage, {                                                                           \
+    constexpr int WV = vec_width<TV>();                                                                 \
+    hipLaunchKernelGGL((k_prolong_q<TV, FO, MO, WV>), grid, dim3(256), 0, s, DF, DC, cact, (TV*)out,    \
+                       (const TV*)coarse, nu);                                                          \
   })
     if (F.op == FX_OP_FLAGS) {
       if (mode == FX_PROLONG_CONSTANT) { FX_PROQ(FX_OP_FLAGS, FX_PROLONG_CONSTANT); }

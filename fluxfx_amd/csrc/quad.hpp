@@ -16,9 +16,19 @@ namespace fx {
 typedef _Float16 half4_t __attribute__((ext_vector_type(4)));
 typedef _Float16 half2_t __attribute__((ext_vector_type(2)));
 
-struct F4 {
-  float v[4];
+template <int W>
+struct FV {
+  float v[W];
 };
+using F4 = FV<4>;
+
+// cells per lane for a 16-byte vector of storage type T
+template <typename T>
+constexpr int vec_width() {
+  return int(16 / sizeof(T));
+}
+
+typedef _Float16 half8_t __attribute__((ext_vector_type(8)));
 
 template <typename T>
 __device__ __forceinline__ F4 ld4(const T* __restrict__ p, int64_t i);
@@ -58,6 +68,62 @@ __device__ __forceinline__ void st4<_Float16>(_Float16* __restrict__ p, int64_t 
   *reinterpret_cast<half4_t*>(p + i) = a;
 }
 
+// generic W-wide loads/stores (W * sizeof(T) is 8 or 16 bytes)
+template <typename T, int W>
+struct VecIO;
+template <>
+struct VecIO<float, 4> {
+  static __device__ __forceinline__ FV<4> ld(const float* __restrict__ p, int64_t i) { return ld4(p, i); }
+  static __device__ __forceinline__ void st(float* __restrict__ p, int64_t i, const FV<4>& v) { st4(p, i, v); }
+};
+template <>
+struct VecIO<bf16, 4> {
+  static __device__ __forceinline__ FV<4> ld(const bf16* __restrict__ p, int64_t i) { return ld4(p, i); }
+  static __device__ __forceinline__ void st(bf16* __restrict__ p, int64_t i, const FV<4>& v) { st4(p, i, v); }
+};
+template <>
+struct VecIO<_Float16, 4> {
+  static __device__ __forceinline__ FV<4> ld(const _Float16* __restrict__ p, int64_t i) { return ld4(p, i); }
+  static __device__ __forceinline__ void st(_Float16* __restrict__ p, int64_t i, const FV<4>& v) { st4(p, i, v); }
+};
+template <>
+struct VecIO<bf16, 8> {
+  static __device__ __forceinline__ FV<8> ld(const bf16* __restrict__ p, int64_t i) {
+    const uint4 a = *reinterpret_cast<const uint4*>(p + i);
+    const uint32_t u[4] = {a.x, a.y, a.z, a.w};
+    FV<8> r;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      r.v[2 * k] = __uint_as_float(u[k] << 16);
+      r.v[2 * k + 1] = __uint_as_float(u[k] & 0xffff0000u);
+    }
+    return r;
+  }
+  static __device__ __forceinline__ void st(bf16* __restrict__ p, int64_t i, const FV<8>& v) {
+    uint32_t u[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+      u[k] = uint32_t(from_f<bf16>(v.v[2 * k]).bits) | (uint32_t(from_f<bf16>(v.v[2 * k + 1]).bits) << 16);
+    *reinterpret_cast<uint4*>(p + i) = make_uint4(u[0], u[1], u[2], u[3]);
+  }
+};
+template <>
+struct VecIO<_Float16, 8> {
+  static __device__ __forceinline__ FV<8> ld(const _Float16* __restrict__ p, int64_t i) {
+    const half8_t a = *reinterpret_cast<const half8_t*>(p + i);
+    FV<8> r;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) r.v[k] = float(a[k]);
+    return r;
+  }
+  static __device__ __forceinline__ void st(_Float16* __restrict__ p, int64_t i, const FV<8>& v) {
+    half8_t a;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) a[k] = _Float16(v.v[k]);
+    *reinterpret_cast<half8_t*>(p + i) = a;
+  }
+};
+
 // two consecutive values at an even index
 template <typename T>
 __device__ __forceinline__ void st2(T* __restrict__ p, int64_t i, float a, float b);
@@ -75,11 +141,24 @@ __device__ __forceinline__ void st2<_Float16>(_Float16* __restrict__ p, int64_t 
   *reinterpret_cast<half2_t*>(p + i) = h;
 }
 
+// `n` (2 or 4) consecutive coarse values at index i (i % n == 0)
+template <typename T, int N>
+__device__ __forceinline__ void st_coarse(T* __restrict__ p, int64_t i, const float* v) {
+  if (N == 2) {
+    st2(p, i, v[0], v[1]);
+  } else {
+    FV<4> f{{v[0], v[1], v[N > 2 ? 2 : 0], v[N > 3 ? 3 : 0]}};
+    st4(p, i, f);
+  }
+}
+
 // Reads of the operand p in storage T.
 template <typename T>
 struct PLoad {
   const T* __restrict__ p;
   __device__ __forceinline__ F4 quad(int64_t j) const { return ld4(p, j); }
+  template <int W>
+  __device__ __forceinline__ FV<W> vec(int64_t j) const { return VecIO<T, W>::ld(p, j); }
   __device__ __forceinline__ float one(int64_t j) const { return to_f(p[j]); }
 };
 
@@ -176,12 +255,92 @@ __device__ __forceinline__ QuadStencil quad_stencil(const LevelDev& L, int x0, i
 // levels below this size keep the one-cell kernels (more lanes in flight)
 constexpr int64_t kQuadMinCells = int64_t(1) << 15;
 
+// W-wide generalisation of QuadStencil / quad_stencil (W = 4 or 8 cells per lane)
+template <int W>
+struct VecStencil {
+  float s[W];
+  float d[W];
+  float c[W];
+  uint32_t f[W];
+};
+
+template <int W>
+__device__ __forceinline__ void vec_flags(const LevelDev& L, int op, int x0, int y, int z, int64_t i, uint32_t* f) {
+  if (op == FX_OP_FLAGS) {
+    const uint8_t* fl = reinterpret_cast<const uint8_t*>(L.op) + i;
+    if (W == 4) {
+      const uint32_t w = *reinterpret_cast<const uint32_t*>(fl);
+#pragma unroll
+      for (int k = 0; k < W; ++k) f[k] = (w >> (8 * (k & 3))) & 0xffu;
+    } else {
+      const uint2 w = *reinterpret_cast<const uint2*>(fl);
+#pragma unroll
+      for (int k = 0; k < W; ++k) f[k] = ((k < 4 ? w.x : w.y) >> (8 * (k & 3))) & 0xffu;
+    }
+  } else {
+#pragma unroll
+    for (int k = 0; k < W; ++k) f[k] = box_flags(x0 + k, y, z, L.nx, L.ny, L.nz);
+  }
+}
+
+template <int OP, int W, class Ld>
+__device__ __forceinline__ VecStencil<W> vec_stencil(const LevelDev& L, int x0, int y, int z, int64_t i, const Ld& ld,
+                                                     bool read_p) {
+  static_assert(OP != FX_OP_EDGES, "vector stencils cover flag and box levels");
+  const int64_t sy = L.nx, sz = int64_t(L.nx) * L.ny;
+  VecStencil<W> st;
+  vec_flags<W>(L, OP, x0, y, z, i, st.f);
+  FV<W> c{}, ym{}, yp{}, zm{}, zp{};
+  float xl = 0.0f, xr = 0.0f;
+  if (read_p) {
+    c = ld.template vec<W>(i);
+    if (y > 0) ym = ld.template vec<W>(i - sy);
+    if (y + 1 < L.ny) yp = ld.template vec<W>(i + sy);
+    if (z > 0) zm = ld.template vec<W>(i - sz);
+    if (z + 1 < L.nz) zp = ld.template vec<W>(i + sz);
+    if (x0 > 0) xl = ld.one(i - 1);
+    if (x0 + W < L.nx) xr = ld.one(i + W);
+  }
+#pragma unroll
+  for (int k = 0; k < W; ++k) {
+    const uint32_t f = st.f[k];
+    const float ck = c.v[k];
+    const float left = k > 0 ? c.v[k > 0 ? k - 1 : 0] : xl;
+    const float right = k < W - 1 ? c.v[k < W - 1 ? k + 1 : W - 1] : xr;
+    float s = 0.0f, d = 0.0f;
+    if (f & FLAG_XLO) { d += L.wx; s += L.wx * (left - ck); }
+    if (f & FLAG_XHI) { d += L.wx; s += L.wx * (right - ck); }
+    if (f & FLAG_YLO) { d += L.wy; s += L.wy * (ym.v[k] - ck); }
+    if (f & FLAG_YHI) { d += L.wy; s += L.wy * (yp.v[k] - ck); }
+    if (f & FLAG_ZLO) { d += L.wz; s += L.wz * (zm.v[k] - ck); }
+    if (f & FLAG_ZHI) { d += L.wz; s += L.wz * (zp.v[k] - ck); }
+    st.s[k] = s;
+    st.d[k] = d;
+    st.c[k] = ck;
+  }
+  return st;
+}
+
+__device__ __forceinline__ QuadPos vec_pos(const LevelDev& L, int q, int W) {
+  const int qx = L.nx / W;
+  const int row = q / qx;
+  QuadPos r;
+  r.x = (q - row * qx) * W;
+  r.z = row / L.ny;
+  r.y = row - r.z * L.ny;
+  r.i = int64_t(row) * L.nx + r.x;
+  return r;
+}
+
 // host: can a level run the quad kernels?
 inline bool quad_ok(const fx_mg_level& L, const void* a = nullptr, const void* b = nullptr,
-                    const void* c = nullptr) {
+                    const void* c = nullptr, int width = 4) {
   auto al = [](const void* p) { return (reinterpret_cast<uintptr_t>(p) & 15u) == 0; };
-  return L.op != FX_OP_EDGES && L.nx % 4 == 0 && (L.op != FX_OP_FLAGS || al(L.op_data)) && al(a) && al(b) &&
+  return L.op != FX_OP_EDGES && L.nx % width == 0 && (L.op != FX_OP_FLAGS || al(L.op_data)) && al(a) && al(b) &&
          al(c);
 }
+
+// cells per lane of the level-0 multigrid kernels for a storage type
+inline int storage_width(int storage) { return storage == FX_F32 ? 4 : 8; }
 
 }  // namespace fx

@@ -305,6 +305,13 @@ int fx_velocity_advect(const fx_velocity_args* a, void* stream);
 int fx_scalar_predict(fx_grid g, float dt, const float* u, const float* v, const float* w, int32_t channels,
                       const float* const* fields, float* const* preds, const uint8_t* solid,
                       const uint8_t* clearance, void* stream);
+/* fx_scalar_predict plus, per channel and cell, the min (lo) and max (hi) of
+ * the eight donors of the departure sample when the departure point may be
+ * MacCormack-corrected (inside [h/2, L - h/2], halo fluid, trace unclipped),
+ * else lo = +inf, hi = -inf. */
+int fx_scalar_predict_bounds(fx_grid g, float dt, const float* u, const float* v, const float* w, int32_t channels,
+                             const float* const* fields, float* const* preds, float* const* lo, float* const* hi,
+                             const uint8_t* solid, const uint8_t* clearance, void* stream);
 
 typedef struct fx_scalar_args {
   fx_grid g;
@@ -321,6 +328,10 @@ typedef struct fx_scalar_args {
   const uint8_t* solid;
   const uint8_t* clearance;        /* fx_build_clearance of solid, or NULL */
   const fx_sources* sources;       /* host pointer */
+  /* optional (maccormack): donor bounds from fx_scalar_predict_bounds; the
+   * predictor then doubles as the semi-Lagrangian value.  NULL: recompute. */
+  const float* lo[3];
+  const float* hi[3];
 } fx_scalar_args;
 
 /* Fused closed-box scalar transport (reverse sampling, MacCormack correction,

@@ -103,10 +103,10 @@ def test_velocity_advect_matches_oracle(device, maccormack, bounds, masked):
         assert np.median(err) < 1e-6 * scale
 
 
-@pytest.mark.parametrize("maccormack", [False, True])
+@pytest.mark.parametrize("maccormack,bounds", [(False, False), (True, False), (True, True)])
 @pytest.mark.parametrize("masked", [False, True])
 @pytest.mark.parametrize("multi", [False, True])
-def test_scalar_advect_matches_oracle(device, maccormack, masked, multi):
+def test_scalar_advect_matches_oracle(device, maccormack, bounds, masked, multi):
     from fluxfx_amd import _lib
     from fluxfx_amd.backend.detail import DetailPasses
     from fluxfx_amd.backend.collision import SolidMask
@@ -127,7 +127,11 @@ def test_scalar_advect_matches_oracle(device, maccormack, masked, multi):
     fin = [_upload_cells(device, grid, c) for c in chans]
     outs = [device.texture(grid.shape) for _ in chans]
     dp = DetailPasses(grid, device, solids)
-    preds = dp.scalar_predictors(fin, fields, dt) if maccormack else None
+    sb = None
+    if bounds:
+        preds, sb = dp.scalar_predictors_bounds(fin, fields, dt)
+    else:
+        preds = dp.scalar_predictors(fin, fields, dt) if maccormack else None
     srcs = SOURCES if multi else SOURCES[:1]
     table = _fx_sources(srcs, multi)
     args = _lib.fx_scalar_args()
@@ -144,6 +148,8 @@ def test_scalar_advect_matches_oracle(device, maccormack, masked, multi):
         args.pred[k] = preds[k].ptr if maccormack else None
         args.decay[k] = decays[k]
         args.channel_id[k] = k
+        if sb is not None:
+            args.lo[k], args.hi[k] = sb[k][0].ptr, sb[k][1].ptr
     args.solid = solids.solid.data_ptr() if masked else None
     args.clearance = solids.clearance.data_ptr() if masked else None
     args.sources = C.pointer(table)
