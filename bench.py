@@ -147,8 +147,20 @@ def smoother_roofline(device, sim, reps=40):
     # algorithmic bytes per sweep: read p (4) + rhs (4) + flags (1), write p (4)
     bytes_per = 13 * n
     gbs = bytes_per / (ms * 1e-3) / 1e9
-    return {"kernel": "k_smooth<float, FX_OP_FLAGS> (level-0 V-cycle smoother)", "avg_ms": ms,
+    return {"kernel": "k_smooth_q<float, FX_OP_FLAGS, 4> (level-0 V-cycle smoother)", "avg_ms": ms,
             "bytes_per_launch": bytes_per, "achieved_gbs": gbs}
+
+
+def measured_traffic(kernel_key="k_smooth_q<float, 1, 4>"):
+    """HBM bytes per launch of the roofline kernel from the committed rocprofv3
+    FETCH_SIZE/WRITE_SIZE summary (None when absent or for another kernel)."""
+    path = ROOT / "profiles" / "step_256_hbm_traffic.md"
+    if not path.exists():
+        return None
+    for line in path.read_text().splitlines():
+        if line.startswith("roofline-traffic:") and kernel_key in line:
+            return float(line.rsplit("|", 1)[1])
+    return None
 
 
 def copy_calibration(device, sim, reps=5):
@@ -245,7 +257,7 @@ def main():
     }
     if roof:
         line["roofline"] = {"bound": "hbm", "achieved": roof["achieved_gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                            "frac": roof["achieved_gbs"] / HBM_PEAK_GBS, "traffic": None,
+                            "frac": roof["achieved_gbs"] / HBM_PEAK_GBS, "traffic": measured_traffic(),
                             "kernel": roof["kernel"], "avg_ms": roof["avg_ms"],
                             "bytes_per_launch": roof["bytes_per_launch"]}
     if not args.no_cpu:
