@@ -11,9 +11,11 @@ axis-aligned box colliders with edges of 8-32 cells.  Fixed dt = 0.5*h/(1 m/s)
 trilinear prolongation, FP64 direct solve on 8^3, warm-started impulse, run to
 ||b - A q|| <= 1e-5 ||b||.
 
-Modes: plain (FP32 stationary V-cycles, the drop-in), mixed (BF16 V-cycles in
-FP32 iterative refinement) and learned (U-Net preconditioned flexible CG with
-MG fallback), whichever are built.  ``value`` is the steps/s of the
+Modes: plain (FP32 stationary V-cycles, the drop-in), mixed (BF16 V-cycles
+preconditioning FP32 flexible CG) and learned (one U-Net preconditioned FPCG
+step, then BF16 V-cycle FPCG), whichever are built.  With the CPU leg the line
+also carries every field's relative error against the FP64 oracle on the same
+scene at 64^3 after 7 steps (``fp64_parity_64``).  ``value`` is the steps/s of the
 learned+mixed mode (of the mixed mode when no trained weights are present);
 every mode's rate and its ratio to plain are in the JSON line.
 
@@ -199,7 +201,46 @@ def cpu_baseline(n_sample=64, steps=6):
     return {"value": cells_per_s / 256 ** 3, "unit": "steps/s (256^3-equivalent)", "cores": 1, "kind": "port",
             "sample": f"same scene at {n_sample}^3, {steps} steps after 1 warm-up, FP64 NumPy oracle; "
                       f"{cells_per_s:.3e} cell-updates/s scaled to 256^3",
-            "cell_updates_per_s": cells_per_s}
+            "cell_updates_per_s": cells_per_s}, state
+
+
+def parity_vs_oracle(device, state, n_sample, steps, modes):
+    """Relative L2 error of every field after `steps` GPU steps of each mode
+    against the FP64 oracle state of the same scene (pressure over fluid
+    cells with the mean removed; divergence as ||after|| / ||before||)."""
+    og_shape = (n_sample,) * 3
+    solid = state.solid
+    fluid = ~solid if solid is not None else np.ones(og_shape, bool)
+
+    def rel(a, b, mask=None):
+        if mask is not None:
+            a, b = a[mask], b[mask]
+        return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+    out = {}
+    for mode in modes:
+        try:
+            sim, dt = make_sim(device, n_sample, mode)
+        except NotImplementedError:
+            continue
+        for _ in range(steps):
+            sim.step(dt)
+        get = lambda f, shape: f.tensor.detach().cpu().double().numpy().reshape(shape)
+        nz = ny = nx = n_sample
+        faces = [(nz, ny, nx + 1), (nz, ny + 1, nx), (nz + 1, ny, nx)]
+        q = get(sim.projector.pressure, og_shape)
+        qo = state.pressure
+        qa, qb = q[fluid] - q[fluid].mean(), qo[fluid] - qo[fluid].mean()
+        before, after = get(sim.projector.before, og_shape), get(sim.projector.after, og_shape)
+        out[mode] = {"density": rel(get(sim.density, og_shape), state.density),
+                     "temperature": rel(get(sim.temperature, og_shape), state.temperature),
+                     **{"uvw"[a]: rel(get(sim._velocity[a], faces[a]), state.velocity[a]) for a in range(3)},
+                     "pressure": rel(qa, qb),
+                     "divergence_after_over_before": float(np.linalg.norm(after) / max(np.linalg.norm(before),
+                                                                                         1e-300))}
+        sim.close()
+        del sim
+    return out
 
 
 def main():
@@ -261,7 +302,10 @@ def main():
                             "kernel": roof["kernel"], "avg_ms": roof["avg_ms"],
                             "bytes_per_launch": roof["bytes_per_launch"]}
     if not args.no_cpu:
-        line["cpu_baseline"] = cpu_baseline()
+        base, state = cpu_baseline()
+        line["cpu_baseline"] = base
+        line["fp64_parity_64"] = parity_vs_oracle(device, state, 64, 7, [m for m in results if "steps_per_s" in
+                                                                          results[m]])
     print(json.dumps(line), flush=True)
 
 

@@ -118,6 +118,66 @@ def test_step_tolerance_with_solids_and_combustion(device):
     assert m["converged"] and m["relative_residual"] <= 1e-5
 
 
+@pytest.mark.parametrize("precision,krylov", [("BF16", "NONE"), ("BF16", "PCG"), ("FP16", "NONE"),
+                                              ("FP16", "PCG"), ("FP32", "PCG")])
+def test_step_mixed_precision_matches_fp64_oracle(device, precision, krylov):
+    """Mixed-precision projection inside full MacCormack steps with a solid:
+    every advected field, the pressure impulse and the projected divergence
+    stay within the FP32 bound of the FP64 oracle's tolerance-MG step."""
+    og = mac.Grid((32, 32, 32))
+    solid = sphere_mask(og, (16, 16, 18), 5)
+    s = _settings(pressure_tolerance=1e-5, mg_pre=2, mg_post=2, mg_coarsest=8, velocity_advection="MACCORMACK",
+                  scalar_advection="MACCORMACK", pressure_precision=precision, pressure_krylov=krylov)
+    sim, st, og = _case(device, (32, 32, 32), s, solid=solid, steps=3)
+    _compare(sim, st, og, 2e-3)
+    fluid = ~solid
+    q = cells_np(og, sim.projector.pressure)
+    qa, qo = q[fluid] - q[fluid].mean(), st.pressure[fluid] - st.pressure[fluid].mean()
+    assert rel_err(qa, qo) < 5e-3
+    m = sim.measure_projection()
+    assert m["converged"] and m["relative_residual"] <= 1e-5
+    after = cells_np(og, sim.projector.after)
+    before = cells_np(og, sim.projector.before)
+    assert np.sqrt((after ** 2).sum()) <= 2e-5 * np.sqrt((before ** 2).sum())
+
+
+def test_config1_plume_64_jacobi_16_steps(device):
+    """The 64^3 closed-box plume (emitter radius 6 cells at cells (32, 32, 10),
+    density and temperature +1 per step, seeded U(-0.05, 0.05) velocity inside
+    the emitter, buoyancy 1.0*T, MacCormack everywhere, Jacobi 128 iterations,
+    dt = 1/128) over 16 steps against the FP64 oracle."""
+    from fluxfx_amd.physics.config import GridSpec
+    from fluxfx_amd.backend.projected import DenseProjectedSmoke
+    n, dt, steps = 64, 1 / 128, 16
+    og = mac.Grid((n, n, n))
+    c = (32 / n, 32 / n, 10 / n)
+    s = _settings(source_center=c, source_radius=6 / n, source_rate=1 / dt, heat_source_rate=1 / dt,
+                  velocity_advection="MACCORMACK", scalar_advection="MACCORMACK", pressure_solver="JACOBI",
+                  pressure_iterations=128, pressure_cold_multiplier=1)
+    rng = np.random.default_rng(1)
+    vel = []
+    for a in range(3):
+        sz, sy, sx = og.face_array_shape(a)
+        z, y, x = np.meshgrid(np.arange(sz), np.arange(sy), np.arange(sx), indexing="ij")
+        off = [.5, .5, .5]
+        off[2 - a] = 0.0
+        d2 = ((x + off[2]) / n - c[0]) ** 2 + ((y + off[1]) / n - c[1]) ** 2 + ((z + off[0]) / n - c[2]) ** 2
+        vel.append(np.where(d2 < (6 / n) ** 2, rng.uniform(-.05, .05, (sz, sy, sx)), 0.0).astype(np.float32)
+                   .astype(np.float64))
+    sim = DenseProjectedSmoke(GridSpec((n, n, n)), s, device)
+    sim.upload_velocity([v.ravel() for v in vel])
+    sim.upload(np.zeros(n ** 3))
+    sim.upload_temperature(np.zeros(n ** 3))
+    state = ostep.State(og, tuple(vel), np.zeros(og.array_shape), np.zeros(og.array_shape))
+    srcs = [otr.Src(c, 6 / n, 1 / dt, 1 / dt, solid_profile=True)]
+    cfg = ostep.PressureConfig("JACOBI", iterations=128, cold_multiplier=1)
+    for _ in range(steps):
+        sim.step(dt)
+        ostep.step(state, s, dt, srcs, cfg)
+    _compare(sim, state, og, 2e-2, rms_tol=1e-3)
+    assert rel_err(cells_np(og, sim.projector.pressure), state.pressure) < 1e-3
+
+
 def test_step_multi_sources_order_independent(device):
     from fluxfx_amd.physics.emission import Source, Emission
     s = _settings(pressure_solver="JACOBI", pressure_iterations=32, pressure_cold_multiplier=1)
