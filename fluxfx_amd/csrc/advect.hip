@@ -90,15 +90,60 @@ __device__ __forceinline__ V3 velocity_at(const GridDev& g, const Vel& V, V3 p) 
             sample_face(V.w, g.nx, g.ny, g.nz + 1, qx - 0.5f, qy - 0.5f, qz));
 }
 
-// velocityAt at face i of axis AXIS: the own component is the face value
-// itself (the trilinear weights at a lattice node), the others are sampled
+// velocityAt at face (x, y, z) of axis AXIS.  On the lattice the trilinear
+// weights collapse: the own component is the face value itself; another
+// component b sits half a cell off along AXIS and along b and on the lattice
+// along the third axis, so it is the mean of four values (indices clamped as
+// the sampler clamps its coordinates).
 template <int AXIS>
-__device__ __forceinline__ V3 velocity_at_face(const GridDev& g, const Vel& V, V3 p, int i) {
-  float qx = p.x * g.ix, qy = p.y * g.iy, qz = p.z * g.iz;
-  float u = AXIS == 0 ? V.u[i] : sample_face(V.u, g.nx + 1, g.ny, g.nz, qx, qy - 0.5f, qz - 0.5f);
-  float v = AXIS == 1 ? V.v[i] : sample_face(V.v, g.nx, g.ny + 1, g.nz, qx - 0.5f, qy, qz - 0.5f);
-  float w = AXIS == 2 ? V.w[i] : sample_face(V.w, g.nx, g.ny, g.nz + 1, qx - 0.5f, qy - 0.5f, qz);
+__device__ __forceinline__ V3 velocity_at_face(const GridDev& g, const Vel& V, int x, int y, int z, int i) {
+  auto four = [&](const float* F, int b) {
+    // F: component b, dims (nx + (b==0), ny + (b==1), nz + (b==2)); lattice coords of this face in F
+    const int sx = g.nx + (b == 0), sy = g.ny + (b == 1), sz = g.nz + (b == 2);
+    int c[3] = {x, y, z};
+    int lo[3], hi[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      const int size = a == 0 ? sx : (a == 1 ? sy : sz);
+      if (a == AXIS) {          // position i_a, F coordinate i_a - 0.5
+        lo[a] = max(c[a] - 1, 0);
+        hi[a] = min(c[a], size - 1);
+      } else if (a == b) {      // position c + 0.5, F coordinate c + 0.5
+        lo[a] = min(c[a], size - 1);
+        hi[a] = min(c[a] + 1, size - 1);
+      } else {                  // position c + 0.5, F coordinate c (exact)
+        lo[a] = hi[a] = min(c[a], size - 1);
+      }
+    }
+    auto at = [&](int ax, int ay, int az) { return F[ax + sx * (ay + sy * az)]; };
+    // the two half-offset axes are AXIS and b
+    if (AXIS == 0 && b == 1)
+      return 0.25f * (at(lo[0], lo[1], lo[2]) + at(hi[0], lo[1], lo[2]) + at(lo[0], hi[1], lo[2]) +
+                      at(hi[0], hi[1], lo[2]));
+    if ((AXIS == 0 && b == 2) || (AXIS == 2 && b == 0))
+      return 0.25f * (at(lo[0], lo[1], lo[2]) + at(hi[0], lo[1], lo[2]) + at(lo[0], lo[1], hi[2]) +
+                      at(hi[0], lo[1], hi[2]));
+    if (AXIS == 1 && b == 0)
+      return 0.25f * (at(lo[0], lo[1], lo[2]) + at(hi[0], lo[1], lo[2]) + at(lo[0], hi[1], lo[2]) +
+                      at(hi[0], hi[1], lo[2]));
+    // (AXIS, b) = (1, 2) or (2, 1): y and z
+    return 0.25f * (at(lo[0], lo[1], lo[2]) + at(lo[0], hi[1], lo[2]) + at(lo[0], lo[1], hi[2]) +
+                    at(lo[0], hi[1], hi[2]));
+  };
+  const float u = AXIS == 0 ? V.u[i] : four(V.u, 0);
+  const float v = AXIS == 1 ? V.v[i] : four(V.v, 1);
+  const float w = AXIS == 2 ? V.w[i] : four(V.w, 2);
   return v3(u, v, w);
+}
+
+// velocityAt at the centre of cell (x, y, z): every component is the mean of
+// the two faces of the cell along its axis
+__device__ __forceinline__ V3 velocity_at_cell(const GridDev& g, const Vel& V, int x, int y, int z) {
+  const int iu = x + (g.nx + 1) * (y + g.ny * z);
+  const int iv = x + g.nx * (y + (g.ny + 1) * z);
+  const int iw = x + g.nx * (y + g.ny * z);
+  return v3(0.5f * (V.u[iu] + V.u[iu + 1]), 0.5f * (V.v[iv] + V.v[iv + g.nx]),
+            0.5f * (V.w[iw] + V.w[iw + g.nx * g.ny]));
 }
 
 // ---------------------------------------------------------------------------
@@ -294,7 +339,7 @@ __global__ void k_face_predict(GridDev g, Vel V, float dt, const float* __restri
   }
   const float offx = AXIS == 0 ? 0.0f : 0.5f, offy = AXIS == 1 ? 0.0f : 0.5f, offz = AXIS == 2 ? 0.0f : 0.5f;
   V3 p = v3((x + offx) * g.hx, (y + offy) * g.hy, (z + offz) * g.hz);
-  V3 vp = velocity_at_face<AXIS>(g, V, p, int(i));
+  V3 vp = velocity_at_face<AXIS>(g, V, x, y, z, int(i));
   V3 mid = v3(p.x - 0.5f * dt * vp.x, p.y - 0.5f * dt * vp.y, p.z - 0.5f * dt * vp.z);
   if (SOLID) mid = clip_fast(g, solid, clear, p, mid);
   V3 vm = velocity_at(g, V, mid);
@@ -355,7 +400,7 @@ __global__ void k_velocity(VelArgs A, fx_sources src) {
   }
   V3 p = v3((x + offx) * hx, (y + offy) * hy, (z + offz) * hz);
   if (BOUNDS && !blocked) {
-    V3 vp = velocity_at_face<AXIS>(g, A.V, p, int(i));
+    V3 vp = velocity_at_face<AXIS>(g, A.V, x, y, z, int(i));
     const float low = A.pred[i], lo = A.lo[i], hi = A.hi[i];
     value = low;
     if (lo <= hi) {
@@ -378,7 +423,7 @@ __global__ void k_velocity(VelArgs A, fx_sources src) {
     }
   }
   if (!BOUNDS && !blocked) {
-    V3 vp = velocity_at_face<AXIS>(g, A.V, p, int(i));
+    V3 vp = velocity_at_face<AXIS>(g, A.V, x, y, z, int(i));
     V3 mid = v3(p.x - 0.5f * dt * vp.x, p.y - 0.5f * dt * vp.y, p.z - 0.5f * dt * vp.z);
     if (SOLID) mid = clip_fast(g, A.solid, A.clear, p, mid);
     V3 vm = velocity_at(g, A.V, mid);
@@ -507,7 +552,7 @@ __global__ void k_scalar_predict(GridDev g, Vel V, float dt, int channels, Scala
   }
   const float hx = g.hx, hy = g.hy, hz = g.hz;
   V3 p = v3((x + 0.5f) * hx, (y + 0.5f) * hy, (z + 0.5f) * hz);
-  V3 vp = velocity_at(g, V, p);
+  V3 vp = velocity_at_cell(g, V, x, y, z);
   V3 mid = v3(p.x - 0.5f * dt * vp.x, p.y - 0.5f * dt * vp.y, p.z - 0.5f * dt * vp.z);
   if (SOLID) mid = clip_fast(g, solid, clear, p, mid);
   V3 vm = velocity_at(g, V, mid);
@@ -578,7 +623,7 @@ __global__ void k_scalar_advect(ScalarArgs A, fx_sources src) {
   const float dt = A.dt;
   const float hx = g.hx, hy = g.hy, hz = g.hz;
   V3 p = v3((x + 0.5f) * hx, (y + 0.5f) * hy, (z + 0.5f) * hz);
-  V3 vp = velocity_at(g, A.V, p);
+  V3 vp = velocity_at_cell(g, A.V, x, y, z);
   float wsrc[8];
   for (int s = 0; s < src.count; ++s) wsrc[s] = emission_weight(p, src.table + 16 * s);
   if (BOUNDS) {
