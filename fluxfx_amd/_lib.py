@@ -10,7 +10,7 @@ LIB_PATH = Path(__file__).resolve().parent / "libfluxhip.so"
 
 FX_OK, FX_EINVAL, FX_EDEVICE, FX_ESTATE = 0, -1, -2, -3
 FX_F32, FX_BF16, FX_F16 = 0, 1, 2
-FX_OP_BOX, FX_OP_FLAGS, FX_OP_EDGES = 0, 1, 2
+FX_OP_BOX, FX_OP_FLAGS, FX_OP_EDGES, FX_OP_PACKED = 0, 1, 2, 3
 FX_RESTRICT_MEAN, FX_RESTRICT_FULL = 0, 1
 FX_PROLONG_TRILINEAR, FX_PROLONG_RENORM, FX_PROLONG_CONSTANT = 0, 1, 2
 MAX_LEVELS = 16

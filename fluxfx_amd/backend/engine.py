@@ -153,6 +153,25 @@ def conductance_hierarchy(shapes, extent, solid):
     return out
 
 
+def pack_conductances(edges, shape, extent, level):
+    """FX_OP_PACKED words of a rediscretised level: k_a = fraction * 4^level
+    (exact: fractions of open fine faces are multiples of 4^-level) in bits
+    [0,10), [10,20), [20,30), activity in bit 30.  None when 10 bits do not
+    hold 4^level."""
+    scale = 4 ** level
+    if scale > 1023:
+        return None
+    words = np.zeros(edges.shape[:3], np.int64)
+    for a in range(3):
+        k = edges[..., a] / (shape[a] / extent[a]) ** 2 * scale
+        ki = np.rint(k).astype(np.int64)
+        if np.abs(k - ki).max() > 1e-6 or ki.min() < 0 or ki.max() > scale:
+            return None
+        words |= ki << (10 * a)
+    words |= (edges[..., 3] > 0).astype(np.int64) << 30
+    return words.astype(np.int32)
+
+
 class MGLevels:
     """Device arrays + fx_mg_config for one V-cycle hierarchy.
 
@@ -192,10 +211,16 @@ class MGLevels:
             op, data = ops[i]
             lv.op = op
             for a in range(3):
-                lv.w[a] = (shape[a] / extent[a]) ** 2
+                # packed levels carry integer fractions k/4^level: fold 4^-level into w
+                lv.w[a] = (shape[a] / extent[a]) ** 2 / (4 ** i if op == _lib.FX_OP_PACKED else 1)
             lv.op_data = data.data_ptr() if data is not None else None
+            act = None
             if op == _lib.FX_OP_EDGES and data is not None:
-                act = (data.view(-1, 4)[:, 3] > 0).to(device.torch.uint8).contiguous()
+                act = (data.view(-1, 4)[:, 3] > 0)
+            elif op == _lib.FX_OP_PACKED and data is not None:
+                act = (data >> 30) & 1
+            if act is not None:
+                act = act.to(device.torch.uint8).contiguous()
                 self._active.append(act)
                 lv.active = act.data_ptr()
             n = prod(shape)

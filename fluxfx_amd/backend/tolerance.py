@@ -25,7 +25,7 @@ import ctypes as C
 import numpy as np
 
 from .. import _lib
-from .engine import (MGLevels, coarsen_to, conductance_hierarchy, edges_matrix, STORAGE)
+from .engine import (MGLevels, coarsen_to, conductance_hierarchy, edges_matrix, pack_conductances, STORAGE)
 from .pressure import PressureProjector
 
 
@@ -59,9 +59,18 @@ class ToleranceProjector(PressureProjector):
         self.levels = [{"shape": sh} for sh in self.shapes]
         if solids is not None:
             coarse_edges = conductance_hierarchy(self.shapes, grid.extent, solids.numpy())
-            self._edges = [self.device.torch.from_numpy(np.ascontiguousarray(e, np.float32).reshape(-1)).to(
-                self.device.device) for e in coarse_edges]
-            ops = [(_lib.FX_OP_FLAGS, solids.flags)] + [(_lib.FX_OP_EDGES, t) for t in self._edges]
+            torch = self.device.torch
+            self._edges = []
+            ops = [(_lib.FX_OP_FLAGS, solids.flags)]
+            for level, (e, shape) in enumerate(zip(coarse_edges, self.shapes[1:]), start=1):
+                words = pack_conductances(e, shape, grid.extent, level)
+                if words is not None:
+                    t = torch.from_numpy(words.reshape(-1)).to(self.device.device)
+                    ops.append((_lib.FX_OP_PACKED, t))
+                else:
+                    t = torch.from_numpy(np.ascontiguousarray(e, np.float32).reshape(-1)).to(self.device.device)
+                    ops.append((_lib.FX_OP_EDGES, t))
+                self._edges.append(t)
             last = coarse_edges[-1]
             self.prolongation = _lib.FX_PROLONG_RENORM
         else:

@@ -22,6 +22,32 @@ struct Stencil {
   bool active;   // fluid / active cell
 };
 
+constexpr bool op_has_edges(int op) { return op == FX_OP_EDGES || op == FX_OP_PACKED; }
+
+// (w to +x, w to +y, w to +z, active) of cell i of an EDGES or PACKED level
+template <int OP>
+__device__ __forceinline__ float4 edge_at(const LevelDev& L, int64_t i) {
+  if (OP == FX_OP_EDGES) return reinterpret_cast<const float4*>(L.op)[i];
+  const uint32_t u = reinterpret_cast<const uint32_t*>(L.op)[i];
+  return make_float4(float(u & 1023u) * L.wx, float((u >> 10) & 1023u) * L.wy, float((u >> 20) & 1023u) * L.wz,
+                     (u >> 30) & 1u ? 1.0f : 0.0f);
+}
+template <int OP>
+__device__ __forceinline__ float edge_x(const LevelDev& L, int64_t i) {
+  if (OP == FX_OP_EDGES) return reinterpret_cast<const float4*>(L.op)[i].x;
+  return float(reinterpret_cast<const uint32_t*>(L.op)[i] & 1023u) * L.wx;
+}
+template <int OP>
+__device__ __forceinline__ float edge_y(const LevelDev& L, int64_t i) {
+  if (OP == FX_OP_EDGES) return reinterpret_cast<const float4*>(L.op)[i].y;
+  return float((reinterpret_cast<const uint32_t*>(L.op)[i] >> 10) & 1023u) * L.wy;
+}
+template <int OP>
+__device__ __forceinline__ float edge_z(const LevelDev& L, int64_t i) {
+  if (OP == FX_OP_EDGES) return reinterpret_cast<const float4*>(L.op)[i].z;
+  return float((reinterpret_cast<const uint32_t*>(L.op)[i] >> 20) & 1023u) * L.wz;
+}
+
 // Gather the 7-point operator at cell i for the given level operator.
 // read_p = false: p is known to be zero (s = c = 0).
 template <int OP, typename T>
@@ -31,12 +57,11 @@ __device__ __forceinline__ Stencil stencil(const LevelDev& L, int x, int y, int 
   Stencil st{0.0f, 0.0f, 0.0f, true};
   const float c0 = read_p ? to_f(p[i]) : 0.0f;
   st.c = c0;
-  if (OP == FX_OP_EDGES) {
-    const float4* e = reinterpret_cast<const float4*>(L.op);
-    float4 c = e[i];
-    float wxl = x > 0 ? e[i - 1].x : 0.0f;
-    float wyl = y > 0 ? e[i - sy].y : 0.0f;
-    float wzl = z > 0 ? e[i - sz].z : 0.0f;
+  if (op_has_edges(OP)) {
+    float4 c = edge_at<OP>(L, i);
+    float wxl = x > 0 ? edge_x<OP>(L, i - 1) : 0.0f;
+    float wyl = y > 0 ? edge_y<OP>(L, i - sy) : 0.0f;
+    float wzl = z > 0 ? edge_z<OP>(L, i - sz) : 0.0f;
     st.d = c.x + c.y + c.z + wxl + wyl + wzl;
     st.active = c.w > 0.0f;
     if (read_p) {
@@ -68,6 +93,7 @@ __device__ __forceinline__ Stencil stencil(const LevelDev& L, int x, int y, int 
 template <int OP>
 __device__ __forceinline__ bool cell_active(const LevelDev& L, int x, int y, int z, int64_t i) {
   if (OP == FX_OP_EDGES) return reinterpret_cast<const float4*>(L.op)[i].w > 0.0f;
+  if (OP == FX_OP_PACKED) return (reinterpret_cast<const uint32_t*>(L.op)[i] >> 30) & 1u;
   if (OP == FX_OP_FLAGS) return reinterpret_cast<const uint8_t*>(L.op)[i] & FLAG_FLUID;
   return true;
 }
@@ -80,7 +106,7 @@ __device__ __forceinline__ float residual_at(const LevelDev& L, int x, int y, in
   int64_t i = idx3(x, y, z, L.nx, L.ny);
   Stencil st = stencil<OP>(L, x, y, z, i, p, true);
   if (!st.active) return 0.0f;
-  if (OP == FX_OP_EDGES && !(st.d > 0.0f)) return 0.0f;
+  if (op_has_edges(OP) && !(st.d > 0.0f)) return 0.0f;
   return to_f(rhs[i]) * inv_dt - st.s;
 }
 
@@ -92,6 +118,7 @@ inline dim3 level_grid(const fx_mg_level& L) { return grid3(L.nx, L.ny, L.nz, kM
     case FX_OP_BOX: { constexpr int OPV = FX_OP_BOX; __VA_ARGS__; break; }  \
     case FX_OP_FLAGS: { constexpr int OPV = FX_OP_FLAGS; __VA_ARGS__; break; } \
     case FX_OP_EDGES: { constexpr int OPV = FX_OP_EDGES; __VA_ARGS__; break; } \
+    case FX_OP_PACKED: { constexpr int OPV = FX_OP_PACKED; __VA_ARGS__; break; } \
     default: set_error("unknown level operator %d", op); return FX_EINVAL; \
   }
 

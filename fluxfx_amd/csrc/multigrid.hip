@@ -371,7 +371,7 @@ static int launch_smooth(fx_mg_config* c, int l, float inv_dt, bool zero_guess, 
         hipLaunchKernelGGL((k_smooth_q<TV, FX_OP_BOX, WV>), grid, dim3(256), 0, s, D, (const TV*)L.p,
                            (const TV*)L.rhs, (TV*)L.q, inv_dt, c->omega, zero_guess ? 1 : 0, nq);
     });
-  } else if (L.op != FX_OP_EDGES && cells >= (1 << 18)) {
+  } else if (!op_has_edges(L.op) && cells >= (1 << 18)) {
     const int zc = 16;
     int tx = (L.nx + 63) / 64, ty = (L.ny + 3) / 4, tz = (L.nz + zc - 1) / zc;
     dim3 grid(tx * ty * tz);
@@ -455,13 +455,13 @@ static int launch_prolong(const fx_mg_config* c, int l, const void* in, const vo
   FX_REQUIRE(mode >= 0 && mode <= 2, FX_EINVAL, "bad prolongation mode");
   FX_REQUIRE(F.op != FX_OP_FLAGS || C.op != FX_OP_FLAGS, FX_EINVAL, "flags operators are level-0 only");
   const int64_t fcells = int64_t(F.nx) * F.ny * F.nz;
-  const bool coarse_known = mode != FX_PROLONG_RENORM || C.op == FX_OP_BOX || (C.op == FX_OP_EDGES && C.active);
+  const bool coarse_known = mode != FX_PROLONG_RENORM || C.op == FX_OP_BOX || (op_has_edges(C.op) && C.active);
   const int W = storage_width(c->storage);
   if (in == out && fcells >= kQuadMinCells && coarse_known && C.op != FX_OP_FLAGS && quad_ok(F, out, nullptr,
                                                                                              nullptr, W)) {
     const int nu = int(int64_t(F.nx / W) * C.ny * C.nz);
     dim3 grid((nu + 255) / 256);
-    const uint8_t* cact = C.op == FX_OP_EDGES ? C.active : nullptr;
+    const uint8_t* cact = op_has_edges(C.op) ? C.active : nullptr;
 #define FX_PROQ(FO, MO)                                                                                 \
   FX_DISPATCH_T(c->storage, {                                                                           \
     constexpr int WV = vec_width<TV>();                                                                 \
@@ -486,25 +486,20 @@ static int launch_prolong(const fx_mg_config* c, int l, const void* in, const vo
     hipLaunchKernelGGL((k_prolong<TV, FO, CO, MO>), level_grid(F), kBlock, 0, s, DF, DC, (const TV*)in, \
                        (const TV*)coarse, (TV*)out);                                                   \
   })
+  // the coarse operator matters only for the activity test of renormalised
+  // prolongation; piecewise-constant prolongation tests the fine cell only
   if (mode == FX_PROLONG_CONSTANT) {
-    if (F.op == FX_OP_EDGES) { FX_PRO(FX_OP_EDGES, FX_OP_EDGES, FX_PROLONG_CONSTANT); }
-    else if (F.op == FX_OP_FLAGS) { FX_PRO(FX_OP_FLAGS, FX_OP_EDGES, FX_PROLONG_CONSTANT); }
-    else { FX_PRO(FX_OP_BOX, FX_OP_BOX, FX_PROLONG_CONSTANT); }
+    FX_DISPATCH_OP(F.op, { FX_PRO(OPV, FX_OP_BOX, FX_PROLONG_CONSTANT); });
   } else if (mode == FX_PROLONG_TRILINEAR) {
-    if (F.op == FX_OP_BOX) { FX_PRO(FX_OP_BOX, FX_OP_BOX, FX_PROLONG_TRILINEAR); }
-    else if (F.op == FX_OP_FLAGS) { FX_PRO(FX_OP_FLAGS, FX_OP_BOX, FX_PROLONG_TRILINEAR); }
-    else { FX_PRO(FX_OP_EDGES, FX_OP_BOX, FX_PROLONG_TRILINEAR); }
+    FX_DISPATCH_OP(F.op, { FX_PRO(OPV, FX_OP_BOX, FX_PROLONG_TRILINEAR); });
   } else {
     FX_REQUIRE(C.op != FX_OP_FLAGS, FX_EINVAL, "renormalised prolongation needs a coarse edge or box level");
-    if (F.op == FX_OP_FLAGS) {
-      if (C.op == FX_OP_EDGES) { FX_PRO(FX_OP_FLAGS, FX_OP_EDGES, FX_PROLONG_RENORM); }
-      else { FX_PRO(FX_OP_FLAGS, FX_OP_BOX, FX_PROLONG_RENORM); }
-    } else if (F.op == FX_OP_EDGES) {
-      if (C.op == FX_OP_EDGES) { FX_PRO(FX_OP_EDGES, FX_OP_EDGES, FX_PROLONG_RENORM); }
-      else { FX_PRO(FX_OP_EDGES, FX_OP_BOX, FX_PROLONG_RENORM); }
+    if (C.op == FX_OP_EDGES) {
+      FX_DISPATCH_OP(F.op, { FX_PRO(OPV, FX_OP_EDGES, FX_PROLONG_RENORM); });
+    } else if (C.op == FX_OP_PACKED) {
+      FX_DISPATCH_OP(F.op, { FX_PRO(OPV, FX_OP_PACKED, FX_PROLONG_RENORM); });
     } else {
-      if (C.op == FX_OP_EDGES) { FX_PRO(FX_OP_BOX, FX_OP_EDGES, FX_PROLONG_RENORM); }
-      else { FX_PRO(FX_OP_BOX, FX_OP_BOX, FX_PROLONG_RENORM); }
+      FX_DISPATCH_OP(F.op, { FX_PRO(OPV, FX_OP_BOX, FX_PROLONG_RENORM); });
     }
   }
 #undef FX_PRO

@@ -216,7 +216,7 @@ __device__ __forceinline__ uint32_t quad_flags(const LevelDev& L, int op, int x0
 template <int OP, class Ld>
 __device__ __forceinline__ QuadStencil quad_stencil(const LevelDev& L, int x0, int y, int z, int64_t i, const Ld& ld,
                                                     bool read_p) {
-  static_assert(OP != FX_OP_EDGES, "quad stencils cover flag and box levels");
+  static_assert(!op_has_edges(OP), "quad stencils cover flag and box levels");
   const int64_t sy = L.nx, sz = int64_t(L.nx) * L.ny;
   const uint32_t fw = quad_flags(L, OP, x0, y, z, i);
   F4 c{}, ym{}, yp{}, zm{}, zp{};
@@ -286,7 +286,7 @@ __device__ __forceinline__ void vec_flags(const LevelDev& L, int op, int x0, int
 template <int OP, int W, class Ld>
 __device__ __forceinline__ VecStencil<W> vec_stencil(const LevelDev& L, int x0, int y, int z, int64_t i, const Ld& ld,
                                                      bool read_p) {
-  static_assert(OP != FX_OP_EDGES, "vector stencils cover flag and box levels");
+  static_assert(!op_has_edges(OP), "vector stencils cover flag and box levels");
   const int64_t sy = L.nx, sz = int64_t(L.nx) * L.ny;
   VecStencil<W> st;
   vec_flags<W>(L, OP, x0, y, z, i, st.f);
@@ -336,7 +336,7 @@ __device__ __forceinline__ QuadPos vec_pos(const LevelDev& L, int q, int W) {
 inline bool quad_ok(const fx_mg_level& L, const void* a = nullptr, const void* b = nullptr,
                     const void* c = nullptr, int width = 4) {
   auto al = [](const void* p) { return (reinterpret_cast<uintptr_t>(p) & 15u) == 0; };
-  return L.op != FX_OP_EDGES && L.nx % width == 0 && (L.op != FX_OP_FLAGS || al(L.op_data)) && al(a) && al(b) &&
+  return !op_has_edges(L.op) && L.nx % width == 0 && (L.op != FX_OP_FLAGS || al(L.op_data)) && al(a) && al(b) &&
          al(c);
 }
 

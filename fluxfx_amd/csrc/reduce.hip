@@ -332,7 +332,7 @@ int fx_mg_solve(fx_mg_config* cfg, const fx_solve_params* prm, double* ws, fx_so
   if (int e = check_params(prm)) return e;
   FX_REQUIRE(cfg->storage == FX_F32, FX_EINVAL, "fx_mg_solve runs on fp32 levels");
   FX_REQUIRE(ws && info, FX_EINVAL, "bad solve arguments");
-  FX_REQUIRE(cfg->level[0].op != FX_OP_EDGES, FX_EINVAL, "level 0 must be a box or flags operator");
+  FX_REQUIRE(!op_has_edges(cfg->level[0].op), FX_EINVAL, "level 0 must be a box or flags operator");
   hipStream_t s = S(stream);
   fx_mg_level& L0 = cfg->level[0];
   const int64_t n = int64_t(L0.nx) * L0.ny * L0.nz;

@@ -29,11 +29,11 @@ __device__ __forceinline__ float apply_shifted(const LevelDev& L, int X, int Y, 
   auto val = [&](int64_t j) { return x[j] + (e ? s * to_f(e[j]) : 0.0f); };
   center = val(i);
   float acc = 0.0f;
-  if (OP == FX_OP_EDGES) {
-    const float4* ed = reinterpret_cast<const float4*>(L.op);
-    float4 c = ed[i];
+  if (op_has_edges(OP)) {
+    float4 c = edge_at<OP>(L, i);
     active = c.w > 0.0f;
-    float wxl = X > 0 ? ed[i - 1].x : 0.0f, wyl = Y > 0 ? ed[i - sy].y : 0.0f, wzl = Z > 0 ? ed[i - sz].z : 0.0f;
+    float wxl = X > 0 ? edge_x<OP>(L, i - 1) : 0.0f, wyl = Y > 0 ? edge_y<OP>(L, i - sy) : 0.0f,
+          wzl = Z > 0 ? edge_z<OP>(L, i - sz) : 0.0f;
     if (c.x != 0.0f) acc += c.x * (val(i + 1) - center);
     if (wxl != 0.0f) acc += wxl * (val(i - 1) - center);
     if (c.y != 0.0f) acc += c.y * (val(i + sy) - center);
@@ -508,7 +508,7 @@ int fx_pcg_solve(fx_mg_config* pcfg, float* x, const float* b, float* r, float* 
   if (int e = check_config(pcfg)) return e;
   if (int e = check_params(prm)) return e;
   FX_REQUIRE(x && b && r && p && Ap && zold && ws && info, FX_EINVAL, "pcg: null array");
-  FX_REQUIRE(pcfg->level[0].op != FX_OP_EDGES, FX_EINVAL, "level 0 must be a box or flags operator");
+  FX_REQUIRE(!op_has_edges(pcfg->level[0].op), FX_EINVAL, "level 0 must be a box or flags operator");
   hipStream_t s = S(stream);
   fx_mg_level& L0 = pcfg->level[0];
   PcgVectors v{x, r, p, Ap, zold};
@@ -526,7 +526,7 @@ int fx_learned_pcg_solve(const fx_unet* net, fx_mg_config* mg, float* x, const f
              "learned pcg: null array");
   FX_REQUIRE(sweeps >= 0 && sweeps % 2 == 0, FX_EINVAL, "learned pcg: smoothing sweeps must be even");
   fx_mg_level& L0 = mg->level[0];
-  FX_REQUIRE(L0.op != FX_OP_EDGES, FX_EINVAL, "level 0 must be a box or flags operator");
+  FX_REQUIRE(!op_has_edges(L0.op), FX_EINVAL, "level 0 must be a box or flags operator");
   FX_REQUIRE(net->nx == L0.nx && net->ny == L0.ny && net->nz == L0.nz, FX_EINVAL, "network grid differs from level 0");
   hipStream_t s = S(stream);
   fx_solve_params lp = *prm;
@@ -570,7 +570,7 @@ int fx_mixed_solve(fx_mg_config* ecfg, float* x, float* x_spare, const float* b,
   if (int e = check_config(ecfg)) return e;
   if (int e = check_params(prm)) return e;
   FX_REQUIRE(x && x_spare && b && ws && info && x_out && x != x_spare, FX_EINVAL, "mixed solve: bad arrays");
-  FX_REQUIRE(ecfg->level[0].op != FX_OP_EDGES, FX_EINVAL, "level 0 must be a box or flags operator");
+  FX_REQUIRE(!op_has_edges(ecfg->level[0].op), FX_EINVAL, "level 0 must be a box or flags operator");
   hipStream_t s = S(stream);
   fx_mg_level& L0 = ecfg->level[0];
   const int64_t n = int64_t(L0.nx) * L0.ny * L0.nz;

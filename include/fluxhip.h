@@ -129,7 +129,13 @@ int fx_combine(const float* a, const float* b, float* out, int64_t n, void* stre
  * multigrid engine (backend/multigrid.py:20-95, backend/solid_multigrid.py:7-76)
  * ------------------------------------------------------------------------ */
 enum { FX_F32 = 0, FX_BF16 = 1, FX_F16 = 2 };
-enum { FX_OP_BOX = 0, FX_OP_FLAGS = 1, FX_OP_EDGES = 2 };
+/* Level operators.  BOX: closed box, no data.  FLAGS: uint8 open-face bits per
+ * cell (bit 2a low face of axis a open, 2a+1 high face, bit 6 fluid).  EDGES:
+ * float4 (wx, wy, wz, active) conductances to the +x/+y/+z neighbour.  PACKED:
+ * uint32 per cell, bits [0,10) kx, [10,20) ky, [20,30) kz, bit 30 active, the
+ * conductance to the +a neighbour being k_a * level.w[a] (rediscretised masked
+ * levels: fractions k/4^l of open fine faces, exact in 10 bits up to l = 5). */
+enum { FX_OP_BOX = 0, FX_OP_FLAGS = 1, FX_OP_EDGES = 2, FX_OP_PACKED = 3 };
 enum { FX_RESTRICT_MEAN = 0, FX_RESTRICT_FULL = 1 };
 enum { FX_PROLONG_TRILINEAR = 0, FX_PROLONG_RENORM = 1, FX_PROLONG_CONSTANT = 2 };
 
@@ -147,7 +153,7 @@ typedef struct fx_mg_level {
   void* p;
   void* q;
   void* rhs;
-  const uint8_t* active;  /* optional, FX_OP_EDGES levels: 1 = active cell (fast renormalised prolongation) */
+  const uint8_t* active;  /* optional, EDGES/PACKED levels: 1 = active cell (fast renormalised prolongation) */
 } fx_mg_level;
 
 typedef struct fx_mg_config {
