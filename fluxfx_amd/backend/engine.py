@@ -172,6 +172,20 @@ def pack_conductances(edges, shape, extent, level):
     return words.astype(np.int32)
 
 
+def activity_bytes(active):
+    """fx_mg_level.active bytes of a level [z,y,x]: bit 0 = active cell, bit 1 =
+    the cell and its 26 neighbours (indices clamped to the grid) are active."""
+    a = np.asarray(active, bool)
+    pad = np.pad(a, 1, mode="edge")
+    nz, ny, nx = a.shape
+    full = np.ones_like(a)
+    for dz in range(3):
+        for dy in range(3):
+            for dx in range(3):
+                full &= pad[dz:dz + nz, dy:dy + ny, dx:dx + nx]
+    return (a.astype(np.uint8) | (full.astype(np.uint8) << 1)).astype(np.uint8)
+
+
 class MGLevels:
     """Device arrays + fx_mg_config for one V-cycle hierarchy.
 
@@ -216,11 +230,12 @@ class MGLevels:
             lv.op_data = data.data_ptr() if data is not None else None
             act = None
             if op == _lib.FX_OP_EDGES and data is not None:
-                act = (data.view(-1, 4)[:, 3] > 0)
+                act = data.view(-1, 4)[:, 3].cpu().numpy() > 0
             elif op == _lib.FX_OP_PACKED and data is not None:
-                act = (data >> 30) & 1
+                act = ((data.cpu().numpy().view(np.uint32) >> 30) & 1) > 0
             if act is not None:
-                act = act.to(device.torch.uint8).contiguous()
+                host = activity_bytes(act.reshape(shape[2], shape[1], shape[0]))
+                act = device.torch.from_numpy(host.ravel()).to(device.device)
                 self._active.append(act)
                 lv.active = act.data_ptr()
             n = prod(shape)

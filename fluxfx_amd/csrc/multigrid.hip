@@ -89,7 +89,8 @@ __global__ __launch_bounds__(256) void k_smooth_q(LevelDev L, const T* __restric
 }
 
 // Vector residual + 8-child mean: lane u covers fine x W*u .. W*u+W-1 of rows
-// 2Y, 2Y+1 and planes 2Z, 2Z+1 and writes the W/2 coarse cells above them.
+// 2Y, 2Y+1 and planes 2Z, 2Z+1 and writes the W/2 coarse cells above them
+// (bit for bit the sums of k_residual_restrict).
 template <typename T, int OP, int W>
 __global__ __launch_bounds__(256) void k_residual_restrict_q(LevelDev L, int cny, const T* __restrict__ p,
                                                              const T* __restrict__ rhs, T* __restrict__ crhs,
@@ -112,9 +113,14 @@ __global__ __launch_bounds__(256) void k_residual_restrict_q(LevelDev L, int cny
       const FV<W> b = VecIO<T, W>::ld(rhs, i);
       float r[W];
 #pragma unroll
-      for (int k = 0; k < W; ++k) r[k] = (st.f[k] & FLAG_FLUID) ? b.v[k] * inv_dt - st.s[k] : 0.0f;
+      for (int k = 0; k < W; ++k)
+        r[k] = ((st.f[k] & FLAG_FLUID) && (!op_has_edges(OP) || st.d[k] > 0.0f)) ? b.v[k] * inv_dt - st.s[k] : 0.0f;
+      // one add per child, in the (dz, dy, dx) order of k_residual_restrict
 #pragma unroll
-      for (int j = 0; j < W / 2; ++j) acc[j] += r[2 * j] + r[2 * j + 1];
+      for (int j = 0; j < W / 2; ++j) {
+        acc[j] += r[2 * j];
+        acc[j] += r[2 * j + 1];
+      }
     }
   }
 #pragma unroll
@@ -125,12 +131,48 @@ __global__ __launch_bounds__(256) void k_residual_restrict_q(LevelDev L, int cny
 // Vector prolongation (in place on p): lane u owns the fine W x 2 x 2 block
 // above coarse cells (W/2)u .. (W/2)u + W/2 - 1 of (Y, Z) and reads the clamped
 // (W/2 + 2) x 3 x 3 coarse window once.  Per fine cell the eight trilinear
-// terms are summed in the same order as k_prolong.  cact: coarse active bytes
-// (renormalised mode on an edge level), else null.
+// terms are summed in the same order as k_prolong.  cact: coarse activity bytes
+// (renormalised mode on an edge or packed level), else null.  Bit 1 of a
+// coarse cell's byte says its whole clamped 3x3x3 neighbourhood is active; when
+// it is set for all of a lane's coarse cells every weight sum is exactly 1 and
+// the lane skips the activity window (same result, bit for bit).
+template <int W, bool RN>
+__device__ __forceinline__ void prolong_row(const float (&cv)[3][3][W / 2 + 2], uint64_t amask, int dz, int dy,
+                                            float (&corr)[W]) {
+  constexpr int CW = W / 2 + 2;
+#pragma unroll
+  for (int k = 0; k < W; ++k) {
+    // even fine x: coarse (x/2 - 1, x/2) with upper weight 3/4; odd: (x/2, x/2 + 1) with 1/4
+    const int xb = (k + 1) >> 1;
+    const float fz = dz ? 0.25f : 0.75f, fy = dy ? 0.25f : 0.75f, fx = (k & 1) ? 0.25f : 0.75f;
+    float num = 0.0f, den = 0.0f;
+#pragma unroll
+    for (int tz = 0; tz < 2; ++tz) {
+      const float wz = tz ? fz : 1.0f - fz;
+#pragma unroll
+      for (int ty = 0; ty < 2; ++ty) {
+        const float wy = ty ? fy : 1.0f - fy;
+#pragma unroll
+        for (int tx = 0; tx < 2; ++tx) {
+          const float wgt = wz * wy * (tx ? fx : 1.0f - fx);
+          const float v = cv[dz + tz][dy + ty][xb + tx];
+          if (RN) {
+            if (!((amask >> (((dz + tz) * 3 + dy + ty) * CW + xb + tx)) & 1u)) continue;
+            den += wgt;
+          }
+          num = __builtin_fmaf(wgt, v, num);
+        }
+      }
+    }
+    corr[k] = RN ? (den > 1e-12f ? num / den : 0.0f) : num;
+  }
+}
+
 template <typename T, int FOP, int MODE, int W>
 __global__ __launch_bounds__(256) void k_prolong_q(LevelDev F, LevelDev C, const uint8_t* __restrict__ cact,
                                                    T* __restrict__ p, const T* __restrict__ coarse, int nu) {
   constexpr int CW = W / 2 + 2;
+  static_assert(9 * CW <= 64, "activity window must fit one 64-bit mask");
   const int u = xcd_tile(blockIdx.x, gridDim.x) * 256 + threadIdx.x;
   if (u >= nu) return;
   const int un = F.nx / W;
@@ -138,7 +180,8 @@ __global__ __launch_bounds__(256) void k_prolong_q(LevelDev F, LevelDev C, const
   const int Z = crow / C.ny, Y = crow - Z * C.ny;
   const int cx0 = (W / 2) * ux;
   float cv[3][3][CW];
-  float av[3][3][CW];
+  uint64_t amask = ~uint64_t(0);
+  bool renorm = false;
   if (MODE == FX_PROLONG_CONSTANT) {
 #pragma unroll
     for (int k = 0; k < W / 2; ++k) cv[1][1][k] = to_f(coarse[idx3(cx0 + k, Y, Z, C.nx, C.ny)]);
@@ -154,9 +197,28 @@ __global__ __launch_bounds__(256) void k_prolong_q(LevelDev F, LevelDev C, const
         const int cy = min(max(Y - 1 + b, 0), C.ny - 1);
         const int64_t row = (int64_t(cz) * C.ny + cy) * C.nx;
 #pragma unroll
-        for (int k = 0; k < CW; ++k) {
-          cv[a][b][k] = to_f(coarse[row + xs[k]]);
-          av[a][b][k] = (MODE == FX_PROLONG_RENORM && cact) ? (cact[row + xs[k]] ? 1.0f : 0.0f) : 1.0f;
+        for (int k = 0; k < CW; ++k) cv[a][b][k] = to_f(coarse[row + xs[k]]);
+      }
+    }
+    if (MODE == FX_PROLONG_RENORM && cact) {
+      const int64_t own = idx3(cx0, Y, Z, C.nx, C.ny);
+      uint32_t all = 2u;
+#pragma unroll
+      for (int k = 0; k < W / 2; ++k) all &= cact[own + k];
+      renorm = all == 0u;
+    }
+    if (renorm) {
+      amask = 0;
+#pragma unroll
+      for (int a = 0; a < 3; ++a) {
+        const int cz = min(max(Z - 1 + a, 0), C.nz - 1);
+#pragma unroll
+        for (int b = 0; b < 3; ++b) {
+          const int cy = min(max(Y - 1 + b, 0), C.ny - 1);
+          const int64_t row = (int64_t(cz) * C.ny + cy) * C.nx;
+#pragma unroll
+          for (int k = 0; k < CW; ++k)
+            if (cact[row + xs[k]]) amask |= uint64_t(1) << ((a * 3 + b) * CW + k);
         }
       }
     }
@@ -170,40 +232,19 @@ __global__ __launch_bounds__(256) void k_prolong_q(LevelDev F, LevelDev C, const
       uint32_t fl[W];
       vec_flags<W>(F, FOP, x0, y, z, i, fl);
       FV<W> r = VecIO<T, W>::ld(p, i);
+      float corr[W];
+      if (MODE == FX_PROLONG_CONSTANT) {
+#pragma unroll
+        for (int k = 0; k < W; ++k) corr[k] = cv[1][1][k >> 1];
+      } else if (MODE == FX_PROLONG_RENORM && renorm) {
+        prolong_row<W, true>(cv, amask, dz, dy, corr);
+      } else {
+        prolong_row<W, false>(cv, amask, dz, dy, corr);
+      }
 #pragma unroll
       for (int k = 0; k < W; ++k) {
-        float corr;
-        if (MODE == FX_PROLONG_CONSTANT) {
-          corr = cv[1][1][k >> 1];
-        } else {
-          // even fine x: coarse (x/2 - 1, x/2) with upper weight 3/4; odd: (x/2, x/2 + 1) with 1/4
-          const int xb = (k + 1) >> 1;
-          const float fz = dz ? 0.25f : 0.75f, fy = dy ? 0.25f : 0.75f, fx = (k & 1) ? 0.25f : 0.75f;
-          float num = 0.0f, den = 0.0f;
-#pragma unroll
-          for (int tz = 0; tz < 2; ++tz) {
-            const float wz = tz ? fz : 1.0f - fz;
-#pragma unroll
-            for (int ty = 0; ty < 2; ++ty) {
-              const float wy = ty ? fy : 1.0f - fy;
-#pragma unroll
-              for (int tx = 0; tx < 2; ++tx) {
-                const float wgt = wz * wy * (tx ? fx : 1.0f - fx);
-                const float a = av[dz + tz][dy + ty][xb + tx];
-                const float v = cv[dz + tz][dy + ty][xb + tx];
-                if (MODE == FX_PROLONG_RENORM) {
-                  if (a == 0.0f) continue;
-                  den += wgt;
-                }
-                num += wgt * v;
-              }
-            }
-          }
-          corr = num;
-          if (MODE == FX_PROLONG_RENORM) corr = den > 1e-12f ? num / den : 0.0f;
-        }
         const bool active = (fl[k] & FLAG_FLUID) != 0;
-        r.v[k] = (FOP == FX_OP_BOX || active) ? r.v[k] + corr : 0.0f;
+        r.v[k] = (FOP == FX_OP_BOX || active) ? r.v[k] + corr[k] : 0.0f;
       }
       VecIO<T, W>::st(p, i, r);
     }
@@ -268,7 +309,10 @@ __global__ void k_restrict_full(int cnx, int cny, int cnz, int nx, int ny, int n
   coarse[idx3(X, Y, Z, cnx, cny)] = from_f<T>(acc);
 }
 
-// prolongation; out may alias in (each fine cell reads only its own value)
+// prolongation; out may alias in (each fine cell reads only its own value).
+// The trilinear terms are explicit fused multiply-adds in a fixed order, so the
+// vector kernel (whose products the compiler could otherwise share between
+// cells and leave unfused) gives the same bits.
 template <typename T, int FOP, int COP, int MODE>
 __global__ void k_prolong(LevelDev F, LevelDev C, const T* in, const T* __restrict__ coarse, T* out) {
   int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y * blockDim.y + threadIdx.y, z = blockIdx.z;
@@ -301,7 +345,7 @@ __global__ void k_prolong(LevelDev F, LevelDev C, const T* in, const T* __restri
           if (!cell_active<COP>(C, cx, cy, cz, ci)) continue;
           den += wgt;
         }
-        num += wgt * to_f(coarse[ci]);
+        num = __builtin_fmaf(wgt, to_f(coarse[ci]), num);
       }
     }
   }
@@ -359,18 +403,19 @@ static int launch_smooth(fx_mg_config* c, int l, float inv_dt, bool zero_guess, 
   LevelDev D = dev_level(L);
   const int64_t cells = int64_t(L.nx) * L.ny * L.nz;
   const int W = storage_width(c->storage);
-  if (cells >= kQuadMinCells && quad_ok(L, L.p, L.q, L.rhs, W)) {
+  if (cells >= kQuadMinCells && vec_ok(L, L.p, L.q, L.rhs, W)) {
     const int nq = int(cells / W);
     dim3 grid((nq + 255) / 256);
+#define FX_SMQ(OPC)                                                                                      \
+  hipLaunchKernelGGL((k_smooth_q<TV, OPC, WV>), grid, dim3(256), 0, s, D, (const TV*)L.p, (const TV*)L.rhs, \
+                     (TV*)L.q, inv_dt, c->omega, zero_guess ? 1 : 0, nq)
     FX_DISPATCH_T(c->storage, {
       constexpr int WV = vec_width<TV>();
-      if (L.op == FX_OP_FLAGS)
-        hipLaunchKernelGGL((k_smooth_q<TV, FX_OP_FLAGS, WV>), grid, dim3(256), 0, s, D, (const TV*)L.p,
-                           (const TV*)L.rhs, (TV*)L.q, inv_dt, c->omega, zero_guess ? 1 : 0, nq);
-      else
-        hipLaunchKernelGGL((k_smooth_q<TV, FX_OP_BOX, WV>), grid, dim3(256), 0, s, D, (const TV*)L.p,
-                           (const TV*)L.rhs, (TV*)L.q, inv_dt, c->omega, zero_guess ? 1 : 0, nq);
+      if (L.op == FX_OP_FLAGS) FX_SMQ(FX_OP_FLAGS);
+      else if (L.op == FX_OP_PACKED) FX_SMQ(FX_OP_PACKED);
+      else FX_SMQ(FX_OP_BOX);
     });
+#undef FX_SMQ
   } else if (!op_has_edges(L.op) && cells >= (1 << 18)) {
     const int zc = 16;
     int tx = (L.nx + 63) / 64, ty = (L.ny + 3) / 4, tz = (L.nz + zc - 1) / zc;
@@ -411,18 +456,19 @@ static int launch_residual_restrict(fx_mg_config* c, int l, float inv_dt, hipStr
   LevelDev D = dev_level(F);
   const int64_t fcells = int64_t(F.nx) * F.ny * F.nz;
   const int W = storage_width(c->storage);
-  if (c->restriction == FX_RESTRICT_MEAN && fcells >= kQuadMinCells && quad_ok(F, F.p, F.rhs, C.rhs, W)) {
+  if (c->restriction == FX_RESTRICT_MEAN && fcells >= kQuadMinCells && vec_ok(F, F.p, F.rhs, C.rhs, W)) {
     const int nu = int(int64_t(F.nx / W) * C.ny * C.nz);
     dim3 grid((nu + 255) / 256);
+#define FX_RRQ(OPC)                                                                                         \
+  hipLaunchKernelGGL((k_residual_restrict_q<TV, OPC, WV>), grid, dim3(256), 0, s, D, C.ny, (const TV*)F.p, \
+                     (const TV*)F.rhs, (TV*)C.rhs, inv_dt, nu)
     FX_DISPATCH_T(c->storage, {
       constexpr int WV = vec_width<TV>();
-      if (F.op == FX_OP_FLAGS)
-        hipLaunchKernelGGL((k_residual_restrict_q<TV, FX_OP_FLAGS, WV>), grid, dim3(256), 0, s, D, C.ny,
-                           (const TV*)F.p, (const TV*)F.rhs, (TV*)C.rhs, inv_dt, nu);
-      else
-        hipLaunchKernelGGL((k_residual_restrict_q<TV, FX_OP_BOX, WV>), grid, dim3(256), 0, s, D, C.ny,
-                           (const TV*)F.p, (const TV*)F.rhs, (TV*)C.rhs, inv_dt, nu);
+      if (F.op == FX_OP_FLAGS) FX_RRQ(FX_OP_FLAGS);
+      else if (F.op == FX_OP_PACKED) FX_RRQ(FX_OP_PACKED);
+      else FX_RRQ(FX_OP_BOX);
     });
+#undef FX_RRQ
     FX_LAUNCHED("k_residual_restrict_q");
   } else if (c->restriction == FX_RESTRICT_MEAN) {
     FX_DISPATCH_T(c->storage, FX_DISPATCH_OP(F.op, {
@@ -457,8 +503,8 @@ static int launch_prolong(const fx_mg_config* c, int l, const void* in, const vo
   const int64_t fcells = int64_t(F.nx) * F.ny * F.nz;
   const bool coarse_known = mode != FX_PROLONG_RENORM || C.op == FX_OP_BOX || (op_has_edges(C.op) && C.active);
   const int W = storage_width(c->storage);
-  if (in == out && fcells >= kQuadMinCells && coarse_known && C.op != FX_OP_FLAGS && quad_ok(F, out, nullptr,
-                                                                                             nullptr, W)) {
+  if (in == out && fcells >= kQuadMinCells && coarse_known && C.op != FX_OP_FLAGS &&
+      vec_ok(F, out, nullptr, nullptr, W)) {
     const int nu = int(int64_t(F.nx / W) * C.ny * C.nz);
     dim3 grid((nu + 255) / 256);
     const uint8_t* cact = op_has_edges(C.op) ? C.active : nullptr;
@@ -472,6 +518,10 @@ static int launch_prolong(const fx_mg_config* c, int l, const void* in, const vo
       if (mode == FX_PROLONG_CONSTANT) { FX_PROQ(FX_OP_FLAGS, FX_PROLONG_CONSTANT); }
       else if (mode == FX_PROLONG_TRILINEAR) { FX_PROQ(FX_OP_FLAGS, FX_PROLONG_TRILINEAR); }
       else { FX_PROQ(FX_OP_FLAGS, FX_PROLONG_RENORM); }
+    } else if (F.op == FX_OP_PACKED) {
+      if (mode == FX_PROLONG_CONSTANT) { FX_PROQ(FX_OP_PACKED, FX_PROLONG_CONSTANT); }
+      else if (mode == FX_PROLONG_TRILINEAR) { FX_PROQ(FX_OP_PACKED, FX_PROLONG_TRILINEAR); }
+      else { FX_PROQ(FX_OP_PACKED, FX_PROLONG_RENORM); }
     } else {
       if (mode == FX_PROLONG_CONSTANT) { FX_PROQ(FX_OP_BOX, FX_PROLONG_CONSTANT); }
       else if (mode == FX_PROLONG_TRILINEAR) { FX_PROQ(FX_OP_BOX, FX_PROLONG_TRILINEAR); }

@@ -264,9 +264,34 @@ struct VecStencil {
   uint32_t f[W];
 };
 
+// W consecutive packed operator words at i (i % 4 == 0, 16-byte aligned)
+template <int W>
+struct UV {
+  uint32_t v[W];
+};
+template <int W>
+__device__ __forceinline__ UV<W> ld_words(const uint32_t* __restrict__ p, int64_t i) {
+  UV<W> r;
+#pragma unroll
+  for (int h = 0; h < W / 4; ++h) {
+    const uint4 a = *reinterpret_cast<const uint4*>(p + i + 4 * h);
+    r.v[4 * h] = a.x;
+    r.v[4 * h + 1] = a.y;
+    r.v[4 * h + 2] = a.z;
+    r.v[4 * h + 3] = a.w;
+  }
+  return r;
+}
+
+// per-cell bits of W cells: open-face/fluid flags (FLAGS, BOX) or FLAG_FLUID
+// for an active cell (PACKED)
 template <int W>
 __device__ __forceinline__ void vec_flags(const LevelDev& L, int op, int x0, int y, int z, int64_t i, uint32_t* f) {
-  if (op == FX_OP_FLAGS) {
+  if (op == FX_OP_PACKED) {
+    const UV<W> w = ld_words<W>(reinterpret_cast<const uint32_t*>(L.op), i);
+#pragma unroll
+    for (int k = 0; k < W; ++k) f[k] = ((w.v[k] >> 30) & 1u) ? uint32_t(FLAG_FLUID) : 0u;
+  } else if (op == FX_OP_FLAGS) {
     const uint8_t* fl = reinterpret_cast<const uint8_t*>(L.op) + i;
     if (W == 4) {
       const uint32_t w = *reinterpret_cast<const uint32_t*>(fl);
@@ -283,10 +308,66 @@ __device__ __forceinline__ void vec_flags(const LevelDev& L, int op, int x0, int
   }
 }
 
+// W-wide 7-point operator of a PACKED level: the cell's own word holds the
+// conductances to +x/+y/+z, the -x/-y/-z ones come from the words of the cells
+// below.  Sums keep the order of the one-cell edge stencil (d = +x +y +z -x -y
+// -z; s over +x, -x, +y, -y, +z, -z, zero conductances skipped), so results
+// match it bit for bit.  f = FLAG_FLUID for active cells.
+template <int W, class Ld>
+__device__ __forceinline__ VecStencil<W> vec_stencil_packed(const LevelDev& L, int x0, int y, int z, int64_t i,
+                                                            const Ld& ld, bool read_p) {
+  const int64_t sy = L.nx, sz = int64_t(L.nx) * L.ny;
+  const uint32_t* __restrict__ pk = reinterpret_cast<const uint32_t*>(L.op);
+  const UV<W> wo = ld_words<W>(pk, i);
+  UV<W> wym{}, wzm{};
+  uint32_t wxl = 0;
+  if (y > 0) wym = ld_words<W>(pk, i - sy);
+  if (z > 0) wzm = ld_words<W>(pk, i - sz);
+  if (x0 > 0) wxl = pk[i - 1];
+  FV<W> c{}, ym{}, yp{}, zm{}, zp{};
+  float xl = 0.0f, xr = 0.0f;
+  if (read_p) {
+    c = ld.template vec<W>(i);
+    if (y > 0) ym = ld.template vec<W>(i - sy);
+    if (y + 1 < L.ny) yp = ld.template vec<W>(i + sy);
+    if (z > 0) zm = ld.template vec<W>(i - sz);
+    if (z + 1 < L.nz) zp = ld.template vec<W>(i + sz);
+    if (x0 > 0) xl = ld.one(i - 1);
+    if (x0 + W < L.nx) xr = ld.one(i + W);
+  }
+  VecStencil<W> st;
+#pragma unroll
+  for (int k = 0; k < W; ++k) {
+    const uint32_t u = wo.v[k];
+    const float cxp = float(u & 1023u) * L.wx, cyp = float((u >> 10) & 1023u) * L.wy,
+                czp = float((u >> 20) & 1023u) * L.wz;
+    const float cxm = float((k > 0 ? wo.v[k > 0 ? k - 1 : 0] : wxl) & 1023u) * L.wx;
+    const float cym = float((wym.v[k] >> 10) & 1023u) * L.wy, czm = float((wzm.v[k] >> 20) & 1023u) * L.wz;
+    const float ck = c.v[k];
+    const float left = k > 0 ? c.v[k > 0 ? k - 1 : 0] : xl;
+    const float right = k < W - 1 ? c.v[k < W - 1 ? k + 1 : W - 1] : xr;
+    float s = 0.0f;
+    if (read_p) {
+      if (cxp != 0.0f) s += cxp * (right - ck);
+      if (cxm != 0.0f) s += cxm * (left - ck);
+      if (cyp != 0.0f) s += cyp * (yp.v[k] - ck);
+      if (cym != 0.0f) s += cym * (ym.v[k] - ck);
+      if (czp != 0.0f) s += czp * (zp.v[k] - ck);
+      if (czm != 0.0f) s += czm * (zm.v[k] - ck);
+    }
+    st.s[k] = s;
+    st.d[k] = cxp + cyp + czp + cxm + cym + czm;
+    st.c[k] = ck;
+    st.f[k] = ((u >> 30) & 1u) ? uint32_t(FLAG_FLUID) : 0u;
+  }
+  return st;
+}
+
 template <int OP, int W, class Ld>
 __device__ __forceinline__ VecStencil<W> vec_stencil(const LevelDev& L, int x0, int y, int z, int64_t i, const Ld& ld,
                                                      bool read_p) {
-  static_assert(!op_has_edges(OP), "vector stencils cover flag and box levels");
+  static_assert(OP != FX_OP_EDGES, "vector stencils cover flag, box and packed levels");
+  if constexpr (OP == FX_OP_PACKED) return vec_stencil_packed<W>(L, x0, y, z, i, ld, read_p);
   const int64_t sy = L.nx, sz = int64_t(L.nx) * L.ny;
   VecStencil<W> st;
   vec_flags<W>(L, OP, x0, y, z, i, st.f);
@@ -332,12 +413,21 @@ __device__ __forceinline__ QuadPos vec_pos(const LevelDev& L, int q, int W) {
   return r;
 }
 
-// host: can a level run the quad kernels?
+// host: can a level run the quad kernels of the solvers (box or flags
+// operator, rows a multiple of the width, 16-byte aligned arrays)?
 inline bool quad_ok(const fx_mg_level& L, const void* a = nullptr, const void* b = nullptr,
                     const void* c = nullptr, int width = 4) {
   auto al = [](const void* p) { return (reinterpret_cast<uintptr_t>(p) & 15u) == 0; };
   return !op_has_edges(L.op) && L.nx % width == 0 && (L.op != FX_OP_FLAGS || al(L.op_data)) && al(a) && al(b) &&
          al(c);
+}
+
+// host: can a level run the vector multigrid kernels (quad_ok, or a packed
+// operator with 16-byte aligned words)?
+inline bool vec_ok(const fx_mg_level& L, const void* a, const void* b, const void* c, int width) {
+  if (L.op != FX_OP_PACKED) return quad_ok(L, a, b, c, width);
+  auto al = [](const void* p) { return (reinterpret_cast<uintptr_t>(p) & 15u) == 0; };
+  return L.nx % width == 0 && al(L.op_data) && al(a) && al(b) && al(c);
 }
 
 // cells per lane of the level-0 multigrid kernels for a storage type

@@ -153,7 +153,9 @@ typedef struct fx_mg_level {
   void* p;
   void* q;
   void* rhs;
-  const uint8_t* active;  /* optional, EDGES/PACKED levels: 1 = active cell (fast renormalised prolongation) */
+  const uint8_t* active;  /* optional, EDGES/PACKED levels (fast renormalised prolongation): bit 0 = active
+                            cell; bit 1 (may be left 0) = the cell and its 26 neighbours, indices clamped
+                            to the level, are all active */
 } fx_mg_level;
 
 typedef struct fx_mg_config {

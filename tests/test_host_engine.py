@@ -44,3 +44,19 @@ def test_conductance_hierarchy_matches_oracle():
         for a in range(3):
             assert np.array_equal(e[..., a], cond[..., a] * (shape[a] / og.extent[a]) ** 2)
         assert np.array_equal(e[..., 3] > 0, vol > 0)
+
+
+def test_activity_bytes_neighbourhood_bit():
+    from fluxfx_amd.backend.engine import activity_bytes
+    rng = np.random.default_rng(5)
+    act = rng.random((6, 5, 7)) > 0.15
+    act[:, :2, :3] = True
+    got = activity_bytes(act)
+    nz, ny, nx = act.shape
+    for z in range(nz):
+        for y in range(ny):
+            for x in range(nx):
+                full = all(act[min(max(z + c, 0), nz - 1), min(max(y + b, 0), ny - 1), min(max(x + a, 0), nx - 1)]
+                           for c in (-1, 0, 1) for b in (-1, 0, 1) for a in (-1, 0, 1))
+                assert got[z, y, x] == int(act[z, y, x]) | (int(full) << 1)
+    assert (got & 2).any() and not (got & 2).all()
