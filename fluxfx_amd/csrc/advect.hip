@@ -11,6 +11,7 @@ struct GridDev {
   float ex, ey, ez;  // extent (m)
   float hx, hy, hz;  // cell size (m)
   float ix, iy, iz;  // cells per metre
+  float rx, rx1;     // 1/nx and 1/(nx + 1): row splitting of the thread index
 };
 
 struct V3 {
@@ -19,61 +20,113 @@ struct V3 {
 __device__ __forceinline__ V3 v3(float x, float y, float z) { return V3{x, y, z}; }
 
 // ---------------------------------------------------------------------------
+// addressing: every transport array holds fewer than 2^30 elements and every
+// z-plane fewer than 2^24 (checked on the host), so an element is a kernel-
+// argument base plus a 32-bit byte offset (the loads keep the base in scalar
+// registers) and coordinate products are 24-bit multiplies.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint32_t mul24(uint32_t a, uint32_t b) { return uint32_t(__umul24(a, b)); }
+__device__ __forceinline__ uint32_t lin(uint32_t x, uint32_t y, uint32_t z, uint32_t sx, uint32_t sy) {
+  return x + mul24(sx, y + mul24(sy, z));
+}
+__device__ __forceinline__ float ldf(const float* __restrict__ base, uint32_t off) {
+  return *reinterpret_cast<const float*>(reinterpret_cast<const char*>(base) + off);
+}
+__device__ __forceinline__ void stf(float* __restrict__ base, uint32_t off, float v) {
+  *reinterpret_cast<float*>(reinterpret_cast<char*>(base) + off) = v;
+}
+
+// Thread -> element (x, y, z) of an sx x sy x sz array: blocks of 256 threads
+// tile each z-plane (blockIdx.y) contiguously, so rows of nx + 1 faces leave
+// no idle lanes.  The row split uses the fp32 quotient (exact index < 2^24,
+// at most one off) and one correction.  i: element index.
+__device__ __forceinline__ bool thread_site(uint32_t sx, uint32_t sy, float rsx, uint32_t& x, uint32_t& y,
+                                            uint32_t& z, uint32_t& i) {
+  const uint32_t plane = sx * sy;
+  const uint32_t j = blockIdx.x * 256u + threadIdx.x;
+  if (j >= plane) return false;
+  uint32_t q = uint32_t(float(j) * rsx);
+  int r = int(j) - int(mul24(q, sx));
+  if (r < 0) {
+    q -= 1u;
+    r += int(sx);
+  } else if (r >= int(sx)) {
+    q += 1u;
+    r -= int(sx);
+  }
+  x = uint32_t(r);
+  y = q;
+  z = blockIdx.y;
+  i = j + mul24(plane, z);
+  return true;
+}
+
+// ---------------------------------------------------------------------------
 // face sampling: q clamped to [0,size-1], corner index min(b+o,size-1)
 // ---------------------------------------------------------------------------
-// 32-bit indices: every array here has fewer than 2^31 elements (checked by check_grid)
-__device__ __forceinline__ float sample_face(const float* __restrict__ F, int sx, int sy, int sz, float qx, float qy,
-                                             float qz) {
+// Trilinear footprint: byte offset of the low corner and byte steps to the
+// high corner along x/y/z (0 where the high index is clamped onto the low one).
+struct Tri {
+  uint32_t o, dx, dy, dz;
+  float fx, fy, fz;
+};
+
+__device__ __forceinline__ Tri tri_face(int sx, int sy, int sz, float qx, float qy, float qz) {
   qx = fminf(fmaxf(qx, 0.0f), float(sx - 1));
   qy = fminf(fmaxf(qy, 0.0f), float(sy - 1));
   qz = fminf(fmaxf(qz, 0.0f), float(sz - 1));
-  int bx = int(floorf(qx)), by = int(floorf(qy)), bz = int(floorf(qz));
-  float fx = qx - bx, fy = qy - by, fz = qz - bz;
-  int x1 = min(bx + 1, sx - 1), y1 = min(by + 1, sy - 1), z1 = min(bz + 1, sz - 1);
-  const int pl = sx * sy;
-  int r00 = by * sx + bz * pl, r10 = y1 * sx + bz * pl;
-  int r01 = by * sx + z1 * pl, r11 = y1 * sx + z1 * pl;
-  float gx = 1.0f - fx, gy = 1.0f - fy, gz = 1.0f - fz;
-  float v = 0.0f;
-  v += gx * gy * gz * F[r00 + bx];
-  v += fx * gy * gz * F[r00 + x1];
-  v += gx * fy * gz * F[r10 + bx];
-  v += fx * fy * gz * F[r10 + x1];
-  v += gx * gy * fz * F[r01 + bx];
-  v += fx * gy * fz * F[r01 + x1];
-  v += gx * fy * fz * F[r11 + bx];
-  v += fx * fy * fz * F[r11 + x1];
-  return v;
+  const float lx = floorf(qx), ly = floorf(qy), lz = floorf(qz);
+  const uint32_t bx = uint32_t(lx), by = uint32_t(ly), bz = uint32_t(lz);
+  Tri t;
+  t.fx = qx - lx;
+  t.fy = qy - ly;
+  t.fz = qz - lz;
+  t.o = lin(bx, by, bz, sx, sy) << 2;
+  t.dx = bx + 1u < uint32_t(sx) ? 4u : 0u;
+  t.dy = by + 1u < uint32_t(sy) ? uint32_t(sx) << 2 : 0u;
+  t.dz = bz + 1u < uint32_t(sz) ? mul24(uint32_t(sx), uint32_t(sy)) << 2 : 0u;
+  return t;
+}
+
+// the eight donors in the order (x, y, z) = 000, 100, 010, 110, 001, 101, 011, 111
+__device__ __forceinline__ void tri_donors(const float* __restrict__ F, const Tri& t, float (&d)[8]) {
+  const uint32_t o1 = t.o + t.dx, o2 = t.o + t.dy, o3 = o2 + t.dx;
+  d[0] = ldf(F, t.o);
+  d[1] = ldf(F, o1);
+  d[2] = ldf(F, o2);
+  d[3] = ldf(F, o3);
+  d[4] = ldf(F, t.o + t.dz);
+  d[5] = ldf(F, o1 + t.dz);
+  d[6] = ldf(F, o2 + t.dz);
+  d[7] = ldf(F, o3 + t.dz);
+}
+
+// trilinear value as nested linear interpolations (x, then y, then z)
+__device__ __forceinline__ float lerpf(float a, float b, float t) { return __builtin_fmaf(t, b - a, a); }
+__device__ __forceinline__ float tri_blend(const Tri& t, const float (&d)[8]) {
+  const float c00 = lerpf(d[0], d[1], t.fx), c10 = lerpf(d[2], d[3], t.fx);
+  const float c01 = lerpf(d[4], d[5], t.fx), c11 = lerpf(d[6], d[7], t.fx);
+  return lerpf(lerpf(c00, c10, t.fy), lerpf(c01, c11, t.fy), t.fz);
+}
+
+__device__ __forceinline__ float sample_face(const float* __restrict__ F, int sx, int sy, int sz, float qx, float qy,
+                                             float qz) {
+  float d[8];
+  const Tri t = tri_face(sx, sy, sz, qx, qy, qz);
+  tri_donors(F, t, d);
+  return tri_blend(t, d);
 }
 
 // sample_face plus the min/max of its eight donors (MacCormack clamp; the
 // donor set equals the clamp set whenever q lies inside the array)
 __device__ __forceinline__ float sample_face_mm(const float* __restrict__ F, int sx, int sy, int sz, float qx, float qy,
                                                 float qz, float& lo, float& hi) {
-  qx = fminf(fmaxf(qx, 0.0f), float(sx - 1));
-  qy = fminf(fmaxf(qy, 0.0f), float(sy - 1));
-  qz = fminf(fmaxf(qz, 0.0f), float(sz - 1));
-  int bx = int(floorf(qx)), by = int(floorf(qy)), bz = int(floorf(qz));
-  float fx = qx - bx, fy = qy - by, fz = qz - bz;
-  int x1 = min(bx + 1, sx - 1), y1 = min(by + 1, sy - 1), z1 = min(bz + 1, sz - 1);
-  const int pl = sx * sy;
-  int r00 = by * sx + bz * pl, r10 = y1 * sx + bz * pl;
-  int r01 = by * sx + z1 * pl, r11 = y1 * sx + z1 * pl;
-  const float d0 = F[r00 + bx], d1 = F[r00 + x1], d2 = F[r10 + bx], d3 = F[r10 + x1];
-  const float d4 = F[r01 + bx], d5 = F[r01 + x1], d6 = F[r11 + bx], d7 = F[r11 + x1];
-  float gx = 1.0f - fx, gy = 1.0f - fy, gz = 1.0f - fz;
-  float v = 0.0f;
-  v += gx * gy * gz * d0;
-  v += fx * gy * gz * d1;
-  v += gx * fy * gz * d2;
-  v += fx * fy * gz * d3;
-  v += gx * gy * fz * d4;
-  v += fx * gy * fz * d5;
-  v += gx * fy * fz * d6;
-  v += fx * fy * fz * d7;
-  lo = fminf(fminf(fminf(d0, d1), fminf(d2, d3)), fminf(fminf(d4, d5), fminf(d6, d7)));
-  hi = fmaxf(fmaxf(fmaxf(d0, d1), fmaxf(d2, d3)), fmaxf(fmaxf(d4, d5), fmaxf(d6, d7)));
-  return v;
+  float d[8];
+  const Tri t = tri_face(sx, sy, sz, qx, qy, qz);
+  tri_donors(F, t, d);
+  lo = fminf(fminf(fminf(d[0], d[1]), fminf(d[2], d[3])), fminf(fminf(d[4], d[5]), fminf(d[6], d[7])));
+  hi = fmaxf(fmaxf(fmaxf(d[0], d[1]), fmaxf(d[2], d[3])), fmaxf(fmaxf(d[4], d[5]), fmaxf(d[6], d[7])));
+  return tri_blend(t, d);
 }
 
 struct Vel {
@@ -96,7 +149,7 @@ __device__ __forceinline__ V3 velocity_at(const GridDev& g, const Vel& V, V3 p) 
 // along the third axis, so it is the mean of four values (indices clamped as
 // the sampler clamps its coordinates).
 template <int AXIS>
-__device__ __forceinline__ V3 velocity_at_face(const GridDev& g, const Vel& V, int x, int y, int z, int i) {
+__device__ __forceinline__ V3 velocity_at_face(const GridDev& g, const Vel& V, int x, int y, int z, uint32_t ib) {
   auto four = [&](const float* F, int b) {
     // F: component b, dims (nx + (b==0), ny + (b==1), nz + (b==2)); lattice coords of this face in F
     const int sx = g.nx + (b == 0), sy = g.ny + (b == 1), sz = g.nz + (b == 2);
@@ -115,35 +168,28 @@ __device__ __forceinline__ V3 velocity_at_face(const GridDev& g, const Vel& V, i
         lo[a] = hi[a] = min(c[a], size - 1);
       }
     }
-    auto at = [&](int ax, int ay, int az) { return F[ax + sx * (ay + sy * az)]; };
-    // the two half-offset axes are AXIS and b
-    if (AXIS == 0 && b == 1)
-      return 0.25f * (at(lo[0], lo[1], lo[2]) + at(hi[0], lo[1], lo[2]) + at(lo[0], hi[1], lo[2]) +
-                      at(hi[0], hi[1], lo[2]));
-    if ((AXIS == 0 && b == 2) || (AXIS == 2 && b == 0))
-      return 0.25f * (at(lo[0], lo[1], lo[2]) + at(hi[0], lo[1], lo[2]) + at(lo[0], lo[1], hi[2]) +
-                      at(hi[0], lo[1], hi[2]));
-    if (AXIS == 1 && b == 0)
-      return 0.25f * (at(lo[0], lo[1], lo[2]) + at(hi[0], lo[1], lo[2]) + at(lo[0], hi[1], lo[2]) +
-                      at(hi[0], hi[1], lo[2]));
-    // (AXIS, b) = (1, 2) or (2, 1): y and z
-    return 0.25f * (at(lo[0], lo[1], lo[2]) + at(lo[0], hi[1], lo[2]) + at(lo[0], lo[1], hi[2]) +
-                    at(lo[0], hi[1], hi[2]));
+    // the two half-offset axes are AXIS and b (a1 < a2); the four values are
+    // summed in the order lo, +a1, +a2, +a1+a2
+    const int a1 = AXIS < b ? AXIS : b, a2 = AXIS < b ? b : AXIS;
+    const uint32_t stride[3] = {4u, uint32_t(sx) << 2, mul24(uint32_t(sx), uint32_t(sy)) << 2};
+    const uint32_t o = lin(lo[0], lo[1], lo[2], sx, sy) << 2;
+    const uint32_t s1 = hi[a1] != lo[a1] ? stride[a1] : 0u, s2 = hi[a2] != lo[a2] ? stride[a2] : 0u;
+    return 0.25f * (ldf(F, o) + ldf(F, o + s1) + ldf(F, o + s2) + ldf(F, o + s1 + s2));
   };
-  const float u = AXIS == 0 ? V.u[i] : four(V.u, 0);
-  const float v = AXIS == 1 ? V.v[i] : four(V.v, 1);
-  const float w = AXIS == 2 ? V.w[i] : four(V.w, 2);
+  const float u = AXIS == 0 ? ldf(V.u, ib) : four(V.u, 0);
+  const float v = AXIS == 1 ? ldf(V.v, ib) : four(V.v, 1);
+  const float w = AXIS == 2 ? ldf(V.w, ib) : four(V.w, 2);
   return v3(u, v, w);
 }
 
 // velocityAt at the centre of cell (x, y, z): every component is the mean of
 // the two faces of the cell along its axis
 __device__ __forceinline__ V3 velocity_at_cell(const GridDev& g, const Vel& V, int x, int y, int z) {
-  const int iu = x + (g.nx + 1) * (y + g.ny * z);
-  const int iv = x + g.nx * (y + (g.ny + 1) * z);
-  const int iw = x + g.nx * (y + g.ny * z);
-  return v3(0.5f * (V.u[iu] + V.u[iu + 1]), 0.5f * (V.v[iv] + V.v[iv + g.nx]),
-            0.5f * (V.w[iw] + V.w[iw + g.nx * g.ny]));
+  const uint32_t iu = lin(x, y, z, g.nx + 1, g.ny) << 2;
+  const uint32_t iv = lin(x, y, z, g.nx, g.ny + 1) << 2;
+  const uint32_t iw = lin(x, y, z, g.nx, g.ny) << 2;
+  return v3(0.5f * (ldf(V.u, iu) + ldf(V.u, iu + 4u)), 0.5f * (ldf(V.v, iv) + ldf(V.v, iv + (uint32_t(g.nx) << 2))),
+            0.5f * (ldf(V.w, iw) + ldf(V.w, iw + (mul24(uint32_t(g.nx), uint32_t(g.ny)) << 2))));
 }
 
 // ---------------------------------------------------------------------------
@@ -151,7 +197,7 @@ __device__ __forceinline__ V3 velocity_at_cell(const GridDev& g, const Vel& V, i
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ bool solid_cell(const GridDev& g, const uint8_t* __restrict__ s, int x, int y, int z) {
   if (x < 0 || y < 0 || z < 0 || x >= g.nx || y >= g.ny || z >= g.nz) return true;
-  return s[x + g.nx * (y + g.ny * z)] != 0;
+  return s[lin(x, y, z, g.nx, g.ny)] != 0;
 }
 
 // 3-D DDA clip of origin->target at the first solid cell.  A trace that is
@@ -215,24 +261,36 @@ __device__ bool safe_correction(const GridDev& g, const uint8_t* __restrict__ s,
   return dist3(clip_trace(g, s, p, arr), arr) < 1e-7f;
 }
 
-// Clearance: Chebyshev distance (cells, capped at 255) from a cell to the
-// nearest solid or out-of-range cell.  A segment whose cells all lie within
-// that distance of its origin cell crosses no solid, so the DDA can be skipped;
-// a 3x3x3 halo is solid-free exactly when the clearance exceeds 1.
+// Clearance: Chebyshev distance (cells) from a cell to the nearest solid or
+// out-of-range cell; distances above 15 are stored as 255 ("at least 16").
+// A 3x3x3 halo is solid-free exactly when the clearance exceeds 1.
 __device__ __forceinline__ int clearance_at(const GridDev& g, const uint8_t* __restrict__ cl, V3 p) {
   int cx = int(floorf(p.x * g.ix)), cy = int(floorf(p.y * g.iy)), cz = int(floorf(p.z * g.iz));
   if (cx < 0 || cy < 0 || cz < 0 || cx >= g.nx || cy >= g.ny || cz >= g.nz) return 0;
-  return cl[cx + g.nx * (cy + g.ny * cz)];
+  return cl[lin(cx, cy, cz, g.nx, g.ny)];
 }
 
-__device__ __forceinline__ V3 clip_fast(const GridDev& g, const uint8_t* __restrict__ s,
-                                        const uint8_t* __restrict__ cl, V3 o, V3 t) {
-  if (cl) {
-    float r = fmaxf(fmaxf(fabsf((t.x - o.x) * g.ix), fabsf((t.y - o.y) * g.iy)),
-                    fabsf((t.z - o.z) * g.iz));
-    if (float(clearance_at(g, cl, o)) > ceilf(r) + 1.0f) return t;
+// Solid tests from the clearance c = min(clearance, 16) of one origin cell
+// (c < 0: no clearance field).  A segment o -> t of Chebyshev length r cells
+// with c > ceil(r) + 1 crosses no solid (its cells lie within ceil(r) of the
+// origin cell), and, the distance being 1-Lipschitz, every cell within one
+// cell of t is fluid and in range: the DDA clip returns t, and the halo test
+// of t fails.
+struct Reach {
+  float c;
+  __device__ __forceinline__ bool free(const GridDev& g, V3 o, V3 t) const {
+    const float r = fmaxf(fmaxf(fabsf((t.x - o.x) * g.ix), fabsf((t.y - o.y) * g.iy)), fabsf((t.z - o.z) * g.iz));
+    return c > ceilf(r) + 1.0f;
   }
-  return clip_trace(g, s, o, t);
+};
+
+__device__ __forceinline__ Reach reach_of(const GridDev& g, const uint8_t* __restrict__ cl, V3 o) {
+  return Reach{cl ? float(min(clearance_at(g, cl, o), 16)) : -1.0f};
+}
+
+__device__ __forceinline__ V3 clip_fast(const GridDev& g, const uint8_t* __restrict__ s, const Reach& R, V3 o,
+                                        V3 t) {
+  return R.free(g, o, t) ? t : clip_trace(g, s, o, t);
 }
 
 __device__ __forceinline__ bool halo_fast(const GridDev& g, const uint8_t* __restrict__ s,
@@ -241,12 +299,21 @@ __device__ __forceinline__ bool halo_fast(const GridDev& g, const uint8_t* __res
   return solid_halo(g, s, p);
 }
 
-__device__ bool safe_fast(const GridDev& g, const uint8_t* __restrict__ s, const uint8_t* __restrict__ cl, V3 p,
-                          V3 dep, V3 mid, V3 arr) {
+// the departure/arrival conditions of safe_correction with origin p (R its reach)
+__device__ bool safe_fast(const GridDev& g, const uint8_t* __restrict__ s, const uint8_t* __restrict__ cl,
+                          const Reach& R, V3 p, V3 dep, V3 mid, V3 arr) {
+  if (R.free(g, p, dep) && R.free(g, p, mid) && R.free(g, p, arr)) return true;
   if (halo_fast(g, s, cl, p) || halo_fast(g, s, cl, dep) || halo_fast(g, s, cl, arr)) return false;
-  if (dist3(clip_fast(g, s, cl, p, dep), dep) >= 1e-7f) return false;
-  if (dist3(clip_fast(g, s, cl, p, mid), mid) >= 1e-7f) return false;
-  return dist3(clip_fast(g, s, cl, p, arr), arr) < 1e-7f;
+  if (dist3(clip_fast(g, s, R, p, dep), dep) >= 1e-7f) return false;
+  if (dist3(clip_fast(g, s, R, p, mid), mid) >= 1e-7f) return false;
+  return dist3(clip_fast(g, s, R, p, arr), arr) < 1e-7f;
+}
+
+// unclipped trace p -> t whose end point may be corrected (halo fluid)
+__device__ __forceinline__ bool end_free(const GridDev& g, const uint8_t* __restrict__ s,
+                                         const uint8_t* __restrict__ cl, const Reach& R, V3 p, V3 t) {
+  if (R.free(g, p, t)) return true;
+  return !halo_fast(g, s, cl, t) && dist3(clip_fast(g, s, R, p, t), t) < 1e-7f;
 }
 
 // separable L-infinity distance transform, one axis per pass:
@@ -323,41 +390,44 @@ __global__ void k_face_predict(GridDev g, Vel V, float dt, const float* __restri
                                float* __restrict__ lo_out, float* __restrict__ hi_out,
                                const uint8_t* __restrict__ solid, const uint8_t* __restrict__ clear) {
   const int sx = g.nx + (AXIS == 0), sy = g.ny + (AXIS == 1), sz = g.nz + (AXIS == 2);
-  int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y * blockDim.y + threadIdx.y, z = blockIdx.z;
-  if (x >= sx || y >= sy) return;
-  int64_t i = idx3(x, y, z, sx, sy);
-  if (SOLID) {
+  uint32_t ux, uy, uz, ue;
+  if (!thread_site(sx, sy, AXIS == 0 ? g.rx1 : g.rx, ux, uy, uz, ue)) return;
+  const int x = int(ux), y = int(uy), z = int(uz);
+  const uint32_t i = ue << 2;  // byte offset of this face
+  const float offx = AXIS == 0 ? 0.0f : 0.5f, offy = AXIS == 1 ? 0.0f : 0.5f, offz = AXIS == 2 ? 0.0f : 0.5f;
+  V3 p = v3((x + offx) * g.hx, (y + offy) * g.hy, (z + offz) * g.hz);
+  const Reach R = SOLID ? reach_of(g, clear, p) : Reach{-1.0f};
+  // clearance >= 2 at the cell above the face: both cells of the face are fluid
+  if (SOLID && !(R.c >= 2.0f)) {
     int ox = AXIS == 0, oy = AXIS == 1, oz = AXIS == 2;
     if (solid_cell(g, solid, x, y, z) || solid_cell(g, solid, x - ox, y - oy, z - oz)) {
-      out[i] = 0.0f;
+      stf(out, i, 0.0f);
       if (BOUNDS) {
-        lo_out[i] = __builtin_inff();
-        hi_out[i] = -__builtin_inff();
+        stf(lo_out, i, __builtin_inff());
+        stf(hi_out, i, -__builtin_inff());
       }
       return;
     }
   }
-  const float offx = AXIS == 0 ? 0.0f : 0.5f, offy = AXIS == 1 ? 0.0f : 0.5f, offz = AXIS == 2 ? 0.0f : 0.5f;
-  V3 p = v3((x + offx) * g.hx, (y + offy) * g.hy, (z + offz) * g.hz);
-  V3 vp = velocity_at_face<AXIS>(g, V, x, y, z, int(i));
+  V3 vp = velocity_at_face<AXIS>(g, V, x, y, z, i);
   V3 mid = v3(p.x - 0.5f * dt * vp.x, p.y - 0.5f * dt * vp.y, p.z - 0.5f * dt * vp.z);
-  if (SOLID) mid = clip_fast(g, solid, clear, p, mid);
+  if (SOLID) mid = clip_fast(g, solid, R, p, mid);
   V3 vm = velocity_at(g, V, mid);
   V3 dep = v3(p.x - dt * vm.x, p.y - dt * vm.y, p.z - dt * vm.z);
-  if (SOLID) dep = clip_fast(g, solid, clear, p, dep);
+  if (SOLID) dep = clip_fast(g, solid, R, p, dep);
   if (!BOUNDS) {
-    out[i] = sample_face(in, sx, sy, sz, dep.x * g.ix - offx, dep.y * g.iy - offy, dep.z * g.iz - offz);
+    stf(out, i, sample_face(in, sx, sy, sz, dep.x * g.ix - offx, dep.y * g.iy - offy, dep.z * g.iz - offz));
     return;
   }
   float lo, hi;
-  out[i] = sample_face_mm(in, sx, sy, sz, dep.x * g.ix - offx, dep.y * g.iy - offy, dep.z * g.iz - offz, lo, hi);
+  stf(out, i, sample_face_mm(in, sx, sy, sz, dep.x * g.ix - offx, dep.y * g.iy - offy, dep.z * g.iz - offz, lo, hi));
   const float lbx = offx * g.hx, lby = offy * g.hy, lbz = offz * g.hz;
   const float hbx = (float(sx - 1) + offx) * g.hx, hby = (float(sy - 1) + offy) * g.hy,
               hbz = (float(sz - 1) + offz) * g.hz;
   bool ok = !(dep.x < lbx || dep.y < lby || dep.z < lbz || dep.x > hbx || dep.y > hby || dep.z > hbz);
-  if (SOLID && ok) ok = !halo_fast(g, solid, clear, dep) && dist3(clip_fast(g, solid, clear, p, dep), dep) < 1e-7f;
-  lo_out[i] = ok ? lo : __builtin_inff();
-  hi_out[i] = ok ? hi : -__builtin_inff();
+  if (SOLID && ok) ok = end_free(g, solid, clear, R, p, dep);
+  stf(lo_out, i, ok ? lo : __builtin_inff());
+  stf(hi_out, i, ok ? hi : -__builtin_inff());
 }
 
 // ---------------------------------------------------------------------------
@@ -385,23 +455,26 @@ template <int AXIS, bool SOLID, bool MC, bool BOUNDS = false>
 __global__ void k_velocity(VelArgs A, fx_sources src) {
   const GridDev& g = A.g;
   const int sx = g.nx + (AXIS == 0), sy = g.ny + (AXIS == 1), sz = g.nz + (AXIS == 2);
-  int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y * blockDim.y + threadIdx.y, z = blockIdx.z;
-  if (x >= sx || y >= sy) return;
-  int64_t i = idx3(x, y, z, sx, sy);
+  uint32_t ux, uy, uz, ue;
+  if (!thread_site(sx, sy, AXIS == 0 ? g.rx1 : g.rx, ux, uy, uz, ue)) return;
+  const int x = int(ux), y = int(uy), z = int(uz);
+  const uint32_t i = ue << 2;  // byte offset of this face
   const float* orig = AXIS == 0 ? A.V.u : (AXIS == 1 ? A.V.v : A.V.w);
   const float offx = AXIS == 0 ? 0.0f : 0.5f, offy = AXIS == 1 ? 0.0f : 0.5f, offz = AXIS == 2 ? 0.0f : 0.5f;
   const float hx = g.hx, hy = g.hy, hz = g.hz;
   const float dt = A.dt;
   float value = 0.0f;
   bool blocked = false;
-  if (SOLID) {
+  V3 p = v3((x + offx) * hx, (y + offy) * hy, (z + offz) * hz);
+  const Reach R = SOLID ? reach_of(g, A.clear, p) : Reach{-1.0f};
+  // clearance >= 2 at the cell above the face: both cells of the face are fluid
+  if (SOLID && !(R.c >= 2.0f)) {
     int ox = AXIS == 0, oy = AXIS == 1, oz = AXIS == 2;
     blocked = solid_cell(g, A.solid, x, y, z) || solid_cell(g, A.solid, x - ox, y - oy, z - oz);
   }
-  V3 p = v3((x + offx) * hx, (y + offy) * hy, (z + offz) * hz);
   if (BOUNDS && !blocked) {
-    V3 vp = velocity_at_face<AXIS>(g, A.V, x, y, z, int(i));
-    const float low = A.pred[i], lo = A.lo[i], hi = A.hi[i];
+    V3 vp = velocity_at_face<AXIS>(g, A.V, x, y, z, i);
+    const float low = ldf(A.pred, i), lo = ldf(A.lo, i), hi = ldf(A.hi, i);
     value = low;
     if (lo <= hi) {
       const float hbx = (float(sx - 1) + offx) * hx, hby = (float(sy - 1) + offy) * hy,
@@ -411,24 +484,24 @@ __global__ void k_velocity(VelArgs A, fx_sources src) {
       V3 arr = v3(p.x + dt * va.x, p.y + dt * va.y, p.z + dt * va.z);
       bool ok = !(arr.x < offx * hx || arr.y < offy * hy || arr.z < offz * hz || arr.x > hbx || arr.y > hby ||
                   arr.z > hbz);
-      if (SOLID && ok)
+      if (SOLID && ok && !(R.free(g, p, rmid) && R.free(g, p, arr)))
         ok = !halo_fast(g, A.solid, A.clear, p) && !halo_fast(g, A.solid, A.clear, arr) &&
-             dist3(clip_fast(g, A.solid, A.clear, p, rmid), rmid) < 1e-7f &&
-             dist3(clip_fast(g, A.solid, A.clear, p, arr), arr) < 1e-7f;
+             dist3(clip_fast(g, A.solid, R, p, rmid), rmid) < 1e-7f &&
+             dist3(clip_fast(g, A.solid, R, p, arr), arr) < 1e-7f;
       if (ok) {
         float rev = sample_face(A.pred, sx, sy, sz, arr.x * g.ix - offx, arr.y * g.iy - offy, arr.z * g.iz - offz);
-        float corr = A.pred[i] + 0.5f * (orig[i] - rev);
+        float corr = ldf(A.pred, i) + 0.5f * (ldf(orig, i) - rev);
         value = 0.5f * low + 0.5f * fminf(fmaxf(corr, lo), hi);
       }
     }
   }
   if (!BOUNDS && !blocked) {
-    V3 vp = velocity_at_face<AXIS>(g, A.V, x, y, z, int(i));
+    V3 vp = velocity_at_face<AXIS>(g, A.V, x, y, z, i);
     V3 mid = v3(p.x - 0.5f * dt * vp.x, p.y - 0.5f * dt * vp.y, p.z - 0.5f * dt * vp.z);
-    if (SOLID) mid = clip_fast(g, A.solid, A.clear, p, mid);
+    if (SOLID) mid = clip_fast(g, A.solid, R, p, mid);
     V3 vm = velocity_at(g, A.V, mid);
     V3 dep = v3(p.x - dt * vm.x, p.y - dt * vm.y, p.z - dt * vm.z);
-    if (SOLID) dep = clip_fast(g, A.solid, A.clear, p, dep);
+    if (SOLID) dep = clip_fast(g, A.solid, R, p, dep);
     float qx = dep.x * g.ix - offx, qy = dep.y * g.iy - offy, qz = dep.z * g.iz - offz;
     float lo = 0.0f, hi = 0.0f;
     float low = MC ? sample_face_mm(orig, sx, sy, sz, qx, qy, qz, lo, hi) : sample_face(orig, sx, sy, sz, qx, qy, qz);
@@ -440,13 +513,13 @@ __global__ void k_velocity(VelArgs A, fx_sources src) {
       V3 va = velocity_at(g, A.V, rmid);
       V3 arr = v3(p.x + dt * va.x, p.y + dt * va.y, p.z + dt * va.z);
       bool ok = true;
-      if (SOLID) ok = safe_fast(g, A.solid, A.clear, p, dep, rmid, arr);
+      if (SOLID) ok = safe_fast(g, A.solid, A.clear, R, p, dep, rmid, arr);
       ok = ok && !(dep.x < lbx || dep.y < lby || dep.z < lbz || dep.x > hbx || dep.y > hby || dep.z > hbz);
       ok = ok && !(arr.x < lbx || arr.y < lby || arr.z < lbz || arr.x > hbx || arr.y > hby || arr.z > hbz);
       if (ok) {
         // dep lies inside the face lattice here, so the sampled donors are the clamp set
         float rev = sample_face(A.pred, sx, sy, sz, arr.x * g.ix - offx, arr.y * g.iy - offy, arr.z * g.iz - offz);
-        float corr = A.pred[i] + 0.5f * (orig[i] - rev);
+        float corr = ldf(A.pred, i) + 0.5f * (ldf(orig, i) - rev);
         value = 0.5f * low + 0.5f * fminf(fmaxf(corr, lo), hi);
       }
     }
@@ -467,38 +540,55 @@ __global__ void k_velocity(VelArgs A, fx_sources src) {
   }
   if (AXIS == 2 && A.density) {
     int zb = min(max(z - 1, 0), g.nz - 1), za = min(max(z, 0), g.nz - 1);
-    int64_t ib = idx3(x, y, zb, g.nx, g.ny), ia = idx3(x, y, za, g.nx, g.ny);
-    float rho = 0.5f * (A.density[ib] + A.density[ia]);
-    float theta = 0.5f * (A.temperature[ib] + A.temperature[ia]);
+    const uint32_t ib = lin(x, y, zb, g.nx, g.ny) << 2, ia = lin(x, y, za, g.nx, g.ny) << 2;
+    float rho = 0.5f * (ldf(A.density, ib) + ldf(A.density, ia));
+    float theta = 0.5f * (ldf(A.temperature, ib) + ldf(A.temperature, ia));
     value += dt * (A.lift * theta - A.weight * rho);
   }
-  A.out[i] = value;
+  stf(A.out, i, value);
 }
 
 // ---------------------------------------------------------------------------
 // scalar sampling (closed box: integer indices clamped)
 // ---------------------------------------------------------------------------
+// Corner k = (k & 1, k >> 1 & 1, k >> 2) of a cell-array sample: element
+// offset of the low corner plus element steps to the high corners along
+// x/y/z (0 where the high index clamps onto the low one).
 struct Corners {
-  int idx[8];
+  uint32_t o, dx, dy, dz;
   float w[8];
   float norm;  // 1 / sum of fluid weights (solid sampling)
+  __device__ __forceinline__ uint32_t at(int k) const {
+    return o + ((k & 1) ? dx : 0u) + ((k & 2) ? dy : 0u) + ((k & 4) ? dz : 0u);
+  }
 };
 
 // zero_ext: corners outside the grid contribute 0 (shaders/mac_advect_density.glsl
 // fetchZero); otherwise integer indices are clamped (shaders/closed_sample.glsl).
 __device__ __forceinline__ void closed_corners(const GridDev& g, float qx, float qy, float qz, Corners& c,
                                                bool zero_ext = false) {
-  int bx = int(floorf(qx)), by = int(floorf(qy)), bz = int(floorf(qz));
-  float fx = qx - bx, fy = qy - by, fz = qz - bz;
-#pragma unroll
-  for (int k = 0; k < 8; ++k) {
-    int dx = k & 1, dy = (k >> 1) & 1, dz = k >> 2;
-    int ux = bx + dx, uy = by + dy, uz = bz + dz;
-    int cx = min(max(ux, 0), g.nx - 1), cy = min(max(uy, 0), g.ny - 1), cz = min(max(uz, 0), g.nz - 1);
-    c.idx[k] = cx + g.nx * (cy + g.ny * cz);
-    c.w[k] = (dx ? fx : 1.0f - fx) * (dy ? fy : 1.0f - fy) * (dz ? fz : 1.0f - fz);
-    if (zero_ext && (ux != cx || uy != cy || uz != cz)) c.w[k] = 0.0f;
+  const float lx = floorf(qx), ly = floorf(qy), lz = floorf(qz);
+  const int bx = int(lx), by = int(ly), bz = int(lz);
+  const float fx = qx - lx, fy = qy - ly, fz = qz - lz;
+  const int x0 = min(max(bx, 0), g.nx - 1), x1 = min(max(bx + 1, 0), g.nx - 1);
+  const int y0 = min(max(by, 0), g.ny - 1), y1 = min(max(by + 1, 0), g.ny - 1);
+  const int z0 = min(max(bz, 0), g.nz - 1), z1 = min(max(bz + 1, 0), g.nz - 1);
+  c.o = lin(x0, y0, z0, g.nx, g.ny);
+  c.dx = x1 != x0 ? 1u : 0u;
+  c.dy = y1 != y0 ? uint32_t(g.nx) : 0u;
+  c.dz = z1 != z0 ? mul24(uint32_t(g.nx), uint32_t(g.ny)) : 0u;
+  // per-axis weights of the low/high corner; an out-of-range corner weighs 0
+  float wx[2] = {1.0f - fx, fx}, wy[2] = {1.0f - fy, fy}, wz[2] = {1.0f - fz, fz};
+  if (zero_ext) {
+    if (x0 != bx) wx[0] = 0.0f;
+    if (x1 != bx + 1) wx[1] = 0.0f;
+    if (y0 != by) wy[0] = 0.0f;
+    if (y1 != by + 1) wy[1] = 0.0f;
+    if (z0 != bz) wz[0] = 0.0f;
+    if (z1 != bz + 1) wz[1] = 0.0f;
   }
+#pragma unroll
+  for (int k = 0; k < 8; ++k) c.w[k] = wx[k & 1] * wy[(k >> 1) & 1] * wz[k >> 2];
 }
 
 // Solid corners drop out and the rest renormalise (shaders/closed_sample.glsl:9-19):
@@ -509,7 +599,7 @@ __device__ __forceinline__ void fluid_weights(Corners& c, const uint8_t* __restr
   float t = 0.0f;
 #pragma unroll
   for (int k = 0; k < 8; ++k) {
-    if (solid[c.idx[k]]) c.w[k] = 0.0f;
+    if (solid[c.at(k)]) c.w[k] = 0.0f;
     t += c.w[k];
   }
   c.norm = t > 1e-8f ? 1.0f / t : 0.0f;
@@ -519,7 +609,7 @@ template <bool SOLID>
 __device__ __forceinline__ float closed_sample(const float* __restrict__ f, const Corners& c) {
   float v = 0.0f;
 #pragma unroll
-  for (int k = 0; k < 8; ++k) v += c.w[k] * f[c.idx[k]];
+  for (int k = 0; k < 8; ++k) v += c.w[k] * ldf(f, c.at(k) << 2);
   if (!SOLID) return v;
   return c.norm > 0.0f ? v * c.norm : 0.0f;
 }
@@ -537,55 +627,55 @@ struct ScalarSet {
 template <bool SOLID, bool BOUNDS = false>
 __global__ void k_scalar_predict(GridDev g, Vel V, float dt, int channels, ScalarSet S_,
                                  const uint8_t* __restrict__ solid, const uint8_t* __restrict__ clear) {
-  int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y * blockDim.y + threadIdx.y, z = blockIdx.z;
-  if (x >= g.nx || y >= g.ny) return;
-  int64_t i = idx3(x, y, z, g.nx, g.ny);
-  if (SOLID && solid[i]) {
+  uint32_t ux, uy, uz, ue;
+  if (!thread_site(g.nx, g.ny, g.rx, ux, uy, uz, ue)) return;
+  const int x = int(ux), y = int(uy), z = int(uz);
+  const uint32_t i = ue << 2;  // byte offset of this cell
+  const float hx = g.hx, hy = g.hy, hz = g.hz;
+  V3 p = v3((x + 0.5f) * hx, (y + 0.5f) * hy, (z + 0.5f) * hz);
+  const Reach R = SOLID ? reach_of(g, clear, p) : Reach{-1.0f};
+  // clearance >= 1: the cell is fluid
+  if (SOLID && !(R.c >= 1.0f) && solid[ue]) {
     for (int k = 0; k < channels; ++k) {
-      S_.o[k][i] = 0.0f;
+      stf(S_.o[k], i, 0.0f);
       if (BOUNDS) {
-        S_.lo[k][i] = __builtin_inff();
-        S_.hi[k][i] = -__builtin_inff();
+        stf(S_.lo[k], i, __builtin_inff());
+        stf(S_.hi[k], i, -__builtin_inff());
       }
     }
     return;
   }
-  const float hx = g.hx, hy = g.hy, hz = g.hz;
-  V3 p = v3((x + 0.5f) * hx, (y + 0.5f) * hy, (z + 0.5f) * hz);
   V3 vp = velocity_at_cell(g, V, x, y, z);
   V3 mid = v3(p.x - 0.5f * dt * vp.x, p.y - 0.5f * dt * vp.y, p.z - 0.5f * dt * vp.z);
-  if (SOLID) mid = clip_fast(g, solid, clear, p, mid);
+  if (SOLID) mid = clip_fast(g, solid, R, p, mid);
   V3 vm = velocity_at(g, V, mid);
   V3 dep = v3(p.x - dt * vm.x, p.y - dt * vm.y, p.z - dt * vm.z);
-  if (SOLID) dep = clip_fast(g, solid, clear, p, dep);
+  if (SOLID) dep = clip_fast(g, solid, R, p, dep);
   Corners c;
   closed_corners(g, dep.x * g.ix - 0.5f, dep.y * g.iy - 0.5f, dep.z * g.iz - 0.5f, c);
-  int donors[8];
-#pragma unroll
-  for (int q = 0; q < 8; ++q) donors[q] = c.idx[q];
-  fluid_weights<SOLID>(c, solid);
+  fluid_weights<SOLID>(c, solid);  // the donors (c.at) stay all eight corners
   bool ok = true;
   if (BOUNDS) {
     const float lx = 0.5f * hx, ly = 0.5f * hy, lz = 0.5f * hz;
     const float ux = g.ex - 0.5f * hx, uy = g.ey - 0.5f * hy, uz = g.ez - 0.5f * hz;
     ok = !(dep.x < lx || dep.y < ly || dep.z < lz || dep.x > ux || dep.y > uy || dep.z > uz);
-    if (SOLID && ok) ok = !halo_fast(g, solid, clear, dep) && dist3(clip_fast(g, solid, clear, p, dep), dep) < 1e-7f;
+    if (SOLID && ok) ok = end_free(g, solid, clear, R, p, dep);
   }
   for (int k = 0; k < channels; ++k) {
     const float* f = S_.f[k];
-    S_.o[k][i] = closed_sample<SOLID>(f, c);
+    stf(S_.o[k], i, closed_sample<SOLID>(f, c));
     if (BOUNDS) {
       float lo = __builtin_inff(), hi = -__builtin_inff();
       if (ok) {
 #pragma unroll
         for (int q = 0; q < 8; ++q) {
-          const float d = f[donors[q]];
+          const float d = ldf(f, c.at(q) << 2);
           lo = fminf(lo, d);
           hi = fmaxf(hi, d);
         }
       }
-      S_.lo[k][i] = lo;
-      S_.hi[k][i] = hi;
+      stf(S_.lo[k], i, lo);
+      stf(S_.hi[k], i, hi);
     }
   }
 }
@@ -613,16 +703,19 @@ struct ScalarArgs {
 template <bool SOLID, bool MC, bool BOUNDS = false>
 __global__ void k_scalar_advect(ScalarArgs A, fx_sources src) {
   const GridDev& g = A.g;
-  int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y * blockDim.y + threadIdx.y, z = blockIdx.z;
-  if (x >= g.nx || y >= g.ny) return;
-  int64_t i = idx3(x, y, z, g.nx, g.ny);
-  if (SOLID && A.solid[i]) {
-    for (int k = 0; k < A.channels; ++k) A.out[k][i] = 0.0f;
-    return;
-  }
+  uint32_t ux, uy, uz, ue;
+  if (!thread_site(g.nx, g.ny, g.rx, ux, uy, uz, ue)) return;
+  const int x = int(ux), y = int(uy), z = int(uz);
+  const uint32_t i = ue << 2;  // byte offset of this cell
   const float dt = A.dt;
   const float hx = g.hx, hy = g.hy, hz = g.hz;
   V3 p = v3((x + 0.5f) * hx, (y + 0.5f) * hy, (z + 0.5f) * hz);
+  const Reach R = SOLID ? reach_of(g, A.clear, p) : Reach{-1.0f};
+  // clearance >= 1: the cell is fluid
+  if (SOLID && !(R.c >= 1.0f) && A.solid[ue]) {
+    for (int k = 0; k < A.channels; ++k) stf(A.out[k], i, 0.0f);
+    return;
+  }
   V3 vp = velocity_at_cell(g, A.V, x, y, z);
   float wsrc[8];
   for (int s = 0; s < src.count; ++s) wsrc[s] = emission_weight(p, src.table + 16 * s);
@@ -634,19 +727,19 @@ __global__ void k_scalar_advect(ScalarArgs A, fx_sources src) {
     const float ux = g.ex - 0.5f * hx, uy = g.ey - 0.5f * hy, uz = g.ez - 0.5f * hz;
     bool ok_arr = !(arr.x < 0.5f * hx || arr.y < 0.5f * hy || arr.z < 0.5f * hz || arr.x > ux || arr.y > uy ||
                     arr.z > uz);
-    if (SOLID && ok_arr)
+    if (SOLID && ok_arr && !(R.free(g, p, rmid) && R.free(g, p, arr)))
       ok_arr = !halo_fast(g, A.solid, A.clear, p) && !halo_fast(g, A.solid, A.clear, arr) &&
-               dist3(clip_fast(g, A.solid, A.clear, p, rmid), rmid) < 1e-7f &&
-               dist3(clip_fast(g, A.solid, A.clear, p, arr), arr) < 1e-7f;
+               dist3(clip_fast(g, A.solid, R, p, rmid), rmid) < 1e-7f &&
+               dist3(clip_fast(g, A.solid, R, p, arr), arr) < 1e-7f;
     Corners ca;
     closed_corners(g, arr.x * g.ix - 0.5f, arr.y * g.iy - 0.5f, arr.z * g.iz - 0.5f, ca);
     fluid_weights<SOLID>(ca, A.solid);
     for (int k = 0; k < A.channels; ++k) {
-      const float low = A.pred[k][i], lo = A.lo[k][i], hi = A.hi[k][i];
+      const float low = ldf(A.pred[k], i), lo = ldf(A.lo[k], i), hi = ldf(A.hi[k], i);
       float value = low;
       if (ok_arr && lo <= hi) {
         float rev = closed_sample<SOLID>(A.pred[k], ca);
-        float corr = low + 0.5f * (A.f[k][i] - rev);
+        float corr = low + 0.5f * (ldf(A.f[k], i) - rev);
         value = fminf(fmaxf(corr, lo), hi);
       }
       value *= A.keep[k];
@@ -664,21 +757,18 @@ __global__ void k_scalar_advect(ScalarArgs A, fx_sources src) {
         }
       }
       if (ch == 0 && (src.multi || any_target)) value = fmaxf(value, target);
-      A.out[k][i] = value;
+      stf(A.out[k], i, value);
     }
     return;
   }
   V3 mid = v3(p.x - 0.5f * dt * vp.x, p.y - 0.5f * dt * vp.y, p.z - 0.5f * dt * vp.z);
-  if (SOLID) mid = clip_fast(g, A.solid, A.clear, p, mid);
+  if (SOLID) mid = clip_fast(g, A.solid, R, p, mid);
   V3 vm = velocity_at(g, A.V, mid);
   V3 dep = v3(p.x - dt * vm.x, p.y - dt * vm.y, p.z - dt * vm.z);
-  if (SOLID) dep = clip_fast(g, A.solid, A.clear, p, dep);
+  if (SOLID) dep = clip_fast(g, A.solid, R, p, dep);
   Corners c;
   closed_corners(g, dep.x * g.ix - 0.5f, dep.y * g.iy - 0.5f, dep.z * g.iz - 0.5f, c, A.zero_ext != 0);
-  // min/max donors: all eight corners of the departure cell, solid or not
-  int donors[8];
-#pragma unroll
-  for (int q = 0; q < 8; ++q) donors[q] = c.idx[q];
+  // min/max donors: all eight corners of the departure cell (c.at), solid or not
   fluid_weights<SOLID>(c, A.solid);
   bool ok = false;
   Corners ca;
@@ -687,7 +777,7 @@ __global__ void k_scalar_advect(ScalarArgs A, fx_sources src) {
     V3 va = velocity_at(g, A.V, rmid);
     V3 arr = v3(p.x + dt * va.x, p.y + dt * va.y, p.z + dt * va.z);
     ok = true;
-    if (SOLID) ok = safe_fast(g, A.solid, A.clear, p, dep, rmid, arr);
+    if (SOLID) ok = safe_fast(g, A.solid, A.clear, R, p, dep, rmid, arr);
     float lx = 0.5f * hx, ly = 0.5f * hy, lz = 0.5f * hz;
     float ux = g.ex - 0.5f * hx, uy = g.ey - 0.5f * hy, uz = g.ez - 0.5f * hz;
     ok = ok && !(dep.x < lx || dep.y < ly || dep.z < lz || dep.x > ux || dep.y > uy || dep.z > uz);
@@ -700,11 +790,11 @@ __global__ void k_scalar_advect(ScalarArgs A, fx_sources src) {
     float value = closed_sample<SOLID>(f, c);
     if (MC && ok) {
       float rev = closed_sample<SOLID>(A.pred[k], ca);
-      float corr = A.pred[k][i] + 0.5f * (f[i] - rev);
+      float corr = ldf(A.pred[k], i) + 0.5f * (ldf(f, i) - rev);
       float lo = 3.4e38f, hi = -3.4e38f;
 #pragma unroll
       for (int q = 0; q < 8; ++q) {
-        float d = f[donors[q]];
+        float d = ldf(f, c.at(q) << 2);
         lo = fminf(lo, d);
         hi = fmaxf(hi, d);
       }
@@ -725,7 +815,7 @@ __global__ void k_scalar_advect(ScalarArgs A, fx_sources src) {
       }
     }
     if (ch == 0 && (src.multi || any_target)) value = fmaxf(value, target);
-    A.out[k][i] = value;
+    stf(A.out[k], i, value);
   }
 }
 
@@ -778,7 +868,26 @@ static const dim3 kBlock(64, 4, 1);
 
 GridDev dev_grid(const fx_grid& g) {
   return GridDev{g.nx,        g.ny,        g.nz,        g.ex,        g.ey,        g.ez,
-                 g.ex / g.nx, g.ey / g.ny, g.ez / g.nz, g.nx / g.ex, g.ny / g.ey, g.nz / g.ez};
+                 g.ex / g.nx, g.ey / g.ny, g.ez / g.nz, g.nx / g.ex, g.ny / g.ey, g.nz / g.ez,
+                 1.0f / float(g.nx), 1.0f / float(g.nx + 1)};
+}
+
+// transport kernels: 256 threads per block, blocks tile each z-plane of an
+// sx x sy x sz array (thread_site)
+static const dim3 kSiteBlock(256);
+static dim3 site_grid(int sx, int sy, int sz) { return dim3((unsigned(sx) * unsigned(sy) + 255u) / 256u, sz); }
+
+// 32-bit byte offsets and 24-bit products of the transport kernels: every
+// face array below 2^30 elements, every face plane below 2^24
+int check_transport_grid(const fx_grid& g) {
+  if (int e = check_grid(g)) return e;
+  const int64_t nx = g.nx, ny = g.ny, nz = g.nz;
+  FX_REQUIRE((nx + 1) * ny * nz < (int64_t(1) << 30) && nx * (ny + 1) * nz < (int64_t(1) << 30) &&
+                 nx * ny * (nz + 1) < (int64_t(1) << 30),
+             FX_EINVAL, "transport arrays are limited to 2^30 elements (grid %dx%dx%d)", g.nx, g.ny, g.nz);
+  FX_REQUIRE((nx + 1) * (ny + 1) < (int64_t(1) << 24) && nz < 65536, FX_EINVAL,
+             "transport planes are limited to 2^24 faces and 65535 planes (grid %dx%dx%d)", g.nx, g.ny, g.nz);
+  return FX_OK;
 }
 
 int check_sources(const fx_sources* s) {
@@ -800,7 +909,7 @@ extern "C" {
 
 int fx_face_predict(fx_grid g, float dt, const float* u, const float* v, const float* w, float* pu, float* pv,
                     float* pw, const uint8_t* solid, const uint8_t* clearance, void* stream) {
-  if (int e = check_grid(g)) return e;
+  if (int e = check_transport_grid(g)) return e;
   FX_REQUIRE(u && v && w && pu && pv && pw, FX_EINVAL, "null face array");
   FX_REQUIRE(pu != u && pv != v && pw != w && pu != v && pu != w && pv != u && pv != w && pw != u && pw != v,
              FX_EINVAL, "Advection input/output textures must be distinct");
@@ -808,8 +917,8 @@ int fx_face_predict(fx_grid g, float dt, const float* u, const float* v, const f
   GridDev G = dev_grid(g);
   Vel V{u, v, w};
   hipStream_t s = S(stream);
-  dim3 gx = grid3(g.nx + 1, g.ny, g.nz, kBlock), gy = grid3(g.nx, g.ny + 1, g.nz, kBlock),
-       gz = grid3(g.nx, g.ny, g.nz + 1, kBlock);
+  dim3 gx = site_grid(g.nx + 1, g.ny, g.nz), gy = site_grid(g.nx, g.ny + 1, g.nz), gz = site_grid(g.nx, g.ny, g.nz + 1);
+  const dim3 kBlock = kSiteBlock;
   float* nul = nullptr;
   if (solid) {
     hipLaunchKernelGGL((k_face_predict<0, true, false>), gx, kBlock, 0, s, G, V, dt, u, pu, nul, nul, solid, clearance);
@@ -830,7 +939,7 @@ int fx_face_predict(fx_grid g, float dt, const float* u, const float* v, const f
 int fx_face_predict_bounds(fx_grid g, float dt, const float* u, const float* v, const float* w, float* const* pred,
                            float* const* lo, float* const* hi, const uint8_t* solid, const uint8_t* clearance,
                            void* stream) {
-  if (int e = check_grid(g)) return e;
+  if (int e = check_transport_grid(g)) return e;
   FX_REQUIRE(u && v && w && pred && lo && hi, FX_EINVAL, "null face array");
   const float* ins[3] = {u, v, w};
   for (int a = 0; a < 3; ++a) {
@@ -844,8 +953,8 @@ int fx_face_predict_bounds(fx_grid g, float dt, const float* u, const float* v, 
   GridDev G = dev_grid(g);
   Vel V{u, v, w};
   hipStream_t s = S(stream);
-  dim3 gx = grid3(g.nx + 1, g.ny, g.nz, kBlock), gy = grid3(g.nx, g.ny + 1, g.nz, kBlock),
-       gz = grid3(g.nx, g.ny, g.nz + 1, kBlock);
+  dim3 gx = site_grid(g.nx + 1, g.ny, g.nz), gy = site_grid(g.nx, g.ny + 1, g.nz), gz = site_grid(g.nx, g.ny, g.nz + 1);
+  const dim3 kBlock = kSiteBlock;
   if (solid) {
     hipLaunchKernelGGL((k_face_predict<0, true, true>), gx, kBlock, 0, s, G, V, dt, u, pred[0], lo[0], hi[0], solid,
                        clearance);
@@ -867,7 +976,7 @@ int fx_face_predict_bounds(fx_grid g, float dt, const float* u, const float* v, 
 
 int fx_velocity_advect(const fx_velocity_args* a, void* stream) {
   FX_REQUIRE(a, FX_EINVAL, "null arguments");
-  if (int e = check_grid(a->g)) return e;
+  if (int e = check_transport_grid(a->g)) return e;
   if (int e = check_sources(a->sources)) return e;
   FX_REQUIRE(a->u && a->v && a->w && a->ou && a->ov && a->ow, FX_EINVAL, "null face array");
   const float* ins[6] = {a->u, a->v, a->w, a->pu, a->pv, a->pw};
@@ -881,8 +990,8 @@ int fx_velocity_advect(const fx_velocity_args* a, void* stream) {
   Vel V{a->u, a->v, a->w};
   hipStream_t s = S(stream);
   const fx_grid& g = a->g;
-  dim3 grids[3] = {grid3(g.nx + 1, g.ny, g.nz, kBlock), grid3(g.nx, g.ny + 1, g.nz, kBlock),
-                   grid3(g.nx, g.ny, g.nz + 1, kBlock)};
+  dim3 grids[3] = {site_grid(g.nx + 1, g.ny, g.nz), site_grid(g.nx, g.ny + 1, g.nz), site_grid(g.nx, g.ny, g.nz + 1)};
+  const dim3 kBlock = kSiteBlock;
   const float* preds[3] = {a->pu, a->pv, a->pw};
   float* o3[3] = {a->ou, a->ov, a->ow};
   const float* los[3] = {a->lu, a->lv, a->lw};
@@ -913,7 +1022,7 @@ int fx_velocity_advect(const fx_velocity_args* a, void* stream) {
 int fx_scalar_predict(fx_grid g, float dt, const float* u, const float* v, const float* w, int32_t channels,
                       const float* const* fields, float* const* preds, const uint8_t* solid,
                       const uint8_t* clearance, void* stream) {
-  if (int e = check_grid(g)) return e;
+  if (int e = check_transport_grid(g)) return e;
   FX_REQUIRE(channels >= 1 && channels <= 3, FX_EINVAL, "channels must be 1..3");
   FX_REQUIRE(u && v && w && fields && preds, FX_EINVAL, "null array");
   ScalarSet set{};
@@ -923,7 +1032,8 @@ int fx_scalar_predict(fx_grid g, float dt, const float* u, const float* v, const
     set.f[k] = fields[k];
     set.o[k] = preds[k];
   }
-  dim3 gr = grid3(g.nx, g.ny, g.nz, kBlock);
+  dim3 gr = site_grid(g.nx, g.ny, g.nz);
+  const dim3 kBlock = kSiteBlock;
   Vel V{u, v, w};
   if (solid)
     hipLaunchKernelGGL((k_scalar_predict<true>), gr, kBlock, 0, S(stream), dev_grid(g), V, dt, channels, set, solid,
@@ -938,7 +1048,7 @@ int fx_scalar_predict(fx_grid g, float dt, const float* u, const float* v, const
 int fx_scalar_predict_bounds(fx_grid g, float dt, const float* u, const float* v, const float* w, int32_t channels,
                              const float* const* fields, float* const* preds, float* const* lo, float* const* hi,
                              const uint8_t* solid, const uint8_t* clearance, void* stream) {
-  if (int e = check_grid(g)) return e;
+  if (int e = check_transport_grid(g)) return e;
   FX_REQUIRE(channels >= 1 && channels <= 3, FX_EINVAL, "channels must be 1..3");
   FX_REQUIRE(u && v && w && fields && preds && lo && hi, FX_EINVAL, "null array");
   ScalarSet set{};
@@ -953,7 +1063,8 @@ int fx_scalar_predict_bounds(fx_grid g, float dt, const float* u, const float* v
     set.lo[k] = lo[k];
     set.hi[k] = hi[k];
   }
-  dim3 gr = grid3(g.nx, g.ny, g.nz, kBlock);
+  dim3 gr = site_grid(g.nx, g.ny, g.nz);
+  const dim3 kBlock = kSiteBlock;
   Vel V{u, v, w};
   if (solid)
     hipLaunchKernelGGL((k_scalar_predict<true, true>), gr, kBlock, 0, S(stream), dev_grid(g), V, dt, channels, set,
@@ -967,7 +1078,7 @@ int fx_scalar_predict_bounds(fx_grid g, float dt, const float* u, const float* v
 
 int fx_scalar_advect(const fx_scalar_args* a, void* stream) {
   FX_REQUIRE(a, FX_EINVAL, "null arguments");
-  if (int e = check_grid(a->g)) return e;
+  if (int e = check_transport_grid(a->g)) return e;
   if (int e = check_sources(a->sources)) return e;
   FX_REQUIRE(a->channels >= 1 && a->channels <= 3, FX_EINVAL, "channels must be 1..3");
   FX_REQUIRE(a->u && a->v && a->w, FX_EINVAL, "null velocity");
@@ -1004,7 +1115,8 @@ int fx_scalar_advect(const fx_scalar_args* a, void* stream) {
       FX_REQUIRE(a->out[j] != a->lo[k] && a->out[j] != a->hi[k], FX_EINVAL,
                  "Advection input/output textures must be distinct");
   }
-  dim3 gr = grid3(a->g.nx, a->g.ny, a->g.nz, kBlock);
+  dim3 gr = site_grid(a->g.nx, a->g.ny, a->g.nz);
+  const dim3 kBlock = kSiteBlock;
   hipStream_t s = S(stream);
   if (a->solid) {
     if (bounds) hipLaunchKernelGGL((k_scalar_advect<true, true, true>), gr, kBlock, 0, s, A, *a->sources);
