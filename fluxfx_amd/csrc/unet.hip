@@ -33,7 +33,8 @@ struct ConvDev {
   int up0, relu;
 };
 
-__device__ __forceinline__ uint16_t bf16_bits(float v) { return from_f<bf16>(v).bits; }
+// round to nearest even (one v_cvt_pk_bf16_f32 on gfx950)
+__device__ __forceinline__ uint16_t bf16_bits(float v) { return __builtin_bit_cast(uint16_t, (__bf16)v); }
 __device__ __forceinline__ float bf16_val(uint16_t b) { return __uint_as_float(uint32_t(b) << 16); }
 
 // 8 channels [ci0, ci0+8) of voxel (x, y, z) of the concatenated input; zero
@@ -61,72 +62,164 @@ __device__ __forceinline__ bf16x8_t as_bf16x8(uint4 u) { return __builtin_bit_ca
 // voxels; wave w owns plane w of the tile as four 16-voxel M tiles (one per
 // row) times all output channels.  Per input-channel chunk (8, 16 or 32
 // channels) the block stages the 18 x 6 x 6 halo tile in LDS (zero outside
-// the grid; upsampling and concatenation resolved while staging), then every
-// K step reads its A operands from LDS at a per-tap offset: no per-lane
-// bounds or index arithmetic in the inner loop.  A voxel occupies 16 bytes
-// more than its channels in LDS, which spreads the 16 lanes of a fragment
-// read over all 64 banks.
+// the grid; upsampling and concatenation resolved while staging) together
+// with the chunk's weights in B-operand order.
+//
+// The K loop runs over "sets": groups of 32/cw (dz, dx) tap offsets (lane
+// group lg takes offset (8 lg)/cw of the set and channels (8 lg) mod cw).  For
+// a set a wave reads the six A rows y = -1 .. 4 of its plane once and feeds
+// them to the three dy taps (output row t with tap dy uses row t + dy), so a
+// set costs 6 A reads + 3 B reads per 12 x NT MFMAs.  With one 16-channel
+// output tile (FULLB) the weights of all 27 taps of the chunk sit in LDS and
+// a set may mix dz planes; with more output tiles LDS holds one dz slice at a
+// time and sets group dx offsets only.  A voxel occupies 16 bytes more than
+// its channels in LDS, which spreads the 16 lanes of a fragment read over all
+// 64 banks.
 constexpr int kTX = 16, kTY = 4, kTZ = 4;
 constexpr int kHX = kTX + 2, kHY = kTY + 2, kHZ = kTZ + 2;
 constexpr int kHalo = kHX * kHY * kHZ;  // 648 voxels
 constexpr int kChunk = 32;              // input channels per staged chunk
 constexpr int kVoxBytes = kChunk * 2 + 16;
+constexpr int kFrag = 1024;             // one B operand: 64 lanes x 16 bytes
 
-template <int NT>
+// LDS bytes of the staged weights: FULLB 9 sets (cw = 32) x 3 dy, else 3 sets x 3 dy, NT tiles each
+template <int NT, bool FULLB>
+constexpr int conv_b_bytes() {
+  return (FULLB ? 27 : 9) * NT * kFrag;
+}
+
+// CIN > 0: input channel count known at compile time (chunk loops unrolled,
+// LDS offsets immediate); CIN = 0: any multiple of 8 (d.cin)
+template <int CIN, int NT>
 __global__ __launch_bounds__(256) void k_conv_fwd(ConvDev d, const uint16_t* __restrict__ in0,
                                                   const uint16_t* __restrict__ in1, const uint16_t* __restrict__ wt,
                                                   const float* __restrict__ bias, uint16_t* __restrict__ out,
                                                   int tiles_x, int tiles_y) {
+  constexpr bool FULLB = NT == 1;
   __shared__ __attribute__((aligned(16))) unsigned char tile[kHalo * kVoxBytes];
+  __shared__ __attribute__((aligned(16))) unsigned char wlds[conv_b_bytes<NT, FULLB>()];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int li = lane & 15, lg = lane >> 4;
   const int b = blockIdx.x;
   const int tx = b % tiles_x, rest = b / tiles_x, ty = rest % tiles_y, tz = rest / tiles_y;
   const int x0 = tx * kTX, y0 = ty * kTY, z0 = tz * kTZ;
+  const int cin = CIN > 0 ? CIN : d.cin;
   floatx4_t acc[kTY][NT];
 #pragma unroll
   for (int t = 0; t < kTY; ++t)
 #pragma unroll
     for (int n = 0; n < NT; ++n) acc[t][n] = floatx4_t{0.0f, 0.0f, 0.0f, 0.0f};
-  // LDS voxel of output (li, row t, plane wave) at tap (0,0,0): halo coords (li+1, t+1, wave+1)
-  const int lbase = ((wave + 1) * kHY + 1) * kHX + li + 1;
-  for (int cbase = 0; cbase < d.cin; cbase += kChunk) {
-    const int cw = min(kChunk, d.cin - cbase);  // 8, 16 or 32
-    const int groups = cw >> 3;
-    __syncthreads();  // previous chunk fully consumed
-    for (int e = threadIdx.x; e < kHalo * groups; e += 256) {
-      const int hv = e / groups, g = e - hv * groups;
-      const int hx = hv % kHX, hr = hv / kHX, hy = hr % kHY, hz = hr / kHY;
-      const uint4 v = conv_load_a(d, in0, in1, x0 - 1 + hx, y0 - 1 + hy, z0 - 1 + hz, cbase + 8 * g);
-      *reinterpret_cast<uint4*>(tile + hv * kVoxBytes + g * 16) = v;
+  // element offsets of this thread's halo voxels (one per 256 threads) in in0
+  // (upsampled when up0) and in1; -1: outside the grid (zero padding)
+  constexpr int kSlotsA = (kHalo + 255) / 256;
+  int64_t v0[kSlotsA], v1[kSlotsA];
+#pragma unroll
+  for (int q = 0; q < kSlotsA; ++q) {
+    const int hv = int(threadIdx.x) + 256 * q;
+    const int hx = hv % kHX, hr = hv / kHX, hy = hr % kHY, hz = hr / kHY;
+    const int x = x0 - 1 + hx, y = y0 - 1 + hy, z = z0 - 1 + hz;
+    v0[q] = v1[q] = -1;
+    if (hv < kHalo && x >= 0 && y >= 0 && z >= 0 && x < d.nx && y < d.ny && z < d.nz) {
+      const int64_t v = (int64_t(z) * d.ny + y) * d.nx + x;
+      v1[q] = v * d.c1;
+      v0[q] = (d.up0 ? (int64_t(z >> 1) * (d.ny >> 1) + (y >> 1)) * (d.nx >> 1) + (x >> 1) : v) * d.c0;
     }
-    __syncthreads();
+  }
+#pragma unroll
+  for (int cbase = 0; cbase < (CIN > 0 ? CIN : 4096); cbase += kChunk) {
+    if (CIN == 0 && cbase >= cin) break;
+    const int cw = min(kChunk, cin - cbase);  // 8, 16 or 32
+    const int groups = cw >> 3;
     const int shift = cw == 32 ? 5 : (cw == 16 ? 4 : 3);
-    const int ksteps = (27 * cw + 31) >> 5;
-    for (int ks = 0; ks < ksteps; ++ks) {
-      const int kk = ks * 32 + lg * 8;
-      const int tap = kk >> shift, cg = kk & (cw - 1);
-      uint4 a[kTY];
-      uint4 bw[NT];
-      if (tap < 27) {
-        const int dz = tap / 9 - 1, dy = (tap / 3) % 3 - 1, dx = tap % 3 - 1;
-        const int off = (lbase + (dz * kHY + dy) * kHX + dx) * kVoxBytes + cg * 2;
+    const int per_set = 32 >> shift;            // tap offsets per set
+    const int kof = (lg * 8) >> shift;          // this lane group's offset within a set
+    const int cg = (lg * 8) & (cw - 1);         // and its channel offset
+    const int noff = FULLB ? 9 : 3;             // (dz, dx) offsets per pass
+    const int nsets = (noff + per_set - 1) / per_set;
+    // staging: every load of a fill is issued before the first LDS store, so a
+    // fill costs one memory latency (loads in registers, then stores)
+    uint4 va[kSlotsA][4];
 #pragma unroll
-        for (int t = 0; t < kTY; ++t) a[t] = *reinterpret_cast<const uint4*>(tile + off + t * kHX * kVoxBytes);
-        const uint16_t* wp = wt + int64_t(li) * d.kp + tap * d.cin + cbase + cg;
+    for (int q = 0; q < kSlotsA; ++q)
 #pragma unroll
-        for (int n = 0; n < NT; ++n) bw[n] = *reinterpret_cast<const uint4*>(wp + int64_t(n) * 16 * d.kp);
-      } else {
-#pragma unroll
-        for (int t = 0; t < kTY; ++t) a[t] = make_uint4(0, 0, 0, 0);
-#pragma unroll
-        for (int n = 0; n < NT; ++n) bw[n] = make_uint4(0, 0, 0, 0);
+      for (int g = 0; g < 4; ++g) {
+        va[q][g] = make_uint4(0, 0, 0, 0);
+        const int ci = cbase + 8 * g;
+        if (g < groups) {
+          if (ci < d.c0) {
+            if (v0[q] >= 0) va[q][g] = *reinterpret_cast<const uint4*>(in0 + v0[q] + ci);
+          } else if (v1[q] >= 0) {
+            va[q][g] = *reinterpret_cast<const uint4*>(in1 + v1[q] + (ci - d.c0));
+          }
+        }
       }
+    __syncthreads();  // previous chunk fully consumed
 #pragma unroll
-      for (int t = 0; t < kTY; ++t)
+    for (int q = 0; q < kSlotsA; ++q) {
+      const int hv = int(threadIdx.x) + 256 * q;
+      if (hv < kHalo)
 #pragma unroll
-        for (int n = 0; n < NT; ++n)
-          acc[t][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(as_bf16x8(a[t]), as_bf16x8(bw[n]), acc[t][n], 0, 0, 0);
+        for (int g = 0; g < 4; ++g)
+          if (g < groups) *reinterpret_cast<uint4*>(tile + hv * kVoxBytes + g * 16) = va[q][g];
+    }
+    for (int pass = 0; pass < (FULLB ? 1 : 3); ++pass) {
+      // weights of this pass in B-operand order: fragment ((set * 3 + dy) * NT + n),
+      // lane-major; item (fragment, channel li) gathers the four lane groups
+      constexpr int kItems = conv_b_bytes<NT, FULLB>() / kFrag * 16;
+      constexpr int kSlotsB = (kItems + 255) / 256;
+      const int nitems = nsets * 3 * NT * 16;
+      uint4 vb[kSlotsB][4];
+#pragma unroll
+      for (int q = 0; q < kSlotsB; ++q) {
+        const int item = int(threadIdx.x) + 256 * q;
+        const int frag = item >> 4, co_lo = item & 15;
+        const int n = frag % NT, sd = frag / NT, dyi = sd % 3, set = sd / 3;
+        const uint16_t* wrow = wt + int64_t(16 * n + co_lo) * d.kp + cbase + dyi * 3 * cin;
+#pragma unroll
+        for (int g = 0; g < 4; ++g) {
+          vb[q][g] = make_uint4(0, 0, 0, 0);
+          const int k = set * per_set + ((g * 8) >> shift);
+          if (item < nitems && k < noff) {
+            const int dzi = FULLB ? k / 3 : pass, dxi = FULLB ? k - 3 * (k / 3) : k;
+            vb[q][g] = *reinterpret_cast<const uint4*>(wrow + (dzi * 9 + dxi) * cin + ((g * 8) & (cw - 1)));
+          }
+        }
+      }
+      if (!FULLB && pass > 0) __syncthreads();  // previous dz slice consumed
+#pragma unroll
+      for (int q = 0; q < kSlotsB; ++q) {
+        const int item = int(threadIdx.x) + 256 * q;
+        if (item < nitems)
+#pragma unroll
+          for (int g = 0; g < 4; ++g)
+            *reinterpret_cast<uint4*>(wlds + (item >> 4) * kFrag + (g * 16 + (item & 15)) * 16) = vb[q][g];
+      }
+      __syncthreads();
+#pragma unroll
+      for (int set = 0; set < (FULLB ? 9 : 3); ++set) {
+        if (set >= nsets) break;
+        // this lane group's tap offset (padding offsets read offset 0: their weights are zero)
+        int k = set * per_set + kof;
+        if (k >= noff) k = 0;
+        const int dz = FULLB ? k / 3 - 1 : pass - 1, dx = FULLB ? k % 3 - 1 : k - 1;
+        const int base = (((wave + 1 + dz) * kHY) * kHX + li + 1 + dx) * kVoxBytes + cg * 2;
+        uint4 a[kTY + 2];
+#pragma unroll
+        for (int r = 0; r < kTY + 2; ++r) a[r] = *reinterpret_cast<const uint4*>(tile + base + r * kHX * kVoxBytes);
+#pragma unroll
+        for (int dyi = 0; dyi < 3; ++dyi) {
+          uint4 bw[NT];
+#pragma unroll
+          for (int n = 0; n < NT; ++n)
+            bw[n] = *reinterpret_cast<const uint4*>(wlds + ((set * 3 + dyi) * NT + n) * kFrag + lane * 16);
+#pragma unroll
+          for (int t = 0; t < kTY; ++t)
+#pragma unroll
+            for (int n = 0; n < NT; ++n)
+              acc[t][n] =
+                  __builtin_amdgcn_mfma_f32_16x16x32_bf16(as_bf16x8(a[t + dyi]), as_bf16x8(bw[n]), acc[t][n], 0, 0, 0);
+        }
+      }
     }
   }
   // lane (li, lg) holds x = x0 + 4*lg + j of row t, output channel 16*n + li
@@ -136,18 +229,230 @@ __global__ __launch_bounds__(256) void k_conv_fwd(ConvDev d, const uint16_t* __r
   for (int t = 0; t < kTY; ++t) {
     const int y = y0 + t;
     if (y >= d.ny) continue;
+    uint16_t* orow = out + ((int64_t(z) * d.ny + y) * d.nx + x0 + 4 * lg) * d.cout + li;
 #pragma unroll
     for (int n = 0; n < NT; ++n) {
-      const int co = 16 * n + li;
-      const float bv = bias ? bias[co] : 0.0f;
+      const float bv = bias ? bias[16 * n + li] : 0.0f;
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
-        const int x = x0 + 4 * lg + j;
-        if (x < d.nx) {
+        if (x0 + 4 * lg + j < d.nx) {
           float r = acc[t][n][j] + bv;
           if (d.relu) r = fmaxf(r, 0.0f);
-          out[((int64_t(z) * d.ny + y) * d.nx + x) * d.cout + co] = bf16_bits(r);
+          orow[j * d.cout + 16 * n] = bf16_bits(r);
         }
+      }
+    }
+  }
+}
+
+// Pipelined form of k_conv_fwd for a compile-time channel count: a block
+// owns `tiles` consecutive 16 x 4 x 4 tiles along z of one (x, y) column and
+// walks the steps (tile, chunk, dz pass).  While a step computes from LDS, the
+// global loads of the next step's halo (and weights) are already in flight in
+// registers.  With one output tile and one chunk (BREG) every wave keeps the
+// B operands of all its K steps in registers for the whole block and LDS
+// holds only the halo tile.  Same sums per output as k_conv_fwd.
+template <int CIN, int NT>
+__global__ __launch_bounds__(256) void k_conv_pipe(ConvDev d, const uint16_t* __restrict__ in0,
+                                                   const uint16_t* __restrict__ in1,
+                                                   const uint16_t* __restrict__ wt, const float* __restrict__ bias,
+                                                   uint16_t* __restrict__ out, int tiles_x, int tiles_y, int tiles) {
+  constexpr bool FULLB = NT == 1;
+  constexpr int NCH = (CIN + 31) / 32, PASSES = FULLB ? 1 : 3;
+  constexpr bool BREG = FULLB && NCH == 1;
+  constexpr int kSlotsA = (kHalo + 255) / 256;
+  constexpr int kItems = conv_b_bytes<NT, FULLB>() / kFrag * 16, kSlotsB = BREG ? 1 : (kItems + 255) / 256;
+  constexpr int COUT = 16 * NT;
+  __shared__ __attribute__((aligned(16))) unsigned char tile[kHalo * kVoxBytes];
+  __shared__ __attribute__((aligned(16))) unsigned char wlds[BREG ? 16 : conv_b_bytes<NT, FULLB>()];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int li = lane & 15, lg = lane >> 4;
+  const int b = blockIdx.x;
+  const int tx = b % tiles_x, rest = b / tiles_x, ty = rest % tiles_y, tzg = rest / tiles_y;
+  const int x0 = tx * kTX, y0 = ty * kTY;
+  const int nsteps = tiles * NCH * PASSES;
+  auto chunk_w = [](int c) { return CIN % 32 == 0 ? 32 : (c == NCH - 1 ? CIN % 32 : 32); };
+  auto tile_z0 = [&](int k) { return (tzg * tiles + k) * kTZ; };
+  floatx4_t acc[kTY][NT];
+#pragma unroll
+  for (int t = 0; t < kTY; ++t)
+#pragma unroll
+    for (int n = 0; n < NT; ++n) acc[t][n] = floatx4_t{0.0f, 0.0f, 0.0f, 0.0f};
+  // this thread's halo voxels: element offsets at tile 0 and per-tile voxel
+  // increments (tile k lies k * 4 planes further along z; 2 coarse planes when
+  // in0 is upsampled)
+  const int64_t plane = int64_t(d.nx) * d.ny;
+  const int64_t step0 = d.up0 ? 2 * int64_t(d.nx >> 1) * (d.ny >> 1) : 4 * plane;
+  int64_t hv0[kSlotsA], hv1[kSlotsA];
+  int hzs[kSlotsA];
+  bool hxy[kSlotsA];
+#pragma unroll
+  for (int q = 0; q < kSlotsA; ++q) {
+    const int hv = int(threadIdx.x) + 256 * q;
+    const int hx = hv % kHX, hr = hv / kHX, hy = hr % kHY, hz = hr / kHY;
+    const int x = x0 - 1 + hx, y = y0 - 1 + hy, z = tile_z0(0) - 1 + hz;
+    hxy[q] = hv < kHalo && x >= 0 && y >= 0 && x < d.nx && y < d.ny;
+    hzs[q] = z;
+    const int xc = max(x, 0), yc = max(y, 0);  // outside voxels are never loaded
+    const int64_t v = (int64_t(z) * d.ny + yc) * d.nx + xc;
+    hv1[q] = v * d.c1;
+    hv0[q] = (d.up0 ? (int64_t(z >> 1) * (d.ny >> 1) + (yc >> 1)) * (d.nx >> 1) + (xc >> 1) : v) * d.c0;
+  }
+  uint4 va[kSlotsA][4], vb[kSlotsB][4];
+  auto load_a = [&](int k, int c) {
+    const int cbase = 32 * c, groups = chunk_w(c) >> 3;
+    const int kz = k * kTZ;
+#pragma unroll
+    for (int q = 0; q < kSlotsA; ++q) {
+      const int z = hzs[q] + kz;
+      const bool inside = hxy[q] && z >= 0 && z < d.nz;
+      const int64_t v0 = hv0[q] + k * step0 * d.c0, v1 = hv1[q] + k * 4 * plane * d.c1;
+#pragma unroll
+      for (int g = 0; g < 4; ++g) {
+        va[q][g] = make_uint4(0, 0, 0, 0);
+        const int ci = cbase + 8 * g;
+        if (inside && g < groups)
+          va[q][g] = ci < d.c0 ? *reinterpret_cast<const uint4*>(in0 + v0 + ci)
+                               : *reinterpret_cast<const uint4*>(in1 + v1 + (ci - d.c0));
+      }
+    }
+  };
+  auto store_a = [&](int c) {
+    const int groups = chunk_w(c) >> 3;
+#pragma unroll
+    for (int q = 0; q < kSlotsA; ++q) {
+      const int hv = int(threadIdx.x) + 256 * q;
+      if (hv < kHalo)
+#pragma unroll
+        for (int g = 0; g < 4; ++g)
+          if (g < groups) *reinterpret_cast<uint4*>(tile + hv * kVoxBytes + g * 16) = va[q][g];
+    }
+  };
+  auto load_b = [&](int c, int pass) {
+    const int cbase = 32 * c, cw = chunk_w(c), shift = cw == 32 ? 5 : (cw == 16 ? 4 : 3);
+    const int per_set = 32 >> shift, noff = FULLB ? 9 : 3, nsets = (noff + per_set - 1) / per_set;
+    const int nitems = nsets * 3 * NT * 16;
+#pragma unroll
+    for (int q = 0; q < kSlotsB; ++q) {
+      const int item = int(threadIdx.x) + 256 * q;
+      const int frag = item >> 4, co_lo = item & 15;
+      const int n = frag % NT, sd = frag / NT, dyi = sd % 3, set = sd / 3;
+      const uint16_t* wrow = wt + int64_t(16 * n + co_lo) * d.kp + cbase + dyi * 3 * CIN;
+#pragma unroll
+      for (int g = 0; g < 4; ++g) {
+        vb[q][g] = make_uint4(0, 0, 0, 0);
+        const int k = set * per_set + ((g * 8) >> shift);
+        if (item < nitems && k < noff) {
+          const int dzi = FULLB ? k / 3 : pass, dxi = FULLB ? k - 3 * (k / 3) : k;
+          vb[q][g] = *reinterpret_cast<const uint4*>(wrow + (dzi * 9 + dxi) * CIN + ((g * 8) & (cw - 1)));
+        }
+      }
+    }
+  };
+  auto store_b = [&](int c) {
+    const int cw = chunk_w(c), shift = cw == 32 ? 5 : (cw == 16 ? 4 : 3);
+    const int per_set = 32 >> shift, noff = FULLB ? 9 : 3, nsets = (noff + per_set - 1) / per_set;
+    const int nitems = nsets * 3 * NT * 16;
+#pragma unroll
+    for (int q = 0; q < kSlotsB; ++q) {
+      const int item = int(threadIdx.x) + 256 * q;
+      if (item < nitems)
+#pragma unroll
+        for (int g = 0; g < 4; ++g)
+          *reinterpret_cast<uint4*>(wlds + (item >> 4) * kFrag + (g * 16 + (item & 15)) * 16) = vb[q][g];
+    }
+  };
+  // BREG: the wave's B fragments (set, dy) for all K steps, straight from global
+  constexpr int kCW0 = CIN >= 32 ? 32 : CIN;
+  constexpr int kShift0 = kCW0 == 32 ? 5 : (kCW0 == 16 ? 4 : 3);
+  constexpr int kSets0 = (9 + (32 >> kShift0) - 1) / (32 >> kShift0);
+  uint4 breg[BREG ? kSets0 * 3 : 1];
+  if (BREG) {
+#pragma unroll
+    for (int set = 0; set < kSets0; ++set)
+#pragma unroll
+      for (int dyi = 0; dyi < 3; ++dyi) {
+        const int k = set * (32 >> kShift0) + ((lg * 8) >> kShift0);
+        breg[set * 3 + dyi] = make_uint4(0, 0, 0, 0);
+        if (k < 9)
+          breg[set * 3 + dyi] = *reinterpret_cast<const uint4*>(
+              wt + int64_t(li) * d.kp + ((k / 3) * 9 + dyi * 3 + k % 3) * CIN + ((lg * 8) & (kCW0 - 1)));
+      }
+  }
+  // weights change at every step of the slice form; with FULLB at every chunk
+  // change (every step when NCH > 1, else only at the first)
+  auto new_b = [&](int s) { return !BREG && (!FULLB || NCH > 1 || s == 0); };
+  load_a(0, 0);
+  if (new_b(0)) load_b(0, 0);
+  for (int s = 0; s < nsteps; ++s) {
+    const int pass = s % PASSES, c = (s / PASSES) % NCH, k = s / (PASSES * NCH);
+    if (tile_z0(k) >= d.nz) break;  // uniform over the block
+    __syncthreads();  // previous step fully consumed
+    if (pass == 0) store_a(c);
+    if (new_b(s)) store_b(c);
+    __syncthreads();
+    if (s + 1 < nsteps) {
+      const int p1 = (s + 1) % PASSES, c1 = ((s + 1) / PASSES) % NCH, k1 = (s + 1) / (PASSES * NCH);
+      if (tile_z0(k1) < d.nz) {
+        if (p1 == 0) load_a(k1, c1);
+        if (new_b(s + 1)) load_b(c1, p1);
+      }
+    }
+    {
+      const int cw = chunk_w(c), shift = cw == 32 ? 5 : (cw == 16 ? 4 : 3);
+      const int per_set = 32 >> shift, noff = FULLB ? 9 : 3, nsets = (noff + per_set - 1) / per_set;
+      const int kof = (lg * 8) >> shift, cg = (lg * 8) & (cw - 1);
+#pragma unroll
+      for (int set = 0; set < (FULLB ? 9 : 3); ++set) {
+        if (set >= nsets) break;
+        int kk = set * per_set + kof;
+        if (kk >= noff) kk = 0;  // padding offsets: their weights are zero
+        const int dz = FULLB ? kk / 3 - 1 : pass - 1, dx = FULLB ? kk % 3 - 1 : kk - 1;
+        const int base = (((wave + 1 + dz) * kHY) * kHX + li + 1 + dx) * kVoxBytes + cg * 2;
+        uint4 a[kTY + 2];
+#pragma unroll
+        for (int r = 0; r < kTY + 2; ++r) a[r] = *reinterpret_cast<const uint4*>(tile + base + r * kHX * kVoxBytes);
+#pragma unroll
+        for (int dyi = 0; dyi < 3; ++dyi) {
+          uint4 bw[NT];
+#pragma unroll
+          for (int n = 0; n < NT; ++n)
+            bw[n] = BREG ? breg[BREG ? set * 3 + dyi : 0]
+                         : *reinterpret_cast<const uint4*>(wlds + ((set * 3 + dyi) * NT + n) * kFrag + lane * 16);
+#pragma unroll
+          for (int t = 0; t < kTY; ++t)
+#pragma unroll
+            for (int n = 0; n < NT; ++n)
+              acc[t][n] =
+                  __builtin_amdgcn_mfma_f32_16x16x32_bf16(as_bf16x8(a[t + dyi]), as_bf16x8(bw[n]), acc[t][n], 0, 0, 0);
+        }
+      }
+    }
+    if (pass == PASSES - 1 && c == NCH - 1) {
+      // lane (li, lg) holds x = x0 + 4*lg + j of row t, output channel 16*n + li
+      const int z = tile_z0(k) + wave;
+      float bv[NT];
+#pragma unroll
+      for (int n = 0; n < NT; ++n) bv[n] = bias ? bias[16 * n + li] : 0.0f;
+#pragma unroll
+      for (int t = 0; t < kTY; ++t) {
+        const int y = y0 + t;
+        if (z < d.nz && y < d.ny) {
+          uint16_t* orow = out + ((int64_t(z) * d.ny + y) * d.nx + x0 + 4 * lg) * COUT + li;
+#pragma unroll
+          for (int j = 0; j < 4; ++j) {
+            if (x0 + 4 * lg + j < d.nx) {
+#pragma unroll
+              for (int n = 0; n < NT; ++n) {
+                float r = acc[t][n][j] + bv[n];
+                if (d.relu) r = fmaxf(r, 0.0f);
+                orow[j * COUT + 16 * n] = bf16_bits(r);
+              }
+            }
+          }
+        }
+#pragma unroll
+        for (int n = 0; n < NT; ++n) acc[t][n] = floatx4_t{0.0f, 0.0f, 0.0f, 0.0f};
       }
     }
   }
@@ -555,14 +860,40 @@ int conv_forward(const fx_conv_desc* c, const void* in0, const void* in1, const 
   const uint16_t* a1 = (const uint16_t*)in1;
   const uint16_t* w = (const uint16_t*)wt;
   uint16_t* o = (uint16_t*)out;
-  switch (c->cout / 16) {
-    case 1: hipLaunchKernelGGL(k_conv_fwd<1>, grid, dim3(256), 0, s, d, a0, a1, w, bias, o, tiles_x, tiles_y); break;
-    case 2: hipLaunchKernelGGL(k_conv_fwd<2>, grid, dim3(256), 0, s, d, a0, a1, w, bias, o, tiles_x, tiles_y); break;
-    case 3: hipLaunchKernelGGL(k_conv_fwd<3>, grid, dim3(256), 0, s, d, a0, a1, w, bias, o, tiles_x, tiles_y); break;
-    case 4: hipLaunchKernelGGL(k_conv_fwd<4>, grid, dim3(256), 0, s, d, a0, a1, w, bias, o, tiles_x, tiles_y); break;
-    case 5: hipLaunchKernelGGL(k_conv_fwd<5>, grid, dim3(256), 0, s, d, a0, a1, w, bias, o, tiles_x, tiles_y); break;
-    default: hipLaunchKernelGGL(k_conv_fwd<6>, grid, dim3(256), 0, s, d, a0, a1, w, bias, o, tiles_x, tiles_y); break;
+  // the U-Net's forward and data-gradient layers: pipelined kernel, a few
+  // z tiles per block (about 1024 blocks or more)
+  const int columns = tiles_x * tiles_y;
+  const int per_block = std::max(1, std::min(8, tiles_z * columns / 1024));
+  const dim3 pgrid(columns * ((tiles_z + per_block - 1) / per_block));
+#define FX_PIPE(CI, NT) \
+  hipLaunchKernelGGL((k_conv_pipe<CI, NT>), pgrid, dim3(256), 0, s, d, a0, a1, w, bias, o, tiles_x, tiles_y, per_block)
+#define FX_CONV(NT) \
+  hipLaunchKernelGGL((k_conv_fwd<0, NT>), grid, dim3(256), 0, s, d, a0, a1, w, bias, o, tiles_x, tiles_y)
+  const int nt = c->cout / 16, ci = d.cin;
+  if (nt == 1 && ci == 8) FX_PIPE(8, 1);
+  else if (nt == 1 && ci == 16) FX_PIPE(16, 1);
+  else if (nt == 1 && ci == 32) FX_PIPE(32, 1);
+  else if (nt == 1 && ci == 48) FX_PIPE(48, 1);
+  else if (nt == 2 && ci == 16) FX_PIPE(16, 2);
+  else if (nt == 2 && ci == 32) FX_PIPE(32, 2);
+  else if (nt == 2 && ci == 64) FX_PIPE(64, 2);
+  else if (nt == 2 && ci == 96) FX_PIPE(96, 2);
+  else if (nt == 3 && ci == 16) FX_PIPE(16, 3);
+  else if (nt == 4 && ci == 32) FX_PIPE(32, 4);
+  else if (nt == 4 && ci == 64) FX_PIPE(64, 4);
+  else if (nt == 6 && ci == 32) FX_PIPE(32, 6);
+  else {
+    switch (nt) {
+      case 1: FX_CONV(1); break;
+      case 2: FX_CONV(2); break;
+      case 3: FX_CONV(3); break;
+      case 4: FX_CONV(4); break;
+      case 5: FX_CONV(5); break;
+      default: FX_CONV(6); break;
+    }
   }
+#undef FX_CONV
+#undef FX_PIPE
   FX_LAUNCHED("k_conv_fwd");
   return FX_OK;
 }
