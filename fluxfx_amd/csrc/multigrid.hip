@@ -8,7 +8,12 @@
 namespace fx {
 
 // ---------------------------------------------------------------------------
-// smoothing: out = active && d>0 ? p + omega*((s - rhs*inv_dt)/d - p) : 0
+// smoothing: out = active && d>0 ? p + omega*((s - rhs*inv_dt)/d - p) : 0.
+// Every smoother variant evaluates it as the same fused operations, with the
+// division as a hardware reciprocal (1 ulp), so they agree bit for bit.
+__device__ __forceinline__ float jacobi_update(float c, float s, float b, float d, float inv_dt, float omega) {
+  return __builtin_fmaf(omega * __builtin_fmaf(-b, inv_dt, s), __builtin_amdgcn_rcpf(d), c);
+}
 // zero_guess: p is known to be 0 (fresh coarse level or error equation)
 // ---------------------------------------------------------------------------
 template <typename T, int OP>
@@ -20,7 +25,7 @@ __global__ void k_smooth(LevelDev L, const T* __restrict__ p, const T* __restric
   Stencil st = stencil<OP>(L, x, y, z, i, p, !zero_guess);
   float r = 0.0f;
   // correction form of p + omega*((sum w p_n - rhs)/d - p)
-  if (st.active && st.d > 0.0f) r = st.c + omega * (st.s - to_f(rhs[i]) * inv_dt) / st.d;
+  if (st.active && st.d > 0.0f) r = jacobi_update(st.c, st.s, to_f(rhs[i]), st.d, inv_dt, omega);
   out[i] = from_f<T>(r);
 }
 
@@ -59,7 +64,7 @@ __global__ __launch_bounds__(256) void k_smooth_zm(LevelDev L, const T* __restri
     if (f & FLAG_ZLO) { d += L.wz; sdiff += L.wz * (pm - pc); }
     if (f & FLAG_ZHI) { d += L.wz; sdiff += L.wz * (pn - pc); }
     float r = 0.0f;
-    if ((f & FLAG_FLUID) && d > 0.0f) r = pc + omega * (sdiff - to_f(rhs[i]) * inv_dt) / d;
+    if ((f & FLAG_FLUID) && d > 0.0f) r = jacobi_update(pc, sdiff, to_f(rhs[i]), d, inv_dt, omega);
     out[i] = from_f<T>(r);
     pm = pc;
     pc = pn;
@@ -83,7 +88,7 @@ __global__ __launch_bounds__(256) void k_smooth_q(LevelDev L, const T* __restric
 #pragma unroll
   for (int k = 0; k < W; ++k) {
     r.v[k] = 0.0f;
-    if ((st.f[k] & FLAG_FLUID) && st.d[k] > 0.0f) r.v[k] = st.c[k] + omega * (st.s[k] - b.v[k] * inv_dt) / st.d[k];
+    if ((st.f[k] & FLAG_FLUID) && st.d[k] > 0.0f) r.v[k] = jacobi_update(st.c[k], st.s[k], b.v[k], st.d[k], inv_dt, omega);
   }
   VecIO<T, W>::st(out, at.i, r);
 }

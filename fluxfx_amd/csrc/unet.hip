@@ -245,6 +245,14 @@ __global__ __launch_bounds__(256) void k_conv_fwd(ConvDev d, const uint16_t* __r
   }
 }
 
+// Block barrier ordering LDS only: global stores in flight (conv outputs,
+// never read back by the block) are not waited for, unlike __syncthreads().
+__device__ __forceinline__ void lds_barrier() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
 // Pipelined form of k_conv_fwd for a compile-time channel count: a block
 // owns `tiles` consecutive 16 x 4 x 4 tiles along z of one (x, y) column and
 // walks the steps (tile, chunk, dz pass).  While a step computes from LDS, the
@@ -387,10 +395,10 @@ __global__ __launch_bounds__(256) void k_conv_pipe(ConvDev d, const uint16_t* __
   for (int s = 0; s < nsteps; ++s) {
     const int pass = s % PASSES, c = (s / PASSES) % NCH, k = s / (PASSES * NCH);
     if (tile_z0(k) >= d.nz) break;  // uniform over the block
-    __syncthreads();  // previous step fully consumed
+    lds_barrier();  // previous step fully consumed
     if (pass == 0) store_a(c);
     if (new_b(s)) store_b(c);
-    __syncthreads();
+    lds_barrier();
     if (s + 1 < nsteps) {
       const int p1 = (s + 1) % PASSES, c1 = ((s + 1) / PASSES) % NCH, k1 = (s + 1) / (PASSES * NCH);
       if (tile_z0(k1) < d.nz) {
@@ -615,19 +623,36 @@ __global__ void k_bias_grad(int nvox, int cout, const uint16_t* __restrict__ g, 
 }
 
 // 2x2x2 mean of a channels-last tensor (fine grid nx, ny, nz)
+// thread = (coarse voxel, 8-channel group): eight 16-byte child loads, one
+// 16-byte store; per channel the children are summed in (dz, dy, dx) order
 __global__ void k_pool(int nx, int ny, int nz, int c, const uint16_t* __restrict__ in, uint16_t* __restrict__ out) {
   const int hx = nx >> 1, hy = ny >> 1, hz = nz >> 1;
-  const int64_t n = int64_t(hx) * hy * hz * c;
+  const int groups = c >> 3;
+  const int64_t n = int64_t(hx) * hy * hz * groups;
   for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += int64_t(gridDim.x) * blockDim.x) {
-    const int ch = int(i % c);
-    const int64_t v = i / c;
-    const int X = int(v % hx), Y = int((v / hx) % hy), Z = int(v / (int64_t(hx) * hy));
-    float s = 0.0f;
+    const int g = int(i % groups);
+    const int64_t v = i / groups;
+    const int X = int(v % hx);
+    const int64_t row = v / hx;
+    const int Y = int(row % hy), Z = int(row / hy);
+    float s[8] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
     for (int dz = 0; dz < 2; ++dz)
+#pragma unroll
       for (int dy = 0; dy < 2; ++dy)
-        for (int dx = 0; dx < 2; ++dx)
-          s += bf16_val(in[((int64_t(2 * Z + dz) * ny + 2 * Y + dy) * nx + 2 * X + dx) * c + ch]);
-    out[i] = bf16_bits(s * 0.125f);
+#pragma unroll
+        for (int dx = 0; dx < 2; ++dx) {
+          const uint4 a = *reinterpret_cast<const uint4*>(
+              in + ((int64_t(2 * Z + dz) * ny + 2 * Y + dy) * nx + 2 * X + dx) * c + 8 * g);
+          const uint32_t u[4] = {a.x, a.y, a.z, a.w};
+#pragma unroll
+          for (int k = 0; k < 8; ++k) s[k] += bf16_val(uint16_t((u[k >> 1] >> (16 * (k & 1))) & 0xffffu));
+        }
+    uint32_t o[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+      o[k] = uint32_t(bf16_bits(s[2 * k] * 0.125f)) | (uint32_t(bf16_bits(s[2 * k + 1] * 0.125f)) << 16);
+    *reinterpret_cast<uint4*>(out + v * c + 8 * g) = make_uint4(o[0], o[1], o[2], o[3]);
   }
 }
 
@@ -683,6 +708,7 @@ __global__ void k_unet_input(int64_t n, const float* __restrict__ r, const uint8
                              const double* __restrict__ ws, int q, uint16_t* __restrict__ out) {
   const double ss = ws[q], cnt = ws[q + 1];
   const float scale = (ss > 0.0 && cnt > 0.0) ? float(1.0 / sqrt(ss / cnt)) : 0.0f;
+  // one voxel per thread: each store instruction writes 1 KB contiguous
   for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += int64_t(gridDim.x) * blockDim.x) {
     const bool s = solid && solid[i];
     uint4 w;
@@ -696,10 +722,18 @@ __global__ void k_unet_input(int64_t n, const float* __restrict__ r, const uint8
 __global__ void k_fluid_norm(int64_t n, const float* __restrict__ r, const uint8_t* __restrict__ solid, double* ws,
                              int q) {
   double s = 0.0, c = 0.0;
-  for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += int64_t(gridDim.x) * blockDim.x) {
-    if (solid && solid[i]) continue;
-    s += double(r[i]) * double(r[i]);
-    c += 1.0;
+  // four voxels per thread and iteration (n % 4 == 0), added in index order
+  for (int64_t i = 4 * (int64_t(blockIdx.x) * blockDim.x + threadIdx.x); i < n;
+       i += 4 * int64_t(gridDim.x) * blockDim.x) {
+    const float4 rv = *reinterpret_cast<const float4*>(r + i);
+    const uint32_t sv = solid ? *reinterpret_cast<const uint32_t*>(solid + i) : 0u;
+    const float rr[4] = {rv.x, rv.y, rv.z, rv.w};
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      if ((sv >> (8 * k)) & 0xffu) continue;
+      s += double(rr[k]) * double(rr[k]);
+      c += 1.0;
+    }
   }
   block_partial(s, ws, q);
   block_partial(c, ws, q + 1);
@@ -847,6 +881,12 @@ static int check_conv(const fx_conv_desc* c) {
 }
 
 static bool al16(const void* p) { return (reinterpret_cast<uintptr_t>(p) & 15u) == 0; }
+static bool al4(const void* p) { return (reinterpret_cast<uintptr_t>(p) & 3u) == 0; }
+
+// blocks of 256 threads for `items` work items (grid-stride beyond 16384 blocks)
+static dim3 item_grid(int64_t items) {
+  return dim3(unsigned(std::max<int64_t>(1, std::min<int64_t>((items + 255) / 256, 16384))));
+}
 
 int conv_forward(const fx_conv_desc* c, const void* in0, const void* in1, const void* wt, const float* bias,
                  void* out, hipStream_t s) {
@@ -914,16 +954,16 @@ int unet_forward(const fx_unet* net, const float* r, const uint8_t* solid, doubl
   hipLaunchKernelGGL(k_fluid_norm, dim3(kRedBlocks), dim3(kRedThreads), 0, s, n, r, solid, ws, 8);
   FX_LAUNCHED("k_fluid_norm");
   ws_finalize(ws, 8, 2, s);
-  hipLaunchKernelGGL(k_unet_input, dim3(2048), dim3(256), 0, s, n, r, solid, (const double*)ws, 8,
+  hipLaunchKernelGGL(k_unet_input, item_grid(n), dim3(256), 0, s, n, r, solid, (const double*)ws, 8,
                      (uint16_t*)net->input);
   FX_LAUNCHED("k_unet_input");
   if (int e = conv(0, 0, 8, 0, 16, 0, net->input, nullptr)) return e;
   if (int e = conv(1, 0, 16, 0, 16, 0, net->act[0], nullptr)) return e;
-  hipLaunchKernelGGL(k_pool, dim3(2048), dim3(256), 0, s, nx, ny, nz, 16, (const uint16_t*)net->act[1],
+  hipLaunchKernelGGL(k_pool, item_grid(n / 8 * 2), dim3(256), 0, s, nx, ny, nz, 16, (const uint16_t*)net->act[1],
                      (uint16_t*)net->pool1);
   if (int e = conv(2, 1, 16, 0, 32, 0, net->pool1, nullptr)) return e;
   if (int e = conv(3, 1, 32, 0, 32, 0, net->act[2], nullptr)) return e;
-  hipLaunchKernelGGL(k_pool, dim3(2048), dim3(256), 0, s, nx >> 1, ny >> 1, nz >> 1, 32,
+  hipLaunchKernelGGL(k_pool, item_grid(n / 64 * 4), dim3(256), 0, s, nx >> 1, ny >> 1, nz >> 1, 32,
                      (const uint16_t*)net->act[3], (uint16_t*)net->pool2);
   FX_LAUNCHED("k_pool");
   if (int e = conv(4, 2, 32, 0, 64, 0, net->pool2, nullptr)) return e;
@@ -932,7 +972,7 @@ int unet_forward(const fx_unet* net, const float* r, const uint8_t* solid, doubl
   if (int e = conv(7, 1, 32, 0, 32, 0, net->act[6], nullptr)) return e;
   if (int e = conv(8, 0, 32, 16, 16, 1, net->act[7], net->act[1])) return e;
   if (int e = conv(9, 0, 16, 0, 16, 0, net->act[8], nullptr)) return e;
-  hipLaunchKernelGGL(k_head, dim3(2048), dim3(256), 0, s, n, (const uint16_t*)net->act[9], net->head,
+  hipLaunchKernelGGL(k_head, item_grid(n), dim3(256), 0, s, n, (const uint16_t*)net->act[9], net->head,
                      net->head + 16, solid, output_rms ? (const double*)ws : nullptr, 8, post_scale, z);
   FX_LAUNCHED("k_head");
   return FX_OK;
@@ -1039,8 +1079,10 @@ int fx_relu_mask(int64_t n, void* g, const void* act, void* stream) {
 }
 
 int fx_pool(int32_t nx, int32_t ny, int32_t nz, int32_t c, const void* in, void* out, void* stream) {
-  FX_REQUIRE(nx % 2 == 0 && ny % 2 == 0 && nz % 2 == 0 && c > 0 && in && out, FX_EINVAL, "pool: bad arguments");
-  hipLaunchKernelGGL(k_pool, dim3(2048), dim3(256), 0, S(stream), nx, ny, nz, c, (const uint16_t*)in,
+  FX_REQUIRE(nx % 2 == 0 && ny % 2 == 0 && nz % 2 == 0 && c > 0 && c % 8 == 0 && in && out && al16(in) && al16(out),
+             FX_EINVAL, "pool: bad arguments (even grid, channels a multiple of 8, 16-byte aligned arrays)");
+  hipLaunchKernelGGL(k_pool, item_grid(int64_t(nx / 2) * (ny / 2) * (nz / 2) * (c / 8)), dim3(256), 0, S(stream), nx,
+                     ny, nz, c, (const uint16_t*)in,
                      (uint16_t*)out);
   FX_LAUNCHED("k_pool");
   return FX_OK;
@@ -1067,20 +1109,23 @@ int fx_concat_backward(int32_t nx, int32_t ny, int32_t nz, int32_t c0, int32_t c
 }
 
 int fx_unet_input(int64_t n, const float* r, const uint8_t* solid, double* ws, void* out, void* stream) {
-  FX_REQUIRE(n > 0 && r && ws && out && al16(out), FX_EINVAL, "unet input: bad arguments");
+  FX_REQUIRE(n > 0 && n % 4 == 0 && r && ws && out && al16(out) && al16(r) && (!solid || al4(solid)), FX_EINVAL,
+             "unet input: bad arguments (n a multiple of 4, 16-byte aligned r and out)");
   hipStream_t s = S(stream);
   hipLaunchKernelGGL(k_fluid_norm, dim3(kRedBlocks), dim3(kRedThreads), 0, s, n, r, solid, ws, 8);
   FX_LAUNCHED("k_fluid_norm");
   ws_finalize(ws, 8, 2, s);
-  hipLaunchKernelGGL(k_unet_input, dim3(2048), dim3(256), 0, s, n, r, solid, (const double*)ws, 8, (uint16_t*)out);
+  hipLaunchKernelGGL(k_unet_input, item_grid(n), dim3(256), 0, s, n, r, solid, (const double*)ws, 8,
+                     (uint16_t*)out);
   FX_LAUNCHED("k_unet_input");
   return FX_OK;
 }
 
 int fx_unet_head(int64_t n, const void* act, const float* w, const float* b, const uint8_t* solid, const double* ws,
                  float post_scale, float* z, void* stream) {
-  FX_REQUIRE(n > 0 && act && w && b && z && al16(act), FX_EINVAL, "unet head: bad arguments");
-  hipLaunchKernelGGL(k_head, dim3(2048), dim3(256), 0, S(stream), n, (const uint16_t*)act, w, b, solid, ws, 8,
+  FX_REQUIRE(n > 0 && n % 4 == 0 && act && w && b && z && al16(act) && al16(z) && (!solid || al4(solid)), FX_EINVAL,
+             "unet head: bad arguments (n a multiple of 4, 16-byte aligned act and z)");
+  hipLaunchKernelGGL(k_head, item_grid(n), dim3(256), 0, S(stream), n, (const uint16_t*)act, w, b, solid, ws, 8,
                      post_scale, z);
   FX_LAUNCHED("k_head");
   return FX_OK;

@@ -396,7 +396,8 @@ int fx_conv_pack_weights(int32_t cin, int32_t cpack, int32_t cout, int32_t flip,
                          void* stream);
 /* g[i] = act[i] > 0 ? g[i] : 0 (bf16, in place) */
 int fx_relu_mask(int64_t n, void* g, const void* act, void* stream);
-/* 2x2x2 mean of a channels-last bf16 tensor on the fine grid (nx, ny, nz) */
+/* 2x2x2 mean of a channels-last bf16 tensor on the fine grid (nx, ny, nz);
+ * c a multiple of 8, 16-byte aligned arrays */
 int fx_pool(int32_t nx, int32_t ny, int32_t nz, int32_t c, const void* in, void* out, void* stream);
 /* out = skip + (adjoint of the 2x2x2 mean)(gpool) on the fine grid; skip may be NULL */
 int fx_pool_backward_add(int32_t nx, int32_t ny, int32_t nz, int32_t c, const void* gpool, const void* skip,
@@ -406,10 +407,12 @@ int fx_pool_backward_add(int32_t nx, int32_t ny, int32_t nz, int32_t c, const vo
 int fx_concat_backward(int32_t nx, int32_t ny, int32_t nz, int32_t c0, int32_t c1, const void* g, void* gup,
                        void* gskip, void* stream);
 /* packed network input [n][8]: (r / rms_fluid(r), solid, 0...) with
- * ws[8] = sum_fluid r^2, ws[9] = fluid count (device side) */
+ * ws[8] = sum_fluid r^2, ws[9] = fluid count (device side); n a multiple of 4,
+ * r and out 16-byte aligned, solid (optional) 4-byte aligned */
 int fx_unet_input(int64_t n, const float* r, const uint8_t* solid, double* ws, void* out, void* stream);
 /* z[v] = fluid ? (sum_c act[v][c] w[c] + b) * s : 0, s = rms_fluid(r) * post_scale
- * from ws[8..9] (fx_unet_input), or s = post_scale when ws == NULL */
+ * from ws[8..9] (fx_unet_input), or s = post_scale when ws == NULL; n a multiple
+ * of 4, act and z 16-byte aligned, solid (optional) 4-byte aligned */
 int fx_unet_head(int64_t n, const void* act, const float* w, const float* b, const uint8_t* solid, const double* ws,
                  float post_scale, float* z, void* stream);
 /* head backward: gact = gout (x) w; dwb[0..15] += sum gout*act, dwb[16] += sum gout */
