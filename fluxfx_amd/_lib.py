@@ -102,7 +102,7 @@ SIGNATURES = {
     "fx_divergence": ([fx_grid, P, P, P, P, P, P], C.c_int),
     "fx_jacobi_sweep": ([fx_grid, P, P, P, P, F32, F32, P], C.c_int),
     "fx_gradient": ([fx_grid, P, P, P, P, P, P, P, P, F32, P], C.c_int),
-    "fx_gradient_split": ([fx_grid, P, P, P, P, P, P, P, P, P, F32, P], C.c_int),
+    "fx_gradient_split": ([fx_grid, P, P, P, P, P, P, P, P, P, P, F32, P], C.c_int),
     "fx_increment_rhs": ([fx_grid, P, P, P, P, P, P], C.c_int),
     "fx_conv3d_forward": ([C.POINTER(fx_conv_desc), P, P, P, P, P, P], C.c_int),
     "fx_conv3d_weight_grad": ([C.POINTER(fx_conv_desc), P, P, P, P, P, I64, P, P, I32, P], C.c_int),

@@ -151,11 +151,12 @@ class ToleranceProjector(PressureProjector):
 
     def apply_gradient(self, fields, outputs, scale):
         inc = self._increment_fields()
+        # the same pass writes the impulse x0 + d (next warm start, diagnostics) into spare
         _lib.call("fx_gradient_split", self._g, self.flags_ptr, self.pressure.ptr, inc["d"].ptr,
-                  *(f.ptr for f in fields), *(f.ptr for f in outputs), float(scale), self.device.stream)
-        # impulse for the next warm start and for diagnostics: x0 + d, fluid mean removed
-        _lib.call("fx_combine", self.pressure.ptr, inc["d"].ptr, self.spare.ptr, self.pressure.size, self.device.stream)
+                  *(f.ptr for f in fields), *(f.ptr for f in outputs), self.spare.ptr, float(scale),
+                  self.device.stream)
         self.pressure, self.spare = self.spare, self.pressure
+        # fluid mean removed
         _lib.call("fx_project_mean", self._g, self.flags_ptr, self.pressure.ptr, self.ws.data_ptr(), self.device.stream)
 
     def _increment_fields(self):

@@ -200,6 +200,130 @@ __global__ void k_gradient(int nx, int ny, int nz, float ix, float iy, float iz,
   }
 }
 
+// ---------------------------------------------------------------------------
+// Vector forms of divergence and gradient (nx % 4 == 0, 16-byte aligned cell,
+// V, W and flag arrays): lane q owns the four cells x0..x0+3 of one row, reads
+// cells, V and W rows and the flag word with 16-byte / 4-byte loads (U rows
+// are nx+1 long, so U goes through scalar loads), and blocks are spread so
+// each XCD sweeps a contiguous range of rows, keeping the y/z neighbour rows
+// in its own L2.  Same operations per face/cell as the one-cell kernels.
+__device__ __forceinline__ int xcd_row_tile(int b, int nb) {
+  return (nb % 8 == 0) ? (b % 8) * (nb / 8) + b / 8 : b;
+}
+
+__device__ __forceinline__ float4 ld4f(const float* __restrict__ p, int i) {
+  return *reinterpret_cast<const float4*>(p + i);
+}
+__device__ __forceinline__ void st4f(float* __restrict__ p, int i, float4 v) {
+  *reinterpret_cast<float4*>(p + i) = v;
+}
+__device__ __forceinline__ float comp(const float4& v, int k) {
+  return k == 0 ? v.x : (k == 1 ? v.y : (k == 2 ? v.z : v.w));
+}
+
+__device__ __forceinline__ uint32_t row_flags(const uint8_t* __restrict__ flags, int x0, int y, int z, int nx, int ny,
+                                              int nz, int c) {
+  if (flags) return *reinterpret_cast<const uint32_t*>(flags + c);
+  uint32_t w = 0;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) w |= uint32_t(box_flags(x0 + k, y, z, nx, ny, nz)) << (8 * k);
+  return w;
+}
+
+__global__ __launch_bounds__(256) void k_divergence_q(int nx, int ny, int nz, float ix, float iy, float iz,
+                                                      const uint8_t* __restrict__ flags, const float* __restrict__ u,
+                                                      const float* __restrict__ v, const float* __restrict__ w,
+                                                      float* __restrict__ div, int nq) {
+  const int q = xcd_row_tile(blockIdx.x, gridDim.x) * 256 + threadIdx.x;
+  if (q >= nq) return;
+  const int qx = nx >> 2, row = q / qx, x0 = (q - row * qx) << 2, z = row / ny, y = row - z * ny;
+  const int c = row * nx + x0, iu = row * (nx + 1) + x0, iv = (y + (ny + 1) * z) * nx + x0, sz = nx * ny;
+  const uint32_t fw = row_flags(flags, x0, y, z, nx, ny, nz, c);
+  float uu[5];
+#pragma unroll
+  for (int k = 0; k < 5; ++k) uu[k] = u[iu + k];
+  const float4 v0 = ld4f(v, iv), v1 = ld4f(v, iv + nx), w0 = ld4f(w, c), w1 = ld4f(w, c + sz);
+  float d[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const uint8_t f = uint8_t(fw >> (8 * k));
+    d[k] = 0.0f;
+    if (f & FLAG_FLUID) {
+      const float a0 = (f & FLAG_XLO) ? uu[k] : 0.0f, a1 = (f & FLAG_XHI) ? uu[k + 1] : 0.0f;
+      const float b0 = (f & FLAG_YLO) ? comp(v0, k) : 0.0f, b1 = (f & FLAG_YHI) ? comp(v1, k) : 0.0f;
+      const float c0 = (f & FLAG_ZLO) ? comp(w0, k) : 0.0f, c1 = (f & FLAG_ZHI) ? comp(w1, k) : 0.0f;
+      d[k] = (a1 - a0) * ix + (b1 - b0) * iy + (c1 - c0) * iz;
+    }
+  }
+  st4f(div, c, make_float4(d[0], d[1], d[2], d[3]));
+}
+
+// Gradient correction of the faces at the low sides of the lane's cells (and
+// the closed faces at x = nx, y = ny, z = nz next to the last cells); with dp
+// the neighbour differences of p and dp are formed separately, and psum (if
+// given) receives p + dp.
+__global__ __launch_bounds__(256) void k_gradient_q(int nx, int ny, int nz, float ix, float iy, float iz, float scale,
+                                                    const uint8_t* __restrict__ flags, const float* __restrict__ p,
+                                                    const float* __restrict__ dp, const float* __restrict__ u,
+                                                    const float* __restrict__ v, const float* __restrict__ w,
+                                                    float* __restrict__ uo, float* __restrict__ vo,
+                                                    float* __restrict__ wo, float* __restrict__ psum, int nq) {
+  const int q = xcd_row_tile(blockIdx.x, gridDim.x) * 256 + threadIdx.x;
+  if (q >= nq) return;
+  const int qx = nx >> 2, row = q / qx, x0 = (q - row * qx) << 2, z = row / ny, y = row - z * ny;
+  const int c = row * nx + x0, iu = row * (nx + 1) + x0, iv = (y + (ny + 1) * z) * nx + x0, sz = nx * ny;
+  const uint32_t fw = row_flags(flags, x0, y, z, nx, ny, nz, c);
+  const float4 pc = ld4f(p, c);
+  const float pxm = x0 > 0 ? p[c - 1] : 0.0f;
+  const float4 pym = y > 0 ? ld4f(p, c - nx) : make_float4(0.f, 0.f, 0.f, 0.f);
+  const float4 pzm = z > 0 ? ld4f(p, c - sz) : make_float4(0.f, 0.f, 0.f, 0.f);
+  float4 dc = make_float4(0.f, 0.f, 0.f, 0.f), dym = dc, dzm = dc;
+  float dxm = 0.0f;
+  if (dp) {
+    dc = ld4f(dp, c);
+    dxm = x0 > 0 ? dp[c - 1] : 0.0f;
+    if (y > 0) dym = ld4f(dp, c - nx);
+    if (z > 0) dzm = ld4f(dp, c - sz);
+  }
+  float uu[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) uu[k] = u[iu + k];
+  const float4 vv = ld4f(v, iv), ww = ld4f(w, c);
+  float ur[4], vr[4], wr[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const uint8_t f = uint8_t(fw >> (8 * k));
+    const float pk = comp(pc, k);
+    // x face at x0 + k: open iff interior and the cell's low-x bit is set
+    ur[k] = 0.0f;
+    if (x0 + k > 0 && (f & FLAG_XLO)) {
+      float g = pk - (k == 0 ? pxm : comp(pc, k - 1));
+      if (dp) g += comp(dc, k) - (k == 0 ? dxm : comp(dc, k - 1));
+      ur[k] = uu[k] - scale * g * ix;
+    }
+    vr[k] = 0.0f;
+    if (y > 0 && (f & FLAG_YLO)) {
+      float g = pk - comp(pym, k);
+      if (dp) g += comp(dc, k) - comp(dym, k);
+      vr[k] = comp(vv, k) - scale * g * iy;
+    }
+    wr[k] = 0.0f;
+    if (z > 0 && (f & FLAG_ZLO)) {
+      float g = pk - comp(pzm, k);
+      if (dp) g += comp(dc, k) - comp(dzm, k);
+      wr[k] = comp(ww, k) - scale * g * iz;
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < 4; ++k) uo[iu + k] = ur[k];
+  if (x0 + 4 == nx) uo[iu + 4] = 0.0f;
+  st4f(vo, iv, make_float4(vr[0], vr[1], vr[2], vr[3]));
+  if (y == ny - 1) st4f(vo, iv + nx, make_float4(0.f, 0.f, 0.f, 0.f));
+  st4f(wo, c, make_float4(wr[0], wr[1], wr[2], wr[3]));
+  if (z == nz - 1) st4f(wo, c + sz, make_float4(0.f, 0.f, 0.f, 0.f));
+  if (psum) st4f(psum, c, make_float4(pc.x + dc.x, pc.y + dc.y, pc.z + dc.z, pc.w + dc.w));
+}
+
 __global__ void k_combine(int64_t n, const float* __restrict__ a, const float* __restrict__ b, float* __restrict__ out) {
   int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i < n) out[i] = a[i] + b[i];
@@ -225,6 +349,34 @@ __global__ void k_seed(int nx, int ny, int nz, float ex, float ey, float ez, int
   float qx = px - 0.5f * ex, qy = py - 0.5f * ey;
   float v = a == 0 ? bx - omega * qy : (a == 1 ? by + omega * qx : bz);
   out[idx3(x, y, z, sx, sy)] = v;
+}
+
+// host: can the row-vector divergence/gradient kernels run?
+static bool rows_vec_ok(const fx_grid& g, const void* a, const void* b, const void* c, const void* d,
+                        const void* e = nullptr, const void* f = nullptr) {
+  auto al = [](const void* ptr) { return (reinterpret_cast<uintptr_t>(ptr) & 15u) == 0; };
+  return g.nx % 4 == 0 && al(a) && al(b) && al(c) && al(d) && al(e) && al(f);
+}
+
+static int launch_gradient(const fx_grid& g, const uint8_t* flags, const float* p, const float* dp, const float* u,
+                           const float* v, const float* w, float* uo, float* vo, float* wo, float* psum,
+                           float scale, hipStream_t s, const char* name) {
+  const float ix = g.nx / g.ex, iy = g.ny / g.ey, iz = g.nz / g.ez;
+  if (rows_vec_ok(g, flags, p, dp, v, w, vo) && rows_vec_ok(g, wo, psum, nullptr, nullptr)) {
+    const int nq = g.nx / 4 * g.ny * g.nz;
+    hipLaunchKernelGGL(k_gradient_q, dim3((nq + 255) / 256), dim3(256), 0, s, g.nx, g.ny, g.nz, ix, iy, iz, scale,
+                       flags, p, dp, u, v, w, uo, vo, wo, psum, nq);
+  } else {
+    dim3 gr = grid3(g.nx + 1, g.ny + 1, g.nz + 1, kBlock);
+    hipLaunchKernelGGL(k_gradient, gr, kBlock, 0, s, g.nx, g.ny, g.nz, ix, iy, iz, scale, flags, p, dp, u, v, w, uo,
+                       vo, wo);
+    if (psum) {
+      const int64_t n = int64_t(g.nx) * g.ny * g.nz;
+      hipLaunchKernelGGL(k_combine, dim3((n + 255) / 256), dim3(256), 0, s, n, p, dp, psum);
+    }
+  }
+  FX_LAUNCHED(name);
+  return FX_OK;
 }
 
 }  // namespace fx
@@ -308,9 +460,15 @@ int fx_divergence(fx_grid g, const uint8_t* flags, const float* u, const float* 
                   void* stream) {
   if (int e = check_grid(g)) return e;
   FX_REQUIRE(u && v && w && div, FX_EINVAL, "null array");
-  dim3 gr = grid3(g.nx, g.ny, g.nz, kBlock);
-  hipLaunchKernelGGL(k_divergence, gr, kBlock, 0, S(stream), g.nx, g.ny, g.nz, g.nx / g.ex, g.ny / g.ey,
-                     g.nz / g.ez, flags, u, v, w, div);
+  if (rows_vec_ok(g, flags, v, w, div)) {
+    const int nq = g.nx / 4 * g.ny * g.nz;
+    hipLaunchKernelGGL(k_divergence_q, dim3((nq + 255) / 256), dim3(256), 0, S(stream), g.nx, g.ny, g.nz,
+                       g.nx / g.ex, g.ny / g.ey, g.nz / g.ez, flags, u, v, w, div, nq);
+  } else {
+    dim3 gr = grid3(g.nx, g.ny, g.nz, kBlock);
+    hipLaunchKernelGGL(k_divergence, gr, kBlock, 0, S(stream), g.nx, g.ny, g.nz, g.nx / g.ex, g.ny / g.ey,
+                       g.nz / g.ez, flags, u, v, w, div);
+  }
   FX_LAUNCHED("fx_divergence");
   return FX_OK;
 }
@@ -333,23 +491,17 @@ int fx_gradient(fx_grid g, const uint8_t* flags, const float* p, const float* u,
   if (int e = check_grid(g)) return e;
   FX_REQUIRE(p && u && v && w && uo && vo && wo, FX_EINVAL, "null array");
   FX_REQUIRE(uo != u && vo != v && wo != w, FX_EINVAL, "Advection input/output textures must be distinct");
-  dim3 gr = grid3(g.nx + 1, g.ny + 1, g.nz + 1, kBlock);
-  hipLaunchKernelGGL(k_gradient, gr, kBlock, 0, S(stream), g.nx, g.ny, g.nz, g.nx / g.ex, g.ny / g.ey,
-                     g.nz / g.ez, scale, flags, p, (const float*)nullptr, u, v, w, uo, vo, wo);
-  FX_LAUNCHED("fx_gradient");
-  return FX_OK;
+  return launch_gradient(g, flags, p, nullptr, u, v, w, uo, vo, wo, nullptr, scale, S(stream), "fx_gradient");
 }
 
 int fx_gradient_split(fx_grid g, const uint8_t* flags, const float* p, const float* dp, const float* u,
-                      const float* v, const float* w, float* uo, float* vo, float* wo, float scale, void* stream) {
+                      const float* v, const float* w, float* uo, float* vo, float* wo, float* psum, float scale,
+                      void* stream) {
   if (int e = check_grid(g)) return e;
   FX_REQUIRE(p && dp && u && v && w && uo && vo && wo, FX_EINVAL, "null array");
   FX_REQUIRE(uo != u && vo != v && wo != w, FX_EINVAL, "Advection input/output textures must be distinct");
-  dim3 gr = grid3(g.nx + 1, g.ny + 1, g.nz + 1, kBlock);
-  hipLaunchKernelGGL(k_gradient, gr, kBlock, 0, S(stream), g.nx, g.ny, g.nz, g.nx / g.ex, g.ny / g.ey,
-                     g.nz / g.ez, scale, flags, p, dp, u, v, w, uo, vo, wo);
-  FX_LAUNCHED("fx_gradient_split");
-  return FX_OK;
+  FX_REQUIRE(psum != p && psum != dp, FX_EINVAL, "gradient: the pressure sum must not alias p or dp");
+  return launch_gradient(g, flags, p, dp, u, v, w, uo, vo, wo, psum, scale, S(stream), "fx_gradient_split");
 }
 
 int fx_combine(const float* a, const float* b, float* out, int64_t n, void* stream) {

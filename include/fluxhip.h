@@ -117,9 +117,11 @@ int fx_gradient(fx_grid g, const uint8_t* flags, const float* p, const float* u,
 
 /* As fx_gradient with the pressure split p + dp: the neighbour differences of
  * both parts are formed before they are added, so an impulse carried as
- * warm start p plus solver increment dp keeps its full accuracy in fp32. */
+ * warm start p plus solver increment dp keeps its full accuracy in fp32.
+ * psum (may be NULL, must not alias p or dp) receives p + dp in the same pass. */
 int fx_gradient_split(fx_grid g, const uint8_t* flags, const float* p, const float* dp, const float* u,
-                      const float* v, const float* w, float* uo, float* vo, float* wo, float scale, void* stream);
+                      const float* v, const float* w, float* uo, float* vo, float* wo, float* psum, float scale,
+                      void* stream);
 
 
 /* out = a + b (n floats). */

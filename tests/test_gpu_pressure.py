@@ -53,6 +53,34 @@ def test_walls_divergence_gradient(device, shape, extent, masked):
         assert rel_err(g, o) < TOL
 
 
+@pytest.mark.parametrize("shape", [(9, 7, 5), (16, 12, 8)])
+@pytest.mark.parametrize("masked", [False, True])
+def test_gradient_split_writes_sum(device, shape, masked):
+    """fx_gradient_split (row-vector kernel when nx % 4 == 0, one-cell kernel
+    otherwise): faces match the FP64 gradient of p + dp, psum = p + dp exactly."""
+    from fluxfx_amd import _lib
+    from fluxfx_amd.backend.pressure import PressureProjector
+    from fluxfx_amd.backend.device import grid_struct
+    extent = (1.3, .8, 1.7)
+    grid, og, rng, solid, solids = _setup(device, shape, extent, masked, seed=4)
+    fields, vel = _upload_faces(device, grid, random_faces(og, rng))
+    vel = mac.walls(vel, og, solid)
+    pp = PressureProjector(grid, device=device, solids=solids)
+    outs = [device.texture(s) for s in grid.face_shapes]
+    p = rng.standard_normal(og.array_shape).astype(np.float32)
+    dp = (1e-3 * rng.standard_normal(og.array_shape)).astype(np.float32)
+    tp = device.torch.from_numpy(p.ravel()).to(device.device)
+    tdp = device.torch.from_numpy(dp.ravel()).to(device.device)
+    tsum = device.torch.full_like(tp, 7.0)
+    _lib.call("fx_gradient_split", grid_struct(grid), pp.flags_ptr, tp.data_ptr(), tdp.data_ptr(),
+              *(f.ptr for f in fields), *(o.ptr for o in outs), tsum.data_ptr(), 0.37, device.stream)
+    device.torch.cuda.synchronize()
+    assert np.array_equal(tsum.cpu().numpy().reshape(og.array_shape), p + dp)
+    ref = mac.gradient_correct(vel, p.astype(np.float64) + dp.astype(np.float64), og, 0.37, solid)
+    for g, o in zip(faces_np(og, outs), ref):
+        assert rel_err(g, o) < TOL
+
+
 @pytest.mark.parametrize("masked", [False, True])
 def test_jacobi_projection_matches_oracle(device, masked):
     from fluxfx_amd.physics.pressure import PressureSettings
