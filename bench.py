@@ -37,6 +37,8 @@ sys.path.insert(0, str(ROOT))
 METRIC = "256³ smoke steps/s at 1e-5 rel. residual: learned+mixed vs plain FP32 MG"
 WEIGHTS = ROOT / "fluxfx_amd" / "weights" / "unet_128.npz"
 HBM_PEAK_GBS = 8000.0
+LEARNED_LEVEL = 1
+LEARNED_ITERATIONS = 1
 
 
 def scene(n, seed=3):
@@ -94,8 +96,10 @@ def settings_for(n, mode):
     elif mode == "learned":
         if not WEIGHTS.exists():
             raise NotImplementedError(f"no trained U-Net weights at {WEIGHTS.relative_to(ROOT)}")
+        # the 128^3-trained network runs one level down (on the 8-child mean of the
+        # 256^3 residual), its output prolonged and smoothed like a V-cycle correction
         kw.update(pressure_precision="BF16", pressure_krylov="LEARNED", learned_weights=str(WEIGHTS),
-                  learned_iterations=1, learned_sweeps=2)
+                  learned_iterations=LEARNED_ITERATIONS, learned_sweeps=2, learned_level=LEARNED_LEVEL)
     return PressureSettings(**kw), dt
 
 

@@ -1,4 +1,5 @@
-"""MacCormack predictor/reverse passes (backend/detail.py:16-37).
+"""MacCormack predictor/reverse passes and vorticity confinement
+(backend/detail.py:16-49).
 
 The projected step uses the fused multi-channel predictor and a correct kernel
 that samples the reverse trace itself; these methods keep the reference's
@@ -20,6 +21,7 @@ class DetailPasses:
         self.predictor = self.reverse = None
         self.face_predictor = None
         self.face_bounds = None
+        self.curl = None
         self._scalar_pred = []
         self._scalar_bounds = []
 
@@ -30,7 +32,7 @@ class DetailPasses:
         return (8 * cells if self.predictor is not None else 0) + \
             (4 * faces if self.face_predictor is not None else 0) + \
             (8 * faces if self.face_bounds is not None else 0) + 4 * cells * len(self._scalar_pred) + \
-            8 * cells * len(self._scalar_bounds)
+            8 * cells * len(self._scalar_bounds) + (16 * cells if self.curl is not None else 0)
 
     @property
     def _solid(self):
@@ -108,9 +110,18 @@ class DetailPasses:
         return self.face_predictor, self.face_bounds
 
     def confine(self, fields, outputs, dt, strength, limit):
-        raise NotImplementedError("vorticity confinement is not part of this build")
+        """Vorticity confinement (backend/detail.py:39-49): curl into the
+        four-channel ``curl`` field, then every face component of ``fields``
+        plus dt times the bounded confinement force into ``outputs``."""
+        if any(o is f for o in outputs for f in fields):
+            raise ValueError("Advection input/output textures must be distinct")
+        if self.curl is None:
+            self.curl = self.device.texture(self.grid.shape, channels=4, nonnegative=False)
+        _lib.call("fx_curl", self._g, *(ptr(f) for f in fields), ptr(self.curl), self.device.stream)
+        _lib.call("fx_confine", self._g, *(ptr(f) for f in fields), ptr(self.curl), *(ptr(o) for o in outputs),
+                  float(dt), float(strength), float(limit), self.device.stream)
 
     def close(self):
-        self.predictor = self.reverse = self.face_predictor = self.face_bounds = None
+        self.predictor = self.reverse = self.face_predictor = self.face_bounds = self.curl = None
         self._scalar_pred = []
         self._scalar_bounds = []

@@ -9,7 +9,8 @@ flag.  Step order (backend/projected.py:149-202):
      departure samples
   2. fused velocity transport + jet relaxation + buoyancy from the
      pre-transport density/temperature            -> provisional U, V, W
-  3. projection into the old velocity storage
+  3. optional vorticity confinement, then projection into the old velocity
+     storage
   4. fused multi-channel scalar predictor (scalar MacCormack)
   5. fused scalar transport: reverse trace, correction, clamp, decay, sources
   6. fused combustion
@@ -26,6 +27,7 @@ from .combustion import CombustionPasses
 from .detail import DetailPasses
 from .device import HipDevice, ptr
 from .mac import DenseThermalAdvection, source_row
+from .moving_collision import MovingPressureProjector, MovingSolidMask
 from .multigrid import MultigridPressureProjector, use_multigrid
 from .pressure import PressureProjector, SolidPressureProjector
 from .solid_multigrid import SolidMultigridProjector
@@ -54,9 +56,16 @@ class DenseProjectedSmoke(DenseThermalAdvection):
         settings = settings or PressureSettings()
         grid = grid or GridSpec()
         device = device or HipDevice()
-        if settings.moving_colliders:
-            raise ValueError("Moving colliders are not part of this build")
-        if colliders or solid_mask is not None:
+        if settings.moving_colliders and colliders:
+            # moving rigid colliders: owner mask, wall velocities, masked Jacobi
+            # with moving walls (backend/projected.py:17-29)
+            if solid_mask is not None:
+                raise ValueError("Moving colliders are analytic shapes; a voxel mask is static")
+            if settings.pressure_tolerance > 0:
+                raise ValueError("Moving colliders use the masked Jacobi projector; set pressure_tolerance to 0")
+            self.solids = MovingSolidMask(grid, colliders, device)
+            projector_class = projector_class or MovingPressureProjector
+        elif colliders or solid_mask is not None:
             self.solids = SolidMask(grid, colliders, device, mask=solid_mask)
             if settings.pressure_tolerance <= 0:
                 projector_class = None
@@ -129,7 +138,21 @@ class DenseProjectedSmoke(DenseThermalAdvection):
         self.projector.settings = settings
 
     def move_colliders(self, colliders, dt):
-        raise ValueError("Enable Moving colliders and Reset first")
+        """Move the rigid colliders to new poses reached over dt before a step:
+        new mask and wall velocities, then cells and faces whose solidity
+        changed are emptied (backend/projected.py:103-115)."""
+        if not self.settings.moving_colliders or not getattr(self.solids, "moving", False):
+            raise ValueError("Enable Moving colliders and Reset first")
+        validate_dt(dt)
+        if self.faulted:
+            raise RuntimeError("Reset required after a failed GPU step")
+        try:
+            self.solids.update(colliders, dt)
+            self.solids.remap(self)
+            self.projector.ready = False
+        except Exception:
+            self.faulted = True
+            raise
 
     def measure_projection(self):
         return self.projector.metrics() | {"step": self.steps, "simulation_time": self.time}
@@ -163,8 +186,8 @@ class DenseProjectedSmoke(DenseThermalAdvection):
 
     def _step(self, dt, emission, emitters=None):
         s = self.settings
-        if s.turbulence_strength > 0 or s.vorticity_strength > 0:
-            raise NotImplementedError("turbulence and vorticity confinement are not part of this build")
+        if s.turbulence_strength > 0:
+            raise NotImplementedError("turbulence is not part of this build")
         table = self._multi_sources if emitters is not None else self._single_source(emission)
         solid = self.solids.solid if self.solids is not None else None
         clearance = self.solids.clearance if self.solids is not None else None
@@ -178,7 +201,14 @@ class DenseProjectedSmoke(DenseThermalAdvection):
                               lift=s.thermal_lift, weight=s.density_weight, solid=solid, clearance=clearance,
                               bounds=bounds)
         # the old velocity is consumed: its storage receives the projected field
-        self.projector.project(provisional, self._velocity, dt)
+        if s.vorticity_strength > 0:
+            # confined field into the old velocity storage, projected into the back buffers
+            # (backend/projected.py:172-175)
+            self.detail.confine(provisional, self._velocity, dt, s.vorticity_strength, s.vorticity_limit)
+            self.projector.project(self._velocity, self._velocity_back, dt)
+            self._velocity, self._velocity_back = self._velocity_back, self._velocity
+        else:
+            self.projector.project(provisional, self._velocity, dt)
         fields = [self._front, self._temperature]
         outs = [self._back, self._temperature_back]
         decays = [s.dissipation, s.cooling]

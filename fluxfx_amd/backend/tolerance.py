@@ -214,9 +214,10 @@ class ToleranceProjector(PressureProjector):
         arr = eng.arrays[0]
         used = {eng.cfg.level[0].p, eng.cfg.level[0].q}
         zold = next(t for t in (arr["z"], arr["p"], arr["q"]) if t.data_ptr() not in used)
+        # d2 is free during a Krylov solve: second direction buffer for the fused update
         _lib.call("fx_pcg_solve", C.byref(eng.cfg), inc["d"].ptr, inc["r0"].ptr, v["r"].data_ptr(),
-                  v["p"].data_ptr(), v["Ap"].data_ptr(), zold.data_ptr(), C.byref(params), self.ws.data_ptr(),
-                  C.byref(info), self.device.stream)
+                  v["p"].data_ptr(), v["Ap"].data_ptr(), zold.data_ptr(), inc["d2"].ptr, C.byref(params),
+                  self.ws.data_ptr(), C.byref(info), self.device.stream)
         return info
 
     def attach_network(self, network):
@@ -243,20 +244,38 @@ class ToleranceProjector(PressureProjector):
             self._kvec = {k: self.device.zeros(n) for k in ("r", "p", "Ap", "zold", "z", "rT", "ztmp")}
         v = self._kvec
         inc = self._inc
-        buf = net.buffers(self.grid.shape)
+        # the network runs on level 0 or on level 1 (learned_level)
+        level = int(self.settings.learned_level)
+        shape = self.shapes[level]
+        buf = net.buffers(shape)
         arr = eng.arrays[0]
         used = {eng.cfg.level[0].p, eng.cfg.level[0].q}
         mg_zold = next(t for t in (arr["z"], arr["p"], arr["q"]) if t.data_ptr() not in used)
-        w = [(n / e) ** 2 for n, e in zip(self.grid.shape, self.grid.extent)]
+        w = [(n / e) ** 2 for n, e in zip(shape, self.grid.extent)]
         post = 3.0 / sum(w)
-        solid = self.solids.solid.data_ptr() if self.solids is not None else None
+        solid = self._network_mask(level)
         info = _lib.fx_solve_info()
         _lib.call("fx_learned_pcg_solve", C.byref(buf["net"]), C.byref(eng.cfg), inc["d"].ptr, inc["r0"].ptr,
                   v["r"].data_ptr(), v["p"].data_ptr(), v["Ap"].data_ptr(), v["zold"].data_ptr(),
-                  v["z"].data_ptr(), v["rT"].data_ptr(), v["ztmp"].data_ptr(), mg_zold.data_ptr(), solid,
+                  v["z"].data_ptr(), v["rT"].data_ptr(), v["ztmp"].data_ptr(), mg_zold.data_ptr(), inc["d2"].ptr,
+                  solid,
                   float(post), int(self.settings.learned_iterations), int(self.settings.learned_sweeps),
                   C.byref(params), self.ws.data_ptr(), C.byref(info), self.device.stream)
         return info
+
+    def _network_mask(self, level):
+        """Solid mask on the network's grid: level 0's, or on level 1 a cell is
+        solid when all eight children are (None without solids)."""
+        if self.solids is None:
+            return None
+        if level == 0:
+            return self.solids.solid.data_ptr()
+        if getattr(self, "_coarse_solid", None) is None:
+            nx, ny, nz = self.grid.shape
+            fine = self.solids.numpy().reshape(nz // 2, 2, ny // 2, 2, nx // 2, 2)
+            coarse = fine.all(axis=(1, 3, 5)).astype(np.uint8)
+            self._coarse_solid = self.device.torch.from_numpy(coarse.ravel()).to(self.device.device)
+        return self._coarse_solid.data_ptr()
 
     def solve_metrics(self):
         i = self.last_info

@@ -21,6 +21,8 @@ FLAGS = [
     # packed fp32 instructions are kept out of the generated code
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-munsafe-fp-atomics", "-Wall", "-Wno-unused-function", "-Wno-unused-variable",
+    # a value-returning function that can fall off its end is an error, not a warning
+    "-Werror=return-type",
 ]
 
 

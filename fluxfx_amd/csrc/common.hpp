@@ -59,11 +59,8 @@ template <>
 __device__ __forceinline__ float from_f<float>(float v) { return v; }
 template <>
 __device__ __forceinline__ bf16 from_f<bf16>(float v) {
-  // round to nearest even; NaN stays NaN
-  uint32_t u = __float_as_uint(v);
-  if ((u & 0x7f800000u) == 0x7f800000u && (u & 0x007fffffu)) return bf16{uint16_t((u >> 16) | 0x40)};
-  u += 0x7fffu + ((u >> 16) & 1u);
-  return bf16{uint16_t(u >> 16)};
+  // round to nearest even, NaN stays NaN: gfx950's hardware conversion
+  return bf16{__builtin_bit_cast(uint16_t, (__bf16)v)};
 }
 template <>
 __device__ __forceinline__ _Float16 from_f<_Float16>(float v) { return _Float16(v); }
@@ -130,6 +127,7 @@ __device__ __forceinline__ void block_partial(double v, double* ws, int q) {
 
 // host helpers (reduce.hip)
 void ws_finalize(double* ws, int q0, int nq, hipStream_t s);
+void ws_finalize_mask(double* ws, uint32_t mask, hipStream_t s);
 int ws_read(const double* ws, int q0, int nq, double* host, hipStream_t s);
 
 inline dim3 grid3(int nx, int ny, int nz, dim3 block) {

@@ -16,17 +16,23 @@ __device__ __forceinline__ float jacobi_update(float c, float s, float b, float 
 }
 // zero_guess: p is known to be 0 (fresh coarse level or error equation)
 // ---------------------------------------------------------------------------
+// One damped-Jacobi update of cell i (correction form of p + omega*((sum w p_n - rhs)/d - p)).
+template <typename T, int OP>
+__device__ __forceinline__ float smooth_cell(const LevelDev& L, int x, int y, int z, int64_t i, const T* p,
+                                             const T* __restrict__ rhs, float inv_dt, float omega, bool read_p) {
+  Stencil st = stencil<OP>(L, x, y, z, i, p, read_p);
+  float r = 0.0f;
+  if (st.active && st.d > 0.0f) r = jacobi_update(st.c, st.s, to_f(rhs[i]), st.d, inv_dt, omega);
+  return r;
+}
+
 template <typename T, int OP>
 __global__ void k_smooth(LevelDev L, const T* __restrict__ p, const T* __restrict__ rhs, T* __restrict__ out,
                          float inv_dt, float omega, int zero_guess) {
   int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y * blockDim.y + threadIdx.y, z = blockIdx.z;
   if (x >= L.nx || y >= L.ny) return;
   int64_t i = idx3(x, y, z, L.nx, L.ny);
-  Stencil st = stencil<OP>(L, x, y, z, i, p, !zero_guess);
-  float r = 0.0f;
-  // correction form of p + omega*((sum w p_n - rhs)/d - p)
-  if (st.active && st.d > 0.0f) r = jacobi_update(st.c, st.s, to_f(rhs[i]), st.d, inv_dt, omega);
-  out[i] = from_f<T>(r);
+  out[i] = from_f<T>(smooth_cell<T, OP>(L, x, y, z, i, p, rhs, inv_dt, omega, !zero_guess));
 }
 
 // Level-0 smoother for box/flags operators: each thread marches a column of
@@ -82,13 +88,14 @@ __global__ __launch_bounds__(256) void k_smooth_q(LevelDev L, const T* __restric
   const int q = xcd_tile(blockIdx.x, gridDim.x) * 256 + threadIdx.x;
   if (q >= nq) return;
   const QuadPos at = vec_pos(L, q, W);
+  const FV<W> b = VecIO<T, W>::ld(rhs, at.i);  // issued with the stencil loads
   const VecStencil<W> st = vec_stencil<OP, W>(L, at.x, at.y, at.z, at.i, PLoad<T>{p}, !zero_guess);
-  const FV<W> b = VecIO<T, W>::ld(rhs, at.i);
   FV<W> r;
 #pragma unroll
   for (int k = 0; k < W; ++k) {
-    r.v[k] = 0.0f;
-    if ((st.f[k] & FLAG_FLUID) && st.d[k] > 0.0f) r.v[k] = jacobi_update(st.c[k], st.s[k], b.v[k], st.d[k], inv_dt, omega);
+    // evaluated for every cell and selected (no per-cell branch)
+    const float u = jacobi_update(st.c[k], st.s[k], b.v[k], st.d[k], inv_dt, omega);
+    r.v[k] = ((st.f[k] & FLAG_FLUID) && st.d[k] > 0.0f) ? u : 0.0f;
   }
   VecIO<T, W>::st(out, at.i, r);
 }
@@ -319,15 +326,12 @@ __global__ void k_restrict_full(int cnx, int cny, int cnz, int nx, int ny, int n
 // vector kernel (whose products the compiler could otherwise share between
 // cells and leave unfused) gives the same bits.
 template <typename T, int FOP, int COP, int MODE>
-__global__ void k_prolong(LevelDev F, LevelDev C, const T* in, const T* __restrict__ coarse, T* out) {
-  int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y * blockDim.y + threadIdx.y, z = blockIdx.z;
-  if (x >= F.nx || y >= F.ny) return;
-  int64_t i = idx3(x, y, z, F.nx, F.ny);
+__device__ __forceinline__ float prolong_cell(const LevelDev& F, const LevelDev& C, int x, int y, int z, int64_t i,
+                                              float in_val, const T* __restrict__ coarse) {
   if (MODE == FX_PROLONG_CONSTANT) {
     float r = 0.0f;
-    if (cell_active<FOP>(F, x, y, z, i)) r = to_f(in[i]) + to_f(coarse[idx3(x >> 1, y >> 1, z >> 1, C.nx, C.ny)]);
-    out[i] = from_f<T>(r);
-    return;
+    if (cell_active<FOP>(F, x, y, z, i)) r = in_val + to_f(coarse[idx3(x >> 1, y >> 1, z >> 1, C.nx, C.ny)]);
+    return r;
   }
   // cell-centred trilinear: q = (c+0.5)*0.5-0.5, base = floor(q)
   int bx = (x >> 1) - ((x & 1) ? 0 : 1), by = (y >> 1) - ((y & 1) ? 0 : 1), bz = (z >> 1) - ((z & 1) ? 0 : 1);
@@ -356,9 +360,86 @@ __global__ void k_prolong(LevelDev F, LevelDev C, const T* in, const T* __restri
   }
   float corr = num;
   if (MODE == FX_PROLONG_RENORM) corr = den > 1e-12f ? num / den : 0.0f;
-  float r = to_f(in[i]) + corr;
+  float r = in_val + corr;
   if (FOP != FX_OP_BOX && !cell_active<FOP>(F, x, y, z, i)) r = 0.0f;
-  out[i] = from_f<T>(r);
+  return r;
+}
+
+template <typename T, int FOP, int COP, int MODE>
+__global__ void k_prolong(LevelDev F, LevelDev C, const T* in, const T* __restrict__ coarse, T* out) {
+  int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y * blockDim.y + threadIdx.y, z = blockIdx.z;
+  if (x >= F.nx || y >= F.ny) return;
+  int64_t i = idx3(x, y, z, F.nx, F.ny);
+  out[i] = from_f<T>(prolong_cell<T, FOP, COP, MODE>(F, C, x, y, z, i, to_f(in[i]), coarse));
+}
+
+// ---------------------------------------------------------------------------
+// Small levels (at most kTailCells cells) in one block of 1024 threads: the
+// level's iterate lives in LDS in its storage type (rounded after every sweep
+// exactly as the level arrays are), so a level's pre-smoothing + residual
+// restriction, and its prolongation + post-smoothing, are one launch each
+// instead of one launch per sweep.  Per-cell arithmetic: smooth_cell,
+// residual_at and prolong_cell, shared with the one-cell kernels.
+// ---------------------------------------------------------------------------
+constexpr int kTailCells = 4096;
+constexpr int kTailThreads = 1024;
+
+template <typename T, int OP>
+__device__ __forceinline__ int tail_sweeps(const LevelDev& L, T (&buf)[2][kTailCells], int cur, const T* __restrict__ rhs,
+                                           float inv_dt, float omega, int sweeps, bool zero_first) {
+  const int n = L.nx * L.ny * L.nz;
+  for (int k = 0; k < sweeps; ++k) {
+    for (int i = threadIdx.x; i < n; i += kTailThreads) {
+      const int x = i % L.nx, r = i / L.nx, y = r % L.ny, z = r / L.ny;
+      buf[cur ^ 1][i] = from_f<T>(smooth_cell<T, OP>(L, x, y, z, i, buf[cur], rhs, inv_dt, omega,
+                                                     !(zero_first && k == 0)));
+    }
+    __syncthreads();
+    cur ^= 1;
+  }
+  return cur;
+}
+
+// pre-smoothing from p_in (or from zero) stored to pout, then crhs = 8-child mean residual
+template <typename T, int OP>
+__global__ __launch_bounds__(kTailThreads) void k_tail_down(LevelDev L, int cnx, int cny, int cnz, const T* p_in,
+                                                            const T* __restrict__ rhs, T* pout, T* __restrict__ crhs,
+                                                            float inv_dt, float omega, int sweeps, int zero_guess) {
+  __shared__ T buf[2][kTailCells];
+  const int n = L.nx * L.ny * L.nz;
+  for (int i = threadIdx.x; i < n; i += kTailThreads) buf[0][i] = zero_guess ? from_f<T>(0.0f) : p_in[i];
+  __syncthreads();
+  const int cur = tail_sweeps<T, OP>(L, buf, 0, rhs, inv_dt, omega, sweeps, zero_guess != 0);
+  for (int i = threadIdx.x; i < n; i += kTailThreads) pout[i] = buf[cur][i];
+  const int cn = cnx * cny * cnz;
+  for (int c = threadIdx.x; c < cn; c += kTailThreads) {
+    const int X = c % cnx, r = c / cnx, Y = r % cny, Z = r / cny;
+    float acc = 0.0f;
+#pragma unroll
+    for (int dz = 0; dz < 2; ++dz)
+#pragma unroll
+      for (int dy = 0; dy < 2; ++dy)
+#pragma unroll
+        for (int dx = 0; dx < 2; ++dx)
+          acc += residual_at<T, OP>(L, 2 * X + dx, 2 * Y + dy, 2 * Z + dz, buf[cur], rhs, inv_dt);
+    crhs[c] = from_f<T>(acc * 0.125f);
+  }
+}
+
+// p = prolong(p_in, coarse), then post-smoothing, stored to pout
+template <typename T, int FOP, int COP, int MODE>
+__global__ __launch_bounds__(kTailThreads) void k_tail_up(LevelDev F, LevelDev C, const T* p_in,
+                                                          const T* __restrict__ coarse, const T* __restrict__ rhs,
+                                                          T* pout, float inv_dt, float omega, int sweeps) {
+  __shared__ T buf[2][kTailCells];
+  const int n = F.nx * F.ny * F.nz;
+  for (int i = threadIdx.x; i < n; i += kTailThreads) {
+    const int x = i % F.nx, r = i / F.nx, y = r % F.ny, z = r / F.ny;
+    buf[0][i] = from_f<T>(prolong_cell<T, FOP, COP, MODE>(F, C, x, y, z, i, to_f(p_in[i]), coarse));
+  }
+  __syncthreads();
+  const int cur = tail_sweeps<T, FOP>(F, buf, 0, rhs, inv_dt, omega, sweeps, false);
+  for (int i = threadIdx.x; i < n; i += kTailThreads) pout[i] = buf[cur][i];
 }
 
 // p = scale * M rhs, one wavefront per row, FP64 accumulation
@@ -589,12 +670,75 @@ static int launch_fill_zero(fx_mg_config* c, void* ptr, int64_t n, hipStream_t s
   return FX_OK;
 }
 
+// level l runs its pre- and post-smoothing in single-block tail kernels
+static bool tail_ok(const fx_mg_config* c, int l) {
+  const fx_mg_level& L = c->level[l];
+  return !(c->options & FX_MG_NO_TAIL) && l + 1 < c->levels && c->restriction == FX_RESTRICT_MEAN && int64_t(L.nx) * L.ny * L.nz <= kTailCells &&
+         c->prolongation >= 0 && c->prolongation <= 2 && c->pre >= 0 && c->post >= 0 &&
+         (c->prolongation != FX_PROLONG_RENORM || c->level[l + 1].op != FX_OP_FLAGS);
+}
+
+// host side of smooth_n's ping-pong: the array holding the iterate after `count` sweeps
+static void *tail_swap(fx_mg_level& L, int count) {
+  if (count & 1) {
+    void* t = L.p;
+    L.p = L.q;
+    L.q = t;
+  }
+  return L.p;
+}
+
+static int launch_tail_down(fx_mg_config* c, int l, float inv_dt, bool zero_guess, hipStream_t s) {
+  fx_mg_level& L = c->level[l];
+  const fx_mg_level& C = c->level[l + 1];
+  LevelDev D = dev_level(L);
+  const void* p_in = L.p;
+  void* pout = tail_swap(L, c->pre);
+  FX_DISPATCH_T(c->storage, FX_DISPATCH_OP(L.op, {
+    hipLaunchKernelGGL((k_tail_down<TV, OPV>), dim3(1), dim3(kTailThreads), 0, s, D, C.nx, C.ny, C.nz,
+                       (const TV*)p_in, (const TV*)L.rhs, (TV*)pout, (TV*)C.rhs, inv_dt, c->omega, c->pre,
+                       zero_guess ? 1 : 0);
+  }));
+  FX_LAUNCHED("k_tail_down");
+  return FX_OK;
+}
+
+static int launch_tail_up(fx_mg_config* c, int l, float inv_dt, hipStream_t s) {
+  fx_mg_level& F = c->level[l];
+  const fx_mg_level& C = c->level[l + 1];
+  LevelDev DF = dev_level(F), DC = dev_level(C);
+  const void* p_in = F.p;
+  void* pout = tail_swap(F, c->post);
+  const int mode = c->prolongation;
+#define FX_TUP(FO, CO, MO)                                                                                   FX_DISPATCH_T(c->storage, {                                                                                  hipLaunchKernelGGL((k_tail_up<TV, FO, CO, MO>), dim3(1), dim3(kTailThreads), 0, s, DF, DC,                                   (const TV*)p_in, (const TV*)C.p, (const TV*)F.rhs, (TV*)pout, inv_dt, c->omega,                            c->post);                                                                             })
+  // coarse operator as launch_prolong picks it: the activity test of renormalised prolongation only
+  if (mode == FX_PROLONG_CONSTANT) {
+    FX_DISPATCH_OP(F.op, { FX_TUP(OPV, FX_OP_BOX, FX_PROLONG_CONSTANT); });
+  } else if (mode == FX_PROLONG_TRILINEAR) {
+    FX_DISPATCH_OP(F.op, { FX_TUP(OPV, FX_OP_BOX, FX_PROLONG_TRILINEAR); });
+  } else if (C.op == FX_OP_EDGES) {
+    FX_DISPATCH_OP(F.op, { FX_TUP(OPV, FX_OP_EDGES, FX_PROLONG_RENORM); });
+  } else if (C.op == FX_OP_PACKED) {
+    FX_DISPATCH_OP(F.op, { FX_TUP(OPV, FX_OP_PACKED, FX_PROLONG_RENORM); });
+  } else {
+    FX_DISPATCH_OP(F.op, { FX_TUP(OPV, FX_OP_BOX, FX_PROLONG_RENORM); });
+  }
+#undef FX_TUP
+  FX_LAUNCHED("k_tail_up");
+  return FX_OK;
+}
+
 // Recursive V-cycle.  zero_guess: the level's current iterate is 0.
 int vcycle(fx_mg_config* c, int l, float inv_dt, bool zero_guess, hipStream_t s) {
   fx_mg_level& L = c->level[l];
   if (l == c->levels - 1) {
     if (c->coarse_sweeps > 0) return smooth_n(c, l, c->coarse_sweeps, inv_dt, zero_guess, s);
     return launch_direct(c, s);
+  }
+  if (tail_ok(c, l)) {
+    if (int e = launch_tail_down(c, l, inv_dt, zero_guess, s)) return e;
+    if (int e = vcycle(c, l + 1, 1.0f, true, s)) return e;
+    return launch_tail_up(c, l, inv_dt, s);
   }
   if (c->pre > 0) {
     if (int e = smooth_n(c, l, c->pre, inv_dt, zero_guess, s)) return e;

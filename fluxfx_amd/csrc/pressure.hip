@@ -103,38 +103,45 @@ __device__ __forceinline__ bool face_open(const uint8_t* flags, int a, int x, in
 }
 
 // Threads cover (nx+1, ny+1, nz+1); each writes the U, V, W face it owns.
+// Closed faces read 0, or the wall velocity of moving colliders (bu/bv/bw,
+// shaders/moving_boundary.glsl) when given.
+__device__ __forceinline__ float closed_value(const float* __restrict__ b, int64_t i) { return b ? b[i] : 0.0f; }
+
 __global__ void k_enforce_walls(int nx, int ny, int nz, const uint8_t* __restrict__ flags,
                                 const float* __restrict__ u, const float* __restrict__ v,
-                                const float* __restrict__ w, float* __restrict__ uo, float* __restrict__ vo,
-                                float* __restrict__ wo) {
+                                const float* __restrict__ w, const float* __restrict__ bu,
+                                const float* __restrict__ bv, const float* __restrict__ bw, float* __restrict__ uo,
+                                float* __restrict__ vo, float* __restrict__ wo) {
   int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y * blockDim.y + threadIdx.y, z = blockIdx.z;
   if (x > nx || y > ny) return;
   if (y < ny && z < nz) {
     int64_t i = idx3(x, y, z, nx + 1, ny);
-    uo[i] = face_open(flags, 0, x, y, z, nx, ny, nz) ? u[i] : 0.0f;
+    uo[i] = face_open(flags, 0, x, y, z, nx, ny, nz) ? u[i] : closed_value(bu, i);
   }
   if (x < nx && z < nz) {
     int64_t i = idx3(x, y, z, nx, ny + 1);
-    vo[i] = face_open(flags, 1, x, y, z, nx, ny, nz) ? v[i] : 0.0f;
+    vo[i] = face_open(flags, 1, x, y, z, nx, ny, nz) ? v[i] : closed_value(bv, i);
   }
   if (x < nx && y < ny) {
     int64_t i = idx3(x, y, z, nx, ny);
-    wo[i] = face_open(flags, 2, x, y, z, nx, ny, nz) ? w[i] : 0.0f;
+    wo[i] = face_open(flags, 2, x, y, z, nx, ny, nz) ? w[i] : closed_value(bw, i);
   }
 }
 
 __global__ void k_divergence(int nx, int ny, int nz, float ix, float iy, float iz,
                              const uint8_t* __restrict__ flags, const float* __restrict__ u,
-                             const float* __restrict__ v, const float* __restrict__ w, float* __restrict__ div) {
+                             const float* __restrict__ v, const float* __restrict__ w, const float* __restrict__ bu,
+                             const float* __restrict__ bv, const float* __restrict__ bw, float* __restrict__ div) {
   int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y * blockDim.y + threadIdx.y, z = blockIdx.z;
   if (x >= nx || y >= ny) return;
   uint8_t f = cell_flags(flags, x, y, z, nx, ny, nz);
   float d = 0.0f;
   if (f & FLAG_FLUID) {
-    int64_t iu = idx3(x, y, z, nx + 1, ny), iv = idx3(x, y, z, nx, ny + 1), iw = idx3(x, y, z, nx, ny);
-    float u0 = (f & FLAG_XLO) ? u[iu] : 0.0f, u1 = (f & FLAG_XHI) ? u[iu + 1] : 0.0f;
-    float v0 = (f & FLAG_YLO) ? v[iv] : 0.0f, v1 = (f & FLAG_YHI) ? v[iv + nx] : 0.0f;
-    float w0 = (f & FLAG_ZLO) ? w[iw] : 0.0f, w1 = (f & FLAG_ZHI) ? w[iw + int64_t(nx) * ny] : 0.0f;
+    const int64_t iu = idx3(x, y, z, nx + 1, ny), iv = idx3(x, y, z, nx, ny + 1), iw = idx3(x, y, z, nx, ny);
+    const int64_t iv1 = iv + nx, iw1 = iw + int64_t(nx) * ny;
+    float u0 = (f & FLAG_XLO) ? u[iu] : closed_value(bu, iu), u1 = (f & FLAG_XHI) ? u[iu + 1] : closed_value(bu, iu + 1);
+    float v0 = (f & FLAG_YLO) ? v[iv] : closed_value(bv, iv), v1 = (f & FLAG_YHI) ? v[iv1] : closed_value(bv, iv1);
+    float w0 = (f & FLAG_ZLO) ? w[iw] : closed_value(bw, iw), w1 = (f & FLAG_ZHI) ? w[iw1] : closed_value(bw, iw1);
     d = (u1 - u0) * ix + (v1 - v0) * iy + (w1 - w0) * iz;
   }
   div[idx3(x, y, z, nx, ny)] = d;
@@ -164,6 +171,7 @@ __global__ void k_jacobi(int nx, int ny, int nz, float wx, float wy, float wz, f
 __global__ void k_gradient(int nx, int ny, int nz, float ix, float iy, float iz, float scale,
                            const uint8_t* __restrict__ flags, const float* __restrict__ p,
                            const float* __restrict__ dp, const float* __restrict__ u, const float* __restrict__ v, const float* __restrict__ w,
+                           const float* __restrict__ bu, const float* __restrict__ bv, const float* __restrict__ bw,
                            float* __restrict__ uo, float* __restrict__ vo, float* __restrict__ wo) {
   int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y * blockDim.y + threadIdx.y, z = blockIdx.z;
   if (x > nx || y > ny) return;
@@ -171,7 +179,7 @@ __global__ void k_gradient(int nx, int ny, int nz, float ix, float iy, float iz,
   int64_t c = idx3(x, y, z, nx, ny);
   if (y < ny && z < nz) {
     int64_t i = idx3(x, y, z, nx + 1, ny);
-    float r = 0.0f;
+    float r = closed_value(bu, i);
     if (face_open(flags, 0, x, y, z, nx, ny, nz)) {
       float g = p[c] - p[c - 1];
       if (dp) g += dp[c] - dp[c - 1];
@@ -181,7 +189,7 @@ __global__ void k_gradient(int nx, int ny, int nz, float ix, float iy, float iz,
   }
   if (x < nx && z < nz) {
     int64_t i = idx3(x, y, z, nx, ny + 1);
-    float r = 0.0f;
+    float r = closed_value(bv, i);
     if (face_open(flags, 1, x, y, z, nx, ny, nz)) {
       float g = p[c] - p[c - sy];
       if (dp) g += dp[c] - dp[c - sy];
@@ -190,7 +198,7 @@ __global__ void k_gradient(int nx, int ny, int nz, float ix, float iy, float iz,
     vo[i] = r;
   }
   if (x < nx && y < ny) {
-    float r = 0.0f;
+    float r = closed_value(bw, c);
     if (face_open(flags, 2, x, y, z, nx, ny, nz)) {
       float g = p[c] - p[c - sz];
       if (dp) g += dp[c] - dp[c - sz];
@@ -273,17 +281,18 @@ __global__ __launch_bounds__(256) void k_gradient_q(int nx, int ny, int nz, floa
   const int qx = nx >> 2, row = q / qx, x0 = (q - row * qx) << 2, z = row / ny, y = row - z * ny;
   const int c = row * nx + x0, iu = row * (nx + 1) + x0, iv = (y + (ny + 1) * z) * nx + x0, sz = nx * ny;
   const uint32_t fw = row_flags(flags, x0, y, z, nx, ny, nz, c);
-  const float4 pc = ld4f(p, c);
-  const float pxm = x0 > 0 ? p[c - 1] : 0.0f;
-  const float4 pym = y > 0 ? ld4f(p, c - nx) : make_float4(0.f, 0.f, 0.f, 0.f);
-  const float4 pzm = z > 0 ? ld4f(p, c - sz) : make_float4(0.f, 0.f, 0.f, 0.f);
+  // unconditional loads (the own cell stands in for a missing lower
+  // neighbour: its face is closed and never reads it), so one wait covers all
+  const int cxm = x0 > 0 ? c - 1 : c, cym = y > 0 ? c - nx : c, czm = z > 0 ? c - sz : c;
+  const float4 pc = ld4f(p, c), pym = ld4f(p, cym), pzm = ld4f(p, czm);
+  const float pxm = p[cxm];
   float4 dc = make_float4(0.f, 0.f, 0.f, 0.f), dym = dc, dzm = dc;
   float dxm = 0.0f;
   if (dp) {
     dc = ld4f(dp, c);
-    dxm = x0 > 0 ? dp[c - 1] : 0.0f;
-    if (y > 0) dym = ld4f(dp, c - nx);
-    if (z > 0) dzm = ld4f(dp, c - sz);
+    dxm = dp[cxm];
+    dym = ld4f(dp, cym);
+    dzm = ld4f(dp, czm);
   }
   float uu[4];
 #pragma unroll
@@ -360,16 +369,19 @@ static bool rows_vec_ok(const fx_grid& g, const void* a, const void* b, const vo
 
 static int launch_gradient(const fx_grid& g, const uint8_t* flags, const float* p, const float* dp, const float* u,
                            const float* v, const float* w, float* uo, float* vo, float* wo, float* psum,
-                           float scale, hipStream_t s, const char* name) {
+                           float scale, hipStream_t s, const char* name, const float* const* walls = nullptr) {
   const float ix = g.nx / g.ex, iy = g.ny / g.ey, iz = g.nz / g.ez;
-  if (rows_vec_ok(g, flags, p, dp, v, w, vo) && rows_vec_ok(g, wo, psum, nullptr, nullptr)) {
+  const float* bu = walls ? walls[0] : nullptr;
+  const float* bv = walls ? walls[1] : nullptr;
+  const float* bw = walls ? walls[2] : nullptr;
+  if (!walls && rows_vec_ok(g, flags, p, dp, v, w, vo) && rows_vec_ok(g, wo, psum, nullptr, nullptr)) {
     const int nq = g.nx / 4 * g.ny * g.nz;
     hipLaunchKernelGGL(k_gradient_q, dim3((nq + 255) / 256), dim3(256), 0, s, g.nx, g.ny, g.nz, ix, iy, iz, scale,
                        flags, p, dp, u, v, w, uo, vo, wo, psum, nq);
   } else {
     dim3 gr = grid3(g.nx + 1, g.ny + 1, g.nz + 1, kBlock);
-    hipLaunchKernelGGL(k_gradient, gr, kBlock, 0, s, g.nx, g.ny, g.nz, ix, iy, iz, scale, flags, p, dp, u, v, w, uo,
-                       vo, wo);
+    hipLaunchKernelGGL(k_gradient, gr, kBlock, 0, s, g.nx, g.ny, g.nz, ix, iy, iz, scale, flags, p, dp, u, v, w, bu,
+                       bv, bw, uo, vo, wo);
     if (psum) {
       const int64_t n = int64_t(g.nx) * g.ny * g.nz;
       hipLaunchKernelGGL(k_combine, dim3((n + 255) / 256), dim3(256), 0, s, n, p, dp, psum);
@@ -451,8 +463,22 @@ int fx_enforce_walls(fx_grid g, const uint8_t* flags, const float* u, const floa
   FX_REQUIRE(u && v && w && uo && vo && wo, FX_EINVAL, "null face array");
   FX_REQUIRE(uo != u && vo != v && wo != w, FX_EINVAL, "Advection input/output textures must be distinct");
   dim3 gr = grid3(g.nx + 1, g.ny + 1, g.nz + 1, kBlock);
-  hipLaunchKernelGGL(k_enforce_walls, gr, kBlock, 0, S(stream), g.nx, g.ny, g.nz, flags, u, v, w, uo, vo, wo);
+  hipLaunchKernelGGL(k_enforce_walls, gr, kBlock, 0, S(stream), g.nx, g.ny, g.nz, flags, u, v, w,
+                     (const float*)nullptr, (const float*)nullptr, (const float*)nullptr, uo, vo, wo);
   FX_LAUNCHED("fx_enforce_walls");
+  return FX_OK;
+}
+
+int fx_enforce_walls_moving(fx_grid g, const uint8_t* flags, const float* u, const float* v, const float* w,
+                            const float* bu, const float* bv, const float* bw, float* uo, float* vo, float* wo,
+                            void* stream) {
+  if (int e = check_grid(g)) return e;
+  FX_REQUIRE(flags && u && v && w && bu && bv && bw && uo && vo && wo, FX_EINVAL, "null array");
+  FX_REQUIRE(uo != u && vo != v && wo != w, FX_EINVAL, "Advection input/output textures must be distinct");
+  dim3 gr = grid3(g.nx + 1, g.ny + 1, g.nz + 1, kBlock);
+  hipLaunchKernelGGL(k_enforce_walls, gr, kBlock, 0, S(stream), g.nx, g.ny, g.nz, flags, u, v, w, bu, bv, bw, uo, vo,
+                     wo);
+  FX_LAUNCHED("fx_enforce_walls_moving");
   return FX_OK;
 }
 
@@ -467,9 +493,21 @@ int fx_divergence(fx_grid g, const uint8_t* flags, const float* u, const float* 
   } else {
     dim3 gr = grid3(g.nx, g.ny, g.nz, kBlock);
     hipLaunchKernelGGL(k_divergence, gr, kBlock, 0, S(stream), g.nx, g.ny, g.nz, g.nx / g.ex, g.ny / g.ey,
-                       g.nz / g.ez, flags, u, v, w, div);
+                       g.nz / g.ez, flags, u, v, w, (const float*)nullptr, (const float*)nullptr,
+                       (const float*)nullptr, div);
   }
   FX_LAUNCHED("fx_divergence");
+  return FX_OK;
+}
+
+int fx_divergence_moving(fx_grid g, const uint8_t* flags, const float* u, const float* v, const float* w,
+                         const float* bu, const float* bv, const float* bw, float* div, void* stream) {
+  if (int e = check_grid(g)) return e;
+  FX_REQUIRE(flags && u && v && w && bu && bv && bw && div, FX_EINVAL, "null array");
+  dim3 gr = grid3(g.nx, g.ny, g.nz, kBlock);
+  hipLaunchKernelGGL(k_divergence, gr, kBlock, 0, S(stream), g.nx, g.ny, g.nz, g.nx / g.ex, g.ny / g.ey, g.nz / g.ez,
+                     flags, u, v, w, bu, bv, bw, div);
+  FX_LAUNCHED("fx_divergence_moving");
   return FX_OK;
 }
 
@@ -502,6 +540,17 @@ int fx_gradient_split(fx_grid g, const uint8_t* flags, const float* p, const flo
   FX_REQUIRE(uo != u && vo != v && wo != w, FX_EINVAL, "Advection input/output textures must be distinct");
   FX_REQUIRE(psum != p && psum != dp, FX_EINVAL, "gradient: the pressure sum must not alias p or dp");
   return launch_gradient(g, flags, p, dp, u, v, w, uo, vo, wo, psum, scale, S(stream), "fx_gradient_split");
+}
+
+int fx_gradient_moving(fx_grid g, const uint8_t* flags, const float* p, const float* u, const float* v,
+                       const float* w, const float* bu, const float* bv, const float* bw, float* uo, float* vo,
+                       float* wo, float scale, void* stream) {
+  if (int e = check_grid(g)) return e;
+  FX_REQUIRE(flags && p && u && v && w && bu && bv && bw && uo && vo && wo, FX_EINVAL, "null array");
+  FX_REQUIRE(uo != u && vo != v && wo != w, FX_EINVAL, "Advection input/output textures must be distinct");
+  const float* walls[3] = {bu, bv, bw};
+  return launch_gradient(g, flags, p, nullptr, u, v, w, uo, vo, wo, nullptr, scale, S(stream), "fx_gradient_moving",
+                         walls);
 }
 
 int fx_combine(const float* a, const float* b, float* out, int64_t n, void* stream) {

@@ -6,25 +6,31 @@
 
 namespace fx {
 
-__global__ void k_finalize(double* ws, int q0, int nq) {
+// One block per quantity: block b folds the partials of the b-th set bit of
+// `mask` into ws[q] (the same summation order for every quantity and launch).
+__global__ void k_finalize(double* ws, uint32_t mask) {
   __shared__ double part[kRedThreads / 64];
-  for (int q = q0; q < q0 + nq; ++q) {
-    double v = 0.0;
-    for (int b = threadIdx.x; b < kRedBlocks; b += kRedThreads) v += ws[kWsBase + kRedBlocks * q + b];
-    v = wave_sum(v);
-    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = v;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-      double s = 0.0;
-      for (int k = 0; k < kRedThreads / 64; ++k) s += part[k];
-      ws[q] = s;
-    }
-    __syncthreads();
+  uint32_t m = mask;
+  for (unsigned k = 0; k < blockIdx.x; ++k) m &= m - 1;
+  const int q = __builtin_ctz(m);
+  double v = 0.0;
+  for (int b = threadIdx.x; b < kRedBlocks; b += kRedThreads) v += ws[kWsBase + kRedBlocks * q + b];
+  v = wave_sum(v);
+  if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = 0.0;
+    for (int k = 0; k < kRedThreads / 64; ++k) t += part[k];
+    ws[q] = t;
   }
 }
 
+void ws_finalize_mask(double* ws, uint32_t mask, hipStream_t s) {
+  if (mask) hipLaunchKernelGGL(k_finalize, dim3(__builtin_popcount(mask)), dim3(kRedThreads), 0, s, ws, mask);
+}
+
 void ws_finalize(double* ws, int q0, int nq, hipStream_t s) {
-  hipLaunchKernelGGL(k_finalize, dim3(1), dim3(kRedThreads), 0, s, ws, q0, nq);
+  ws_finalize_mask(ws, ((nq >= 32 ? 0u : (1u << nq)) - 1u) << q0, s);
 }
 
 // Small pinned host buffer for the scalar read-backs of the solvers.
@@ -226,8 +232,7 @@ int resnorm_q(const fx_mg_level& L0, const float* x, const float* b, double* ws,
     });
   }
   FX_LAUNCHED("k_resnorm");
-  ws_finalize(ws, qr, 1, s);
-  if (qb != qr) ws_finalize(ws, qb, 1, s);
+  ws_finalize_mask(ws, (1u << qr) | (1u << qb), s);
   return FX_OK;
 }
 

@@ -13,7 +13,10 @@ tolerance-driven solvers:
   preconditioner, MG fallback).
 * ``learned_weights``/``learned_iterations``: the U-Net's trained parameters
   (.npz written by UNetPreconditioner.save) and how many U-Net preconditioned
-  iterations run before the V-cycle preconditioned fallback takes over.
+  iterations run before the V-cycle preconditioned fallback takes over;
+  ``learned_level`` 1 runs the network on the 8-child mean of the residual one
+  multigrid level down (a 128^3-trained network on a 256^3 grid sees its own
+  resolution) and prolongs its output before the smoothing sweeps.
 * ``mg_pre``/``mg_post``/``mg_coarsest``/``mg_restriction``/``mg_levels``:
   V-cycle shape.  ``mg_coarsest`` = 4 with 3/3 sweeps is the reference
   (backend/multigrid.py:57-81); the configs use 2/2 and an 8^3 FP64 direct solve.
@@ -101,6 +104,8 @@ class PressureSettings(ThermalSettings):
     learned_weights: str = ""
     learned_iterations: int = 2
     learned_sweeps: int = 2
+    # grid the U-Net runs on: 0 = the pressure grid, 1 = one multigrid level coarser
+    learned_level: int = 0
 
     def __post_init__(self):
         super().__post_init__()
@@ -141,6 +146,8 @@ class PressureSettings(ThermalSettings):
             raise ValueError("Smoothing sweeps must be integers in [0, 16], not both zero")
         if self.mg_coarsest not in (2, 4, 8, 16):
             raise ValueError("Coarsest multigrid side must be 2, 4, 8 or 16")
+        if self.learned_level not in (0, 1):
+            raise ValueError("The learned preconditioner runs on level 0 or 1")
 
 
 def divergence_stats(before, after):

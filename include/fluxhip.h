@@ -124,6 +124,43 @@ int fx_gradient_split(fx_grid g, const uint8_t* flags, const float* p, const flo
                       void* stream);
 
 
+/* Moving colliders (MovingSolidDevice, backend/moving_collision.py:26-41 with
+ * shaders/moving_boundary.glsl): as fx_enforce_walls / fx_divergence /
+ * fx_gradient, but every closed face (domain wall or solid) reads or is set to
+ * the wall velocity bu/bv/bw of fx_wall_velocity instead of 0. */
+int fx_enforce_walls_moving(fx_grid g, const uint8_t* flags, const float* u, const float* v, const float* w,
+                            const float* bu, const float* bv, const float* bw, float* uo, float* vo, float* wo,
+                            void* stream);
+int fx_divergence_moving(fx_grid g, const uint8_t* flags, const float* u, const float* v, const float* w,
+                         const float* bu, const float* bv, const float* bw, float* div, void* stream);
+int fx_gradient_moving(fx_grid g, const uint8_t* flags, const float* p, const float* u, const float* v,
+                       const float* w, const float* bu, const float* bv, const float* bw, float* uo, float* vo,
+                       float* wo, float scale, void* stream);
+
+/* Owner raster of moving collider `owner` (1..8): mask[c] = owner where the
+ * unit sphere/box under the inverse rows r (3 x 4) covers the cell centre and
+ * no earlier collider owns the cell (MovingSolidDevice.update,
+ * backend/moving_collision.py:53-71, shaders/moving_mask.glsl:1-11). */
+int fx_owner_raster(fx_grid g, int32_t kind, const float* r, int32_t owner, uint8_t* mask, void* stream);
+
+/* Rigid wall velocity on interior faces next to solid cells: rows (host,
+ * count x 3 x 4 floats) are each collider's velocity_rows; the low cell's
+ * owner, else the high cell's, is evaluated at the face centre; 0 elsewhere
+ * (shaders/moving_wall.glsl:1-19). */
+int fx_wall_velocity(fx_grid g, const uint8_t* owner, int32_t count, const float* rows, float* bu, float* bv,
+                     float* bw, void* stream);
+
+/* Cells solid now or before start empty (MovingSolidDevice.remap,
+ * backend/moving_collision.py:73-83, shaders/moving_remap_scalar.glsl). */
+int fx_remap_scalar(fx_grid g, const uint8_t* solid, const uint8_t* old_solid, const float* in, float* out,
+                    void* stream);
+
+/* Faces blocked now take the wall velocity, faces blocked before start at
+ * rest, others keep their value, domain walls 0 (shaders/moving_remap_face.glsl). */
+int fx_remap_faces(fx_grid g, const uint8_t* solid, const uint8_t* old_solid, const float* bu, const float* bv,
+                   const float* bw, const float* u, const float* v, const float* w, float* uo, float* vo, float* wo,
+                   void* stream);
+
 /* out = a + b (n floats). */
 int fx_combine(const float* a, const float* b, float* out, int64_t n, void* stream);
 
@@ -160,6 +197,11 @@ typedef struct fx_mg_level {
                             to the level, are all active */
 } fx_mg_level;
 
+/* fx_mg_config.options: run every sweep of the small levels (<= 4096 cells) as
+ * its own launch instead of the fused single-block pre/post kernels (same
+ * results bit for bit; for tests) */
+#define FX_MG_NO_TAIL 1
+
 typedef struct fx_mg_config {
   int32_t levels;
   int32_t storage;        /* FX_F32 / FX_BF16 / FX_F16 */
@@ -170,7 +212,7 @@ typedef struct fx_mg_config {
   int32_t coarse_sweeps;  /* > 0: smooth the coarsest level this many times; 0: direct */
   const double* coarse_matrix; /* device, m x m FP64 pseudo-inverse, m = cells of coarsest */
   float coarse_scale;     /* direct: p = coarse_scale * M rhs */
-  int32_t pad_;
+  int32_t options;        /* FX_MG_* bits */
   fx_mg_level level[16];
 } fx_mg_config;
 
@@ -257,10 +299,11 @@ int fx_mixed_solve(fx_mg_config* ecfg, float* x, float* x_spare, const float* b,
 
 /* Flexible preconditioned CG (Polak-Ribiere beta, restarted from the true
  * residual if the recurrence drifts) on L x = b with one V-cycle of pcfg as
- * preconditioner.  r, p, Ap: fp32 scratch; zold: scratch in pcfg's storage.
- * ws: FX_WS_DOUBLES doubles. */
+ * preconditioner.  r, p, Ap: fp32 scratch; zold: scratch in pcfg's storage;
+ * p2 (may be NULL): a second fp32 direction buffer, with which the direction
+ * update is fused into the operator apply.  ws: FX_WS_DOUBLES doubles. */
 int fx_pcg_solve(fx_mg_config* pcfg, float* x, const float* b, float* r, float* p, float* Ap, void* zold,
-                 const fx_solve_params* prm, double* ws, fx_solve_info* info, void* stream);
+                 float* p2, const fx_solve_params* prm, double* ws, fx_solve_info* info, void* stream);
 
 /* --------------------------------------------------------------------------
  * transport (backend/detail.py:16-37, backend/projected.py:150-195)
@@ -362,6 +405,17 @@ int fx_combustion(int64_t n, float dt, float ignition, float burn_rate, float he
  * (AdaptiveTimestep.bounds, backend/timestep.py:24-33, shaders/max_reduce.glsl). */
 int fx_max_abs(int32_t count, const float* const* arrays, const int64_t* sizes, float* out, void* stream);
 
+/* Cell-centred vorticity (omega, |omega|) as float4 per cell, x-fast, from the
+ * face velocities (DetailPasses.confine, backend/detail.py:39-49,
+ * shaders/curl.glsl:1-18).  curl4: 4 floats per cell, 16-byte aligned. */
+int fx_curl(fx_grid g, const float* u, const float* v, const float* w, float* curl4, void* stream);
+
+/* Confined faces: in + dt * the mean confinement force of the two cells of the
+ * face (backend/detail.py:39-49, shaders/confinement.glsl:1-25); the force is
+ * strength*h*cross(grad|w|/|grad|w||, w) capped at `limit`. */
+int fx_confine(fx_grid g, const float* u, const float* v, const float* w, const float* curl4, float* uo, float* vo,
+               float* wo, float dt, float strength, float limit, void* stream);
+
 
 /* --------------------------------------------------------------------------
  * learned preconditioner (no reference counterpart): 3-level 3D U-Net on
@@ -458,9 +512,13 @@ int fx_unet_forward(const fx_unet* net, const float* r, const uint8_t* solid, do
  * (info->fallback = 1).  Vectors: x (in/out), b, r, p, Ap, zold, z, rT fp32
  * of the level-0 size; mg->level[0] p/q/rhs are the V-cycle's work arrays and
  * mg_zold a fourth level-0 array, all in mg->storage (the fallback swaps
- * level[0].p with mg_zold as fx_pcg_solve does). */
+ * level[0].p with mg_zold as fx_pcg_solve does); p2 as in fx_pcg_solve.
+ * The network grid is level 0's, or level 1's: then the network sees the
+ * 8-child mean of the residual, its output is prolonged like a V-cycle
+ * correction before the sweeps, `solid` is level 1's mask (a coarse cell is
+ * solid when all eight children are) and post_scale is level 1's h^2. */
 int fx_learned_pcg_solve(const fx_unet* net, fx_mg_config* mg, float* x, const float* b, float* r, float* p,
-                         float* Ap, float* zold, float* z, float* rT, float* ztmp, void* mg_zold,
+                         float* Ap, float* zold, float* z, float* rT, float* ztmp, void* mg_zold, float* p2,
                          const uint8_t* solid, float post_scale, int32_t learned_iters, int32_t sweeps,
                          const fx_solve_params* prm, double* ws, fx_solve_info* info, void* stream);
 

@@ -242,3 +242,83 @@ def primitive_rows(center, size):
     """Axis-aligned inverse rows (physics/collision.py:28-32)."""
     return tuple(tuple((1 / size[i] if i == j else 0.0) for j in range(3)) + (-center[i] / size[i],)
                  for i in range(3))
+
+
+# ----------------------------------------------------------------------------
+# moving colliders (backend/moving_collision.py:7-87)
+# ----------------------------------------------------------------------------
+def raster_owners(grid, colliders):
+    """Owner index per cell, 0 = fluid: the first collider (1-based) whose
+    shape contains the cell centre (shaders/moving_mask.glsl:1-11)."""
+    owner = np.zeros(grid.array_shape, dtype=np.uint8)
+    for k, c in enumerate(colliders, start=1):
+        inside = raster_colliders(grid, [c])
+        owner = np.where((owner == 0) & inside, k, owner).astype(np.uint8)
+    return owner
+
+
+def wall_velocity(grid, owner, rows_per_owner):
+    """Rigid wall velocity on every interior face next to a solid cell
+    (shaders/moving_wall.glsl:1-19): owner = the low cell's collider, else the
+    high cell's; value = component a of that collider's velocity rows at the
+    face centre; 0 elsewhere and on the domain walls.
+    ``rows_per_owner[k-1]``: three affine rows (velocity_rows) of owner k."""
+    h = grid.cell_size
+    out = []
+    for a in range(3):
+        fz, fy, fx = grid.face_array_shape(a)
+        z, y, x = np.meshgrid(np.arange(fz), np.arange(fy), np.arange(fx), indexing="ij")
+        f = (x, y, z)
+        n = grid.shape[a]
+        interior = (f[a] > 0) & (f[a] < n)
+        e = [0, 0, 0]
+        e[a] = 1
+        hi = np.zeros(grid.face_array_shape(a), dtype=np.int64)
+        lo = np.zeros_like(hi)
+        cz, cy, cx = np.minimum(z, grid.shape[2] - 1), np.minimum(y, grid.shape[1] - 1), np.minimum(x, grid.shape[0] - 1)
+        hi[interior] = owner[cz[interior], cy[interior], cx[interior]]
+        lz, ly, lx = z - e[2], y - e[1], x - e[0]
+        lo[interior] = owner[lz[interior], ly[interior], lx[interior]]
+        own = np.where(lo > 0, lo, hi)
+        off = [.5, .5, .5]
+        off[a] = 0.0
+        p = [(x + off[0]) * h[0], (y + off[1]) * h[1], (z + off[2]) * h[2]]
+        val = np.zeros(grid.face_array_shape(a))
+        for k, rows in enumerate(rows_per_owner, start=1):
+            r = rows[a]
+            val = np.where(interior & (own == k), r[0] * p[0] + r[1] * p[1] + r[2] * p[2] + r[3], val)
+        out.append(val)
+    return tuple(out)
+
+
+def remap_scalar(field, solid, old_solid):
+    """Cells solid now or before start empty (shaders/moving_remap_scalar.glsl:1-7)."""
+    return np.where(solid.astype(bool) | old_solid.astype(bool), 0.0, field)
+
+
+def remap_faces(velocity, grid, solid, old_solid, boundary):
+    """Faces blocked now take the wall velocity, faces blocked before start at
+    rest, the rest keep their value; domain-wall faces 0
+    (shaders/moving_remap_face.glsl:1-12)."""
+    now = face_open(grid, solid)
+    before = face_open(grid, old_solid)
+    box = face_open(grid, None)
+    out = []
+    for a in range(3):
+        blocked = box[a] & ~now[a]
+        previous = box[a] & ~before[a]
+        out.append(np.where(~box[a], 0.0, np.where(blocked, boundary[a], np.where(previous, 0.0, velocity[a]))))
+    return tuple(out)
+
+
+def walls_moving(velocity, grid, solid, boundary):
+    """pressure_walls.glsl with moving_boundary.glsl: closed faces take the wall
+    velocity (zero on the domain walls)."""
+    return tuple(np.where(o, v, b) for v, o, b in zip(velocity, face_open(grid, solid), boundary))
+
+
+def gradient_correct_moving(velocity, p, grid, scale, solid, boundary):
+    """pressure_gradient.glsl with moving_boundary.glsl: open faces corrected,
+    closed faces take the wall velocity."""
+    out = gradient_correct(velocity, p, grid, scale, solid)
+    return tuple(np.where(o, g, b) for g, o, b in zip(out, face_open(grid, solid), boundary))

@@ -84,12 +84,37 @@ class State:
     pressure: Optional[np.ndarray] = None
     last_dt: Optional[float] = None
     info: dict = field(default_factory=dict)
+    # moving colliders: per-cell owner (0 fluid) and the wall velocity per face axis
+    owner: Optional[np.ndarray] = None
+    boundary: Optional[tuple] = None
+
+
+def move_colliders(state, previous, colliders, dt, rows_fn):
+    """DenseProjectedSmoke.move_colliders (backend/projected.py:103-115):
+    rasterise the new owners, wall velocities from the rigid motion, then
+    remap the fields (backend/moving_collision.py:53-83).  ``previous`` and
+    ``colliders``: (kind, rows) pairs; ``rows_fn(old, new, dt)`` gives a
+    collider's velocity rows."""
+    grid = state.grid
+    old_solid = state.solid if state.solid is not None else np.zeros(grid.array_shape, bool)
+    owner = mac.raster_owners(grid, colliders)
+    solid = owner > 0
+    boundary = mac.wall_velocity(grid, owner, [rows_fn(o, n, dt) for o, n in zip(previous, colliders)])
+    state.density = mac.remap_scalar(state.density, solid, old_solid)
+    state.temperature = mac.remap_scalar(state.temperature, solid, old_solid)
+    if state.fuel is not None:
+        state.fuel = mac.remap_scalar(state.fuel, solid, old_solid)
+        state.flame = np.zeros(grid.array_shape)
+    state.velocity = mac.remap_faces(state.velocity, grid, solid, old_solid, boundary)
+    state.owner, state.solid, state.boundary = owner, solid, boundary
 
 
 def project(state, provisional, dt, cfg, mg_cache=None):
     """PressureProjector.project (backend/pressure.py:57-83) for every projector kind."""
     grid, solid = state.grid, state.solid
-    bounded = mac.walls(provisional, grid, solid)
+    moving = state.boundary is not None
+    bounded = mac.walls_moving(provisional, grid, solid, state.boundary) if moving else \
+        mac.walls(provisional, grid, solid)
     before = mac.divergence(bounded, grid, solid)
     impulse = cfg.kind != "JACOBI"
     prev = state.last_dt
@@ -142,7 +167,10 @@ def project(state, provisional, dt, cfg, mg_cache=None):
         info["residual_history"] = hist
     else:
         raise ValueError(cfg.kind)
-    corrected = mac.gradient_correct(bounded, p, grid, solve_dt, solid)
+    if moving:
+        corrected = mac.gradient_correct_moving(bounded, p, grid, solve_dt, solid, state.boundary)
+    else:
+        corrected = mac.gradient_correct(bounded, p, grid, solve_dt, solid)
     after = mac.divergence(corrected, grid, solid)
     info["before"] = before
     info["after"] = after
@@ -156,7 +184,8 @@ def step(state, settings, dt, sources, cfg, multi=False, mg_cache=None):
 
     ``settings`` carries velocity_advection, scalar_advection, thermal_lift,
     density_weight, dissipation, cooling, combustion_enabled, ignition_temperature,
-    burn_rate, heat_yield, smoke_yield (physics/pressure.py:7-36 names).
+    burn_rate, heat_yield, smoke_yield, vorticity_strength, vorticity_limit
+    (physics/pressure.py:7-36 names).
     ``sources`` is a list of ``transport.Src``.
     """
     grid, solid = state.grid, state.solid
@@ -167,6 +196,10 @@ def step(state, settings, dt, sources, cfg, multi=False, mg_cache=None):
     w = transport.buoyancy(adv[2], state.density, state.temperature, dt,
                            settings.thermal_lift, settings.density_weight)
     provisional = (adv[0], adv[1], w)
+    if getattr(settings, "vorticity_strength", 0.0) > 0:
+        # confinement of the provisional field before the projection (backend/projected.py:172-175)
+        provisional, _ = transport.confine(provisional, grid, dt, settings.vorticity_strength,
+                                           settings.vorticity_limit)
     projected, info = project(state, provisional, dt, cfg, mg_cache)
     smc = settings.scalar_advection == "MACCORMACK"
     fields = [state.density, state.temperature]

@@ -440,3 +440,69 @@ def max_abs(field):
     a = np.abs(np.asarray(field, dtype=np.float64))
     a = np.where(np.isfinite(a), a, 3.4e38)
     return float(a.max()) if a.size else 0.0
+
+
+# ----------------------------------------------------------------------------
+# vorticity confinement (backend/detail.py:39-49)
+# ----------------------------------------------------------------------------
+def _clamped(a, z, y, x):
+    """a[z, y, x] with every index clamped into the array (texelFetch of a
+    clamped coordinate)."""
+    nz, ny, nx = a.shape[:3]
+    return a[np.clip(z, 0, nz - 1), np.clip(y, 0, ny - 1), np.clip(x, 0, nx - 1)]
+
+
+def centered_velocity(vel, grid, z, y, x):
+    """centeredVelocity (shaders/curl.glsl:1-7): the cell index is clamped into
+    the grid, each component is the mean of the cell's two faces along it."""
+    nx, ny, nz = grid.shape
+    z, y, x = np.clip(z, 0, nz - 1), np.clip(y, 0, ny - 1), np.clip(x, 0, nx - 1)
+    u, v, w = vel
+    return (0.5 * (u[z, y, x] + u[z, y, x + 1]), 0.5 * (v[z, y, x] + v[z, y + 1, x]),
+            0.5 * (w[z, y, x] + w[z + 1, y, x]))
+
+
+def curl(vel, grid):
+    """Cell-centred vorticity and its magnitude, [z, y, x, 4] (shaders/curl.glsl:8-18):
+    central differences of the centred velocity over clamped neighbours."""
+    nx, ny, nz = grid.shape
+    ix, iy, iz = grid.inv_cell
+    z, y, x = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+    cv = lambda dz, dy, dx: centered_velocity(vel, grid, z + dz, y + dy, x + dx)
+    dX = [(a - b) * 0.5 * ix for a, b in zip(cv(0, 0, 1), cv(0, 0, -1))]
+    dY = [(a - b) * 0.5 * iy for a, b in zip(cv(0, 1, 0), cv(0, -1, 0))]
+    dZ = [(a - b) * 0.5 * iz for a, b in zip(cv(1, 0, 0), cv(-1, 0, 0))]
+    om = np.stack([dY[2] - dZ[1], dZ[0] - dX[2], dX[1] - dY[0]], axis=-1)
+    return np.concatenate([om, np.sqrt((om * om).sum(-1, keepdims=True))], axis=-1)
+
+
+def confinement_force(cu, grid, strength, limit, z, y, x):
+    """forceAt (shaders/confinement.glsl:1-15) at cell indices (clamped on
+    every curl read): strength*h*cross(grad|w| / |grad|w||, w), scaled down to
+    at most `limit`; zero where |grad|w|| < 1e-8."""
+    ix, iy, iz = grid.inv_cell
+    mag = lambda dz, dy, dx: _clamped(cu, z + dz, y + dy, x + dx)[..., 3]
+    g = np.stack([0.5 * ix * (mag(0, 0, 1) - mag(0, 0, -1)), 0.5 * iy * (mag(0, 1, 0) - mag(0, -1, 0)),
+                  0.5 * iz * (mag(1, 0, 0) - mag(-1, 0, 0))], axis=-1)
+    gm = np.sqrt((g * g).sum(-1, keepdims=True))
+    h = 1.0 / max(ix, iy, iz)
+    om = _clamped(cu, z, y, x)[..., :3]
+    f = strength * h * np.cross(g / np.where(gm < 1e-8, 1.0, gm), om)
+    fm = np.sqrt((f * f).sum(-1, keepdims=True))
+    f = f * np.minimum(1.0, limit / np.maximum(fm, 1e-12))
+    return np.where(gm < 1e-8, 0.0, f)
+
+
+def confine(vel, grid, dt, strength, limit):
+    """Confined faces (shaders/confinement.glsl:16-25): face c of axis a gains
+    dt times component a of the mean force of cells c and c - e_a."""
+    cu = curl(vel, grid)
+    out = []
+    for a in range(3):
+        sz, sy, sx = grid.face_array_shape(a)
+        z, y, x = np.meshgrid(np.arange(sz), np.arange(sy), np.arange(sx), indexing="ij")
+        e = [(0, 0, 1), (0, 1, 0), (1, 0, 0)][a]
+        f = 0.5 * (confinement_force(cu, grid, strength, limit, z, y, x) +
+                   confinement_force(cu, grid, strength, limit, z - e[0], y - e[1], x - e[2]))
+        out.append(vel[a] + dt * f[..., a])
+    return tuple(out), cu

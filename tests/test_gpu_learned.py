@@ -27,9 +27,10 @@ def test_training_reduces_held_out_loss(trained):
     assert np.isfinite(after) and after < 0.5 * before, (before, after, hist)
 
 
-def _learned_solve(device, net, sample, learned, budget, sweeps=2):
+def _learned_solve(device, net, sample, learned, budget, sweeps=2, level=0):
     """Relative residual and metrics of a LEARNED solve with `learned` U-Net
-    iterations inside a total budget (budget == learned: no fallback)."""
+    iterations inside a total budget (budget == learned: no fallback); the
+    network runs on multigrid level `level`."""
     from fluxfx_amd.backend.collision import SolidMask
     from fluxfx_amd.backend.tolerance import ToleranceProjector
     from fluxfx_amd.physics.config import GridSpec
@@ -39,7 +40,7 @@ def _learned_solve(device, net, sample, learned, budget, sweeps=2):
     solids = None if solid is None else SolidMask(grid, device=device, mask=solid.cpu().numpy())
     s = PressureSettings(pressure_tolerance=1e-5, mg_pre=2, mg_post=2, mg_coarsest=8, pressure_max_cycles=budget,
                          pressure_precision="FP32", pressure_krylov="LEARNED", learned_iterations=learned,
-                         learned_sweeps=sweeps)
+                         learned_sweeps=sweeps, learned_level=level)
     pp = ToleranceProjector(grid, s, device, solids=solids)
     pp.attach_network(net)
     pp.before.tensor.copy_(b)
@@ -70,4 +71,32 @@ def test_learned_solve_reaches_tolerance_with_fallback(device, trained):
     """Two U-Net iterations, then V-cycle FPCG finishes at 1e-5."""
     net, held_out, *_ = trained
     m = _learned_solve(device, net, held_out[0], 2, 60)
+    assert m["converged"] and m["relative_residual"] <= 1e-5 and m["fallback"], m
+
+
+@pytest.fixture(scope="module")
+def trained_coarse(device):
+    """A network trained on pairs harvested at N/2, for level-1 application on N^3 solves."""
+    from fluxfx_amd.learned import UNetPreconditioner
+    from fluxfx_amd import training
+    train_set = training.harvest(device, N // 2, scenes=6, steps_per_scene=4, seed0=11000)
+    net = UNetPreconditioner(device, seed=2)
+    training.train(net, train_set, (N // 2,) * 3, steps=400, lr=2e-3, log_every=100)
+    held_out = training.harvest(device, N, scenes=2, steps_per_scene=2, seed0=21000)
+    return net, held_out
+
+
+def test_coarse_level_network_beats_untrained_and_reaches_tolerance(device, trained_coarse):
+    """Level-1 application (restricted residual in, prolonged correction out):
+    the trained network leaves a smaller true residual than the untrained one
+    after the same FPCG iterations, and the MG fallback finishes at 1e-5."""
+    from fluxfx_amd.learned import UNetPreconditioner
+    net, held_out = trained_coarse
+    fresh = UNetPreconditioner(device, seed=2)
+    print("coarse-level learned preconditioner: ablation and fallback", flush=True)
+    for sample in held_out[:2]:
+        mt = _learned_solve(device, net, sample, 2, 2, level=1)
+        mf = _learned_solve(device, fresh, sample, 2, 2, level=1)
+        assert not mt["fallback"] and mt["relative_residual"] < 0.5 * mf["relative_residual"], (mt, mf)
+    m = _learned_solve(device, net, held_out[0], 1, 60, level=1)
     assert m["converged"] and m["relative_residual"] <= 1e-5 and m["fallback"], m

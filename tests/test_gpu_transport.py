@@ -220,3 +220,60 @@ def test_clearance_matches_brute_force(device):
             d = min(d, int(np.max(np.abs(np.stack([px - x, py - y, pz - z])), axis=0).min()))
         exp[z, y, x] = d if d <= 15 else 255
     assert np.array_equal(got, exp)
+
+
+def _confine_fields(device, og, vel):
+    from fluxfx_amd.physics.config import GridSpec
+    from fluxfx_amd.backend.detail import DetailPasses
+    grid = GridSpec(og.shape, og.extent, max_dim=512)
+    f32 = [np.asarray(v, np.float32) for v in vel]
+    fields = [device.texture(sh, v.ravel(), nonnegative=False) for sh, v in zip(grid.face_shapes, f32)]
+    outs = [device.texture(sh) for sh in grid.face_shapes]
+    return DetailPasses(grid, device), fields, outs, tuple(v.astype(np.float64) for v in f32)
+
+
+@pytest.mark.parametrize("shape,extent", [((12, 10, 8), (1.3, .8, 1.7)), ((16, 16, 16), (1., 1., 1.))])
+def test_curl_and_confinement_match_oracle(device, shape, extent):
+    """fx_curl / fx_confine against the FP64 restatement of curl.glsl and
+    confinement.glsl on a smooth random flow (relative 1e-5)."""
+    og = mac.Grid(shape, extent)
+    rng = np.random.default_rng(9)
+    vel = []
+    for a in range(3):
+        sz, sy, sx = og.face_array_shape(a)
+        z, y, x = np.meshgrid(np.arange(sz), np.arange(sy), np.arange(sx), indexing="ij")
+        k = rng.uniform(.2, .6, 3)
+        vel.append(np.sin(k[0] * x + 1.3 * a) * np.cos(k[1] * y - .4) * np.sin(k[2] * z + .7 + a))
+    det, fields, outs, vel = _confine_fields(device, og, vel)
+    det.confine(fields, outs, .04, 3.0, 2.0)
+    ref, cu = otr.confine(vel, og, .04, 3.0, 2.0)
+    got = det.curl.tensor.detach().cpu().double().numpy().reshape(cu.shape)
+    assert rel_err(got, cu) < 1e-5
+    for g, o in zip(faces_np(og, outs), ref):
+        assert rel_err(g, o) < 1e-5
+
+
+def test_confinement_reference_gates(device):
+    """scripts/detail_validate.py:60-84 on the GPU: solid-body rotation has
+    curl (0, 0, 2) in the interior (1e-5) and no interior force (1e-6); a
+    nonuniform rotation gets a force that moves faces by 1e-4 .. dt*limit."""
+    n = 12
+    og = mac.Grid((n, n, n))
+    base = []
+    for a in range(3):
+        sz, sy, sx = og.face_array_shape(a)
+        z, y, x = np.meshgrid(np.arange(sz), np.arange(sy), np.arange(sx), indexing="ij")
+        off = otr.FACE_OFFSETS[a]
+        px, py = (x + off[0]) / n, (y + off[1]) / n
+        base.append((-(py - .5), px - .5, 0.0 * px)[a])
+    det, fields, outs, vel = _confine_fields(device, og, base)
+    det.confine(fields, outs, .04, 4, 2)
+    cu = det.curl.tensor.detach().cpu().double().numpy().reshape(n, n, n, 4)[3:n - 3, 3:n - 3, 3:n - 3]
+    assert np.abs(cu[..., 2] - 2).max() < 1e-5 and np.abs(cu[..., 0]).max() < 1e-5
+    for g, o in zip(faces_np(og, outs), vel):
+        assert np.abs(g - o)[4:-4, 4:-4, 4:-4].max() < 1e-6
+    mod = [v * (1 + .5 * np.sin(np.arange(v.size) * .1)).reshape(v.shape) for v in base]
+    det, fields, outs, vel = _confine_fields(device, og, mod)
+    det.confine(fields, outs, .04, 20, .5)
+    delta = max(np.abs(g - o).max() for g, o in zip(faces_np(og, outs), vel))
+    assert 1e-4 < delta <= .04 * .5 + 1e-6

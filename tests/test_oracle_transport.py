@@ -243,3 +243,86 @@ def test_timestep_reference_vectors():
 def test_max_abs_handles_nonfinite():
     assert tr.max_abs([1.0, -3.0, 2.0]) == 3.0
     assert tr.max_abs([1.0, float("nan")]) == 3.4e38
+
+
+def _rotation_faces(grid, omega=1.0, modulate=None):
+    """Solid-body rotation about the vertical axis through the domain centre,
+    sampled at face positions (scripts/detail_validate.py:60-69), optionally
+    scaled by 1 + 0.5 sin(0.1 i) over the flat face index (:80)."""
+    n = grid.shape
+    out = []
+    for a in range(3):
+        sz, sy, sx = grid.face_array_shape(a)
+        z, y, x = np.meshgrid(np.arange(sz), np.arange(sy), np.arange(sx), indexing="ij")
+        off = tr.FACE_OFFSETS[a]
+        px, py = (x + off[0]) / n[0], (y + off[1]) / n[1]
+        v = (-omega * (py - .5), omega * (px - .5), 0.0 * px)[a]
+        if modulate:
+            v = v * (1 + .5 * np.sin(np.arange(v.size) * .1)).reshape(v.shape)
+        out.append(v)
+    return tuple(out)
+
+
+def test_solid_rotation_curl_and_no_interior_force():
+    """detail_validate.py:70-78: curl = (0, 0, 2) in the interior, and a
+    constant curl leaves interior faces untouched."""
+    n = 12
+    g = mac.Grid((n, n, n))
+    vel = _rotation_faces(g)
+    out, cu = tr.confine(vel, g, .04, 4, 2)
+    inner = cu[3:n - 3, 3:n - 3, 3:n - 3]
+    assert np.abs(inner[..., 2] - 2).max() < 1e-12 and np.abs(inner[..., :2]).max() < 1e-12
+    assert np.allclose(inner[..., 3], 2.0)
+    for o, v in zip(out, vel):
+        assert np.abs(o - v)[4:-4, 4:-4, 4:-4].max() < 1e-12
+
+
+def test_affine_flow_curl_closed_form():
+    """An affine velocity u = A p has the constant curl of A away from the
+    clamped boundary cells (central differences are exact on linear fields)."""
+    n = (10, 8, 6)
+    g = mac.Grid(n, (1.3, .8, 1.7))
+    A = np.array([[.3, -1.1, .4], [.7, .2, -.5], [1.3, .9, -.5]])
+    vel = []
+    for a in range(3):
+        sz, sy, sx = g.face_array_shape(a)
+        z, y, x = np.meshgrid(np.arange(sz), np.arange(sy), np.arange(sx), indexing="ij")
+        off = tr.FACE_OFFSETS[a]
+        p = [(x + off[0]) * g.cell_size[0], (y + off[1]) * g.cell_size[1], (z + off[2]) * g.cell_size[2]]
+        vel.append(A[a, 0] * p[0] + A[a, 1] * p[1] + A[a, 2] * p[2])
+    cu = tr.curl(vel, g)
+    expect = np.array([A[2, 1] - A[1, 2], A[0, 2] - A[2, 0], A[1, 0] - A[0, 1]])
+    inner = cu[1:-1, 1:-1, 1:-1, :3]
+    assert np.abs(inner - expect).max() < 1e-12
+
+
+def test_confinement_spins_up_a_vortex():
+    """The force is strength*h*(N x w) with N pointing up the |w| gradient: on
+    the flank of a counter-clockwise vortex whose |w| peaks at its core the
+    force points along the flow (a swapped cross product would brake it)."""
+    n = 24
+    g = mac.Grid((n, n, n))
+    vel = []
+    for a in range(3):
+        sz, sy, sx = g.face_array_shape(a)
+        z, y, x = np.meshgrid(np.arange(sz), np.arange(sy), np.arange(sx), indexing="ij")
+        off = tr.FACE_OFFSETS[a]
+        px, py = (x + off[0]) / n - .5, (y + off[1]) / n - .5
+        s = np.exp(-(px * px + py * py) / .02)
+        vel.append((-py * s, px * s, 0.0 * px)[a])
+    out, cu = tr.confine(vel, g, .05, 2.0, 10.0)
+    k, j = n // 2, n // 2
+    i = n // 2 + 1  # on the +x flank of the core: flow +y, w_z > 0 and falling outwards
+    assert vel[1][k, j, i] > 0 and cu[k, j, i, 2] > 0
+    assert out[1][k, j, i] - vel[1][k, j, i] > 1e-4
+
+
+def test_confinement_force_is_bounded():
+    """detail_validate.py:79-84: a nonuniform rotation gets a finite force,
+    each face changes by at most dt * limit."""
+    n = 12
+    g = mac.Grid((n, n, n))
+    vel = _rotation_faces(g, modulate=True)
+    out, _ = tr.confine(vel, g, .04, 20, .5)
+    delta = max(np.abs(o - v).max() for o, v in zip(out, vel))
+    assert 1e-4 < delta <= .04 * .5 + 1e-12
