@@ -12,12 +12,17 @@ trilinear prolongation, FP64 direct solve on 8^3, warm-started impulse, run to
 ||b - A q|| <= 1e-5 ||b||.
 
 Modes: plain (FP32 stationary V-cycles, the drop-in), mixed (BF16 V-cycles
-preconditioning FP32 flexible CG) and learned (one U-Net preconditioned FPCG
-step, then BF16 V-cycle FPCG), whichever are built.  With the CPU leg the line
-also carries every field's relative error against the FP64 oracle on the same
-scene at 64^3 after 7 steps (``fp64_parity_64``).  ``value`` is the steps/s of the
-learned+mixed mode (of the mixed mode when no trained weights are present);
-every mode's rate and its ratio to plain are in the JSON line.
+preconditioning FP32 flexible CG) and learned (one FPCG step preconditioned by
+the 128^3-trained U-Net applied to the 8-child mean of the residual on the
+128^3 level, its output prolonged and smoothed, then BF16 V-cycle FPCG),
+whichever are built.  With the CPU leg the line also carries every field's
+relative error against the FP64 oracle on the same scene at 64^3 after 7 steps
+(``fp64_parity_64``).  ``value`` is the steps/s of the learned+mixed mode (of
+the mixed mode when no trained weights are present); every mode's rate and
+its ratio to plain are in the JSON line.
+
+``--workload fire`` runs a 384^3 dense fire instead (fuel and heat emitter,
+combustion, vorticity confinement, a static sphere collider; no CPU leg).
 
 Prints ONE JSON line.  Single GPU only.
 """
@@ -82,7 +87,19 @@ def initial_velocity(n, center_cells, radius_cells, seed=3):
     return out
 
 
-def settings_for(n, mode):
+def fire_sphere(n, seed=5):
+    """Static sphere collider of the fire workload: radius 24 cells at 384^3,
+    seeded centre in the upper half of the domain (cells)."""
+    rng = np.random.default_rng(seed)
+    return tuple(rng.uniform(.3, .7, 3) * n * np.array([1, 1, 1.2])), 24 * n / 384
+
+
+def sphere_mask(n, center, radius):
+    z, y, x = np.meshgrid(*(np.arange(n) + .5,) * 3, indexing="ij")
+    return (x - center[0]) ** 2 + (y - center[1]) ** 2 + (z - center[2]) ** 2 <= radius ** 2
+
+
+def settings_for(n, mode, workload="smoke"):
     from fluxfx_amd.physics.pressure import PressureSettings
     center_cells, radius_cells, _ = scene(n)
     dt = 0.5 / n
@@ -91,6 +108,12 @@ def settings_for(n, mode):
               thermal_lift=1.0, density_weight=0.0, source_profile="SOLID", density_mode="RATE",
               initial_temperature=0.0, velocity_advection="MACCORMACK", scalar_advection="MACCORMACK",
               pressure_tolerance=1e-5, pressure_max_cycles=60, mg_pre=2, mg_post=2, mg_coarsest=8)
+    if workload == "fire":
+        # fuel and heat injected by the emitter; a quarter of the fuel above the
+        # ignition temperature burns per step, heat release 1.0, smoke yield 0.5;
+        # vorticity confinement 0.3
+        kw.update(source_rate=0.0, fuel_source_rate=1.0 / dt, combustion_enabled=True, ignition_temperature=0.3,
+                  burn_rate=-math.log(0.75) / dt, heat_yield=1.0, smoke_yield=0.5, vorticity_strength=0.3)
     if mode == "mixed":
         kw.update(pressure_precision="BF16", pressure_krylov="PCG")
     elif mode == "learned":
@@ -103,13 +126,14 @@ def settings_for(n, mode):
     return PressureSettings(**kw), dt
 
 
-def make_sim(device, n, mode):
+def make_sim(device, n, mode, workload="smoke"):
     from fluxfx_amd.physics.config import GridSpec
     from fluxfx_amd.backend.projected import DenseProjectedSmoke
     center_cells, radius_cells, boxes = scene(n)
-    settings, dt = settings_for(n, mode)
+    settings, dt = settings_for(n, mode, workload)
     grid = GridSpec((n, n, n), (1.0, 1.0, 1.0), max_dim=1024)
-    sim = DenseProjectedSmoke(grid, settings, device, solid_mask=box_mask(n, boxes).ravel())
+    solid = box_mask(n, boxes) if workload == "smoke" else sphere_mask(n, *fire_sphere(n))
+    sim = DenseProjectedSmoke(grid, settings, device, solid_mask=solid.ravel())
     sim.upload(np.zeros(n ** 3, np.float32))
     sim.upload_temperature(np.zeros(n ** 3, np.float32))
     sim.upload_velocity([v.ravel() for v in initial_velocity(n, center_cells, radius_cells)])
@@ -252,9 +276,12 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--n", type=int, default=256)
+    ap.add_argument("--n", type=int, default=None, help="grid side (default 256 smoke, 384 fire)")
     ap.add_argument("--modes", default="plain,mixed,learned")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--workload", choices=("smoke", "fire"), default="smoke",
+                    help="smoke: the 256^3 headline scene; fire: 384^3 dense fire (combustion, vorticity "
+                         "confinement, static sphere collider), no CPU leg")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: the step runs on a single MI355X; "
@@ -262,12 +289,13 @@ def main():
         sys.exit(2)
     from fluxfx_amd.backend.device import HipDevice
     device = HipDevice(0)
-    n = args.n
+    fire = args.workload == "fire"
+    n = args.n or (384 if fire else 256)
     results = {}
     roof = None
     for mode in [m for m in args.modes.split(",") if m]:
         try:
-            sim, dt = make_sim(device, n, mode)
+            sim, dt = make_sim(device, n, mode, args.workload)
             wall, cycles, m = timed_run(device, sim, dt, args.steps, args.warmup)
         except NotImplementedError as exc:
             results[mode] = {"status": f"not built: {exc}"}
@@ -287,12 +315,19 @@ def main():
     best_mode = next((m for m in ("learned", "mixed", "plain") if m in ok), None)
     value = ok[best_mode]["steps_per_s"] if best_mode else 0.0
     plain = results.get("plain", {}).get("steps_per_s")
+    if fire:
+        metric = f"{n}³ fire steps/s at 1e-5 rel. residual (dense grid: combustion, vorticity confinement, sphere)"
+        workload = (f"{n}^3 fire step: fuel and heat emitter, combustion (25% burn per step above T=0.3, heat "
+                    f"1.0, smoke 0.5), vorticity confinement 0.3, static sphere collider, MacCormack, "
+                    f"projected to 1e-5 relative residual")
+    else:
+        metric = METRIC
+        workload = f"{n}^3 smoke step with 8 box colliders, MacCormack, projected to 1e-5 relative residual"
     line = {
-        "metric": METRIC, "value": value, "unit": "steps/s", "n_gpus": 1, "steps": args.steps,
+        "metric": metric, "value": value, "unit": "steps/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": 1e3 / value if value else None, "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
-        "config": {"workload": f"{n}^3 smoke step with 8 box colliders, MacCormack, projected to 1e-5 "
-                               f"relative residual", "grid": [n, n, n], "dt": 0.5 / n, "mode": best_mode,
+        "config": {"workload": workload, "grid": [n, n, n], "dt": 0.5 / n, "mode": best_mode,
                    "parallelism": "single GPU"},
         "modes": results,
         "speedup_vs_plain": (value / plain) if (plain and value) else None,
@@ -305,7 +340,7 @@ def main():
                             "frac": roof["achieved_gbs"] / HBM_PEAK_GBS, "traffic": measured_traffic(),
                             "kernel": roof["kernel"], "avg_ms": roof["avg_ms"],
                             "bytes_per_launch": roof["bytes_per_launch"]}
-    if not args.no_cpu:
+    if not args.no_cpu and not fire:
         base, state = cpu_baseline()
         line["cpu_baseline"] = base
         line["fp64_parity_64"] = parity_vs_oracle(device, state, 64, 7, [m for m in results if "steps_per_s" in

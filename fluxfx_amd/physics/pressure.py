@@ -56,6 +56,13 @@ _MESSAGES = {
 }
 
 
+# fields with no reference counterpart (kept apart in checkpoints so that the
+# reference's loader reads the settings it knows)
+EXTENSION_FIELDS = ("pressure_tolerance", "pressure_max_cycles", "pressure_precision", "pressure_krylov", "mg_pre",
+                    "mg_post", "mg_coarsest", "mg_restriction", "learned_weights", "learned_iterations",
+                    "learned_sweeps", "learned_level")
+
+
 def _int_in(value, low, high):
     return type(value) is int and low <= value <= high
 

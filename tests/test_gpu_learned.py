@@ -36,7 +36,8 @@ def _learned_solve(device, net, sample, learned, budget, sweeps=2, level=0):
     from fluxfx_amd.physics.config import GridSpec
     from fluxfx_amd.physics.pressure import PressureSettings
     b, target, solid, inv_count = sample
-    grid = GridSpec((N, N, N), (1., 1., 1.))
+    n = round(b.numel() ** (1 / 3))
+    grid = GridSpec((n, n, n), (1., 1., 1.))
     solids = None if solid is None else SolidMask(grid, device=device, mask=solid.cpu().numpy())
     s = PressureSettings(pressure_tolerance=1e-5, mg_pre=2, mg_post=2, mg_coarsest=8, pressure_max_cycles=budget,
                          pressure_precision="FP32", pressure_krylov="LEARNED", learned_iterations=learned,
@@ -75,24 +76,21 @@ def test_learned_solve_reaches_tolerance_with_fallback(device, trained):
 
 
 @pytest.fixture(scope="module")
-def trained_coarse(device):
-    """A network trained on pairs harvested at N/2, for level-1 application on N^3 solves."""
-    from fluxfx_amd.learned import UNetPreconditioner
+def held_out_fine(device):
+    """Held-out pairs at (2N)^3, for level-1 application of the N^3-trained network."""
     from fluxfx_amd import training
-    train_set = training.harvest(device, N // 2, scenes=6, steps_per_scene=4, seed0=11000)
-    net = UNetPreconditioner(device, seed=2)
-    training.train(net, train_set, (N // 2,) * 3, steps=400, lr=2e-3, log_every=100)
-    held_out = training.harvest(device, N, scenes=2, steps_per_scene=2, seed0=21000)
-    return net, held_out
+    return training.harvest(device, 2 * N, scenes=2, steps_per_scene=2, seed0=21000)
 
 
-def test_coarse_level_network_beats_untrained_and_reaches_tolerance(device, trained_coarse):
-    """Level-1 application (restricted residual in, prolonged correction out):
-    the trained network leaves a smaller true residual than the untrained one
-    after the same FPCG iterations, and the MG fallback finishes at 1e-5."""
+def test_coarse_level_network_beats_untrained_and_reaches_tolerance(device, trained, held_out_fine):
+    """Level-1 application (restricted residual in, prolonged correction out)
+    of the N^3-trained network on (2N)^3 solves: the trained network leaves a
+    smaller true residual than the untrained one after the same FPCG
+    iterations, and the MG fallback finishes at 1e-5."""
     from fluxfx_amd.learned import UNetPreconditioner
-    net, held_out = trained_coarse
-    fresh = UNetPreconditioner(device, seed=2)
+    net = trained[0]
+    held_out = held_out_fine
+    fresh = UNetPreconditioner(device, seed=1)
     print("coarse-level learned preconditioner: ablation and fallback", flush=True)
     for sample in held_out[:2]:
         mt = _learned_solve(device, net, sample, 2, 2, level=1)

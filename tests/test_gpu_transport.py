@@ -254,10 +254,13 @@ def test_curl_and_confinement_match_oracle(device, shape, extent):
 
 
 def test_confinement_reference_gates(device):
-    """scripts/detail_validate.py:60-84 on the GPU: solid-body rotation has
-    curl (0, 0, 2) in the interior (1e-5) and no interior force (1e-6); a
-    nonuniform rotation gets a force that moves faces by 1e-4 .. dt*limit."""
-    n = 12
+    """scripts/detail_validate.py:60-84 on the GPU (its 32^3 grid): solid-body
+    rotation has curl (0, 0, 2) in the interior (1e-5) and no interior force
+    (1e-6); a nonuniform rotation gets a force that moves faces by
+    1e-4 .. dt*limit.  On a power-of-two grid the rotation's face values and
+    their differences are exact in fp32, so |w| is exactly uniform and its
+    gradient stays under the shader's 1e-8 cut-off, as in the reference."""
+    n = 32
     og = mac.Grid((n, n, n))
     base = []
     for a in range(3):
