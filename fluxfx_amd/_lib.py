@@ -13,7 +13,6 @@ FX_F32, FX_BF16, FX_F16 = 0, 1, 2
 FX_OP_BOX, FX_OP_FLAGS, FX_OP_EDGES, FX_OP_PACKED = 0, 1, 2, 3
 FX_RESTRICT_MEAN, FX_RESTRICT_FULL = 0, 1
 FX_PROLONG_TRILINEAR, FX_PROLONG_RENORM, FX_PROLONG_CONSTANT = 0, 1, 2
-FX_MG_NO_TAIL = 1  # fx_mg_config.options: one launch per small-level sweep
 MAX_LEVELS = 16
 FX_WS_DOUBLES = 16400  # FP64 reduction workspace (16 scalars + 16 x 1024 block partials)
 
@@ -41,7 +40,7 @@ class fx_mg_config(C.Structure):
     _fields_ = [("levels", C.c_int32), ("storage", C.c_int32), ("pre", C.c_int32), ("post", C.c_int32),
                 ("omega", C.c_float), ("restriction", C.c_int32), ("prolongation", C.c_int32),
                 ("coarse_sweeps", C.c_int32), ("coarse_matrix", C.c_void_p), ("coarse_scale", C.c_float),
-                ("options", C.c_int32), ("level", fx_mg_level * MAX_LEVELS)]
+                ("pad_", C.c_int32), ("level", fx_mg_level * MAX_LEVELS)]
 
 
 class fx_conv_desc(C.Structure):
@@ -120,7 +119,7 @@ SIGNATURES = {
     "fx_adam": ([I64, P, P, P, P, I32, F32, F32, F32, F32, P], C.c_int),
     "fx_to_bf16": ([I64, P, P, P], C.c_int),
     "fx_unet_forward": ([C.POINTER(fx_unet), P, P, P, F32, I32, P, P], C.c_int),
-    "fx_learned_pcg_solve": ([C.POINTER(fx_unet), C.POINTER(fx_mg_config), P, P, P, P, P, P, P, P, P, P, P, P,
+    "fx_learned_pcg_solve": ([C.POINTER(fx_unet), C.POINTER(fx_mg_config), P, P, P, P, P, P, P, P, P, P, P,
                               F32, I32, I32, C.POINTER(fx_solve_params), P, C.POINTER(fx_solve_info), P], C.c_int),
     "fx_combine": ([P, P, P, I64, P], C.c_int),
     "fx_curl": ([fx_grid, P, P, P, P, P], C.c_int),
@@ -145,7 +144,7 @@ SIGNATURES = {
                     C.c_int),
     "fx_mixed_solve": ([C.POINTER(fx_mg_config), P, P, P, C.POINTER(fx_solve_params), P,
                         C.POINTER(fx_solve_info), C.POINTER(C.POINTER(C.c_float)), P], C.c_int),
-    "fx_pcg_solve": ([C.POINTER(fx_mg_config), P, P, P, P, P, P, P, C.POINTER(fx_solve_params), P,
+    "fx_pcg_solve": ([C.POINTER(fx_mg_config), P, P, P, P, P, P, C.POINTER(fx_solve_params), P,
                       C.POINTER(fx_solve_info), P], C.c_int),
     "fx_face_predict": ([fx_grid, F32, P, P, P, P, P, P, P, P, P], C.c_int),
     "fx_face_predict_bounds": ([fx_grid, F32, P, P, P, P, P, P, P, P, P], C.c_int),

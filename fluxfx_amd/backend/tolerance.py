@@ -214,10 +214,9 @@ class ToleranceProjector(PressureProjector):
         arr = eng.arrays[0]
         used = {eng.cfg.level[0].p, eng.cfg.level[0].q}
         zold = next(t for t in (arr["z"], arr["p"], arr["q"]) if t.data_ptr() not in used)
-        # d2 is free during a Krylov solve: second direction buffer for the fused update
         _lib.call("fx_pcg_solve", C.byref(eng.cfg), inc["d"].ptr, inc["r0"].ptr, v["r"].data_ptr(),
-                  v["p"].data_ptr(), v["Ap"].data_ptr(), zold.data_ptr(), inc["d2"].ptr, C.byref(params),
-                  self.ws.data_ptr(), C.byref(info), self.device.stream)
+                  v["p"].data_ptr(), v["Ap"].data_ptr(), zold.data_ptr(), C.byref(params), self.ws.data_ptr(),
+                  C.byref(info), self.device.stream)
         return info
 
     def attach_network(self, network):
@@ -257,8 +256,7 @@ class ToleranceProjector(PressureProjector):
         info = _lib.fx_solve_info()
         _lib.call("fx_learned_pcg_solve", C.byref(buf["net"]), C.byref(eng.cfg), inc["d"].ptr, inc["r0"].ptr,
                   v["r"].data_ptr(), v["p"].data_ptr(), v["Ap"].data_ptr(), v["zold"].data_ptr(),
-                  v["z"].data_ptr(), v["rT"].data_ptr(), v["ztmp"].data_ptr(), mg_zold.data_ptr(), inc["d2"].ptr,
-                  solid,
+                  v["z"].data_ptr(), v["rT"].data_ptr(), v["ztmp"].data_ptr(), mg_zold.data_ptr(), solid,
                   float(post), int(self.settings.learned_iterations), int(self.settings.learned_sweeps),
                   C.byref(params), self.ws.data_ptr(), C.byref(info), self.device.stream)
         return info

@@ -373,75 +373,6 @@ __global__ void k_prolong(LevelDev F, LevelDev C, const T* in, const T* __restri
   out[i] = from_f<T>(prolong_cell<T, FOP, COP, MODE>(F, C, x, y, z, i, to_f(in[i]), coarse));
 }
 
-// ---------------------------------------------------------------------------
-// Small levels (at most kTailCells cells) in one block of 1024 threads: the
-// level's iterate lives in LDS in its storage type (rounded after every sweep
-// exactly as the level arrays are), so a level's pre-smoothing + residual
-// restriction, and its prolongation + post-smoothing, are one launch each
-// instead of one launch per sweep.  Per-cell arithmetic: smooth_cell,
-// residual_at and prolong_cell, shared with the one-cell kernels.
-// ---------------------------------------------------------------------------
-constexpr int kTailCells = 4096;
-constexpr int kTailThreads = 1024;
-
-template <typename T, int OP>
-__device__ __forceinline__ int tail_sweeps(const LevelDev& L, T (&buf)[2][kTailCells], int cur, const T* __restrict__ rhs,
-                                           float inv_dt, float omega, int sweeps, bool zero_first) {
-  const int n = L.nx * L.ny * L.nz;
-  for (int k = 0; k < sweeps; ++k) {
-    for (int i = threadIdx.x; i < n; i += kTailThreads) {
-      const int x = i % L.nx, r = i / L.nx, y = r % L.ny, z = r / L.ny;
-      buf[cur ^ 1][i] = from_f<T>(smooth_cell<T, OP>(L, x, y, z, i, buf[cur], rhs, inv_dt, omega,
-                                                     !(zero_first && k == 0)));
-    }
-    __syncthreads();
-    cur ^= 1;
-  }
-  return cur;
-}
-
-// pre-smoothing from p_in (or from zero) stored to pout, then crhs = 8-child mean residual
-template <typename T, int OP>
-__global__ __launch_bounds__(kTailThreads) void k_tail_down(LevelDev L, int cnx, int cny, int cnz, const T* p_in,
-                                                            const T* __restrict__ rhs, T* pout, T* __restrict__ crhs,
-                                                            float inv_dt, float omega, int sweeps, int zero_guess) {
-  __shared__ T buf[2][kTailCells];
-  const int n = L.nx * L.ny * L.nz;
-  for (int i = threadIdx.x; i < n; i += kTailThreads) buf[0][i] = zero_guess ? from_f<T>(0.0f) : p_in[i];
-  __syncthreads();
-  const int cur = tail_sweeps<T, OP>(L, buf, 0, rhs, inv_dt, omega, sweeps, zero_guess != 0);
-  for (int i = threadIdx.x; i < n; i += kTailThreads) pout[i] = buf[cur][i];
-  const int cn = cnx * cny * cnz;
-  for (int c = threadIdx.x; c < cn; c += kTailThreads) {
-    const int X = c % cnx, r = c / cnx, Y = r % cny, Z = r / cny;
-    float acc = 0.0f;
-#pragma unroll
-    for (int dz = 0; dz < 2; ++dz)
-#pragma unroll
-      for (int dy = 0; dy < 2; ++dy)
-#pragma unroll
-        for (int dx = 0; dx < 2; ++dx)
-          acc += residual_at<T, OP>(L, 2 * X + dx, 2 * Y + dy, 2 * Z + dz, buf[cur], rhs, inv_dt);
-    crhs[c] = from_f<T>(acc * 0.125f);
-  }
-}
-
-// p = prolong(p_in, coarse), then post-smoothing, stored to pout
-template <typename T, int FOP, int COP, int MODE>
-__global__ __launch_bounds__(kTailThreads) void k_tail_up(LevelDev F, LevelDev C, const T* p_in,
-                                                          const T* __restrict__ coarse, const T* __restrict__ rhs,
-                                                          T* pout, float inv_dt, float omega, int sweeps) {
-  __shared__ T buf[2][kTailCells];
-  const int n = F.nx * F.ny * F.nz;
-  for (int i = threadIdx.x; i < n; i += kTailThreads) {
-    const int x = i % F.nx, r = i / F.nx, y = r % F.ny, z = r / F.ny;
-    buf[0][i] = from_f<T>(prolong_cell<T, FOP, COP, MODE>(F, C, x, y, z, i, to_f(p_in[i]), coarse));
-  }
-  __syncthreads();
-  const int cur = tail_sweeps<T, FOP>(F, buf, 0, rhs, inv_dt, omega, sweeps, false);
-  for (int i = threadIdx.x; i < n; i += kTailThreads) pout[i] = buf[cur][i];
-}
-
 // p = scale * M rhs, one wavefront per row, FP64 accumulation
 template <typename T>
 __global__ void k_direct(int m, const double* __restrict__ M, const T* __restrict__ rhs, T* __restrict__ out,
@@ -670,75 +601,12 @@ static int launch_fill_zero(fx_mg_config* c, void* ptr, int64_t n, hipStream_t s
   return FX_OK;
 }
 
-// level l runs its pre- and post-smoothing in single-block tail kernels
-static bool tail_ok(const fx_mg_config* c, int l) {
-  const fx_mg_level& L = c->level[l];
-  return !(c->options & FX_MG_NO_TAIL) && l + 1 < c->levels && c->restriction == FX_RESTRICT_MEAN && int64_t(L.nx) * L.ny * L.nz <= kTailCells &&
-         c->prolongation >= 0 && c->prolongation <= 2 && c->pre >= 0 && c->post >= 0 &&
-         (c->prolongation != FX_PROLONG_RENORM || c->level[l + 1].op != FX_OP_FLAGS);
-}
-
-// host side of smooth_n's ping-pong: the array holding the iterate after `count` sweeps
-static void *tail_swap(fx_mg_level& L, int count) {
-  if (count & 1) {
-    void* t = L.p;
-    L.p = L.q;
-    L.q = t;
-  }
-  return L.p;
-}
-
-static int launch_tail_down(fx_mg_config* c, int l, float inv_dt, bool zero_guess, hipStream_t s) {
-  fx_mg_level& L = c->level[l];
-  const fx_mg_level& C = c->level[l + 1];
-  LevelDev D = dev_level(L);
-  const void* p_in = L.p;
-  void* pout = tail_swap(L, c->pre);
-  FX_DISPATCH_T(c->storage, FX_DISPATCH_OP(L.op, {
-    hipLaunchKernelGGL((k_tail_down<TV, OPV>), dim3(1), dim3(kTailThreads), 0, s, D, C.nx, C.ny, C.nz,
-                       (const TV*)p_in, (const TV*)L.rhs, (TV*)pout, (TV*)C.rhs, inv_dt, c->omega, c->pre,
-                       zero_guess ? 1 : 0);
-  }));
-  FX_LAUNCHED("k_tail_down");
-  return FX_OK;
-}
-
-static int launch_tail_up(fx_mg_config* c, int l, float inv_dt, hipStream_t s) {
-  fx_mg_level& F = c->level[l];
-  const fx_mg_level& C = c->level[l + 1];
-  LevelDev DF = dev_level(F), DC = dev_level(C);
-  const void* p_in = F.p;
-  void* pout = tail_swap(F, c->post);
-  const int mode = c->prolongation;
-#define FX_TUP(FO, CO, MO)                                                                                   FX_DISPATCH_T(c->storage, {                                                                                  hipLaunchKernelGGL((k_tail_up<TV, FO, CO, MO>), dim3(1), dim3(kTailThreads), 0, s, DF, DC,                                   (const TV*)p_in, (const TV*)C.p, (const TV*)F.rhs, (TV*)pout, inv_dt, c->omega,                            c->post);                                                                             })
-  // coarse operator as launch_prolong picks it: the activity test of renormalised prolongation only
-  if (mode == FX_PROLONG_CONSTANT) {
-    FX_DISPATCH_OP(F.op, { FX_TUP(OPV, FX_OP_BOX, FX_PROLONG_CONSTANT); });
-  } else if (mode == FX_PROLONG_TRILINEAR) {
-    FX_DISPATCH_OP(F.op, { FX_TUP(OPV, FX_OP_BOX, FX_PROLONG_TRILINEAR); });
-  } else if (C.op == FX_OP_EDGES) {
-    FX_DISPATCH_OP(F.op, { FX_TUP(OPV, FX_OP_EDGES, FX_PROLONG_RENORM); });
-  } else if (C.op == FX_OP_PACKED) {
-    FX_DISPATCH_OP(F.op, { FX_TUP(OPV, FX_OP_PACKED, FX_PROLONG_RENORM); });
-  } else {
-    FX_DISPATCH_OP(F.op, { FX_TUP(OPV, FX_OP_BOX, FX_PROLONG_RENORM); });
-  }
-#undef FX_TUP
-  FX_LAUNCHED("k_tail_up");
-  return FX_OK;
-}
-
 // Recursive V-cycle.  zero_guess: the level's current iterate is 0.
 int vcycle(fx_mg_config* c, int l, float inv_dt, bool zero_guess, hipStream_t s) {
   fx_mg_level& L = c->level[l];
   if (l == c->levels - 1) {
     if (c->coarse_sweeps > 0) return smooth_n(c, l, c->coarse_sweeps, inv_dt, zero_guess, s);
     return launch_direct(c, s);
-  }
-  if (tail_ok(c, l)) {
-    if (int e = launch_tail_down(c, l, inv_dt, zero_guess, s)) return e;
-    if (int e = vcycle(c, l + 1, 1.0f, true, s)) return e;
-    return launch_tail_up(c, l, inv_dt, s);
   }
   if (c->pre > 0) {
     if (int e = smooth_n(c, l, c->pre, inv_dt, zero_guess, s)) return e;

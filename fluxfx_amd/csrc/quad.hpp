@@ -222,15 +222,13 @@ __device__ __forceinline__ QuadStencil quad_stencil(const LevelDev& L, int x0, i
   F4 c{}, ym{}, yp{}, zm{}, zp{};
   float xl = 0.0f, xr = 0.0f;
   if (read_p) {
-    // every load unconditional (a neighbour outside the level reads the own
-    // cell instead, and its closed face never uses it), so one wait covers all
     c = ld.quad(i);
-    ym = ld.quad(y > 0 ? i - sy : i);
-    yp = ld.quad(y + 1 < L.ny ? i + sy : i);
-    zm = ld.quad(z > 0 ? i - sz : i);
-    zp = ld.quad(z + 1 < L.nz ? i + sz : i);
-    xl = ld.one(x0 > 0 ? i - 1 : i);
-    xr = ld.one(x0 + 4 < L.nx ? i + 4 : i);
+    if (y > 0) ym = ld.quad(i - sy);
+    if (y + 1 < L.ny) yp = ld.quad(i + sy);
+    if (z > 0) zm = ld.quad(i - sz);
+    if (z + 1 < L.nz) zp = ld.quad(i + sz);
+    if (x0 > 0) xl = ld.one(i - 1);
+    if (x0 + 4 < L.nx) xr = ld.one(i + 4);
   }
   QuadStencil st;
 #pragma unroll
@@ -285,6 +283,35 @@ __device__ __forceinline__ UV<W> ld_words(const uint32_t* __restrict__ p, int64_
   return r;
 }
 
+// Neighbour operand loads of a W-wide stencil.  The 8-wide (16-bit storage)
+// kernels issue every load unconditionally (a neighbour outside the level
+// reads the lane's own cells, which a closed face or zero conductance never
+// uses), so one wait covers them all and the per-cell branches go; the 4-wide
+// fp32 kernels keep the guarded loads, which hold fewer registers (more waves
+// in flight).
+template <int W, class Ld>
+__device__ __forceinline__ void vec_neighbours(const LevelDev& L, int x0, int y, int z, int64_t i, const Ld& ld,
+                                               FV<W>& c, FV<W>& ym, FV<W>& yp, FV<W>& zm, FV<W>& zp, float& xl,
+                                               float& xr) {
+  const int64_t sy = L.nx, sz = int64_t(L.nx) * L.ny;
+  c = ld.template vec<W>(i);
+  if constexpr (W == 8) {
+    ym = ld.template vec<W>(y > 0 ? i - sy : i);
+    yp = ld.template vec<W>(y + 1 < L.ny ? i + sy : i);
+    zm = ld.template vec<W>(z > 0 ? i - sz : i);
+    zp = ld.template vec<W>(z + 1 < L.nz ? i + sz : i);
+    xl = ld.one(x0 > 0 ? i - 1 : i);
+    xr = ld.one(x0 + W < L.nx ? i + W : i);
+  } else {
+    if (y > 0) ym = ld.template vec<W>(i - sy);
+    if (y + 1 < L.ny) yp = ld.template vec<W>(i + sy);
+    if (z > 0) zm = ld.template vec<W>(i - sz);
+    if (z + 1 < L.nz) zp = ld.template vec<W>(i + sz);
+    if (x0 > 0) xl = ld.one(i - 1);
+    if (x0 + W < L.nx) xr = ld.one(i + W);
+  }
+}
+
 // per-cell bits of W cells: open-face/fluid flags (FLAGS, BOX) or FLAG_FLUID
 // for an active cell (PACKED)
 template <int W>
@@ -320,29 +347,29 @@ __device__ __forceinline__ VecStencil<W> vec_stencil_packed(const LevelDev& L, i
                                                             const Ld& ld, bool read_p) {
   const int64_t sy = L.nx, sz = int64_t(L.nx) * L.ny;
   const uint32_t* __restrict__ pk = reinterpret_cast<const uint32_t*>(L.op);
-  // unconditional loads (own cell for a neighbour outside the level); the
-  // words of missing lower neighbours are zeroed after the loads, the p
-  // values sit behind zero conductances
+  // the words of missing lower neighbours read as zero (unconditional loads
+  // and a select in the 8-wide kernels, guarded loads in the 4-wide ones)
   const UV<W> wo = ld_words<W>(pk, i);
-  UV<W> wym = ld_words<W>(pk, y > 0 ? i - sy : i), wzm = ld_words<W>(pk, z > 0 ? i - sz : i);
-  uint32_t wxl = pk[x0 > 0 ? i - 1 : i];
+  UV<W> wym{}, wzm{};
+  uint32_t wxl = 0;
+  if constexpr (W == 8) {
+    wym = ld_words<W>(pk, y > 0 ? i - sy : i);
+    wzm = ld_words<W>(pk, z > 0 ? i - sz : i);
+    wxl = pk[x0 > 0 ? i - 1 : i];
 #pragma unroll
-  for (int k = 0; k < W; ++k) {
-    wym.v[k] = y > 0 ? wym.v[k] : 0u;
-    wzm.v[k] = z > 0 ? wzm.v[k] : 0u;
+    for (int k = 0; k < W; ++k) {
+      wym.v[k] = y > 0 ? wym.v[k] : 0u;
+      wzm.v[k] = z > 0 ? wzm.v[k] : 0u;
+    }
+    wxl = x0 > 0 ? wxl : 0u;
+  } else {
+    if (y > 0) wym = ld_words<W>(pk, i - sy);
+    if (z > 0) wzm = ld_words<W>(pk, i - sz);
+    if (x0 > 0) wxl = pk[i - 1];
   }
-  wxl = x0 > 0 ? wxl : 0u;
   FV<W> c{}, ym{}, yp{}, zm{}, zp{};
   float xl = 0.0f, xr = 0.0f;
-  if (read_p) {
-    c = ld.template vec<W>(i);
-    ym = ld.template vec<W>(y > 0 ? i - sy : i);
-    yp = ld.template vec<W>(y + 1 < L.ny ? i + sy : i);
-    zm = ld.template vec<W>(z > 0 ? i - sz : i);
-    zp = ld.template vec<W>(z + 1 < L.nz ? i + sz : i);
-    xl = ld.one(x0 > 0 ? i - 1 : i);
-    xr = ld.one(x0 + W < L.nx ? i + W : i);
-  }
+  if (read_p) vec_neighbours<W>(L, x0, y, z, i, ld, c, ym, yp, zm, zp, xl, xr);
   VecStencil<W> st;
 #pragma unroll
   for (int k = 0; k < W; ++k) {
@@ -381,16 +408,7 @@ __device__ __forceinline__ VecStencil<W> vec_stencil(const LevelDev& L, int x0, 
   vec_flags<W>(L, OP, x0, y, z, i, st.f);
   FV<W> c{}, ym{}, yp{}, zm{}, zp{};
   float xl = 0.0f, xr = 0.0f;
-  if (read_p) {
-    // unconditional loads: see quad_stencil
-    c = ld.template vec<W>(i);
-    ym = ld.template vec<W>(y > 0 ? i - sy : i);
-    yp = ld.template vec<W>(y + 1 < L.ny ? i + sy : i);
-    zm = ld.template vec<W>(z > 0 ? i - sz : i);
-    zp = ld.template vec<W>(z + 1 < L.nz ? i + sz : i);
-    xl = ld.one(x0 > 0 ? i - 1 : i);
-    xr = ld.one(x0 + W < L.nx ? i + W : i);
-  }
+  if (read_p) vec_neighbours<W>(L, x0, y, z, i, ld, c, ym, yp, zm, zp, xl, xr);
 #pragma unroll
   for (int k = 0; k < W; ++k) {
     const uint32_t f = st.f[k];

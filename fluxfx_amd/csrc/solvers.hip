@@ -229,62 +229,6 @@ __global__ void k_pcg_apply_q(LevelDev L, const float* __restrict__ p, float* __
   block_partial(acc, ws, q);
 }
 
-// Reads of the search direction p_new = z*zscale + beta*p_old (p_old ignored
-// when `first`), formed on the fly at every stencil point.
-template <typename T>
-struct DirLoad {
-  const T* __restrict__ z;
-  const float* __restrict__ p;
-  float zs, beta;
-  int first;
-  __device__ __forceinline__ float at(float zv, float pv) const {
-    return first ? zv * zs : __builtin_fmaf(beta, pv, zv * zs);
-  }
-  __device__ __forceinline__ F4 quad(int64_t j) const {
-    const F4 zv = ld4(z, j);
-    const F4 pv = first ? F4{} : ld4(p, j);
-    F4 r;
-#pragma unroll
-    for (int k = 0; k < 4; ++k) r.v[k] = at(zv.v[k], pv.v[k]);
-    return r;
-  }
-  __device__ __forceinline__ float one(int64_t j) const { return at(to_f(z[j]), first ? 0.0f : p[j]); }
-};
-
-__device__ __forceinline__ float pcg_beta(const double* ws, int q_new, int q_cross, int q_old, int first) {
-  if (first) return 0.0f;
-  const double den = ws[q_old];
-  const float beta = den != 0.0 ? float((ws[q_new] - ws[q_cross]) / den) : 0.0f;
-  return fmaxf(beta, 0.0f);
-}
-
-// Fused direction update and operator apply: p_new = z*zscale + beta*p_old
-// (beta = max(0, (rz_new - rz_cross)/rz_old) from ws), Ap = -L p_new, partial
-// p_new.Ap -> q.  p_old and p_new live in different buffers (every lane reads
-// its neighbours' p_old).
-template <int OP, typename T>
-__global__ void k_pcg_dir_apply_q(LevelDev L, const double* __restrict__ ws_in, int q_new, int q_cross, int q_old,
-                                  int first, const T* __restrict__ z, float zscale, const float* __restrict__ p_old,
-                                  float* __restrict__ p_new, float* __restrict__ Ap, double* ws, int q, int nq) {
-  const float beta = pcg_beta(ws_in, q_new, q_cross, q_old, first);
-  const DirLoad<T> ld{z, p_old, zscale, beta, first};
-  double acc = 0.0;
-  FX_FOR_QUADS(nq) {
-    const QuadPos at = quad_pos(L, q_);
-    const QuadStencil st = quad_stencil<OP>(L, at.x, at.y, at.z, at.i, ld, true);
-    F4 a, pc;
-#pragma unroll
-    for (int k = 0; k < 4; ++k) {
-      a.v[k] = (st.f[k] & FLAG_FLUID) ? -st.s[k] : 0.0f;
-      pc.v[k] = st.c[k];
-      acc += double(st.c[k]) * double(a.v[k]);
-    }
-    st4(Ap, at.i, a);
-    st4(p_new, at.i, pc);
-  }
-  block_partial(acc, ws, q);
-}
-
 template <typename T>
 __global__ void k_pcg_update_q(int nq, double* ws, int q_rz, int q_pap, float* __restrict__ x,
                                const float* __restrict__ p, float* __restrict__ r, const float* __restrict__ Ap,
@@ -399,7 +343,6 @@ struct PcgVectors {
   float* p;
   float* Ap;
   void* zold;
-  float* p2;  // second direction buffer (may be null): enables the fused direction + apply kernel
 };
 
 template <typename Precond>
@@ -411,7 +354,6 @@ int pcg_loop(const fx_mg_level& L0, int storage, void* rT, void** z_slot, const 
   const bool fp16 = storage == FX_F16;
   // every vector (and both preconditioner slots) 16-byte aligned: quad kernels
   const bool quad = quad_ok(L0, v.x, v.r, v.p) && quad_ok(L0, v.Ap, v.zold, b) && quad_ok(L0, rT, *z_slot);
-  const bool fused = quad && v.p2 && quad_ok(L0, v.p2);
   const int nq = int(n / 4);
   int ref_q;
   if (int e = reference_slot(prm, n, b, ws, s, &ref_q)) return e;
@@ -459,8 +401,7 @@ int pcg_loop(const fx_mg_level& L0, int storage, void* rT, void** z_slot, const 
     }
     if (int e = precond(s)) return e;
     float zscale = float(-scale);
-    // direction p = z*zscale + beta*p from slots (q_new, q_cross, q_old); the
-    // fused path forms it inside the next operator apply
+    // direction p = z*zscale + beta*p from slots (q_new, q_cross, q_old)
     int dir_first = 1, dq_new = q_rz, dq_cross = q_rz, dq_old = q_rz;
     auto dot_rz = [&](int q) -> int {
       FX_DISPATCH_T(storage, {
@@ -485,24 +426,9 @@ int pcg_loop(const fx_mg_level& L0, int storage, void* rT, void** z_slot, const 
       return FX_OK;
     };
     if (int e = dot_rz(q_rz)) return e;
-    if (!fused)
-      if (int e = direction()) return e;
+    if (int e = direction()) return e;
     FX_LAUNCHED("pcg start");
     while (it < prm->max_iters) {
-      if (fused) {
-        // p2 = z*zscale + beta*p, Ap = -L p2 (z is still the current preconditioned residual)
-        FX_DISPATCH_T(storage, {
-          if (L0.op == FX_OP_FLAGS)
-            hipLaunchKernelGGL((k_pcg_dir_apply_q<FX_OP_FLAGS, TV>), kRedGrid, kRedBlock, 0, s, D, ws, dq_new,
-                               dq_cross, dq_old, dir_first, (const TV*)*z_slot, zscale, v.p, v.p2, v.Ap, ws, 3, nq);
-          else
-            hipLaunchKernelGGL((k_pcg_dir_apply_q<FX_OP_BOX, TV>), kRedGrid, kRedBlock, 0, s, D, ws, dq_new,
-                               dq_cross, dq_old, dir_first, (const TV*)*z_slot, zscale, v.p, v.p2, v.Ap, ws, 3, nq);
-        });
-        float* t = v.p;
-        v.p = v.p2;
-        v.p2 = t;
-      }
       // the preconditioned residual becomes z_old; its buffer is reused next
       void* t = v.zold;
       v.zold = *z_slot;
@@ -510,9 +436,7 @@ int pcg_loop(const fx_mg_level& L0, int storage, void* rT, void** z_slot, const 
       float zscale_old = zscale;
       double next_scale = scale;
       if (fp16) next_scale = fmax(sqrt(rr) * rms_div * 0.3, 1e-30);
-      if (fused) {
-        // applied above
-      } else if (quad) {
+      if (quad) {
         if (L0.op == FX_OP_FLAGS)
           hipLaunchKernelGGL((k_pcg_apply_q<FX_OP_FLAGS>), kRedGrid, kRedBlock, 0, s, D, v.p, v.Ap, ws, 3, nq);
         else
@@ -548,8 +472,7 @@ int pcg_loop(const fx_mg_level& L0, int storage, void* rT, void** z_slot, const 
       dq_new = q_next;
       dq_cross = 2;
       dq_old = q_rz;
-      if (!fused)
-        if (int e = direction()) return e;
+      if (int e = direction()) return e;
       FX_LAUNCHED("pcg direction");
       int tmp = q_rz;
       q_rz = q_next;
@@ -582,21 +505,20 @@ using namespace fx;
 extern "C" {
 
 int fx_pcg_solve(fx_mg_config* pcfg, float* x, const float* b, float* r, float* p, float* Ap, void* zold,
-                 float* p2, const fx_solve_params* prm, double* ws, fx_solve_info* info, void* stream) {
+                 const fx_solve_params* prm, double* ws, fx_solve_info* info, void* stream) {
   if (int e = check_config(pcfg)) return e;
   if (int e = check_params(prm)) return e;
   FX_REQUIRE(x && b && r && p && Ap && zold && ws && info, FX_EINVAL, "pcg: null array");
   FX_REQUIRE(!op_has_edges(pcfg->level[0].op), FX_EINVAL, "level 0 must be a box or flags operator");
   hipStream_t s = S(stream);
   fx_mg_level& L0 = pcfg->level[0];
-  FX_REQUIRE(p2 != x && p2 != r && p2 != p && p2 != Ap && p2 != b, FX_EINVAL, "pcg: p2 aliases another vector");
-  PcgVectors v{x, r, p, Ap, zold, p2};
+  PcgVectors v{x, r, p, Ap, zold};
   auto precond = [&](hipStream_t st) { return vcycle(pcfg, 0, 1.0f, true, st); };
   return pcg_loop(L0, pcfg->storage, L0.rhs, &L0.p, b, v, prm, ws, info, precond, s);
 }
 
 int fx_learned_pcg_solve(const fx_unet* net, fx_mg_config* mg, float* x, const float* b, float* r, float* p,
-                         float* Ap, float* zold, float* z, float* rT, float* ztmp, void* mg_zold, float* p2,
+                         float* Ap, float* zold, float* z, float* rT, float* ztmp, void* mg_zold,
                          const uint8_t* solid, float post_scale, int32_t learned_iters, int32_t sweeps,
                          const fx_solve_params* prm, double* ws, fx_solve_info* info, void* stream) {
   if (int e = check_config(mg)) return e;
@@ -618,9 +540,7 @@ int fx_learned_pcg_solve(const fx_unet* net, fx_mg_config* mg, float* x, const f
   fx_solve_params lp = *prm;
   lp.max_iters = std::min(prm->max_iters, std::max(0, learned_iters));
   void* zs = z;
-  FX_REQUIRE(p2 != x && p2 != r && p2 != p && p2 != Ap && p2 != b && p2 != z && p2 != rT && p2 != ztmp && p2 != zold,
-             FX_EINVAL, "learned pcg: p2 aliases another vector");
-  PcgVectors v{x, r, p, Ap, zold, p2};
+  PcgVectors v{x, r, p, Ap, zold};
   // z = U-Net(rT) (on level 0, or prolonged from level 1), then `sweeps`
   // damped-Jacobi sweeps on L z = rT from it (the network carries the smooth
   // part of the solution, the sweeps the rough part)
@@ -658,7 +578,7 @@ int fx_learned_pcg_solve(const fx_unet* net, fx_mg_config* mg, float* x, const f
   fp.max_iters = std::max(0, prm->max_iters - first.iterations);
   fp.check_from = 0;
   fx_solve_info second{};
-  PcgVectors v2{x, r, p, Ap, mg_zold, p2};
+  PcgVectors v2{x, r, p, Ap, mg_zold};
   auto mg_precond = [&](hipStream_t st) { return vcycle(mg, 0, 1.0f, true, st); };
   if (int e = pcg_loop(L0, mg->storage, L0.rhs, &L0.p, b, v2, &fp, ws, &second, mg_precond, s)) return e;
   second.iterations += first.iterations;

@@ -4,7 +4,9 @@
 
 Harvests scenes x steps-per-scene (divergence, solid -> impulse) pairs from
 plain FP32 tolerance-multigrid solves of seeded random scenes (seeds from
-10000), trains with Adam (lr, betas 0.9/0.999, eps 1e-8, one pair per step,
+10000) -- or, with --increments, the warm-started solves' increment problems
+of (2n)^3 scenes restricted to n^3, for the network run one level down --
+trains with Adam (lr, betas 0.9/0.999, eps 1e-8, one pair per step,
 cosine decay to lr/20), reports the masked MSE on held-out scenes (seeds from
 20000) and writes the fp32 parameters (.npz) plus a JSON summary beside them.
 """
@@ -25,6 +27,9 @@ def main(argv=None):
     ap.add_argument("--lr", type=float, default=1e-3)
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--out", required=True)
+    ap.add_argument("--increments", action="store_true",
+                    help="pairs from warm-started increment solves at (2n)^3, restricted to n^3 (for "
+                         "learned_level = 1 on a (2n)^3 grid)")
     args = ap.parse_args(argv)
     from .backend.device import HipDevice
     from .learned import UNetPreconditioner
@@ -36,8 +41,13 @@ def main(argv=None):
     device = HipDevice(0)
     shape = (args.n,) * 3
     t0 = time.time()
-    train_set = training.harvest(device, args.n, args.scenes, args.steps_per_scene, seed0=10000, log=log)
-    held_out = training.harvest(device, args.n, args.held_out, 2, seed0=20000, log=log)
+    if args.increments:
+        train_set = training.harvest_increments(device, 2 * args.n, args.scenes, args.steps_per_scene, seed0=10000,
+                                                log=log)
+        held_out = training.harvest_increments(device, 2 * args.n, args.held_out, 2, seed0=20000, log=log)
+    else:
+        train_set = training.harvest(device, args.n, args.scenes, args.steps_per_scene, seed0=10000, log=log)
+        held_out = training.harvest(device, args.n, args.held_out, 2, seed0=20000, log=log)
     t_harvest = time.time() - t0
     net = UNetPreconditioner(device, seed=args.seed)
     before = training.evaluate(net, held_out, shape)
@@ -59,7 +69,8 @@ def main(argv=None):
     after = training.evaluate(net, held_out, shape)
     log(f"held-out loss after training {after:.4e}")
     net.save(args.out)
-    summary = {"n": args.n, "pairs": len(train_set), "held_out_pairs": len(held_out), "steps": steps,
+    summary = {"n": args.n, "pairs": len(train_set), "data": "increments" if args.increments else "solutions",
+               "held_out_pairs": len(held_out), "steps": steps,
                "lr": args.lr, "betas": [0.9, 0.999], "eps": 1e-8, "held_out_loss_before": before,
                "held_out_loss_after": after, "harvest_s": t_harvest, "train_s": t_train,
                "ms_per_step": 1e3 * t_train / max(steps, 1), "history": hist}

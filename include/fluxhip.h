@@ -197,11 +197,6 @@ typedef struct fx_mg_level {
                             to the level, are all active */
 } fx_mg_level;
 
-/* fx_mg_config.options: run every sweep of the small levels (<= 4096 cells) as
- * its own launch instead of the fused single-block pre/post kernels (same
- * results bit for bit; for tests) */
-#define FX_MG_NO_TAIL 1
-
 typedef struct fx_mg_config {
   int32_t levels;
   int32_t storage;        /* FX_F32 / FX_BF16 / FX_F16 */
@@ -212,7 +207,7 @@ typedef struct fx_mg_config {
   int32_t coarse_sweeps;  /* > 0: smooth the coarsest level this many times; 0: direct */
   const double* coarse_matrix; /* device, m x m FP64 pseudo-inverse, m = cells of coarsest */
   float coarse_scale;     /* direct: p = coarse_scale * M rhs */
-  int32_t options;        /* FX_MG_* bits */
+  int32_t pad_;
   fx_mg_level level[16];
 } fx_mg_config;
 
@@ -299,11 +294,10 @@ int fx_mixed_solve(fx_mg_config* ecfg, float* x, float* x_spare, const float* b,
 
 /* Flexible preconditioned CG (Polak-Ribiere beta, restarted from the true
  * residual if the recurrence drifts) on L x = b with one V-cycle of pcfg as
- * preconditioner.  r, p, Ap: fp32 scratch; zold: scratch in pcfg's storage;
- * p2 (may be NULL): a second fp32 direction buffer, with which the direction
- * update is fused into the operator apply.  ws: FX_WS_DOUBLES doubles. */
+ * preconditioner.  r, p, Ap: fp32 scratch; zold: scratch in pcfg's storage.
+ * ws: FX_WS_DOUBLES doubles. */
 int fx_pcg_solve(fx_mg_config* pcfg, float* x, const float* b, float* r, float* p, float* Ap, void* zold,
-                 float* p2, const fx_solve_params* prm, double* ws, fx_solve_info* info, void* stream);
+                 const fx_solve_params* prm, double* ws, fx_solve_info* info, void* stream);
 
 /* --------------------------------------------------------------------------
  * transport (backend/detail.py:16-37, backend/projected.py:150-195)
@@ -512,13 +506,13 @@ int fx_unet_forward(const fx_unet* net, const float* r, const uint8_t* solid, do
  * (info->fallback = 1).  Vectors: x (in/out), b, r, p, Ap, zold, z, rT fp32
  * of the level-0 size; mg->level[0] p/q/rhs are the V-cycle's work arrays and
  * mg_zold a fourth level-0 array, all in mg->storage (the fallback swaps
- * level[0].p with mg_zold as fx_pcg_solve does); p2 as in fx_pcg_solve.
+ * level[0].p with mg_zold as fx_pcg_solve does).
  * The network grid is level 0's, or level 1's: then the network sees the
  * 8-child mean of the residual, its output is prolonged like a V-cycle
  * correction before the sweeps, `solid` is level 1's mask (a coarse cell is
  * solid when all eight children are) and post_scale is level 1's h^2. */
 int fx_learned_pcg_solve(const fx_unet* net, fx_mg_config* mg, float* x, const float* b, float* r, float* p,
-                         float* Ap, float* zold, float* z, float* rT, float* ztmp, void* mg_zold, float* p2,
+                         float* Ap, float* zold, float* z, float* rT, float* ztmp, void* mg_zold,
                          const uint8_t* solid, float post_scale, int32_t learned_iters, int32_t sweeps,
                          const fx_solve_params* prm, double* ws, fx_solve_info* info, void* stream);
 

@@ -27,7 +27,7 @@ def _archive(path, meta_update=None, arrays_update=None, n=4):
     ({"format": "fluxfx-checkpoint-0"}, None, "Unsupported checkpoint format"),
     ({"time": -1.0}, None, "Invalid checkpoint clock"),
     ({"steps": 1.5}, None, "Invalid checkpoint clock"),
-    ({"last_dt": 0.5}, None, ""),
+    ({"last_dt": 0.5}, None, "timestep must be finite"),
     (None, {"density": np.zeros(3, np.float32)}, "Invalid field: density"),
     (None, {"temperature": np.full(64, np.nan, np.float32)}, "Invalid field: temperature"),
     (None, {"density": np.full(64, -1.0, np.float32)}, "Negative field: density"),

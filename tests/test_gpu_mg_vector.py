@@ -164,3 +164,4 @@ def test_vector_kernels_match_one_cell_kernels(device, storage, level0):
     act1 = torch.from_numpy(act.ravel()).to(device.device)
     res = _run_pair(device, dtype, st, op0, data0, words1, act1)
     assert any(np.count_nonzero(v) for v in res.values())
+
