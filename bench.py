@@ -13,8 +13,10 @@ trilinear prolongation, FP64 direct solve on 8^3, warm-started impulse, run to
 
 Modes: plain (FP32 stationary V-cycles, the drop-in), mixed (BF16 V-cycles
 preconditioning FP32 flexible CG) and learned (one FPCG step preconditioned by
-the 128^3-trained U-Net applied to the 8-child mean of the residual on the
-128^3 level, its output prolonged and smoothed, then BF16 V-cycle FPCG),
+a U-Net applied to the 8-child mean of the residual on the 128^3 level, its
+output prolonged and smoothed, then BF16 V-cycle FPCG; the network was trained
+on the warm-started solves' increment problems of seeded random 256^3 scenes,
+restricted to 128^3),
 whichever are built.  With the CPU leg the line also carries every field's
 relative error against the FP64 oracle on the same scene at 64^3 after 7 steps
 (``fp64_parity_64``).  ``value`` is the steps/s of the learned+mixed mode (of
@@ -40,7 +42,7 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 METRIC = "256³ smoke steps/s at 1e-5 rel. residual: learned+mixed vs plain FP32 MG"
-WEIGHTS = ROOT / "fluxfx_amd" / "weights" / "unet_128.npz"
+WEIGHTS = ROOT / "fluxfx_amd" / "weights" / "unet_128_inc.npz"
 HBM_PEAK_GBS = 8000.0
 LEARNED_LEVEL = 1
 LEARNED_ITERATIONS = 1
@@ -118,9 +120,9 @@ def settings_for(n, mode, workload="smoke"):
         kw.update(pressure_precision="BF16", pressure_krylov="PCG")
     elif mode == "learned":
         if not WEIGHTS.exists():
-            raise NotImplementedError(f"no trained U-Net weights at {WEIGHTS.relative_to(ROOT)}")
-        # the 128^3-trained network runs one level down (on the 8-child mean of the
-        # 256^3 residual), its output prolonged and smoothed like a V-cycle correction
+            raise NotImplementedError(f"no trained U-Net weights at {WEIGHTS}")
+        # the network runs one level down (on the 8-child mean of the 256^3
+        # residual), its output prolonged and smoothed like a V-cycle correction
         kw.update(pressure_precision="BF16", pressure_krylov="LEARNED", learned_weights=str(WEIGHTS),
                   learned_iterations=LEARNED_ITERATIONS, learned_sweeps=2, learned_level=LEARNED_LEVEL)
     return PressureSettings(**kw), dt
@@ -282,7 +284,11 @@ def main():
     ap.add_argument("--workload", choices=("smoke", "fire"), default="smoke",
                     help="smoke: the 256^3 headline scene; fire: 384^3 dense fire (combustion, vorticity "
                          "confinement, static sphere collider), no CPU leg")
+    ap.add_argument("--weights", default=None, help="U-Net weights (.npz) of the learned mode")
     args = ap.parse_args()
+    if args.weights:
+        global WEIGHTS
+        WEIGHTS = Path(args.weights).resolve()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: the step runs on a single MI355X; "
               "multi-GPU runs are not part of this build", file=sys.stderr)

@@ -63,7 +63,13 @@ __device__ __forceinline__ bf16 from_f<bf16>(float v) {
   return bf16{__builtin_bit_cast(uint16_t, (__bf16)v)};
 }
 template <>
-__device__ __forceinline__ _Float16 from_f<_Float16>(float v) { return _Float16(v); }
+__device__ __forceinline__ _Float16 from_f<_Float16>(float v) {
+  // the fp32 value is rounded to fp16 on its own: the empty asm keeps the
+  // compiler from folding the conversion into the producing fma (a mixed
+  // fma rounds once, straight to fp16), so every kernel rounds alike
+  asm("" : "+v"(v));
+  return _Float16(v);
+}
 
 // ---------------------------------------------------------------------------
 // indexing helpers (flat x-fast: x + nx*(y + ny*z))

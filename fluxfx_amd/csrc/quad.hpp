@@ -64,7 +64,8 @@ __device__ __forceinline__ void st4<bf16>(bf16* __restrict__ p, int64_t i, const
 }
 template <>
 __device__ __forceinline__ void st4<_Float16>(_Float16* __restrict__ p, int64_t i, const F4& v) {
-  half4_t a = {_Float16(v.v[0]), _Float16(v.v[1]), _Float16(v.v[2]), _Float16(v.v[3])};
+  half4_t a = {from_f<_Float16>(v.v[0]), from_f<_Float16>(v.v[1]), from_f<_Float16>(v.v[2]),
+               from_f<_Float16>(v.v[3])};
   *reinterpret_cast<half4_t*>(p + i) = a;
 }
 
@@ -119,7 +120,7 @@ struct VecIO<_Float16, 8> {
   static __device__ __forceinline__ void st(_Float16* __restrict__ p, int64_t i, const FV<8>& v) {
     half8_t a;
 #pragma unroll
-    for (int k = 0; k < 8; ++k) a[k] = _Float16(v.v[k]);
+    for (int k = 0; k < 8; ++k) a[k] = from_f<_Float16>(v.v[k]);
     *reinterpret_cast<half8_t*>(p + i) = a;
   }
 };
@@ -137,7 +138,7 @@ __device__ __forceinline__ void st2<bf16>(bf16* __restrict__ p, int64_t i, float
 }
 template <>
 __device__ __forceinline__ void st2<_Float16>(_Float16* __restrict__ p, int64_t i, float a, float b) {
-  half2_t h = {_Float16(a), _Float16(b)};
+  half2_t h = {from_f<_Float16>(a), from_f<_Float16>(b)};
   *reinterpret_cast<half2_t*>(p + i) = h;
 }
 

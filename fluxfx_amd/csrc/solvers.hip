@@ -567,6 +567,11 @@ int fx_learned_pcg_solve(const fx_unet* net, fx_mg_config* mg, float* x, const f
     sm.level[0].q = ztmp;
     sm.level[0].rhs = rT;
     if (int e = smooth_level(&sm, 0, sweeps, st)) return e;
+    // the sweeps leave the result in either buffer: it becomes z, the other one scratch
+    if (sm.level[0].p != zs) {
+      ztmp = static_cast<float*>(zs);
+      zs = sm.level[0].p;
+    }
     return int(FX_OK);
   };
   fx_solve_info first{};

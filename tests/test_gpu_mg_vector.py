@@ -137,18 +137,20 @@ def _shape_of(key):
 def _floats(bits, dtype):
     import torch
     if bits.dtype == np.int16:
+        if dtype == torch.float16:
+            return bits.view(np.float16).astype(np.float64)
         return (bits.astype(np.int32) << 16).view(np.float32).astype(np.float64)
     return bits.view(np.float32).astype(np.float64)
 
 
-@pytest.mark.parametrize("storage", ["FP32", "BF16"])
+@pytest.mark.parametrize("storage", ["FP32", "BF16", "FP16"])
 @pytest.mark.parametrize("level0", ["packed", "flags"])
 def test_vector_kernels_match_one_cell_kernels(device, storage, level0):
     import torch
     from fluxfx_amd.backend.engine import activity_bytes
     print(f"vector-vs-one-cell MG kernels: storage={storage} level0={level0}", flush=True)
-    dtype = {"FP32": torch.float32, "BF16": torch.bfloat16}[storage]
-    st = {"FP32": _lib.FX_F32, "BF16": _lib.FX_BF16}[storage]
+    dtype = {"FP32": torch.float32, "BF16": torch.bfloat16, "FP16": torch.float16}[storage]
+    st = {"FP32": _lib.FX_F32, "BF16": _lib.FX_BF16, "FP16": _lib.FX_F16}[storage]
     rng = np.random.default_rng(3)
     if level0 == "packed":
         w0, _ = _packed_words(rng, (NX, NY, NZ), 4)
