@@ -581,7 +581,8 @@ int fx_learned_pcg_solve(const fx_unet* net, fx_mg_config* mg, float* x, const f
   // fallback: V-cycle preconditioned FPCG from the learned iterate
   fx_solve_params fp = *prm;
   fp.max_iters = std::max(0, prm->max_iters - first.iterations);
-  fp.check_from = 0;
+  // host residual reads start where the whole solve's schedule puts them
+  fp.check_from = std::max(0, prm->check_from - first.iterations);
   fx_solve_info second{};
   PcgVectors v2{x, r, p, Ap, mg_zold};
   auto mg_precond = [&](hipStream_t st) { return vcycle(mg, 0, 1.0f, true, st); };
