@@ -23,6 +23,9 @@ relative error against the FP64 oracle on the same scene at 64^3 after 7 steps
 the mixed mode when no trained weights are present); every mode's rate and
 its ratio to plain are in the JSON line.
 
+``--workload bricks`` times the sparse brick pool at 256^3 instead (support
+scan, gather and scatter of the reference's 112^3-box occupancy case).
+
 ``--workload fire`` runs a 384^3 dense fire instead (fuel and heat emitter,
 combustion, vorticity confinement, a static sphere collider; no CPU leg).
 
@@ -273,6 +276,78 @@ def parity_vs_oracle(device, state, n_sample, steps, modes):
     return out
 
 
+def bricks_bench(device, n, steps, warmup, cpu):
+    """Sparse brick pool at n^3 with the reference's occupancy case (a 112^3
+    box of 256^3, 8.374 %, 2744 8^3 bricks, docs/SPARSE_BRICKS.md): one cycle
+    is the support scan of the dense field, the gather of the active bricks
+    and their scatter back (topology update excluded: host work, timed apart)."""
+    import torch
+    from fluxfx_amd.native import BrickPool
+    lo, m = (72 * n) // 256 // 8 * 8, (112 * n) // 256 // 8 * 8
+    g = torch.Generator(device="cuda").manual_seed(5)
+    dense = torch.zeros((n, n, n), device="cuda")
+    dense[lo:lo + m, lo:lo + m, lo:lo + m] = torch.rand((m, m, m), device="cuda", generator=g) + 0.5
+    out = torch.zeros_like(dense)
+    pool = BrickPool(8, 4096, 2 ** 31)
+    flags = BrickPool.support(dense, 8, 0.0)
+    t0 = time.perf_counter()
+    pool.activate_flags(flags)
+    pool.sync()
+    topo_ms = 1e3 * (time.perf_counter() - t0)
+    nb = len(pool.snapshot())
+
+    def cycle():
+        BrickPool.support(dense, 8, 0.0)
+        pool.gather(dense)
+        pool.scatter(out)
+
+    def timed(fn, reps):
+        for _ in range(warmup):
+            fn()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            fn()
+        e1.record()
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / reps
+
+    cyc_ms = timed(cycle, steps)
+    sup_ms = timed(lambda: BrickPool.support(dense, 8, 0.0), 4 * steps)
+    assert torch.equal(out, dense)
+    scan_bytes = n ** 3 * 4
+    line = {
+        "metric": f"{n}³ brick pool cycles/s (support scan + gather + scatter, 8³ bricks, 8.4 % occupancy)",
+        "value": 1e3 / cyc_ms, "unit": "cycles/s", "n_gpus": 1, "steps": steps, "warmup": warmup,
+        "ms_per_step": cyc_ms, "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"{n}^3 field, {m}^3 occupied box at {lo}, {nb} active 8^3 bricks",
+                   "parallelism": "single GPU"},
+        "topology_update_ms": topo_ms,
+        "roofline": {"bound": "hbm", "achieved": scan_bytes / (sup_ms * 1e-3) / 1e9, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": scan_bytes / (sup_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "k_brick_support_rows (support scan of the dense field)", "avg_ms": sup_ms,
+                     "bytes_per_launch": scan_bytes},
+    }
+    if cpu:
+        from oracle import bricks as ob
+        host = dense.cpu().numpy()
+        coords = [r[:3] for r in pool.snapshot()]
+        t0 = time.perf_counter()
+        reps = 0
+        while time.perf_counter() - t0 < 10.0 or reps == 0:
+            f = ob.support_flags(host, 8, 0.0)
+            bricks = ob.gather(host, coords, 8)
+            reps += 1
+        cpu_ms = 1e3 * (time.perf_counter() - t0) / reps
+        assert int(f.sum()) == nb and bricks.shape[0] == nb
+        line["cpu_baseline"] = {"value": 1e3 / cpu_ms, "unit": "cycles/s", "cores": 1, "kind": "port",
+                                "sample": f"oracle support scan + gather of the same {n}^3 field, {reps} cycles "
+                                          "(NumPy, one core; the scatter is not timed on the CPU)"}
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -281,9 +356,9 @@ def main():
     ap.add_argument("--n", type=int, default=None, help="grid side (default 256 smoke, 384 fire)")
     ap.add_argument("--modes", default="plain,mixed,learned")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--workload", choices=("smoke", "fire"), default="smoke",
+    ap.add_argument("--workload", choices=("smoke", "fire", "bricks"), default="smoke",
                     help="smoke: the 256^3 headline scene; fire: 384^3 dense fire (combustion, vorticity "
-                         "confinement, static sphere collider), no CPU leg")
+                         "confinement, static sphere collider), no CPU leg; bricks: the 256^3 sparse brick pool")
     ap.add_argument("--weights", default=None, help="U-Net weights (.npz) of the learned mode")
     args = ap.parse_args()
     if args.weights:
@@ -297,6 +372,9 @@ def main():
     device = HipDevice(0)
     fire = args.workload == "fire"
     n = args.n or (384 if fire else 256)
+    if args.workload == "bricks":
+        bricks_bench(device, n, args.steps, args.warmup, not args.no_cpu)
+        return
     results = {}
     roof = None
     for mode in [m for m in args.modes.split(",") if m]:

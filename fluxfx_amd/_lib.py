@@ -88,6 +88,14 @@ class fx_scalar_args(C.Structure):
 
 P = C.c_void_p
 I32, I64, F32 = C.c_int32, C.c_int64, C.c_float
+U32, U64, F64 = C.c_uint32, C.c_uint64, C.c_double
+
+
+class fx_bricks_dev(C.Structure):
+    _fields_ = [("field", C.c_void_p), ("coords", C.c_void_p), ("active", C.c_void_p), ("neighbors", C.c_void_p),
+                ("side", C.c_int32), ("capacity", C.c_uint32), ("active_count", C.c_uint32),
+                ("reallocations", C.c_uint32)]
+
 
 SIGNATURES = {
     "fx_last_error": ([], C.c_char_p),
@@ -131,6 +139,26 @@ SIGNATURES = {
     "fx_remap_scalar": ([fx_grid, P, P, P, P, P], C.c_int),
     "fx_remap_faces": ([fx_grid, P, P, P, P, P, P, P, P, P, P, P], C.c_int),
     "fx_confine": ([fx_grid, P, P, P, P, P, P, P, F32, F32, F32, P], C.c_int),
+    "fx_bricks_create": ([I32, U32, U64, C.POINTER(P)], C.c_int),
+    "fx_bricks_destroy": ([P], C.c_int),
+    "fx_bricks_close": ([P], C.c_int),
+    "fx_bricks_activate": ([P, I32, I32, I32, C.POINTER(U32)], C.c_int),
+    "fx_bricks_deactivate": ([P, I32, I32, I32, C.POINTER(I32)], C.c_int),
+    "fx_bricks_lookup": ([P, I32, I32, I32, C.POINTER(I32)], C.c_int),
+    "fx_bricks_neighbors": ([P, I32, I32, I32, C.POINTER(I32)], C.c_int),
+    "fx_bricks_activate_box": ([P, I32, I32, I32, I32, I32, I32], C.c_int),
+    "fx_bricks_activate_flags": ([P, P, I32, I32, I32], C.c_int),
+    "fx_bricks_update_regions": ([P, P, I32, I32, F64, I32, I32], C.c_int),
+    "fx_bricks_count": ([P, C.POINTER(U32), C.POINTER(U32)], C.c_int),
+    "fx_bricks_snapshot": ([P, P, U32, C.POINTER(U32)], C.c_int),
+    "fx_bricks_host_bytes": ([P, C.POINTER(U64)], C.c_int),
+    "fx_bricks_reserve": ([P, U32, U32, C.POINTER(U32)], C.c_int),
+    "fx_bricks_sync": ([P, P], C.c_int),
+    "fx_bricks_device": ([P, C.POINTER(fx_bricks_dev)], C.c_int),
+    "fx_bricks_gather": ([P, I32, I32, I32, P, P], C.c_int),
+    "fx_bricks_scatter": ([P, I32, I32, I32, P, P], C.c_int),
+    "fx_brick_support": ([I32, I32, I32, I32, P, F32, P, P], C.c_int),
+    "fx_bricks_probe": ([P, C.POINTER(U32), C.POINTER(F32), P], C.c_int),
     "fx_mg_vcycle": ([C.POINTER(fx_mg_config), F32, P], C.c_int),
     "fx_mg_smooth": ([C.POINTER(fx_mg_config), I32, I32, F32, P], C.c_int),
     "fx_mg_residual": ([C.POINTER(fx_mg_config), I32, F32, P, P], C.c_int),

@@ -516,6 +516,82 @@ int fx_learned_pcg_solve(const fx_unet* net, fx_mg_config* mg, float* x, const f
                          const uint8_t* solid, float post_scale, int32_t learned_iters, int32_t sweeps,
                          const fx_solve_params* prm, double* ws, fx_solve_info* info, void* stream);
 
+/* ------------------------------------------------------------------------
+ * Sparse brick pool (FluxFX_Source/native/bricks.hpp, regions.hpp,
+ * brick_pool.hpp; Python surface native/__init__.py:31-55 BrickPool).
+ * Bricks are side^3 voxels (side 8 or 16) addressed by signed brick
+ * coordinates in [-1048576, 1048575]; a slot is the brick's row in the pool
+ * (stable until deactivation, then reusable).  Topology and region calls are
+ * host-only; the device mirror is refreshed by fx_bricks_sync.  Errors:
+ * FX_EINVAL for invalid arguments (the reference's std::invalid_argument),
+ * FX_ESTATE for capacity exhaustion and closed pools (std::runtime_error).
+ * ------------------------------------------------------------------------ */
+typedef struct fx_bricks fx_bricks;
+
+/* Device arrays of a synced pool: field[slot * side^3 + (lz*side + ly)*side + lx],
+ * coords[4*slot + {0,1,2}] = brick x, y, z, active[0..active_count),
+ * neighbors[6*slot + d] for d = -X, +X, -Y, +Y, -Z, +Z (-1: none). */
+typedef struct fx_bricks_dev {
+  float* field;
+  int32_t* coords;
+  uint32_t* active;
+  int32_t* neighbors;
+  int32_t side;
+  uint32_t capacity;
+  uint32_t active_count;
+  uint32_t reallocations; /* device field reallocations by capacity growth */
+} fx_bricks_dev;
+
+/* BrickPool(side, capacity, budget_bytes) (brick_pool.hpp:16-31): capacity
+ * 1..262144, side 8 or 16, capacity*(4*side^3 + 28) (256-byte rounded) <= budget. */
+int fx_bricks_create(int32_t side, uint32_t capacity, uint64_t budget_bytes, fx_bricks** out);
+int fx_bricks_destroy(fx_bricks* b);
+/* BrickPool.close: device memory released; later calls fail with FX_ESTATE. */
+int fx_bricks_close(fx_bricks* b);
+/* BrickTopology::activate (bricks.hpp:55-64): idempotent; slot of the brick. */
+int fx_bricks_activate(fx_bricks* b, int32_t x, int32_t y, int32_t z, uint32_t* slot);
+/* BrickTopology::deactivate (bricks.hpp:65-70): *removed = 0 when absent. */
+int fx_bricks_deactivate(fx_bricks* b, int32_t x, int32_t y, int32_t z, int32_t* removed);
+/* BrickTopology::lookup (bricks.hpp:52-54): -1 when absent. */
+int fx_bricks_lookup(const fx_bricks* b, int32_t x, int32_t y, int32_t z, int32_t* slot);
+/* BrickTopology::neighbors (bricks.hpp:71-79): six slots, -1 when absent. */
+int fx_bricks_neighbors(const fx_bricks* b, int32_t x, int32_t y, int32_t z, int32_t* out6);
+/* BrickTopology::activate_box (bricks.hpp:85-93): all-or-nothing. */
+int fx_bricks_activate_box(fx_bricks* b, int32_t x, int32_t y, int32_t z, int32_t nx, int32_t ny, int32_t nz);
+/* Field-driven activation (no reference counterpart; the support scan of
+ * docs/DENSITY_CAPACITY.md): activates brick (x, y, z) for every nonzero
+ * flags[(z*gy + y)*gx + x] (host array), in z, y, x order, all-or-nothing. */
+int fx_bricks_activate_flags(fx_bricks* b, const uint8_t* flags, int32_t gx, int32_t gy, int32_t gz);
+/* RegionTracker::update (regions.hpp:14-51): sources are `count` rows of
+ * (x, y, z, radius, vx, vy, vz) in domain units. */
+int fx_bricks_update_regions(fx_bricks* b, const double* sources, int32_t count, int32_t grid, double dt,
+                             int32_t linger, int32_t halo);
+int fx_bricks_count(const fx_bricks* b, uint32_t* active, uint32_t* capacity);
+/* BrickPool.snapshot (brick_bindings.inc:124-135): rows of (x, y, z, slot, age)
+ * in active-list order; max_rows >= the active count. */
+int fx_bricks_snapshot(const fx_bricks* b, int32_t* rows, uint32_t max_rows, uint32_t* count);
+int fx_bricks_host_bytes(const fx_bricks* b, uint64_t* bytes);
+/* Capacity doubling (docs/DENSITY_CAPACITY.md): the capacity doubles until it
+ * holds `needed` bricks, capped at `limit` (e.g. the domain's brick count);
+ * slots, coordinates and (after the next sync) field values are kept. */
+int fx_bricks_reserve(fx_bricks* b, uint32_t needed, uint32_t limit, uint32_t* capacity);
+/* Upload the topology and rebuild the device neighbour table on the GPU
+ * (brick_pool.hpp probe preparation); synchronises the stream. */
+int fx_bricks_sync(fx_bricks* b, void* stream);
+int fx_bricks_device(const fx_bricks* b, fx_bricks_dev* out);
+/* field of every active brick <- dense nx*ny*nz (x fastest) at brick*side
+ * (voxels outside the grid read 0), and the reverse. */
+int fx_bricks_gather(fx_bricks* b, int32_t nx, int32_t ny, int32_t nz, const float* dense, void* stream);
+int fx_bricks_scatter(fx_bricks* b, int32_t nx, int32_t ny, int32_t nz, float* dense, void* stream);
+/* flags[(bz*by + by_)*bx + bx_] = any |dense| > threshold in that brick
+ * (bx = ceil(nx/side) ...), device arrays. */
+int fx_brick_support(int32_t side, int32_t nx, int32_t ny, int32_t nz, const float* dense, float threshold,
+                     uint8_t* flags, void* stream);
+/* BrickPool.probe (brick_bindings.inc:64-90): resets every reserved voxel to
+ * a sentinel, writes a per-voxel pattern of (slot, voxel, neighbour ids) into
+ * active bricks and verifies all slots on the host. */
+int fx_bricks_probe(fx_bricks* b, uint32_t* mismatches, float* gpu_ms, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

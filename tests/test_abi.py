@@ -54,6 +54,7 @@ def test_struct_sizes():
     assert C.sizeof(lib.fx_mg_config) == 48 + 16 * C.sizeof(lib.fx_mg_level)
     assert C.sizeof(lib.fx_conv_desc) == 32
     assert C.sizeof(lib.fx_unet) == 16 + 8 * (10 + 10 + 1 + 10 + 3)
+    assert C.sizeof(lib.fx_bricks_dev) == 4 * 8 + 4 * 4
 
 
 def test_version_and_error_path_without_gpu():

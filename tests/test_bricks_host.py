@@ -1,0 +1,256 @@
+"""Sparse brick topology and active regions on the host (no GPU): the oracle
+and libfluxhip's brick pool both replay the reference's trace
+(tests/golden/brick_trace.jsonl.gz, written by the reference's own
+bricks.hpp/regions.hpp under a seeded command sequence) value for value and
+error for error, and restate the reference's unit tests
+(native/bricks_test.cpp, native/regions_test.cpp)."""
+import gzip
+import json
+import random
+import subprocess
+from pathlib import Path
+
+import pytest
+
+from oracle import bricks as ob
+
+ROOT = Path(__file__).resolve().parents[1]
+TRACE = ROOT / "tests" / "golden" / "brick_trace.jsonl.gz"
+
+
+def _trace():
+    with gzip.open(TRACE, "rt") as f:
+        return [json.loads(line) for line in f]
+
+
+def _pool(capacity, side=8):
+    from fluxfx_amd import _lib
+    if not _lib.LIB_PATH.exists():
+        pytest.skip("libfluxhip.so is not built")
+    from fluxfx_amd.native import BrickPool
+    return BrickPool(side, capacity, 2 ** 30)
+
+
+class _OracleDriver:
+    def __init__(self, capacity):
+        self.t = ob.Topology(capacity)
+        self.r = ob.Regions(capacity)
+
+    def do(self, op, args):
+        if op == "activate":
+            return self.t.activate(args)
+        if op == "deactivate":
+            return self.t.deactivate(args)
+        if op == "lookup":
+            return self.t.lookup(args)
+        if op == "neighbors":
+            return list(self.t.neighbors(args))
+        if op == "activate_box":
+            return self.t.activate_box(args[0], args[1])
+        if op == "update_regions":
+            return self.r.update(self.t, [tuple(s) for s in args[0]], *args[1:])
+        if op == "snapshot":
+            return [[*self.t.coords[s], s, self.r.ages[s]] for s in self.t.active]
+        raise AssertionError(op)
+
+
+class _LibDriver:
+    def __init__(self, capacity):
+        self.p = _pool(capacity)
+
+    def do(self, op, args):
+        p = self.p
+        if op in ("activate", "deactivate", "lookup"):
+            return getattr(p, op)(*args)
+        if op == "neighbors":
+            return list(p.neighbors(*args))
+        if op == "activate_box":
+            return p.activate_box(*args[0], *args[1])
+        if op == "update_regions":
+            return p.update_regions(args[0], *args[1:])
+        if op == "snapshot":
+            return [list(r) for r in p.snapshot()]
+        raise AssertionError(op)
+
+
+def _replay(driver_cls):
+    trace = _trace()
+    assert trace[0]["op"] == "create"
+    d = driver_cls(trace[0]["args"][0])
+    errors = 0
+    for k, step in enumerate(trace[1:], 1):
+        try:
+            ret = d.do(step["op"], step["args"])
+            err = None
+        except (ValueError, RuntimeError) as e:
+            ret, err = None, [type(e).__name__, str(e)]
+        assert err == step["err"], (k, step, err)
+        if err is None:
+            assert ret == step["ret"], (k, step, ret)
+        else:
+            errors += 1
+    return len(trace) - 1, errors
+
+
+def test_oracle_replays_reference_trace():
+    n, errors = _replay(_OracleDriver)
+    assert n == 2500 and errors > 100
+
+
+def test_library_replays_reference_trace():
+    n, errors = _replay(_LibDriver)
+    assert n == 2500 and errors > 100
+
+
+@pytest.mark.parametrize("impl", ["oracle", "library"])
+def test_reference_topology_unit_cases(impl):
+    """native/bricks_test.cpp:9-43: idempotent activation, neighbours, slot
+    reuse, extreme coordinates, 20000 randomized operations against a plain
+    map (all neighbours checked), and all-or-nothing boxes."""
+    if impl == "oracle":
+        t = ob.Topology(64)
+        act, deact, look, nb = t.activate, t.deactivate, t.lookup, t.neighbors
+
+        def active_count():
+            return len(t.active)
+    else:
+        p = _pool(64)
+        act = lambda c: p.activate(*c)  # noqa: E731
+        deact = lambda c: p.deactivate(*c)  # noqa: E731
+        look = lambda c: p.lookup(*c)  # noqa: E731
+        nb = lambda c: p.neighbors(*c)  # noqa: E731
+
+        def active_count():
+            return len(p.snapshot())
+    center = act((0, 0, 0))
+    assert center == act((0, 0, 0))
+    east = act((1, 0, 0))
+    assert nb((0, 0, 0))[1] == east and nb((1, 0, 0))[0] == center
+    deact((1, 0, 0))
+    assert nb((0, 0, 0))[1] == -1
+    assert act((-1, 0, 0)) == east
+    act((1048575, -1048576, 0))
+    assert nb((1048575, -1048576, 0))[1] == -1
+    for c in ((1048575, -1048576, 0), (-1, 0, 0), (0, 0, 0)):
+        deact(c)
+    ref = {}
+    rng = random.Random(78)
+    for _ in range(20000):
+        c = (rng.randrange(8) - 4, rng.randrange(8) - 4, rng.randrange(8) - 4)
+        if rng.randrange(2):
+            if len(ref) < 64 or c in ref:
+                ref[c] = act(c)
+            else:
+                with pytest.raises(RuntimeError, match="exhausted"):
+                    act(c)
+        else:
+            assert deact(c) == (ref.pop(c, None) is not None)
+    assert active_count() == len(ref)
+    for (x, y, z), slot in ref.items():
+        assert look((x, y, z)) == slot
+        adj = [(x - 1, y, z), (x + 1, y, z), (x, y - 1, z), (x, y + 1, z), (x, y, z - 1), (x, y, z + 1)]
+        assert list(nb((x, y, z))) == [ref.get(a, -1) for a in adj]
+    if impl == "oracle":
+        t8 = ob.Topology(8)
+        t8.activate_box((0, 0, 0), (2, 2, 2))
+        with pytest.raises(RuntimeError):
+            t8.activate_box((1, 0, 0), (2, 2, 2))
+        assert len(t8.active) == 8 and t8.lookup((2, 0, 0)) == -1
+    else:
+        p8 = _pool(8)
+        p8.activate_box(0, 0, 0, 2, 2, 2)
+        with pytest.raises(RuntimeError):
+            p8.activate_box(1, 0, 0, 2, 2, 2)
+        assert len(p8.snapshot()) == 8 and p8.lookup(2, 0, 0) == -1
+
+
+@pytest.mark.parametrize("impl", ["oracle", "library"])
+def test_reference_region_unit_cases(impl):
+    """native/regions_test.cpp:5-24: source activation, directional
+    prediction, halo clipping, hysteresis, atomic capacity failure, duplicate
+    sources and 4000 alternating updates."""
+    if impl == "oracle":
+        t, r = ob.Topology(64), ob.Regions(64)
+
+        def update(src, linger, halo):
+            r.update(t, src, 8, .1, linger, halo)
+
+        def active():
+            return list(t.active)
+
+        def age(slot):
+            return r.ages[slot]
+
+        look = t.lookup
+    else:
+        p = _pool(64)
+
+        def update(src, linger, halo):
+            p.update_regions(src, 8, .1, linger, halo)
+
+        def active():
+            return [row[3] for row in p.snapshot()]
+
+        def age(slot):
+            return {row[3]: row[4] for row in p.snapshot()}[slot]
+
+        look = lambda c: p.lookup(*c)  # noqa: E731
+    a = (.3, .3, .3, .01, 0., 0., 0.)
+    update([a], 3, 0)
+    assert len(active()) == 1 and look((2, 2, 2)) >= 0
+    slot = active()[0]
+    update([], 3, 0)
+    assert age(slot) == 1
+    update([], 3, 0)
+    assert age(slot) == 2 and len(active()) == 1
+    update([], 3, 0)
+    assert not active()
+    a = (.3, .3, .3, .01, 3., 0., 0.)
+    update([a], 3, 0)
+    assert look((4, 2, 2)) >= 0 and look((1, 2, 2)) < 0
+    before = active()
+    with pytest.raises(RuntimeError, match="exceeds pool capacity"):
+        update([(.5, .5, .5, 2., 0., 0., 0.)], 3, 0)
+    assert active() == before and all(age(s) == 0 for s in before)
+    update([], 1, 0)
+    assert not active()
+    update([(0., 0., 0., .01, 0., 0., 0.)], 3, 1)
+    assert len(active()) == 8
+    update([(-2., -2., -2., .01, 0., 0., 0.)], 1, 0)
+    assert not active()
+    update([a, a], 3, 0)
+    assert len(active()) == 3
+    for _ in range(2000):
+        update([a], 3, 0)
+        update([], 3, 0)
+    assert len(active()) == 3 and all(age(s) == 1 for s in active())
+
+
+def test_capacity_growth_keeps_slots_and_hands_out_new_slots_last():
+    """Doubling until the request fits (docs/DENSITY_CAPACITY.md), capped at
+    the limit; existing slots, coordinates and the free order are kept."""
+    p = _pool(4)
+    slots = [p.activate(x, 0, 0) for x in range(3)]
+    p.deactivate(1, 0, 0)
+    assert p.reserve(3) == 4  # fits: no growth
+    assert p.reserve(9) == 16
+    with pytest.raises(RuntimeError):
+        p.reserve(21, limit=20)
+    assert [p.lookup(x, 0, 0) for x in range(3)] == [slots[0], -1, slots[2]]
+    # the freed slot 1 and slot 3 come back before the grown slots 4..15
+    got = [p.activate(x, 1, 0) for x in range(14)]
+    assert got[:2] == [slots[1], 3] and got[2:] == list(range(4, 16))
+    with pytest.raises(RuntimeError, match="exhausted"):
+        p.activate(0, 2, 0)
+
+
+def test_reference_test_programs_pass_when_built():
+    """oracle/_ref holds the reference's own brick and region tests, built from
+    its sources by oracle/build_ref.sh where the reference is present."""
+    ref = ROOT / "oracle" / "_ref"
+    progs = [ref / "bricks_test", ref / "regions_test"]
+    if not all(p.exists() for p in progs):
+        pytest.skip("oracle/_ref is not built here")
+    for p in progs:
+        out = subprocess.run([str(p)], capture_output=True, text=True, timeout=120)
+        assert out.returncode == 0 and out.stdout.startswith("PASS"), out.stdout + out.stderr
