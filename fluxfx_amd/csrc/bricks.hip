@@ -424,6 +424,83 @@ __global__ __launch_bounds__(256) void k_brick_copy4(int side, const uint32_t* _
   }
 }
 
+// Transport-aware preflight (native/projection.mm:126-138 requiredDensity,
+// with sampleFaceField/sampleVelocity of :101-103 and emission of :115-121):
+// a brick of the cubic n^3 domain is flagged when any of its cells is
+// emitted into, or the RK2 back-trace of the cell centre (velocity in cells
+// per step, MAC faces) lands where one of the 8 donor cells holds density or
+// is emitted into.  Evaluated in fp32 in the reference's operation order with
+// contraction off, so the float32 oracle reproduces every flag.
+struct ReqArgs {
+  int n, side, grid;
+  const float* density;  // n^3
+  const float* u;        // (n+1) n n
+  const float* v;        // n (n+1) n
+  const float* w;        // n n (n+1)
+  float cx, cy, cz, radius, rate;
+};
+
+__device__ inline float face_at(const ReqArgs& A, int a, int x, int y, int z) {
+  const int sx = A.n + (a == 0), sy = A.n + (a == 1), sz = A.n + (a == 2);
+  if (x < 0 || y < 0 || z < 0 || x >= sx || y >= sy || z >= sz) return 0.0f;
+  const float* f = a == 0 ? A.u : (a == 1 ? A.v : A.w);
+  return f[(int64_t(z) * sy + y) * sx + x];
+}
+
+__device__ inline float sample_face_cells(const ReqArgs& A, int a, float px, float py, float pz) {
+#pragma clang fp contract(off)
+  const float qx = fminf(fmaxf(px - (a == 0 ? 0.0f : 0.5f), 0.0f), float(A.n + (a == 0) - 1));
+  const float qy = fminf(fmaxf(py - (a == 1 ? 0.0f : 0.5f), 0.0f), float(A.n + (a == 1) - 1));
+  const float qz = fminf(fmaxf(pz - (a == 2 ? 0.0f : 0.5f), 0.0f), float(A.n + (a == 2) - 1));
+  const float bx = floorf(qx), by = floorf(qy), bz = floorf(qz);
+  const float fx = qx - bx, fy = qy - by, fz = qz - bz;
+  float acc = 0.0f;
+  for (int z = 0; z < 2; ++z)
+    for (int y = 0; y < 2; ++y)
+      for (int x = 0; x < 2; ++x) {
+        const float wx = x ? fx : 1.0f - fx, wy = y ? fy : 1.0f - fy, wz = z ? fz : 1.0f - fz;
+        const float t = wx * wy;
+        const float wgt = t * wz;
+        const float val = face_at(A, a, int(bx) + x, int(by) + y, int(bz) + z);
+        const float term = wgt * val;
+        acc = acc + term;
+      }
+  return acc;
+}
+
+__device__ inline bool emitted(const ReqArgs& A, int x, int y, int z) {
+#pragma clang fp contract(off)
+  const float dx = (float(x) + 0.5f - A.cx) / A.radius, dy = (float(y) + 0.5f - A.cy) / A.radius,
+              dz = (float(z) + 0.5f - A.cz) / A.radius;
+  const float dd = dx * dx + dy * dy + dz * dz;
+  return A.rate * fmaxf(0.0f, 1.0f - dd) > 0.0f;
+}
+
+__global__ __launch_bounds__(256) void k_brick_required(ReqArgs A, uint8_t* __restrict__ flags) {
+#pragma clang fp contract(off)
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  const int n = A.n;
+  if (i >= int64_t(n) * n * n) return;
+  const int x = int(i % n), y = int((i / n) % n), z = int(i / (int64_t(n) * n));
+  bool need = emitted(A, x, y, z);
+  const float px = float(x) + 0.5f, py = float(y) + 0.5f, pz = float(z) + 0.5f;
+  const float mx = px - 0.5f * sample_face_cells(A, 0, px, py, pz);
+  const float my = py - 0.5f * sample_face_cells(A, 1, px, py, pz);
+  const float mz = pz - 0.5f * sample_face_cells(A, 2, px, py, pz);
+  const float top = float(n - 1);
+  const float qx = fminf(fmaxf(px - sample_face_cells(A, 0, mx, my, mz) - 0.5f, 0.0f), top);
+  const float qy = fminf(fmaxf(py - sample_face_cells(A, 1, mx, my, mz) - 0.5f, 0.0f), top);
+  const float qz = fminf(fmaxf(pz - sample_face_cells(A, 2, mx, my, mz) - 0.5f, 0.0f), top);
+  const int bx = int(floorf(qx)), by = int(floorf(qy)), bz = int(floorf(qz));
+  for (int dz = 0; dz < 2 && !need; ++dz)
+    for (int dy = 0; dy < 2 && !need; ++dy)
+      for (int dx = 0; dx < 2 && !need; ++dx) {
+        const int cx = min(max(bx + dx, 0), n - 1), cy = min(max(by + dy, 0), n - 1), cz = min(max(bz + dz, 0), n - 1);
+        need = A.density[(int64_t(cz) * n + cy) * n + cx] > 0.0f || emitted(A, cx, cy, cz);
+      }
+  if (need) flags[(int64_t(z / A.side) * A.grid + y / A.side) * A.grid + x / A.side] = 1;
+}
+
 // Diagnostic probe: every reserved voxel is set to the sentinel, then each
 // active brick writes, per voxel, a pattern of its slot, voxel index and the
 // six neighbour ids; the host verifies every slot.
@@ -805,6 +882,22 @@ int fx_brick_support(int32_t side, int32_t nx, int32_t ny, int32_t nz, const flo
                        dense, threshold, flags);
   }
   FX_LAUNCHED("k_brick_support");
+  return FX_OK;
+}
+
+int fx_brick_required(int32_t side, int32_t n, const float* density, const float* u, const float* v, const float* w,
+                      const float* emitter, uint8_t* flags, void* stream) {
+  FX_REQUIRE(side == 8 || side == 16, FX_EINVAL, "side: 8 or 16");
+  FX_REQUIRE(n >= 1 && n <= 1024 && n % side == 0, FX_EINVAL, "required: n must be 1..1024 and a multiple of side");
+  FX_REQUIRE(density && u && v && w && emitter && flags, FX_EINVAL, "required: null array");
+  ReqArgs A{n, side, n / side, density, u, v, w, emitter[0], emitter[1], emitter[2], emitter[3], emitter[4]};
+  FX_REQUIRE(A.radius > 0.0f && std::isfinite(A.cx) && std::isfinite(A.cy) && std::isfinite(A.cz) &&
+                 std::isfinite(A.rate),
+             FX_EINVAL, "required: emitter (cx, cy, cz, radius > 0, rate) in cells");
+  const int64_t cells = int64_t(n) * n * n;
+  hipLaunchKernelGGL(k_fill_u8, dim3(64), dim3(256), 0, S(stream), int64_t(A.grid) * A.grid * A.grid, flags);
+  hipLaunchKernelGGL(k_brick_required, dim3(unsigned((cells + 255) / 256)), dim3(256), 0, S(stream), A, flags);
+  FX_LAUNCHED("k_brick_required");
   return FX_OK;
 }
 

@@ -5,7 +5,8 @@ for exhausted capacity or a closed pool) and report dictionaries, plus the
 device-side operations of this build: ``sync`` (topology upload and the
 GPU-built neighbour table), ``gather``/``scatter`` between a dense field and
 the active bricks, ``support`` (per-brick occupancy of a dense field),
-``activate_flags`` and ``reserve`` (capacity doubling).
+``required`` (the reference's transport-aware preflight), ``activate_flags``
+and ``reserve`` (capacity doubling).
 
 Topology and region commands run on the host and need no GPU; ``sync``,
 ``probe``, ``gather``, ``scatter`` and ``support`` run on ``cuda:0`` (the
@@ -210,6 +211,25 @@ class BrickPool:
         flags = torch.empty(shape, dtype=torch.uint8, device=dense.device)
         _lib.call("fx_brick_support", int(side), nx, ny, nz, dense.data_ptr(), float(threshold), flags.data_ptr(),
                   BrickPool._stream())
+        return flags
+
+    @staticmethod
+    def required(density, u, v, w, side, emitter):
+        """Transport-aware preflight (native/projection.mm:126-138): uint8 cuda
+        tensor (g, g, g), g = n // side, of the bricks some cell of which is
+        emitted into or can receive a donor through its RK2 back-trace.
+        density (n, n, n); u/v/w the MAC face tensors in cells per step;
+        emitter (cx, cy, cz, radius, rate) in cells."""
+        import torch
+        n = BrickPool._dense(density)[0]
+        for t in (u, v, w):
+            if t.dtype != torch.float32 or not t.is_cuda or not t.is_contiguous():
+                raise ValueError("faces are contiguous float32 cuda tensors")
+        g = n // side if side and n % side == 0 else 0
+        flags = torch.empty((max(g, 1),) * 3, dtype=torch.uint8, device=density.device)
+        em = (C.c_float * 5)(*[float(e) for e in emitter])
+        _lib.call("fx_brick_required", int(side), n, density.data_ptr(), u.data_ptr(), v.data_ptr(), w.data_ptr(),
+                  em, flags.data_ptr(), BrickPool._stream())
         return flags
 
     # -- helpers --------------------------------------------------------------

@@ -587,6 +587,15 @@ int fx_bricks_scatter(fx_bricks* b, int32_t nx, int32_t ny, int32_t nz, float* d
  * (bx = ceil(nx/side) ...), device arrays. */
 int fx_brick_support(int32_t side, int32_t nx, int32_t ny, int32_t nz, const float* dense, float threshold,
                      uint8_t* flags, void* stream);
+/* Transport-aware preflight (native/projection.mm:126-138 requiredDensity):
+ * flags[(bz*g + by)*g + bx] (g = n/side, device array, cleared here) is 1
+ * when some cell of the brick is inside the emitter or its RK2 back-trace
+ * (velocity u/v/w on the n^3 MAC faces, in cells per step) has a donor cell
+ * with density > 0 or inside the emitter.  density: n^3 cells; emitter:
+ * host floats (cx, cy, cz, radius, rate) in cells (the reference: 0.5n,
+ * 0.5n, 0.45n, n/12, 0.1). */
+int fx_brick_required(int32_t side, int32_t n, const float* density, const float* u, const float* v, const float* w,
+                      const float* emitter, uint8_t* flags, void* stream);
 /* BrickPool.probe (brick_bindings.inc:64-90): resets every reserved voxel to
  * a sentinel, writes a per-voxel pattern of (slot, voxel, neighbour ids) into
  * active bricks and verifies all slots on the host. */

@@ -8,10 +8,12 @@ active order and every error depend only on the free stack and the active
 list, not on the hash), and NumPy-free loops.  Pinned by the reference's own
 tests (native/bricks_test.cpp, native/regions_test.cpp) restated in
 tests/test_bricks_host.py, and by the trace that oracle/_ref's build of the
-reference headers writes (tests/golden/brick_trace.json).
+reference headers writes (tests/golden/brick_trace.jsonl.gz).
 
 Also the dense brick helpers the GPU kernels are checked against: the
-per-brick support flags of a dense field and the gather/scatter of bricks.
+per-brick support flags of a dense field, the gather of bricks, and the
+transport-aware preflight flags (native/projection.mm:126-138), the last
+pinned by closed forms (no flow, uniform displacement, the emitter ball).
 """
 import math
 
@@ -184,3 +186,68 @@ def gather(dense, coords, side):
             continue
         out[k, zs.start - z0:zs.stop - z0, ys.start - y0:ys.stop - y0, xs.start - x0:xs.stop - x0] = dense[zs, ys, xs]
     return out
+
+
+def required_flags(density, u, v, w, side, emitter):
+    """Transport-aware brick preflight (native/projection.mm:126-138
+    requiredDensity, sampleFaceField/sampleVelocity :101-103, emission
+    :115-121) on a cubic n^3 grid in float32, in the kernel's operation order:
+    a cell is needed when it is emitted into, or a donor among the 8 cells
+    around its RK2 back-trace (clamped) holds density > 0 or is emitted into.
+    density (n, n, n); u (n, n, n+1), v (n, n+1, n), w (n+1, n, n) indexed
+    [z, y, x]; emitter (cx, cy, cz, radius, rate) in cells.  Returns (g, g, g)
+    uint8 flags, g = n // side."""
+    f32 = np.float32
+    n = density.shape[0]
+    cx, cy, cz, radius, rate = (f32(e) for e in emitter)
+    faces = (np.asarray(u, f32), np.asarray(v, f32), np.asarray(w, f32))
+
+    def emitted(x, y, z):
+        dx = (x.astype(f32) + f32(.5) - cx) / radius
+        dy = (y.astype(f32) + f32(.5) - cy) / radius
+        dz = (z.astype(f32) + f32(.5) - cz) / radius
+        dd = dx * dx + dy * dy + dz * dz
+        return rate * np.maximum(f32(0), f32(1) - dd) > f32(0)
+
+    def face_at(a, x, y, z):
+        f = faces[a]
+        sz, sy, sx = f.shape
+        ok = (x >= 0) & (y >= 0) & (z >= 0) & (x < sx) & (y < sy) & (z < sz)
+        return np.where(ok, f[np.clip(z, 0, sz - 1), np.clip(y, 0, sy - 1), np.clip(x, 0, sx - 1)], f32(0))
+
+    def sample(a, px, py, pz):
+        size = [n + (a == 0), n + (a == 1), n + (a == 2)]
+        q = [np.minimum(np.maximum(p - (f32(0) if a == k else f32(.5)), f32(0)), f32(size[k] - 1))
+             for k, p in enumerate((px, py, pz))]
+        b = [np.floor(t) for t in q]
+        fr = [t - bt for t, bt in zip(q, b)]
+        bi = [bt.astype(np.int64) for bt in b]
+        acc = np.zeros_like(px)
+        for zz in (0, 1):
+            for yy in (0, 1):
+                for xx in (0, 1):
+                    wx = fr[0] if xx else f32(1) - fr[0]
+                    wy = fr[1] if yy else f32(1) - fr[1]
+                    wz = fr[2] if zz else f32(1) - fr[2]
+                    wgt = (wx * wy) * wz
+                    acc = acc + wgt * face_at(a, bi[0] + xx, bi[1] + yy, bi[2] + zz)
+        return acc
+
+    z, y, x = np.meshgrid(np.arange(n), np.arange(n), np.arange(n), indexing="ij")
+    need = emitted(x, y, z)
+    px, py, pz = (t.astype(f32) + f32(.5) for t in (x, y, z))
+    mx = px - f32(.5) * sample(0, px, py, pz)
+    my = py - f32(.5) * sample(1, px, py, pz)
+    mz = pz - f32(.5) * sample(2, px, py, pz)
+    top = f32(n - 1)
+    q = [np.minimum(np.maximum(p - sample(a, mx, my, mz) - f32(.5), f32(0)), top)
+         for a, p in enumerate((px, py, pz))]
+    b = [np.floor(t).astype(np.int64) for t in q]
+    dens = np.asarray(density)
+    for dz in (0, 1):
+        for dy in (0, 1):
+            for dx in (0, 1):
+                c = [np.clip(b[0] + dx, 0, n - 1), np.clip(b[1] + dy, 0, n - 1), np.clip(b[2] + dz, 0, n - 1)]
+                need |= (dens[c[2], c[1], c[0]] > 0) | emitted(c[0], c[1], c[2])
+    g = n // side
+    return need.reshape(g, side, g, side, g, side).any(axis=(1, 3, 5)).astype(np.uint8)

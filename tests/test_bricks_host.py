@@ -10,6 +10,7 @@ import random
 import subprocess
 from pathlib import Path
 
+import numpy as np
 import pytest
 
 from oracle import bricks as ob
@@ -254,3 +255,43 @@ def test_reference_test_programs_pass_when_built():
     for p in progs:
         out = subprocess.run([str(p)], capture_output=True, text=True, timeout=120)
         assert out.returncode == 0 and out.stdout.startswith("PASS"), out.stdout + out.stderr
+
+
+def _faces(n, value=(0.0, 0.0, 0.0)):
+    u = np.full((n, n, n + 1), value[0], np.float32)
+    v = np.full((n, n + 1, n), value[1], np.float32)
+    w = np.full((n + 1, n, n), value[2], np.float32)
+    return u, v, w
+
+
+def test_required_flags_closed_forms():
+    """The preflight's conservative footprint: with no flow a cell needs its
+    2x2x2 upper block (clamped); a uniform 1.25-cell displacement per step
+    moves the footprint to offsets -2..-1; an emitter flags its own bricks."""
+    n, side = 32, 8
+    far = (1e6, 1e6, 1e6, 1.0, 0.1)  # emitter far outside: never emitted
+    dens = np.zeros((n, n, n), np.float32)
+    p = (13, 20, 7)  # x, y, z of the single donor
+    dens[p[2], p[1], p[0]] = 1.0
+    still = ob.required_flags(dens, *_faces(n), side, far)
+    need = np.zeros((n, n, n), bool)
+    for z in range(n):
+        for y in range(n):
+            for x in range(n):
+                need[z, y, x] = all(c in (pc, min(pc + 1, n - 1)) or (c + 1 == pc) for c, pc in
+                                    zip((x, y, z), p)) and all(pc - c in (0, 1) for c, pc in zip((x, y, z), p))
+    expect = need.reshape(4, 8, 4, 8, 4, 8).any(axis=(1, 3, 5)).astype(np.uint8)
+    np.testing.assert_array_equal(still, expect)
+    moved = ob.required_flags(dens, *_faces(n, (1.25, 1.25, 1.25)), side, far)
+    need = np.zeros((n, n, n), bool)
+    for z in range(n):
+        for y in range(n):
+            for x in range(n):
+                need[z, y, x] = all(pc - (c - 2) in (0, 1) for c, pc in zip((x, y, z), p))
+    expect = need.reshape(4, 8, 4, 8, 4, 8).any(axis=(1, 3, 5)).astype(np.uint8)
+    np.testing.assert_array_equal(moved, expect)
+    em = ob.required_flags(np.zeros_like(dens), *_faces(n), side, (.5 * n, .5 * n, .45 * n, n / 12, .1))
+    # the emitter ball (radius 2.67 cells around (16, 16, 14.4)) reaches cells
+    # 13..18 in x and y and 12..16 in z: the 8 central bricks
+    flagged = {tuple(int(v) for v in t) for t in np.argwhere(em)}
+    assert flagged == {(z, y, x) for z in (1, 2) for y in (1, 2) for x in (1, 2)}

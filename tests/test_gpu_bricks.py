@@ -128,3 +128,37 @@ def test_round_trip_256_reference_box(device):
     out = torch.zeros_like(dense)
     pool.scatter(out)
     assert torch.equal(out, dense)
+
+
+def test_required_flags_match_oracle(device):
+    """The transport-aware preflight on a 64^3 grid with a smooth random MAC
+    flow of up to ~2.5 cells per step, a density blob and the reference's
+    emitter (0.5n, 0.5n, 0.45n, radius n/12, rate 0.1): every brick flag
+    equals the float32 oracle's."""
+    import ctypes as C
+    import torch
+    from fluxfx_amd import _lib
+    print("brick required flags 64^3", flush=True)
+    n, side = 64, 8
+    rng = np.random.default_rng(12)
+
+    def smooth(shape):
+        f = rng.standard_normal(shape)
+        for ax in range(3):
+            for _ in range(3):
+                f = (f + np.roll(f, 1, ax) + np.roll(f, -1, ax)) / 3
+        return (2.5 * f / np.abs(f).max()).astype(np.float32)
+
+    u, v, w = smooth((n, n, n + 1)), smooth((n, n + 1, n)), smooth((n + 1, n, n))
+    dens = _blob_field(rng, (n, n, n))
+    dens[np.abs(dens) < 0.5] = 0.0
+    emitter = (.5 * n, .5 * n, .45 * n, n / 12, .1)
+    expect = ob.required_flags(dens, u, v, w, side, emitter)
+    g = n // side
+    flags = torch.zeros((g, g, g), dtype=torch.uint8, device="cuda")
+    t = [torch.from_numpy(np.ascontiguousarray(a)).cuda() for a in (dens, u, v, w)]
+    em = (C.c_float * 5)(*emitter)
+    _lib.call("fx_brick_required", side, n, *(x.data_ptr() for x in t), em, flags.data_ptr(), None)
+    got = flags.cpu().numpy()
+    np.testing.assert_array_equal(got, expect)
+    assert 0 < expect.sum() < expect.size
