@@ -316,6 +316,17 @@ def bricks_bench(device, n, steps, warmup, cpu):
     cyc_ms = timed(cycle, steps)
     sup_ms = timed(lambda: BrickPool.support(dense, 8, 0.0), 4 * steps)
     assert torch.equal(out, dense)
+    coords = [r[:3] for r in pool.snapshot()]
+    # sparse RK2 transport of the brick field at the reference's displacement
+    # (0.6, 0.2, 0.1) cells per step (native/__init__.py compare_transport)
+    uvw = [torch.full(sh, d, device="cuda") for sh, d in (((n, n, n + 1), .6), ((n, n + 1, n), .2),
+                                                           ((n + 1, n, n), .1))]
+    need = BrickPool.required(dense, *uvw, 8, (1e6, 1e6, 1e6, 1.0, 0.0))
+    pool.activate_flags(need.cpu().numpy())
+    pool.sync()
+    nb_t = len(pool.snapshot())
+    pool.gather(dense)
+    tr_ms = timed(lambda: pool.transport(n, *uvw), steps)
     scan_bytes = n ** 3 * 4
     line = {
         "metric": f"{n}³ brick pool cycles/s (support scan + gather + scatter, 8³ bricks, 8.4 % occupancy)",
@@ -325,6 +336,8 @@ def bricks_bench(device, n, steps, warmup, cpu):
         "config": {"workload": f"{n}^3 field, {m}^3 occupied box at {lo}, {nb} active 8^3 bricks",
                    "parallelism": "single GPU"},
         "topology_update_ms": topo_ms,
+        "transport": {"ms": tr_ms, "bricks": nb_t, "displacement_cells": [.6, .2, .1],
+                      "voxel_updates_per_s": nb_t * 512 / (tr_ms * 1e-3)},
         "roofline": {"bound": "hbm", "achieved": scan_bytes / (sup_ms * 1e-3) / 1e9, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": scan_bytes / (sup_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, "traffic": None,
                      "kernel": "k_brick_support_rows (support scan of the dense field)", "avg_ms": sup_ms,
@@ -333,7 +346,6 @@ def bricks_bench(device, n, steps, warmup, cpu):
     if cpu:
         from oracle import bricks as ob
         host = dense.cpu().numpy()
-        coords = [r[:3] for r in pool.snapshot()]
         t0 = time.perf_counter()
         reps = 0
         while time.perf_counter() - t0 < 10.0 or reps == 0:

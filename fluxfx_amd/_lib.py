@@ -160,6 +160,7 @@ SIGNATURES = {
     "fx_brick_support": ([I32, I32, I32, I32, P, F32, P, P], C.c_int),
     "fx_bricks_probe": ([P, C.POINTER(U32), C.POINTER(F32), P], C.c_int),
     "fx_brick_required": ([I32, I32, P, P, P, P, P, P, P], C.c_int),
+    "fx_bricks_transport": ([P, I32, P, P, P, P], C.c_int),
     "fx_mg_vcycle": ([C.POINTER(fx_mg_config), F32, P], C.c_int),
     "fx_mg_smooth": ([C.POINTER(fx_mg_config), I32, I32, F32, P], C.c_int),
     "fx_mg_residual": ([C.POINTER(fx_mg_config), I32, F32, P, P], C.c_int),

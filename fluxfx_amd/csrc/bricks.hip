@@ -501,6 +501,65 @@ __global__ __launch_bounds__(256) void k_brick_required(ReqArgs A, uint8_t* __re
   if (need) flags[(int64_t(z / A.side) * A.grid + y / A.side) * A.grid + x / A.side] = 1;
 }
 
+// Brick map of a cubic n^3 domain: map[(bz*g + by)*g + bx] = slot or -1.
+__global__ void k_brick_map(int n_active, const uint32_t* __restrict__ active, const int4* __restrict__ coords,
+                            int g, int32_t* __restrict__ map) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n_active) return;
+  const uint32_t slot = active[t];
+  const int4 c = coords[slot];
+  if (c.x >= 0 && c.y >= 0 && c.z >= 0 && c.x < g && c.y < g && c.z < g)
+    map[(int64_t(c.z) * g + c.y) * g + c.x] = int32_t(slot);
+}
+
+// Semi-Lagrangian transport of the brick field (native/projection.mm:139-147
+// transportDensity): every voxel of an active brick inside the n^3 domain is
+// traced back through the MAC velocity (RK2, cells per step) and samples the
+// old field trilinearly at the clamped donor cells; donors in inactive bricks
+// read 0, voxels outside the domain get 0.  Same fp32 operations as the
+// preflight (contraction off), so it equals the dense float32 transport
+// wherever the active bricks cover the preflight's flags.
+__global__ __launch_bounds__(256) void k_brick_transport(ReqArgs A, const uint32_t* __restrict__ active,
+                                                         const int4* __restrict__ coords,
+                                                         const int32_t* __restrict__ map,
+                                                         const float* __restrict__ src, float* __restrict__ dst) {
+#pragma clang fp contract(off)
+  const uint32_t slot = active[blockIdx.x];
+  const int4 bc = coords[slot];
+  const int side = A.side, n = A.n, g = A.grid, vox = side * side * side;
+  for (int l = threadIdx.x; l < vox; l += blockDim.x) {
+    const int x = bc.x * side + l % side, y = bc.y * side + (l / side) % side, z = bc.z * side + l / (side * side);
+    float value = 0.0f;
+    if (x >= 0 && y >= 0 && z >= 0 && x < n && y < n && z < n) {
+      const float px = float(x) + 0.5f, py = float(y) + 0.5f, pz = float(z) + 0.5f;
+      const float mx = px - 0.5f * sample_face_cells(A, 0, px, py, pz);
+      const float my = py - 0.5f * sample_face_cells(A, 1, px, py, pz);
+      const float mz = pz - 0.5f * sample_face_cells(A, 2, px, py, pz);
+      const float top = float(n - 1);
+      const float qx = fminf(fmaxf(px - sample_face_cells(A, 0, mx, my, mz) - 0.5f, 0.0f), top);
+      const float qy = fminf(fmaxf(py - sample_face_cells(A, 1, mx, my, mz) - 0.5f, 0.0f), top);
+      const float qz = fminf(fmaxf(pz - sample_face_cells(A, 2, mx, my, mz) - 0.5f, 0.0f), top);
+      const float fbx = floorf(qx), fby = floorf(qy), fbz = floorf(qz);
+      const float fx = qx - fbx, fy = qy - fby, fz = qz - fbz;
+      const int bx = int(fbx), by = int(fby), bz = int(fbz);
+      for (int dz = 0; dz < 2; ++dz)
+        for (int dy = 0; dy < 2; ++dy)
+          for (int dx = 0; dx < 2; ++dx) {
+            const int cx = min(bx + dx, n - 1), cy = min(by + dy, n - 1), cz = min(bz + dz, n - 1);
+            const int32_t s = map[(int64_t(cz / side) * g + cy / side) * g + cx / side];
+            const float d = s < 0 ? 0.0f
+                                  : src[size_t(s) * vox + ((cz % side) * side + cy % side) * side + cx % side];
+            const float wx = dx ? fx : 1.0f - fx, wy = dy ? fy : 1.0f - fy, wz = dz ? fz : 1.0f - fz;
+            const float t = wx * wy;
+            const float wgt = t * wz;
+            const float term = wgt * d;
+            value = value + term;
+          }
+    }
+    dst[size_t(slot) * vox + l] = value;
+  }
+}
+
 // Diagnostic probe: every reserved voxel is set to the sentinel, then each
 // active brick writes, per voxel, a pattern of its slot, voxel index and the
 // six neighbour ids; the host verifies every slot.
@@ -541,6 +600,10 @@ struct fx_bricks {
   uint32_t dev_capacity = 0;
   uint32_t synced_active = 0;
   uint32_t reallocations = 0;
+  float* field2 = nullptr;     // transport target, capacity * side^3 (on demand)
+  uint32_t field2_capacity = 0;
+  int32_t* map = nullptr;      // brick map of the last transport domain (on demand)
+  int64_t map_cells = 0;
   bool closed = false;
 };
 
@@ -553,8 +616,12 @@ static uint64_t pool_bytes(uint32_t side, uint32_t capacity) {
 
 static void release(fx_bricks* b) {
   for (void* p : {(void*)b->field, (void*)b->coords, (void*)b->active, (void*)b->nbr, (void*)b->keys,
-                  (void*)b->vals})
+                  (void*)b->vals, (void*)b->field2, (void*)b->map})
     if (p) (void)hipFree(p);
+  b->field2 = nullptr;
+  b->field2_capacity = 0;
+  b->map = nullptr;
+  b->map_cells = 0;
   b->field = nullptr;
   b->coords = nullptr;
   b->active = nullptr;
@@ -898,6 +965,42 @@ int fx_brick_required(int32_t side, int32_t n, const float* density, const float
   hipLaunchKernelGGL(k_fill_u8, dim3(64), dim3(256), 0, S(stream), int64_t(A.grid) * A.grid * A.grid, flags);
   hipLaunchKernelGGL(k_brick_required, dim3(unsigned((cells + 255) / 256)), dim3(256), 0, S(stream), A, flags);
   FX_LAUNCHED("k_brick_required");
+  return FX_OK;
+}
+
+int fx_bricks_transport(fx_bricks* b, int32_t n, const float* u, const float* v, const float* w, void* stream) {
+  if (int e = ready(b)) return e;
+  FX_REQUIRE(n >= 1 && n <= 1024 && n % b->side == 0, FX_EINVAL, "transport: n must be 1..1024 and a multiple of side");
+  FX_REQUIRE(u && v && w, FX_EINVAL, "transport: null velocity");
+  FX_REQUIRE(b->field, FX_ESTATE, "brick pool has no device arrays: call fx_bricks_sync first");
+  hipStream_t s = S(stream);
+  const size_t vox = size_t(b->side) * b->side * b->side;
+  if (b->field2_capacity != b->dev_capacity) {
+    if (b->field2) FX_HIP(hipFree(b->field2));
+    b->field2 = nullptr;
+    b->field2_capacity = 0;
+    FX_HIP(hipMalloc(&b->field2, sizeof(float) * vox * b->dev_capacity));
+    b->field2_capacity = b->dev_capacity;
+  }
+  const int g = n / b->side;
+  const int64_t cells = int64_t(g) * g * g;
+  if (b->map_cells < cells) {
+    if (b->map) FX_HIP(hipFree(b->map));
+    b->map = nullptr;
+    b->map_cells = 0;
+    FX_HIP(hipMalloc(&b->map, sizeof(int32_t) * cells));
+    b->map_cells = cells;
+  }
+  hipLaunchKernelGGL(k_fill_i32, dim3(256), dim3(256), 0, s, cells, b->map, -1);
+  if (b->synced_active) {
+    hipLaunchKernelGGL(k_brick_map, dim3((b->synced_active + 255) / 256), dim3(256), 0, s, int(b->synced_active),
+                       b->active, b->coords, g, b->map);
+    ReqArgs A{n, b->side, g, nullptr, u, v, w, 0.0f, 0.0f, 0.0f, 1.0f, 0.0f};
+    hipLaunchKernelGGL(k_brick_transport, dim3(b->synced_active), dim3(b->side == 8 ? 256 : 256), 0, s, A, b->active,
+                       b->coords, b->map, b->field, b->field2);
+  }
+  FX_LAUNCHED("k_brick_transport");
+  std::swap(b->field, b->field2);
   return FX_OK;
 }
 

@@ -213,6 +213,16 @@ class BrickPool:
                   BrickPool._stream())
         return flags
 
+    def transport(self, n, u, v, w):
+        """One semi-Lagrangian RK2 step of the brick field on the cubic n^3
+        domain (native/projection.mm:139-147): u/v/w are the MAC face tensors
+        in cells per step; donors in inactive bricks read 0."""
+        import torch
+        for t in (u, v, w):
+            if t.dtype != torch.float32 or not t.is_cuda or not t.is_contiguous():
+                raise ValueError("faces are contiguous float32 cuda tensors")
+        _lib.call("fx_bricks_transport", self._h, int(n), u.data_ptr(), v.data_ptr(), w.data_ptr(), self._stream())
+
     @staticmethod
     def required(density, u, v, w, side, emitter):
         """Transport-aware preflight (native/projection.mm:126-138): uint8 cuda

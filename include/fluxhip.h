@@ -596,6 +596,13 @@ int fx_brick_support(int32_t side, int32_t nx, int32_t ny, int32_t nz, const flo
  * 0.5n, 0.45n, n/12, 0.1). */
 int fx_brick_required(int32_t side, int32_t n, const float* density, const float* u, const float* v, const float* w,
                       const float* emitter, uint8_t* flags, void* stream);
+/* One semi-Lagrangian step of the brick field on the cubic n^3 domain
+ * (native/projection.mm:139-147 transportDensity): each voxel of an active
+ * brick samples the old field trilinearly at its RK2 back-trace through the
+ * MAC velocity u/v/w (cells per step, dense n^3 face arrays); donors in
+ * inactive bricks read 0.  The result replaces the field (a second pool
+ * buffer and an n/side brick map are allocated on the first call). */
+int fx_bricks_transport(fx_bricks* b, int32_t n, const float* u, const float* v, const float* w, void* stream);
 /* BrickPool.probe (brick_bindings.inc:64-90): resets every reserved voxel to
  * a sentinel, writes a per-voxel pattern of (slot, voxel, neighbour ids) into
  * active bricks and verifies all slots on the host. */

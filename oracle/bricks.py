@@ -251,3 +251,51 @@ def required_flags(density, u, v, w, side, emitter):
                 need |= (dens[c[2], c[1], c[0]] > 0) | emitted(c[0], c[1], c[2])
     g = n // side
     return need.reshape(g, side, g, side, g, side).any(axis=(1, 3, 5)).astype(np.uint8)
+
+
+def transport_dense(density, u, v, w):
+    """One semi-Lagrangian RK2 step of a dense cubic field in float32
+    (native/projection.mm:139-147 transportDensity with every cell solved),
+    in the kernel's operation order: clamped back-trace, trilinear weights
+    (wx*wy)*wz, terms summed in z, y, x corner order."""
+    f32 = np.float32
+    n = density.shape[0]
+    dens = np.asarray(density, f32)
+    faces = (np.asarray(u, f32), np.asarray(v, f32), np.asarray(w, f32))
+
+    def face_at(a, x, y, z):
+        f = faces[a]
+        sz, sy, sx = f.shape
+        ok = (x >= 0) & (y >= 0) & (z >= 0) & (x < sx) & (y < sy) & (z < sz)
+        return np.where(ok, f[np.clip(z, 0, sz - 1), np.clip(y, 0, sy - 1), np.clip(x, 0, sx - 1)], f32(0))
+
+    def trilinear(q, top, read):
+        b = [np.floor(t) for t in q]
+        fr = [t - bt for t, bt in zip(q, b)]
+        bi = [bt.astype(np.int64) for bt in b]
+        acc = np.zeros_like(q[0])
+        for zz in (0, 1):
+            for yy in (0, 1):
+                for xx in (0, 1):
+                    wx = fr[0] if xx else f32(1) - fr[0]
+                    wy = fr[1] if yy else f32(1) - fr[1]
+                    wz = fr[2] if zz else f32(1) - fr[2]
+                    acc = acc + ((wx * wy) * wz) * read(bi[0] + xx, bi[1] + yy, bi[2] + zz)
+        return acc
+
+    def sample(a, px, py, pz):
+        size = [n + (a == 0), n + (a == 1), n + (a == 2)]
+        q = [np.minimum(np.maximum(p - (f32(0) if a == k else f32(.5)), f32(0)), f32(size[k] - 1))
+             for k, p in enumerate((px, py, pz))]
+        return trilinear(q, None, lambda x, y, z: face_at(a, x, y, z))
+
+    z, y, x = np.meshgrid(np.arange(n), np.arange(n), np.arange(n), indexing="ij")
+    px, py, pz = (t.astype(f32) + f32(.5) for t in (x, y, z))
+    mx = px - f32(.5) * sample(0, px, py, pz)
+    my = py - f32(.5) * sample(1, px, py, pz)
+    mz = pz - f32(.5) * sample(2, px, py, pz)
+    top = f32(n - 1)
+    q = [np.minimum(np.maximum(p - sample(a, mx, my, mz) - f32(.5), f32(0)), top)
+         for a, p in enumerate((px, py, pz))]
+    return trilinear(q, top, lambda cx, cy, cz: dens[np.minimum(cz, n - 1), np.minimum(cy, n - 1),
+                                                      np.minimum(cx, n - 1)])

@@ -295,3 +295,18 @@ def test_required_flags_closed_forms():
     # 13..18 in x and y and 12..16 in z: the 8 central bricks
     flagged = {tuple(int(v) for v in t) for t in np.argwhere(em)}
     assert flagged == {(z, y, x) for z in (1, 2) for y in (1, 2) for x in (1, 2)}
+
+
+def test_transport_dense_closed_forms():
+    """The float32 transport oracle: no flow is the identity, a uniform
+    1-cell displacement shifts the field (clamped at the low faces), a
+    uniform half-cell displacement averages neighbour pairs exactly."""
+    n = 16
+    rng = np.random.default_rng(3)
+    dens = (rng.integers(0, 64, (n, n, n)) / 8).astype(np.float32)  # dyadic values: exact averages
+    np.testing.assert_array_equal(ob.transport_dense(dens, *_faces(n)), dens)
+    shifted = ob.transport_dense(dens, *_faces(n, (1.0, 0.0, 0.0)))
+    np.testing.assert_array_equal(shifted[:, :, 1:], dens[:, :, :-1])
+    np.testing.assert_array_equal(shifted[:, :, 0], dens[:, :, 0])
+    half = ob.transport_dense(dens, *_faces(n, (0.0, 0.0, 0.5)))
+    np.testing.assert_array_equal(half[1:], (dens[1:] + dens[:-1]) / 2)
