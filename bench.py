@@ -186,10 +186,10 @@ def smoother_roofline(device, sim, reps=40):
             "bytes_per_launch": bytes_per, "achieved_gbs": gbs}
 
 
-def measured_traffic(kernel_key="k_smooth_q<float, 1, 4>"):
+def measured_traffic(kernel_key="k_smooth_q<float, 1, 4>", summary="step_256_hbm_traffic.md"):
     """HBM bytes per launch of the roofline kernel from the committed rocprofv3
     FETCH_SIZE/WRITE_SIZE summary (None when absent or for another kernel)."""
-    path = ROOT / "profiles" / "step_256_hbm_traffic.md"
+    path = ROOT / "profiles" / summary
     if not path.exists():
         return None
     for line in path.read_text().splitlines():
@@ -282,6 +282,7 @@ def bricks_bench(device, n, steps, warmup, cpu):
     is the support scan of the dense field, the gather of the active bricks
     and their scatter back (topology update excluded: host work, timed apart)."""
     import torch
+    from fluxfx_amd import _lib
     from fluxfx_amd.native import BrickPool
     lo, m = (72 * n) // 256 // 8 * 8, (112 * n) // 256 // 8 * 8
     g = torch.Generator(device="cuda").manual_seed(5)
@@ -313,6 +314,10 @@ def bricks_bench(device, n, steps, warmup, cpu):
         torch.cuda.synchronize()
         return e0.elapsed_time(e1) / reps
 
+    # the copy kernel of known size that rocprof's FETCH/WRITE counters are scaled by
+    for _ in range(5):
+        _lib.call("fx_scale", dense.data_ptr(), out.data_ptr(), n ** 3, 1.0, None)
+    torch.cuda.synchronize()
     cyc_ms = timed(cycle, steps)
     sup_ms = timed(lambda: BrickPool.support(dense, 8, 0.0), 4 * steps)
     assert torch.equal(out, dense)
@@ -339,7 +344,8 @@ def bricks_bench(device, n, steps, warmup, cpu):
         "transport": {"ms": tr_ms, "bricks": nb_t, "displacement_cells": [.6, .2, .1],
                       "voxel_updates_per_s": nb_t * 512 / (tr_ms * 1e-3)},
         "roofline": {"bound": "hbm", "achieved": scan_bytes / (sup_ms * 1e-3) / 1e9, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": scan_bytes / (sup_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, "traffic": None,
+                     "unit": "GB/s", "frac": scan_bytes / (sup_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                     "traffic": measured_traffic("k_brick_support_rows", "bricks_256_hbm_traffic.md"),
                      "kernel": "k_brick_support_rows (support scan of the dense field)", "avg_ms": sup_ms,
                      "bytes_per_launch": scan_bytes},
     }
