@@ -996,7 +996,7 @@ int fx_bricks_transport(fx_bricks* b, int32_t n, const float* u, const float* v,
     hipLaunchKernelGGL(k_brick_map, dim3((b->synced_active + 255) / 256), dim3(256), 0, s, int(b->synced_active),
                        b->active, b->coords, g, b->map);
     ReqArgs A{n, b->side, g, nullptr, u, v, w, 0.0f, 0.0f, 0.0f, 1.0f, 0.0f};
-    hipLaunchKernelGGL(k_brick_transport, dim3(b->synced_active), dim3(b->side == 8 ? 256 : 256), 0, s, A, b->active,
+    hipLaunchKernelGGL(k_brick_transport, dim3(b->synced_active), dim3(256), 0, s, A, b->active,
                        b->coords, b->map, b->field, b->field2);
   }
   FX_LAUNCHED("k_brick_transport");
