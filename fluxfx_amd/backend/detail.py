@@ -22,6 +22,7 @@ class DetailPasses:
         self.face_predictor = None
         self.face_bounds = None
         self.curl = None
+        self.cell_force = None
         self._scalar_pred = []
         self._scalar_bounds = []
 
@@ -32,7 +33,7 @@ class DetailPasses:
         return (8 * cells if self.predictor is not None else 0) + \
             (4 * faces if self.face_predictor is not None else 0) + \
             (8 * faces if self.face_bounds is not None else 0) + 4 * cells * len(self._scalar_pred) + \
-            8 * cells * len(self._scalar_bounds) + (16 * cells if self.curl is not None else 0)
+            8 * cells * len(self._scalar_bounds) + (32 * cells if self.curl is not None else 0)
 
     @property
     def _solid(self):
@@ -117,11 +118,14 @@ class DetailPasses:
             raise ValueError("Advection input/output textures must be distinct")
         if self.curl is None:
             self.curl = self.device.texture(self.grid.shape, channels=4, nonnegative=False)
+            self.cell_force = self.device.texture(self.grid.shape, channels=4, nonnegative=False)  # 3 planes used
         _lib.call("fx_curl", self._g, *(ptr(f) for f in fields), ptr(self.curl), self.device.stream)
-        _lib.call("fx_confine", self._g, *(ptr(f) for f in fields), ptr(self.curl), *(ptr(o) for o in outputs),
-                  float(dt), float(strength), float(limit), self.device.stream)
+        # the cell forces once, then each face the mean of its two cells' (bit-identical to fx_confine)
+        _lib.call("fx_confine_split", self._g, *(ptr(f) for f in fields), ptr(self.curl), ptr(self.cell_force),
+                  *(ptr(o) for o in outputs), float(dt), float(strength), float(limit), self.device.stream)
 
     def close(self):
         self.predictor = self.reverse = self.face_predictor = self.face_bounds = self.curl = None
+        self.cell_force = None
         self._scalar_pred = []
         self._scalar_bounds = []

@@ -79,6 +79,47 @@ __global__ void k_confine(const float4* __restrict__ cu, int nx, int ny, int nz,
   out[i] = in[i] + dt * fa;
 }
 
+// Split form: the force of every cell once (stored as three component
+// planes), then each face adds the mean of its two cells' stored components;
+// faces on the domain planes (one cell outside) evaluate that cell's force
+// from the clamped reads directly, as the one-pass kernels do, so both forms
+// agree bit for bit.
+__global__ void k_cell_force(const float4* __restrict__ cu, int nx, int ny, int nz, float ix, float iy, float iz,
+                             float strength, float limit, float* __restrict__ force, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int x = i % nx, r = i / nx, y = r % ny, z = r / ny;
+  const float3 f = confinement_force(cu, nx, ny, nz, ix, iy, iz, strength, limit, x, y, z);
+  force[i] = f.x;
+  force[n + i] = f.y;
+  force[2 * n + i] = f.z;
+}
+
+template <int AXIS>
+__device__ __forceinline__ float force_comp(const float3& f) {
+  return AXIS == 0 ? f.x : (AXIS == 1 ? f.y : f.z);
+}
+
+template <int AXIS>
+__global__ void k_confine_split(const float4* __restrict__ cu, const float* __restrict__ force, int nx, int ny,
+                                int nz, float ix, float iy, float iz, const float* __restrict__ in,
+                                float* __restrict__ out, float dt, float strength, float limit, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int sx = nx + (AXIS == 0), sy = ny + (AXIS == 1);
+  const int x = i % sx, r = i / sx, y = r % sy, z = r / sy;
+  const int c = AXIS == 0 ? x : (AXIS == 1 ? y : z), size = AXIS == 0 ? nx : (AXIS == 1 ? ny : nz);
+  const float* plane = force + int64_t(AXIS) * nx * ny * nz;
+  const int64_t cell = x + int64_t(nx) * (y + int64_t(ny) * z);  // a cell index when c < size
+  const int64_t below = cell - (AXIS == 0 ? 1 : (AXIS == 1 ? nx : int64_t(nx) * ny));
+  const float a = c < size ? plane[cell]
+                           : force_comp<AXIS>(confinement_force(cu, nx, ny, nz, ix, iy, iz, strength, limit, x, y, z));
+  const float b = c > 0 ? plane[below]
+                        : force_comp<AXIS>(confinement_force(cu, nx, ny, nz, ix, iy, iz, strength, limit,
+                                                             x - (AXIS == 0), y - (AXIS == 1), z - (AXIS == 2)));
+  out[i] = in[i] + dt * (0.5f * (a + b));
+}
+
 }  // namespace fx
 
 using namespace fx;
@@ -118,4 +159,28 @@ int fx_confine(fx_grid g, const float* u, const float* v, const float* w, const 
   return FX_OK;
 }
 
+int fx_confine_split(fx_grid g, const float* u, const float* v, const float* w, const float* curl4, float* force3,
+                     float* uo, float* vo, float* wo, float dt, float strength, float limit, void* stream) {
+  if (int e = check_grid(g)) return e;
+  FX_REQUIRE(u && v && w && curl4 && force3 && uo && vo && wo, FX_EINVAL, "confine: null array");
+  FX_REQUIRE(uo != u && vo != v && wo != w, FX_EINVAL, "Advection input/output textures must be distinct");
+  FX_REQUIRE(std::isfinite(strength) && strength >= 0.0f && std::isfinite(limit) && limit > 0.0f, FX_EINVAL,
+             "confine: strength must be >= 0 and limit > 0");
+  FX_REQUIRE((reinterpret_cast<uintptr_t>(curl4) & 15u) == 0, FX_EINVAL, "confine: curl field must be 16-byte aligned");
+  const float ix = g.nx / g.ex, iy = g.ny / g.ey, iz = g.nz / g.ez;
+  const float4* cu = reinterpret_cast<const float4*>(curl4);
+  const int n = g.nx * g.ny * g.nz;
+  const int nu = (g.nx + 1) * g.ny * g.nz, nv = g.nx * (g.ny + 1) * g.nz, nw = g.nx * g.ny * (g.nz + 1);
+  hipStream_t s = S(stream);
+  hipLaunchKernelGGL(k_cell_force, dim3((n + 255) / 256), dim3(256), 0, s, cu, g.nx, g.ny, g.nz, ix, iy, iz, strength,
+                     limit, force3, n);
+  hipLaunchKernelGGL(k_confine_split<0>, dim3((nu + 255) / 256), dim3(256), 0, s, cu, force3, g.nx, g.ny, g.nz, ix,
+                     iy, iz, u, uo, dt, strength, limit, nu);
+  hipLaunchKernelGGL(k_confine_split<1>, dim3((nv + 255) / 256), dim3(256), 0, s, cu, force3, g.nx, g.ny, g.nz, ix,
+                     iy, iz, v, vo, dt, strength, limit, nv);
+  hipLaunchKernelGGL(k_confine_split<2>, dim3((nw + 255) / 256), dim3(256), 0, s, cu, force3, g.nx, g.ny, g.nz, ix,
+                     iy, iz, w, wo, dt, strength, limit, nw);
+  FX_LAUNCHED("fx_confine_split");
+  return FX_OK;
+}
 }  // extern "C"

@@ -409,6 +409,10 @@ int fx_curl(fx_grid g, const float* u, const float* v, const float* w, float* cu
  * strength*h*cross(grad|w|/|grad|w||, w) capped at `limit`. */
 int fx_confine(fx_grid g, const float* u, const float* v, const float* w, const float* curl4, float* uo, float* vo,
                float* wo, float dt, float strength, float limit, void* stream);
+/* fx_confine with the cell forces computed once: force3 is caller scratch of
+ * 3*nx*ny*nz floats (component planes); results equal fx_confine's bit for bit. */
+int fx_confine_split(fx_grid g, const float* u, const float* v, const float* w, const float* curl4, float* force3,
+                     float* uo, float* vo, float* wo, float dt, float strength, float limit, void* stream);
 
 
 /* --------------------------------------------------------------------------

@@ -310,3 +310,29 @@ def test_transport_dense_closed_forms():
     np.testing.assert_array_equal(shifted[:, :, 0], dens[:, :, 0])
     half = ob.transport_dense(dens, *_faces(n, (0.0, 0.0, 0.5)))
     np.testing.assert_array_equal(half[1:], (dens[1:] + dens[:-1]) / 2)
+
+
+def test_pool_errors_follow_the_reference():
+    """Constructor, budget, closed-pool and argument errors
+    (native/brick_bindings.inc:9-22, brick_pool.hpp:16-31, :33)."""
+    from fluxfx_amd import _lib
+    if not _lib.LIB_PATH.exists():
+        pytest.skip("libfluxhip.so is not built")
+    from fluxfx_amd.native import BrickPool
+    for side, cap in ((12, 64), (8, 0), (16, 262145)):
+        with pytest.raises(ValueError, match="side: 8 or 16; capacity: 1..262144"):
+            BrickPool(side, cap, 2 ** 30)
+    with pytest.raises(ValueError, match="exceed budget"):
+        BrickPool(8, 4096, 1 << 20)  # 4096 * (2048 + 28) B > 1 MiB
+    pool = BrickPool(8, 16, 1 << 20)
+    with pytest.raises(ValueError, match="At most 64 sources"):
+        pool.update_regions([(0.5,) * 7] * 65)
+    with pytest.raises(ValueError, match="Use 1..32 dispatches"):
+        pool.probe(0)
+    with pytest.raises(ValueError, match="Brick coordinates"):
+        pool.lookup(0, 2 ** 21, 0)
+    assert pool.stats()["capacity"] == 16 and not pool.stats()["closed"]
+    pool.close()
+    assert pool.stats()["closed"]
+    with pytest.raises(RuntimeError, match="closed"):
+        pool.activate(0, 0, 0)

@@ -253,6 +253,27 @@ def test_curl_and_confinement_match_oracle(device, shape, extent):
         assert rel_err(g, o) < 1e-5
 
 
+@pytest.mark.parametrize("shape,extent", [((12, 10, 8), (1.3, .8, 1.7)), ((33, 20, 17), (1., .6, .5))])
+def test_split_confinement_equals_one_pass(device, shape, extent):
+    """fx_confine_split (cell forces once, faces from the stored planes, the
+    domain-plane faces from clamped reads) equals fx_confine bit for bit."""
+    from fluxfx_amd import _lib
+    from fluxfx_amd.physics.config import GridSpec
+    og = mac.Grid(shape, extent)
+    rng = np.random.default_rng(17)
+    vel = [rng.standard_normal(og.face_array_shape(a)) for a in range(3)]
+    det, fields, outs, _ = _confine_fields(device, og, vel)
+    det.confine(fields, outs, .05, 2.5, 1.5)
+    grid = GridSpec(og.shape, og.extent, max_dim=512)
+    outs2 = [device.texture(sh) for sh in grid.face_shapes]
+    _lib.call("fx_confine", _lib.make_grid(og.shape, og.extent), *(f.ptr for f in fields), det.curl.ptr,
+              *(o.ptr for o in outs2), .05, 2.5, 1.5, device.stream)
+    for a, b in zip(outs, outs2):
+        x = a.tensor.detach().cpu().numpy().view(np.uint32)
+        y = b.tensor.detach().cpu().numpy().view(np.uint32)
+        assert np.array_equal(x, y), int((x != y).sum())
+
+
 def test_confinement_reference_gates(device):
     """scripts/detail_validate.py:60-84 on the GPU (its 32^3 grid): solid-body
     rotation has curl (0, 0, 2) in the interior (1e-5) and no interior force
