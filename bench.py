@@ -14,7 +14,7 @@ trilinear prolongation, FP64 direct solve on 8^3, warm-started impulse, run to
 Modes: plain (FP32 stationary V-cycles, the drop-in), mixed (BF16 V-cycles
 preconditioning FP32 flexible CG) and learned (one FPCG step preconditioned by
 a U-Net applied to the 8-child mean of the residual on the 128^3 level, its
-output prolonged and smoothed, then BF16 V-cycle FPCG; the network was trained
+output prolonged as a V-cycle correction, then BF16 V-cycle FPCG; the network was trained
 on the warm-started solves' increment problems of seeded random 256^3 scenes,
 restricted to 128^3),
 whichever are built.  With the CPU leg the line also carries every field's
@@ -49,6 +49,9 @@ WEIGHTS = ROOT / "fluxfx_amd" / "weights" / "unet_128_inc.npz"
 HBM_PEAK_GBS = 8000.0
 LEARNED_LEVEL = 1
 LEARNED_ITERATIONS = 1
+# damped-Jacobi sweeps after the prolonged network correction: at 256^3 they
+# buy no iterations (0/2/4/6 sweeps: 7.30/7.35/7.40/7.40 per step), so none run
+LEARNED_SWEEPS = 0
 
 
 def scene(n, seed=3):
@@ -125,9 +128,10 @@ def settings_for(n, mode, workload="smoke"):
         if not WEIGHTS.exists():
             raise NotImplementedError(f"no trained U-Net weights at {WEIGHTS}")
         # the network runs one level down (on the 8-child mean of the 256^3
-        # residual), its output prolonged and smoothed like a V-cycle correction
+        # residual), its output prolonged like a V-cycle correction
         kw.update(pressure_precision="BF16", pressure_krylov="LEARNED", learned_weights=str(WEIGHTS),
-                  learned_iterations=LEARNED_ITERATIONS, learned_sweeps=2, learned_level=LEARNED_LEVEL)
+                  learned_iterations=LEARNED_ITERATIONS, learned_sweeps=LEARNED_SWEEPS,
+                  learned_level=LEARNED_LEVEL)
     return PressureSettings(**kw), dt
 
 
