@@ -51,6 +51,11 @@ class DenseProjectedSmoke(DenseThermalAdvection):
         self.projector = None
         self.detail = None
         self.faulted = False
+        # observer(stage, fields): called inside step() with the provisional
+        # velocity before the projection ("provisional") and the transported
+        # scalars before combustion ("transported"), so a caller can read the
+        # intermediate fields of a step; None in normal use
+        self.observer = None
         self._source_data = None
         self._multi_sources = None
         settings = settings or PressureSettings()
@@ -205,9 +210,13 @@ class DenseProjectedSmoke(DenseThermalAdvection):
             # confined field into the old velocity storage, projected into the back buffers
             # (backend/projected.py:172-175)
             self.detail.confine(provisional, self._velocity, dt, s.vorticity_strength, s.vorticity_limit)
+            if self.observer is not None:
+                self.observer("provisional", list(self._velocity))
             self.projector.project(self._velocity, self._velocity_back, dt)
             self._velocity, self._velocity_back = self._velocity_back, self._velocity
         else:
+            if self.observer is not None:
+                self.observer("provisional", provisional)
             self.projector.project(provisional, self._velocity, dt)
         fields = [self._front, self._temperature]
         outs = [self._back, self._temperature_back]
@@ -243,6 +252,10 @@ class DenseProjectedSmoke(DenseThermalAdvection):
         self._temperature, self._temperature_back = self._temperature_back, self._temperature
         if self.combustion:
             self.combustion.fuel, self.combustion.spare = self.combustion.spare, self.combustion.fuel
+        if self.observer is not None:
+            self.observer("transported", [self._front, self._temperature] +
+                          ([self.combustion.fuel] if self.combustion else []))
+        if self.combustion:
             self.combustion.react(self, dt)
         self.time += dt
         self.steps += 1

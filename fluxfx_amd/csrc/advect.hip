@@ -284,8 +284,28 @@ struct Reach {
   }
 };
 
-__device__ __forceinline__ Reach reach_of(const GridDev& g, const uint8_t* __restrict__ cl, V3 o) {
-  return Reach{cl ? float(min(clearance_at(g, cl, o), 16)) : -1.0f};
+// The trace origin of a face or cell site is given by its integer cell: a
+// face lies on the low boundary of cell (x, y, z), and floor() of its
+// floating-point position could fall a rounding step into the cell below.
+__device__ __forceinline__ int clearance_cell(const GridDev& g, const uint8_t* __restrict__ cl, int cx, int cy,
+                                              int cz) {
+  if (cx < 0 || cy < 0 || cz < 0 || cx >= g.nx || cy >= g.ny || cz >= g.nz) return 0;
+  return cl[lin(cx, cy, cz, g.nx, g.ny)];
+}
+
+__device__ __forceinline__ Reach reach_cell(const GridDev& g, const uint8_t* __restrict__ cl, int cx, int cy,
+                                            int cz) {
+  return Reach{cl ? float(min(clearance_cell(g, cl, cx, cy, cz), 16)) : -1.0f};
+}
+
+__device__ __forceinline__ bool halo_cell(const GridDev& g, const uint8_t* __restrict__ s,
+                                          const uint8_t* __restrict__ cl, int cx, int cy, int cz) {
+  if (cl) return clearance_cell(g, cl, cx, cy, cz) <= 1;
+  for (int z = -1; z <= 1; ++z)
+    for (int y = -1; y <= 1; ++y)
+      for (int x = -1; x <= 1; ++x)
+        if (solid_cell(g, s, cx + x, cy + y, cz + z)) return true;
+  return false;
 }
 
 __device__ __forceinline__ V3 clip_fast(const GridDev& g, const uint8_t* __restrict__ s, const Reach& R, V3 o,
@@ -301,9 +321,9 @@ __device__ __forceinline__ bool halo_fast(const GridDev& g, const uint8_t* __res
 
 // the departure/arrival conditions of safe_correction with origin p (R its reach)
 __device__ bool safe_fast(const GridDev& g, const uint8_t* __restrict__ s, const uint8_t* __restrict__ cl,
-                          const Reach& R, V3 p, V3 dep, V3 mid, V3 arr) {
+                          const Reach& R, V3 p, int pcx, int pcy, int pcz, V3 dep, V3 mid, V3 arr) {
   if (R.free(g, p, dep) && R.free(g, p, mid) && R.free(g, p, arr)) return true;
-  if (halo_fast(g, s, cl, p) || halo_fast(g, s, cl, dep) || halo_fast(g, s, cl, arr)) return false;
+  if (halo_cell(g, s, cl, pcx, pcy, pcz) || halo_fast(g, s, cl, dep) || halo_fast(g, s, cl, arr)) return false;
   if (dist3(clip_fast(g, s, R, p, dep), dep) >= 1e-7f) return false;
   if (dist3(clip_fast(g, s, R, p, mid), mid) >= 1e-7f) return false;
   return dist3(clip_fast(g, s, R, p, arr), arr) < 1e-7f;
@@ -396,7 +416,7 @@ __global__ void k_face_predict(GridDev g, Vel V, float dt, const float* __restri
   const uint32_t i = ue << 2;  // byte offset of this face
   const float offx = AXIS == 0 ? 0.0f : 0.5f, offy = AXIS == 1 ? 0.0f : 0.5f, offz = AXIS == 2 ? 0.0f : 0.5f;
   V3 p = v3((x + offx) * g.hx, (y + offy) * g.hy, (z + offz) * g.hz);
-  const Reach R = SOLID ? reach_of(g, clear, p) : Reach{-1.0f};
+  const Reach R = SOLID ? reach_cell(g, clear, x, y, z) : Reach{-1.0f};
   // clearance >= 2 at the cell above the face: both cells of the face are fluid
   if (SOLID && !(R.c >= 2.0f)) {
     int ox = AXIS == 0, oy = AXIS == 1, oz = AXIS == 2;
@@ -466,7 +486,7 @@ __global__ void k_velocity(VelArgs A, fx_sources src) {
   float value = 0.0f;
   bool blocked = false;
   V3 p = v3((x + offx) * hx, (y + offy) * hy, (z + offz) * hz);
-  const Reach R = SOLID ? reach_of(g, A.clear, p) : Reach{-1.0f};
+  const Reach R = SOLID ? reach_cell(g, A.clear, x, y, z) : Reach{-1.0f};
   // clearance >= 2 at the cell above the face: both cells of the face are fluid
   if (SOLID && !(R.c >= 2.0f)) {
     int ox = AXIS == 0, oy = AXIS == 1, oz = AXIS == 2;
@@ -485,7 +505,7 @@ __global__ void k_velocity(VelArgs A, fx_sources src) {
       bool ok = !(arr.x < offx * hx || arr.y < offy * hy || arr.z < offz * hz || arr.x > hbx || arr.y > hby ||
                   arr.z > hbz);
       if (SOLID && ok && !(R.free(g, p, rmid) && R.free(g, p, arr)))
-        ok = !halo_fast(g, A.solid, A.clear, p) && !halo_fast(g, A.solid, A.clear, arr) &&
+        ok = !halo_cell(g, A.solid, A.clear, x, y, z) && !halo_fast(g, A.solid, A.clear, arr) &&
              dist3(clip_fast(g, A.solid, R, p, rmid), rmid) < 1e-7f &&
              dist3(clip_fast(g, A.solid, R, p, arr), arr) < 1e-7f;
       if (ok) {
@@ -513,7 +533,7 @@ __global__ void k_velocity(VelArgs A, fx_sources src) {
       V3 va = velocity_at(g, A.V, rmid);
       V3 arr = v3(p.x + dt * va.x, p.y + dt * va.y, p.z + dt * va.z);
       bool ok = true;
-      if (SOLID) ok = safe_fast(g, A.solid, A.clear, R, p, dep, rmid, arr);
+      if (SOLID) ok = safe_fast(g, A.solid, A.clear, R, p, x, y, z, dep, rmid, arr);
       ok = ok && !(dep.x < lbx || dep.y < lby || dep.z < lbz || dep.x > hbx || dep.y > hby || dep.z > hbz);
       ok = ok && !(arr.x < lbx || arr.y < lby || arr.z < lbz || arr.x > hbx || arr.y > hby || arr.z > hbz);
       if (ok) {
@@ -633,7 +653,7 @@ __global__ void k_scalar_predict(GridDev g, Vel V, float dt, int channels, Scala
   const uint32_t i = ue << 2;  // byte offset of this cell
   const float hx = g.hx, hy = g.hy, hz = g.hz;
   V3 p = v3((x + 0.5f) * hx, (y + 0.5f) * hy, (z + 0.5f) * hz);
-  const Reach R = SOLID ? reach_of(g, clear, p) : Reach{-1.0f};
+  const Reach R = SOLID ? reach_cell(g, clear, x, y, z) : Reach{-1.0f};
   // clearance >= 1: the cell is fluid
   if (SOLID && !(R.c >= 1.0f) && solid[ue]) {
     for (int k = 0; k < channels; ++k) {
@@ -710,7 +730,7 @@ __global__ void k_scalar_advect(ScalarArgs A, fx_sources src) {
   const float dt = A.dt;
   const float hx = g.hx, hy = g.hy, hz = g.hz;
   V3 p = v3((x + 0.5f) * hx, (y + 0.5f) * hy, (z + 0.5f) * hz);
-  const Reach R = SOLID ? reach_of(g, A.clear, p) : Reach{-1.0f};
+  const Reach R = SOLID ? reach_cell(g, A.clear, x, y, z) : Reach{-1.0f};
   // clearance >= 1: the cell is fluid
   if (SOLID && !(R.c >= 1.0f) && A.solid[ue]) {
     for (int k = 0; k < A.channels; ++k) stf(A.out[k], i, 0.0f);
@@ -728,7 +748,7 @@ __global__ void k_scalar_advect(ScalarArgs A, fx_sources src) {
     bool ok_arr = !(arr.x < 0.5f * hx || arr.y < 0.5f * hy || arr.z < 0.5f * hz || arr.x > ux || arr.y > uy ||
                     arr.z > uz);
     if (SOLID && ok_arr && !(R.free(g, p, rmid) && R.free(g, p, arr)))
-      ok_arr = !halo_fast(g, A.solid, A.clear, p) && !halo_fast(g, A.solid, A.clear, arr) &&
+      ok_arr = !halo_cell(g, A.solid, A.clear, x, y, z) && !halo_fast(g, A.solid, A.clear, arr) &&
                dist3(clip_fast(g, A.solid, R, p, rmid), rmid) < 1e-7f &&
                dist3(clip_fast(g, A.solid, R, p, arr), arr) < 1e-7f;
     Corners ca;
@@ -777,7 +797,7 @@ __global__ void k_scalar_advect(ScalarArgs A, fx_sources src) {
     V3 va = velocity_at(g, A.V, rmid);
     V3 arr = v3(p.x + dt * va.x, p.y + dt * va.y, p.z + dt * va.z);
     ok = true;
-    if (SOLID) ok = safe_fast(g, A.solid, A.clear, R, p, dep, rmid, arr);
+    if (SOLID) ok = safe_fast(g, A.solid, A.clear, R, p, x, y, z, dep, rmid, arr);
     float lx = 0.5f * hx, ly = 0.5f * hy, lz = 0.5f * hz;
     float ux = g.ex - 0.5f * hx, uy = g.ey - 0.5f * hy, uz = g.ez - 0.5f * hz;
     ok = ok && !(dep.x < lx || dep.y < ly || dep.z < lz || dep.x > ux || dep.y > uy || dep.z > uz);

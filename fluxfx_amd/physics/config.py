@@ -1,15 +1,43 @@
 """Grid descriptor and base advection settings.
 
-Mirrors physics/config.py:6-62 of the reference (same names, validation and
-errors).  One addition: ``max_dim`` lifts the reference's 128-cell cap for the
-256^3/384^3 workloads; its default keeps the reference behaviour.
+Interface of physics/config.py:6-62 of the reference: the names ``GridSpec``,
+``AdvectionSettings``, ``workgroups`` and ``validate_dt``, their fields,
+defaults, properties and ``ValueError`` messages.  One addition: ``max_dim``
+lifts the reference's 128-cell cap for the 256^3/384^3 workloads; its default
+keeps the reference behaviour.
+
+Validation is table driven: each settings class lists (check, message) rules
+evaluated in order, and the first failing rule raises its message.
 """
 from dataclasses import dataclass
-from math import ceil, isfinite, prod
+from math import isfinite, prod
+
+# one float32 value per cell or face on the device
+FLOAT_BYTES = 4
+
+
+def _finite(*values):
+    return all(isfinite(v) for v in values)
+
+
+def _raise_first(rules, obj):
+    for check, message in rules:
+        if not check(obj):
+            raise ValueError(message)
+
+
+def _grid_rules(max_dim):
+    def dims_ok(g):
+        return len(g.shape) == 3 and all(type(n) is int and 2 <= n <= max_dim for n in g.shape)
+
+    return ((dims_ok, f"P0 grid dimensions must be integers in [2, {max_dim}]"),
+            (lambda g: len(g.extent) == 3 and all(isfinite(e) and e > 0 for e in g.extent),
+             "Domain extents must be finite and positive"))
 
 
 @dataclass(frozen=True)
 class GridSpec:
+    """Cell counts (nx, ny, nz) and the domain size in metres."""
     shape: tuple = (64, 64, 64)
     extent: tuple = (1.0, 1.0, 1.0)
     max_dim: int = 128
@@ -17,30 +45,47 @@ class GridSpec:
     def __post_init__(self):
         if type(self.max_dim) is not int or self.max_dim < 2:
             raise ValueError("max_dim must be an integer >= 2")
-        if len(self.shape) != 3 or any(type(n) is not int or not 2 <= n <= self.max_dim for n in self.shape):
-            raise ValueError(f"P0 grid dimensions must be integers in [2, {self.max_dim}]")
-        if len(self.extent) != 3 or any(not isfinite(x) or x <= 0 for x in self.extent):
-            raise ValueError("Domain extents must be finite and positive")
+        _raise_first(_grid_rules(self.max_dim), self)
 
     @property
     def cell_size(self):
-        return tuple(e / n for e, n in zip(self.extent, self.shape))
+        return tuple(length / count for length, count in zip(self.extent, self.shape))
 
     @property
     def face_shapes(self):
-        return tuple(tuple(n + (1 if i == a else 0) for i, n in enumerate(self.shape)) for a in range(3))
+        """Staggered U, V, W arrays: one extra face along their own axis."""
+        out = []
+        for axis in range(3):
+            shape = list(self.shape)
+            shape[axis] += 1
+            out.append(tuple(shape))
+        return tuple(out)
 
     @property
     def cells(self):
         return prod(self.shape)
 
     @property
-    def mac_bytes(self):
-        return self.density_bytes + 2 * 4 * sum(prod(s) for s in self.face_shapes)
+    def density_bytes(self):
+        """Front and back density buffers."""
+        return 2 * FLOAT_BYTES * self.cells
 
     @property
-    def density_bytes(self):
-        return 2 * 4 * prod(self.shape)
+    def mac_bytes(self):
+        """Density buffers plus front and back U, V, W."""
+        faces = sum(prod(s) for s in self.face_shapes)
+        return self.density_bytes + 2 * FLOAT_BYTES * faces
+
+
+_ADVECTION_RULES = (
+    (lambda s: len(s.velocity) == 3 and len(s.source_center) == 3,
+     "Velocity and source center must have three components"),
+    (lambda s: _finite(*s.velocity, *s.source_center, s.angular_speed, s.source_radius, s.source_rate,
+                       s.dissipation),
+     "Simulation parameters must be finite"),
+    (lambda s: s.source_radius > 0 and s.source_rate >= 0 and s.dissipation >= 0,
+     "Radius must be positive; source rate and dissipation nonnegative"),
+)
 
 
 @dataclass(frozen=True)
@@ -53,22 +98,16 @@ class AdvectionSettings:
     dissipation: float = 0.1
 
     def __post_init__(self):
-        if len(self.velocity) != 3 or len(self.source_center) != 3:
-            raise ValueError("Velocity and source center must have three components")
-        values = (*self.velocity, self.angular_speed, *self.source_center, self.source_radius,
-                  self.source_rate, self.dissipation)
-        if not all(isfinite(v) for v in values):
-            raise ValueError("Simulation parameters must be finite")
-        if self.source_radius <= 0 or self.source_rate < 0 or self.dissipation < 0:
-            raise ValueError("Radius must be positive; source rate and dissipation nonnegative")
+        _raise_first(_ADVECTION_RULES, self)
 
 
 def workgroups(shape, local_size=(4, 4, 4)):
-    if len(shape) != 3 or len(local_size) != 3 or any(n <= 0 for n in (*shape, *local_size)):
+    """Groups per axis that cover ``shape`` with ``local_size`` blocks."""
+    if len(shape) != 3 or len(local_size) != 3 or min(*shape, *local_size) <= 0:
         raise ValueError("Dispatch requires three positive dimensions")
-    return tuple(ceil(n / l) for n, l in zip(shape, local_size))
+    return tuple(-(-count // block) for count, block in zip(shape, local_size))
 
 
 def validate_dt(dt):
-    if not isfinite(dt) or not 0 < dt <= 0.1:
+    if not (isfinite(dt) and 0 < dt <= 0.1):
         raise ValueError("P0 timestep must be finite and in (0, 0.1] seconds")

@@ -70,6 +70,9 @@ class PressureConfig:
     restriction: str = "MEAN"
     max_cycles: int = 50
     warm_start: bool = True
+    # TOLERANCE only: run exactly this many V-cycles instead of stopping at tol
+    # (parity tests give both sides the same amount of work)
+    fixed_cycles: Optional[int] = None
 
 
 @dataclass
@@ -162,7 +165,10 @@ def project(state, provisional, dt, cfg, mg_cache=None):
             # a start no better than zero is dropped (||b - A x0|| >= ||b||)
             if mg.relative_residual(x0, mg.project_mean(before)) >= 1.0:
                 x0 = None
-        p, hist = mg.solve(before, x0, cfg.tol, cfg.max_cycles)
+        if cfg.fixed_cycles is not None:
+            p, hist = mg.solve(before, x0, 0.0, cfg.fixed_cycles)
+        else:
+            p, hist = mg.solve(before, x0, cfg.tol, cfg.max_cycles)
         info["cycles"] = len(hist) - 1
         info["residual_history"] = hist
     else:

@@ -118,12 +118,21 @@ def clip_solid_trace(solid, grid, origin, target):
     return tuple(result[i].reshape(shp) for i in range(3))
 
 
+def cell_of(coord):
+    """floor() of a cell coordinate, taking a coordinate within 1e-9 cells of
+    a lattice plane as lying on it (the exact-arithmetic value of points built
+    as integer multiples of the cell size, which FP64 can put a rounding step
+    below the plane)."""
+    r = np.round(coord)
+    return np.where(np.abs(coord - r) <= 1e-9, r, np.floor(coord)).astype(np.int64)
+
+
 def solid_halo(solid, grid, px, py, pz):
     """Any solid cell in the 3x3x3 block around floor(p) (solid_boundary.glsl:39-44)."""
     n, e = grid.shape, grid.extent
-    cx = _i(px / e[0] * n[0])
-    cy = _i(py / e[1] * n[1])
-    cz = _i(pz / e[2] * n[2])
+    cx = cell_of(px / e[0] * n[0])
+    cy = cell_of(py / e[1] * n[1])
+    cz = cell_of(pz / e[2] * n[2])
     res = np.zeros(np.broadcast(cx, cy, cz).shape, dtype=bool)
     for dz in (-1, 0, 1):
         for dy in (-1, 0, 1):
