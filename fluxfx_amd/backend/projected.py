@@ -36,7 +36,7 @@ from .tolerance import ToleranceProjector
 
 def select_projector(grid, settings, solids):
     """Projector choice of backend/projected.py:14-43, plus the tolerance mode."""
-    if settings.pressure_tolerance > 0:
+    if settings.pressure_tolerance > 0 or settings.pressure_fixed_cycles > 0:
         return ToleranceProjector
     if solids is not None:
         return SolidMultigridProjector if use_multigrid(grid, settings) else SolidPressureProjector
@@ -51,10 +51,12 @@ class DenseProjectedSmoke(DenseThermalAdvection):
         self.projector = None
         self.detail = None
         self.faulted = False
-        # observer(stage, fields): called inside step() with the provisional
-        # velocity before the projection ("provisional") and the transported
-        # scalars before combustion ("transported"), so a caller can read the
-        # intermediate fields of a step; None in normal use
+        # observer(stage, fields): called inside step() with the transported
+        # velocity before vorticity confinement ("advected", only when
+        # confinement runs), the provisional velocity before the projection
+        # ("provisional") and the transported scalars before combustion
+        # ("transported"), so a caller can read the intermediate fields of a
+        # step; None in normal use
         self.observer = None
         self._source_data = None
         self._multi_sources = None
@@ -66,8 +68,9 @@ class DenseProjectedSmoke(DenseThermalAdvection):
             # with moving walls (backend/projected.py:17-29)
             if solid_mask is not None:
                 raise ValueError("Moving colliders are analytic shapes; a voxel mask is static")
-            if settings.pressure_tolerance > 0:
-                raise ValueError("Moving colliders use the masked Jacobi projector; set pressure_tolerance to 0")
+            if settings.pressure_tolerance > 0 or settings.pressure_fixed_cycles > 0:
+                raise ValueError("Moving colliders use the masked Jacobi projector; set pressure_tolerance "
+                                 "and pressure_fixed_cycles to 0")
             self.solids = MovingSolidMask(grid, colliders, device)
             projector_class = projector_class or MovingPressureProjector
         elif colliders or solid_mask is not None:
@@ -209,6 +212,8 @@ class DenseProjectedSmoke(DenseThermalAdvection):
         if s.vorticity_strength > 0:
             # confined field into the old velocity storage, projected into the back buffers
             # (backend/projected.py:172-175)
+            if self.observer is not None:
+                self.observer("advected", provisional)
             self.detail.confine(provisional, self._velocity, dt, s.vorticity_strength, s.vorticity_limit)
             if self.observer is not None:
                 self.observer("provisional", list(self._velocity))

@@ -49,10 +49,8 @@ class ToleranceProjector(PressureProjector):
     def __init__(self, grid, settings=None, device=None, *, solids=None):
         super().__init__(grid, settings, device, solids=solids)
         s = self.settings
-        if s.pressure_tolerance <= 0:
-            raise ValueError("ToleranceProjector needs pressure_tolerance > 0")
-        if s.mg_restriction == "FULL_WEIGHTING" and solids is not None:
-            raise ValueError("Full-weighting restriction is defined for unmasked grids only")
+        if s.pressure_tolerance <= 0 and s.pressure_fixed_cycles <= 0:
+            raise ValueError("ToleranceProjector needs pressure_tolerance > 0 or pressure_fixed_cycles > 0")
         self.shapes = coarsen_to(grid.shape, s.mg_coarsest)
         if len(self.shapes) < 2:
             raise ValueError("Tolerance multigrid needs at least two levels")
@@ -128,8 +126,14 @@ class ToleranceProjector(PressureProjector):
         _lib.call("fx_increment_rhs", self._g, self.flags_ptr, self.pressure.ptr, self.before.ptr, inc["r0"].ptr,
                   self.ws.data_ptr(), stream)
         inc["d"].clear()
-        params = _lib.fx_solve_params(float(s.pressure_tolerance), int(s.pressure_max_cycles),
-                                      int(self._check_from), 7, 1, 0)
+        fixed = int(s.pressure_fixed_cycles)
+        if fixed:
+            # exactly `fixed` V-cycles: the residual is checked once, after the last
+            # (check_from = budget), so the tolerance never ends the loop early
+            params = _lib.fx_solve_params(1e-30, fixed, fixed, 7, 1, 0)
+        else:
+            params = _lib.fx_solve_params(float(s.pressure_tolerance), int(s.pressure_max_cycles),
+                                          int(self._check_from), 7, 1, 0)
         if s.pressure_precision == "FP32" and s.pressure_krylov == "NONE":
             info = self._solve_stationary(params)
         elif s.pressure_krylov == "NONE":
@@ -145,7 +149,7 @@ class ToleranceProjector(PressureProjector):
         self.last_info = info
         self.last_cycles = info.iterations
         self.last_iterations = info.iterations * (len(self.shapes) - 1) * (s.mg_pre + s.mg_post)
-        if not info.converged:
+        if not info.converged and not fixed:
             raise RuntimeError(f"pressure solve missed tolerance {s.pressure_tolerance:g}: "
                                f"residual {info.residual / max(info.bnorm, 1e-300):.3e} after {info.iterations}")
 
@@ -281,6 +285,7 @@ class ToleranceProjector(PressureProjector):
             return {}
         rel = i.residual / i.bnorm if i.bnorm > 0 else 0.0
         return {"v_cycles": i.iterations, "converged": bool(i.converged), "relative_residual": rel,
+                "fixed_cycles": int(self.settings.pressure_fixed_cycles),
                 "bnorm": i.bnorm, "residual0": i.residual0, "fallback": bool(i.fallback),
                 "precision": self.settings.pressure_precision, "krylov": self.settings.pressure_krylov,
                 "tolerance": self.settings.pressure_tolerance, "levels": [v["shape"] for v in self.levels]}

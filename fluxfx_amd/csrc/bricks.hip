@@ -101,7 +101,16 @@ class BrickTopo {
     coords_[slot] = c;
     pos_[slot] = uint32_t(active_.size());
     active_.push_back(slot);
+    fresh_.push_back(slot);
     return slot;
+  }
+
+  // slots activated since the last call (a slot may repeat): their payload
+  // still holds whatever the slot held before and must read as zero
+  std::vector<uint32_t> take_fresh() {
+    std::vector<uint32_t> out;
+    out.swap(fresh_);
+    return out;
   }
 
   bool deactivate(BC c) {
@@ -171,7 +180,7 @@ class BrickTopo {
   std::vector<uint64_t> keys_;
   std::vector<int32_t> vals_;
   std::vector<BC> coords_;
-  std::vector<uint32_t> pos_, free_, active_;
+  std::vector<uint32_t> pos_, free_, active_, fresh_;
   size_t tombstones_ = 0;
 
   static uint64_t pack_key_wrapped(BC c) {
@@ -397,6 +406,12 @@ __global__ __launch_bounds__(256) void k_brick_support_rows(int side, int nx, in
   }
 }
 
+// one block per listed slot: zero that brick's payload
+__global__ void k_brick_clear(int vox, const uint32_t* __restrict__ slots, float* __restrict__ field) {
+  float* brick = field + size_t(slots[blockIdx.x]) * vox;
+  for (int i = threadIdx.x; i < vox; i += blockDim.x) brick[i] = 0.0f;
+}
+
 __global__ void k_fill_u8(int64_t n, uint8_t* __restrict__ a) {
   for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += int64_t(gridDim.x) * blockDim.x)
     a[i] = 0;
@@ -604,6 +619,7 @@ struct fx_bricks {
   uint32_t field2_capacity = 0;
   int32_t* map = nullptr;      // brick map of the last transport domain (on demand)
   int64_t map_cells = 0;
+  uint32_t* fresh = nullptr;   // capacity: slots whose payload a sync clears
   bool closed = false;
 };
 
@@ -629,6 +645,8 @@ static void release(fx_bricks* b) {
   b->keys = nullptr;
   b->vals = nullptr;
   b->table = 0;
+  if (b->fresh) (void)hipFree(b->fresh);
+  b->fresh = nullptr;
   b->dev_capacity = 0;
 }
 
@@ -652,6 +670,7 @@ static int alloc_device(fx_bricks* b, hipStream_t s) {
   FX_HIP(hipMalloc(&b->coords, sizeof(int4) * cap));
   FX_HIP(hipMalloc(&b->active, sizeof(uint32_t) * cap));
   FX_HIP(hipMalloc(&b->nbr, sizeof(int32_t) * 6 * cap));
+  FX_HIP(hipMalloc(&b->fresh, sizeof(uint32_t) * cap));
   b->table = b->topo->keys().size();
   FX_HIP(hipMalloc(&b->keys, sizeof(uint64_t) * b->table));
   FX_HIP(hipMalloc(&b->vals, sizeof(int32_t) * b->table));
@@ -872,6 +891,22 @@ int fx_bricks_sync(fx_bricks* b, void* stream) {
     FX_HIP(hipMemcpyAsync(b->active, act.data(), sizeof(uint32_t) * act.size(), hipMemcpyHostToDevice, s));
   FX_HIP(hipMemcpyAsync(b->keys, t.keys().data(), sizeof(uint64_t) * b->table, hipMemcpyHostToDevice, s));
   FX_HIP(hipMemcpyAsync(b->vals, t.vals().data(), sizeof(int32_t) * b->table, hipMemcpyHostToDevice, s));
+  // newly activated slots start empty, as new brick storage does in the reference
+  std::vector<uint32_t> fresh;
+  {
+    std::vector<char> seen(cap, 0);
+    for (uint32_t slot : b->topo->take_fresh())
+      if (slot < cap && !seen[slot]) {
+        seen[slot] = 1;
+        fresh.push_back(slot);
+      }
+  }
+  if (!fresh.empty()) {
+    FX_HIP(hipMemcpyAsync(b->fresh, fresh.data(), sizeof(uint32_t) * fresh.size(), hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(k_brick_clear, dim3(unsigned(fresh.size())), dim3(256), 0, s, b->side * b->side * b->side,
+                       b->fresh, b->field);
+    FX_LAUNCHED("k_brick_clear");
+  }
   hipLaunchKernelGGL(k_fill_i32, dim3(256), dim3(256), 0, s, int64_t(cap) * 6, b->nbr, -1);
   const int n = int(act.size());
   if (n) {

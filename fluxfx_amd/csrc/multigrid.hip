@@ -300,11 +300,19 @@ __global__ void k_restrict_mean(int cnx, int cny, int nx, int ny, const T* __res
 
 // Adjoint of clamped trilinear prolongation with unit row sums: per axis
 // fine (2i-1, 2i, 2i+1, 2i+2) weighted (1,3,3,1)/8, out-of-range folded onto the edge.
-template <typename T>
-__global__ void k_restrict_full(int cnx, int cny, int cnz, int nx, int ny, int nz, const T* __restrict__ fine,
+// On masked levels the fine residual is zero in solid cells, and a coarse
+// cell with no fluid child (inactive) gets rhs 0.
+template <typename T, int COP>
+__global__ void k_restrict_full(LevelDev C, int nx, int ny, int nz, const T* __restrict__ fine,
                                 T* __restrict__ coarse) {
+  const int cnx = C.nx, cny = C.ny;
   int X = blockIdx.x * blockDim.x + threadIdx.x, Y = blockIdx.y * blockDim.y + threadIdx.y, Z = blockIdx.z;
   if (X >= cnx || Y >= cny) return;
+  const int64_t ci = idx3(X, Y, Z, cnx, cny);
+  if (!cell_active<COP>(C, X, Y, Z, ci)) {
+    coarse[ci] = from_f<T>(0.0f);
+    return;
+  }
   const float wt[4] = {0.125f, 0.375f, 0.375f, 0.125f};
   float acc = 0.0f;
   for (int kz = 0; kz < 4; ++kz) {
@@ -318,7 +326,7 @@ __global__ void k_restrict_full(int cnx, int cny, int cnz, int nx, int ny, int n
       }
     }
   }
-  coarse[idx3(X, Y, Z, cnx, cny)] = from_f<T>(acc);
+  coarse[ci] = from_f<T>(acc);
 }
 
 // prolongation; out may alias in (each fine cell reads only its own value).
@@ -500,10 +508,11 @@ static int launch_residual_restrict(fx_mg_config* c, int l, float inv_dt, hipStr
                          (const TV*)F.rhs, (TV*)F.q, inv_dt);
     }));
     FX_LAUNCHED("k_residual");
-    FX_DISPATCH_T(c->storage, {
-      hipLaunchKernelGGL((k_restrict_full<TV>), level_grid(C), kBlock, 0, s, C.nx, C.ny, C.nz, F.nx, F.ny, F.nz,
+    const LevelDev DC = dev_level(C);
+    FX_DISPATCH_T(c->storage, FX_DISPATCH_OP(C.op, {
+      hipLaunchKernelGGL((k_restrict_full<TV, OPV>), level_grid(C), kBlock, 0, s, DC, F.nx, F.ny, F.nz,
                          (const TV*)F.q, (TV*)C.rhs);
-    });
+    }));
     FX_LAUNCHED("k_restrict_full");
   }
   return FX_OK;
@@ -681,9 +690,10 @@ int fx_mg_restrict(const fx_mg_config* cfg, int32_t index, const void* fine, voi
                          (const TV*)fine, (TV*)coarse);
     });
   } else {
+    // a plain restriction of any array: no coarse masking
     FX_DISPATCH_T(cfg->storage, {
-      hipLaunchKernelGGL((k_restrict_full<TV>), level_grid(C), kBlock, 0, S(stream), C.nx, C.ny, C.nz, F.nx, F.ny,
-                         F.nz, (const TV*)fine, (TV*)coarse);
+      hipLaunchKernelGGL((k_restrict_full<TV, FX_OP_BOX>), level_grid(C), kBlock, 0, S(stream), dev_level(C), F.nx,
+                         F.ny, F.nz, (const TV*)fine, (TV*)coarse);
     });
   }
   FX_LAUNCHED("fx_mg_restrict");

@@ -217,10 +217,7 @@ class BrickPool:
         """One semi-Lagrangian RK2 step of the brick field on the cubic n^3
         domain (native/projection.mm:139-147): u/v/w are the MAC face tensors
         in cells per step; donors in inactive bricks read 0."""
-        import torch
-        for t in (u, v, w):
-            if t.dtype != torch.float32 or not t.is_cuda or not t.is_contiguous():
-                raise ValueError("faces are contiguous float32 cuda tensors")
+        self._faces(int(n), u, v, w)
         _lib.call("fx_bricks_transport", self._h, int(n), u.data_ptr(), v.data_ptr(), w.data_ptr(), self._stream())
 
     @staticmethod
@@ -231,10 +228,12 @@ class BrickPool:
         density (n, n, n); u/v/w the MAC face tensors in cells per step;
         emitter (cx, cy, cz, radius, rate) in cells."""
         import torch
-        n = BrickPool._dense(density)[0]
-        for t in (u, v, w):
-            if t.dtype != torch.float32 or not t.is_cuda or not t.is_contiguous():
-                raise ValueError("faces are contiguous float32 cuda tensors")
+        shape = tuple(int(d) for d in density.shape)
+        if len(shape) != 3 or len(set(shape)) != 1:
+            raise ValueError(f"required: density must be a cube (n, n, n), got {shape}")
+        n = shape[0]
+        BrickPool._faces(n, u, v, w)
+        BrickPool._dense(density)
         g = n // side if side and n % side == 0 else 0
         flags = torch.empty((max(g, 1),) * 3, dtype=torch.uint8, device=density.device)
         em = (C.c_float * 5)(*[float(e) for e in emitter])
@@ -243,6 +242,22 @@ class BrickPool:
         return flags
 
     # -- helpers --------------------------------------------------------------
+    @staticmethod
+    def _faces(n, u, v, w):
+        """MAC faces of the cubic n^3 domain: U (n, n, n+1), V (n, n+1, n),
+        W (n+1, n, n) elements, x fastest (any view of that many elements),
+        contiguous float32 on the GPU.  Sizes are checked first, so a wrong
+        shape never reaches a kernel."""
+        import torch
+        for name, t, axis in (("u", u, 0), ("v", v, 1), ("w", w, 2)):
+            want = [n, n, n]
+            want[2 - axis] += 1
+            if t.numel() != want[0] * want[1] * want[2] or (t.dim() == 3 and list(t.shape) != want):
+                raise ValueError(f"{name}: the n={n} grid's faces have shape {tuple(want)}, got {tuple(t.shape)}")
+        for t in (u, v, w):
+            if t.dtype != torch.float32 or not t.is_cuda or not t.is_contiguous():
+                raise ValueError("faces are contiguous float32 cuda tensors")
+
     @staticmethod
     def _dense(t):
         import torch

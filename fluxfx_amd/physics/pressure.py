@@ -17,6 +17,9 @@ tolerance-driven solvers:
   ``learned_level`` 1 runs the network on the 8-child mean of the residual one
   multigrid level down (a 128^3-trained network on a 256^3 grid sees its own
   resolution) and prolongs its output before the smoothing sweeps.
+* ``pressure_fixed_cycles``: > 0 runs exactly that many stationary V-cycles of
+  the config-mode hierarchy below (the configs' fixed budget, e.g. 8 per step)
+  instead of solving to a tolerance.
 * ``mg_pre``/``mg_post``/``mg_coarsest``/``mg_restriction``/``mg_levels``:
   V-cycle shape.  ``mg_coarsest`` = 4 with 3/3 sweeps is the reference
   (backend/multigrid.py:57-81); the configs use 2/2 and an 8^3 FP64 direct solve.
@@ -58,9 +61,9 @@ _MESSAGES = {
 
 # fields with no reference counterpart (kept apart in checkpoints so that the
 # reference's loader reads the settings it knows)
-EXTENSION_FIELDS = ("pressure_tolerance", "pressure_max_cycles", "pressure_precision", "pressure_krylov", "mg_pre",
-                    "mg_post", "mg_coarsest", "mg_restriction", "learned_weights", "learned_iterations",
-                    "learned_sweeps", "learned_level")
+EXTENSION_FIELDS = ("pressure_tolerance", "pressure_max_cycles", "pressure_fixed_cycles", "pressure_precision",
+                    "pressure_krylov", "mg_pre", "mg_post", "mg_coarsest", "mg_restriction", "learned_weights",
+                    "learned_iterations", "learned_sweeps", "learned_level")
 
 
 def _int_in(value, low, high):
@@ -101,6 +104,8 @@ class PressureSettings(ThermalSettings):
     # tolerance-driven solvers (no reference counterpart)
     pressure_tolerance: float = 0.0
     pressure_max_cycles: int = 40
+    # > 0: exactly this many V-cycles of the config-mode hierarchy per step, no tolerance
+    pressure_fixed_cycles: int = 0
     pressure_precision: str = "FP32"
     pressure_krylov: str = "NONE"
     mg_pre: int = 3
@@ -149,12 +154,20 @@ class PressureSettings(ThermalSettings):
             raise ValueError("Pressure tolerance must lie in [0, 1)")
         if not _int_in(self.pressure_max_cycles, 1, 1000):
             raise ValueError("Maximum pressure cycles must be an integer in [1, 1000]")
+        if not _int_in(self.pressure_fixed_cycles, 0, 64):
+            raise ValueError("Fixed pressure cycles must be an integer in [0, 64]")
+        if self.pressure_fixed_cycles and (self.pressure_tolerance > 0 or self.pressure_krylov != "NONE"):
+            raise ValueError("Fixed pressure cycles run stationary V-cycles without a tolerance")
         if not _int_in(self.mg_pre, 0, 16) or not _int_in(self.mg_post, 0, 16) or self.mg_pre + self.mg_post == 0:
             raise ValueError("Smoothing sweeps must be integers in [0, 16], not both zero")
         if self.mg_coarsest not in (2, 4, 8, 16):
             raise ValueError("Coarsest multigrid side must be 2, 4, 8 or 16")
         if self.learned_level not in (0, 1):
             raise ValueError("The learned preconditioner runs on level 0 or 1")
+        if not _int_in(self.learned_iterations, 0, 1000):
+            raise ValueError("Learned iterations must be an integer in [0, 1000]")
+        if not _int_in(self.learned_sweeps, 0, 16) or self.learned_sweeps % 2:
+            raise ValueError("Learned smoothing sweeps must be an even integer in [0, 16]")
 
 
 def divergence_stats(before, after):

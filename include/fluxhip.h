@@ -3,8 +3,13 @@
  *
  * Every entry point takes plain integers, floats and pointers, returns 0 on
  * success or a negative FX_E* code, and leaves a message for fx_last_error().
- * Device pointers come from the caller (torch tensors own the memory); no entry
- * point allocates device memory.  `stream` is a hipStream_t (NULL = default).
+ * Device pointers come from the caller (torch tensors own the memory), and the
+ * dense-step, solver and network entry points allocate no device memory.  The
+ * one exception is the brick pool handle (fx_bricks_*): it owns its device
+ * mirror and allocates it in fx_bricks_sync (on capacity growth, which also
+ * synchronises the device) and fx_bricks_transport (its second payload buffer
+ * and brick map, on first use or growth).  `stream` is a hipStream_t
+ * (NULL = default).
  *
  * Layout everywhere is the reference's flat x-fast index x + nx*(y + ny*z)
  * (physics/reference.py:3).  Face arrays are U (nx+1,ny,nz), V (nx,ny+1,nz),

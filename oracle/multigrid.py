@@ -360,8 +360,6 @@ class MaskedMG:
         self.solid = solid
         self.pre, self.post = pre, post
         self.restriction = restriction
-        if restriction == "FULL_WEIGHTING" and solid is not None and solid.any():
-            raise ValueError("full-weighting restriction is defined for unmasked grids only")
         self.levels = conductance_levels(grid.shape, solid, coarsest)
         self.invs = [tuple(s / e for s, e in zip(shape, grid.extent)) for shape, _, _ in self.levels]
         shape, cond, _ = self.levels[-1]
@@ -379,8 +377,13 @@ class MaskedMG:
         for _ in range(self.pre):
             p = cond_smooth(p, rhs, cond, inv)
         r = np.where(vol > 0, rhs - cond_apply(p, cond, inv), 0.0)
-        crhs = restrict_full_weighting(r) if self.restriction == "FULL_WEIGHTING" else restrict_mean(r)
         cshape, _, cvol = self.levels[index + 1]
+        if self.restriction == "FULL_WEIGHTING":
+            # masked levels: r is zero in solid cells, and a coarse cell with no
+            # fluid child (volume 0, no equation) gets rhs 0
+            crhs = np.where(cvol > 0, restrict_full_weighting(r), 0.0)
+        else:
+            crhs = restrict_mean(r)
         cp = self.vcycle(np.zeros((cshape[2], cshape[1], cshape[0])), crhs, index + 1)
         p = p + np.where(vol > 0, prolong_masked(cp, cvol, p.shape), 0.0)
         for _ in range(self.post):

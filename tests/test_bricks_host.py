@@ -336,3 +336,24 @@ def test_pool_errors_follow_the_reference():
     assert pool.stats()["closed"]
     with pytest.raises(RuntimeError, match="closed"):
         pool.activate(0, 0, 0)
+
+
+def test_brick_shapes_are_checked_before_any_launch():
+    """BrickPool.required/transport reject a non-cubic density and mis-sized
+    face arrays on the host (plain CPU tensors reach no kernel)."""
+    import torch
+    from fluxfx_amd.native import BrickPool
+    n = 16
+    faces = [torch.zeros(n, n, n + 1), torch.zeros(n, n + 1, n), torch.zeros(n + 1, n, n)]
+    with pytest.raises(ValueError, match="cube"):
+        BrickPool.required(torch.zeros(n, n, n + 8), *faces, 8, (4, 4, 4, 2, 1))
+    for k in range(3):
+        bad = list(faces)
+        bad[k] = torch.zeros(n, n, n)
+        with pytest.raises(ValueError, match="faces have shape"):
+            BrickPool.required(torch.zeros(n, n, n), *bad, 8, (4, 4, 4, 2, 1))
+        with pytest.raises(ValueError, match="faces have shape"):
+            BrickPool._faces(n, *bad)
+    # right sizes, but host tensors: refused as not on the GPU
+    with pytest.raises(ValueError, match="cuda"):
+        BrickPool._faces(n, *faces)

@@ -202,3 +202,55 @@ def test_sparse_transport_equals_dense(device, side):
     got = out.cpu().numpy()
     assert np.array_equal(got.view(np.uint32), expect.view(np.uint32)), np.abs(got - expect).max()
     assert 0 < len(pool.snapshot()) < g ** 3
+
+
+@pytest.mark.parametrize("side", [8])
+def test_two_step_transport_reuses_slots_as_empty_bricks(device, side):
+    """Two transport steps with the preflight re-run in between: the bricks
+    the second preflight adds are activated without a gather, in slots that
+    earlier held other data, and must read as empty (as new brick storage
+    does in the reference, native/projection.mm:219,280,300); the result
+    equals two dense float32 steps bit for bit."""
+    import torch
+    from fluxfx_amd.native import BrickPool
+    print(f"two-step sparse transport: side={side}", flush=True)
+    n = 64
+    g = n // side
+    rng = np.random.default_rng(23)
+    shift = np.full((n, n, n + 1), 0.7, np.float32), np.full((n, n + 1, n), -0.4, np.float32), \
+        np.full((n + 1, n, n), 0.9, np.float32)
+    u, v, w = shift
+    dens = np.abs(_blob_field(rng, (n, n, n)))
+    dens[dens < 0.3] = 0.0
+    tu, tv, tw = (torch.from_numpy(np.ascontiguousarray(a)).cuda() for a in (u, v, w))
+    td = torch.from_numpy(dens).cuda()
+    pool = _pool(side, g ** 3)
+    # every slot first holds nonzero data, then every brick is released
+    pool.activate_box(0, 0, 0, g, g, g)
+    pool.sync()
+    pool.gather(torch.full((n, n, n), 7.0, device="cuda"))
+    for z in range(g):
+        for y in range(g):
+            for x in range(g):
+                pool.deactivate(x, y, z)
+    pool.sync()
+    emitter = (1e6, 1e6, 1e6, 1.0, 0.0)
+    need = BrickPool.required(td, tu, tv, tw, side, emitter) | BrickPool.support(td, side, 0.0)
+    pool.activate_flags(need.cpu().numpy())
+    pool.sync()
+    pool.gather(td)
+    pool.transport(n, tu, tv, tw)
+    step1 = torch.zeros_like(td)
+    pool.scatter(step1)
+    # second preflight on the transported field: new bricks join empty
+    before = len(pool.snapshot())
+    need = BrickPool.required(step1, tu, tv, tw, side, emitter) | BrickPool.support(step1, side, 0.0)
+    pool.activate_flags(need.cpu().numpy())
+    pool.sync()
+    assert len(pool.snapshot()) > before
+    pool.transport(n, tu, tv, tw)
+    out = torch.zeros_like(td)
+    pool.scatter(out)
+    expect = ob.transport_dense(ob.transport_dense(dens, u, v, w), u, v, w)
+    got = out.cpu().numpy()
+    assert np.array_equal(got.view(np.uint32), expect.view(np.uint32)), np.abs(got - expect).max()

@@ -5,6 +5,7 @@ import pytest
 
 from oracle import mac, multigrid as omg
 from tests.helpers import faces_np, cells_np, rel_err, random_faces, sphere_mask
+from tests.parity import gauge_free
 
 pytestmark = pytest.mark.gpu
 
@@ -98,10 +99,10 @@ def test_jacobi_projection_matches_oracle(device, masked):
     solve_dt = 1.0 if masked else dt
     p = mac.jacobi_solve(before, og, solve_dt, 37, 2 / 3, solid)
     assert pp.last_iterations == 37
-    assert rel_err(cells_np(og, pp.pressure), p) < 1e-4
+    assert rel_err(cells_np(og, pp.pressure), p) < TOL
     ref = mac.gradient_correct(bounded, p, og, solve_dt, solid)
     for g, o in zip(faces_np(og, outs), ref):
-        assert rel_err(g, o) < 1e-4
+        assert rel_err(g, o) < TOL
     m = pp.metrics()
     assert m["ratio"] < 1.0 and pp.ready
 
@@ -117,12 +118,13 @@ def test_reference_multigrid_matches_oracle(device):
     pp.project(fields, outs, .04)
     before = mac.divergence(mac.walls(vel, og), og)
     p = omg.reference_mg_solve(before, og, 1.0, 3)
-    assert rel_err(cells_np(og, pp.pressure), p) < 1e-4
+    p_gpu = cells_np(og, pp.pressure)
+    assert rel_err(cells_np(og, pp.pressure), p) < TOL
     assert pp.metrics()["v_cycles"] == 3 and pp.last_coarse_solves == 3
     # second step at a different dt rescales the warm start
     pp.project(fields, outs, .02)
-    p2 = omg.reference_mg_solve(before, og, 1.0, 3, p0=p * .5)
-    assert rel_err(cells_np(og, pp.pressure), p2) < 1e-4
+    p2 = omg.reference_mg_solve(before, og, 1.0, 3, p0=p_gpu * .5)
+    assert rel_err(cells_np(og, pp.pressure), p2) < TOL
 
 
 def test_multigrid_operators(device):
@@ -142,7 +144,7 @@ def test_multigrid_operators(device):
     out = torch.zeros(512, dtype=torch.float32, device=device.device)
     mg.residual(0, .04, p, b, out)
     exp = rhs / .04 - mac.laplacian(pressure.reshape(8, 8, 8), og).ravel()
-    assert np.abs(out.cpu().numpy() - exp).max() < 1e-4 * max(1.0, np.abs(exp).max())
+    assert np.abs(out.cpu().numpy() - exp).max() < TOL * max(1.0, np.abs(exp).max())
     coarse = torch.zeros(64, dtype=torch.float32, device=device.device)
     mg.restrict(0, p, coarse)
     assert np.abs(coarse.cpu().numpy() - omg.restrict_mean(pressure.reshape(8, 8, 8)).ravel()).max() < 1e-6
@@ -177,10 +179,10 @@ def test_solid_multigrid_matches_oracle(device):
     before = mac.divergence(mac.walls(vel, og, solid), og, solid)
     p, levels = omg.solid_mg_solve(before, og, solid, 2 * 4)
     assert [lv["shape"] for lv in pp.levels] == [s for s, _ in levels]
-    assert rel_err(cells_np(og, pp.pressure), p) < 1e-4
+    assert rel_err(cells_np(og, pp.pressure), p) < TOL
     ref = mac.gradient_correct(mac.walls(vel, og, solid), p, og, 1.0, solid)
     for g, o in zip(faces_np(og, outs), ref):
-        assert rel_err(g, o) < 1e-4
+        assert rel_err(g, o) < TOL
 
 
 @pytest.mark.parametrize("masked", [False, True])
@@ -188,8 +190,6 @@ def test_solid_multigrid_matches_oracle(device):
 def test_tolerance_multigrid_reaches_1e5(device, masked, restriction):
     from fluxfx_amd.physics.pressure import PressureSettings
     from fluxfx_amd.backend.tolerance import ToleranceProjector
-    if masked and restriction == "FULL_WEIGHTING":
-        pytest.skip("full weighting is defined for unmasked grids only")
     grid, og, rng, solid, solids = _setup(device, (32, 32, 32), (1., 1., 1.), masked, seed=4)
     fields, vel = _upload_faces(device, grid, random_faces(og, rng))
     s = PressureSettings(pressure_tolerance=1e-5, mg_pre=2, mg_post=2, mg_coarsest=8, mg_restriction=restriction)
@@ -207,10 +207,13 @@ def test_tolerance_multigrid_reaches_1e5(device, masked, restriction):
     # same iteration count as the FP64 oracle V-cycle from a zero start (+-1)
     x, hist = mgo.solve(before, None, 1e-5)
     assert abs((len(hist) - 1) - info["v_cycles"]) <= 1
-    fluid = ~solid if masked else np.ones(og.array_shape, bool)
-    qa, xa = q[fluid] - q[fluid].mean(), x[fluid] - x[fluid].mean()
-    # both iterates sit within 1e-5 relative residual of the same solution
-    assert rel_err(qa, xa) < 5e-3
+    # the same number of V-cycles in FP64 lands on the same iterate
+    x, _ = mgo.solve(before, None, 0.0, info["v_cycles"])
+    fluid = ~solid if masked else None
+    assert rel_err(gauge_free(q, fluid), gauge_free(x, fluid)) < TOL
+    ref = mac.gradient_correct(mac.walls(vel, og, solid), x, og, 1.0, solid)
+    for g, o in zip(faces_np(og, outs), ref):
+        assert rel_err(g, o) < TOL
 
 
 @pytest.mark.parametrize("precision,krylov", [("FP32", "PCG"), ("BF16", "NONE"), ("BF16", "PCG"),
@@ -236,10 +239,14 @@ def test_mixed_precision_solvers_reach_1e5(device, precision, krylov, masked):
     b = mgo.project_mean(before)
     q = cells_np(og, pp.pressure)
     assert mgo.relative_residual(q, b) < 2e-5
-    x, _ = mgo.solve(before, None, 1e-9, max_cycles=200)
-    fluid = ~solid if masked else np.ones(og.array_shape, bool)
-    qa, xa = q[fluid] - q[fluid].mean(), x[fluid] - x[fluid].mean()
-    assert rel_err(qa, xa) < 5e-3
+    x, _ = mgo.solve(before, None, 1e-11, max_cycles=300)
+    fluid = ~solid if masked else None
+    # the error a 1e-5 relative residual leaves (measured worst 6.9e-5 for the
+    # pressure, 3.7e-5 for the faces; see MIXED_TOL in test_gpu_step.py)
+    assert rel_err(gauge_free(q, fluid), gauge_free(x, fluid)) < 2e-4
+    ref = mac.gradient_correct(mac.walls(vel, og, solid), x, og, 1.0, solid)
+    for g, o in zip(faces_np(og, outs), ref):
+        assert rel_err(g, o) < 2e-4
     # the corrected velocity is divergence free to the same relative level
     after = mac.divergence(faces_np(og, outs), og, solid)
     assert np.sqrt((after ** 2).sum()) <= 2e-5 * np.sqrt((b ** 2).sum())
