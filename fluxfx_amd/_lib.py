@@ -170,6 +170,7 @@ SIGNATURES = {
     "fx_mg_prolong": ([C.POINTER(fx_mg_config), I32, P, P, P, P], C.c_int),
     "fx_mg_direct": ([C.POINTER(fx_mg_config), P], C.c_int),
     "fx_project_mean": ([fx_grid, P, P, P, P], C.c_int),
+    "fx_project_mean_noalloc": ([fx_grid, P, P, P], C.c_int),
     "fx_residual_norm": ([fx_grid, P, P, P, C.POINTER(C.c_double), C.POINTER(C.c_double), P, P], C.c_int),
     "fx_mg_solve": ([C.POINTER(fx_mg_config), C.POINTER(fx_solve_params), P, C.POINTER(fx_solve_info), P],
                     C.c_int),

@@ -4,6 +4,8 @@
 // quantity and a single finalize block folds them (no atomics).
 #include "quad.hpp"
 
+#include <mutex>
+
 namespace fx {
 
 // One block per quantity: block b folds the partials of the b-th set bit of
@@ -34,6 +36,8 @@ void ws_finalize(double* ws, int q0, int nq, hipStream_t s) {
 }
 
 // Small pinned host buffer for the scalar read-backs of the solvers.
+constexpr int kMaxDevices = 64;
+
 static double* pinned(int n) {
   static thread_local double* buf = nullptr;
   if (!buf) {
@@ -293,6 +297,42 @@ extern "C" {
 int fx_project_mean(fx_grid g, const uint8_t* flags, float* rhs, double* ws, void* stream) {
   if (int e = check_grid(g)) return e;
   FX_REQUIRE(rhs && ws, FX_EINVAL, "null array");
+  return project_mean_level(level0_of(g, flags), rhs, ws, S(stream));
+}
+
+// fx_project_mean without a caller workspace: the reduction workspace is this
+// file-scope device array, one copy per GPU, found by its device address.
+// Every call writes each workspace value it reads (all block partials and
+// the finalised sums), so nothing needs clearing between calls; calls that
+// overlap on one GPU share the array and race.
+static __device__ __align__(256) double scratch[kWsDoubles];
+
+static int scratch_on_current_device(double** out) {
+  static std::mutex mu;
+  static double* ptr[kMaxDevices] = {};
+  int dev = -1;
+  FX_HIP(hipGetDevice(&dev));
+  FX_REQUIRE(dev >= 0 && dev < kMaxDevices, FX_EDEVICE, "device %d beyond the scratch table", dev);
+  std::lock_guard<std::mutex> lock(mu);
+  if (!ptr[dev]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    FX_REQUIRE(e == hipSuccess && p, FX_EDEVICE, "hipGetSymbolAddress(scratch) failed on device %d: %s", dev,
+               hipGetErrorString(e));
+    ptr[dev] = static_cast<double*>(p);
+  }
+  *out = ptr[dev];
+  return FX_OK;
+}
+
+int fx_project_mean_noalloc(fx_grid g, const uint8_t* flags, float* rhs, void* stream) {
+  if (int e = check_grid(g)) return e;
+  FX_REQUIRE(rhs, FX_EINVAL, "null array");
+  // the reductions fold any grid into kRedBlocks partials per quantity, so the
+  // fixed workspace covers every grid check_grid accepts
+  static_assert(kWsDoubles == FX_WS_DOUBLES, "scratch holds one reduction workspace");
+  double* ws = nullptr;
+  if (int e = scratch_on_current_device(&ws)) return e;
   return project_mean_level(level0_of(g, flags), rhs, ws, S(stream));
 }
 

@@ -279,6 +279,16 @@ int fx_increment_rhs(fx_grid g, const uint8_t* flags, float* x0, const float* b,
  * ws: FX_WS_DOUBLES doubles of device scratch. */
 int fx_project_mean(fx_grid g, const uint8_t* flags, float* rhs, double* ws, void* stream);
 
+/* fx_project_mean without the ws argument, for callers that cannot allocate
+ * device memory (code under HIP graph capture, serving loops, harnesses that
+ * forbid allocation).  No counterpart in the reference.  It allocates nothing:
+ * its FP64 workspace is a static device array of FX_WS_DOUBLES doubles
+ * (131 KB) per GPU, held until the process ends, which covers every grid the
+ * library accepts (reductions fold into 1024 block partials).  Calls that
+ * overlap on one GPU share that array and race; results equal fx_project_mean
+ * bit for bit. */
+int fx_project_mean_noalloc(fx_grid g, const uint8_t* flags, float* rhs, void* stream);
+
 /* ||b - A x||_2 and ||b||_2 in FP64 for the level-0 operator (flags NULL = box).
  * Synchronises the stream; writes host doubles. ws: FX_WS_DOUBLES doubles. */
 int fx_residual_norm(fx_grid g, const uint8_t* flags, const float* x, const float* b,
