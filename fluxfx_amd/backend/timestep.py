@@ -1,11 +1,31 @@
-"""Adaptive CFL timestep from device max-abs reductions
-(backend/timestep.py:6-57).  One fused reduction launch replaces the
-reference's 4^3-block chains; five floats are read back."""
+"""Adaptive CFL timestep from device max-abs reductions.
+
+Drop-in for AdaptiveTimestep (backend/timestep.py:6-57): ``bounds(solver)``
+returns max |u|, |v|, |w|, |density|, |temperature| (one fused reduction
+launch instead of the reference's 4^3-block chains; five floats read back),
+``select(solver, max_dt, cfl, emission=None, sources=None)`` the report dict
+with the reference's keys.
+
+The step must keep every point's displacement within ``cfl`` cells.  Over a
+step dt a point moves at most ``rate*dt + acc*dt**2`` cells with
+
+* rate = sum over axes of (largest speed along the axis) / (cell size): the
+  field's maxima, raised to the jet speed of any source that couples one in;
+* acc = sum of the accelerations the step can add, in cells per second^2:
+  buoyancy (lift*T + weight*rho) / h_z on the vertical axis, and the force
+  caps of vorticity confinement and turbulence spread over all three axes;
+
+``physics.timestep.choose_timestep`` then picks dt.  The projectors that
+store the impulse take continuous steps under the AUTO policy, the legacy
+Jacobi projector power-of-two steps.
+"""
 import ctypes as C
 from math import isfinite
 
 from .. import _lib
 from ..physics.timestep import choose_timestep
+
+FIELDS = 5  # u, v, w, density, temperature
 
 
 class AdaptiveTimestep:
@@ -20,34 +40,49 @@ class AdaptiveTimestep:
 
     def bounds(self, solver):
         fields = (*solver._velocity, solver.density, solver.temperature)
-        arrays = (C.c_void_p * 5)(*[f.ptr for f in fields])
-        sizes = (C.c_int64 * 5)(*[f.size for f in fields])
-        _lib.call("fx_max_abs", 5, arrays, sizes, self._out.data_ptr(), self.device.stream)
-        maxima = [float(v) for v in self._out[:5].cpu().tolist()]
-        if any(not isfinite(v) or v >= 1e30 for v in maxima):
+        arrays = (C.c_void_p * FIELDS)(*[f.ptr for f in fields])
+        sizes = (C.c_int64 * FIELDS)(*[f.size for f in fields])
+        _lib.call("fx_max_abs", FIELDS, arrays, sizes, self._out.data_ptr(), self.device.stream)
+        maxima = [float(v) for v in self._out[:FIELDS].cpu().tolist()]
+        if not all(isfinite(v) and v < 1e30 for v in maxima):
             raise RuntimeError("Nonfinite or excessive field values; reset simulation")
         return maxima
 
+    @staticmethod
+    def _speeds(maxima, motions):
+        """Largest speed per axis: the field's, or a coupled source's jet."""
+        speeds = list(maxima[:3])
+        for m in motions:
+            if m.coupling > 0:
+                speeds = [max(s, abs(j)) for s, j in zip(speeds, m.velocity)]
+        return speeds
+
+    def _acceleration(self, settings, density, temperature):
+        inv = [1.0 / h for h in self.grid.cell_size]
+        spread = sum(inv)  # a force cap along any direction, in cells/s^2
+        terms = [(settings.thermal_lift * temperature + settings.density_weight * density) * inv[2]]
+        if settings.vorticity_strength > 0:
+            terms.append(settings.vorticity_limit * spread)
+        if settings.turbulence_strength > 0:
+            terms.append(min(settings.turbulence_strength, settings.turbulence_limit) * spread)
+        return sum(terms)
+
     def select(self, solver, max_dt, cfl, emission=None, sources=None):
-        u, v, w, density, temperature = self.bounds(solver)
-        motions = [s.motion for s in sources] if sources is not None else ([emission] if emission is not None else [])
-        for motion in motions:
-            if motion.coupling > 0:
-                u, v, w = (max(cur, abs(t)) for cur, t in zip((u, v, w), motion.velocity))
-        h = self.grid.cell_size
-        rate = sum(speed / spacing for speed, spacing in zip((u, v, w), h))
+        maxima = self.bounds(solver)
+        if sources is not None:
+            motions = [s.motion for s in sources]
+        else:
+            motions = [] if emission is None else [emission]
+        speeds = self._speeds(maxima, motions)
+        rate = sum(v / h for v, h in zip(speeds, self.grid.cell_size))
         s = solver.settings
-        acceleration = (s.thermal_lift * temperature + s.density_weight * density) / h[2]
-        if s.vorticity_strength > 0:
-            acceleration += s.vorticity_limit * sum(1 / sp for sp in h)
-        if s.turbulence_strength > 0:
-            acceleration += min(s.turbulence_strength, s.turbulence_limit) * sum(1 / sp for sp in h)
-        quantized = (s.timestep_policy == "QUANTIZED" or
-                     (s.timestep_policy == "AUTO" and not solver.projector.uses_impulse))
-        dt = choose_timestep(max_dt, cfl, rate, acceleration, quantized=quantized)
-        return {"dt": dt, "max_dt": max_dt, "target": cfl, "rate_bound": rate,
-                "acceleration_rate_bound": acceleration, "estimated_courant": rate * dt + acceleration * dt * dt,
-                "policy": "QUANTIZED" if quantized else "CONTINUOUS", "max_abs_velocity": [u, v, w],
+        acc = self._acceleration(s, maxima[3], maxima[4])
+        policy = s.timestep_policy
+        if policy == "AUTO":
+            policy = "CONTINUOUS" if solver.projector.uses_impulse else "QUANTIZED"
+        dt = choose_timestep(max_dt, cfl, rate, acc, quantized=policy == "QUANTIZED")
+        return {"dt": dt, "max_dt": max_dt, "target": cfl, "rate_bound": rate, "acceleration_rate_bound": acc,
+                "estimated_courant": rate * dt + acc * dt * dt, "policy": policy, "max_abs_velocity": speeds,
                 "mode": "ADAPTIVE"}
 
     def close(self):
