@@ -95,6 +95,39 @@ __device__ __forceinline__ uint8_t box_flags(int x, int y, int z, int nx, int ny
   return f;
 }
 
+// 7-point operator of one FLAGS/BOX cell in pair form: per axis the two
+// neighbour differences t = (p_lo - c) + (p_hi - c) (a closed face adds +0),
+// s = wx*tx + wy*ty + wz*tz and d = wx*nx + wy*ny + wz*nz with n the open
+// faces per axis.  Every smoother, residual and operator kernel of FLAGS/BOX
+// levels (one cell or several per lane) evaluates exactly these operations,
+// so their results agree bit for bit; differences are taken before the sums,
+// so an offset in p does not cancel catastrophically in fp32.
+__device__ __forceinline__ void flag_cell(uint32_t f, float c, float l, float r, float ym, float yp, float zm,
+                                          float zp, float wx, float wy, float wz, float& s, float& d) {
+  const float tx = ((f & FLAG_XLO) ? l - c : 0.0f) + ((f & FLAG_XHI) ? r - c : 0.0f);
+  const float ty = ((f & FLAG_YLO) ? ym - c : 0.0f) + ((f & FLAG_YHI) ? yp - c : 0.0f);
+  const float tz = ((f & FLAG_ZLO) ? zm - c : 0.0f) + ((f & FLAG_ZHI) ? zp - c : 0.0f);
+  const float nx = float(int((f & FLAG_XLO) != 0) + int((f & FLAG_XHI) != 0));
+  const float ny = float(int((f & FLAG_YLO) != 0) + int((f & FLAG_YHI) != 0));
+  const float nz = float(int((f & FLAG_ZLO) != 0) + int((f & FLAG_ZHI) != 0));
+  s = __builtin_fmaf(wz, tz, __builtin_fmaf(wy, ty, wx * tx));
+  d = __builtin_fmaf(wz, nz, __builtin_fmaf(wy, ny, wx * nx));
+}
+
+// flag_cell of a cell with all six faces open (the same operations with every
+// select taken and n = 2): s only; d is open_diagonal() for every such cell
+__device__ __forceinline__ float open_cell(float c, float l, float r, float ym, float yp, float zm, float zp,
+                                           float wx, float wy, float wz) {
+  const float tx = (l - c) + (r - c), ty = (ym - c) + (yp - c), tz = (zm - c) + (zp - c);
+  return __builtin_fmaf(wz, tz, __builtin_fmaf(wy, ty, wx * tx));
+}
+
+__device__ __forceinline__ float open_diagonal(float wx, float wy, float wz) {
+  return __builtin_fmaf(wz, 2.0f, __builtin_fmaf(wy, 2.0f, wx * 2.0f));
+}
+
+constexpr uint32_t kAllOpen = FLAG_XLO | FLAG_XHI | FLAG_YLO | FLAG_YHI | FLAG_ZLO | FLAG_ZHI | FLAG_FLUID;
+
 // FP64 block reduction helpers (wavefront = 64 lanes)
 __device__ __forceinline__ double wave_sum(double v) {
   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);

@@ -77,15 +77,16 @@ __device__ __forceinline__ Stencil stencil(const LevelDev& L, int x, int y, int 
   } else {
     uint8_t f = OP == FX_OP_FLAGS ? reinterpret_cast<const uint8_t*>(L.op)[i] : box_flags(x, y, z, L.nx, L.ny, L.nz);
     st.active = f & FLAG_FLUID;
-    float s = 0.0f, d = 0.0f;
-    if (f & FLAG_XLO) { d += L.wx; if (read_p) s += L.wx * (to_f(p[i - 1]) - c0); }
-    if (f & FLAG_XHI) { d += L.wx; if (read_p) s += L.wx * (to_f(p[i + 1]) - c0); }
-    if (f & FLAG_YLO) { d += L.wy; if (read_p) s += L.wy * (to_f(p[i - sy]) - c0); }
-    if (f & FLAG_YHI) { d += L.wy; if (read_p) s += L.wy * (to_f(p[i + sy]) - c0); }
-    if (f & FLAG_ZLO) { d += L.wz; if (read_p) s += L.wz * (to_f(p[i - sz]) - c0); }
-    if (f & FLAG_ZHI) { d += L.wz; if (read_p) s += L.wz * (to_f(p[i + sz]) - c0); }
-    st.s = s;
-    st.d = d;
+    float n[6] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+    if (read_p) {
+      if (f & FLAG_XLO) n[0] = to_f(p[i - 1]);
+      if (f & FLAG_XHI) n[1] = to_f(p[i + 1]);
+      if (f & FLAG_YLO) n[2] = to_f(p[i - sy]);
+      if (f & FLAG_YHI) n[3] = to_f(p[i + sy]);
+      if (f & FLAG_ZLO) n[4] = to_f(p[i - sz]);
+      if (f & FLAG_ZHI) n[5] = to_f(p[i + sz]);
+    }
+    flag_cell(f, c0, n[0], n[1], n[2], n[3], n[4], n[5], L.wx, L.wy, L.wz, st.s, st.d);
   }
   return st;
 }

@@ -62,13 +62,15 @@ __global__ __launch_bounds__(256) void k_smooth_zm(LevelDev L, const T* __restri
   for (int z = z0; z < z1; ++z, i += sz) {
     float pn = (!zero_guess && z + 1 < L.nz) ? to_f(p[i + sz]) : 0.0f;
     uint8_t f = OP == FX_OP_FLAGS ? reinterpret_cast<const uint8_t*>(L.op)[i] : box_flags(x, y, z, L.nx, L.ny, L.nz);
-    float sdiff = 0.0f, d = 0.0f;
-    if (f & FLAG_XLO) { d += L.wx; if (!zero_guess) sdiff += L.wx * (to_f(p[i - 1]) - pc); }
-    if (f & FLAG_XHI) { d += L.wx; if (!zero_guess) sdiff += L.wx * (to_f(p[i + 1]) - pc); }
-    if (f & FLAG_YLO) { d += L.wy; if (!zero_guess) sdiff += L.wy * (to_f(p[i - sy]) - pc); }
-    if (f & FLAG_YHI) { d += L.wy; if (!zero_guess) sdiff += L.wy * (to_f(p[i + sy]) - pc); }
-    if (f & FLAG_ZLO) { d += L.wz; sdiff += L.wz * (pm - pc); }
-    if (f & FLAG_ZHI) { d += L.wz; sdiff += L.wz * (pn - pc); }
+    float n[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+    if (!zero_guess) {
+      if (f & FLAG_XLO) n[0] = to_f(p[i - 1]);
+      if (f & FLAG_XHI) n[1] = to_f(p[i + 1]);
+      if (f & FLAG_YLO) n[2] = to_f(p[i - sy]);
+      if (f & FLAG_YHI) n[3] = to_f(p[i + sy]);
+    }
+    float sdiff, d;
+    flag_cell(f, pc, n[0], n[1], n[2], n[3], pm, pn, L.wx, L.wy, L.wz, sdiff, d);
     float r = 0.0f;
     if ((f & FLAG_FLUID) && d > 0.0f) r = jacobi_update(pc, sdiff, to_f(rhs[i]), d, inv_dt, omega);
     out[i] = from_f<T>(r);

@@ -238,15 +238,7 @@ __device__ __forceinline__ QuadStencil quad_stencil(const LevelDev& L, int x0, i
     const float ck = c.v[k];
     const float left = k > 0 ? c.v[k > 0 ? k - 1 : 0] : xl;
     const float right = k < 3 ? c.v[k < 3 ? k + 1 : 3] : xr;
-    float s = 0.0f, d = 0.0f;
-    if (f & FLAG_XLO) { d += L.wx; s += L.wx * (left - ck); }
-    if (f & FLAG_XHI) { d += L.wx; s += L.wx * (right - ck); }
-    if (f & FLAG_YLO) { d += L.wy; s += L.wy * (ym.v[k] - ck); }
-    if (f & FLAG_YHI) { d += L.wy; s += L.wy * (yp.v[k] - ck); }
-    if (f & FLAG_ZLO) { d += L.wz; s += L.wz * (zm.v[k] - ck); }
-    if (f & FLAG_ZHI) { d += L.wz; s += L.wz * (zp.v[k] - ck); }
-    st.s[k] = s;
-    st.d[k] = d;
+    flag_cell(f, ck, left, right, ym.v[k], yp.v[k], zm.v[k], zp.v[k], L.wx, L.wy, L.wz, st.s[k], st.d[k]);
     st.c[k] = ck;
     st.f[k] = f;
   }
@@ -410,21 +402,40 @@ __device__ __forceinline__ VecStencil<W> vec_stencil(const LevelDev& L, int x0, 
   FV<W> c{}, ym{}, yp{}, zm{}, zp{};
   float xl = 0.0f, xr = 0.0f;
   if (read_p) vec_neighbours<W>(L, x0, y, z, i, ld, c, ym, yp, zm, zp, xl, xr);
+  // Fast path, taken when every lane of the wave has all faces of its cells
+  // open (the x walls of the lane's two end cells aside): the selects and
+  // face counts of flag_cell are known, so only the differences remain.  It
+  // evaluates the same operations as flag_cell, bit for bit.
+  bool lane_open = true;
 #pragma unroll
   for (int k = 0; k < W; ++k) {
-    const uint32_t f = st.f[k];
+    const uint32_t need = kAllOpen & ~(k == 0 ? uint32_t(FLAG_XLO) : 0u) & ~(k == W - 1 ? uint32_t(FLAG_XHI) : 0u);
+    lane_open = lane_open && (st.f[k] & need) == need;
+  }
+  if (__all(lane_open)) {
+    const float d_open = open_diagonal(L.wx, L.wy, L.wz);
+#pragma unroll
+    for (int k = 0; k < W; ++k) {
+      const float ck = c.v[k];
+      const float left = k > 0 ? c.v[k > 0 ? k - 1 : 0] : xl;
+      const float right = k < W - 1 ? c.v[k < W - 1 ? k + 1 : W - 1] : xr;
+      if (k == 0 || k == W - 1) {
+        flag_cell(st.f[k], ck, left, right, ym.v[k], yp.v[k], zm.v[k], zp.v[k], L.wx, L.wy, L.wz, st.s[k],
+                  st.d[k]);
+      } else {
+        st.s[k] = open_cell(ck, left, right, ym.v[k], yp.v[k], zm.v[k], zp.v[k], L.wx, L.wy, L.wz);
+        st.d[k] = d_open;
+      }
+      st.c[k] = ck;
+    }
+    return st;
+  }
+#pragma unroll
+  for (int k = 0; k < W; ++k) {
     const float ck = c.v[k];
     const float left = k > 0 ? c.v[k > 0 ? k - 1 : 0] : xl;
     const float right = k < W - 1 ? c.v[k < W - 1 ? k + 1 : W - 1] : xr;
-    float s = 0.0f, d = 0.0f;
-    if (f & FLAG_XLO) { d += L.wx; s += L.wx * (left - ck); }
-    if (f & FLAG_XHI) { d += L.wx; s += L.wx * (right - ck); }
-    if (f & FLAG_YLO) { d += L.wy; s += L.wy * (ym.v[k] - ck); }
-    if (f & FLAG_YHI) { d += L.wy; s += L.wy * (yp.v[k] - ck); }
-    if (f & FLAG_ZLO) { d += L.wz; s += L.wz * (zm.v[k] - ck); }
-    if (f & FLAG_ZHI) { d += L.wz; s += L.wz * (zp.v[k] - ck); }
-    st.s[k] = s;
-    st.d[k] = d;
+    flag_cell(st.f[k], ck, left, right, ym.v[k], yp.v[k], zm.v[k], zp.v[k], L.wx, L.wy, L.wz, st.s[k], st.d[k]);
     st.c[k] = ck;
   }
   return st;

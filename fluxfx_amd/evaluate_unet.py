@@ -1,0 +1,260 @@
+"""Evaluate the learned preconditioner against the same solver without it.
+
+    python -m fluxfx_amd.evaluate_unet --n 128 --seeds 20000:20256 --out table.md
+    python -m fluxfx_amd.evaluate_unet --n 256 --seeds 20000:20016 --in-situ --out table.md
+
+Held-out scenes (seeds from 20000; training used seeds from 10000) are the
+random scenes of ``training.random_scene``: 1-4 emitters, 0-8 sphere/box
+colliders, temperature rate U[0.5, 1.5].
+
+* Cold solves (default): each scene is stepped twice with plain FP32
+  multigrid, and the third step's pre-projection divergence b is solved from
+  zero to 1e-5 relative residual (60 iterations at most) by BF16 V-cycle
+  preconditioned FPCG (no network), and by FPCG preconditioned by the network
+  (on the pressure grid, followed by two damped Jacobi sweeps) in the first
+  iteration or in every iteration, the V-cycle taking over after the
+  network's iterations; each network method also runs with a network whose
+  output is zero (only the sweeps precondition): the same solver without the
+  network.  The residual after one preconditioned step with and without the
+  network is reported too (equal work).
+* In situ (--in-situ, 256^3): each scene runs warm-up steps and then timed
+  steps of the full smoke step in each mode (mixed; the network once on
+  multigrid level 1 then V-cycle FPCG; the network on level 1 in every FPCG
+  iteration), recording V-cycle/FPCG iterations and ms per step.
+
+Prints a markdown table (means over scenes) and, with --out, writes it.
+"""
+import argparse
+import dataclasses
+import time
+
+import numpy as np
+
+WEIGHTS_L0 = "unet_128.npz"
+WEIGHTS_L1 = "unet_128_inc.npz"
+
+
+def _weights(name):
+    from pathlib import Path
+    return str(Path(__file__).resolve().parent / "weights" / name)
+
+
+def _settings(**kw):
+    from .physics.pressure import PressureSettings
+    base = dict(thermal_lift=1.0, density_weight=0.0, dissipation=0.0, cooling=0.0,
+                velocity_advection="MACCORMACK", scalar_advection="MACCORMACK", pressure_tolerance=1e-5,
+                pressure_max_cycles=60, mg_pre=2, mg_post=2, mg_coarsest=8)
+    base.update(kw)
+    return PressureSettings(**base)
+
+
+def zero_network(device):
+    """A U-Net whose output is zero for every input (all parameters 0)."""
+    from .learned import UNetPreconditioner
+    net = UNetPreconditioner(device, seed=0)
+    net.params.zero_()
+    net.pack()
+    return net
+
+
+def cold_solve(device, sample, settings, network=None):
+    """Solve L q = b from zero for one harvested sample; returns iterations,
+    convergence, relative residual, fallback and the solve's wall time (ms,
+    from the call until its last residual read)."""
+    from .backend.collision import SolidMask
+    from .backend.tolerance import ToleranceProjector
+    from .physics.config import GridSpec
+    b, _, solid, _ = sample
+    n = round(b.numel() ** (1 / 3))
+    grid = GridSpec((n, n, n), (1.0, 1.0, 1.0), max_dim=1024)
+    solids = None if solid is None else SolidMask(grid, device=device, mask=solid.cpu().numpy())
+    pp = ToleranceProjector(grid, settings, device, solids=solids)
+    if network is not None:
+        pp.attach_network(network)
+    pp.before.tensor.copy_(b)
+    pp.pressure.clear(format="FLOAT", value=(0.0,))
+    device.torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    try:
+        pp.solve_pressure(1.0, 1.0)
+    except RuntimeError:
+        pass  # budget exhausted: the metrics still hold the true residual
+    device.torch.cuda.synchronize()
+    ms = 1000.0 * (time.perf_counter() - t0)
+    m = pp.solve_metrics()
+    pp.close()
+    return {"iterations": m["v_cycles"], "converged": m["converged"], "residual": m["relative_residual"],
+            "fallback": m["fallback"], "ms": ms}
+
+
+def cold_methods(net, zero):
+    """name -> (settings, network) of the cold-solve comparison."""
+    learned = dict(pressure_precision="BF16", pressure_krylov="LEARNED", learned_sweeps=2, learned_level=0)
+    return {
+        "mixed (no network)": (_settings(pressure_precision="BF16", pressure_krylov="PCG"), None),
+        "network once, then V-cycle FPCG": (_settings(learned_iterations=1, **learned), net),
+        "zero network once, then V-cycle FPCG": (_settings(learned_iterations=1, **learned), zero),
+        "network every iteration": (_settings(learned_iterations=60, **learned), net),
+        "zero network every iteration": (_settings(learned_iterations=60, **learned), zero),
+    }
+
+
+def first_step(device, sample, net, zero):
+    """After one preconditioned FPCG step with the network and with the zero
+    network (equal work: one application, two sweeps): the relative residual,
+    and the network's own prediction error (the fluid-masked mean-squared
+    error of its output against the converged solution, what it was trained
+    to minimise; the zero network's is the mean square of the solution)."""
+    from . import training
+    learned = dict(pressure_precision="BF16", pressure_krylov="LEARNED", learned_sweeps=2, learned_level=0,
+                   learned_iterations=1, pressure_max_cycles=1)
+    n = round(sample[0].numel() ** (1 / 3))
+    out = {}
+    for name, network in (("network", net), ("zero network", zero)):
+        out[name] = cold_solve(device, sample, _settings(**learned), network)["residual"]
+        out[name + " MSE"] = training.evaluate(network, [sample], (n, n, n))
+    return out
+
+
+def evaluate_cold(device, n, seeds, weights=None, log=None):
+    """Per-seed cold-solve results of every method (dict of lists), plus
+    the first-step residuals under ``"first step"``."""
+    from . import training
+    from .learned import UNetPreconditioner
+    net = UNetPreconditioner(device)
+    net.load(weights or _weights(WEIGHTS_L0))
+    zero = zero_network(device)
+    methods = cold_methods(net, zero)
+    out = {k: [] for k in methods}
+    first = []
+    skipped = []
+    for k, seed in enumerate(seeds):
+        try:
+            sample = training.harvest(device, n, 1, 1, seed0=seed)[0]
+        except RuntimeError:
+            # the plain FP32 V-cycle that makes the sample stalls above 1e-5 on
+            # a few scenes; they are left out and counted
+            skipped.append(seed)
+            continue
+        for name, (s, network) in methods.items():
+            out[name].append(cold_solve(device, sample, s, network))
+        first.append(first_step(device, sample, net, zero))
+        if log and (k + 1) % max(1, len(seeds) // 8) == 0:
+            log(f"cold solves: {k + 1}/{len(seeds)} scenes")
+    out["first step"] = first
+    out["skipped"] = skipped
+    return out
+
+
+def evaluate_in_situ(device, n, seeds, warmup=4, steps=6, log=None):
+    """Per-seed mean iterations and ms per step of the full step in each mode."""
+    from . import training
+    from .backend.projected import DenseProjectedSmoke
+    from .physics.config import GridSpec
+    from .learned import UNetPreconditioner
+    torch = device.torch
+    net = UNetPreconditioner(device)
+    net.load(_weights(WEIGHTS_L1))
+    modes = {
+        "mixed": _settings(pressure_precision="BF16", pressure_krylov="PCG"),
+        "learned once (level 1)": _settings(pressure_precision="BF16", pressure_krylov="LEARNED",
+                                            learned_iterations=1, learned_sweeps=0, learned_level=1),
+        "learned every iteration (level 1)": _settings(pressure_precision="BF16", pressure_krylov="LEARNED",
+                                                       learned_iterations=60, learned_sweeps=2,
+                                                       learned_level=1),
+    }
+    grid = GridSpec((n, n, n), (1.0, 1.0, 1.0), max_dim=1024)
+    dt = 0.5 / n
+    out = {k: [] for k in modes}
+    for k, seed in enumerate(seeds):
+        sources, solid = training.random_scene(n, seed)
+        for name, s in modes.items():
+            sim = DenseProjectedSmoke(grid, s, device,
+                                      solid_mask=None if solid is None else solid.ravel().astype(np.uint8))
+            if s.pressure_krylov == "LEARNED":
+                sim.projector.attach_network(net)
+            # warm-up at 1e-4 (an FP32 cold start at 256^3 stalls near 2e-5)
+            sim.projector.settings = dataclasses.replace(s, pressure_tolerance=1e-4)
+            ok = True
+            try:
+                for _ in range(warmup):
+                    sim.step(dt, sources=sources)
+            except RuntimeError:
+                ok = False
+            sim.projector.settings = s
+            its, ms = [], []
+            for _ in range(steps if ok else 0):
+                torch.cuda.synchronize()
+                t0 = time.perf_counter()
+                try:
+                    sim.step(dt, sources=sources)
+                except RuntimeError:
+                    ok = False
+                    break
+                torch.cuda.synchronize()
+                ms.append(1000.0 * (time.perf_counter() - t0))
+                its.append(sim.projector.solve_metrics()["v_cycles"])
+            sim.close()
+            out[name].append({"iterations": float(np.mean(its)) if its else float("nan"),
+                              "ms": float(np.mean(ms)) if ms else float("nan"), "converged": ok})
+        if log:
+            log(f"in situ: {k + 1}/{len(seeds)} scenes")
+    return out
+
+
+def table(results, title):
+    lines = [f"### {title}", "", "| method | scenes | converged | mean iterations | mean ms | "
+             "mean relative residual |", "|---|---|---|---|---|---|"]
+    for name, rows in results.items():
+        if name in ("first step", "skipped"):
+            continue
+        conv = sum(bool(r["converged"]) for r in rows)
+        its = np.nanmean([r["iterations"] for r in rows])
+        ms = np.nanmean([r["ms"] for r in rows])
+        res = np.nanmean([r.get("residual", float("nan")) for r in rows]) if "residual" in rows[0] else float("nan")
+        lines.append(f"| {name} | {len(rows)} | {conv} | {its:.2f} | {ms:.2f} | {res:.2e} |")
+    if "first step" in results:
+        first = results["first step"]
+        lines += ["", "After one preconditioned FPCG step (geometric means over scenes): relative residual "
+                  + ", ".join(f"{k} {np.exp(np.mean([np.log(r[k]) for r in first])):.3e}" for k in first[0]
+                              if "MSE" not in k)
+                  + "; the network's masked MSE against the converged solution "
+                  + ", ".join(f"{k[:-4]} {np.exp(np.mean([np.log(r[k]) for r in first])):.3e}" for k in first[0]
+                              if "MSE" in k) + "."]
+    if results.get("skipped"):
+        lines += ["", f"Scenes left out (the plain FP32 V-cycle making their sample stalled above 1e-5): "
+                  f"{len(results['skipped'])} ({', '.join(map(str, results['skipped']))})."]
+    return "\n".join(lines) + "\n"
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--n", type=int, default=128)
+    ap.add_argument("--seeds", default="20000:20256")
+    ap.add_argument("--in-situ", action="store_true")
+    ap.add_argument("--out", default=None)
+    args = ap.parse_args(argv)
+    from .backend.device import HipDevice
+    lo, hi = (int(v) for v in args.seeds.split(":"))
+    seeds = list(range(lo, hi))
+    device = HipDevice(0)
+
+    def log(msg):
+        print(msg, flush=True)
+
+    t0 = time.time()
+    if args.in_situ:
+        res = evaluate_in_situ(device, args.n, seeds, log=log)
+        title = f"In-situ smoke steps at {args.n}^3, held-out seeds {lo}-{hi - 1}, 1e-5 relative residual"
+    else:
+        res = evaluate_cold(device, args.n, seeds, log=log)
+        title = f"Cold solves at {args.n}^3, held-out seeds {lo}-{hi - 1}, to 1e-5 relative residual"
+    text = table(res, title) + f"\n({time.time() - t0:.0f} s)\n"
+    print(text, flush=True)
+    if args.out:
+        with open(args.out, "w") as f:
+            f.write(text)
+
+
+if __name__ == "__main__":
+    main()

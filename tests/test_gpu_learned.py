@@ -98,3 +98,24 @@ def test_coarse_level_network_beats_untrained_and_reaches_tolerance(device, trai
         assert not mt["fallback"] and mt["relative_residual"] < 0.5 * mf["relative_residual"], (mt, mf)
     m = _learned_solve(device, net, held_out[0], 1, 60, level=1)
     assert m["converged"] and m["relative_residual"] <= 1e-5 and m["fallback"], m
+
+
+def test_committed_network_against_the_solver_without_it(device):
+    """The committed 128^3 network (fluxfx_amd/weights/unet_128.npz) on two
+    held-out scenes at 128^3 (seeds 20000+; training used 10000+), cold
+    solves to 1e-5, against the same solver with the network's output zeroed
+    and against BF16 V-cycle FPCG without a network.  What the network gains:
+    its prediction of the solution has a small fraction of the zero
+    prediction's error (the quantity it was trained on), and used in the
+    first FPCG step it converges with the V-cycle after it.  What it does not
+    gain (printed, the measured result rather than a bound): that first step
+    saves no V-cycle iterations, and as the preconditioner of every iteration
+    it stalls, since later residuals are rough fields it was not trained on."""
+    from fluxfx_amd import evaluate_unet as ev
+    res = ev.evaluate_cold(device, 128, [20000, 20001])
+    print(ev.table(res, "held-out cold solves at 128^3"), flush=True)
+    for first in res["first step"]:
+        assert first["network MSE"] < 0.1 * first["zero network MSE"], first
+    for m in res["network once, then V-cycle FPCG"]:
+        assert m["converged"] and m["residual"] <= 1e-5, m
+    assert all(m["converged"] for m in res["mixed (no network)"])
