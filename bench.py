@@ -20,8 +20,12 @@ restricted to 128^3),
 whichever are built.  With the CPU leg the line also carries every field's
 relative error against the FP64 oracle on the same scene at 64^3 after 7 steps
 (``fp64_parity_64``).  ``value`` is the steps/s of the learned+mixed mode (of
-the mixed mode when no trained weights are present); every mode's rate and
-its ratio to plain are in the JSON line.
+the mixed mode when no trained weights are present): 1000 / the median of
+the K steps' times, each step bracketed by HIP events on the stream its
+kernels run on; every mode's rate (and its mean, minimum and wall-clock
+figures) and its ratio to plain are in the JSON line.  ``roofline`` is the
+level-0 smoother of the BF16 V-cycle (the kernel with the largest share of
+the learned+mixed step), timed with HIP events over 60 back-to-back sweeps.
 
 ``--workload bricks`` times the sparse brick pool at 256^3 instead (support
 scan, gather and scatter of the reference's 112^3-box occupancy case).
@@ -150,47 +154,54 @@ def make_sim(device, n, mode, workload="smoke"):
 
 
 def timed_run(device, sim, dt, steps, warmup):
+    """W untimed steps, then K steps each bracketed by HIP events on the stream
+    the step's kernels run on; returns the per-step event times (ms), the wall
+    time of the K steps (s), the solver iterations per step and the last
+    solve's metrics."""
     torch = device.torch
     for _ in range(warmup):
         sim.step(dt)
     torch.cuda.synchronize()
+    stream = torch.cuda.current_stream()
+    marks = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(steps)]
     cycles = []
     t0 = time.perf_counter()
-    for _ in range(steps):
+    for e0, e1 in marks:
+        e0.record(stream)
         sim.step(dt)
+        e1.record(stream)
         cycles.append(sim.projector.last_cycles)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     m = sim.projector.solve_metrics()
-    return wall, cycles, m
+    return [e0.elapsed_time(e1) for e0, e1 in marks], wall, cycles, m
 
 
-def smoother_roofline(device, sim, reps=40):
-    """Average duration of the level-0 FP32 smoothing kernel (the hottest kernel
-    of the plain solve), timed with HIP events on the stream it runs on."""
+def smoother_roofline(device, sim, reps=60):
+    """Average duration of the level-0 smoothing kernel of the 16-bit V-cycle
+    that preconditions FPCG (the kernel with the largest share of the
+    learned+mixed step), timed with HIP events on the stream it runs on.
+    Algorithmic bytes per sweep: p (2 B) + rhs (2 B) + face flags (1 B) read,
+    the new p (2 B) written, 7 B per cell."""
     torch = device.torch
-    pp = sim.projector
-    eng = pp.engine
-    eng.bind_level0(pp.pressure.tensor, pp.spare.tensor, pp.before.tensor)
+    eng = sim.projector._precond_engine()
     n = sim.grid.cells
     eng.smooth(0, 4)
+    stream = torch.cuda.current_stream()
     start, end = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     torch.cuda.synchronize()
-    start.record()
+    start.record(stream)
     eng.smooth(0, reps)
-    end.record()
+    end.record(stream)
     torch.cuda.synchronize()
-    if eng.tensor(0, "p") is not pp.pressure.tensor:
-        pp.pressure, pp.spare = pp.spare, pp.pressure
     ms = start.elapsed_time(end) / reps
-    # algorithmic bytes per sweep: read p (4) + rhs (4) + flags (1), write p (4)
-    bytes_per = 13 * n
+    bytes_per = 7 * n
     gbs = bytes_per / (ms * 1e-3) / 1e9
-    return {"kernel": "k_smooth_q<float, FX_OP_FLAGS, 4> (level-0 V-cycle smoother)", "avg_ms": ms,
-            "bytes_per_launch": bytes_per, "achieved_gbs": gbs}
+    return {"kernel": "k_smooth_q<bf16, FX_OP_FLAGS, 8> (level-0 smoother of the BF16 V-cycle preconditioner)",
+            "avg_ms": ms, "bytes_per_launch": bytes_per, "achieved_gbs": gbs, "launches": reps}
 
 
-def measured_traffic(kernel_key="k_smooth_q<float, 1, 4>", summary="step_256_hbm_traffic.md"):
+def measured_traffic(kernel_key="k_smooth_q<fx::bf16, 1, 8>", summary="step_256_hbm_traffic.md"):
     """HBM bytes per launch of the roofline kernel from the committed rocprofv3
     FETCH_SIZE/WRITE_SIZE summary (None when absent or for another kernel)."""
     path = ROOT / "profiles" / summary
@@ -213,6 +224,26 @@ def copy_calibration(device, sim, reps=5):
         _lib.call("fx_scale", pp.pressure.ptr, pp.spare.ptr, n, 1.0, device.stream)
     device.torch.cuda.synchronize()
     return {"kernel": "k_scale", "read_bytes": 4 * n, "write_bytes": 4 * n}
+
+
+def host_cpu():
+    """The host's logical CPU count, the CPUs this process may run on, and the
+    CPU model as lscpu names it (/proc/cpuinfo when lscpu is absent)."""
+    import subprocess
+    model = None
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        model = next((ln.split(":", 1)[1].strip() for ln in out.splitlines() if ln.startswith("Model name")), None)
+    except (OSError, subprocess.SubprocessError):
+        pass
+    if model is None:
+        try:
+            with open("/proc/cpuinfo") as f:
+                model = next((ln.split(":", 1)[1].strip() for ln in f if ln.startswith("model name")), None)
+        except OSError:
+            pass
+    usable = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    return {"host_cpu_count": os.cpu_count(), "host_cpus_usable": usable, "host_cpu_model": model}
 
 
 def cpu_baseline(n_sample=64, steps=6):
@@ -238,7 +269,7 @@ def cpu_baseline(n_sample=64, steps=6):
     return {"value": cells_per_s / 256 ** 3, "unit": "steps/s (256^3-equivalent)", "cores": 1, "kind": "port",
             "sample": f"same scene at {n_sample}^3, {steps} steps after 1 warm-up, FP64 NumPy oracle; "
                       f"{cells_per_s:.3e} cell-updates/s scaled to 256^3",
-            "cell_updates_per_s": cells_per_s}, state
+            "cell_updates_per_s": cells_per_s, "wall_s": wall, **host_cpu()}, state
 
 
 def parity_vs_oracle(device, state, n_sample, steps, modes):
@@ -366,14 +397,14 @@ def bricks_bench(device, n, steps, warmup, cpu):
         assert int(f.sum()) == nb and bricks.shape[0] == nb
         line["cpu_baseline"] = {"value": 1e3 / cpu_ms, "unit": "cycles/s", "cores": 1, "kind": "port",
                                 "sample": f"oracle support scan + gather of the same {n}^3 field, {reps} cycles "
-                                          "(NumPy, one core; the scatter is not timed on the CPU)"}
+                                          "(NumPy, one core; the scatter is not timed on the CPU)", **host_cpu()}
     print(json.dumps(line), flush=True)
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--steps", type=int, default=60)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--n", type=int, default=None, help="grid side (default 256 smoke, 384 fire)")
     ap.add_argument("--modes", default="plain,mixed,learned")
@@ -402,16 +433,21 @@ def main():
     for mode in [m for m in args.modes.split(",") if m]:
         try:
             sim, dt = make_sim(device, n, mode, args.workload)
-            wall, cycles, m = timed_run(device, sim, dt, args.steps, args.warmup)
+            step_ms, wall, cycles, m = timed_run(device, sim, dt, args.steps, args.warmup)
         except NotImplementedError as exc:
             results[mode] = {"status": f"not built: {exc}"}
             continue
-        results[mode] = {"steps_per_s": args.steps / wall, "ms_per_step": 1e3 * wall / args.steps,
+        med = float(np.median(step_ms))
+        results[mode] = {"steps_per_s": 1e3 / med, "ms_per_step": med, "ms_per_step_mean": float(np.mean(step_ms)),
+                         "ms_per_step_min": float(np.min(step_ms)), "ms_per_step_max": float(np.max(step_ms)),
+                         "wall_steps_per_s": args.steps / wall,
                          "mean_iterations": float(np.mean(cycles)), "relative_residual": m.get("relative_residual"),
                          "converged": m.get("converged"), "fallback": m.get("fallback")}
-        print(f"# {mode}: {args.steps / wall:.2f} steps/s, {1e3 * wall / args.steps:.3f} ms/step, "
-              f"cycles {np.mean(cycles):.2f}, residual {m.get('relative_residual')}", file=sys.stderr, flush=True)
-        if mode == "plain":
+        print(f"# {mode}: {1e3 / med:.2f} steps/s (median of {args.steps} event-timed steps {med:.3f} ms, wall "
+              f"{1e3 * wall / args.steps:.3f} ms/step), iterations {np.mean(cycles):.2f}, residual "
+              f"{m.get('relative_residual')}", file=sys.stderr, flush=True)
+        if mode in ("mixed", "learned"):
+            # the 16-bit level-0 smoother; the learned mode's figure replaces the mixed one
             roof = smoother_roofline(device, sim)
             copy_calibration(device, sim)
         del sim
