@@ -126,6 +126,20 @@ __device__ __forceinline__ float open_diagonal(float wx, float wy, float wz) {
   return __builtin_fmaf(wz, 2.0f, __builtin_fmaf(wy, 2.0f, wx * 2.0f));
 }
 
+// flag_cell of a cell whose four y and z faces are open: s with only the x
+// faces tested (the same operations, so the same bits); its diagonal is
+// x_diagonal(number of open x faces)
+__device__ __forceinline__ float xend_cell(uint32_t f, float c, float l, float r, float ym, float yp, float zm,
+                                           float zp, float wx, float wy, float wz) {
+  const float tx = ((f & FLAG_XLO) ? l - c : 0.0f) + ((f & FLAG_XHI) ? r - c : 0.0f);
+  const float ty = (ym - c) + (yp - c), tz = (zm - c) + (zp - c);
+  return __builtin_fmaf(wz, tz, __builtin_fmaf(wy, ty, wx * tx));
+}
+
+__device__ __forceinline__ float x_diagonal(int open_x, float wx, float wy, float wz) {
+  return __builtin_fmaf(wz, 2.0f, __builtin_fmaf(wy, 2.0f, wx * float(open_x)));
+}
+
 constexpr uint32_t kAllOpen = FLAG_XLO | FLAG_XHI | FLAG_YLO | FLAG_YHI | FLAG_ZLO | FLAG_ZHI | FLAG_FLUID;
 
 // FP64 block reduction helpers (wavefront = 64 lanes)

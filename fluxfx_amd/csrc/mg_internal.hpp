@@ -4,14 +4,48 @@
 
 namespace fx {
 
+// Division of 0 <= n < 2^31 by an invariant d >= 1 as a multiply-high, an add
+// and a shift (Granlund and Montgomery, "Division by invariant integers using
+// multiplication", PLDI 1994, section 4): with l = ceil(log2 d) and
+// m = floor(2^32 (2^l - d) / d) + 1, n / d = (mulhi(n, m) + n) >> l.  The sum
+// cannot wrap since mulhi(n, m) < n < 2^31.  A power of two has m = 1.
+struct FastDiv {
+  uint32_t m;
+  uint32_t l;
+};
+
+inline FastDiv make_fastdiv(int d) {
+  const uint64_t dd = d > 1 ? uint64_t(d) : 1;
+  uint32_t l = 0;
+  while ((uint64_t(1) << l) < dd) ++l;
+  const uint64_t m = (((uint64_t(1) << l) - dd) << 32) / dd + 1;
+  return FastDiv{uint32_t(m), l};
+}
+
+__device__ __forceinline__ int fdiv(int n, FastDiv f) {
+  return int((__umulhi(uint32_t(n), f.m) + uint32_t(n)) >> f.l);
+}
+
 struct LevelDev {
   int nx, ny, nz;
   float wx, wy, wz;
   const void* op;
+  // divisors of the vector kernels' lane -> (row, plane) maps: nx/4 and nx/8
+  // lanes per row, ny rows per plane, ny/2 coarse rows per plane (1 where the
+  // level's extents do not divide)
+  FastDiv row4, row8, ny1, ny2;
 };
 
 static LevelDev dev_level(const fx_mg_level& L) {
-  return LevelDev{L.nx, L.ny, L.nz, L.w[0], L.w[1], L.w[2], L.op_data};
+  return LevelDev{L.nx, L.ny, L.nz, L.w[0], L.w[1], L.w[2], L.op_data,
+                  make_fastdiv(L.nx / 4), make_fastdiv(L.nx / 8), make_fastdiv(L.ny), make_fastdiv(L.ny / 2)};
+}
+
+// lanes-per-row divisor of a W-cell-per-lane kernel
+template <int W>
+__device__ __forceinline__ FastDiv row_div(const LevelDev& L) {
+  static_assert(W == 4 || W == 8, "vector kernels hold 4 or 8 cells per lane");
+  return W == 4 ? L.row4 : L.row8;
 }
 
 struct Stencil {

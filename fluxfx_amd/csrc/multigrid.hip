@@ -79,27 +79,87 @@ __global__ __launch_bounds__(256) void k_smooth_zm(LevelDev L, const T* __restri
   }
 }
 
-// Vector smoother (level 0 and other large box/flags levels): one lane per W
-// consecutive x cells (one 16-byte load per operand: W = 4 for fp32, 8 for
-// 16-bit storage), cells enumerated x-fastest over the level; block ids are
-// spread so each XCD sweeps a contiguous range of rows.
-template <typename T, int OP, int W>
+// One damped-Jacobi sweep of the W cells of a vector lane.  ZG: p is known to
+// be 0 (not read).  On flag/box levels a wave whose lanes have every face of
+// their cells open (the x walls of a lane's two end cells aside) takes a path
+// without the per-cell selects: every cell is fluid, the inner cells share
+// the diagonal of an open cell and the end cells (whose inner x face is open)
+// that of 1 or 2 open x faces, so two reciprocals per wave serve all cells.  Both paths evaluate the operations of jacobi_update on the
+// values of flag_cell, bit for bit.
+template <int OP, int W, bool ZG, class Ld>
+__device__ __forceinline__ FV<W> vec_smooth(const LevelDev& L, int x0, int y, int z, int64_t i, const Ld& ld,
+                                            const FV<W>& b, float inv_dt, float omega) {
+  FV<W> r;
+  if constexpr (OP == FX_OP_PACKED) {
+    const VecStencil<W> st = vec_stencil_packed<W>(L, x0, y, z, i, ld, !ZG);
+#pragma unroll
+    for (int k = 0; k < W; ++k) {
+      const float u = jacobi_update(st.c[k], st.s[k], b.v[k], st.d[k], inv_dt, omega);
+      r.v[k] = ((st.f[k] & FLAG_FLUID) && st.d[k] > 0.0f) ? u : 0.0f;
+    }
+    return r;
+  } else {
+    uint32_t f[W];
+    vec_flags<W>(L, OP, x0, y, z, i, f);
+    FV<W> c{}, ym{}, yp{}, zm{}, zp{};
+    float xl = 0.0f, xr = 0.0f;
+    if (!ZG) vec_neighbours<W>(L, x0, y, z, i, ld, c, ym, yp, zm, zp, xl, xr);
+    bool lane_open = true;
+#pragma unroll
+    for (int k = 0; k < W; ++k) {
+      const uint32_t need = kAllOpen & ~(k == 0 ? uint32_t(FLAG_XLO) : 0u) & ~(k == W - 1 ? uint32_t(FLAG_XHI) : 0u);
+      lane_open = lane_open && (f[k] & need) == need;
+    }
+    const float d_open = open_diagonal(L.wx, L.wy, L.wz);
+    if (__all(lane_open) && d_open > 0.0f) {
+      const float rd = __builtin_amdgcn_rcpf(d_open);
+      const float d1 = x_diagonal(1, L.wx, L.wy, L.wz);
+      const float rd1 = __builtin_amdgcn_rcpf(d1);
+#pragma unroll
+      for (int k = 0; k < W; ++k) {
+        const float ck = c.v[k];
+        const float left = k > 0 ? c.v[k > 0 ? k - 1 : 0] : xl;
+        const float right = k < W - 1 ? c.v[k < W - 1 ? k + 1 : W - 1] : xr;
+        if (k == 0 || k == W - 1) {
+          const float sk = xend_cell(f[k], ck, left, right, ym.v[k], yp.v[k], zm.v[k], zp.v[k], L.wx, L.wy, L.wz);
+          const bool both = (f[k] & (FLAG_XLO | FLAG_XHI)) == (FLAG_XLO | FLAG_XHI);
+          const float rk = both ? rd : rd1;
+          const bool ok = both || d1 > 0.0f;
+          const float u = __builtin_fmaf(omega * __builtin_fmaf(-b.v[k], inv_dt, sk), rk, ck);
+          r.v[k] = ok ? u : 0.0f;
+        } else {
+          const float sk = open_cell(ck, left, right, ym.v[k], yp.v[k], zm.v[k], zp.v[k], L.wx, L.wy, L.wz);
+          r.v[k] = __builtin_fmaf(omega * __builtin_fmaf(-b.v[k], inv_dt, sk), rd, ck);
+        }
+      }
+      return r;
+    }
+#pragma unroll
+    for (int k = 0; k < W; ++k) {
+      const float ck = c.v[k];
+      const float left = k > 0 ? c.v[k > 0 ? k - 1 : 0] : xl;
+      const float right = k < W - 1 ? c.v[k < W - 1 ? k + 1 : W - 1] : xr;
+      float sk, dk;
+      flag_cell(f[k], ck, left, right, ym.v[k], yp.v[k], zm.v[k], zp.v[k], L.wx, L.wy, L.wz, sk, dk);
+      const float u = jacobi_update(ck, sk, b.v[k], dk, inv_dt, omega);
+      r.v[k] = ((f[k] & FLAG_FLUID) && dk > 0.0f) ? u : 0.0f;
+    }
+    return r;
+  }
+}
+
+// Vector smoother (level 0 and other large box/flags/packed levels): one lane
+// per W consecutive x cells (one 16-byte load per operand: W = 4 for fp32, 8
+// for 16-bit storage), cells enumerated x-fastest over the level; block ids
+// are spread so each XCD sweeps a contiguous range of rows.
+template <typename T, int OP, int W, bool ZG>
 __global__ __launch_bounds__(256) void k_smooth_q(LevelDev L, const T* __restrict__ p, const T* __restrict__ rhs,
-                                                  T* __restrict__ out, float inv_dt, float omega, int zero_guess,
-                                                  int nq) {
+                                                  T* __restrict__ out, float inv_dt, float omega, int nq) {
   const int q = xcd_tile(blockIdx.x, gridDim.x) * 256 + threadIdx.x;
   if (q >= nq) return;
-  const QuadPos at = vec_pos(L, q, W);
+  const QuadPos at = vec_pos<W>(L, q);
   const FV<W> b = VecIO<T, W>::ld(rhs, at.i);  // issued with the stencil loads
-  const VecStencil<W> st = vec_stencil<OP, W>(L, at.x, at.y, at.z, at.i, PLoad<T>{p}, !zero_guess);
-  FV<W> r;
-#pragma unroll
-  for (int k = 0; k < W; ++k) {
-    // evaluated for every cell and selected (no per-cell branch)
-    const float u = jacobi_update(st.c[k], st.s[k], b.v[k], st.d[k], inv_dt, omega);
-    r.v[k] = ((st.f[k] & FLAG_FLUID) && st.d[k] > 0.0f) ? u : 0.0f;
-  }
-  VecIO<T, W>::st(out, at.i, r);
+  VecIO<T, W>::st(out, at.i, vec_smooth<OP, W, ZG>(L, at.x, at.y, at.z, at.i, PLoad<T>{p}, b, inv_dt, omega));
 }
 
 // Vector residual + 8-child mean: lane u covers fine x W*u .. W*u+W-1 of rows
@@ -112,8 +172,8 @@ __global__ __launch_bounds__(256) void k_residual_restrict_q(LevelDev L, int cny
   const int u = xcd_tile(blockIdx.x, gridDim.x) * 256 + threadIdx.x;
   if (u >= nu) return;
   const int un = L.nx / W;
-  const int crow = u / un, ux = u - crow * un;
-  const int Z = crow / cny, Y = crow - Z * cny;
+  const int crow = fdiv(u, row_div<W>(L)), ux = u - crow * un;
+  const int Z = fdiv(crow, L.ny2), Y = crow - Z * cny;
   float acc[W / 2];
 #pragma unroll
   for (int j = 0; j < W / 2; ++j) acc[j] = 0.0f;
@@ -190,8 +250,8 @@ __global__ __launch_bounds__(256) void k_prolong_q(LevelDev F, LevelDev C, const
   const int u = xcd_tile(blockIdx.x, gridDim.x) * 256 + threadIdx.x;
   if (u >= nu) return;
   const int un = F.nx / W;
-  const int crow = u / un, ux = u - crow * un;
-  const int Z = crow / C.ny, Y = crow - Z * C.ny;
+  const int crow = fdiv(u, row_div<W>(F)), ux = u - crow * un;
+  const int Z = fdiv(crow, C.ny1), Y = crow - Z * C.ny;
   const int cx0 = (W / 2) * ux;
   float cv[3][3][CW];
   uint64_t amask = ~uint64_t(0);
@@ -433,9 +493,15 @@ static int launch_smooth(fx_mg_config* c, int l, float inv_dt, bool zero_guess, 
   if (cells >= kQuadMinCells && vec_ok(L, L.p, L.q, L.rhs, W)) {
     const int nq = int(cells / W);
     dim3 grid((nq + 255) / 256);
-#define FX_SMQ(OPC)                                                                                      \
-  hipLaunchKernelGGL((k_smooth_q<TV, OPC, WV>), grid, dim3(256), 0, s, D, (const TV*)L.p, (const TV*)L.rhs, \
-                     (TV*)L.q, inv_dt, c->omega, zero_guess ? 1 : 0, nq)
+#define FX_SMQ(OPC)                                                                                          \
+  do {                                                                                                        \
+    if (zero_guess)                                                                                           \
+      hipLaunchKernelGGL((k_smooth_q<TV, OPC, WV, true>), grid, dim3(256), 0, s, D, (const TV*)L.p,           \
+                         (const TV*)L.rhs, (TV*)L.q, inv_dt, c->omega, nq);                                   \
+    else                                                                                                      \
+      hipLaunchKernelGGL((k_smooth_q<TV, OPC, WV, false>), grid, dim3(256), 0, s, D, (const TV*)L.p,          \
+                         (const TV*)L.rhs, (TV*)L.q, inv_dt, c->omega, nq);                                   \
+  } while (0)
     FX_DISPATCH_T(c->storage, {
       constexpr int WV = vec_width<TV>();
       if (L.op == FX_OP_FLAGS) FX_SMQ(FX_OP_FLAGS);

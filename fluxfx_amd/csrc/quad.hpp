@@ -196,10 +196,10 @@ struct QuadPos {
 
 __device__ __forceinline__ QuadPos quad_pos(const LevelDev& L, int q) {
   const int qx = L.nx >> 2;
-  const int row = q / qx;
+  const int row = fdiv(q, L.row4);
   QuadPos r;
   r.x = (q - row * qx) << 2;
-  r.z = row / L.ny;
+  r.z = fdiv(row, L.ny1);
   r.y = row - r.z * L.ny;
   r.i = int64_t(row) * L.nx + r.x;
   return r;
@@ -441,12 +441,13 @@ __device__ __forceinline__ VecStencil<W> vec_stencil(const LevelDev& L, int x0, 
   return st;
 }
 
-__device__ __forceinline__ QuadPos vec_pos(const LevelDev& L, int q, int W) {
+template <int W>
+__device__ __forceinline__ QuadPos vec_pos(const LevelDev& L, int q) {
   const int qx = L.nx / W;
-  const int row = q / qx;
+  const int row = fdiv(q, row_div<W>(L));
   QuadPos r;
   r.x = (q - row * qx) * W;
-  r.z = row / L.ny;
+  r.z = fdiv(row, L.ny1);
   r.y = row - r.z * L.ny;
   r.i = int64_t(row) * L.nx + r.x;
   return r;
