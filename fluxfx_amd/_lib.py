@@ -40,7 +40,7 @@ class fx_mg_config(C.Structure):
     _fields_ = [("levels", C.c_int32), ("storage", C.c_int32), ("pre", C.c_int32), ("post", C.c_int32),
                 ("omega", C.c_float), ("restriction", C.c_int32), ("prolongation", C.c_int32),
                 ("coarse_sweeps", C.c_int32), ("coarse_matrix", C.c_void_p), ("coarse_scale", C.c_float),
-                ("pad_", C.c_int32), ("level", fx_mg_level * MAX_LEVELS)]
+                ("separate_legs", C.c_int32), ("level", fx_mg_level * MAX_LEVELS)]
 
 
 class fx_conv_desc(C.Structure):

@@ -457,6 +457,326 @@ __global__ void k_direct(int m, const double* __restrict__ M, const T* __restric
   if (lane == 0) out[row] = from_f<T>(float(acc * double(scale)));
 }
 
+// ---------------------------------------------------------------------------
+// Fused V-cycle legs of the levels below the finest.  A V-cycle visits such a
+// level twice: going down it smooths from zero (`pre` sweeps) and restricts
+// the residual by the 8-child mean; coming up it adds the prolonged coarse
+// correction and smooths `post` times.  Each leg is one launch here instead
+// of three: a workgroup owns a kFx x kFy x kFz block of fine cells, stages its
+// operands with a halo into LDS and recomputes the halo's sweeps itself
+// (temporal blocking).  Values are rounded to the storage type exactly where
+// the unfused path stores them and each cell's arithmetic is that of the
+// one-cell kernels, so both paths give the same bits.  Cells outside the
+// level hold 0 in every staged array (zero conductances, inactive), which the
+// one-cell kernels' bounds tests amount to.
+constexpr int kFx = 8, kFy = 8, kFz = 8;  // fine cells per workgroup
+constexpr int kLegThreads = 512;
+
+// k over [0, N) in kLegThreads strides, unrolled so that a thread's loads of
+// a staging pass are all in flight together
+#define FX_TILE_FOR(N, k)                                                                 \
+  _Pragma("unroll") for (int j_ = 0; j_ < ((N) + kLegThreads - 1) / kLegThreads; ++j_)   \
+    if (const int k = int(threadIdx.x) + j_ * kLegThreads; k < (N))
+
+// A box of tile-relative cells [-H, k + H) per axis, x fastest.
+template <int H>
+struct TileBox {
+  static constexpr int ex = kFx + 2 * H, ey = kFy + 2 * H, ez = kFz + 2 * H, n = ex * ey * ez;
+  static __device__ __forceinline__ int at(int x, int y, int z) { return (x + H) + ex * ((y + H) + ey * (z + H)); }
+  static __device__ __forceinline__ void coords(int k, int& x, int& y, int& z) {
+    x = k % ex - H;
+    y = (k / ex) % ey - H;
+    z = k / (ex * ey) - H;
+  }
+};
+
+// 7-point operator of a BOX or PACKED cell from staged words (halo HW) and
+// staged p (halo HP): the operations of stencil<OP>.
+template <int OP, int HW, int HP>
+__device__ __forceinline__ Stencil tile_stencil(const LevelDev& L, const uint32_t* wd, const float* pv, int x, int y,
+                                                int z, int gx, int gy, int gz, bool read_p) {
+  using BW = TileBox<HW>;
+  using BP = TileBox<HP>;
+  Stencil st{0.0f, 0.0f, 0.0f, true};
+  const float c0 = read_p ? pv[BP::at(x, y, z)] : 0.0f;
+  st.c = c0;
+  if constexpr (OP == FX_OP_PACKED) {
+    const uint32_t u = wd[BW::at(x, y, z)];
+    const float cx = float(u & 1023u) * L.wx, cy = float((u >> 10) & 1023u) * L.wy,
+                cz = float((u >> 20) & 1023u) * L.wz;
+    const float wxl = float(wd[BW::at(x - 1, y, z)] & 1023u) * L.wx;
+    const float wyl = float((wd[BW::at(x, y - 1, z)] >> 10) & 1023u) * L.wy;
+    const float wzl = float((wd[BW::at(x, y, z - 1)] >> 20) & 1023u) * L.wz;
+    st.d = cx + cy + cz + wxl + wyl + wzl;
+    st.active = (u >> 30) & 1u;
+    if (read_p) {
+      float s = 0.0f;
+      if (cx != 0.0f) s += cx * (pv[BP::at(x + 1, y, z)] - c0);
+      if (wxl != 0.0f) s += wxl * (pv[BP::at(x - 1, y, z)] - c0);
+      if (cy != 0.0f) s += cy * (pv[BP::at(x, y + 1, z)] - c0);
+      if (wyl != 0.0f) s += wyl * (pv[BP::at(x, y - 1, z)] - c0);
+      if (cz != 0.0f) s += cz * (pv[BP::at(x, y, z + 1)] - c0);
+      if (wzl != 0.0f) s += wzl * (pv[BP::at(x, y, z - 1)] - c0);
+      st.s = s;
+    }
+  } else {
+    static_assert(OP == FX_OP_BOX, "fused legs cover box and packed levels");
+    const uint8_t f = box_flags(gx, gy, gz, L.nx, L.ny, L.nz);
+    float n[6] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+    if (read_p) {
+      if (f & FLAG_XLO) n[0] = pv[BP::at(x - 1, y, z)];
+      if (f & FLAG_XHI) n[1] = pv[BP::at(x + 1, y, z)];
+      if (f & FLAG_YLO) n[2] = pv[BP::at(x, y - 1, z)];
+      if (f & FLAG_YHI) n[3] = pv[BP::at(x, y + 1, z)];
+      if (f & FLAG_ZLO) n[4] = pv[BP::at(x, y, z - 1)];
+      if (f & FLAG_ZHI) n[5] = pv[BP::at(x, y, z + 1)];
+    }
+    flag_cell(f, c0, n[0], n[1], n[2], n[3], n[4], n[5], L.wx, L.wy, L.wz, st.s, st.d);
+  }
+  return st;
+}
+
+__device__ __forceinline__ bool in_level(const LevelDev& L, int gx, int gy, int gz) {
+  return gx >= 0 && gy >= 0 && gz >= 0 && gx < L.nx && gy < L.ny && gz < L.nz;
+}
+
+// A level array on TileBox<H> staged through registers into LDS laid out on
+// TileBox<HL> (HL >= H), 0 outside the level.  load() only issues loads
+// (out-of-level lanes read element 0 and discard it), store() writes LDS: a
+// kernel issues the loads of all its arrays before the first store, so one
+// memory round trip covers its staging.
+template <int H, int HL, typename T>
+struct Staged {
+  using B = TileBox<H>;
+  static constexpr int NJ = (B::n + kLegThreads - 1) / kLegThreads;
+  T v[NJ];
+  bool ok[NJ];
+  int at[NJ];
+  __device__ __forceinline__ void load(const LevelDev& L, int x0, int y0, int z0, const T* __restrict__ src) {
+#pragma unroll
+    for (int j = 0; j < NJ; ++j) {
+      const int k = int(threadIdx.x) + j * kLegThreads;
+      int x, y, z;
+      B::coords(k < B::n ? k : B::n - 1, x, y, z);
+      const int gx = x0 + x, gy = y0 + y, gz = z0 + z;
+      ok[j] = k < B::n && in_level(L, gx, gy, gz);
+      at[j] = k < B::n ? TileBox<HL>::at(x, y, z) : -1;
+      v[j] = src[ok[j] ? idx3(gx, gy, gz, L.nx, L.ny) : 0];
+    }
+  }
+  __device__ __forceinline__ void store_float(float* dst) const {
+#pragma unroll
+    for (int j = 0; j < NJ; ++j)
+      if (at[j] >= 0) dst[at[j]] = ok[j] ? to_f(v[j]) : 0.0f;
+  }
+  __device__ __forceinline__ void store_word(uint32_t* dst) const {
+#pragma unroll
+    for (int j = 0; j < NJ; ++j)
+      if (at[j] >= 0) dst[at[j]] = ok[j] ? uint32_t(v[j]) : 0u;
+  }
+};
+
+// One damped-Jacobi sweep over the cells of TileBox<HO> from p (read_p) or
+// from zero, rounded to T; p, the result and rhs (rb) are laid out on
+// TileBox<HL>, HL >= HO + 1, the words on TileBox<HW>.
+template <typename T, int OP, int HW, int HL, int HO>
+__device__ __forceinline__ void tile_sweep(const LevelDev& L, int x0, int y0, int z0, const uint32_t* wd,
+                                           const float* pin, const float* rb, float* pout, bool read_p, float inv_dt,
+                                           float omega) {
+  using BO = TileBox<HO>;
+  using BL = TileBox<HL>;
+  FX_TILE_FOR(BO::n, k) {
+    int x, y, z;
+    BO::coords(k, x, y, z);
+    const int gx = x0 + x, gy = y0 + y, gz = z0 + z;
+    float r = 0.0f;
+    if (in_level(L, gx, gy, gz)) {
+      const Stencil st = tile_stencil<OP, HW, HL>(L, wd, pin, x, y, z, gx, gy, gz, read_p);
+      if (st.active && st.d > 0.0f)
+        r = to_f(from_f<T>(jacobi_update(st.c, st.s, rb[BL::at(x, y, z)], st.d, inv_dt, omega)));
+    }
+    pout[BL::at(x, y, z)] = r;
+  }
+}
+
+// Down leg: S sweeps from zero, written to p; the 8-child mean of the
+// residual after them, written to the coarse rhs.  Sweep j runs on the tile
+// with a halo of S - j + 1 cells; p arrays and rhs are laid out on halo S.
+template <typename T, int OP, int S>
+__global__ __launch_bounds__(kLegThreads) void k_leg_down(LevelDev L, int tiles_x, int tiles_y,
+                                                          const T* __restrict__ rhs, T* __restrict__ p,
+                                                          T* __restrict__ crhs, float inv_dt, float omega) {
+  static_assert(S >= 1 && S <= 3, "the fused legs stage one to three sweeps");
+  using BW = TileBox<S + 1>;
+  using BA = TileBox<S>;
+  using BR = TileBox<0>;
+  __shared__ uint32_t wd[OP == FX_OP_PACKED ? BW::n : 1];
+  __shared__ float rb[BA::n];
+  __shared__ float pa[BA::n];
+  __shared__ float pb[S >= 2 ? BA::n : 1];
+  __shared__ float res[BR::n];
+  const int t = blockIdx.x;
+  const int x0 = (t % tiles_x) * kFx, y0 = ((t / tiles_x) % tiles_y) * kFy, z0 = (t / tiles_x / tiles_y) * kFz;
+  Staged<S + 1, S + 1, uint32_t> sw;
+  Staged<S, S, T> sr;
+  if constexpr (OP == FX_OP_PACKED) sw.load(L, x0, y0, z0, reinterpret_cast<const uint32_t*>(L.op));
+  sr.load(L, x0, y0, z0, rhs);
+  if constexpr (OP == FX_OP_PACKED) sw.store_word(wd);
+  sr.store_float(rb);
+  __syncthreads();
+  tile_sweep<T, OP, S + 1, S, S>(L, x0, y0, z0, wd, pa, rb, pa, false, inv_dt, omega);
+  __syncthreads();
+  if constexpr (S >= 2) {
+    tile_sweep<T, OP, S + 1, S, S - 1>(L, x0, y0, z0, wd, pa, rb, pb, true, inv_dt, omega);
+    __syncthreads();
+  }
+  if constexpr (S >= 3) {
+    tile_sweep<T, OP, S + 1, S, S - 2>(L, x0, y0, z0, wd, pb, rb, pa, true, inv_dt, omega);
+    __syncthreads();
+  }
+  const float* pf = (S % 2 == 1) ? pa : pb;  // the last sweep's output (halo 1)
+  FX_TILE_FOR(BR::n, k) {
+    int x, y, z;
+    BR::coords(k, x, y, z);
+    const int gx = x0 + x, gy = y0 + y, gz = z0 + z;
+    float r = 0.0f;
+    if (in_level(L, gx, gy, gz)) {
+      p[idx3(gx, gy, gz, L.nx, L.ny)] = from_f<T>(pf[BA::at(x, y, z)]);
+      // residual_at: rhs*inv_dt - s on active cells
+      const Stencil st = tile_stencil<OP, S + 1, S>(L, wd, pf, x, y, z, gx, gy, gz, true);
+      if (st.active && (OP != FX_OP_PACKED || st.d > 0.0f)) r = rb[BA::at(x, y, z)] * inv_dt - st.s;
+    }
+    res[k] = r;
+  }
+  __syncthreads();
+  const int cnx = L.nx / 2, cny = L.ny / 2, cnz = L.nz / 2;
+  FX_TILE_FOR((kFx / 2) * (kFy / 2) * (kFz / 2), k) {
+    const int cx = k % (kFx / 2), cy = (k / (kFx / 2)) % (kFy / 2), cz = k / ((kFx / 2) * (kFy / 2));
+    const int X = x0 / 2 + cx, Y = y0 / 2 + cy, Z = z0 / 2 + cz;
+    if (X >= cnx || Y >= cny || Z >= cnz) continue;
+    float acc = 0.0f;
+#pragma unroll
+    for (int dz = 0; dz < 2; ++dz)
+#pragma unroll
+      for (int dy = 0; dy < 2; ++dy)
+#pragma unroll
+        for (int dx = 0; dx < 2; ++dx) acc += res[BR::at(2 * cx + dx, 2 * cy + dy, 2 * cz + dz)];
+    crhs[idx3(X, Y, Z, cnx, cny)] = from_f<T>(acc * 0.125f);
+  }
+}
+
+// Up leg: p + prolonged coarse correction (prolong_cell's operations) on the
+// tile with a halo of S cells, then S sweeps (sweep j on halo S - j), the
+// last one written to out (out and p are distinct arrays).
+template <typename T, int OP, int MODE, int S>
+__global__ __launch_bounds__(kLegThreads) void k_leg_up(LevelDev F, LevelDev C, const uint8_t* __restrict__ cact, int tiles_x, int tiles_y,
+                                                        const T* __restrict__ p, const T* __restrict__ coarse,
+                                                        const T* __restrict__ rhs, T* __restrict__ out, float inv_dt,
+                                                        float omega) {
+  static_assert(S >= 1 && S <= 3, "the fused legs stage one to three sweeps");
+  using BU = TileBox<S>;
+  using BV = TileBox<S - 1>;
+  using BR = TileBox<0>;
+  // coarse window [x0/2 - 2, x0/2 + kFx/2 + 2) per axis: the clamped trilinear
+  // neighbours of every fine cell within 3 cells of the tile
+  constexpr int CWX = kFx / 2 + 4, CWY = kFy / 2 + 4, CWZ = kFz / 2 + 4, CWN = CWX * CWY * CWZ;
+  __shared__ uint32_t wd[OP == FX_OP_PACKED ? BU::n : 1];
+  __shared__ float cw[CWN];
+  __shared__ uint8_t ca[MODE == FX_PROLONG_RENORM ? CWN : 1];
+  __shared__ float rb[BU::n];
+  __shared__ float pa[BU::n];
+  __shared__ float pb[S >= 2 ? BU::n : 1];
+  const int t = blockIdx.x;
+  const int x0 = (t % tiles_x) * kFx, y0 = ((t / tiles_x) % tiles_y) * kFy, z0 = (t / tiles_x / tiles_y) * kFz;
+  const int wx0 = x0 / 2 - 2, wy0 = y0 / 2 - 2, wz0 = z0 / 2 - 2;
+  Staged<S, S, uint32_t> sw;
+  Staged<S - 1, S, T> sr;
+  Staged<S, S, T> sp;
+  if constexpr (OP == FX_OP_PACKED) sw.load(F, x0, y0, z0, reinterpret_cast<const uint32_t*>(F.op));
+  sr.load(F, x0, y0, z0, rhs);
+  sp.load(F, x0, y0, z0, p);
+  // the coarse window with clamped indices (every index valid)
+  float cv[(CWN + kLegThreads - 1) / kLegThreads];
+  uint32_t cflag[(CWN + kLegThreads - 1) / kLegThreads];
+#pragma unroll
+  for (int j = 0; j < (CWN + kLegThreads - 1) / kLegThreads; ++j) {
+    const int k = min(int(threadIdx.x) + j * kLegThreads, CWN - 1);
+    const int cx = min(max(wx0 + k % CWX, 0), C.nx - 1), cy = min(max(wy0 + (k / CWX) % CWY, 0), C.ny - 1),
+              cz = min(max(wz0 + k / (CWX * CWY), 0), C.nz - 1);
+    const int64_t ci = idx3(cx, cy, cz, C.nx, C.ny);
+    cv[j] = to_f(coarse[ci]);
+    cflag[j] = (MODE == FX_PROLONG_RENORM && cact) ? cact[ci] : 1u;
+  }
+  if constexpr (OP == FX_OP_PACKED) sw.store_word(wd);
+  sr.store_float(rb);
+  sp.store_float(pa);
+#pragma unroll
+  for (int j = 0; j < (CWN + kLegThreads - 1) / kLegThreads; ++j) {
+    const int k = int(threadIdx.x) + j * kLegThreads;
+    if (k < CWN) {
+      cw[k] = cv[j];
+      if (MODE == FX_PROLONG_RENORM) ca[k] = cflag[j] & 1u;
+    }
+  }
+  __syncthreads();
+  FX_TILE_FOR(BU::n, k) {
+    int x, y, z;
+    BU::coords(k, x, y, z);
+    const int gx = x0 + x, gy = y0 + y, gz = z0 + z;
+    float r = 0.0f;
+    if (in_level(F, gx, gy, gz)) {
+      const int bx = (gx >> 1) - ((gx & 1) ? 0 : 1), by = (gy >> 1) - ((gy & 1) ? 0 : 1),
+                bz = (gz >> 1) - ((gz & 1) ? 0 : 1);
+      const float fx = (gx & 1) ? 0.25f : 0.75f, fy = (gy & 1) ? 0.25f : 0.75f, fz = (gz & 1) ? 0.25f : 0.75f;
+      float num = 0.0f, den = 0.0f;
+#pragma unroll
+      for (int dz = 0; dz < 2; ++dz) {
+        const float wz = dz ? fz : 1.0f - fz;
+#pragma unroll
+        for (int dy = 0; dy < 2; ++dy) {
+          const float wy = dy ? fy : 1.0f - fy;
+#pragma unroll
+          for (int dx = 0; dx < 2; ++dx) {
+            const float wgt = wz * wy * (dx ? fx : 1.0f - fx);
+            const int j = (bx + dx - wx0) + CWX * ((by + dy - wy0) + CWY * (bz + dz - wz0));
+            if (MODE == FX_PROLONG_RENORM) {
+              if (!ca[j]) continue;
+              den += wgt;
+            }
+            num = __builtin_fmaf(wgt, cw[j], num);
+          }
+        }
+      }
+      float corr = num;
+      if (MODE == FX_PROLONG_RENORM) corr = den > 1e-12f ? num / den : 0.0f;
+      r = pa[k] + corr;
+      if (OP == FX_OP_PACKED && !((wd[BU::at(x, y, z)] >> 30) & 1u)) r = 0.0f;
+      r = to_f(from_f<T>(r));
+    }
+    pa[k] = r;
+  }
+  __syncthreads();
+  if constexpr (S >= 2) {
+    tile_sweep<T, OP, S, S, S - 1>(F, x0, y0, z0, wd, pa, rb, pb, true, inv_dt, omega);
+    __syncthreads();
+  }
+  if constexpr (S >= 3) {
+    tile_sweep<T, OP, S, S, S - 2>(F, x0, y0, z0, wd, pb, rb, pa, true, inv_dt, omega);
+    __syncthreads();
+  }
+  const float* pf = (S % 2 == 1) ? pa : pb;  // input of the last sweep (halo 1)
+  FX_TILE_FOR(BR::n, k) {
+    int x, y, z;
+    BR::coords(k, x, y, z);
+    const int gx = x0 + x, gy = y0 + y, gz = z0 + z;
+    if (!in_level(F, gx, gy, gz)) continue;
+    const Stencil st = tile_stencil<OP, S, S>(F, wd, pf, x, y, z, gx, gy, gz, true);
+    float r = 0.0f;
+    if (st.active && st.d > 0.0f) r = jacobi_update(st.c, st.s, rb[BU::at(x, y, z)], st.d, inv_dt, omega);
+    out[idx3(gx, gy, gz, F.nx, F.ny)] = from_f<T>(r);
+  }
+}
+
 static const dim3 kBlock = kMgBlock;
 
 // ---------------------------------------------------------------------------
@@ -678,12 +998,110 @@ static int launch_fill_zero(fx_mg_config* c, void* ptr, int64_t n, hipStream_t s
   return FX_OK;
 }
 
+// Levels above this many cells keep separate launches: there the vector
+// kernels run at full occupancy, and the halo recomputation of a fused leg
+// (about 6 stencil evaluations per cell on 8^3 blocks) costs more than the
+// launches it saves (measured at 128^3: 42 + 48 us fused against 22 + 25 us).
+constexpr int64_t kLegMaxCells = int64_t(1) << 18;
+
+// Can level l (not the coarsest) run its V-cycle legs fused?
+static bool legs_fused(const fx_mg_config* c, int l, bool zero_guess) {
+  const fx_mg_level& F = c->level[l];
+  const fx_mg_level& C = c->level[l + 1];
+  if (int64_t(F.nx) * F.ny * F.nz > kLegMaxCells) return false;
+  // renormalised prolongation reads the coarse activity bytes (a box level is all active)
+  const bool renorm_known = C.op == FX_OP_BOX || (op_has_edges(C.op) && C.active);
+  return !c->separate_legs && l >= 1 && zero_guess && c->pre >= 1 && c->pre <= 3 && c->post >= 1 && c->post <= 3 &&
+         c->restriction == FX_RESTRICT_MEAN &&
+         (c->prolongation == FX_PROLONG_TRILINEAR || (c->prolongation == FX_PROLONG_RENORM && renorm_known)) &&
+         (F.op == FX_OP_BOX || F.op == FX_OP_PACKED) && F.p != F.q;
+}
+
+static dim3 leg_grid(const fx_mg_level& L, int& tiles_x, int& tiles_y) {
+  tiles_x = (L.nx + kFx - 1) / kFx;
+  tiles_y = (L.ny + kFy - 1) / kFy;
+  return dim3(tiles_x * tiles_y * ((L.nz + kFz - 1) / kFz));
+}
+
+// down leg: pre-sweeps from zero into the array the separate sweeps would
+// leave them in (L.p after an even count, L.q after an odd one), restricted
+// residual into the coarse rhs
+static int launch_leg_down(fx_mg_config* c, int l, float inv_dt, hipStream_t s) {
+  fx_mg_level& L = c->level[l];
+  fx_mg_level& C = c->level[l + 1];
+  int tx, ty;
+  const dim3 grid = leg_grid(L, tx, ty);
+  const LevelDev D = dev_level(L);
+  void* dst = (c->pre % 2) ? L.q : L.p;
+#define FX_LEG_DOWN(OPL, SW)                                                                                   \
+  hipLaunchKernelGGL((k_leg_down<TV, OPL, SW>), grid, dim3(kLegThreads), 0, s, D, tx, ty, (const TV*)L.rhs,     \
+                     (TV*)dst, (TV*)C.rhs, inv_dt, c->omega)
+#define FX_LEG_DOWN_S(OPL)                                                                                     \
+  do {                                                                                                         \
+    if (c->pre == 1) FX_LEG_DOWN(OPL, 1);                                                                      \
+    else if (c->pre == 2) FX_LEG_DOWN(OPL, 2);                                                                 \
+    else FX_LEG_DOWN(OPL, 3);                                                                                  \
+  } while (0)
+  FX_DISPATCH_T(c->storage, {
+    if (L.op == FX_OP_PACKED) FX_LEG_DOWN_S(FX_OP_PACKED);
+    else FX_LEG_DOWN_S(FX_OP_BOX);
+  });
+#undef FX_LEG_DOWN_S
+#undef FX_LEG_DOWN
+  FX_LAUNCHED("k_leg_down");
+  if (c->pre % 2) {
+    L.q = L.p;
+    L.p = dst;
+  }
+  return FX_OK;
+}
+
+// up leg: prolonged correction and post-sweeps from L.p into L.q, then p/q swap
+static int launch_leg_up(fx_mg_config* c, int l, float inv_dt, hipStream_t s) {
+  fx_mg_level& L = c->level[l];
+  const fx_mg_level& C = c->level[l + 1];
+  int tx, ty;
+  const dim3 grid = leg_grid(L, tx, ty);
+  const LevelDev DF = dev_level(L), DC = dev_level(C);
+  const uint8_t* cact = op_has_edges(C.op) ? C.active : nullptr;
+#define FX_LEG_UP(OPF, MODE, SW)                                                                               \
+  hipLaunchKernelGGL((k_leg_up<TV, OPF, MODE, SW>), grid, dim3(kLegThreads), 0, s, DF, DC, cact, tx, ty,       \
+                     (const TV*)L.p, (const TV*)C.p, (const TV*)L.rhs, (TV*)L.q, inv_dt, c->omega)
+#define FX_LEG_UP_S(OPF, MODE)                                                                                 \
+  do {                                                                                                         \
+    if (c->post == 1) FX_LEG_UP(OPF, MODE, 1);                                                                 \
+    else if (c->post == 2) FX_LEG_UP(OPF, MODE, 2);                                                            \
+    else FX_LEG_UP(OPF, MODE, 3);                                                                              \
+  } while (0)
+  FX_DISPATCH_T(c->storage, {
+    if (c->prolongation == FX_PROLONG_RENORM) {
+      if (L.op == FX_OP_PACKED) FX_LEG_UP_S(FX_OP_PACKED, FX_PROLONG_RENORM);
+      else FX_LEG_UP_S(FX_OP_BOX, FX_PROLONG_RENORM);
+    } else {
+      if (L.op == FX_OP_PACKED) FX_LEG_UP_S(FX_OP_PACKED, FX_PROLONG_TRILINEAR);
+      else FX_LEG_UP_S(FX_OP_BOX, FX_PROLONG_TRILINEAR);
+    }
+  });
+#undef FX_LEG_UP_S
+#undef FX_LEG_UP
+  FX_LAUNCHED("k_leg_up");
+  void* t = L.p;
+  L.p = L.q;
+  L.q = t;
+  return FX_OK;
+}
+
 // Recursive V-cycle.  zero_guess: the level's current iterate is 0.
 int vcycle(fx_mg_config* c, int l, float inv_dt, bool zero_guess, hipStream_t s) {
   fx_mg_level& L = c->level[l];
   if (l == c->levels - 1) {
     if (c->coarse_sweeps > 0) return smooth_n(c, l, c->coarse_sweeps, inv_dt, zero_guess, s);
     return launch_direct(c, s);
+  }
+  if (legs_fused(c, l, zero_guess)) {
+    if (int e = launch_leg_down(c, l, inv_dt, s)) return e;
+    if (int e = vcycle(c, l + 1, 1.0f, true, s)) return e;
+    return launch_leg_up(c, l, inv_dt, s);
   }
   if (c->pre > 0) {
     if (int e = smooth_n(c, l, c->pre, inv_dt, zero_guess, s)) return e;

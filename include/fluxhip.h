@@ -212,7 +212,10 @@ typedef struct fx_mg_config {
   int32_t coarse_sweeps;  /* > 0: smooth the coarsest level this many times; 0: direct */
   const double* coarse_matrix; /* device, m x m FP64 pseudo-inverse, m = cells of coarsest */
   float coarse_scale;     /* direct: p = coarse_scale * M rhs */
-  int32_t pad_;
+  int32_t separate_legs;  /* 0: on levels below the finest, each V-cycle leg (the pre-sweeps from zero with
+                            the restriction, the prolongation with the post-sweeps) runs as one fused launch
+                            when pre = post = 2, the restriction is the 8-child mean and the prolongation
+                            trilinear or renormalised (same results); 1: always separate launches */
   fx_mg_level level[16];
 } fx_mg_config;
 
