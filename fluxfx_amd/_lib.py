@@ -178,6 +178,7 @@ SIGNATURES = {
                         C.POINTER(fx_solve_info), C.POINTER(C.POINTER(C.c_float)), P], C.c_int),
     "fx_pcg_solve": ([C.POINTER(fx_mg_config), P, P, P, P, P, P, C.POINTER(fx_solve_params), P,
                       C.POINTER(fx_solve_info), P], C.c_int),
+    "fx_transport_fast_paths": ([C.c_int32], C.c_int),
     "fx_face_predict": ([fx_grid, F32, P, P, P, P, P, P, P, P, P], C.c_int),
     "fx_face_predict_bounds": ([fx_grid, F32, P, P, P, P, P, P, P, P, P], C.c_int),
     "fx_build_clearance": ([fx_grid, P, P, P, P], C.c_int),

@@ -2,7 +2,15 @@
 // + buoyancy, multi-channel scalar predictor and fused reverse/correct/source
 // kernel, combustion and max-abs reduction.  One thread per face or cell;
 // trilinear gathers stay in L1/L2 (neighbouring lanes read neighbouring data).
+#include <atomic>
+
 #include "common.hpp"
+
+// Floating-point expressions in this file are evaluated as written: a product
+// feeding a sum is fused only where an explicit fma says so.  The interior
+// fast path and the general path then round alike (an implicit contraction
+// could fuse an expression in one path and not in the other).
+#pragma clang fp contract(off)
 
 namespace fx {
 
@@ -11,13 +19,20 @@ struct GridDev {
   float ex, ey, ez;  // extent (m)
   float hx, hy, hz;  // cell size (m)
   float ix, iy, iz;  // cells per metre
-  float rx, rx1;     // 1/nx and 1/(nx + 1): row splitting of the thread index
+  int fast;          // 0: every site takes the general path (fx_transport_fast_paths)
 };
 
 struct V3 {
   float x, y, z;
 };
 __device__ __forceinline__ V3 v3(float x, float y, float z) { return V3{x, y, z}; }
+
+// trace point p + s v, one rounding per component
+__device__ __forceinline__ V3 trace(V3 p, float s, V3 v) {
+  return v3(__builtin_fmaf(s, v.x, p.x), __builtin_fmaf(s, v.y, p.y), __builtin_fmaf(s, v.z, p.z));
+}
+// lattice coordinate x / h - off of an array whose sample i sits at (i + off) h
+__device__ __forceinline__ float lat(float x, float inv_h, float off) { return __builtin_fmaf(x, inv_h, -off); }
 
 // ---------------------------------------------------------------------------
 // addressing: every transport array holds fewer than 2^30 elements and every
@@ -36,28 +51,35 @@ __device__ __forceinline__ void stf(float* __restrict__ base, uint32_t off, floa
   *reinterpret_cast<float*>(reinterpret_cast<char*>(base) + off) = v;
 }
 
-// Thread -> element (x, y, z) of an sx x sy x sz array: blocks of 256 threads
-// tile each z-plane (blockIdx.y) contiguously, so rows of nx + 1 faces leave
-// no idle lanes.  The row split uses the fp32 quotient (exact index < 2^24,
-// at most one off) and one correction.  i: element index.
-__device__ __forceinline__ bool thread_site(uint32_t sx, uint32_t sy, float rsx, uint32_t& x, uint32_t& y,
-                                            uint32_t& z, uint32_t& i) {
-  const uint32_t plane = sx * sy;
-  const uint32_t j = blockIdx.x * 256u + threadIdx.x;
-  if (j >= plane) return false;
-  uint32_t q = uint32_t(float(j) * rsx);
-  int r = int(j) - int(mul24(q, sx));
-  if (r < 0) {
-    q -= 1u;
-    r += int(sx);
-  } else if (r >= int(sx)) {
-    q += 1u;
-    r -= int(sx);
+// Thread -> element (x, y, z) of an sx x sy x sz array: a block of 256
+// threads covers 16 x 16 sites of one z-plane, its four waves 16 x 4 sites
+// each, so a wave's sites lie within a few cells of each other (only waves
+// near a wall or a solid leave the interior fast path).
+// Block order: the hardware hands workgroups to the 8 XCDs round-robin.  The
+// i-th block an XCD receives is given the next block of that XCD's current
+// run of kXcdRun consecutive blocks (runs dealt to the XCDs in turn), so an
+// XCD works through whole runs of neighbouring planes, which its own L2 keeps
+// for the trilinear gathers, while the runs of all XCDs stay spread over the
+// grid (balanced work where the plume is).  i: element index.
+constexpr uint32_t kSiteTile = 16;
+constexpr uint32_t kXcds = 8;
+constexpr uint32_t kXcdRun = 512;
+__device__ __forceinline__ bool thread_site(uint32_t sx, uint32_t sy, uint32_t& x, uint32_t& y, uint32_t& z,
+                                            uint32_t& i) {
+  const uint32_t tiles_x = (sx + kSiteTile - 1) / kSiteTile;
+  const uint32_t nb = gridDim.x * gridDim.y;
+  uint32_t b = blockIdx.x + gridDim.x * blockIdx.y;
+  if (b < nb - nb % (kXcds * kXcdRun)) {  // the last partial round keeps its order
+    const uint32_t xcd = b % kXcds, k = b / kXcds;
+    b = ((k / kXcdRun) * kXcds + xcd) * kXcdRun + k % kXcdRun;
   }
-  x = uint32_t(r);
-  y = q;
-  z = blockIdx.y;
-  i = j + mul24(plane, z);
+  const uint32_t plane = b / gridDim.x, t = b - plane * gridDim.x;
+  const uint32_t ty = t / tiles_x, tx = t - ty * tiles_x;
+  x = tx * kSiteTile + (threadIdx.x % kSiteTile);
+  y = ty * kSiteTile + (threadIdx.x / kSiteTile);
+  z = plane;
+  if (x >= sx || y >= sy) return false;
+  i = x + mul24(sx, y) + mul24(sx * sy, z);
   return true;
 }
 
@@ -336,6 +358,93 @@ __device__ __forceinline__ bool end_free(const GridDev& g, const uint8_t* __rest
   return !halo_fast(g, s, cl, t) && dist3(clip_fast(g, s, R, p, t), t) < 1e-7f;
 }
 
+// ---------------------------------------------------------------------------
+// Interior fast path.  Most sites lie at least 4 cells from every solid and
+// every wall, and their traces end within 2 cells of the site (CFL): there
+// every trilinear corner is in range and every solid test passes, so the
+// clamps, the solid tests and the bounds tests are known.  A wave takes the
+// fast path when all its lanes qualify (tested before each gather, so no lane
+// reads out of range), else the general path; both evaluate the same
+// operations on the same values, so they give the same bits.
+// ---------------------------------------------------------------------------
+// site cell (x, y, z) at least 4 cells from every solid and wall: clearance
+// >= 4 (the clearance counts out-of-range cells as solid), or without solids
+// the cell index in [3, n - 4] on every axis
+template <bool SOLID>
+__device__ __forceinline__ bool open_site(const GridDev& g, const Reach& R, int x, int y, int z) {
+  if (!g.fast) return false;
+  if (SOLID) return R.c >= 4.0f;
+  return x >= 3 && y >= 3 && z >= 3 && x <= g.nx - 4 && y <= g.ny - 4 && z <= g.nz - 4;
+}
+
+// t within 2 cells (Chebyshev) of the site position o.  With an open site
+// (i) every trilinear corner of a face or cell sample at t is in range: the
+// coordinate stays at least 1 from both ends of the sampled array, and
+// (ii) Reach.free(o, t) holds: ceil(r) + 1 <= 3 < 4 <= c.
+__device__ __forceinline__ bool near_site(const GridDev& g, V3 o, V3 t) {
+  return fmaxf(fmaxf(fabsf((t.x - o.x) * g.ix), fabsf((t.y - o.y) * g.iy)), fabsf((t.z - o.z) * g.iz)) <= 2.0f;
+}
+
+// tri_face for q at least 1 from both ends of every axis: the clamps are
+// identities and the high corner is in range
+__device__ __forceinline__ Tri tri_face_in(uint32_t sx, uint32_t sy, float qx, float qy, float qz) {
+  const float lx = floorf(qx), ly = floorf(qy), lz = floorf(qz);
+  Tri t;
+  t.fx = qx - lx;
+  t.fy = qy - ly;
+  t.fz = qz - lz;
+  t.o = lin(uint32_t(lx), uint32_t(ly), uint32_t(lz), sx, sy) << 2;
+  t.dx = 4u;
+  t.dy = sx << 2;
+  t.dz = mul24(sx, sy) << 2;
+  return t;
+}
+
+__device__ __forceinline__ float sample_face_in(const float* __restrict__ F, uint32_t sx, uint32_t sy, float qx,
+                                                float qy, float qz) {
+  float d[8];
+  const Tri t = tri_face_in(sx, sy, qx, qy, qz);
+  tri_donors(F, t, d);
+  return tri_blend(t, d);
+}
+
+__device__ __forceinline__ float sample_face_mm_in(const float* __restrict__ F, uint32_t sx, uint32_t sy, float qx,
+                                                   float qy, float qz, float& lo, float& hi) {
+  float d[8];
+  const Tri t = tri_face_in(sx, sy, qx, qy, qz);
+  tri_donors(F, t, d);
+  lo = fminf(fminf(fminf(d[0], d[1]), fminf(d[2], d[3])), fminf(fminf(d[4], d[5]), fminf(d[6], d[7])));
+  hi = fmaxf(fmaxf(fmaxf(d[0], d[1]), fmaxf(d[2], d[3])), fmaxf(fmaxf(d[4], d[5]), fmaxf(d[6], d[7])));
+  return tri_blend(t, d);
+}
+
+// velocity_at for a point within 2 cells of an open site
+__device__ __forceinline__ V3 velocity_at_in(const GridDev& g, const Vel& V, V3 p) {
+  float qx = p.x * g.ix, qy = p.y * g.iy, qz = p.z * g.iz;
+  return v3(sample_face_in(V.u, g.nx + 1, g.ny, qx, qy - 0.5f, qz - 0.5f),
+            sample_face_in(V.v, g.nx, g.ny + 1, qx - 0.5f, qy, qz - 0.5f),
+            sample_face_in(V.w, g.nx, g.ny, qx - 0.5f, qy - 0.5f, qz));
+}
+
+// velocity_at_face at an open site: every neighbour index is in range
+template <int AXIS>
+__device__ __forceinline__ V3 velocity_at_face_in(const GridDev& g, const Vel& V, int x, int y, int z, uint32_t ib) {
+  auto four = [&](const float* F, int b) {
+    const uint32_t sx = g.nx + (b == 0), sy = g.ny + (b == 1);
+    int lo[3] = {x, y, z};
+    lo[AXIS] -= 1;  // F coordinate i_a - 0.5 along AXIS: cells i_a - 1 and i_a
+    const int a1 = AXIS < b ? AXIS : b, a2 = AXIS < b ? b : AXIS;
+    const uint32_t stride[3] = {4u, sx << 2, mul24(sx, sy) << 2};
+    const uint32_t o = lin(lo[0], lo[1], lo[2], sx, sy) << 2;
+    const uint32_t s1 = stride[a1], s2 = stride[a2];
+    return 0.25f * (ldf(F, o) + ldf(F, o + s1) + ldf(F, o + s2) + ldf(F, o + s1 + s2));
+  };
+  const float u = AXIS == 0 ? ldf(V.u, ib) : four(V.u, 0);
+  const float v = AXIS == 1 ? ldf(V.v, ib) : four(V.v, 1);
+  const float w = AXIS == 2 ? ldf(V.w, ib) : four(V.w, 2);
+  return v3(u, v, w);
+}
+
 // separable L-infinity distance transform, one axis per pass:
 // pass 0: d = min |k| over solid/out-of-range cells c+k*e_x (|k| <= 254)
 // pass 1,2: d = min over |k| of max(|k|, d_prev[c+k*e_a]), out-of-range -> |k|
@@ -411,12 +520,32 @@ __global__ void k_face_predict(GridDev g, Vel V, float dt, const float* __restri
                                const uint8_t* __restrict__ solid, const uint8_t* __restrict__ clear) {
   const int sx = g.nx + (AXIS == 0), sy = g.ny + (AXIS == 1), sz = g.nz + (AXIS == 2);
   uint32_t ux, uy, uz, ue;
-  if (!thread_site(sx, sy, AXIS == 0 ? g.rx1 : g.rx, ux, uy, uz, ue)) return;
+  if (!thread_site(sx, sy, ux, uy, uz, ue)) return;
   const int x = int(ux), y = int(uy), z = int(uz);
   const uint32_t i = ue << 2;  // byte offset of this face
   const float offx = AXIS == 0 ? 0.0f : 0.5f, offy = AXIS == 1 ? 0.0f : 0.5f, offz = AXIS == 2 ? 0.0f : 0.5f;
   V3 p = v3((x + offx) * g.hx, (y + offy) * g.hy, (z + offz) * g.hz);
   const Reach R = SOLID ? reach_cell(g, clear, x, y, z) : Reach{-1.0f};
+  if (__all(open_site<SOLID>(g, R, x, y, z))) {
+    const V3 vp = velocity_at_face_in<AXIS>(g, V, x, y, z, i);
+    const V3 mid = trace(p, -0.5f * dt, vp);
+    if (__all(near_site(g, p, mid))) {
+      const V3 vm = velocity_at_in(g, V, mid);
+      const V3 dep = trace(p, -dt, vm);
+      if (__all(near_site(g, p, dep))) {
+        const float qx = lat(dep.x, g.ix, offx), qy = lat(dep.y, g.iy, offy), qz = lat(dep.z, g.iz, offz);
+        if (!BOUNDS) {
+          stf(out, i, sample_face_in(in, sx, sy, qx, qy, qz));
+          return;
+        }
+        float lo, hi;
+        stf(out, i, sample_face_mm_in(in, sx, sy, qx, qy, qz, lo, hi));
+        stf(lo_out, i, lo);
+        stf(hi_out, i, hi);
+        return;
+      }
+    }
+  }
   // clearance >= 2 at the cell above the face: both cells of the face are fluid
   if (SOLID && !(R.c >= 2.0f)) {
     int ox = AXIS == 0, oy = AXIS == 1, oz = AXIS == 2;
@@ -430,17 +559,17 @@ __global__ void k_face_predict(GridDev g, Vel V, float dt, const float* __restri
     }
   }
   V3 vp = velocity_at_face<AXIS>(g, V, x, y, z, i);
-  V3 mid = v3(p.x - 0.5f * dt * vp.x, p.y - 0.5f * dt * vp.y, p.z - 0.5f * dt * vp.z);
+  V3 mid = trace(p, -0.5f * dt, vp);
   if (SOLID) mid = clip_fast(g, solid, R, p, mid);
   V3 vm = velocity_at(g, V, mid);
-  V3 dep = v3(p.x - dt * vm.x, p.y - dt * vm.y, p.z - dt * vm.z);
+  V3 dep = trace(p, -dt, vm);
   if (SOLID) dep = clip_fast(g, solid, R, p, dep);
   if (!BOUNDS) {
-    stf(out, i, sample_face(in, sx, sy, sz, dep.x * g.ix - offx, dep.y * g.iy - offy, dep.z * g.iz - offz));
+    stf(out, i, sample_face(in, sx, sy, sz, lat(dep.x, g.ix, offx), lat(dep.y, g.iy, offy), lat(dep.z, g.iz, offz)));
     return;
   }
   float lo, hi;
-  stf(out, i, sample_face_mm(in, sx, sy, sz, dep.x * g.ix - offx, dep.y * g.iy - offy, dep.z * g.iz - offz, lo, hi));
+  stf(out, i, sample_face_mm(in, sx, sy, sz, lat(dep.x, g.ix, offx), lat(dep.y, g.iy, offy), lat(dep.z, g.iz, offz), lo, hi));
   const float lbx = offx * g.hx, lby = offy * g.hy, lbz = offz * g.hz;
   const float hbx = (float(sx - 1) + offx) * g.hx, hby = (float(sy - 1) + offy) * g.hy,
               hbz = (float(sz - 1) + offz) * g.hz;
@@ -476,7 +605,7 @@ __global__ void k_velocity(VelArgs A, fx_sources src) {
   const GridDev& g = A.g;
   const int sx = g.nx + (AXIS == 0), sy = g.ny + (AXIS == 1), sz = g.nz + (AXIS == 2);
   uint32_t ux, uy, uz, ue;
-  if (!thread_site(sx, sy, AXIS == 0 ? g.rx1 : g.rx, ux, uy, uz, ue)) return;
+  if (!thread_site(sx, sy, ux, uy, uz, ue)) return;
   const int x = int(ux), y = int(uy), z = int(uz);
   const uint32_t i = ue << 2;  // byte offset of this face
   const float* orig = AXIS == 0 ? A.V.u : (AXIS == 1 ? A.V.v : A.V.w);
@@ -487,21 +616,52 @@ __global__ void k_velocity(VelArgs A, fx_sources src) {
   bool blocked = false;
   V3 p = v3((x + offx) * hx, (y + offy) * hy, (z + offz) * hz);
   const Reach R = SOLID ? reach_cell(g, A.clear, x, y, z) : Reach{-1.0f};
+  bool fast = false;
+  if ((BOUNDS || !MC) && __all(open_site<SOLID>(g, R, x, y, z))) {
+    const V3 vp = velocity_at_face_in<AXIS>(g, A.V, x, y, z, i);
+    if (BOUNDS) {
+      const V3 rmid = trace(p, 0.5f * dt, vp);
+      if (__all(near_site(g, p, rmid))) {
+        const V3 va = velocity_at_in(g, A.V, rmid);
+        const V3 arr = trace(p, dt, va);
+        if (__all(near_site(g, p, arr))) {
+          const float low = ldf(A.pred, i), lo = ldf(A.lo, i), hi = ldf(A.hi, i);
+          value = low;
+          if (lo <= hi) {
+            const float rev = sample_face_in(A.pred, sx, sy, lat(arr.x, g.ix, offx), lat(arr.y, g.iy, offy), lat(arr.z, g.iz, offz));
+            const float corr = __builtin_fmaf(0.5f, ldf(orig, i) - rev, ldf(A.pred, i));
+            value = __builtin_fmaf(0.5f, low, 0.5f * fminf(fmaxf(corr, lo), hi));
+          }
+          fast = true;
+        }
+      }
+    } else {
+      const V3 mid = trace(p, -0.5f * dt, vp);
+      if (__all(near_site(g, p, mid))) {
+        const V3 vm = velocity_at_in(g, A.V, mid);
+        const V3 dep = trace(p, -dt, vm);
+        if (__all(near_site(g, p, dep))) {
+          value = sample_face_in(orig, sx, sy, lat(dep.x, g.ix, offx), lat(dep.y, g.iy, offy), lat(dep.z, g.iz, offz));
+          fast = true;
+        }
+      }
+    }
+  }
   // clearance >= 2 at the cell above the face: both cells of the face are fluid
-  if (SOLID && !(R.c >= 2.0f)) {
+  if (SOLID && !fast && !(R.c >= 2.0f)) {
     int ox = AXIS == 0, oy = AXIS == 1, oz = AXIS == 2;
     blocked = solid_cell(g, A.solid, x, y, z) || solid_cell(g, A.solid, x - ox, y - oy, z - oz);
   }
-  if (BOUNDS && !blocked) {
+  if (BOUNDS && !fast && !blocked) {
     V3 vp = velocity_at_face<AXIS>(g, A.V, x, y, z, i);
     const float low = ldf(A.pred, i), lo = ldf(A.lo, i), hi = ldf(A.hi, i);
     value = low;
     if (lo <= hi) {
       const float hbx = (float(sx - 1) + offx) * hx, hby = (float(sy - 1) + offy) * hy,
                   hbz = (float(sz - 1) + offz) * hz;
-      V3 rmid = v3(p.x + 0.5f * dt * vp.x, p.y + 0.5f * dt * vp.y, p.z + 0.5f * dt * vp.z);
+      V3 rmid = trace(p, 0.5f * dt, vp);
       V3 va = velocity_at(g, A.V, rmid);
-      V3 arr = v3(p.x + dt * va.x, p.y + dt * va.y, p.z + dt * va.z);
+      V3 arr = trace(p, dt, va);
       bool ok = !(arr.x < offx * hx || arr.y < offy * hy || arr.z < offz * hz || arr.x > hbx || arr.y > hby ||
                   arr.z > hbz);
       if (SOLID && ok && !(R.free(g, p, rmid) && R.free(g, p, arr)))
@@ -509,38 +669,38 @@ __global__ void k_velocity(VelArgs A, fx_sources src) {
              dist3(clip_fast(g, A.solid, R, p, rmid), rmid) < 1e-7f &&
              dist3(clip_fast(g, A.solid, R, p, arr), arr) < 1e-7f;
       if (ok) {
-        float rev = sample_face(A.pred, sx, sy, sz, arr.x * g.ix - offx, arr.y * g.iy - offy, arr.z * g.iz - offz);
-        float corr = ldf(A.pred, i) + 0.5f * (ldf(orig, i) - rev);
-        value = 0.5f * low + 0.5f * fminf(fmaxf(corr, lo), hi);
+        float rev = sample_face(A.pred, sx, sy, sz, lat(arr.x, g.ix, offx), lat(arr.y, g.iy, offy), lat(arr.z, g.iz, offz));
+        float corr = __builtin_fmaf(0.5f, ldf(orig, i) - rev, ldf(A.pred, i));
+        value = __builtin_fmaf(0.5f, low, 0.5f * fminf(fmaxf(corr, lo), hi));
       }
     }
   }
-  if (!BOUNDS && !blocked) {
+  if (!BOUNDS && !fast && !blocked) {
     V3 vp = velocity_at_face<AXIS>(g, A.V, x, y, z, i);
-    V3 mid = v3(p.x - 0.5f * dt * vp.x, p.y - 0.5f * dt * vp.y, p.z - 0.5f * dt * vp.z);
+    V3 mid = trace(p, -0.5f * dt, vp);
     if (SOLID) mid = clip_fast(g, A.solid, R, p, mid);
     V3 vm = velocity_at(g, A.V, mid);
-    V3 dep = v3(p.x - dt * vm.x, p.y - dt * vm.y, p.z - dt * vm.z);
+    V3 dep = trace(p, -dt, vm);
     if (SOLID) dep = clip_fast(g, A.solid, R, p, dep);
-    float qx = dep.x * g.ix - offx, qy = dep.y * g.iy - offy, qz = dep.z * g.iz - offz;
+    float qx = lat(dep.x, g.ix, offx), qy = lat(dep.y, g.iy, offy), qz = lat(dep.z, g.iz, offz);
     float lo = 0.0f, hi = 0.0f;
     float low = MC ? sample_face_mm(orig, sx, sy, sz, qx, qy, qz, lo, hi) : sample_face(orig, sx, sy, sz, qx, qy, qz);
     value = low;
     if (MC) {
       float lbx = offx * hx, lby = offy * hy, lbz = offz * hz;
       float hbx = (float(sx - 1) + offx) * hx, hby = (float(sy - 1) + offy) * hy, hbz = (float(sz - 1) + offz) * hz;
-      V3 rmid = v3(p.x + 0.5f * dt * vp.x, p.y + 0.5f * dt * vp.y, p.z + 0.5f * dt * vp.z);
+      V3 rmid = trace(p, 0.5f * dt, vp);
       V3 va = velocity_at(g, A.V, rmid);
-      V3 arr = v3(p.x + dt * va.x, p.y + dt * va.y, p.z + dt * va.z);
+      V3 arr = trace(p, dt, va);
       bool ok = true;
       if (SOLID) ok = safe_fast(g, A.solid, A.clear, R, p, x, y, z, dep, rmid, arr);
       ok = ok && !(dep.x < lbx || dep.y < lby || dep.z < lbz || dep.x > hbx || dep.y > hby || dep.z > hbz);
       ok = ok && !(arr.x < lbx || arr.y < lby || arr.z < lbz || arr.x > hbx || arr.y > hby || arr.z > hbz);
       if (ok) {
         // dep lies inside the face lattice here, so the sampled donors are the clamp set
-        float rev = sample_face(A.pred, sx, sy, sz, arr.x * g.ix - offx, arr.y * g.iy - offy, arr.z * g.iz - offz);
-        float corr = ldf(A.pred, i) + 0.5f * (ldf(orig, i) - rev);
-        value = 0.5f * low + 0.5f * fminf(fmaxf(corr, lo), hi);
+        float rev = sample_face(A.pred, sx, sy, sz, lat(arr.x, g.ix, offx), lat(arr.y, g.iy, offy), lat(arr.z, g.iz, offz));
+        float corr = __builtin_fmaf(0.5f, ldf(orig, i) - rev, ldf(A.pred, i));
+        value = __builtin_fmaf(0.5f, low, 0.5f * fminf(fmaxf(corr, lo), hi));
       }
     }
   }
@@ -555,7 +715,7 @@ __global__ void k_velocity(VelArgs A, fx_sources src) {
     }
     if (total > 0.0f) {
       float blend = -expm1f(-dt * total);
-      value = value + blend * (weighted / total - value);
+      value = __builtin_fmaf(blend, weighted / total - value, value);
     }
   }
   if (AXIS == 2 && A.density) {
@@ -563,7 +723,7 @@ __global__ void k_velocity(VelArgs A, fx_sources src) {
     const uint32_t ib = lin(x, y, zb, g.nx, g.ny) << 2, ia = lin(x, y, za, g.nx, g.ny) << 2;
     float rho = 0.5f * (ldf(A.density, ib) + ldf(A.density, ia));
     float theta = 0.5f * (ldf(A.temperature, ib) + ldf(A.temperature, ia));
-    value += dt * (A.lift * theta - A.weight * rho);
+    value = __builtin_fmaf(dt, A.lift * theta - A.weight * rho, value);
   }
   stf(A.out, i, value);
 }
@@ -629,9 +789,33 @@ template <bool SOLID>
 __device__ __forceinline__ float closed_sample(const float* __restrict__ f, const Corners& c) {
   float v = 0.0f;
 #pragma unroll
-  for (int k = 0; k < 8; ++k) v += c.w[k] * ldf(f, c.at(k) << 2);
+  for (int k = 0; k < 8; ++k) v = __builtin_fmaf(c.w[k], ldf(f, c.at(k) << 2), v);
   if (!SOLID) return v;
   return c.norm > 0.0f ? v * c.norm : 0.0f;
+}
+
+// closed_corners for q at least 1 from both ends of every axis (no clamps;
+// no corner out of range, so zero extension changes nothing)
+__device__ __forceinline__ void closed_corners_in(const GridDev& g, float qx, float qy, float qz, Corners& c) {
+  const float lx = floorf(qx), ly = floorf(qy), lz = floorf(qz);
+  const float fx = qx - lx, fy = qy - ly, fz = qz - lz;
+  c.o = lin(uint32_t(lx), uint32_t(ly), uint32_t(lz), g.nx, g.ny);
+  c.dx = 1u;
+  c.dy = uint32_t(g.nx);
+  c.dz = mul24(uint32_t(g.nx), uint32_t(g.ny));
+  float wx[2] = {1.0f - fx, fx}, wy[2] = {1.0f - fy, fy}, wz[2] = {1.0f - fz, fz};
+#pragma unroll
+  for (int k = 0; k < 8; ++k) c.w[k] = wx[k & 1] * wy[(k >> 1) & 1] * wz[k >> 2];
+}
+
+// fluid_weights when all eight corners are fluid
+template <bool SOLID>
+__device__ __forceinline__ void fluid_norm(Corners& c) {
+  if (!SOLID) return;
+  float t = 0.0f;
+#pragma unroll
+  for (int k = 0; k < 8; ++k) t += c.w[k];
+  c.norm = t > 1e-8f ? 1.0f / t : 0.0f;
 }
 
 struct ScalarSet {
@@ -648,12 +832,41 @@ template <bool SOLID, bool BOUNDS = false>
 __global__ void k_scalar_predict(GridDev g, Vel V, float dt, int channels, ScalarSet S_,
                                  const uint8_t* __restrict__ solid, const uint8_t* __restrict__ clear) {
   uint32_t ux, uy, uz, ue;
-  if (!thread_site(g.nx, g.ny, g.rx, ux, uy, uz, ue)) return;
+  if (!thread_site(g.nx, g.ny, ux, uy, uz, ue)) return;
   const int x = int(ux), y = int(uy), z = int(uz);
   const uint32_t i = ue << 2;  // byte offset of this cell
   const float hx = g.hx, hy = g.hy, hz = g.hz;
   V3 p = v3((x + 0.5f) * hx, (y + 0.5f) * hy, (z + 0.5f) * hz);
   const Reach R = SOLID ? reach_cell(g, clear, x, y, z) : Reach{-1.0f};
+  if (__all(open_site<SOLID>(g, R, x, y, z))) {
+    const V3 vp = velocity_at_cell(g, V, x, y, z);
+    const V3 mid = trace(p, -0.5f * dt, vp);
+    if (__all(near_site(g, p, mid))) {
+      const V3 vm = velocity_at_in(g, V, mid);
+      const V3 dep = trace(p, -dt, vm);
+      if (__all(near_site(g, p, dep))) {
+        Corners c;
+        closed_corners_in(g, lat(dep.x, g.ix, 0.5f), lat(dep.y, g.iy, 0.5f), lat(dep.z, g.iz, 0.5f), c);
+        fluid_norm<SOLID>(c);
+        for (int k = 0; k < channels; ++k) {
+          const float* f = S_.f[k];
+          stf(S_.o[k], i, closed_sample<SOLID>(f, c));
+          if (BOUNDS) {
+            float lo = __builtin_inff(), hi = -__builtin_inff();
+#pragma unroll
+            for (int q = 0; q < 8; ++q) {
+              const float d = ldf(f, c.at(q) << 2);
+              lo = fminf(lo, d);
+              hi = fmaxf(hi, d);
+            }
+            stf(S_.lo[k], i, lo);
+            stf(S_.hi[k], i, hi);
+          }
+        }
+        return;
+      }
+    }
+  }
   // clearance >= 1: the cell is fluid
   if (SOLID && !(R.c >= 1.0f) && solid[ue]) {
     for (int k = 0; k < channels; ++k) {
@@ -666,13 +879,13 @@ __global__ void k_scalar_predict(GridDev g, Vel V, float dt, int channels, Scala
     return;
   }
   V3 vp = velocity_at_cell(g, V, x, y, z);
-  V3 mid = v3(p.x - 0.5f * dt * vp.x, p.y - 0.5f * dt * vp.y, p.z - 0.5f * dt * vp.z);
+  V3 mid = trace(p, -0.5f * dt, vp);
   if (SOLID) mid = clip_fast(g, solid, R, p, mid);
   V3 vm = velocity_at(g, V, mid);
-  V3 dep = v3(p.x - dt * vm.x, p.y - dt * vm.y, p.z - dt * vm.z);
+  V3 dep = trace(p, -dt, vm);
   if (SOLID) dep = clip_fast(g, solid, R, p, dep);
   Corners c;
-  closed_corners(g, dep.x * g.ix - 0.5f, dep.y * g.iy - 0.5f, dep.z * g.iz - 0.5f, c);
+  closed_corners(g, lat(dep.x, g.ix, 0.5f), lat(dep.y, g.iy, 0.5f), lat(dep.z, g.iz, 0.5f), c);
   fluid_weights<SOLID>(c, solid);  // the donors (c.at) stay all eight corners
   bool ok = true;
   if (BOUNDS) {
@@ -724,7 +937,7 @@ template <bool SOLID, bool MC, bool BOUNDS = false>
 __global__ void k_scalar_advect(ScalarArgs A, fx_sources src) {
   const GridDev& g = A.g;
   uint32_t ux, uy, uz, ue;
-  if (!thread_site(g.nx, g.ny, g.rx, ux, uy, uz, ue)) return;
+  if (!thread_site(g.nx, g.ny, ux, uy, uz, ue)) return;
   const int x = int(ux), y = int(uy), z = int(uz);
   const uint32_t i = ue << 2;  // byte offset of this cell
   const float dt = A.dt;
@@ -741,25 +954,34 @@ __global__ void k_scalar_advect(ScalarArgs A, fx_sources src) {
   for (int s = 0; s < src.count; ++s) wsrc[s] = emission_weight(p, src.table + 16 * s);
   if (BOUNDS) {
     // forward trace and its fluid/clip conditions (the departure ones are in lo <= hi)
-    V3 rmid = v3(p.x + 0.5f * dt * vp.x, p.y + 0.5f * dt * vp.y, p.z + 0.5f * dt * vp.z);
-    V3 va = velocity_at(g, A.V, rmid);
-    V3 arr = v3(p.x + dt * va.x, p.y + dt * va.y, p.z + dt * va.z);
-    const float ux = g.ex - 0.5f * hx, uy = g.ey - 0.5f * hy, uz = g.ez - 0.5f * hz;
-    bool ok_arr = !(arr.x < 0.5f * hx || arr.y < 0.5f * hy || arr.z < 0.5f * hz || arr.x > ux || arr.y > uy ||
-                    arr.z > uz);
-    if (SOLID && ok_arr && !(R.free(g, p, rmid) && R.free(g, p, arr)))
-      ok_arr = !halo_cell(g, A.solid, A.clear, x, y, z) && !halo_fast(g, A.solid, A.clear, arr) &&
-               dist3(clip_fast(g, A.solid, R, p, rmid), rmid) < 1e-7f &&
-               dist3(clip_fast(g, A.solid, R, p, arr), arr) < 1e-7f;
+    V3 rmid = trace(p, 0.5f * dt, vp);
+    // interior fast path (open site, trace within 2 cells): same values
+    bool fast = __all(open_site<SOLID>(g, R, x, y, z)) && __all(near_site(g, p, rmid));
+    V3 va = fast ? velocity_at_in(g, A.V, rmid) : velocity_at(g, A.V, rmid);
+    V3 arr = trace(p, dt, va);
+    fast = fast && __all(near_site(g, p, arr));
+    bool ok_arr = true;
     Corners ca;
-    closed_corners(g, arr.x * g.ix - 0.5f, arr.y * g.iy - 0.5f, arr.z * g.iz - 0.5f, ca);
-    fluid_weights<SOLID>(ca, A.solid);
+    if (fast) {
+      closed_corners_in(g, lat(arr.x, g.ix, 0.5f), lat(arr.y, g.iy, 0.5f), lat(arr.z, g.iz, 0.5f), ca);
+      fluid_norm<SOLID>(ca);
+    } else {
+      const float ux = g.ex - 0.5f * hx, uy = g.ey - 0.5f * hy, uz = g.ez - 0.5f * hz;
+      ok_arr = !(arr.x < 0.5f * hx || arr.y < 0.5f * hy || arr.z < 0.5f * hz || arr.x > ux || arr.y > uy ||
+                 arr.z > uz);
+      if (SOLID && ok_arr && !(R.free(g, p, rmid) && R.free(g, p, arr)))
+        ok_arr = !halo_cell(g, A.solid, A.clear, x, y, z) && !halo_fast(g, A.solid, A.clear, arr) &&
+                 dist3(clip_fast(g, A.solid, R, p, rmid), rmid) < 1e-7f &&
+                 dist3(clip_fast(g, A.solid, R, p, arr), arr) < 1e-7f;
+      closed_corners(g, lat(arr.x, g.ix, 0.5f), lat(arr.y, g.iy, 0.5f), lat(arr.z, g.iz, 0.5f), ca);
+      fluid_weights<SOLID>(ca, A.solid);
+    }
     for (int k = 0; k < A.channels; ++k) {
       const float low = ldf(A.pred[k], i), lo = ldf(A.lo[k], i), hi = ldf(A.hi[k], i);
       float value = low;
       if (ok_arr && lo <= hi) {
         float rev = closed_sample<SOLID>(A.pred[k], ca);
-        float corr = low + 0.5f * (ldf(A.f[k], i) - rev);
+        float corr = __builtin_fmaf(0.5f, ldf(A.f[k], i) - rev, low);
         value = fminf(fmaxf(corr, lo), hi);
       }
       value *= A.keep[k];
@@ -773,7 +995,7 @@ __global__ void k_scalar_advect(ScalarArgs A, fx_sources src) {
           target = fmaxf(target, rate * wsrc[s]);
           any_target = true;
         } else {
-          value += dt * rate * wsrc[s];
+          value = __builtin_fmaf(dt * rate, wsrc[s], value);
         }
       }
       if (ch == 0 && (src.multi || any_target)) value = fmaxf(value, target);
@@ -781,28 +1003,28 @@ __global__ void k_scalar_advect(ScalarArgs A, fx_sources src) {
     }
     return;
   }
-  V3 mid = v3(p.x - 0.5f * dt * vp.x, p.y - 0.5f * dt * vp.y, p.z - 0.5f * dt * vp.z);
+  V3 mid = trace(p, -0.5f * dt, vp);
   if (SOLID) mid = clip_fast(g, A.solid, R, p, mid);
   V3 vm = velocity_at(g, A.V, mid);
-  V3 dep = v3(p.x - dt * vm.x, p.y - dt * vm.y, p.z - dt * vm.z);
+  V3 dep = trace(p, -dt, vm);
   if (SOLID) dep = clip_fast(g, A.solid, R, p, dep);
   Corners c;
-  closed_corners(g, dep.x * g.ix - 0.5f, dep.y * g.iy - 0.5f, dep.z * g.iz - 0.5f, c, A.zero_ext != 0);
+  closed_corners(g, lat(dep.x, g.ix, 0.5f), lat(dep.y, g.iy, 0.5f), lat(dep.z, g.iz, 0.5f), c, A.zero_ext != 0);
   // min/max donors: all eight corners of the departure cell (c.at), solid or not
   fluid_weights<SOLID>(c, A.solid);
   bool ok = false;
   Corners ca;
   if (MC) {
-    V3 rmid = v3(p.x + 0.5f * dt * vp.x, p.y + 0.5f * dt * vp.y, p.z + 0.5f * dt * vp.z);
+    V3 rmid = trace(p, 0.5f * dt, vp);
     V3 va = velocity_at(g, A.V, rmid);
-    V3 arr = v3(p.x + dt * va.x, p.y + dt * va.y, p.z + dt * va.z);
+    V3 arr = trace(p, dt, va);
     ok = true;
     if (SOLID) ok = safe_fast(g, A.solid, A.clear, R, p, x, y, z, dep, rmid, arr);
     float lx = 0.5f * hx, ly = 0.5f * hy, lz = 0.5f * hz;
     float ux = g.ex - 0.5f * hx, uy = g.ey - 0.5f * hy, uz = g.ez - 0.5f * hz;
     ok = ok && !(dep.x < lx || dep.y < ly || dep.z < lz || dep.x > ux || dep.y > uy || dep.z > uz);
     ok = ok && !(arr.x < lx || arr.y < ly || arr.z < lz || arr.x > ux || arr.y > uy || arr.z > uz);
-    closed_corners(g, arr.x * g.ix - 0.5f, arr.y * g.iy - 0.5f, arr.z * g.iz - 0.5f, ca);
+    closed_corners(g, lat(arr.x, g.ix, 0.5f), lat(arr.y, g.iy, 0.5f), lat(arr.z, g.iz, 0.5f), ca);
     fluid_weights<SOLID>(ca, A.solid);
   }
   for (int k = 0; k < A.channels; ++k) {
@@ -810,7 +1032,7 @@ __global__ void k_scalar_advect(ScalarArgs A, fx_sources src) {
     float value = closed_sample<SOLID>(f, c);
     if (MC && ok) {
       float rev = closed_sample<SOLID>(A.pred[k], ca);
-      float corr = ldf(A.pred[k], i) + 0.5f * (ldf(f, i) - rev);
+      float corr = __builtin_fmaf(0.5f, ldf(f, i) - rev, ldf(A.pred[k], i));
       float lo = 3.4e38f, hi = -3.4e38f;
 #pragma unroll
       for (int q = 0; q < 8; ++q) {
@@ -831,7 +1053,7 @@ __global__ void k_scalar_advect(ScalarArgs A, fx_sources src) {
         target = fmaxf(target, rate * wsrc[s]);
         any_target = true;
       } else {
-        value += dt * rate * wsrc[s];
+        value = __builtin_fmaf(dt * rate, wsrc[s], value);
       }
     }
     if (ch == 0 && (src.multi || any_target)) value = fmaxf(value, target);
@@ -886,16 +1108,20 @@ __global__ void k_max_abs(MaxArgs A, float* out) {
 
 static const dim3 kBlock(64, 4, 1);
 
+static std::atomic<int> g_fast_paths{1};
+
 GridDev dev_grid(const fx_grid& g) {
   return GridDev{g.nx,        g.ny,        g.nz,        g.ex,        g.ey,        g.ez,
                  g.ex / g.nx, g.ey / g.ny, g.ez / g.nz, g.nx / g.ex, g.ny / g.ey, g.nz / g.ez,
-                 1.0f / float(g.nx), 1.0f / float(g.nx + 1)};
+                 g_fast_paths.load()};
 }
 
-// transport kernels: 256 threads per block, blocks tile each z-plane of an
-// sx x sy x sz array (thread_site)
+// transport kernels: 256 threads per block, 16 x 16 sites of one z-plane of an
+// sx x sy x sz array per block (thread_site)
 static const dim3 kSiteBlock(256);
-static dim3 site_grid(int sx, int sy, int sz) { return dim3((unsigned(sx) * unsigned(sy) + 255u) / 256u, sz); }
+static dim3 site_grid(int sx, int sy, int sz) {
+  return dim3(((unsigned(sx) + kSiteTile - 1) / kSiteTile) * ((unsigned(sy) + kSiteTile - 1) / kSiteTile), sz);
+}
 
 // 32-bit byte offsets and 24-bit products of the transport kernels: every
 // face array below 2^30 elements, every face plane below 2^24
@@ -926,6 +1152,10 @@ int check_sources(const fx_sources* s) {
 using namespace fx;
 
 extern "C" {
+
+int fx_transport_fast_paths(int32_t enable) {
+  return g_fast_paths.exchange(enable ? 1 : 0);
+}
 
 int fx_face_predict(fx_grid g, float dt, const float* u, const float* v, const float* w, float* pu, float* pv,
                     float* pw, const uint8_t* solid, const uint8_t* clearance, void* stream) {

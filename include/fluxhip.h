@@ -328,6 +328,15 @@ int fx_pcg_solve(fx_mg_config* pcfg, float* x, const float* b, float* r, float* 
  * solid is within reach; results are unchanged.  scratch: n bytes. */
 int fx_build_clearance(fx_grid g, const uint8_t* solid, uint8_t* clearance, uint8_t* scratch, void* stream);
 
+/* Interior fast path of the transport kernels (no reference counterpart; for
+ * tests that compare the two paths): a wave whose sites all lie at least 4
+ * cells from every solid and wall, with traces within 2 cells, skips the
+ * clamps and solid tests, which are identities there; results are the same
+ * bits either way.  enable = 0 sends every site through the general path, 1
+ * (the default) restores the fast path.  Process-wide, read when a transport
+ * entry point is called.  Returns the previous setting. */
+int fx_transport_fast_paths(int32_t enable);
+
 /* Face predictor for U, V and W (DetailPasses.transport_velocity,
  * backend/detail.py:27-37, shaders/transport_face.glsl:1-20). */
 int fx_face_predict(fx_grid g, float dt, const float* u, const float* v, const float* w, float* pu,
