@@ -418,12 +418,21 @@ __device__ __forceinline__ float sample_face_mm_in(const float* __restrict__ F, 
   return tri_blend(t, d);
 }
 
-// velocity_at for a point within 2 cells of an open site
+// velocity_at for a point within 2 cells of an open site.  The 24 donor
+// loads are issued together (the scheduling barrier keeps the compiler from
+// waiting on one component's donors before loading the next component's), so
+// the sample costs one memory round trip.
 __device__ __forceinline__ V3 velocity_at_in(const GridDev& g, const Vel& V, V3 p) {
   float qx = p.x * g.ix, qy = p.y * g.iy, qz = p.z * g.iz;
-  return v3(sample_face_in(V.u, g.nx + 1, g.ny, qx, qy - 0.5f, qz - 0.5f),
-            sample_face_in(V.v, g.nx, g.ny + 1, qx - 0.5f, qy, qz - 0.5f),
-            sample_face_in(V.w, g.nx, g.ny, qx - 0.5f, qy - 0.5f, qz));
+  const Tri tu = tri_face_in(g.nx + 1, g.ny, qx, qy - 0.5f, qz - 0.5f);
+  const Tri tv = tri_face_in(g.nx, g.ny + 1, qx - 0.5f, qy, qz - 0.5f);
+  const Tri tw = tri_face_in(g.nx, g.ny, qx - 0.5f, qy - 0.5f, qz);
+  float du[8], dv[8], dw[8];
+  tri_donors(V.u, tu, du);
+  tri_donors(V.v, tv, dv);
+  tri_donors(V.w, tw, dw);
+  __builtin_amdgcn_sched_barrier(0);
+  return v3(tri_blend(tu, du), tri_blend(tv, dv), tri_blend(tw, dw));
 }
 
 // velocity_at_face at an open site: every neighbour index is in range
@@ -482,6 +491,13 @@ __device__ float emission_weight(V3 p, const float* row) {
   float radius = row[3];
   V3 st = v3(row[4], row[5], row[6]);
   bool solid_profile = row[14] > 0.5f;
+  // A point more than 2 radii outside the box around the swept segment lies
+  // at least 2 radii from the segment: the tests below give 0 for it with a
+  // margin far beyond their rounding, so it is answered here.
+  const float m = 2.0f * radius;
+  if (p.x < fminf(c.x, st.x) - m || p.x > fmaxf(c.x, st.x) + m || p.y < fminf(c.y, st.y) - m ||
+      p.y > fmaxf(c.y, st.y) + m || p.z < fminf(c.z, st.z) - m || p.z > fmaxf(c.z, st.z) + m)
+    return 0.0f;
   float sgx = c.x - st.x, sgy = c.y - st.y, sgz = c.z - st.z;
   float l2 = sgx * sgx + sgy * sgy + sgz * sgz;
   float r2 = radius * radius;
@@ -509,51 +525,61 @@ __device__ float emission_weight(V3 p, const float* row) {
 }
 
 // ---------------------------------------------------------------------------
-// face predictor (transport_face)
+// face transport: one thread per site (x, y, z) of the (nx + 1) x (ny + 1) x
+// (nz + 1) lattice handles the U, V and W faces with that index (each where it
+// exists), so a thread carries three independent gather chains and a wave of
+// open sites takes the interior fast path for all three at once.
 // ---------------------------------------------------------------------------
+template <int AXIS>
+struct FaceAxis {
+  static constexpr float offx = AXIS == 0 ? 0.0f : 0.5f, offy = AXIS == 1 ? 0.0f : 0.5f,
+                         offz = AXIS == 2 ? 0.0f : 0.5f;
+  static __device__ __forceinline__ int sx(const GridDev& g) { return g.nx + (AXIS == 0); }
+  static __device__ __forceinline__ int sy(const GridDev& g) { return g.ny + (AXIS == 1); }
+  static __device__ __forceinline__ int sz(const GridDev& g) { return g.nz + (AXIS == 2); }
+  static __device__ __forceinline__ bool exists(const GridDev& g, int x, int y, int z) {
+    return x < sx(g) && y < sy(g) && z < sz(g);
+  }
+  // byte offset of face (x, y, z)
+  static __device__ __forceinline__ uint32_t at(const GridDev& g, int x, int y, int z) {
+    return lin(x, y, z, sx(g), sy(g)) << 2;
+  }
+  static __device__ __forceinline__ V3 pos(const GridDev& g, int x, int y, int z) {
+    return v3((x + offx) * g.hx, (y + offy) * g.hy, (z + offz) * g.hz);
+  }
+  static __device__ __forceinline__ float qx(const GridDev& g, V3 t) { return lat(t.x, g.ix, offx); }
+  static __device__ __forceinline__ float qy(const GridDev& g, V3 t) { return lat(t.y, g.iy, offy); }
+  static __device__ __forceinline__ float qz(const GridDev& g, V3 t) { return lat(t.z, g.iz, offz); }
+};
+
+struct FaceIO {
+  const float* in[3];
+  float* out[3];
+  float* lo[3];
+  float* hi[3];
+};
+
+// face predictor (transport_face), general path of one face.
 // BOUNDS: also store the donor min/max of the departure sample when that
 // sample may be MacCormack-corrected (departure inside the face lattice, its
 // halo fluid and the trace to it unclipped), else lo = +inf, hi = -inf
 template <int AXIS, bool SOLID, bool BOUNDS>
-__global__ void k_face_predict(GridDev g, Vel V, float dt, const float* __restrict__ in, float* __restrict__ out,
-                               float* __restrict__ lo_out, float* __restrict__ hi_out,
-                               const uint8_t* __restrict__ solid, const uint8_t* __restrict__ clear) {
-  const int sx = g.nx + (AXIS == 0), sy = g.ny + (AXIS == 1), sz = g.nz + (AXIS == 2);
-  uint32_t ux, uy, uz, ue;
-  if (!thread_site(sx, sy, ux, uy, uz, ue)) return;
-  const int x = int(ux), y = int(uy), z = int(uz);
-  const uint32_t i = ue << 2;  // byte offset of this face
-  const float offx = AXIS == 0 ? 0.0f : 0.5f, offy = AXIS == 1 ? 0.0f : 0.5f, offz = AXIS == 2 ? 0.0f : 0.5f;
-  V3 p = v3((x + offx) * g.hx, (y + offy) * g.hy, (z + offz) * g.hz);
-  const Reach R = SOLID ? reach_cell(g, clear, x, y, z) : Reach{-1.0f};
-  if (__all(open_site<SOLID>(g, R, x, y, z))) {
-    const V3 vp = velocity_at_face_in<AXIS>(g, V, x, y, z, i);
-    const V3 mid = trace(p, -0.5f * dt, vp);
-    if (__all(near_site(g, p, mid))) {
-      const V3 vm = velocity_at_in(g, V, mid);
-      const V3 dep = trace(p, -dt, vm);
-      if (__all(near_site(g, p, dep))) {
-        const float qx = lat(dep.x, g.ix, offx), qy = lat(dep.y, g.iy, offy), qz = lat(dep.z, g.iz, offz);
-        if (!BOUNDS) {
-          stf(out, i, sample_face_in(in, sx, sy, qx, qy, qz));
-          return;
-        }
-        float lo, hi;
-        stf(out, i, sample_face_mm_in(in, sx, sy, qx, qy, qz, lo, hi));
-        stf(lo_out, i, lo);
-        stf(hi_out, i, hi);
-        return;
-      }
-    }
-  }
+__device__ void face_predict_general(const GridDev& g, const Vel& V, float dt, const FaceIO& F,
+                                     const uint8_t* __restrict__ solid, const uint8_t* __restrict__ clear,
+                                     const Reach& R, int x, int y, int z) {
+  using FA = FaceAxis<AXIS>;
+  const int sx = FA::sx(g), sy = FA::sy(g), sz = FA::sz(g);
+  const uint32_t i = FA::at(g, x, y, z);
+  const V3 p = FA::pos(g, x, y, z);
+  float* out = F.out[AXIS];
   // clearance >= 2 at the cell above the face: both cells of the face are fluid
   if (SOLID && !(R.c >= 2.0f)) {
     int ox = AXIS == 0, oy = AXIS == 1, oz = AXIS == 2;
     if (solid_cell(g, solid, x, y, z) || solid_cell(g, solid, x - ox, y - oy, z - oz)) {
       stf(out, i, 0.0f);
       if (BOUNDS) {
-        stf(lo_out, i, __builtin_inff());
-        stf(hi_out, i, -__builtin_inff());
+        stf(F.lo[AXIS], i, __builtin_inff());
+        stf(F.hi[AXIS], i, -__builtin_inff());
       }
       return;
     }
@@ -565,18 +591,71 @@ __global__ void k_face_predict(GridDev g, Vel V, float dt, const float* __restri
   V3 dep = trace(p, -dt, vm);
   if (SOLID) dep = clip_fast(g, solid, R, p, dep);
   if (!BOUNDS) {
-    stf(out, i, sample_face(in, sx, sy, sz, lat(dep.x, g.ix, offx), lat(dep.y, g.iy, offy), lat(dep.z, g.iz, offz)));
+    stf(out, i, sample_face(F.in[AXIS], sx, sy, sz, FA::qx(g, dep), FA::qy(g, dep), FA::qz(g, dep)));
     return;
   }
   float lo, hi;
-  stf(out, i, sample_face_mm(in, sx, sy, sz, lat(dep.x, g.ix, offx), lat(dep.y, g.iy, offy), lat(dep.z, g.iz, offz), lo, hi));
-  const float lbx = offx * g.hx, lby = offy * g.hy, lbz = offz * g.hz;
-  const float hbx = (float(sx - 1) + offx) * g.hx, hby = (float(sy - 1) + offy) * g.hy,
-              hbz = (float(sz - 1) + offz) * g.hz;
+  stf(out, i, sample_face_mm(F.in[AXIS], sx, sy, sz, FA::qx(g, dep), FA::qy(g, dep), FA::qz(g, dep), lo, hi));
+  const float lbx = FA::offx * g.hx, lby = FA::offy * g.hy, lbz = FA::offz * g.hz;
+  const float hbx = (float(sx - 1) + FA::offx) * g.hx, hby = (float(sy - 1) + FA::offy) * g.hy,
+              hbz = (float(sz - 1) + FA::offz) * g.hz;
   bool ok = !(dep.x < lbx || dep.y < lby || dep.z < lbz || dep.x > hbx || dep.y > hby || dep.z > hbz);
   if (SOLID && ok) ok = end_free(g, solid, clear, R, p, dep);
-  stf(lo_out, i, ok ? lo : __builtin_inff());
-  stf(hi_out, i, ok ? hi : -__builtin_inff());
+  stf(F.lo[AXIS], i, ok ? lo : __builtin_inff());
+  stf(F.hi[AXIS], i, ok ? hi : -__builtin_inff());
+}
+
+// interior fast path of one face: departure point, then the stores
+template <int AXIS>
+__device__ __forceinline__ V3 face_mid_in(const GridDev& g, const Vel& V, float dt, int x, int y, int z, V3& p,
+                                          uint32_t& i) {
+  using FA = FaceAxis<AXIS>;
+  i = FA::at(g, x, y, z);
+  p = FA::pos(g, x, y, z);
+  return trace(p, -0.5f * dt, velocity_at_face_in<AXIS>(g, V, x, y, z, i));
+}
+
+template <int AXIS, bool BOUNDS>
+__device__ __forceinline__ void face_store_in(const GridDev& g, const FaceIO& F, uint32_t i, V3 dep) {
+  using FA = FaceAxis<AXIS>;
+  const float qx = FA::qx(g, dep), qy = FA::qy(g, dep), qz = FA::qz(g, dep);
+  if (!BOUNDS) {
+    stf(F.out[AXIS], i, sample_face_in(F.in[AXIS], FA::sx(g), FA::sy(g), qx, qy, qz));
+    return;
+  }
+  float lo, hi;
+  stf(F.out[AXIS], i, sample_face_mm_in(F.in[AXIS], FA::sx(g), FA::sy(g), qx, qy, qz, lo, hi));
+  stf(F.lo[AXIS], i, lo);
+  stf(F.hi[AXIS], i, hi);
+}
+
+template <bool SOLID, bool BOUNDS>
+__global__ void k_face_predict(GridDev g, Vel V, float dt, FaceIO F, const uint8_t* __restrict__ solid,
+                               const uint8_t* __restrict__ clear) {
+  uint32_t ux, uy, uz, ue;
+  if (!thread_site(g.nx + 1, g.ny + 1, ux, uy, uz, ue)) return;
+  const int x = int(ux), y = int(uy), z = int(uz);
+  const Reach R = SOLID ? reach_cell(g, clear, x, y, z) : Reach{-1.0f};
+  if (__all(open_site<SOLID>(g, R, x, y, z))) {
+    // an open site has all three faces
+    V3 p0, p1, p2;
+    uint32_t i0, i1, i2;
+    const V3 m0 = face_mid_in<0>(g, V, dt, x, y, z, p0, i0), m1 = face_mid_in<1>(g, V, dt, x, y, z, p1, i1),
+             m2 = face_mid_in<2>(g, V, dt, x, y, z, p2, i2);
+    if (__all(near_site(g, p0, m0) && near_site(g, p1, m1) && near_site(g, p2, m2))) {
+      const V3 d0 = trace(p0, -dt, velocity_at_in(g, V, m0)), d1 = trace(p1, -dt, velocity_at_in(g, V, m1)),
+               d2 = trace(p2, -dt, velocity_at_in(g, V, m2));
+      if (__all(near_site(g, p0, d0) && near_site(g, p1, d1) && near_site(g, p2, d2))) {
+        face_store_in<0, BOUNDS>(g, F, i0, d0);
+        face_store_in<1, BOUNDS>(g, F, i1, d1);
+        face_store_in<2, BOUNDS>(g, F, i2, d2);
+        return;
+      }
+    }
+  }
+  if (FaceAxis<0>::exists(g, x, y, z)) face_predict_general<0, SOLID, BOUNDS>(g, V, dt, F, solid, clear, R, x, y, z);
+  if (FaceAxis<1>::exists(g, x, y, z)) face_predict_general<1, SOLID, BOUNDS>(g, V, dt, F, solid, clear, R, x, y, z);
+  if (FaceAxis<2>::exists(g, x, y, z)) face_predict_general<2, SOLID, BOUNDS>(g, V, dt, F, solid, clear, R, x, y, z);
 }
 
 // ---------------------------------------------------------------------------
@@ -586,75 +665,45 @@ struct VelArgs {
   GridDev g;
   Vel V;
   float dt;
-  const float* pred;
-  float* out;
+  const float* pred[3];
+  float* out[3];
   const float* density;
   const float* temperature;
   float lift, weight;
   const uint8_t* solid;
   const uint8_t* clear;
-  const float* lo;  // donor bounds from the predictor pass (BOUNDS variant)
-  const float* hi;
+  const float* lo[3];  // donor bounds from the predictor pass (BOUNDS variant)
+  const float* hi[3];
 };
 
-// BOUNDS (MacCormack only): the predictor pass already sampled the old field at
-// the departure point and stored the donor bounds, so only the forward trace
-// and the reverse sample remain here.
-template <int AXIS, bool SOLID, bool MC, bool BOUNDS = false>
-__global__ void k_velocity(VelArgs A, fx_sources src) {
+// Transported value of one face before the jet relaxation and buoyancy
+// (general path).  BOUNDS (MacCormack only): the predictor pass already
+// sampled the old field at the departure point and stored the donor bounds,
+// so only the forward trace and the reverse sample remain here.  blocked: a
+// face with a solid cell on either side (value 0, no relaxation).
+template <int AXIS, bool SOLID, bool MC, bool BOUNDS>
+__device__ float velocity_general(const VelArgs& A, const Reach& R, int x, int y, int z, bool& blocked) {
+  using FA = FaceAxis<AXIS>;
   const GridDev& g = A.g;
-  const int sx = g.nx + (AXIS == 0), sy = g.ny + (AXIS == 1), sz = g.nz + (AXIS == 2);
-  uint32_t ux, uy, uz, ue;
-  if (!thread_site(sx, sy, ux, uy, uz, ue)) return;
-  const int x = int(ux), y = int(uy), z = int(uz);
-  const uint32_t i = ue << 2;  // byte offset of this face
+  const int sx = FA::sx(g), sy = FA::sy(g), sz = FA::sz(g);
+  const uint32_t i = FA::at(g, x, y, z);
   const float* orig = AXIS == 0 ? A.V.u : (AXIS == 1 ? A.V.v : A.V.w);
-  const float offx = AXIS == 0 ? 0.0f : 0.5f, offy = AXIS == 1 ? 0.0f : 0.5f, offz = AXIS == 2 ? 0.0f : 0.5f;
+  const float* pred = A.pred[AXIS];
+  const float offx = FA::offx, offy = FA::offy, offz = FA::offz;
   const float hx = g.hx, hy = g.hy, hz = g.hz;
   const float dt = A.dt;
+  const V3 p = FA::pos(g, x, y, z);
   float value = 0.0f;
-  bool blocked = false;
-  V3 p = v3((x + offx) * hx, (y + offy) * hy, (z + offz) * hz);
-  const Reach R = SOLID ? reach_cell(g, A.clear, x, y, z) : Reach{-1.0f};
-  bool fast = false;
-  if ((BOUNDS || !MC) && __all(open_site<SOLID>(g, R, x, y, z))) {
-    const V3 vp = velocity_at_face_in<AXIS>(g, A.V, x, y, z, i);
-    if (BOUNDS) {
-      const V3 rmid = trace(p, 0.5f * dt, vp);
-      if (__all(near_site(g, p, rmid))) {
-        const V3 va = velocity_at_in(g, A.V, rmid);
-        const V3 arr = trace(p, dt, va);
-        if (__all(near_site(g, p, arr))) {
-          const float low = ldf(A.pred, i), lo = ldf(A.lo, i), hi = ldf(A.hi, i);
-          value = low;
-          if (lo <= hi) {
-            const float rev = sample_face_in(A.pred, sx, sy, lat(arr.x, g.ix, offx), lat(arr.y, g.iy, offy), lat(arr.z, g.iz, offz));
-            const float corr = __builtin_fmaf(0.5f, ldf(orig, i) - rev, ldf(A.pred, i));
-            value = __builtin_fmaf(0.5f, low, 0.5f * fminf(fmaxf(corr, lo), hi));
-          }
-          fast = true;
-        }
-      }
-    } else {
-      const V3 mid = trace(p, -0.5f * dt, vp);
-      if (__all(near_site(g, p, mid))) {
-        const V3 vm = velocity_at_in(g, A.V, mid);
-        const V3 dep = trace(p, -dt, vm);
-        if (__all(near_site(g, p, dep))) {
-          value = sample_face_in(orig, sx, sy, lat(dep.x, g.ix, offx), lat(dep.y, g.iy, offy), lat(dep.z, g.iz, offz));
-          fast = true;
-        }
-      }
-    }
-  }
+  blocked = false;
   // clearance >= 2 at the cell above the face: both cells of the face are fluid
-  if (SOLID && !fast && !(R.c >= 2.0f)) {
+  if (SOLID && !(R.c >= 2.0f)) {
     int ox = AXIS == 0, oy = AXIS == 1, oz = AXIS == 2;
     blocked = solid_cell(g, A.solid, x, y, z) || solid_cell(g, A.solid, x - ox, y - oy, z - oz);
   }
-  if (BOUNDS && !fast && !blocked) {
+  if (blocked) return 0.0f;
+  if (BOUNDS) {
     V3 vp = velocity_at_face<AXIS>(g, A.V, x, y, z, i);
-    const float low = ldf(A.pred, i), lo = ldf(A.lo, i), hi = ldf(A.hi, i);
+    const float low = ldf(pred, i), lo = ldf(A.lo[AXIS], i), hi = ldf(A.hi[AXIS], i);
     value = low;
     if (lo <= hi) {
       const float hbx = (float(sx - 1) + offx) * hx, hby = (float(sy - 1) + offy) * hy,
@@ -669,42 +718,81 @@ __global__ void k_velocity(VelArgs A, fx_sources src) {
              dist3(clip_fast(g, A.solid, R, p, rmid), rmid) < 1e-7f &&
              dist3(clip_fast(g, A.solid, R, p, arr), arr) < 1e-7f;
       if (ok) {
-        float rev = sample_face(A.pred, sx, sy, sz, lat(arr.x, g.ix, offx), lat(arr.y, g.iy, offy), lat(arr.z, g.iz, offz));
-        float corr = __builtin_fmaf(0.5f, ldf(orig, i) - rev, ldf(A.pred, i));
+        float rev = sample_face(pred, sx, sy, sz, FA::qx(g, arr), FA::qy(g, arr), FA::qz(g, arr));
+        float corr = __builtin_fmaf(0.5f, ldf(orig, i) - rev, ldf(pred, i));
         value = __builtin_fmaf(0.5f, low, 0.5f * fminf(fmaxf(corr, lo), hi));
       }
     }
+    return value;
   }
-  if (!BOUNDS && !fast && !blocked) {
-    V3 vp = velocity_at_face<AXIS>(g, A.V, x, y, z, i);
-    V3 mid = trace(p, -0.5f * dt, vp);
-    if (SOLID) mid = clip_fast(g, A.solid, R, p, mid);
-    V3 vm = velocity_at(g, A.V, mid);
-    V3 dep = trace(p, -dt, vm);
-    if (SOLID) dep = clip_fast(g, A.solid, R, p, dep);
-    float qx = lat(dep.x, g.ix, offx), qy = lat(dep.y, g.iy, offy), qz = lat(dep.z, g.iz, offz);
-    float lo = 0.0f, hi = 0.0f;
-    float low = MC ? sample_face_mm(orig, sx, sy, sz, qx, qy, qz, lo, hi) : sample_face(orig, sx, sy, sz, qx, qy, qz);
-    value = low;
-    if (MC) {
-      float lbx = offx * hx, lby = offy * hy, lbz = offz * hz;
-      float hbx = (float(sx - 1) + offx) * hx, hby = (float(sy - 1) + offy) * hy, hbz = (float(sz - 1) + offz) * hz;
-      V3 rmid = trace(p, 0.5f * dt, vp);
-      V3 va = velocity_at(g, A.V, rmid);
-      V3 arr = trace(p, dt, va);
-      bool ok = true;
-      if (SOLID) ok = safe_fast(g, A.solid, A.clear, R, p, x, y, z, dep, rmid, arr);
-      ok = ok && !(dep.x < lbx || dep.y < lby || dep.z < lbz || dep.x > hbx || dep.y > hby || dep.z > hbz);
-      ok = ok && !(arr.x < lbx || arr.y < lby || arr.z < lbz || arr.x > hbx || arr.y > hby || arr.z > hbz);
-      if (ok) {
-        // dep lies inside the face lattice here, so the sampled donors are the clamp set
-        float rev = sample_face(A.pred, sx, sy, sz, lat(arr.x, g.ix, offx), lat(arr.y, g.iy, offy), lat(arr.z, g.iz, offz));
-        float corr = __builtin_fmaf(0.5f, ldf(orig, i) - rev, ldf(A.pred, i));
-        value = __builtin_fmaf(0.5f, low, 0.5f * fminf(fmaxf(corr, lo), hi));
-      }
+  V3 vp = velocity_at_face<AXIS>(g, A.V, x, y, z, i);
+  V3 mid = trace(p, -0.5f * dt, vp);
+  if (SOLID) mid = clip_fast(g, A.solid, R, p, mid);
+  V3 vm = velocity_at(g, A.V, mid);
+  V3 dep = trace(p, -dt, vm);
+  if (SOLID) dep = clip_fast(g, A.solid, R, p, dep);
+  float qx = FA::qx(g, dep), qy = FA::qy(g, dep), qz = FA::qz(g, dep);
+  float lo = 0.0f, hi = 0.0f;
+  float low = MC ? sample_face_mm(orig, sx, sy, sz, qx, qy, qz, lo, hi) : sample_face(orig, sx, sy, sz, qx, qy, qz);
+  value = low;
+  if (MC) {
+    float lbx = offx * hx, lby = offy * hy, lbz = offz * hz;
+    float hbx = (float(sx - 1) + offx) * hx, hby = (float(sy - 1) + offy) * hy, hbz = (float(sz - 1) + offz) * hz;
+    V3 rmid = trace(p, 0.5f * dt, vp);
+    V3 va = velocity_at(g, A.V, rmid);
+    V3 arr = trace(p, dt, va);
+    bool ok = true;
+    if (SOLID) ok = safe_fast(g, A.solid, A.clear, R, p, x, y, z, dep, rmid, arr);
+    ok = ok && !(dep.x < lbx || dep.y < lby || dep.z < lbz || dep.x > hbx || dep.y > hby || dep.z > hbz);
+    ok = ok && !(arr.x < lbx || arr.y < lby || arr.z < lbz || arr.x > hbx || arr.y > hby || arr.z > hbz);
+    if (ok) {
+      // dep lies inside the face lattice here, so the sampled donors are the clamp set
+      float rev = sample_face(pred, sx, sy, sz, FA::qx(g, arr), FA::qy(g, arr), FA::qz(g, arr));
+      float corr = __builtin_fmaf(0.5f, ldf(orig, i) - rev, ldf(pred, i));
+      value = __builtin_fmaf(0.5f, low, 0.5f * fminf(fmaxf(corr, lo), hi));
     }
   }
+  return value;
+}
+
+// interior fast path of one face: the forward trace's midpoint (BOUNDS) or
+// the back-trace's (semi-Lagrangian)
+template <int AXIS, bool BOUNDS>
+__device__ __forceinline__ V3 velocity_mid_in(const VelArgs& A, int x, int y, int z, V3& p, uint32_t& i) {
+  using FA = FaceAxis<AXIS>;
+  i = FA::at(A.g, x, y, z);
+  p = FA::pos(A.g, x, y, z);
+  return trace(p, (BOUNDS ? 0.5f : -0.5f) * A.dt, velocity_at_face_in<AXIS>(A.g, A.V, x, y, z, i));
+}
+
+// value of one face from its interior trace end t (arrival for BOUNDS,
+// departure for semi-Lagrangian)
+template <int AXIS, bool BOUNDS>
+__device__ __forceinline__ float velocity_value_in(const VelArgs& A, uint32_t i, V3 t) {
+  using FA = FaceAxis<AXIS>;
+  const GridDev& g = A.g;
+  const float* orig = AXIS == 0 ? A.V.u : (AXIS == 1 ? A.V.v : A.V.w);
+  if (!BOUNDS) return sample_face_in(orig, FA::sx(g), FA::sy(g), FA::qx(g, t), FA::qy(g, t), FA::qz(g, t));
+  const float* pred = A.pred[AXIS];
+  const float low = ldf(pred, i), lo = ldf(A.lo[AXIS], i), hi = ldf(A.hi[AXIS], i);
+  float value = low;
+  if (lo <= hi) {
+    const float rev = sample_face_in(pred, FA::sx(g), FA::sy(g), FA::qx(g, t), FA::qy(g, t), FA::qz(g, t));
+    const float corr = __builtin_fmaf(0.5f, ldf(orig, i) - rev, ldf(pred, i));
+    value = __builtin_fmaf(0.5f, low, 0.5f * fminf(fmaxf(corr, lo), hi));
+  }
+  return value;
+}
+
+// jet relaxation (unblocked faces) and buoyancy (W), then the store
+template <int AXIS>
+__device__ __forceinline__ void velocity_finish(const VelArgs& A, const fx_sources& src, int x, int y, int z,
+                                                float value, bool blocked) {
+  using FA = FaceAxis<AXIS>;
+  const GridDev& g = A.g;
+  const float dt = A.dt;
   if (!blocked) {
+    const V3 p = FA::pos(g, x, y, z);
     // jet relaxation: coupling rates summed first (order independent)
     float total = 0.0f, weighted = 0.0f;
     for (int s = 0; s < src.count; ++s) {
@@ -725,7 +813,48 @@ __global__ void k_velocity(VelArgs A, fx_sources src) {
     float theta = 0.5f * (ldf(A.temperature, ib) + ldf(A.temperature, ia));
     value = __builtin_fmaf(dt, A.lift * theta - A.weight * rho, value);
   }
-  stf(A.out, i, value);
+  stf(A.out[AXIS], FA::at(g, x, y, z), value);
+}
+
+template <bool SOLID, bool MC, bool BOUNDS>
+__global__ void k_velocity(VelArgs A, fx_sources src) {
+  const GridDev& g = A.g;
+  uint32_t ux, uy, uz, ue;
+  if (!thread_site(g.nx + 1, g.ny + 1, ux, uy, uz, ue)) return;
+  const int x = int(ux), y = int(uy), z = int(uz);
+  const Reach R = SOLID ? reach_cell(g, A.clear, x, y, z) : Reach{-1.0f};
+  if ((BOUNDS || !MC) && __all(open_site<SOLID>(g, R, x, y, z))) {
+    V3 p0, p1, p2;
+    uint32_t i0, i1, i2;
+    const V3 m0 = velocity_mid_in<0, BOUNDS>(A, x, y, z, p0, i0), m1 = velocity_mid_in<1, BOUNDS>(A, x, y, z, p1, i1),
+             m2 = velocity_mid_in<2, BOUNDS>(A, x, y, z, p2, i2);
+    if (__all(near_site(g, p0, m0) && near_site(g, p1, m1) && near_site(g, p2, m2))) {
+      const float s = BOUNDS ? A.dt : -A.dt;
+      const V3 t0 = trace(p0, s, velocity_at_in(g, A.V, m0)), t1 = trace(p1, s, velocity_at_in(g, A.V, m1)),
+               t2 = trace(p2, s, velocity_at_in(g, A.V, m2));
+      if (__all(near_site(g, p0, t0) && near_site(g, p1, t1) && near_site(g, p2, t2))) {
+        const float v0 = velocity_value_in<0, BOUNDS>(A, i0, t0), v1 = velocity_value_in<1, BOUNDS>(A, i1, t1),
+                    v2 = velocity_value_in<2, BOUNDS>(A, i2, t2);
+        velocity_finish<0>(A, src, x, y, z, v0, false);
+        velocity_finish<1>(A, src, x, y, z, v1, false);
+        velocity_finish<2>(A, src, x, y, z, v2, false);
+        return;
+      }
+    }
+  }
+  bool blocked;
+  if (FaceAxis<0>::exists(g, x, y, z)) {
+    const float v = velocity_general<0, SOLID, MC, BOUNDS>(A, R, x, y, z, blocked);
+    velocity_finish<0>(A, src, x, y, z, v, blocked);
+  }
+  if (FaceAxis<1>::exists(g, x, y, z)) {
+    const float v = velocity_general<1, SOLID, MC, BOUNDS>(A, R, x, y, z, blocked);
+    velocity_finish<1>(A, src, x, y, z, v, blocked);
+  }
+  if (FaceAxis<2>::exists(g, x, y, z)) {
+    const float v = velocity_general<2, SOLID, MC, BOUNDS>(A, R, x, y, z, blocked);
+    velocity_finish<2>(A, src, x, y, z, v, blocked);
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -808,6 +937,22 @@ __device__ __forceinline__ void closed_corners_in(const GridDev& g, float qx, fl
   for (int k = 0; k < 8; ++k) c.w[k] = wx[k & 1] * wy[(k >> 1) & 1] * wz[k >> 2];
 }
 
+// the eight corner values of a corner set, in corner order
+__device__ __forceinline__ void corner_values(const float* __restrict__ f, const Corners& c, float (&d)[8]) {
+#pragma unroll
+  for (int k = 0; k < 8; ++k) d[k] = ldf(f, c.at(k) << 2);
+}
+
+// closed_sample on loaded corner values
+template <bool SOLID>
+__device__ __forceinline__ float weighted_sample(const Corners& c, const float (&d)[8]) {
+  float v = 0.0f;
+#pragma unroll
+  for (int k = 0; k < 8; ++k) v = __builtin_fmaf(c.w[k], d[k], v);
+  if (!SOLID) return v;
+  return c.norm > 0.0f ? v * c.norm : 0.0f;
+}
+
 // fluid_weights when all eight corners are fluid
 template <bool SOLID>
 __device__ __forceinline__ void fluid_norm(Corners& c) {
@@ -849,15 +994,15 @@ __global__ void k_scalar_predict(GridDev g, Vel V, float dt, int channels, Scala
         closed_corners_in(g, lat(dep.x, g.ix, 0.5f), lat(dep.y, g.iy, 0.5f), lat(dep.z, g.iz, 0.5f), c);
         fluid_norm<SOLID>(c);
         for (int k = 0; k < channels; ++k) {
-          const float* f = S_.f[k];
-          stf(S_.o[k], i, closed_sample<SOLID>(f, c));
+          float d[8];
+          corner_values(S_.f[k], c, d);
+          stf(S_.o[k], i, weighted_sample<SOLID>(c, d));
           if (BOUNDS) {
             float lo = __builtin_inff(), hi = -__builtin_inff();
 #pragma unroll
             for (int q = 0; q < 8; ++q) {
-              const float d = ldf(f, c.at(q) << 2);
-              lo = fminf(lo, d);
-              hi = fmaxf(hi, d);
+              lo = fminf(lo, d[q]);
+              hi = fmaxf(hi, d[q]);
             }
             stf(S_.lo[k], i, lo);
             stf(S_.hi[k], i, hi);
@@ -980,7 +1125,14 @@ __global__ void k_scalar_advect(ScalarArgs A, fx_sources src) {
       const float low = ldf(A.pred[k], i), lo = ldf(A.lo[k], i), hi = ldf(A.hi[k], i);
       float value = low;
       if (ok_arr && lo <= hi) {
-        float rev = closed_sample<SOLID>(A.pred[k], ca);
+        float rev;
+        if (fast) {
+          float d[8];
+          corner_values(A.pred[k], ca, d);
+          rev = weighted_sample<SOLID>(ca, d);
+        } else {
+          rev = closed_sample<SOLID>(A.pred[k], ca);
+        }
         float corr = __builtin_fmaf(0.5f, ldf(A.f[k], i) - rev, low);
         value = fminf(fmaxf(corr, lo), hi);
       }
@@ -1131,8 +1283,8 @@ int check_transport_grid(const fx_grid& g) {
   FX_REQUIRE((nx + 1) * ny * nz < (int64_t(1) << 30) && nx * (ny + 1) * nz < (int64_t(1) << 30) &&
                  nx * ny * (nz + 1) < (int64_t(1) << 30),
              FX_EINVAL, "transport arrays are limited to 2^30 elements (grid %dx%dx%d)", g.nx, g.ny, g.nz);
-  FX_REQUIRE((nx + 1) * (ny + 1) < (int64_t(1) << 24) && nz < 65536, FX_EINVAL,
-             "transport planes are limited to 2^24 faces and 65535 planes (grid %dx%dx%d)", g.nx, g.ny, g.nz);
+  FX_REQUIRE((nx + 1) * (ny + 1) < (int64_t(1) << 24) && nz + 1 < 65536, FX_EINVAL,
+             "transport planes are limited to 2^24 faces and 65534 cells along z (grid %dx%dx%d)", g.nx, g.ny, g.nz);
   return FX_OK;
 }
 
@@ -1166,22 +1318,12 @@ int fx_face_predict(fx_grid g, float dt, const float* u, const float* v, const f
   FX_REQUIRE(std::isfinite(dt), FX_EINVAL, "dt must be finite");
   GridDev G = dev_grid(g);
   Vel V{u, v, w};
-  hipStream_t s = S(stream);
-  dim3 gx = site_grid(g.nx + 1, g.ny, g.nz), gy = site_grid(g.nx, g.ny + 1, g.nz), gz = site_grid(g.nx, g.ny, g.nz + 1);
-  const dim3 kBlock = kSiteBlock;
-  float* nul = nullptr;
-  if (solid) {
-    hipLaunchKernelGGL((k_face_predict<0, true, false>), gx, kBlock, 0, s, G, V, dt, u, pu, nul, nul, solid, clearance);
-    hipLaunchKernelGGL((k_face_predict<1, true, false>), gy, kBlock, 0, s, G, V, dt, v, pv, nul, nul, solid, clearance);
-    hipLaunchKernelGGL((k_face_predict<2, true, false>), gz, kBlock, 0, s, G, V, dt, w, pw, nul, nul, solid, clearance);
-  } else {
-    hipLaunchKernelGGL((k_face_predict<0, false, false>), gx, kBlock, 0, s, G, V, dt, u, pu, nul, nul, solid,
-                       clearance);
-    hipLaunchKernelGGL((k_face_predict<1, false, false>), gy, kBlock, 0, s, G, V, dt, v, pv, nul, nul, solid,
-                       clearance);
-    hipLaunchKernelGGL((k_face_predict<2, false, false>), gz, kBlock, 0, s, G, V, dt, w, pw, nul, nul, solid,
-                       clearance);
-  }
+  FaceIO F{{u, v, w}, {pu, pv, pw}, {nullptr, nullptr, nullptr}, {nullptr, nullptr, nullptr}};
+  const dim3 grid = site_grid(g.nx + 1, g.ny + 1, g.nz + 1);
+  if (solid)
+    hipLaunchKernelGGL((k_face_predict<true, false>), grid, kSiteBlock, 0, S(stream), G, V, dt, F, solid, clearance);
+  else
+    hipLaunchKernelGGL((k_face_predict<false, false>), grid, kSiteBlock, 0, S(stream), G, V, dt, F, solid, clearance);
   FX_LAUNCHED("fx_face_predict");
   return FX_OK;
 }
@@ -1202,24 +1344,12 @@ int fx_face_predict_bounds(fx_grid g, float dt, const float* u, const float* v, 
   FX_REQUIRE(std::isfinite(dt), FX_EINVAL, "dt must be finite");
   GridDev G = dev_grid(g);
   Vel V{u, v, w};
-  hipStream_t s = S(stream);
-  dim3 gx = site_grid(g.nx + 1, g.ny, g.nz), gy = site_grid(g.nx, g.ny + 1, g.nz), gz = site_grid(g.nx, g.ny, g.nz + 1);
-  const dim3 kBlock = kSiteBlock;
-  if (solid) {
-    hipLaunchKernelGGL((k_face_predict<0, true, true>), gx, kBlock, 0, s, G, V, dt, u, pred[0], lo[0], hi[0], solid,
-                       clearance);
-    hipLaunchKernelGGL((k_face_predict<1, true, true>), gy, kBlock, 0, s, G, V, dt, v, pred[1], lo[1], hi[1], solid,
-                       clearance);
-    hipLaunchKernelGGL((k_face_predict<2, true, true>), gz, kBlock, 0, s, G, V, dt, w, pred[2], lo[2], hi[2], solid,
-                       clearance);
-  } else {
-    hipLaunchKernelGGL((k_face_predict<0, false, true>), gx, kBlock, 0, s, G, V, dt, u, pred[0], lo[0], hi[0], solid,
-                       clearance);
-    hipLaunchKernelGGL((k_face_predict<1, false, true>), gy, kBlock, 0, s, G, V, dt, v, pred[1], lo[1], hi[1], solid,
-                       clearance);
-    hipLaunchKernelGGL((k_face_predict<2, false, true>), gz, kBlock, 0, s, G, V, dt, w, pred[2], lo[2], hi[2], solid,
-                       clearance);
-  }
+  FaceIO F{{u, v, w}, {pred[0], pred[1], pred[2]}, {lo[0], lo[1], lo[2]}, {hi[0], hi[1], hi[2]}};
+  const dim3 grid = site_grid(g.nx + 1, g.ny + 1, g.nz + 1);
+  if (solid)
+    hipLaunchKernelGGL((k_face_predict<true, true>), grid, kSiteBlock, 0, S(stream), G, V, dt, F, solid, clearance);
+  else
+    hipLaunchKernelGGL((k_face_predict<false, true>), grid, kSiteBlock, 0, S(stream), G, V, dt, F, solid, clearance);
   FX_LAUNCHED("fx_face_predict_bounds");
   return FX_OK;
 }
@@ -1240,31 +1370,32 @@ int fx_velocity_advect(const fx_velocity_args* a, void* stream) {
   Vel V{a->u, a->v, a->w};
   hipStream_t s = S(stream);
   const fx_grid& g = a->g;
-  dim3 grids[3] = {site_grid(g.nx + 1, g.ny, g.nz), site_grid(g.nx, g.ny + 1, g.nz), site_grid(g.nx, g.ny, g.nz + 1)};
-  const dim3 kBlock = kSiteBlock;
-  const float* preds[3] = {a->pu, a->pv, a->pw};
-  float* o3[3] = {a->ou, a->ov, a->ow};
-  const float* los[3] = {a->lu, a->lv, a->lw};
-  const float* his[3] = {a->hu, a->hv, a->hw};
   const bool bounds = a->maccormack && a->lu && a->lv && a->lw && a->hu && a->hv && a->hw;
-  for (int axis = 0; axis < 3; ++axis) {
-    VelArgs A{G, V, a->dt, preds[axis], o3[axis], a->density, a->temperature, a->thermal_lift, a->density_weight,
-              a->solid, a->solid ? a->clearance : nullptr, bounds ? los[axis] : nullptr,
-              bounds ? his[axis] : nullptr};
-#define FX_VEL(AX, SO, MC, BO) \
-  hipLaunchKernelGGL((k_velocity<AX, SO, MC, BO>), grids[AX], kBlock, 0, s, A, *a->sources)
-#define FX_VEL_AX(AX)                                                                             \
-  if (a->solid) {                                                                                 \
-    if (bounds) FX_VEL(AX, true, true, true);                                                     \
-    else if (a->maccormack) FX_VEL(AX, true, true, false); else FX_VEL(AX, true, false, false);   \
-  } else {                                                                                        \
-    if (bounds) FX_VEL(AX, false, true, true);                                                    \
-    else if (a->maccormack) FX_VEL(AX, false, true, false); else FX_VEL(AX, false, false, false); \
+  VelArgs A{G,
+            V,
+            a->dt,
+            {a->pu, a->pv, a->pw},
+            {a->ou, a->ov, a->ow},
+            a->density,
+            a->temperature,
+            a->thermal_lift,
+            a->density_weight,
+            a->solid,
+            a->solid ? a->clearance : nullptr,
+            {bounds ? a->lu : nullptr, bounds ? a->lv : nullptr, bounds ? a->lw : nullptr},
+            {bounds ? a->hu : nullptr, bounds ? a->hv : nullptr, bounds ? a->hw : nullptr}};
+  const dim3 grid = site_grid(g.nx + 1, g.ny + 1, g.nz + 1);
+#define FX_VEL(SO, MC, BO) hipLaunchKernelGGL((k_velocity<SO, MC, BO>), grid, kSiteBlock, 0, s, A, *a->sources)
+  if (a->solid) {
+    if (bounds) FX_VEL(true, true, true);
+    else if (a->maccormack) FX_VEL(true, true, false);
+    else FX_VEL(true, false, false);
+  } else {
+    if (bounds) FX_VEL(false, true, true);
+    else if (a->maccormack) FX_VEL(false, true, false);
+    else FX_VEL(false, false, false);
   }
-    if (axis == 0) { FX_VEL_AX(0) } else if (axis == 1) { FX_VEL_AX(1) } else { FX_VEL_AX(2) }
-#undef FX_VEL_AX
 #undef FX_VEL
-  }
   FX_LAUNCHED("fx_velocity_advect");
   return FX_OK;
 }
