@@ -97,8 +97,20 @@ class fx_bricks_dev(C.Structure):
                 ("reallocations", C.c_uint32)]
 
 
+class fx_projection_report(C.Structure):
+    _fields_ = ([(k, C.c_uint32) for k in ("resolution", "side", "iterations", "mode", "cycles", "levels",
+                                            "active_bricks", "solve_cells")] +
+                [(k, C.c_uint64) for k in ("sparse_bytes", "dense_bytes", "sparse_hierarchy_bytes",
+                                            "dense_hierarchy_bytes")] +
+                [(k, C.c_double) for k in ("topology_ms", "sparse_gpu_ms", "dense_gpu_ms", "max_error",
+                                            "pressure_error", "divergence_error", "before_rms", "after_rms",
+                                            "residual_rms", "wall_error", "padding_error")])
+
+
 SIGNATURES = {
     "fx_last_error": ([], C.c_char_p),
+    "fx_projection_compare": ([U32, U32, U32, U32, U32, U64, P, P, P, P, C.POINTER(fx_projection_report), P],
+                              C.c_int),
     "fx_version": ([], C.c_int),
     "fx_solid_raster": ([fx_grid, I32, P, P, P], C.c_int),
     "fx_build_flags": ([fx_grid, P, P, P], C.c_int),

@@ -285,3 +285,44 @@ class BrickPool:
             return torch.cuda.get_device_name(0) if torch.cuda.is_available() else "none"
         except Exception:  # noqa: BLE001 - a report field, not an error path
             return "none"
+
+
+# ---------------------------------------------------------------------------
+# projection on brick storage (native/__init__.py:70-81)
+# ---------------------------------------------------------------------------
+def _projection(resolution, side, iterations, cycles, mode, budget_bytes):
+    for name, v in (("resolution", resolution), ("side", side), ("iterations", iterations), ("cycles", cycles),
+                    ("mode", mode), ("budget_bytes", budget_bytes)):
+        if type(v) is not int:
+            raise TypeError(f"{name} must be an int")
+    if (resolution not in (32, 64, 128) or side not in (8, 16) or not 0 <= iterations <= 1024 or
+            not 0 <= cycles <= 16 or not 0 <= mode <= 4 or not 0 < budget_bytes <= 2 ** 30):
+        raise ValueError("Projection: n 32/64/128, brick 8/16, iterations 0..1024, mode 0..4, budget 1..1GiB")
+    n = resolution
+    faces = np.empty(3 * (n + 1) * n * n, np.float32)
+    cells = [np.empty(n ** 3, np.float32) for _ in range(3)]
+    rep = _lib.fx_projection_report()
+    _lib.call("fx_projection_compare", n, side, iterations, cycles, mode, budget_bytes, faces.ctypes.data,
+              *(c.ctypes.data for c in cells), C.byref(rep), BrickPool._stream())
+    out = {name: getattr(rep, name) for name, _ in rep._fields_}
+    out.update(faces=faces.tobytes(), pressure=cells[0].tobytes(), before=cells[1].tobytes(),
+               after=cells[2].tobytes())
+    return out
+
+
+def compare_projection(resolution=64, side=8, iterations=128, mode=1, budget_bytes=256 * 2 ** 20):
+    """Fixed-region unit-cell impulse projection by damped Jacobi on brick and
+    dense storage; explicit readback (native/__init__.py:70-73).  The dict
+    carries the reference's fields: sizes, timings, brick-vs-dense errors,
+    RMS divergence before/after and of the residual, wall and padding errors,
+    and the brick engine's faces/pressure/before/after (float32 bytes, dense
+    x-fast layout)."""
+    return _projection(resolution, side, iterations, 0, mode, budget_bytes)
+
+
+def compare_multigrid(resolution=64, side=8, cycles=4, mode=1, budget_bytes=256 * 2 ** 20):
+    """Fixed-region geometric V-cycles on brick and dense storage, with
+    diagnostic readback (native/__init__.py:76-81)."""
+    if type(cycles) is not int or not 1 <= cycles <= 16:
+        raise ValueError("Use 1..16 multigrid cycles")
+    return _projection(resolution, side, 0, cycles, mode, budget_bytes)

@@ -8,8 +8,10 @@
  * one exception is the brick pool handle (fx_bricks_*): it owns its device
  * mirror and allocates it in fx_bricks_sync (on capacity growth, which also
  * synchronises the device) and fx_bricks_transport (its second payload buffer
- * and brick map, on first use or growth).  `stream` is a hipStream_t
- * (NULL = default).
+ * and brick map, on first use or growth); and the brick projection comparison
+ * (fx_projection_compare), a diagnostic that allocates both of its engines
+ * within a byte budget and frees them before returning.  `stream` is a
+ * hipStream_t (NULL = default).
  *
  * Layout everywhere is the reference's flat x-fast index x + nx*(y + ny*z)
  * (physics/reference.py:3).  Face arrays are U (nx+1,ny,nz), V (nx,ny+1,nz),
@@ -638,6 +640,43 @@ int fx_bricks_transport(fx_bricks* b, int32_t n, const float* u, const float* v,
  * a sentinel, writes a per-voxel pattern of (slot, voxel, neighbour ids) into
  * active bricks and verifies all slots on the host. */
 int fx_bricks_probe(fx_bricks* b, uint32_t* mismatches, float* gpu_ms, void* stream);
+
+/* --------------------------------------------------------------------------
+ * Projection on brick storage (native/projection.mm:9-168 kernels, 169-462
+ * engine and compare_projection; Python surface native/__init__.py:70-81
+ * compare_projection / compare_multigrid).  Unit cells, impulse pressure,
+ * a fixed analytic solve region (mode 0 whole box, 1 central half, 2 two
+ * boxes, 3 zero velocity, 4 gradient of a cosine q), Neumann outer walls,
+ * q = 0 outside the region and on missing bricks.  Bricks hold both faces of
+ * their cells along each axis; the face at local index side along an axis
+ * belongs to the next brick (canonical ownership) and is never written
+ * unless the brick is the last along that axis.  The sparse engine (bricks
+ * within one cell of the region) and a dense engine run the same seeded
+ * problem: divergence, `iterations` damped-Jacobi sweeps (omega 2/3) or, with
+ * cycles > 0, that many V-cycles (4 + 4 sweeps, 4 x child mean restriction,
+ * clamped trilinear prolongation, 128 sweeps at 8^3, levels down to 8^3),
+ * gradient correction, divergence after.
+ * ------------------------------------------------------------------------ */
+typedef struct fx_projection_report {
+  uint32_t resolution, side, iterations, mode, cycles, levels, active_bricks, solve_cells;
+  uint64_t sparse_bytes, dense_bytes, sparse_hierarchy_bytes, dense_hierarchy_bytes;
+  double topology_ms, sparse_gpu_ms, dense_gpu_ms;
+  double max_error;        /* max |brick - dense| over the projected faces */
+  double pressure_error;   /* max |brick - dense| over q */
+  double divergence_error; /* max |brick - dense| over the divergence after */
+  double before_rms, after_rms, residual_rms;  /* over solve cells; residual from the exported q */
+  double wall_error;       /* max |face| on the outer walls */
+  double padding_error;    /* max |padding face - 12345| (never written) */
+} fx_projection_report;
+
+/* compare_projection / compare_multigrid (projection.mm:370-462): n 32/64/128,
+ * side 8/16, iterations 0..1024, cycles 0..16, mode 0..4, budget 1 B..1 GiB per
+ * engine (checked before any allocation).  Host outputs from the brick engine
+ * in dense layout: faces 3*(n+1)*n^2 (axis-major, each x-fast with n+1 along
+ * its axis), pressure, before (divergence) and after, n^3 each. */
+int fx_projection_compare(uint32_t n, uint32_t side, uint32_t iterations, uint32_t cycles, uint32_t mode,
+                          uint64_t budget, float* faces, float* pressure, float* before, float* after,
+                          fx_projection_report* report, void* stream);
 
 #ifdef __cplusplus
 }

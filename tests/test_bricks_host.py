@@ -357,3 +357,39 @@ def test_brick_shapes_are_checked_before_any_launch():
     # right sizes, but host tensors: refused as not on the GPU
     with pytest.raises(ValueError, match="cuda"):
         BrickPool._faces(n, *faces)
+
+
+def test_projection_arguments_are_checked_before_any_allocation():
+    """compare_projection / compare_multigrid reject bad arguments in Python,
+    and fx_projection_compare checks sizes and the per-engine budget on the
+    host before its first HIP call (native/projection.mm:360-372), so these
+    run without a GPU."""
+    import ctypes as C
+    import numpy as np
+    from fluxfx_amd import _lib
+    from fluxfx_amd.native import compare_multigrid, compare_projection
+    for kw in ({"resolution": 48}, {"side": 4}, {"iterations": 1025}, {"mode": 5}, {"budget_bytes": 0},
+               {"budget_bytes": 2 ** 30 + 1}):
+        with pytest.raises(ValueError, match="Projection: n 32/64/128"):
+            compare_projection(**kw)
+    with pytest.raises(TypeError):
+        compare_projection(resolution=32.0)
+    for cycles in (0, 17, 2.0):
+        with pytest.raises(ValueError, match="1..16 multigrid cycles"):
+            compare_multigrid(cycles=cycles)
+    if not _lib.LIB_PATH.exists():
+        pytest.skip("libfluxhip.so is not built")
+    n = 128
+    faces = np.empty(3 * (n + 1) * n * n, np.float32)
+    cells = [np.empty(n ** 3, np.float32) for _ in range(3)]
+    rep = _lib.fx_projection_report()
+    args = [faces.ctypes.data, *(c.ctypes.data for c in cells), C.byref(rep), None]
+    with pytest.raises(ValueError, match="exceeds per-engine buffer budget"):
+        _lib.call("fx_projection_compare", n, 8, 16, 0, 1, 2 ** 20, *args)
+    # mode 0 at n=128: the fine brick engine takes 90,243,072 B (2 x 4096 x 9 x 64
+    # faces x 12 B + 4 x 2^21 cells x 4 B + map and coordinates); 91 MB holds it
+    # but not its V-cycle levels (4,802,560 B more)
+    with pytest.raises(ValueError, match="hierarchy exceeds"):
+        _lib.call("fx_projection_compare", n, 8, 0, 4, 0, 91_000_000, *args)
+    with pytest.raises(ValueError, match="Projection: n 32/64/128"):
+        _lib.call("fx_projection_compare", 256, 8, 16, 0, 1, 2 ** 20, *args)
