@@ -107,10 +107,36 @@ class fx_projection_report(C.Structure):
                                             "residual_rms", "wall_error", "padding_error")])
 
 
+class fx_coupled_report(C.Structure):
+    _fields_ = ([("projection", fx_projection_report)] +
+                [(k, C.c_uint32) for k in ("steps", "schedule", "dense_full", "global_support", "hybrid", "adaptive",
+                                            "pooled", "expansions", "density_bricks", "density_capacity",
+                                            "density_reallocations", "density_reuses")] +
+                [(k, C.c_uint64) for k in ("density_cells", "density_bytes", "preserved_bytes", "global_copy_bytes",
+                                            "peak_sparse_bytes")] +
+                [(k, C.c_double) for k in ("density_error", "sparse_mass", "dense_mass", "injected_mass",
+                                            "dense_after_rms", "density_topology_ms", "preflight_ms", "sparse_wall_ms",
+                                            "dense_wall_ms")])
+
+
+class fx_advect_report(C.Structure):
+    _fields_ = ([(k, C.c_uint32) for k in ("resolution", "side", "steps", "mode", "active_bricks", "pad_")] +
+                [(k, C.c_uint64) for k in ("sparse_bytes", "dense_bytes", "owned_faces", "padding_faces")] +
+                [(k, C.c_double) for k in ("max_error", "rms_error", "padding_error", "topology_ms", "sparse_wall_ms",
+                                            "dense_wall_ms", "sparse_mass", "dense_mass", "initial_mass")] +
+                [("sparse_gpu_ms", C.c_double * 32), ("dense_gpu_ms", C.c_double * 32)])
+
+
+COUPLED_OPTIONS = {"dense_full": 1, "global_support": 2, "hybrid": 4, "adaptive": 8, "pooled": 16}
+
 SIGNATURES = {
     "fx_last_error": ([], C.c_char_p),
     "fx_projection_compare": ([U32, U32, U32, U32, U32, U64, P, P, P, P, C.POINTER(fx_projection_report), P],
                               C.c_int),
+    "fx_coupled_compare": ([U32, U32, U32, U32, U32, U32, U32, U64, P, P, P, P, P, C.POINTER(fx_coupled_report), P],
+                           C.c_int),
+    "fx_transport_compare": ([U32, U32, U32, F32, F32, F32, U64, P, C.POINTER(fx_advect_report), P], C.c_int),
+    "fx_mac_compare": ([U32, U32, U32, U32, U64, P, C.POINTER(fx_advect_report), P], C.c_int),
     "fx_version": ([], C.c_int),
     "fx_solid_raster": ([fx_grid, I32, P, P, P], C.c_int),
     "fx_build_flags": ([fx_grid, P, P, P], C.c_int),

@@ -6,7 +6,11 @@ device-side operations of this build: ``sync`` (topology upload and the
 GPU-built neighbour table), ``gather``/``scatter`` between a dense field and
 the active bricks, ``support`` (per-brick occupancy of a dense field),
 ``required`` (the reference's transport-aware preflight), ``activate_flags``
-and ``reserve`` (capacity doubling).
+and ``reserve`` (capacity doubling).  The module-level ``compare_*``
+functions are the reference's brick diagnostics (native/__init__.py:69-166):
+scalar and MAC transport, projection and V-cycles, and the coupled step with
+its global, hybrid, adaptive and capacity variants, each run on bricks and
+on dense storage with the comparison figures and the brick results.
 
 Topology and region commands run on the host and need no GPU; ``sync``,
 ``probe``, ``gather``, ``scatter`` and ``support`` run on ``cuda:0`` (the
@@ -290,11 +294,28 @@ class BrickPool:
 # ---------------------------------------------------------------------------
 # projection on brick storage (native/__init__.py:70-81)
 # ---------------------------------------------------------------------------
-def _projection(resolution, side, iterations, cycles, mode, budget_bytes):
-    for name, v in (("resolution", resolution), ("side", side), ("iterations", iterations), ("cycles", cycles),
-                    ("mode", mode), ("budget_bytes", budget_bytes)):
+def _ints(**values):
+    for name, v in values.items():
         if type(v) is not int:
             raise TypeError(f"{name} must be an int")
+
+
+def _report(rep):
+    """Flat dict of a report structure (nested reports merged, arrays as lists)."""
+    out = {}
+    for name, _ in rep._fields_:
+        v = getattr(rep, name)
+        if isinstance(v, C.Structure):
+            out.update(_report(v))
+        elif isinstance(v, C.Array):
+            out[name] = list(v)
+        elif name != "pad_":
+            out[name] = v
+    return out
+
+
+def _projection(resolution, side, iterations, cycles, mode, budget_bytes):
+    _ints(resolution=resolution, side=side, iterations=iterations, cycles=cycles, mode=mode, budget_bytes=budget_bytes)
     if (resolution not in (32, 64, 128) or side not in (8, 16) or not 0 <= iterations <= 1024 or
             not 0 <= cycles <= 16 or not 0 <= mode <= 4 or not 0 < budget_bytes <= 2 ** 30):
         raise ValueError("Projection: n 32/64/128, brick 8/16, iterations 0..1024, mode 0..4, budget 1..1GiB")
@@ -304,7 +325,7 @@ def _projection(resolution, side, iterations, cycles, mode, budget_bytes):
     rep = _lib.fx_projection_report()
     _lib.call("fx_projection_compare", n, side, iterations, cycles, mode, budget_bytes, faces.ctypes.data,
               *(c.ctypes.data for c in cells), C.byref(rep), BrickPool._stream())
-    out = {name: getattr(rep, name) for name, _ in rep._fields_}
+    out = _report(rep)
     out.update(faces=faces.tobytes(), pressure=cells[0].tobytes(), before=cells[1].tobytes(),
                after=cells[2].tobytes())
     return out
@@ -326,3 +347,120 @@ def compare_multigrid(resolution=64, side=8, cycles=4, mode=1, budget_bytes=256 
     if type(cycles) is not int or not 1 <= cycles <= 16:
         raise ValueError("Use 1..16 multigrid cycles")
     return _projection(resolution, side, 0, cycles, mode, budget_bytes)
+
+
+def compare_transport(resolution=128, side=8, steps=8, displacement=(.6, .2, .1), budget_bytes=256 * 2 ** 20):
+    """Constant-displacement scalar transport of a centred cube on bricks and
+    dense storage; displacement in voxels per step (native/__init__.py:69-72).
+    The dict carries sizes, per-step GPU ms, wall ms, brick-vs-dense errors,
+    masses and the brick field as float32 bytes (dense x-fast layout)."""
+    _ints(resolution=resolution, side=side, steps=steps, budget_bytes=budget_bytes)
+    d = tuple(float(v) for v in displacement)
+    if len(d) != 3:
+        raise ValueError("displacement is (dx, dy, dz)")
+    if not 0 < budget_bytes <= 2 ** 30:
+        raise ValueError("resolution 32/64/128/256; side 8/16; steps 1..32; budget <=1 GiB per engine")
+    n = resolution
+    density = np.empty(n ** 3 if n in (32, 64, 128, 256) else 0, np.float32)
+    rep = _lib.fx_advect_report()
+    _lib.call("fx_transport_compare", n, side, steps, *d, budget_bytes, density.ctypes.data, C.byref(rep),
+              BrickPool._stream())
+    out = _report(rep)
+    out.pop("mode")
+    out.pop("owned_faces")
+    out.pop("padding_faces")
+    out.pop("padding_error")
+    out["sparse_gpu_ms"] = out["sparse_gpu_ms"][:steps]
+    out["dense_gpu_ms"] = out["dense_gpu_ms"][:steps]
+    out["density"] = density.tobytes()
+    return out
+
+
+def compare_mac(resolution=64, side=8, steps=3, mode=0, budget_bytes=256 * 2 ** 20):
+    """MAC self-advection on canonically owned brick faces and dense storage:
+    mode 0 localized affine, 1 affine, 2 constant, 3 signed box
+    (native/__init__.py:75-78).  The brick faces come back as float32 bytes in
+    dense layout."""
+    _ints(resolution=resolution, side=side, steps=steps, mode=mode, budget_bytes=budget_bytes)
+    if (resolution not in (32, 64, 128) or side not in (8, 16) or not 0 <= steps <= 8 or not 0 <= mode <= 3 or
+            not 0 < budget_bytes <= 2 ** 30):
+        raise ValueError("MAC: resolution 32/64/128; side 8/16; steps 0..8; mode 0..3; budget <=1 GiB")
+    n = resolution
+    faces = np.empty(3 * (n + 1) * n * n, np.float32)
+    rep = _lib.fx_advect_report()
+    _lib.call("fx_mac_compare", n, side, steps, mode, budget_bytes, faces.ctypes.data, C.byref(rep),
+              BrickPool._stream())
+    out = _report(rep)
+    for k in ("sparse_mass", "dense_mass", "initial_mass"):
+        out.pop(k)
+    out["sparse_gpu_ms"] = out["sparse_gpu_ms"][:steps]
+    out["dense_gpu_ms"] = out["dense_gpu_ms"][:steps]
+    out["faces"] = faces.tobytes()
+    return out
+
+
+def _coupled(resolution, side, steps, cycles, mode, schedule, budget_bytes, **options):
+    if type(steps) is not int or not 1 <= steps <= 16:
+        raise ValueError("Use 1..16 coupled steps")
+    if type(cycles) is not int or not 1 <= cycles <= 16:
+        raise ValueError("Use 1..16 multigrid cycles")
+    _ints(resolution=resolution, side=side, mode=mode, schedule=schedule, budget_bytes=budget_bytes)
+    if (resolution not in (32, 64, 128) or side not in (8, 16) or not 0 <= mode <= 4 or not 0 <= schedule <= 2 or
+            not 0 < budget_bytes <= 2 ** 30):
+        raise ValueError("Projection: n 32/64/128, brick 8/16, iterations 0..1024, mode 0..4, budget 1..1GiB")
+    bits = sum(_lib.COUPLED_OPTIONS[k] for k, on in options.items() if on)
+    n = resolution
+    faces = np.empty(3 * (n + 1) * n * n, np.float32)
+    cells = [np.empty(n ** 3, np.float32) for _ in range(4)]
+    rep = _lib.fx_coupled_report()
+    _lib.call("fx_coupled_compare", n, side, steps, cycles, mode, schedule, bits, budget_bytes, faces.ctypes.data,
+              *(c.ctypes.data for c in cells), C.byref(rep), BrickPool._stream())
+    out = _report(rep)
+    out.update(faces=faces.tobytes(), pressure=cells[0].tobytes(), before=cells[1].tobytes(),
+               after=cells[2].tobytes(), density=cells[3].tobytes())
+    return out
+
+
+def compare_coupled(resolution=64, side=8, steps=3, cycles=4, mode=1, schedule=1, dense_full=False,
+                    budget_bytes=256 * 2 ** 20):
+    """Coupled transport, projection and density on bricks against dense
+    storage; dense_full runs the dense engine on the whole box, the
+    truncation audit (native/__init__.py:84-95)."""
+    if type(steps) is not int or not 1 <= steps <= 16:
+        raise ValueError("Use 1..16 coupled steps")
+    if type(cycles) is not int or not 1 <= cycles <= 16:
+        raise ValueError("Use 1..16 multigrid cycles")
+    if type(dense_full) is not bool:
+        raise ValueError("dense_full must be bool")
+    return _coupled(resolution, side, steps, cycles, mode, schedule, budget_bytes, dense_full=dense_full)
+
+
+def compare_global_coupled(resolution=64, side=8, steps=3, cycles=4, mode=1, schedule=1, budget_bytes=256 * 2 ** 20):
+    """Coupled step with pressure, velocity and density supported on the whole
+    box (the seed keeps its mode) against unrestricted dense storage
+    (native/__init__.py:98-111)."""
+    return _coupled(resolution, side, steps, cycles, mode, schedule, budget_bytes, dense_full=True,
+                    global_support=True)
+
+
+def compare_hybrid_coupled(resolution=64, side=8, steps=3, cycles=4, mode=1, schedule=1, budget_bytes=256 * 2 ** 20):
+    """Dense pressure and velocity with density on a fixed brick support
+    around the central half; missing density bricks read 0
+    (native/__init__.py:114-126)."""
+    return _coupled(resolution, side, steps, cycles, mode, schedule, budget_bytes, dense_full=True,
+                    global_support=True, hybrid=True)
+
+
+def compare_adaptive_coupled(resolution=64, side=8, steps=8, cycles=4, mode=1, schedule=1, budget_bytes=256 * 2 ** 20):
+    """Hybrid step whose density support grows from the centre brick by a
+    full-domain GPU preflight, reallocating the engine on growth
+    (native/__init__.py:129-137)."""
+    return _coupled(resolution, side, steps, cycles, mode, schedule, budget_bytes, dense_full=True,
+                    global_support=True, hybrid=True, adaptive=True)
+
+
+def compare_capacity_coupled(resolution=64, side=8, steps=8, cycles=4, mode=1, schedule=1, budget_bytes=256 * 2 ** 20):
+    """Adaptive step with the density in buffers of their own whose capacity
+    doubles only when exceeded (native/__init__.py:140-148)."""
+    return _coupled(resolution, side, steps, cycles, mode, schedule, budget_bytes, dense_full=True,
+                    global_support=True, hybrid=True, adaptive=True, pooled=True)

@@ -8,10 +8,11 @@
  * one exception is the brick pool handle (fx_bricks_*): it owns its device
  * mirror and allocates it in fx_bricks_sync (on capacity growth, which also
  * synchronises the device) and fx_bricks_transport (its second payload buffer
- * and brick map, on first use or growth); and the brick projection comparison
- * (fx_projection_compare), a diagnostic that allocates both of its engines
- * within a byte budget and frees them before returning.  `stream` is a
- * hipStream_t (NULL = default).
+ * and brick map, on first use or growth); and the brick comparisons
+ * (fx_projection_compare, fx_coupled_compare, fx_transport_compare,
+ * fx_mac_compare), diagnostics that allocate both of their engines within a
+ * byte budget and free them before returning.  `stream` is a hipStream_t
+ * (NULL = default).
  *
  * Layout everywhere is the reference's flat x-fast index x + nx*(y + ny*z)
  * (physics/reference.py:3).  Face arrays are U (nx+1,ny,nz), V (nx,ny+1,nz),
@@ -677,6 +678,84 @@ typedef struct fx_projection_report {
 int fx_projection_compare(uint32_t n, uint32_t side, uint32_t iterations, uint32_t cycles, uint32_t mode,
                           uint64_t budget, float* faces, float* pressure, float* before, float* after,
                           fx_projection_report* report, void* stream);
+
+/* --------------------------------------------------------------------------
+ * Coupled transport + projection + density on brick storage
+ * (native/projection.mm:101-149 kernels, 242-366 engine; Python surface
+ * native/__init__.py:84-166 compare_coupled, compare_global_coupled,
+ * compare_hybrid_coupled, compare_adaptive_coupled, compare_capacity_coupled).
+ * Each step self-advects the faces (RK2, dt = 1 voxel) on the faces next to
+ * the solve region, projects them with `cycles` V-cycles from q = 0, adds
+ * the emitted density (a soft sphere at (0.5, 0.5, 0.45) n of radius n/12,
+ * schedule 0 fixed, 1 swinging in x, 2 on even steps) and advects the
+ * density through the projected faces.  Options: DENSE_FULL runs the dense
+ * engine on the whole box; GLOBAL solves on the whole box (the seed keeps
+ * its mode); HYBRID (needs GLOBAL and DENSE_FULL) keeps pressure and
+ * velocity dense and the density on bricks of its own, a fixed support
+ * around the central half; ADAPTIVE (needs HYBRID) grows that support from
+ * the centre brick by a full-domain preflight before each density pass,
+ * reallocating the engine block; POOLED (needs ADAPTIVE) keeps the density
+ * in buffers of their own whose capacity doubles when exceeded.
+ * ------------------------------------------------------------------------ */
+enum {
+  FX_COUPLED_DENSE_FULL = 1,
+  FX_COUPLED_GLOBAL = 2,
+  FX_COUPLED_HYBRID = 4,
+  FX_COUPLED_ADAPTIVE = 8,
+  FX_COUPLED_POOLED = 16
+};
+
+typedef struct fx_coupled_report {
+  fx_projection_report projection;  /* last step's projection figures */
+  uint32_t steps, schedule, dense_full, global_support, hybrid, adaptive, pooled, expansions;
+  uint32_t density_bricks, density_capacity, density_reallocations, density_reuses;
+  uint64_t density_cells, density_bytes, preserved_bytes, global_copy_bytes, peak_sparse_bytes;
+  double density_error;             /* max |brick - dense| density */
+  double sparse_mass, dense_mass, injected_mass, dense_after_rms;
+  double density_topology_ms, preflight_ms, sparse_wall_ms, dense_wall_ms;
+} fx_coupled_report;
+
+/* compare_coupled and its variants (projection.mm:370-462): steps 1..16,
+ * cycles 1..16, schedule 0..2, other ranges and budget as
+ * fx_projection_compare; density growth is held to the budget with the old
+ * and new storage counted together.  Host outputs as fx_projection_compare
+ * plus density (n^3) from the brick engine in dense layout. */
+int fx_coupled_compare(uint32_t n, uint32_t side, uint32_t steps, uint32_t cycles, uint32_t mode, uint32_t schedule,
+                       uint32_t options, uint64_t budget, float* faces, float* pressure, float* before, float* after,
+                       float* density, fx_coupled_report* report, void* stream);
+
+/* --------------------------------------------------------------------------
+ * Advection diagnostics on brick storage, each next to the same work on
+ * dense storage; missing bricks read 0.
+ * - fx_transport_compare (native/transport.mm; native/__init__.py:69-72
+ *   compare_transport): a centred cube of ones (edge 7n/16) moved `steps`
+ *   times by a constant displacement of at most 4 voxels per axis, each step
+ *   a trilinear sample with clamped indices; the bricks cover the cube's
+ *   reach.  n 32/64/128/256, side 8/16, steps 1..32.
+ * - fx_mac_compare (native/mac.mm; native/__init__.py:75-78 compare_mac):
+ *   MAC faces (mode 0 localized affine, 1 affine, 2 constant, 3 signed box)
+ *   self-advected `steps` times (RK2, dt = 1 voxel, coordinate-clamped
+ *   trilinear samples) on canonically owned brick faces.  n 32/64/128,
+ *   side 8/16, steps 0..8, mode 0..3.
+ * Budget 1 B..1 GiB per engine, checked before any allocation.
+ * ------------------------------------------------------------------------ */
+typedef struct fx_advect_report {
+  uint32_t resolution, side, steps, mode, active_bricks, pad_;
+  uint64_t sparse_bytes, dense_bytes;
+  uint64_t owned_faces, padding_faces;        /* MAC: stored faces by ownership */
+  double max_error, rms_error, padding_error; /* brick vs dense; padding vs 12345 */
+  double topology_ms, sparse_wall_ms, dense_wall_ms;
+  double sparse_mass, dense_mass, initial_mass; /* scalar transport */
+  double sparse_gpu_ms[32], dense_gpu_ms[32];   /* per step */
+} fx_advect_report;
+
+/* density: n^3 floats, the brick engine's field in dense layout */
+int fx_transport_compare(uint32_t n, uint32_t side, uint32_t steps, float dx, float dy, float dz, uint64_t budget,
+                         float* density, fx_advect_report* report, void* stream);
+
+/* faces: 3*(n+1)*n^2 floats, the brick engine's faces in dense layout */
+int fx_mac_compare(uint32_t n, uint32_t side, uint32_t steps, uint32_t mode, uint64_t budget, float* faces,
+                   fx_advect_report* report, void* stream);
 
 #ifdef __cplusplus
 }

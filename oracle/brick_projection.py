@@ -19,12 +19,24 @@ its brick engine against:
   restricted as 4 x the 8-child mean, trilinear prolongation with clamped
   indices, 128 sweeps on 8^3.
 
+* the coupled step (coupled_validate.py:13-33, projection.mm:110-149,
+  347-366): the faces self-advected (RK2, dt = 1 voxel) and cut to the
+  faces next to the solve region, V-cycles from q = 0 on their divergence,
+  the gradient correction, then density += emission (a soft sphere of
+  radius n/12 at (0.5, 0.5, 0.45) n, swinging in x in schedule 1, on even
+  steps only in schedule 2) and density advected through the projected
+  velocity, both kept to the solve region.
+
 Arrays are NumPy (z, y, x); face array a has one more entry along axis a.
 Pinned in tests/test_oracle_brick_projection.py by closed forms and
 invariants: the stencil against the dense Laplacian matrix of a tiny grid,
 the manufactured gradient of mode 4 (projection removes it), exact zero for
 mode 3, sum(div) = 0 for the closed box, energy and divergence decrease,
-and the V-cycle's contraction.
+and the V-cycle's contraction.  The coupled step is pinned through its
+pieces (the face transport against oracle/brick_transport.py's MAC step,
+itself pinned against the dense face predictor), the closed form of the
+zero-velocity mode (density = the emitted mass, exactly), and the divergence
+identity after the correction.
 """
 import numpy as np
 
@@ -202,3 +214,73 @@ def laplacian_matrix(n):
                             A[i, c[0] + n * (c[1] + n * c[2])] += 1.0
                             A[i, i] -= 1.0
     return A
+
+
+# ---------------------------------------------------------------------------
+# coupled transport + projection + density
+# ---------------------------------------------------------------------------
+def emission(n, step, schedule):
+    """Density added at a step (coupled_validate.py:13-17, projection.mm:110-116)."""
+    z, y, x = np.indices((n, n, n), dtype=np.float64)
+    cx = .5 * n + (.05 * n * np.sin(.5 * step) if schedule == 1 else 0.)
+    r = n / 12
+    v = .1 * np.maximum(0, 1 - ((x + .5 - cx) ** 2 + (y + .5 - .5 * n) ** 2 + (z + .5 - .45 * n) ** 2) / r ** 2)
+    if schedule == 2 and step % 2:
+        v[...] = 0
+    return v.astype(np.float32)
+
+
+def transport_faces(fields, m):
+    """Self-advected faces kept next to the solve region, walls zero
+    (coupled_validate.py:21-26, projection.mm:101-109)."""
+    from oracle import brick_transport as bt
+    out = []
+    for a, u in enumerate(bt.mac_step(fields)):
+        adj = np.zeros(u.shape, bool)
+        sl = [slice(None)] * 3
+        sl[2 - a] = slice(None, -1)
+        adj[tuple(sl)] |= m
+        sl[2 - a] = slice(1, None)
+        adj[tuple(sl)] |= m
+        u = u * adj
+        sl[2 - a] = 0
+        u[tuple(sl)] = 0
+        sl[2 - a] = -1
+        u[tuple(sl)] = 0
+        out.append(u.astype(np.float32))
+    return out
+
+
+def transport_density(rho, fields, m):
+    """Cell-centred RK2 back-trace through the faces, coordinate-clamped
+    trilinear sample, kept to the solve region (coupled_validate.py:31-32,
+    projection.mm:138-148)."""
+    from oracle import brick_transport as bt
+    n = rho.shape[0]
+    z, y, x = np.indices((n, n, n), dtype=np.float64)
+    p = [x + .5, y + .5, z + .5]
+    v = bt.velocity(fields, p)
+    mid = [c - .5 * u for c, u in zip(p, v)]
+    v = bt.velocity(fields, mid)
+    return (bt.sample(rho, [c - u - .5 for c, u in zip(p, v)]) * m).astype(np.float32)
+
+
+def coupled(n, steps, cycles, mode, schedule, global_support=False):
+    """`steps` coupled steps from the seed of `mode`; the solve region is the
+    mode's (the whole box with global_support).  Returns the faces, density,
+    the last step's q and the divergence after (coupled_validate.py:18-33)."""
+    fields = seed(n, mode)
+    solve_mode = 0 if global_support else mode
+    m = mask(n, solve_mode)
+    rho = np.zeros((n, n, n), np.float32)
+    q = np.zeros_like(rho)
+    for step in range(steps):
+        adv = transport_faces(fields, m)
+        rhs = divergence(adv, m)
+        q = np.zeros_like(rho)
+        for _ in range(cycles):
+            q = cycle(q, rhs, solve_mode)
+        fields = gradient_correct(adv, q)
+        rho = ((rho + emission(n, step, schedule)) * m).astype(np.float32)
+        rho = transport_density(rho, fields, m)
+    return fields, rho, q, divergence(fields, m)

@@ -67,3 +67,37 @@ def test_version_and_error_path_without_gpu():
     d.c0, d.c1, d.cout = 3, 0, 16
     with pytest.raises(ValueError, match="multiples of 8"):
         lib.call("fx_conv3d_forward", C.byref(d), None, None, None, None, None, None)
+
+
+def test_struct_layouts_match_the_c_compiler(tmp_path):
+    """Size and field offsets of every mirrored struct as gcc lays them out
+    from include/fluxhip.h."""
+    import shutil
+    import subprocess
+    lib = _lib()
+    gcc = shutil.which("gcc")
+    if gcc is None:
+        pytest.skip("gcc is not available")
+    structs = {name: getattr(lib, name) for name in ("fx_grid", "fx_sources", "fx_mg_level", "fx_mg_config",
+                                                     "fx_solve_info", "fx_solve_params", "fx_conv_desc", "fx_unet",
+                                                     "fx_bricks_dev", "fx_velocity_args", "fx_scalar_args",
+                                                     "fx_projection_report", "fx_coupled_report",
+                                                     "fx_advect_report") if hasattr(lib, name)}
+    lines = ['#include <stdio.h>', '#include <stddef.h>', f'#include "{HEADER}"', "int main(void) {"]
+    for name, st in structs.items():
+        lines.append(f'  printf("{name} size %zu\\n", sizeof({name}));')
+        for field, _ in st._fields_:
+            lines.append(f'  printf("{name} {field} %zu\\n", offsetof({name}, {field}));')
+    lines += ["  return 0;", "}"]
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    subprocess.run([gcc, "-std=c11", str(src), "-o", str(exe)], check=True)
+    got = {}
+    for row in subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.splitlines():
+        name, field, value = row.split()
+        got[(name, field)] = int(value)
+    for name, st in structs.items():
+        assert got[(name, "size")] == C.sizeof(st), name
+        for field, _ in st._fields_:
+            assert got[(name, field)] == getattr(st, field).offset, (name, field)

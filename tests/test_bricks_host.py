@@ -393,3 +393,53 @@ def test_projection_arguments_are_checked_before_any_allocation():
         _lib.call("fx_projection_compare", n, 8, 0, 4, 0, 91_000_000, *args)
     with pytest.raises(ValueError, match="Projection: n 32/64/128"):
         _lib.call("fx_projection_compare", 256, 8, 16, 0, 1, 2 ** 20, *args)
+
+
+def test_brick_diagnostics_reject_bad_arguments_before_any_allocation():
+    """compare_transport / compare_mac / compare_coupled (and variants) reject
+    bad arguments in Python or in the library before its first HIP call
+    (native/__init__.py:69-148; transport.mm:98-101, mac.mm:89-90,
+    projection.mm:370-373), so these run without a GPU."""
+    import ctypes as C
+    import numpy as np
+    from fluxfx_amd import _lib, native
+    for kw in ({"steps": 0}, {"steps": 17}, {"cycles": 0}, {"cycles": 17}, {"schedule": 3}, {"resolution": 31},
+               {"side": 4}, {"budget_bytes": 0}, {"budget_bytes": 2 ** 64}):
+        for fn in (native.compare_coupled, native.compare_global_coupled, native.compare_hybrid_coupled,
+                   native.compare_adaptive_coupled, native.compare_capacity_coupled):
+            with pytest.raises(ValueError):
+                fn(**kw)
+    with pytest.raises(ValueError, match="dense_full must be bool"):
+        native.compare_coupled(dense_full=1)
+    for kw in ({"resolution": 31}, {"side": 4}, {"steps": -1}, {"steps": 9}, {"mode": 4}, {"budget_bytes": 0},
+               {"budget_bytes": 2 ** 64}):
+        with pytest.raises((ValueError, TypeError)):
+            native.compare_mac(**kw)
+    if not _lib.LIB_PATH.exists():
+        pytest.skip("libfluxhip.so is not built")
+    rep = _lib.fx_advect_report()
+    buf = np.empty(3 * 257 * 256 * 256, np.float32)
+    for args in ((31, 8, 8, 0., 0., 0.), (32, 4, 8, 0., 0., 0.), (32, 8, 0, 0., 0., 0.), (32, 8, 33, 0., 0., 0.),
+                 (32, 8, 1, float("nan"), 0., 0.), (32, 8, 1, 5., 0., 0.)):
+        with pytest.raises(ValueError):
+            _lib.call("fx_transport_compare", *args, 256 * 2 ** 20, buf.ctypes.data, C.byref(rep), None)
+    # budget checks happen on the host before any allocation
+    with pytest.raises(ValueError, match="exceed per-engine budget"):
+        _lib.call("fx_transport_compare", 128, 8, 8, .6, .2, .1, 2 ** 20, buf.ctypes.data, C.byref(rep), None)
+    with pytest.raises(ValueError, match="exceed per-engine budget"):
+        _lib.call("fx_mac_compare", 128, 8, 3, 0, 20 * 2 ** 20, buf.ctypes.data, C.byref(rep), None)
+    n = 128
+    cells = [np.empty(n ** 3, np.float32) for _ in range(4)]
+    crep = _lib.fx_coupled_report()
+    args = [buf.ctypes.data, *(c.ctypes.data for c in cells), C.byref(crep), None]
+    opts = _lib.COUPLED_OPTIONS
+    with pytest.raises(ValueError, match="requires adaptive"):
+        _lib.call("fx_coupled_compare", 32, 8, 3, 4, 1, 1, opts["pooled"], 2 ** 28, *args)
+    with pytest.raises(ValueError, match="Adaptive requires hybrid"):
+        _lib.call("fx_coupled_compare", 32, 8, 3, 4, 1, 1, opts["adaptive"], 2 ** 28, *args)
+    with pytest.raises(ValueError, match="Hybrid requires global"):
+        _lib.call("fx_coupled_compare", 32, 8, 3, 4, 1, 1, opts["hybrid"] | opts["global_support"], 2 ** 28, *args)
+    with pytest.raises(ValueError, match="unknown coupled option"):
+        _lib.call("fx_coupled_compare", 32, 8, 3, 4, 1, 1, 32, 2 ** 28, *args)
+    with pytest.raises(ValueError, match="exceeds per-engine buffer budget"):
+        _lib.call("fx_coupled_compare", n, 8, 3, 4, 1, 1, 0, 40 * 2 ** 20, *args)

@@ -118,3 +118,59 @@ def test_vcycles_contract(mode):
             return
         ratios.append(_rms(after, m) / _rms(rhs, m))
     assert ratios[0] < 1 and ratios[1] < .1 and ratios[1] < ratios[0]
+
+
+@pytest.mark.parametrize("schedule", [0, 1, 2])
+def test_coupled_zero_velocity_keeps_the_emitted_density(schedule):
+    """Mode 3 has no velocity: the projection is exactly zero and every
+    density sample lands on a cell centre, so the density is the sum of the
+    emitted fields on the solve region (projection.mm:110-116, 442-447)."""
+    n = 32
+    fields, rho, q, after = bp.coupled(n, 3, 2, 3, schedule)
+    assert not any(f.any() for f in fields) and not q.any() and not after.any()
+    total = sum(bp.emission(n, s, schedule).astype(np.float64) for s in range(3))
+    np.testing.assert_allclose(rho, total, rtol=0, atol=1e-6)
+    z, y, x = np.indices((n, n, n), dtype=np.float64)
+    injected = 0.0
+    for s in range(3):
+        if schedule == 2 and s % 2:
+            continue
+        cx = .5 * n + (.05 * n * np.sin(.5 * s) if schedule == 1 else 0.)
+        d2 = ((x + .5 - cx) ** 2 + (y + .5 - .5 * n) ** 2 + (z + .5 - .45 * n) ** 2) / (n / 12) ** 2
+        injected += float(np.sum(.1 * np.maximum(0, 1 - d2)))
+    assert abs(float(rho.astype(np.float64).sum()) - injected) < 1e-4
+    if schedule == 2:
+        np.testing.assert_allclose(rho, bp.emission(n, 0, 2) + bp.emission(n, 2, 2), rtol=0, atol=1e-6)
+
+
+@pytest.mark.parametrize("mode,global_support", [(1, False), (0, False), (1, True)])
+def test_coupled_step_projects_the_transported_faces(mode, global_support):
+    n = 32
+    fields, rho, q, after = bp.coupled(n, 1, 4, mode, 1, global_support)
+    m = bp.mask(n, 0 if global_support else mode)
+    adv = bp.transport_faces(bp.seed(n, mode), m)
+    rhs = bp.divergence(adv, m)
+    # div(u - grad q) = rhs - L q on the region, and the V-cycles shrink it
+    np.testing.assert_allclose(after, bp.residual(q, rhs, m), rtol=0, atol=2e-5)
+    assert _rms(after, m) < .1 * _rms(rhs, m)
+    for a, (f, g) in enumerate(zip(fields, adv)):
+        assert not np.take(f, 0, axis=2 - a).any() and not np.take(f, -1, axis=2 - a).any()
+        sl = [slice(None)] * 3
+        sl[2 - a] = slice(1, -1)
+        np.testing.assert_allclose(f[tuple(sl)], (g[tuple(sl)] - np.diff(q, axis=2 - a)).astype(np.float32),
+                                   rtol=0, atol=1e-7)
+    assert rho.min() >= 0 and rho.sum() > 0 and not rho[~m].any()
+
+
+def test_transported_faces_stay_next_to_the_region():
+    n = 32
+    m = bp.mask(n, 2)
+    for a, u in enumerate(bp.transport_faces(bp.seed(n, 1), m)):
+        adj = np.zeros(u.shape, bool)
+        lo = [slice(None)] * 3
+        hi = list(lo)
+        lo[2 - a] = slice(0, -1)
+        hi[2 - a] = slice(1, None)
+        adj[tuple(lo)] |= m
+        adj[tuple(hi)] |= m
+        assert not u[~adj].any() and u[adj].any()
