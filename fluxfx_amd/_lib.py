@@ -15,6 +15,7 @@ FX_RESTRICT_MEAN, FX_RESTRICT_FULL = 0, 1
 FX_PROLONG_TRILINEAR, FX_PROLONG_RENORM, FX_PROLONG_CONSTANT = 0, 1, 2
 MAX_LEVELS = 16
 FX_WS_DOUBLES = 16400  # FP64 reduction workspace (16 scalars + 16 x 1024 block partials)
+FX_COARSE_CG = -1  # fx_mg_config.coarse_sweeps: coarsest level by FP64 conjugate gradients
 
 
 class GPUUnavailable(RuntimeError):
@@ -172,6 +173,7 @@ SIGNATURES = {
     "fx_enforce_walls_moving": ([fx_grid, P, P, P, P, P, P, P, P, P, P, P], C.c_int),
     "fx_divergence_moving": ([fx_grid, P, P, P, P, P, P, P, P, P], C.c_int),
     "fx_gradient_moving": ([fx_grid, P, P, P, P, P, P, P, P, P, P, P, F32, P], C.c_int),
+    "fx_gradient_split_moving": ([fx_grid, P, P, P, P, P, P, P, P, P, P, P, P, P, F32, P], C.c_int),
     "fx_owner_raster": ([fx_grid, I32, P, I32, P, P], C.c_int),
     "fx_wall_velocity": ([fx_grid, P, I32, P, P, P, P, P], C.c_int),
     "fx_remap_scalar": ([fx_grid, P, P, P, P, P], C.c_int),
@@ -207,6 +209,8 @@ SIGNATURES = {
     "fx_mg_restrict": ([C.POINTER(fx_mg_config), I32, P, P, P], C.c_int),
     "fx_mg_prolong": ([C.POINTER(fx_mg_config), I32, P, P, P, P], C.c_int),
     "fx_mg_direct": ([C.POINTER(fx_mg_config), P], C.c_int),
+    "fx_mg_coarse_cg": ([C.POINTER(fx_mg_config), P, P], C.c_int),
+    "fx_mg_rebuild_masked": ([C.POINTER(fx_mg_config), F64, F64, F64, P, P], C.c_int),
     "fx_project_mean": ([fx_grid, P, P, P, P], C.c_int),
     "fx_project_mean_noalloc": ([fx_grid, P, P, P], C.c_int),
     "fx_residual_norm": ([fx_grid, P, P, P, C.POINTER(C.c_double), C.POINTER(C.c_double), P, P], C.c_int),

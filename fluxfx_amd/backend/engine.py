@@ -190,12 +190,14 @@ class MGLevels:
     """Device arrays + fx_mg_config for one V-cycle hierarchy.
 
     ``ops``: per level (op, op_data_tensor_or_None).  Level 0 p/q/rhs may be
-    bound to projector fields with ``bind_level0``.
+    bound to projector fields with ``bind_level0``.  ``active``: per level a
+    device tensor of activity bytes the caller keeps up to date (or None to
+    derive them here from the operator data).
     """
 
     def __init__(self, device, shapes, extent, storage, ops, *, pre, post, omega=2 / 3,
                  restriction=_lib.FX_RESTRICT_MEAN, prolongation=_lib.FX_PROLONG_TRILINEAR,
-                 coarse_sweeps=0, coarse_matrix=None, coarse_scale=1.0, own_level0=True):
+                 coarse_sweeps=0, coarse_matrix=None, coarse_scale=1.0, own_level0=True, active=None):
         if len(shapes) > _lib.MAX_LEVELS:
             raise ValueError("too many multigrid levels")
         self.device = device
@@ -229,7 +231,11 @@ class MGLevels:
                 lv.w[a] = (shape[a] / extent[a]) ** 2 / (4 ** i if op == _lib.FX_OP_PACKED else 1)
             lv.op_data = data.data_ptr() if data is not None else None
             act = None
-            if op == _lib.FX_OP_EDGES and data is not None:
+            if active is not None and active[i] is not None:
+                # activity bytes kept by the caller (rebuilt on the device)
+                self._active.append(active[i])
+                lv.active = active[i].data_ptr()
+            elif op == _lib.FX_OP_EDGES and data is not None:
                 act = data.view(-1, 4)[:, 3].cpu().numpy() > 0
             elif op == _lib.FX_OP_PACKED and data is not None:
                 act = ((data.cpu().numpy().view(np.uint32) >> 30) & 1) > 0

@@ -6,7 +6,12 @@ rigid wall velocity on solid faces; ``update`` re-rasterises the colliders and
 the wall velocities, ``remap`` empties cells and faces whose solidity changed.
 ``MovingPressureProjector`` is the masked Jacobi projector whose closed faces
 carry the wall velocity (shaders/moving_boundary.glsl in pressure_walls and
-pressure_gradient).
+pressure_gradient).  ``MovingToleranceProjector`` (no reference counterpart:
+the reference projects moving colliders with masked Jacobi only) runs the
+tolerance multigrid solvers (FP32, BF16/FP16, FPCG, learned) on the moving
+mask: its rediscretised levels are rebuilt on the device whenever the mask
+changes, and the coarsest level is solved by FP64 conjugate gradients, as
+there is no fixed pseudo-inverse to keep.
 """
 import ctypes
 
@@ -15,8 +20,9 @@ from ..physics.collider_motion import validate_motion, velocity_rows
 from .collision import SolidMask
 from .device import ptr
 from .pressure import SolidPressureProjector
+from .tolerance import ToleranceProjector
 
-__all__ = ["MovingSolidMask", "MovingPressureProjector"]
+__all__ = ["MovingSolidMask", "MovingPressureProjector", "MovingToleranceProjector"]
 
 
 class MovingSolidMask(SolidMask):
@@ -34,6 +40,8 @@ class MovingSolidMask(SolidMask):
         self.previous.copy_(self.solid)
         self._scratch = self.device.zeros(n, "uint8")
         self.wall_fields = [self.device.texture(s, nonnegative=False) for s in grid.face_shapes]
+        # bumped by every update(): projectors with mask-derived operators rebuild them
+        self.version = 0
 
     def _raster(self, mask, colliders):
         mask.zero_()
@@ -61,6 +69,7 @@ class MovingSolidMask(SolidMask):
         _lib.call("fx_wall_velocity", self._g, self.solid.data_ptr(), len(colliders), rows,
                   *(f.ptr for f in self.wall_fields), self.device.stream)
         self.colliders = tuple(colliders)
+        self.version += 1
 
     def remap(self, solver):
         """Empty the cells and rest the faces the motion covered or exposed
@@ -108,3 +117,79 @@ class MovingPressureProjector(SolidPressureProjector):
 
     def metrics(self):
         return super().metrics() | {"solver": "MOVING_JACOBI", "boundary": "moving_voxel_free_slip"}
+
+
+class MovingToleranceProjector(ToleranceProjector):
+    """Tolerance multigrid with moving walls: closed faces carry the wall
+    velocity in the walls, divergence and gradient passes, and the level
+    operators follow the mask (fx_mg_rebuild_masked) before each solve after
+    the colliders moved."""
+    solver_name = "MOVING_TOLERANCE_MULTIGRID"
+
+    def __init__(self, grid, settings=None, device=None, *, solids=None):
+        if solids is None or not getattr(solids, "moving", False):
+            raise ValueError("MovingToleranceProjector needs a MovingSolidMask")
+        self._ops_version = None
+        self._coarse_solid = None
+        super().__init__(grid, settings, device, solids=solids)
+        self.refresh_operators()
+
+    def _masked_operators(self, solids):
+        """Device arrays of every level's operator and activity bytes, filled
+        by refresh_operators(); no coarsest pseudo-inverse (None)."""
+        torch = self.device.torch
+        ops = [(_lib.FX_OP_FLAGS, solids.flags)]
+        active = [None]
+        for level, shape in enumerate(self.shapes[1:], start=1):
+            n = shape[0] * shape[1] * shape[2]
+            if 4 ** level <= 1023:
+                t = self.device.zeros(n, "int32")
+                ops.append((_lib.FX_OP_PACKED, t))
+            else:
+                t = self.device.zeros(4 * n, "float32")
+                ops.append((_lib.FX_OP_EDGES, t))
+            self._edges.append(t)
+            active.append(self.device.zeros(n, "uint8"))
+        self._active_arrays = active
+        nx, ny, nz = self.shapes[1]
+        self._coarse_solid = torch.zeros(nx * ny * nz, dtype=torch.uint8, device=self.device.device)
+        return ops, None
+
+    def refresh_operators(self):
+        """Rebuild levels 1.. from the current mask's open-face flags."""
+        e = self.grid.extent
+        _lib.call("fx_mg_rebuild_masked", ctypes.byref(self.engine.cfg), float(e[0]), float(e[1]), float(e[2]),
+                  self._coarse_solid.data_ptr(), self.device.stream)
+        self._ops_version = self.solids.version
+
+    def _walls(self):
+        return [ptr(f) for f in self.solids.wall_fields]
+
+    def enforce_walls(self, fields, outputs):
+        _lib.call("fx_enforce_walls_moving", self._g, self.flags_ptr, *(ptr(f) for f in fields), *self._walls(),
+                  *(ptr(f) for f in outputs), self.device.stream)
+
+    def compute_divergence(self, fields, target):
+        _lib.call("fx_divergence_moving", self._g, self.flags_ptr, *(ptr(f) for f in fields), *self._walls(),
+                  ptr(target), self.device.stream)
+
+    def solve_pressure(self, dt, previous_dt):
+        if self._ops_version != self.solids.version:
+            self.refresh_operators()
+        super().solve_pressure(dt, previous_dt)
+
+    def apply_gradient(self, fields, outputs, scale):
+        inc = self._increment_fields()
+        _lib.call("fx_gradient_split_moving", self._g, self.flags_ptr, self.pressure.ptr, inc["d"].ptr,
+                  *(ptr(f) for f in fields), *self._walls(), *(ptr(f) for f in outputs), self.spare.ptr,
+                  float(scale), self.device.stream)
+        self.pressure, self.spare = self.spare, self.pressure
+        _lib.call("fx_project_mean", self._g, self.flags_ptr, self.pressure.ptr, self.ws.data_ptr(), self.device.stream)
+
+    def _network_mask(self, level):
+        if level == 0:
+            return self.solids.solid.data_ptr()
+        return self._coarse_solid.data_ptr()
+
+    def metrics(self):
+        return super().metrics() | {"boundary": "moving_voxel_free_slip"}

@@ -27,7 +27,7 @@ from .combustion import CombustionPasses
 from .detail import DetailPasses
 from .device import HipDevice, ptr
 from .mac import DenseThermalAdvection, source_row
-from .moving_collision import MovingPressureProjector, MovingSolidMask
+from .moving_collision import MovingPressureProjector, MovingSolidMask, MovingToleranceProjector
 from .multigrid import MultigridPressureProjector, use_multigrid
 from .pressure import PressureProjector, SolidPressureProjector
 from .solid_multigrid import SolidMultigridProjector
@@ -65,14 +65,13 @@ class DenseProjectedSmoke(DenseThermalAdvection):
         device = device or HipDevice()
         if settings.moving_colliders and colliders:
             # moving rigid colliders: owner mask, wall velocities, masked Jacobi
-            # with moving walls (backend/projected.py:17-29)
+            # with moving walls (backend/projected.py:17-29), or the tolerance
+            # multigrid on the moving mask when a tolerance or cycle budget is set
             if solid_mask is not None:
                 raise ValueError("Moving colliders are analytic shapes; a voxel mask is static")
-            if settings.pressure_tolerance > 0 or settings.pressure_fixed_cycles > 0:
-                raise ValueError("Moving colliders use the masked Jacobi projector; set pressure_tolerance "
-                                 "and pressure_fixed_cycles to 0")
             self.solids = MovingSolidMask(grid, colliders, device)
-            projector_class = projector_class or MovingPressureProjector
+            tolerance = settings.pressure_tolerance > 0 or settings.pressure_fixed_cycles > 0
+            projector_class = projector_class or (MovingToleranceProjector if tolerance else MovingPressureProjector)
         elif colliders or solid_mask is not None:
             self.solids = SolidMask(grid, colliders, device, mask=solid_mask)
             if settings.pressure_tolerance <= 0:

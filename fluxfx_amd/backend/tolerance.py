@@ -55,29 +55,24 @@ class ToleranceProjector(PressureProjector):
         if len(self.shapes) < 2:
             raise ValueError("Tolerance multigrid needs at least two levels")
         self.levels = [{"shape": sh} for sh in self.shapes]
+        self._edges = []
+        self._active_arrays = None
+        self._coarse_sweeps = 0
         if solids is not None:
-            coarse_edges = conductance_hierarchy(self.shapes, grid.extent, solids.numpy())
-            torch = self.device.torch
-            self._edges = []
-            ops = [(_lib.FX_OP_FLAGS, solids.flags)]
-            for level, (e, shape) in enumerate(zip(coarse_edges, self.shapes[1:]), start=1):
-                words = pack_conductances(e, shape, grid.extent, level)
-                if words is not None:
-                    t = torch.from_numpy(words.reshape(-1)).to(self.device.device)
-                    ops.append((_lib.FX_OP_PACKED, t))
-                else:
-                    t = torch.from_numpy(np.ascontiguousarray(e, np.float32).reshape(-1)).to(self.device.device)
-                    ops.append((_lib.FX_OP_EDGES, t))
-                self._edges.append(t)
-            last = coarse_edges[-1]
+            ops, last = self._masked_operators(solids)
             self.prolongation = _lib.FX_PROLONG_RENORM
         else:
-            self._edges = []
             ops = [(_lib.FX_OP_BOX, None)] * len(self.shapes)
             last = box_edges(self.shapes[-1], grid.extent)
             self.prolongation = _lib.FX_PROLONG_TRILINEAR
         self.ops = ops
-        self.coarse_matrix = np.linalg.pinv(edges_matrix(last), rcond=1e-12, hermitian=True)
+        if last is not None:
+            self.coarse_matrix = np.linalg.pinv(edges_matrix(last), rcond=1e-12, hermitian=True)
+        else:
+            # operators rebuilt on the device every step: FP64 conjugate
+            # gradients on the coarsest level instead of a pseudo-inverse
+            self.coarse_matrix = None
+            self._coarse_sweeps = _lib.FX_COARSE_CG
         self.restriction = _lib.FX_RESTRICT_FULL if s.mg_restriction == "FULL_WEIGHTING" else _lib.FX_RESTRICT_MEAN
         self.engine = self._engine(_lib.FX_F32, own_level0=False)
         self.ws = self.device.zeros(_lib.FX_WS_DOUBLES, "float64")
@@ -88,12 +83,31 @@ class ToleranceProjector(PressureProjector):
         self._inc = None
         self._check_from = 0
 
+    def _masked_operators(self, solids):
+        """Level operators of a static solid mask, built once on the host:
+        level 0's open-face flags, then the rediscretised levels (packed words
+        while they fit, else float4 conductances); returns (ops, the coarsest
+        level's edge records for its pseudo-inverse)."""
+        coarse_edges = conductance_hierarchy(self.shapes, self.grid.extent, solids.numpy())
+        torch = self.device.torch
+        ops = [(_lib.FX_OP_FLAGS, solids.flags)]
+        for level, (e, shape) in enumerate(zip(coarse_edges, self.shapes[1:]), start=1):
+            words = pack_conductances(e, shape, self.grid.extent, level)
+            if words is not None:
+                t = torch.from_numpy(words.reshape(-1)).to(self.device.device)
+                ops.append((_lib.FX_OP_PACKED, t))
+            else:
+                t = torch.from_numpy(np.ascontiguousarray(e, np.float32).reshape(-1)).to(self.device.device)
+                ops.append((_lib.FX_OP_EDGES, t))
+            self._edges.append(t)
+        return ops, coarse_edges[-1]
+
     def _engine(self, storage, own_level0):
         s = self.settings
         return MGLevels(self.device, self.shapes, self.grid.extent, storage, self.ops, pre=s.mg_pre, post=s.mg_post,
                         omega=2 / 3, restriction=self.restriction, prolongation=self.prolongation,
-                        coarse_sweeps=0, coarse_matrix=self.coarse_matrix, coarse_scale=1.0,
-                        own_level0=own_level0)
+                        coarse_sweeps=self._coarse_sweeps, coarse_matrix=self.coarse_matrix, coarse_scale=1.0,
+                        own_level0=own_level0, active=self._active_arrays)
 
     @property
     def uses_impulse(self):

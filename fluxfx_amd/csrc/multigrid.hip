@@ -457,6 +457,147 @@ __global__ void k_direct(int m, const double* __restrict__ M, const T* __restric
   if (lane == 0) out[row] = from_f<T>(float(acc * double(scale)));
 }
 
+// Coarsest level by conjugate gradients in FP64 inside one workgroup, for
+// operators that change between solves (moving colliders), where no
+// pseudo-inverse is kept.  One cell per thread, m <= kCgMaxCells.  With A the
+// level's operator (A p)_c = sum_n w_n (p_n - p_c), it solves B p = -scale rhs
+// for the positive semidefinite B = -A, i.e. A p = scale rhs like the direct
+// solve, and returns the pseudo-inverse's solution: B's null space is the
+// constants on each connected part of the cells with conductance, so the
+// kernel labels the parts (minimum cell index, spread along open faces),
+// removes each part's mean from b (the restricted residual's rounding leaves
+// some, and CG would blow it up along the null space) and from the result.
+// Part sums run over the cells in index order, so results are deterministic.
+// CG starts from p = 0 and ends when ||r|| <= 1e-12 ||b||, when p.Bp <= 0 or
+// after `maxit` iterations; every thread computes the same alpha and beta
+// from the same LDS partial sums, so control flow stays uniform.  Cells
+// without conductance hold 0.  info (nullable): {iterations, ||r|| / ||b||}.
+constexpr int kCgMaxCells = 1024;
+
+template <typename T, int OP>
+__global__ __launch_bounds__(kCgMaxCells) void k_coarse_cg(LevelDev L, const T* __restrict__ rhs, T* __restrict__ out,
+                                                           float scale, int maxit, double* __restrict__ info) {
+  __shared__ double P[kCgMaxCells];
+  __shared__ double M[kCgMaxCells];
+  __shared__ int LB[kCgMaxCells];
+  __shared__ double part[2][kCgMaxCells / 64];
+  const int m = L.nx * L.ny * L.nz;
+  const int i = threadIdx.x;
+  const int nw = blockDim.x >> 6, lane = i & 63, wv = i >> 6;
+  const int sy = L.nx, sz = L.nx * L.ny;
+  // conductances to -x, +x, -y, +y, -z, +z (0 where closed or outside)
+  double w[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  if (i < m) {
+    const int z = i / sz, y = (i - z * sz) / sy, x = i - z * sz - y * sy;
+    if (op_has_edges(OP)) {
+      const float4 c = edge_at<OP>(L, i);
+      w[1] = c.x;
+      w[3] = c.y;
+      w[5] = c.z;
+      w[0] = x > 0 ? edge_x<OP>(L, i - 1) : 0.0f;
+      w[2] = y > 0 ? edge_y<OP>(L, i - sy) : 0.0f;
+      w[4] = z > 0 ? edge_z<OP>(L, i - sz) : 0.0f;
+    } else {
+      const uint8_t f =
+          OP == FX_OP_FLAGS ? reinterpret_cast<const uint8_t*>(L.op)[i] : box_flags(x, y, z, L.nx, L.ny, L.nz);
+      w[0] = (f & FLAG_XLO) ? L.wx : 0.0f;
+      w[1] = (f & FLAG_XHI) ? L.wx : 0.0f;
+      w[2] = (f & FLAG_YLO) ? L.wy : 0.0f;
+      w[3] = (f & FLAG_YHI) ? L.wy : 0.0f;
+      w[4] = (f & FLAG_ZLO) ? L.wz : 0.0f;
+      w[5] = (f & FLAG_ZHI) ? L.wz : 0.0f;
+    }
+  }
+  // a conductance to a neighbour is that neighbour's conductance back, so the
+  // neighbours reached below have conductance themselves
+  const int nb[6] = {i - 1, i + 1, i - sy, i + sy, i - sz, i + sz};
+  const double d = w[0] + w[1] + w[2] + w[3] + w[4] + w[5];
+  const bool live = i < m && d > 0.0;
+  // connected parts: every cell takes the smallest label among itself and its
+  // open neighbours until no label changes
+  int lab = live ? i : -1;
+  LB[i] = lab;
+  __syncthreads();
+  for (;;) {
+    int nl = lab;
+    if (live) {
+#pragma unroll
+      for (int k = 0; k < 6; ++k)
+        if (w[k] != 0.0) nl = min(nl, LB[nb[k]]);
+    }
+    const int changed = nl != lab;
+    __syncthreads();
+    lab = nl;
+    LB[i] = lab;
+    if (!__syncthreads_or(changed)) break;
+  }
+  // v minus the mean of v over the cell's part (each part's first cell sums it)
+  auto part_free = [&](double v) {
+    P[i] = v;
+    __syncthreads();
+    if (live && lab == i) {
+      double s = 0.0;
+      int count = 0;
+      for (int j = i; j < m; ++j)
+        if (LB[j] == i) {
+          s += P[j];
+          ++count;
+        }
+      M[i] = s / double(count);
+    }
+    __syncthreads();
+    const double r = live ? v - M[lab] : 0.0;
+    __syncthreads();  // P is written again by the next use
+    return r;
+  };
+  // block sum, alternating between the two partial buffers: a buffer is
+  // written again only after the barrier of the other buffer's sum, by which
+  // time every thread has read it
+  int buf = 0;
+  auto block_sum = [&](double v) {
+    v = wave_sum(v);
+    if (lane == 0) part[buf][wv] = v;
+    __syncthreads();
+    double s = 0.0;
+    for (int k = 0; k < nw; ++k) s += part[buf][k];
+    buf ^= 1;
+    return s;
+  };
+  const double b = part_free(live ? -double(scale) * double(to_f(rhs[i])) : 0.0);
+  double x = 0.0, r = b, p = b;
+  double rr = block_sum(r * r);
+  const double bb = rr;
+  int it = 0;
+  while (it < maxit && rr > 1e-24 * bb) {
+    P[i] = p;
+    __syncthreads();
+    double q = 0.0;
+    if (live) {
+      q = d * p;
+#pragma unroll
+      for (int k = 0; k < 6; ++k)
+        if (w[k] != 0.0) q -= w[k] * P[nb[k]];
+    }
+    const double pq = block_sum(p * q);
+    if (!(pq > 0.0)) break;
+    const double alpha = rr / pq;
+    x += alpha * p;
+    r -= alpha * q;
+    const double rn = block_sum(r * r);
+    ++it;
+    const double beta = rn / rr;
+    rr = rn;
+    p = r + beta * p;
+  }
+  __syncthreads();  // the last iteration's P reads are done before part_free writes P
+  x = part_free(x);
+  if (i < m) out[i] = from_f<T>(live ? float(x) : 0.0f);
+  if (info && i == 0) {
+    info[0] = double(it);
+    info[1] = bb > 0.0 ? sqrt(rr / bb) : 0.0;
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Fused V-cycle legs of the levels below the finest.  A V-cycle visits such a
 // level twice: going down it smooths from zero (`pre` sweeps) and restricts
@@ -798,8 +939,11 @@ int check_config(const fx_mg_config* c) {
     }
   }
   const fx_mg_level& C = c->level[c->levels - 1];
-  if (c->coarse_sweeps <= 0) {
+  if (c->coarse_sweeps == 0) {
     FX_REQUIRE(c->coarse_matrix, FX_EINVAL, "direct coarse solve needs a matrix");
+  } else if (c->coarse_sweeps < 0) {
+    FX_REQUIRE(C.nx * C.ny * C.nz <= kCgMaxCells, FX_EINVAL,
+               "coarse conjugate gradients limited to %d cells, got %d", kCgMaxCells, C.nx * C.ny * C.nz);
   }
   (void)C;
   return FX_OK;
@@ -971,7 +1115,24 @@ static int launch_prolong(const fx_mg_config* c, int l, const void* in, const vo
   return FX_OK;
 }
 
+static int launch_coarse_cg(const fx_mg_config* c, double* info, hipStream_t s) {
+  const fx_mg_level& C = c->level[c->levels - 1];
+  const int m = C.nx * C.ny * C.nz;
+  FX_REQUIRE(m <= kCgMaxCells, FX_EINVAL, "coarse conjugate gradients limited to %d cells, got %d", kCgMaxCells, m);
+  const dim3 block((m + 63) / 64 * 64);
+  const LevelDev D = dev_level(C);
+  FX_DISPATCH_T(c->storage, {
+    FX_DISPATCH_OP(C.op, {
+      hipLaunchKernelGGL((k_coarse_cg<TV, OPV>), dim3(1), block, 0, s, D, (const TV*)C.rhs, (TV*)C.p, c->coarse_scale,
+                         m, info);
+    });
+  });
+  FX_LAUNCHED("k_coarse_cg");
+  return FX_OK;
+}
+
 static int launch_direct(const fx_mg_config* c, hipStream_t s) {
+  if (c->coarse_sweeps < 0) return launch_coarse_cg(c, nullptr, s);
   const fx_mg_level& C = c->level[c->levels - 1];
   int m = C.nx * C.ny * C.nz;
   FX_REQUIRE(m <= 4096, FX_EINVAL, "direct coarse solve limited to 4096 cells, got %d", m);
@@ -1196,8 +1357,13 @@ int fx_mg_prolong(const fx_mg_config* cfg, int32_t index, const void* fine_in, c
 
 int fx_mg_direct(const fx_mg_config* cfg, void* stream) {
   if (int e = check_config(cfg)) return e;
-  FX_REQUIRE(cfg->coarse_matrix, FX_EINVAL, "no coarse matrix");
+  FX_REQUIRE(cfg->coarse_matrix || cfg->coarse_sweeps < 0, FX_EINVAL, "no coarse matrix");
   return launch_direct(cfg, S(stream));
+}
+
+int fx_mg_coarse_cg(const fx_mg_config* cfg, double* info, void* stream) {
+  if (int e = check_config(cfg)) return e;
+  return launch_coarse_cg(cfg, info, S(stream));
 }
 
 }  // extern "C"

@@ -553,6 +553,18 @@ int fx_gradient_moving(fx_grid g, const uint8_t* flags, const float* p, const fl
                          walls);
 }
 
+int fx_gradient_split_moving(fx_grid g, const uint8_t* flags, const float* p, const float* dp, const float* u,
+                             const float* v, const float* w, const float* bu, const float* bv, const float* bw,
+                             float* uo, float* vo, float* wo, float* psum, float scale, void* stream) {
+  if (int e = check_grid(g)) return e;
+  FX_REQUIRE(flags && p && dp && u && v && w && bu && bv && bw && uo && vo && wo, FX_EINVAL, "null array");
+  FX_REQUIRE(uo != u && vo != v && wo != w, FX_EINVAL, "Advection input/output textures must be distinct");
+  FX_REQUIRE(psum != p && psum != dp, FX_EINVAL, "gradient: the pressure sum must not alias p or dp");
+  const float* walls[3] = {bu, bv, bw};
+  return launch_gradient(g, flags, p, dp, u, v, w, uo, vo, wo, psum, scale, S(stream), "fx_gradient_split_moving",
+                         walls);
+}
+
 int fx_combine(const float* a, const float* b, float* out, int64_t n, void* stream) {
   FX_REQUIRE(a && b && out && n >= 0, FX_EINVAL, "combine: bad arrays");
   if (n == 0) return FX_OK;

@@ -145,6 +145,15 @@ int fx_gradient_moving(fx_grid g, const uint8_t* flags, const float* p, const fl
                        const float* w, const float* bu, const float* bv, const float* bw, float* uo, float* vo,
                        float* wo, float scale, void* stream);
 
+/* fx_gradient_split with moving walls: the gradient of p + dp on open faces,
+ * the wall velocity on closed ones, and psum = p + dp (nullable).  The
+ * tolerance projector's correction for moving colliders (no reference
+ * counterpart: the reference projects moving colliders with masked Jacobi,
+ * backend/projected.py:21-28). */
+int fx_gradient_split_moving(fx_grid g, const uint8_t* flags, const float* p, const float* dp, const float* u,
+                             const float* v, const float* w, const float* bu, const float* bv, const float* bw,
+                             float* uo, float* vo, float* wo, float* psum, float scale, void* stream);
+
 /* Owner raster of moving collider `owner` (1..8): mask[c] = owner where the
  * unit sphere/box under the inverse rows r (3 x 4) covers the cell centre and
  * no earlier collider owns the cell (MovingSolidDevice.update,
@@ -212,7 +221,8 @@ typedef struct fx_mg_config {
   float omega;            /* Jacobi relaxation */
   int32_t restriction;    /* FX_RESTRICT_* */
   int32_t prolongation;   /* FX_PROLONG_* */
-  int32_t coarse_sweeps;  /* > 0: smooth the coarsest level this many times; 0: direct */
+  int32_t coarse_sweeps;  /* > 0: smooth the coarsest level this many times; 0: direct (coarse_matrix);
+                            < 0: FP64 conjugate gradients in one workgroup (fx_mg_coarse_cg) */
   const double* coarse_matrix; /* device, m x m FP64 pseudo-inverse, m = cells of coarsest */
   float coarse_scale;     /* direct: p = coarse_scale * M rhs */
   int32_t separate_legs;  /* 0: on levels below the finest, each V-cycle leg (the pre-sweeps from zero with
@@ -250,6 +260,28 @@ int fx_mg_prolong(const fx_mg_config* cfg, int32_t index, const void* fine_in, c
 /* Coarsest direct solve p = scale * M rhs (shaders/coarse_pressure.glsl:1-12,
  * shaders/solid_mg_direct.glsl:1-12), FP64 accumulation. */
 int fx_mg_direct(const fx_mg_config* cfg, void* stream);
+
+/* Coarsest-level solve by FP64 conjugate gradients in one workgroup (coarsest
+ * level of at most 1024 cells), what the V-cycle runs when coarse_sweeps < 0:
+ * A p = coarse_scale * rhs from p = 0 to ||r|| <= 1e-12 ||b|| (the
+ * pseudo-inverse's solution for a consistent rhs), for operators rebuilt every
+ * step, where no pseudo-inverse is kept.  info (nullable, device, 2 doubles):
+ * iterations taken and the final ||r|| / ||b||.  No reference counterpart. */
+int fx_mg_coarse_cg(const fx_mg_config* cfg, double* info, void* stream);
+
+/* Rebuild levels 1.. of a rediscretised masked hierarchy on the device from
+ * level 0's open-face flags (FX_OP_FLAGS, from fx_build_flags): per level the
+ * packed words (FX_OP_PACKED, levels 1-4) or float4 conductances
+ * (FX_OP_EDGES), and the activity bytes, into cfg->level[l].op_data and
+ * .active (writable device arrays of the level's cell count, l >= 1).  A coarse
+ * face conducts the mean open fraction of the four finer faces it covers and a
+ * coarse cell is active when a child is, the same integers and roundings as
+ * the host construction of static masks.  ex/ey/ez: the domain extent.
+ * coarse_solid (nullable): level 1's solid mask (1 where all eight children
+ * are solid).  For moving colliders (no reference counterpart: the reference
+ * projects them with masked Jacobi, backend/projected.py:21-28). */
+int fx_mg_rebuild_masked(const fx_mg_config* cfg, double ex, double ey, double ez, uint8_t* coarse_solid,
+                         void* stream);
 
 /* Solve info returned by the tolerance-driven solvers (no reference
  * counterpart: the reference runs fixed cycle budgets). */

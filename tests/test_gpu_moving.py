@@ -85,16 +85,18 @@ def test_moving_collider_matches_oracle_and_reference_gates(device, kind):
     sim.close()
 
 
-def test_moving_colliders_reject_tolerance_projector_and_static_use(device):
+def test_moving_colliders_select_tolerance_projector_and_reject_static_use(device):
+    from fluxfx_amd.backend.moving_collision import MovingToleranceProjector
     from fluxfx_amd.backend.projected import DenseProjectedSmoke
     from fluxfx_amd.physics.config import GridSpec
     from fluxfx_amd.physics.collision import primitive
     from fluxfx_amd.physics.pressure import PressureSettings
     grid = GridSpec((16,) * 3)
     c = primitive("SPHERE", (.5, .5, .5), (.2,) * 3)
-    with pytest.raises(ValueError):
-        DenseProjectedSmoke(grid, PressureSettings(moving_colliders=True, pressure_tolerance=1e-5), device,
-                            colliders=(c,))
+    tol = DenseProjectedSmoke(grid, PressureSettings(moving_colliders=True, pressure_tolerance=1e-5), device,
+                              colliders=(c,))
+    assert isinstance(tol.projector, MovingToleranceProjector)
+    tol.close()
     sim = DenseProjectedSmoke(grid, PressureSettings(), device, colliders=(c,))
     with pytest.raises(ValueError):
         sim.move_colliders((c,), .01)
