@@ -178,6 +178,8 @@ SIGNATURES = {
     "fx_wall_velocity": ([fx_grid, P, I32, P, P, P, P, P], C.c_int),
     "fx_remap_scalar": ([fx_grid, P, P, P, P, P], C.c_int),
     "fx_remap_faces": ([fx_grid, P, P, P, P, P, P, P, P, P, P, P], C.c_int),
+    "fx_moving_update_box": ([fx_grid, P, I32, P, P, P, P, P, P, P, P, P, P, P, P], C.c_int),
+    "fx_remap_box": ([fx_grid, P, P, P, P, P, P, P, P, P, P, P, P, P], C.c_int),
     "fx_confine": ([fx_grid, P, P, P, P, P, P, P, F32, F32, F32, P], C.c_int),
     "fx_confine_split": ([fx_grid, P, P, P, P, P, P, P, P, F32, F32, F32, P], C.c_int),
     "fx_bricks_create": ([I32, U32, U64, C.POINTER(P)], C.c_int),

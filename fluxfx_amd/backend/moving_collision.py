@@ -14,6 +14,9 @@ changes, and the coarsest level is solved by FP64 conjugate gradients, as
 there is no fixed pseudo-inverse to keep.
 """
 import ctypes
+import math
+
+import numpy as np
 
 from .. import _lib
 from ..physics.collider_motion import validate_motion, velocity_rows
@@ -39,6 +42,7 @@ class MovingSolidMask(SolidMask):
         self.previous = self.device.zeros(n, "uint8")
         self.previous.copy_(self.solid)
         self._scratch = self.device.zeros(n, "uint8")
+        self._scratch2 = self.device.zeros(n, "uint8")
         self.wall_fields = [self.device.texture(s, nonnegative=False) for s in grid.face_shapes]
         # bumped by every update(): projectors with mask-derived operators rebuild them
         self.version = 0
@@ -53,44 +57,60 @@ class MovingSolidMask(SolidMask):
     @property
     def allocated_bytes(self):
         faces = sum(s[0] * s[1] * s[2] for s in self.grid.face_shapes)
-        return 5 * self.grid.cells + 4 * faces
+        return 6 * self.grid.cells + 4 * faces
+
+    def cell_box(self, colliders):
+        """Half-open cell box {x0, y0, z0, x1, y1, z1} holding every cell whose
+        centre the colliders can cover: each unit sphere/box mapped forward by
+        its inverse rows' inverse gives a centre and half-extents per axis
+        (row norms of the forward matrix: 2-norm for the sphere, 1-norm for the
+        box), widened by a cell for the rasteriser's rounding."""
+        lo = [math.inf] * 3
+        hi = [-math.inf] * 3
+        for c in colliders:
+            rows = np.array(c.inverse_rows, dtype=np.float64)
+            fwd = np.linalg.inv(rows[:, :3])
+            centre = -fwd @ rows[:, 3]
+            half = np.linalg.norm(fwd, axis=1) if c.shape == "SPHERE" else np.abs(fwd).sum(axis=1)
+            for a in range(3):
+                h = self.grid.extent[a] / self.grid.shape[a]
+                lo[a] = min(lo[a], math.floor((centre[a] - half[a]) / h - 0.5) - 1)
+                hi[a] = max(hi[a], math.ceil((centre[a] + half[a]) / h - 0.5) + 2)
+        n = self.grid.shape
+        return [max(0, min(int(lo[a]), n[a])) for a in range(3)] + [max(0, min(int(hi[a]), n[a])) for a in range(3)]
 
     def update(self, colliders, dt):
         """New owners, flags, clearance and wall velocities for the colliders'
-        new poses reached over dt (MovingSolidDevice.update)."""
+        new poses reached over dt (MovingSolidDevice.update), over the box of
+        cells the old and new poses can cover (fx_moving_update_box)."""
         validate_motion(self.colliders, colliders)
         self.previous.copy_(self.solid)
-        self._raster(self.solid, colliders)
-        _lib.call("fx_build_flags", self._g, self.solid.data_ptr(), self.flags.data_ptr(), self.device.stream)
-        _lib.call("fx_build_clearance", self._g, self.solid.data_ptr(), self.clearance.data_ptr(),
-                  self._scratch.data_ptr(), self.device.stream)
+        old_box, new_box = self.cell_box(self.colliders), self.cell_box(colliders)
+        box = [min(a, b) for a, b in zip(old_box[:3], new_box[:3])] + \
+              [max(a, b) for a, b in zip(old_box[3:], new_box[3:])]
+        self._box = (ctypes.c_int32 * 6)(*box)
+        kinds = (ctypes.c_int32 * len(colliders))(*[0 if c.shape == "SPHERE" else 1 for c in colliders])
+        inv = [float(v) for c in colliders for r in c.inverse_rows for v in r]
         table = [v for old, new in zip(self.colliders, colliders) for row in velocity_rows(old, new, dt) for v in row]
-        rows = (ctypes.c_float * len(table))(*table)
-        _lib.call("fx_wall_velocity", self._g, self.solid.data_ptr(), len(colliders), rows,
+        _lib.call("fx_moving_update_box", self._g, self._box, len(colliders), kinds, (ctypes.c_float * len(inv))(*inv),
+                  (ctypes.c_float * len(table))(*table), self.solid.data_ptr(), self.flags.data_ptr(),
+                  self.clearance.data_ptr(), self._scratch.data_ptr(), self._scratch2.data_ptr(),
                   *(f.ptr for f in self.wall_fields), self.device.stream)
         self.colliders = tuple(colliders)
         self.version += 1
 
     def remap(self, solver):
         """Empty the cells and rest the faces the motion covered or exposed
-        (MovingSolidDevice.remap); swaps the solver's ping-pong fields."""
-        g, s, o, st = self._g, self.solid.data_ptr(), self.previous.data_ptr(), self.device.stream
-        for front, back in (("_front", "_back"), ("_temperature", "_temperature_back")):
-            _lib.call("fx_remap_scalar", g, s, o, getattr(solver, front).ptr, getattr(solver, back).ptr, st)
-            a, b = getattr(solver, front), getattr(solver, back)
-            setattr(solver, front, b)
-            setattr(solver, back, a)
-        _lib.call("fx_remap_faces", g, s, o, *(f.ptr for f in self.wall_fields), *(f.ptr for f in solver._velocity),
-                  *(f.ptr for f in solver._velocity_back), st)
-        solver._velocity, solver._velocity_back = solver._velocity_back, solver._velocity
+        (MovingSolidDevice.remap), in place over the last update's box."""
+        fuel = solver.combustion.fuel.ptr if solver.combustion else None
+        _lib.call("fx_remap_box", self._g, self._box, self.solid.data_ptr(), self.previous.data_ptr(),
+                  *(f.ptr for f in self.wall_fields), *(f.ptr for f in solver._velocity), solver._front.ptr,
+                  solver._temperature.ptr, fuel, self.device.stream)
         if solver.combustion:
-            c = solver.combustion
-            _lib.call("fx_remap_scalar", g, s, o, c.fuel.ptr, c.spare.ptr, st)
-            c.fuel, c.spare = c.spare, c.fuel
-            c.flame.clear(format="FLOAT", value=(0.0,))
+            solver.combustion.flame.clear(format="FLOAT", value=(0.0,))
 
     def close(self):
-        self.previous = self._scratch = None
+        self.previous = self._scratch = self._scratch2 = None
         self.wall_fields = []
         self.colliders = ()
         super().close()

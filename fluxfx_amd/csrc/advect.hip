@@ -457,10 +457,12 @@ __device__ __forceinline__ V3 velocity_at_face_in(const GridDev& g, const Vel& V
 // separable L-infinity distance transform, one axis per pass:
 // pass 0: d = min |k| over solid/out-of-range cells c+k*e_x (|k| <= 254)
 // pass 1,2: d = min over |k| of max(|k|, d_prev[c+k*e_a]), out-of-range -> |k|
+// over the cells [x0, x1) x [y0, y1) x [z0 + blockIdx.z] of the grid
 __global__ void k_clearance(int nx, int ny, int nz, int axis, const uint8_t* __restrict__ in, uint8_t* __restrict__ out,
-                            int radius) {
-  int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y * blockDim.y + threadIdx.y, z = blockIdx.z;
-  if (x >= nx || y >= ny) return;
+                            int radius, int x0, int y0, int z0, int x1, int y1) {
+  int x = x0 + blockIdx.x * blockDim.x + threadIdx.x, y = y0 + blockIdx.y * blockDim.y + threadIdx.y,
+      z = z0 + blockIdx.z;
+  if (x >= x1 || y >= y1) return;
   int n = axis == 0 ? nx : (axis == 1 ? ny : nz);
   int c = axis == 0 ? x : (axis == 1 ? y : z);
   int64_t stride = axis == 0 ? 1 : (axis == 1 ? int64_t(nx) : int64_t(nx) * ny);
@@ -1260,6 +1262,28 @@ __global__ void k_max_abs(MaxArgs A, float* out) {
 
 static const dim3 kBlock(64, 4, 1);
 
+// the clearance of the cells in box b = [x0, x1) x [y0, y1) x [z0, z1) from
+// the solid mask, the other cells' kept: a cell's clearance depends on the
+// mask within kClearanceRadius cells, so pass 0 runs over the box widened by
+// the radius along y and z, pass 1 along z (both into scratch arrays, as the
+// widened cells' own clearance must not change), pass 2 over the box
+int launch_clearance_box(const fx_grid& g, const int* b, const uint8_t* solid, uint8_t* clearance, uint8_t* scratch,
+                         uint8_t* scratch2, hipStream_t s) {
+  const int r = kClearanceRadius;
+  const int y0 = max(b[1] - r, 0), y1 = min(b[4] + r, g.ny), z0 = max(b[2] - r, 0), z1 = min(b[5] + r, g.nz);
+  const int bx = b[3] - b[0], by = b[4] - b[1], bz = b[5] - b[2];
+  if (bx <= 0 || by <= 0 || bz <= 0) return FX_OK;
+  auto grid = [](int ex, int ey, int ez) { return dim3((ex + 63) / 64, (ey + 3) / 4, ez); };
+  hipLaunchKernelGGL(k_clearance, grid(bx, y1 - y0, z1 - z0), kBlock, 0, s, g.nx, g.ny, g.nz, 0, solid, scratch, r,
+                     b[0], y0, z0, b[3], y1);
+  hipLaunchKernelGGL(k_clearance, grid(bx, by, z1 - z0), kBlock, 0, s, g.nx, g.ny, g.nz, 1, (const uint8_t*)scratch,
+                     scratch2, r, b[0], b[1], z0, b[3], b[4]);
+  hipLaunchKernelGGL(k_clearance, grid(bx, by, bz), kBlock, 0, s, g.nx, g.ny, g.nz, 2, (const uint8_t*)scratch2,
+                     clearance, r, b[0], b[1], b[2], b[3], b[4]);
+  FX_LAUNCHED("clearance box");
+  return FX_OK;
+}
+
 static std::atomic<int> g_fast_paths{1};
 
 GridDev dev_grid(const fx_grid& g) {
@@ -1534,10 +1558,12 @@ int fx_build_clearance(fx_grid g, const uint8_t* solid, uint8_t* clearance, uint
              FX_EINVAL, "clearance: needs solid, output and scratch arrays");
   dim3 gr = grid3(g.nx, g.ny, g.nz, kBlock);
   hipStream_t s = S(stream);
-  const int radius = 15;  // clearances above 15 cells are stored as 255 (far)
-  hipLaunchKernelGGL(k_clearance, gr, kBlock, 0, s, g.nx, g.ny, g.nz, 0, solid, scratch, radius);
-  hipLaunchKernelGGL(k_clearance, gr, kBlock, 0, s, g.nx, g.ny, g.nz, 1, (const uint8_t*)scratch, clearance, radius);
-  hipLaunchKernelGGL(k_clearance, gr, kBlock, 0, s, g.nx, g.ny, g.nz, 2, (const uint8_t*)clearance, scratch, radius);
+  const int r = kClearanceRadius;
+  hipLaunchKernelGGL(k_clearance, gr, kBlock, 0, s, g.nx, g.ny, g.nz, 0, solid, scratch, r, 0, 0, 0, g.nx, g.ny);
+  hipLaunchKernelGGL(k_clearance, gr, kBlock, 0, s, g.nx, g.ny, g.nz, 1, (const uint8_t*)scratch, clearance, r, 0, 0,
+                     0, g.nx, g.ny);
+  hipLaunchKernelGGL(k_clearance, gr, kBlock, 0, s, g.nx, g.ny, g.nz, 2, (const uint8_t*)clearance, scratch, r, 0, 0,
+                     0, g.nx, g.ny);
   FX_HIP(hipMemcpyAsync(clearance, scratch, size_t(g.nx) * g.ny * g.nz, hipMemcpyDeviceToDevice, s));
   FX_LAUNCHED("fx_build_clearance");
   return FX_OK;

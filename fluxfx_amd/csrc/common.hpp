@@ -178,6 +178,15 @@ __device__ __forceinline__ void block_partial(double v, double* ws, int q) {
   __syncthreads();  // `part` is reused by the next call in the same kernel
 }
 
+// clearances above this many cells are stored as 255 (far)
+constexpr int kClearanceRadius = 15;
+
+// box-restricted mask passes of the moving colliders (advect.hip, pressure.hip);
+// b = {x0, y0, z0, x1, y1, z1}, half-open cell ranges inside the grid
+int launch_clearance_box(const fx_grid& g, const int* b, const uint8_t* solid, uint8_t* clearance, uint8_t* scratch,
+                         uint8_t* scratch2, hipStream_t s);
+int launch_flags_box(const fx_grid& g, const int* b, const uint8_t* solid, uint8_t* flags, hipStream_t s);
+
 // host helpers (reduce.hip)
 void ws_finalize(double* ws, int q0, int nq, hipStream_t s);
 void ws_finalize_mask(double* ws, uint32_t mask, hipStream_t s);

@@ -84,11 +84,202 @@ __global__ void k_remap_face(int nx, int ny, int nz, const uint8_t* __restrict__
   out[i] = value;
 }
 
+// ---------------------------------------------------------------------------
+// Box-restricted update.  With moving colliders only, every solid cell before
+// and after a move lies inside the box around the colliders' old and new
+// poses, so outside it the mask is 0 before and after, the wall velocity 0,
+// and neither the flags nor (farther than the clearance radius) the clearance
+// change: the passes above, run over the box (the flags one cell wider, the
+// clearance kClearanceRadius wider), give the whole-grid results.  Threads map
+// to [x0, x1) x [y0, y1) x [z0 + blockIdx.z] of the cells (faces: the box's
+// faces, the high face of the last cell included).
+// ---------------------------------------------------------------------------
+struct Box {
+  int x0, y0, z0, x1, y1, z1;
+};
+
+__device__ __forceinline__ bool box_cell(const Box& b, int& x, int& y, int& z, int ex, int ey) {
+  x = b.x0 + blockIdx.x * blockDim.x + threadIdx.x;
+  y = b.y0 + blockIdx.y * blockDim.y + threadIdx.y;
+  z = b.z0 + blockIdx.z;
+  return x < b.x1 + ex && y < b.y1 + ey;
+}
+
+// owner = 0 inside the box
+__global__ void k_box_clear(int nx, int ny, Box b, uint8_t* __restrict__ owner) {
+  int x, y, z;
+  if (!box_cell(b, x, y, z, 0, 0)) return;
+  owner[idx3(x, y, z, nx, ny)] = 0;
+}
+
+__global__ void k_box_raster(int nx, int ny, Box b, float hx, float hy, float hz, int kind, WallRows rows, int k,
+                             uint8_t* __restrict__ mask) {
+  int x, y, z;
+  if (!box_cell(b, x, y, z, 0, 0)) return;
+  const float* r0 = rows.r[k][0];
+  const float* r1 = rows.r[k][1];
+  const float* r2 = rows.r[k][2];
+  const float px = (x + 0.5f) * hx, py = (y + 0.5f) * hy, pz = (z + 0.5f) * hz;
+  // the same expression as k_owner_raster, term for term
+  const float qx = r0[0] * px + r0[1] * py + r0[2] * pz + r0[3];
+  const float qy = r1[0] * px + r1[1] * py + r1[2] * pz + r1[3];
+  const float qz = r2[0] * px + r2[1] * py + r2[2] * pz + r2[3];
+  const bool inside = kind == 0 ? (qx * qx + qy * qy + qz * qz <= 1.0f)
+                                : (fabsf(qx) <= 1.0f && fabsf(qy) <= 1.0f && fabsf(qz) <= 1.0f);
+  const int64_t i = idx3(x, y, z, nx, ny);
+  if (inside && mask[i] == 0) mask[i] = uint8_t(k + 1);
+}
+
+template <int A>
+__global__ void k_box_wall_velocity(int nx, int ny, int nz, Box b, float hx, float hy, float hz,
+                                    const uint8_t* __restrict__ owner, WallRows rows, float* __restrict__ out) {
+  int x, y, z;
+  if (!box_cell(b, x, y, z, A == 0, A == 1)) return;
+  const int sx = nx + (A == 0), sy = ny + (A == 1);
+  const int fa = A == 0 ? x : (A == 1 ? y : z), na = A == 0 ? nx : (A == 1 ? ny : nz);
+  float value = 0.0f;
+  if (fa > 0 && fa < na) {
+    const int hi = owner[x + nx * (y + ny * z)];
+    const int lo = owner[(x - (A == 0)) + nx * ((y - (A == 1)) + ny * (z - (A == 2)))];
+    const int own = lo > 0 ? lo : hi;
+    if (own > 0) {
+      const float px = (x + (A == 0 ? 0.0f : 0.5f)) * hx, py = (y + (A == 1 ? 0.0f : 0.5f)) * hy,
+                  pz = (z + (A == 2 ? 0.0f : 0.5f)) * hz;
+      const float* row = rows.r[own - 1][A];
+      value = row[0] * px + row[1] * py + row[2] * pz + row[3];
+    }
+  }
+  out[x + int64_t(sx) * (y + int64_t(sy) * z)] = value;
+}
+
+// in place: the cells of the box solid now or before are emptied
+__global__ void k_box_remap_scalars(int nx, int ny, Box b, const uint8_t* __restrict__ solid,
+                                    const uint8_t* __restrict__ old_solid, float* f0, float* f1, float* f2) {
+  int x, y, z;
+  if (!box_cell(b, x, y, z, 0, 0)) return;
+  const int64_t i = idx3(x, y, z, nx, ny);
+  if (solid[i] || old_solid[i]) {
+    f0[i] = 0.0f;
+    if (f1) f1[i] = 0.0f;
+    if (f2) f2[i] = 0.0f;
+  }
+}
+
+// in place: k_remap_face's rule on the box's faces
+template <int A>
+__global__ void k_box_remap_face(int nx, int ny, int nz, Box b, const uint8_t* __restrict__ solid,
+                                 const uint8_t* __restrict__ old_solid, const float* __restrict__ wall,
+                                 float* __restrict__ f) {
+  int x, y, z;
+  if (!box_cell(b, x, y, z, A == 0, A == 1)) return;
+  const int sx = nx + (A == 0), sy = ny + (A == 1);
+  const int fa = A == 0 ? x : (A == 1 ? y : z), na = A == 0 ? nx : (A == 1 ? ny : nz);
+  const int64_t i = x + int64_t(sx) * (y + int64_t(sy) * z);
+  if (fa > 0 && fa < na) {
+    const int c = x + nx * (y + ny * z);
+    const int l = (x - (A == 0)) + nx * ((y - (A == 1)) + ny * (z - (A == 2)));
+    const bool blocked = solid[c] || solid[l], previous = old_solid[c] || old_solid[l];
+    if (blocked) f[i] = wall[i];
+    else if (previous) f[i] = 0.0f;
+  } else {
+    f[i] = 0.0f;
+  }
+}
+
+int check_box(const fx_grid& g, const int32_t* box, Box& b) {
+  FX_REQUIRE(box, FX_EINVAL, "box: null pointer");
+  b = Box{box[0], box[1], box[2], box[3], box[4], box[5]};
+  FX_REQUIRE(b.x0 >= 0 && b.y0 >= 0 && b.z0 >= 0 && b.x0 <= b.x1 && b.y0 <= b.y1 && b.z0 <= b.z1 && b.x1 <= g.nx &&
+                 b.y1 <= g.ny && b.z1 <= g.nz,
+             FX_EINVAL, "box [%d,%d) x [%d,%d) x [%d,%d) is not inside the %d x %d x %d grid", b.x0, b.x1, b.y0, b.y1,
+             b.z0, b.z1, g.nx, g.ny, g.nz);
+  return FX_OK;
+}
+
+inline dim3 box_grid(const Box& b, int ex, int ey, int ez) {
+  return dim3((b.x1 - b.x0 + ex + 63) / 64, (b.y1 - b.y0 + ey + 3) / 4, b.z1 - b.z0 + ez);
+}
+
 }  // namespace fx
 
 using namespace fx;
 
 extern "C" {
+
+int fx_moving_update_box(fx_grid g, const int32_t* box, int32_t count, const int32_t* kinds,
+                         const float* inverse_rows, const float* wall_rows, uint8_t* owner, uint8_t* flags,
+                         uint8_t* clearance, uint8_t* scratch, uint8_t* scratch2, float* bu, float* bv, float* bw,
+                         void* stream) {
+  if (int e = check_grid(g)) return e;
+  Box b;
+  if (int e = check_box(g, box, b)) return e;
+  FX_REQUIRE(count >= 1 && count <= 8 && kinds && inverse_rows && wall_rows, FX_EINVAL,
+             "moving update: one to eight colliders with kinds and rows");
+  FX_REQUIRE(owner && flags && clearance && scratch && scratch2 && bu && bv && bw, FX_EINVAL,
+             "moving update: null array");
+  FX_REQUIRE(scratch != scratch2 && scratch != clearance && scratch2 != clearance, FX_EINVAL,
+             "moving update: the clearance scratch arrays must be distinct");
+  WallRows inv{}, wall{};
+  for (int k = 0; k < count; ++k) {
+    FX_REQUIRE(kinds[k] == 0 || kinds[k] == 1, FX_EINVAL, "collider kind must be 0 (sphere) or 1 (box)");
+    for (int a = 0; a < 3; ++a)
+      for (int j = 0; j < 4; ++j) {
+        inv.r[k][a][j] = inverse_rows[12 * k + 4 * a + j];
+        wall.r[k][a][j] = wall_rows[12 * k + 4 * a + j];
+      }
+  }
+  if (b.x0 == b.x1 || b.y0 == b.y1 || b.z0 == b.z1) return FX_OK;
+  hipStream_t s = S(stream);
+  const dim3 block(64, 4, 1);
+  const float hx = g.ex / g.nx, hy = g.ey / g.ny, hz = g.ez / g.nz;
+  hipLaunchKernelGGL(k_box_clear, box_grid(b, 0, 0, 0), block, 0, s, g.nx, g.ny, b, owner);
+  for (int k = 0; k < count; ++k)
+    hipLaunchKernelGGL(k_box_raster, box_grid(b, 0, 0, 0), block, 0, s, g.nx, g.ny, b, hx, hy, hz, kinds[k], inv, k,
+                       owner);
+  FX_LAUNCHED("fx_moving_update_box raster");
+  const int wide[6] = {max(b.x0 - 1, 0), max(b.y0 - 1, 0), max(b.z0 - 1, 0),
+                       min(b.x1 + 1, g.nx), min(b.y1 + 1, g.ny), min(b.z1 + 1, g.nz)};
+  if (int e = launch_flags_box(g, wide, owner, flags, s)) return e;
+  const int r = kClearanceRadius;
+  const int far[6] = {max(b.x0 - r, 0), max(b.y0 - r, 0), max(b.z0 - r, 0),
+                      min(b.x1 + r, g.nx), min(b.y1 + r, g.ny), min(b.z1 + r, g.nz)};
+  if (int e = launch_clearance_box(g, far, owner, clearance, scratch, scratch2, s)) return e;
+  hipLaunchKernelGGL(k_box_wall_velocity<0>, box_grid(b, 1, 0, 0), block, 0, s, g.nx, g.ny, g.nz, b, hx, hy, hz, owner,
+                     wall, bu);
+  hipLaunchKernelGGL(k_box_wall_velocity<1>, box_grid(b, 0, 1, 0), block, 0, s, g.nx, g.ny, g.nz, b, hx, hy, hz, owner,
+                     wall, bv);
+  // W faces: the z range of the box's faces is z0 .. z1 (the high face of the last cell)
+  Box bz = b;
+  bz.z1 = b.z1 + 1;
+  hipLaunchKernelGGL(k_box_wall_velocity<2>, box_grid(bz, 0, 0, 0), block, 0, s, g.nx, g.ny, g.nz, bz, hx, hy, hz,
+                     owner, wall, bw);
+  FX_LAUNCHED("fx_moving_update_box");
+  return FX_OK;
+}
+
+int fx_remap_box(fx_grid g, const int32_t* box, const uint8_t* solid, const uint8_t* old_solid, const float* bu,
+                 const float* bv, const float* bw, float* u, float* v, float* w, float* s0, float* s1, float* s2,
+                 void* stream) {
+  if (int e = check_grid(g)) return e;
+  Box b;
+  if (int e = check_box(g, box, b)) return e;
+  FX_REQUIRE(solid && old_solid && bu && bv && bw && u && v && w && s0, FX_EINVAL, "remap box: null array");
+  if (b.x0 == b.x1 || b.y0 == b.y1 || b.z0 == b.z1) return FX_OK;
+  hipStream_t s = S(stream);
+  const dim3 block(64, 4, 1);
+  hipLaunchKernelGGL(k_box_remap_scalars, box_grid(b, 0, 0, 0), block, 0, s, g.nx, g.ny, b, solid, old_solid, s0, s1,
+                     s2);
+  hipLaunchKernelGGL(k_box_remap_face<0>, box_grid(b, 1, 0, 0), block, 0, s, g.nx, g.ny, g.nz, b, solid, old_solid,
+                     bu, u);
+  hipLaunchKernelGGL(k_box_remap_face<1>, box_grid(b, 0, 1, 0), block, 0, s, g.nx, g.ny, g.nz, b, solid, old_solid,
+                     bv, v);
+  Box bz = b;
+  bz.z1 = b.z1 + 1;
+  hipLaunchKernelGGL(k_box_remap_face<2>, box_grid(bz, 0, 0, 0), block, 0, s, g.nx, g.ny, g.nz, bz, solid, old_solid,
+                     bw, w);
+  FX_LAUNCHED("fx_remap_box");
+  return FX_OK;
+}
 
 int fx_owner_raster(fx_grid g, int32_t kind, const float* r, int32_t owner, uint8_t* mask, void* stream) {
   if (int e = check_grid(g)) return e;

@@ -48,9 +48,12 @@ __global__ void k_solid_raster(int nx, int ny, int nz, float hx, float hy, float
   if (inside) solid[idx3(x, y, z, nx, ny)] = 1;
 }
 
-__global__ void k_build_flags(int nx, int ny, int nz, const uint8_t* __restrict__ solid, uint8_t* flags) {
-  int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y * blockDim.y + threadIdx.y, z = blockIdx.z;
-  if (x >= nx || y >= ny) return;
+// open-face flags of the cells [x0, x1) x [y0, y1) x [z0 + blockIdx.z]
+__global__ void k_build_flags(int nx, int ny, int nz, const uint8_t* __restrict__ solid, uint8_t* flags, int x0, int y0,
+                              int z0, int x1, int y1) {
+  int x = x0 + blockIdx.x * blockDim.x + threadIdx.x, y = y0 + blockIdx.y * blockDim.y + threadIdx.y,
+      z = z0 + blockIdx.z;
+  if (x >= x1 || y >= y1) return;
   int64_t i = idx3(x, y, z, nx, ny);
   uint8_t f = box_flags(x, y, z, nx, ny, nz);
   if (solid) {
@@ -367,6 +370,16 @@ static bool rows_vec_ok(const fx_grid& g, const void* a, const void* b, const vo
   return g.nx % 4 == 0 && al(a) && al(b) && al(c) && al(d) && al(e) && al(f);
 }
 
+// open-face flags of the cells in box b = [x0, x1) x [y0, y1) x [z0, z1)
+int launch_flags_box(const fx_grid& g, const int* b, const uint8_t* solid, uint8_t* flags, hipStream_t s) {
+  const int bx = b[3] - b[0], by = b[4] - b[1], bz = b[5] - b[2];
+  if (bx <= 0 || by <= 0 || bz <= 0) return FX_OK;
+  hipLaunchKernelGGL(k_build_flags, dim3((bx + 63) / 64, (by + 3) / 4, bz), kBlock, 0, s, g.nx, g.ny, g.nz, solid,
+                     flags, b[0], b[1], b[2], b[3], b[4]);
+  FX_LAUNCHED("flags box");
+  return FX_OK;
+}
+
 static int launch_gradient(const fx_grid& g, const uint8_t* flags, const float* p, const float* dp, const float* u,
                            const float* v, const float* w, float* uo, float* vo, float* wo, float* psum,
                            float scale, hipStream_t s, const char* name, const float* const* walls = nullptr) {
@@ -427,7 +440,7 @@ int fx_build_flags(fx_grid g, const uint8_t* solid, uint8_t* flags, void* stream
   if (int e = check_grid(g)) return e;
   FX_REQUIRE(flags, FX_EINVAL, "null flags");
   dim3 gr = grid3(g.nx, g.ny, g.nz, kBlock);
-  hipLaunchKernelGGL(k_build_flags, gr, kBlock, 0, S(stream), g.nx, g.ny, g.nz, solid, flags);
+  hipLaunchKernelGGL(k_build_flags, gr, kBlock, 0, S(stream), g.nx, g.ny, g.nz, solid, flags, 0, 0, 0, g.nx, g.ny);
   FX_LAUNCHED("fx_build_flags");
   return FX_OK;
 }

@@ -178,6 +178,28 @@ int fx_remap_faces(fx_grid g, const uint8_t* solid, const uint8_t* old_solid, co
                    const float* bw, const float* u, const float* v, const float* w, float* uo, float* vo, float* wo,
                    void* stream);
 
+/* The update and remap above restricted to a box of cells, for colliders that
+ * are all moving (every solid cell before and after the move lies inside the
+ * box): box = {x0, y0, z0, x1, y1, z1}, half-open cell ranges inside the grid,
+ * covering the colliders' old and new poses.  fx_moving_update_box clears the
+ * owners in the box and rasters the `count` colliders (kinds 0 sphere, 1 box;
+ * 3 x 4 inverse rows each) in order, then rebuilds the open-face flags one
+ * cell wider, the clearance (fx_build_clearance) 15 cells wider through the
+ * two distinct scratch arrays, and the wall velocity (3 x 4 velocity rows per
+ * collider) on the box's faces; outside those ranges none of them can change,
+ * so the results equal the whole-grid passes.  fx_remap_box empties, in place,
+ * the box's cells solid now or before in up to three scalar fields (s1, s2
+ * nullable) and applies fx_remap_faces' rule to the box's faces of u, v, w in
+ * place (MovingSolidDevice.update/remap, backend/moving_collision.py:53-83,
+ * without the whole-grid passes). */
+int fx_moving_update_box(fx_grid g, const int32_t* box, int32_t count, const int32_t* kinds,
+                         const float* inverse_rows, const float* wall_rows, uint8_t* owner, uint8_t* flags,
+                         uint8_t* clearance, uint8_t* scratch, uint8_t* scratch2, float* bu, float* bv, float* bw,
+                         void* stream);
+int fx_remap_box(fx_grid g, const int32_t* box, const uint8_t* solid, const uint8_t* old_solid, const float* bu,
+                 const float* bv, const float* bw, float* u, float* v, float* w, float* s0, float* s1, float* s2,
+                 void* stream);
+
 /* out = a + b (n floats). */
 int fx_combine(const float* a, const float* b, float* out, int64_t n, void* stream);
 

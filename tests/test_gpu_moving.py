@@ -100,3 +100,79 @@ def test_moving_colliders_select_tolerance_projector_and_reject_static_use(devic
     sim = DenseProjectedSmoke(grid, PressureSettings(), device, colliders=(c,))
     with pytest.raises(ValueError):
         sim.move_colliders((c,), .01)
+
+
+def test_box_update_and_remap_equal_the_whole_grid_passes(device):
+    """The per-step update and remap run over the box the colliders' old and
+    new poses cover (fx_moving_update_box, fx_remap_box); the whole-grid
+    passes (fx_owner_raster, fx_build_flags, fx_build_clearance,
+    fx_wall_velocity, fx_remap_scalar, fx_remap_faces) on the same inputs give
+    the same bits everywhere, for a sphere and a turning box that approach each
+    other, one of them leaving the grid."""
+    import ctypes as C
+    from fluxfx_amd import _lib
+    from fluxfx_amd.backend.projected import DenseProjectedSmoke
+    from fluxfx_amd.physics.collision import Collider, primitive
+    from fluxfx_amd.physics.collider_motion import velocity_rows
+    from fluxfx_amd.physics.config import GridSpec
+    torch = device.torch
+    n = (40, 48, 36)
+    grid = GridSpec(n, (1.0, 1.2, 0.9))
+    s = _settings()
+    e = np.array(grid.extent)
+
+    def turned_box(center, half, angle):
+        c, sn = np.cos(angle), np.sin(angle)
+        rows = []
+        for a, h in zip([(c, sn, 0.0), (-sn, c, 0.0), (0.0, 0.0, 1.0)], half):
+            r = [v / h for v in a]
+            rows.append((r[0], r[1], r[2], -(r[0] * center[0] + r[1] * center[1] + r[2] * center[2])))
+        return Collider("BOX", tuple(rows))
+
+    pose = lambda k: (primitive("SPHERE", tuple(e * (.3 + .03 * k, .5, .45)), (.16,) * 3),
+                      turned_box(tuple(e * (.75, .5 - .02 * k, .1 - .02 * k)), (.12, .2, .1), .2 * k))
+    sim = DenseProjectedSmoke(grid, s, device, colliders=pose(0))
+    rng = np.random.default_rng(4)
+    sim.upload(rng.uniform(0, 1, grid.cells))
+    sim.upload_temperature(rng.uniform(0, 1, grid.cells))
+    sim.upload_velocity([rng.uniform(-1, 1, int(np.prod(sh))) for sh in grid.face_shapes])
+    g = _lib.make_grid(grid.shape, grid.extent)
+    st = device.stream
+    dt = .01
+    for k in range(1, 5):
+        print(f"box update: move {k}", flush=True)
+        old = sim.solids.colliders
+        new = pose(k)
+        prev = sim.solids.solid.clone()
+        vel = [f.tensor.clone() for f in sim._velocity]
+        rho, temp = sim.density.tensor.clone(), sim.temperature.tensor.clone()
+        sim.move_colliders(new, dt)
+        # the whole-grid passes on the same inputs
+        owner = torch.zeros_like(prev)
+        for j, c in enumerate(new, start=1):
+            rows = (C.c_float * 12)(*[float(v) for r in c.inverse_rows for v in r])
+            _lib.call("fx_owner_raster", g, 0 if c.shape == "SPHERE" else 1, rows, j, owner.data_ptr(), st)
+        flags = torch.zeros_like(prev)
+        _lib.call("fx_build_flags", g, owner.data_ptr(), flags.data_ptr(), st)
+        clear, scratch = torch.zeros_like(prev), torch.zeros_like(prev)
+        _lib.call("fx_build_clearance", g, owner.data_ptr(), clear.data_ptr(), scratch.data_ptr(), st)
+        walls = [torch.zeros_like(v) for v in vel]
+        table = [v for a, b in zip(old, new) for row in velocity_rows(a, b, dt) for v in row]
+        _lib.call("fx_wall_velocity", g, owner.data_ptr(), len(new), (C.c_float * len(table))(*table),
+                  *(w.data_ptr() for w in walls), st)
+        outs = [torch.empty_like(v) for v in vel]
+        _lib.call("fx_remap_faces", g, owner.data_ptr(), prev.data_ptr(), *(w.data_ptr() for w in walls),
+                  *(v.data_ptr() for v in vel), *(o.data_ptr() for o in outs), st)
+        rho_o, temp_o = torch.empty_like(rho), torch.empty_like(temp)
+        _lib.call("fx_remap_scalar", g, owner.data_ptr(), prev.data_ptr(), rho.data_ptr(), rho_o.data_ptr(), st)
+        _lib.call("fx_remap_scalar", g, owner.data_ptr(), prev.data_ptr(), temp.data_ptr(), temp_o.data_ptr(), st)
+        torch.cuda.synchronize()
+        sol = sim.solids
+        assert torch.equal(sol.solid, owner) and torch.equal(sol.flags, flags) and torch.equal(sol.clearance, clear)
+        for a in range(3):
+            assert torch.equal(sol.wall_fields[a].tensor, walls[a]), a
+            assert torch.equal(sim._velocity[a].tensor, outs[a]), a
+        assert torch.equal(sim.density.tensor, rho_o) and torch.equal(sim.temperature.tensor, temp_o)
+        assert torch.equal(sol.previous, prev)
+        sim.step(dt)
+    sim.close()
