@@ -468,7 +468,9 @@ __global__ void k_direct(int m, const double* __restrict__ M, const T* __restric
 // removes each part's mean from b (the restricted residual's rounding leaves
 // some, and CG would blow it up along the null space) and from the result.
 // Part sums run over the cells in index order, so results are deterministic.
-// CG starts from p = 0 and ends when ||r|| <= 1e-12 ||b||, when p.Bp <= 0 or
+// CG, preconditioned by the diagonal (cells next to nearly closed coarse faces
+// have small diagonals), starts from p = 0 and ends when ||r|| <= 1e-10 ||b||
+// (far below what a V-cycle's coarse correction needs), when p.Bp <= 0 or
 // after `maxit` iterations; every thread computes the same alpha and beta
 // from the same LDS partial sums, so control flow stays uniform.  Cells
 // without conductance hold 0.  info (nullable): {iterations, ||r|| / ||b||}.
@@ -480,7 +482,7 @@ __global__ __launch_bounds__(kCgMaxCells) void k_coarse_cg(LevelDev L, const T* 
   __shared__ double P[kCgMaxCells];
   __shared__ double M[kCgMaxCells];
   __shared__ int LB[kCgMaxCells];
-  __shared__ double part[2][kCgMaxCells / 64];
+  __shared__ double part[2][kCgMaxCells / 64][2];
   const int m = L.nx * L.ny * L.nz;
   const int i = threadIdx.x;
   const int nw = blockDim.x >> 6, lane = i & 63, wv = i >> 6;
@@ -550,25 +552,42 @@ __global__ __launch_bounds__(kCgMaxCells) void k_coarse_cg(LevelDev L, const T* 
     __syncthreads();  // P is written again by the next use
     return r;
   };
-  // block sum, alternating between the two partial buffers: a buffer is
-  // written again only after the barrier of the other buffer's sum, by which
-  // time every thread has read it
+  // block sums of two values, alternating between the two partial buffers: a
+  // buffer is written again only after the barrier of the other buffer's sum,
+  // by which time every thread has read it
   int buf = 0;
-  auto block_sum = [&](double v) {
+  auto block_sum2 = [&](double u, double v, double& su, double& sv) {
+    u = wave_sum(u);
     v = wave_sum(v);
-    if (lane == 0) part[buf][wv] = v;
+    if (lane == 0) {
+      part[buf][wv][0] = u;
+      part[buf][wv][1] = v;
+    }
     __syncthreads();
-    double s = 0.0;
-    for (int k = 0; k < nw; ++k) s += part[buf][k];
+    su = sv = 0.0;
+    for (int k = 0; k < nw; ++k) {
+      su += part[buf][k][0];
+      sv += part[buf][k][1];
+    }
     buf ^= 1;
-    return s;
   };
+  auto block_sum1 = [&](double u) {
+    u = wave_sum(u);
+    if (lane == 0) part[buf][wv][0] = u;
+    __syncthreads();
+    double su = 0.0;
+    for (int k = 0; k < nw; ++k) su += part[buf][k][0];
+    buf ^= 1;
+    return su;
+  };
+  const double inv_d = live ? 1.0 / d : 0.0;
   const double b = part_free(live ? -double(scale) * double(to_f(rhs[i])) : 0.0);
-  double x = 0.0, r = b, p = b;
-  double rr = block_sum(r * r);
+  double x = 0.0, r = b, z = b * inv_d, p = z;
+  double rr, rz;
+  block_sum2(r * r, r * z, rr, rz);
   const double bb = rr;
   int it = 0;
-  while (it < maxit && rr > 1e-24 * bb) {
+  while (it < maxit && rr > 1e-20 * bb) {
     P[i] = p;
     __syncthreads();
     double q = 0.0;
@@ -578,16 +597,18 @@ __global__ __launch_bounds__(kCgMaxCells) void k_coarse_cg(LevelDev L, const T* 
       for (int k = 0; k < 6; ++k)
         if (w[k] != 0.0) q -= w[k] * P[nb[k]];
     }
-    const double pq = block_sum(p * q);
+    const double pq = block_sum1(p * q);
     if (!(pq > 0.0)) break;
-    const double alpha = rr / pq;
+    const double alpha = rz / pq;
     x += alpha * p;
     r -= alpha * q;
-    const double rn = block_sum(r * r);
+    z = r * inv_d;
+    double rz_new;
+    block_sum2(r * r, r * z, rr, rz_new);
     ++it;
-    const double beta = rn / rr;
-    rr = rn;
-    p = r + beta * p;
+    const double beta = rz_new / rz;
+    rz = rz_new;
+    p = z + beta * p;
   }
   __syncthreads();  // the last iteration's P reads are done before part_free writes P
   x = part_free(x);

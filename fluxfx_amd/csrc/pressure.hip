@@ -241,10 +241,12 @@ __device__ __forceinline__ uint32_t row_flags(const uint8_t* __restrict__ flags,
   return w;
 }
 
+// bu/bv/bw (nullable): the wall velocity closed faces read (moving colliders)
 __global__ __launch_bounds__(256) void k_divergence_q(int nx, int ny, int nz, float ix, float iy, float iz,
                                                       const uint8_t* __restrict__ flags, const float* __restrict__ u,
                                                       const float* __restrict__ v, const float* __restrict__ w,
-                                                      float* __restrict__ div, int nq) {
+                                                      const float* __restrict__ bu, const float* __restrict__ bv,
+                                                      const float* __restrict__ bw, float* __restrict__ div, int nq) {
   const int q = xcd_row_tile(blockIdx.x, gridDim.x) * 256 + threadIdx.x;
   if (q >= nq) return;
   const int qx = nx >> 2, row = q / qx, x0 = (q - row * qx) << 2, z = row / ny, y = row - z * ny;
@@ -260,9 +262,12 @@ __global__ __launch_bounds__(256) void k_divergence_q(int nx, int ny, int nz, fl
     const uint8_t f = uint8_t(fw >> (8 * k));
     d[k] = 0.0f;
     if (f & FLAG_FLUID) {
-      const float a0 = (f & FLAG_XLO) ? uu[k] : 0.0f, a1 = (f & FLAG_XHI) ? uu[k + 1] : 0.0f;
-      const float b0 = (f & FLAG_YLO) ? comp(v0, k) : 0.0f, b1 = (f & FLAG_YHI) ? comp(v1, k) : 0.0f;
-      const float c0 = (f & FLAG_ZLO) ? comp(w0, k) : 0.0f, c1 = (f & FLAG_ZHI) ? comp(w1, k) : 0.0f;
+      const float a0 = (f & FLAG_XLO) ? uu[k] : closed_value(bu, iu + k),
+                  a1 = (f & FLAG_XHI) ? uu[k + 1] : closed_value(bu, iu + k + 1);
+      const float b0 = (f & FLAG_YLO) ? comp(v0, k) : closed_value(bv, iv + k),
+                  b1 = (f & FLAG_YHI) ? comp(v1, k) : closed_value(bv, iv + nx + k);
+      const float c0 = (f & FLAG_ZLO) ? comp(w0, k) : closed_value(bw, c + k),
+                  c1 = (f & FLAG_ZHI) ? comp(w1, k) : closed_value(bw, c + sz + k);
       d[k] = (a1 - a0) * ix + (b1 - b0) * iy + (c1 - c0) * iz;
     }
   }
@@ -272,13 +277,16 @@ __global__ __launch_bounds__(256) void k_divergence_q(int nx, int ny, int nz, fl
 // Gradient correction of the faces at the low sides of the lane's cells (and
 // the closed faces at x = nx, y = ny, z = nz next to the last cells); with dp
 // the neighbour differences of p and dp are formed separately, and psum (if
-// given) receives p + dp.
+// given) receives p + dp.  Closed faces take 0, or with bu/bv/bw the wall
+// velocity of moving colliders.
 __global__ __launch_bounds__(256) void k_gradient_q(int nx, int ny, int nz, float ix, float iy, float iz, float scale,
                                                     const uint8_t* __restrict__ flags, const float* __restrict__ p,
                                                     const float* __restrict__ dp, const float* __restrict__ u,
                                                     const float* __restrict__ v, const float* __restrict__ w,
-                                                    float* __restrict__ uo, float* __restrict__ vo,
-                                                    float* __restrict__ wo, float* __restrict__ psum, int nq) {
+                                                    const float* __restrict__ bu, const float* __restrict__ bv,
+                                                    const float* __restrict__ bw, float* __restrict__ uo,
+                                                    float* __restrict__ vo, float* __restrict__ wo,
+                                                    float* __restrict__ psum, int nq) {
   const int q = xcd_row_tile(blockIdx.x, gridDim.x) * 256 + threadIdx.x;
   if (q >= nq) return;
   const int qx = nx >> 2, row = q / qx, x0 = (q - row * qx) << 2, z = row / ny, y = row - z * ny;
@@ -307,32 +315,39 @@ __global__ __launch_bounds__(256) void k_gradient_q(int nx, int ny, int nz, floa
     const uint8_t f = uint8_t(fw >> (8 * k));
     const float pk = comp(pc, k);
     // x face at x0 + k: open iff interior and the cell's low-x bit is set
-    ur[k] = 0.0f;
     if (x0 + k > 0 && (f & FLAG_XLO)) {
       float g = pk - (k == 0 ? pxm : comp(pc, k - 1));
       if (dp) g += comp(dc, k) - (k == 0 ? dxm : comp(dc, k - 1));
       ur[k] = uu[k] - scale * g * ix;
+    } else {
+      ur[k] = closed_value(bu, iu + k);
     }
-    vr[k] = 0.0f;
     if (y > 0 && (f & FLAG_YLO)) {
       float g = pk - comp(pym, k);
       if (dp) g += comp(dc, k) - comp(dym, k);
       vr[k] = comp(vv, k) - scale * g * iy;
+    } else {
+      vr[k] = closed_value(bv, iv + k);
     }
-    wr[k] = 0.0f;
     if (z > 0 && (f & FLAG_ZLO)) {
       float g = pk - comp(pzm, k);
       if (dp) g += comp(dc, k) - comp(dzm, k);
       wr[k] = comp(ww, k) - scale * g * iz;
+    } else {
+      wr[k] = closed_value(bw, c + k);
     }
   }
 #pragma unroll
   for (int k = 0; k < 4; ++k) uo[iu + k] = ur[k];
-  if (x0 + 4 == nx) uo[iu + 4] = 0.0f;
+  if (x0 + 4 == nx) uo[iu + 4] = closed_value(bu, iu + 4);
   st4f(vo, iv, make_float4(vr[0], vr[1], vr[2], vr[3]));
-  if (y == ny - 1) st4f(vo, iv + nx, make_float4(0.f, 0.f, 0.f, 0.f));
+  if (y == ny - 1)
+    st4f(vo, iv + nx, make_float4(closed_value(bv, iv + nx), closed_value(bv, iv + nx + 1),
+                                  closed_value(bv, iv + nx + 2), closed_value(bv, iv + nx + 3)));
   st4f(wo, c, make_float4(wr[0], wr[1], wr[2], wr[3]));
-  if (z == nz - 1) st4f(wo, c + sz, make_float4(0.f, 0.f, 0.f, 0.f));
+  if (z == nz - 1)
+    st4f(wo, c + sz, make_float4(closed_value(bw, c + sz), closed_value(bw, c + sz + 1), closed_value(bw, c + sz + 2),
+                                 closed_value(bw, c + sz + 3)));
   if (psum) st4f(psum, c, make_float4(pc.x + dc.x, pc.y + dc.y, pc.z + dc.z, pc.w + dc.w));
 }
 
@@ -387,10 +402,10 @@ static int launch_gradient(const fx_grid& g, const uint8_t* flags, const float* 
   const float* bu = walls ? walls[0] : nullptr;
   const float* bv = walls ? walls[1] : nullptr;
   const float* bw = walls ? walls[2] : nullptr;
-  if (!walls && rows_vec_ok(g, flags, p, dp, v, w, vo) && rows_vec_ok(g, wo, psum, nullptr, nullptr)) {
+  if (rows_vec_ok(g, flags, p, dp, v, w, vo) && rows_vec_ok(g, wo, psum, nullptr, nullptr)) {
     const int nq = g.nx / 4 * g.ny * g.nz;
     hipLaunchKernelGGL(k_gradient_q, dim3((nq + 255) / 256), dim3(256), 0, s, g.nx, g.ny, g.nz, ix, iy, iz, scale,
-                       flags, p, dp, u, v, w, uo, vo, wo, psum, nq);
+                       flags, p, dp, u, v, w, bu, bv, bw, uo, vo, wo, psum, nq);
   } else {
     dim3 gr = grid3(g.nx + 1, g.ny + 1, g.nz + 1, kBlock);
     hipLaunchKernelGGL(k_gradient, gr, kBlock, 0, s, g.nx, g.ny, g.nz, ix, iy, iz, scale, flags, p, dp, u, v, w, bu,
@@ -502,7 +517,8 @@ int fx_divergence(fx_grid g, const uint8_t* flags, const float* u, const float* 
   if (rows_vec_ok(g, flags, v, w, div)) {
     const int nq = g.nx / 4 * g.ny * g.nz;
     hipLaunchKernelGGL(k_divergence_q, dim3((nq + 255) / 256), dim3(256), 0, S(stream), g.nx, g.ny, g.nz,
-                       g.nx / g.ex, g.ny / g.ey, g.nz / g.ez, flags, u, v, w, div, nq);
+                       g.nx / g.ex, g.ny / g.ey, g.nz / g.ez, flags, u, v, w, (const float*)nullptr,
+                       (const float*)nullptr, (const float*)nullptr, div, nq);
   } else {
     dim3 gr = grid3(g.nx, g.ny, g.nz, kBlock);
     hipLaunchKernelGGL(k_divergence, gr, kBlock, 0, S(stream), g.nx, g.ny, g.nz, g.nx / g.ex, g.ny / g.ey,
@@ -517,9 +533,15 @@ int fx_divergence_moving(fx_grid g, const uint8_t* flags, const float* u, const 
                          const float* bu, const float* bv, const float* bw, float* div, void* stream) {
   if (int e = check_grid(g)) return e;
   FX_REQUIRE(flags && u && v && w && bu && bv && bw && div, FX_EINVAL, "null array");
-  dim3 gr = grid3(g.nx, g.ny, g.nz, kBlock);
-  hipLaunchKernelGGL(k_divergence, gr, kBlock, 0, S(stream), g.nx, g.ny, g.nz, g.nx / g.ex, g.ny / g.ey, g.nz / g.ez,
-                     flags, u, v, w, bu, bv, bw, div);
+  if (rows_vec_ok(g, flags, v, w, div)) {
+    const int nq = g.nx / 4 * g.ny * g.nz;
+    hipLaunchKernelGGL(k_divergence_q, dim3((nq + 255) / 256), dim3(256), 0, S(stream), g.nx, g.ny, g.nz,
+                       g.nx / g.ex, g.ny / g.ey, g.nz / g.ez, flags, u, v, w, bu, bv, bw, div, nq);
+  } else {
+    dim3 gr = grid3(g.nx, g.ny, g.nz, kBlock);
+    hipLaunchKernelGGL(k_divergence, gr, kBlock, 0, S(stream), g.nx, g.ny, g.nz, g.nx / g.ex, g.ny / g.ey,
+                       g.nz / g.ez, flags, u, v, w, bu, bv, bw, div);
+  }
   FX_LAUNCHED("fx_divergence_moving");
   return FX_OK;
 }
