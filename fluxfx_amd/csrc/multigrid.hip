@@ -242,6 +242,34 @@ __device__ __forceinline__ void prolong_row(const float (&cv)[3][3][W / 2 + 2], 
   }
 }
 
+// the W/2 consecutive coarse values at an index that is a multiple of W/2
+// (8-byte aligned: two fp32 or four 16-bit values) in one load
+template <typename T, int N>
+__device__ __forceinline__ void ld_span(const T* __restrict__ p, int64_t i, float* out);
+template <>
+__device__ __forceinline__ void ld_span<float, 2>(const float* __restrict__ p, int64_t i, float* out) {
+  const float2 a = *reinterpret_cast<const float2*>(p + i);
+  out[0] = a.x;
+  out[1] = a.y;
+}
+template <>
+__device__ __forceinline__ void ld_span<bf16, 4>(const bf16* __restrict__ p, int64_t i, float* out) {
+  const uint2 a = *reinterpret_cast<const uint2*>(p + i);
+  out[0] = __uint_as_float(a.x << 16);
+  out[1] = __uint_as_float(a.x & 0xffff0000u);
+  out[2] = __uint_as_float(a.y << 16);
+  out[3] = __uint_as_float(a.y & 0xffff0000u);
+}
+template <>
+__device__ __forceinline__ void ld_span<_Float16, 4>(const _Float16* __restrict__ p, int64_t i, float* out) {
+  typedef _Float16 half4_t __attribute__((ext_vector_type(4)));
+  const half4_t a = *reinterpret_cast<const half4_t*>(p + i);
+  out[0] = float(a[0]);
+  out[1] = float(a[1]);
+  out[2] = float(a[2]);
+  out[3] = float(a[3]);
+}
+
 template <typename T, int FOP, int MODE, int W>
 __global__ __launch_bounds__(256) void k_prolong_q(LevelDev F, LevelDev C, const uint8_t* __restrict__ cact,
                                                    T* __restrict__ p, const T* __restrict__ coarse, int nu) {
@@ -263,6 +291,8 @@ __global__ __launch_bounds__(256) void k_prolong_q(LevelDev F, LevelDev C, const
     int xs[CW];
 #pragma unroll
     for (int k = 0; k < CW; ++k) xs[k] = min(max(cx0 - 1 + k, 0), C.nx - 1);
+    // away from the x ends the window's middle W/2 values are one aligned load
+    const bool inner = cx0 >= 1 && cx0 + W / 2 < C.nx;
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
       const int cz = min(max(Z - 1 + a, 0), C.nz - 1);
@@ -270,8 +300,14 @@ __global__ __launch_bounds__(256) void k_prolong_q(LevelDev F, LevelDev C, const
       for (int b = 0; b < 3; ++b) {
         const int cy = min(max(Y - 1 + b, 0), C.ny - 1);
         const int64_t row = (int64_t(cz) * C.ny + cy) * C.nx;
+        if (inner) {
+          cv[a][b][0] = to_f(coarse[row + cx0 - 1]);
+          ld_span<T, W / 2>(coarse, row + cx0, &cv[a][b][1]);
+          cv[a][b][CW - 1] = to_f(coarse[row + cx0 + W / 2]);
+        } else {
 #pragma unroll
-        for (int k = 0; k < CW; ++k) cv[a][b][k] = to_f(coarse[row + xs[k]]);
+          for (int k = 0; k < CW; ++k) cv[a][b][k] = to_f(coarse[row + xs[k]]);
+        }
       }
     }
     if (MODE == FX_PROLONG_RENORM && cact) {
@@ -1082,7 +1118,9 @@ static int launch_prolong(const fx_mg_config* c, int l, const void* in, const vo
   const int64_t fcells = int64_t(F.nx) * F.ny * F.nz;
   const bool coarse_known = mode != FX_PROLONG_RENORM || C.op == FX_OP_BOX || (op_has_edges(C.op) && C.active);
   const int W = storage_width(c->storage);
-  if (in == out && fcells >= kQuadMinCells && coarse_known && C.op != FX_OP_FLAGS &&
+  // the vector kernel reads W/2 coarse values at a time: an 8-byte aligned coarse array
+  const bool coarse_aligned = (reinterpret_cast<uintptr_t>(coarse) & 7u) == 0;
+  if (in == out && fcells >= kQuadMinCells && coarse_known && C.op != FX_OP_FLAGS && coarse_aligned &&
       vec_ok(F, out, nullptr, nullptr, W)) {
     const int nu = int(int64_t(F.nx / W) * C.ny * C.nz);
     dim3 grid((nu + 255) / 256);
