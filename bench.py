@@ -424,6 +424,72 @@ def bricks_bench(device, n, steps, warmup, cpu):
     print(json.dumps(line), flush=True)
 
 
+def hbm_bench(device, sizes=(256, 320, 384), copy_mib=(64, 128, 256, 512, 1024, 2048), reps=40):
+    """The level-0 smoother per sweep (FP32 and BF16 storage, a 16^3 solid
+    block at the centre) at grid sides whose working set lies below and above
+    the 256 MB last-level cache, and the copy kernel fx_scale (4 B read and
+    4 B written per element, one element per thread) beside the HIP runtime's
+    device-to-device copy across array sizes: the HBM rate a stencil sustains
+    once its arrays no longer fit that cache."""
+    import torch
+    from fluxfx_amd import _lib
+    from fluxfx_amd.backend.collision import SolidMask
+    from fluxfx_amd.backend.tolerance import ToleranceProjector
+    from fluxfx_amd.physics.config import GridSpec
+    from fluxfx_amd.physics.pressure import PressureSettings
+
+    def timed(fn):
+        fn()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            fn()
+        e1.record()
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / reps
+
+    smoother = []
+    for n in sizes:
+        for prec, per_cell in (("FP32", 13), ("BF16", 7)):
+            g = GridSpec((n,) * 3, (1.0,) * 3, max_dim=1024)
+            mask = np.zeros((n, n, n), np.uint8)
+            mask[n // 2 - 8:n // 2 + 8, n // 2 - 8:n // 2 + 8, n // 2 - 8:n // 2 + 8] = 1
+            solids = SolidMask(g, device=device, mask=mask.ravel())
+            s = PressureSettings(pressure_tolerance=1e-5, mg_coarsest=8, pressure_precision=prec,
+                                 pressure_krylov="PCG")
+            eng = ToleranceProjector(g, s, device, solids=solids)._precond_engine()
+            ms = timed(lambda: eng.smooth(0, 1))
+            smoother.append({"n": n, "storage": prec, "us_per_sweep": 1e3 * ms,
+                             "working_set_mb": (3 * (4 if prec == "FP32" else 2) + 1) * n ** 3 / 1e6,
+                             "gbs": per_cell * n ** 3 / (ms * 1e-3) / 1e9})
+            print(f"# smoother {prec} {n}^3: {1e3 * ms:.1f} us per sweep, {smoother[-1]['gbs']:.0f} GB/s",
+                  file=sys.stderr, flush=True)
+            del eng, solids
+            torch.cuda.empty_cache()
+    copies = []
+    for mib in copy_mib:
+        count = mib * 2 ** 20 // 4
+        a = torch.ones(count, device=device.device)
+        b = torch.empty_like(a)
+        ms = timed(lambda: _lib.call("fx_scale", a.data_ptr(), b.data_ptr(), count, 1.0, None))
+        # the HIP runtime's device-to-device copy of the same arrays, for reference
+        ms_rt = timed(lambda: b.copy_(a))
+        copies.append({"mib_each_way": mib, "fx_scale_us": 1e3 * ms, "fx_scale_gbs": 8 * count / (ms * 1e-3) / 1e9,
+                       "runtime_copy_gbs": 8 * count / (ms_rt * 1e-3) / 1e9})
+        print(f"# copy {mib} MiB in + out: fx_scale {copies[-1]['fx_scale_gbs']:.0f} GB/s, runtime copy "
+              f"{copies[-1]['runtime_copy_gbs']:.0f} GB/s", file=sys.stderr, flush=True)
+        del a, b
+        torch.cuda.empty_cache()
+    big = [r for r in smoother if r["working_set_mb"] > 512]
+    line = {"metric": "level-0 smoother GB/s with its working set beyond the last-level cache",
+            "value": min(r["gbs"] for r in big) if big else None, "unit": "GB/s", "n_gpus": 1, "steps": reps,
+            "warmup": 1, "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32/bf16",
+            "data": "synthetic", "config": {"workload": "smoother sweeps and copies across sizes"},
+            "smoother": smoother, "copy": copies, "peak_gbs": HBM_PEAK_GBS}
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -432,10 +498,10 @@ def main():
     ap.add_argument("--n", type=int, default=None, help="grid side (default 256 smoke, 384 fire)")
     ap.add_argument("--modes", default="plain,mixed,learned")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--workload", choices=("smoke", "fire", "bricks"), default="smoke",
+    ap.add_argument("--workload", choices=("smoke", "fire", "bricks", "hbm"), default="smoke",
                     help="smoke: the 256^3 headline scene; fire: 384^3 dense fire (combustion, vorticity "
                          "confinement, a sphere collider moving 0.5 cells per step), no CPU leg; bricks: the "
-                         "256^3 sparse brick pool")
+                         "256^3 sparse brick pool; hbm: smoother and copy bandwidth across sizes")
     ap.add_argument("--weights", default=None, help="U-Net weights (.npz) of the learned mode")
     args = ap.parse_args()
     if args.weights:
@@ -451,6 +517,9 @@ def main():
     n = args.n or (384 if fire else 256)
     if args.workload == "bricks":
         bricks_bench(device, n, args.steps, args.warmup, not args.no_cpu)
+        return
+    if args.workload == "hbm":
+        hbm_bench(device)
         return
     results = {}
     roof = None
