@@ -285,7 +285,7 @@ int fx_mg_direct(const fx_mg_config* cfg, void* stream);
 
 /* Coarsest-level solve by FP64 conjugate gradients in one workgroup (coarsest
  * level of at most 1024 cells), what the V-cycle runs when coarse_sweeps < 0:
- * A p = coarse_scale * rhs from p = 0 to ||r|| <= 1e-10 ||b|| (the
+ * A p = coarse_scale * rhs from p = 0 to ||r|| <= 1e-7 ||b|| (the
  * pseudo-inverse's solution for a consistent rhs), for operators rebuilt every
  * step, where no pseudo-inverse is kept.  info (nullable, device, 2 doubles):
  * iterations taken and the final ||r|| / ||b||.  No reference counterpart. */

