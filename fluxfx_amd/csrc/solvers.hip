@@ -117,12 +117,39 @@ __global__ void k_pcg_apply(LevelDev L, const float* __restrict__ p, float* __re
 }
 
 // alpha = ws[q_rz]/ws[q_pap]; x += alpha p; r -= alpha Ap; rT = -r*inv_s;
+// Scale s of the 16-bit preconditioner residual (rT = -r/s, z_A = -s e).  With
+// slot < 0 the host's values are used as they are (scale 1 for fp32/bf16);
+// otherwise (fp16) s lives in ws[slot] and the update kernel derives the next
+// iteration's scale, max(sqrt(ws[0]) * rms_div * 0.3, 1e-30), from the
+// ||r||^2 finalised after the previous update (the 0.3 anticipates this
+// update's reduction) and leaves it in ws[next]: FP16 then follows the same
+// host residual-check schedule as the other storage types.
+struct ScaleRef {
+  float zscale;  // -s (host value)
+  float inv_s;   // 1/s of the rT this kernel writes (host value)
+  int slot;      // ws slot of the current s, or -1
+  int next;      // ws slot for the next s (update kernel)
+  double rms_div;
+  __device__ __forceinline__ float z(const double* ws) const { return slot < 0 ? zscale : float(-ws[slot]); }
+  // 1/s_next for rT; block 0 stores s_next
+  __device__ __forceinline__ float inv_next(double* ws) const {
+    if (slot < 0) return inv_s;
+    const double s_next = fmax(sqrt(ws[0]) * rms_div * 0.3, 1e-30);
+    if (blockIdx.x == 0 && threadIdx.x == 0) ws[next] = s_next;
+    return float(1.0 / s_next);
+  }
+};
+
+__global__ void k_set_ws(double* ws, int q, double v) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) ws[q] = v;
+}
+
 // partials ||r||^2 -> q_rr, r.(zold*zscale) -> q_cross
 template <typename T>
 __global__ void k_pcg_update(int64_t n, double* ws, int q_rz, int q_pap, float* __restrict__ x,
                              const float* __restrict__ p, float* __restrict__ r, const float* __restrict__ Ap,
-                             const T* __restrict__ zold, float zscale, T* __restrict__ rT, float inv_s, int q_rr,
-                             int q_cross) {
+                             const T* __restrict__ zold, ScaleRef sc, T* __restrict__ rT, int q_rr, int q_cross) {
+  const float zscale = sc.z(ws), inv_s = sc.inv_next(ws);
   double pap = ws[q_pap];
   float alpha = pap > 0.0 ? float(ws[q_rz] / pap) : 0.0f;
   double rr = 0.0, rzo = 0.0;
@@ -140,8 +167,9 @@ __global__ void k_pcg_update(int64_t n, double* ws, int q_rz, int q_pap, float* 
 
 // partial r.(z*zscale) -> q
 template <typename T>
-__global__ void k_dot_rz(int64_t n, const float* __restrict__ r, const T* __restrict__ z, float zscale, double* ws,
+__global__ void k_dot_rz(int64_t n, const float* __restrict__ r, const T* __restrict__ z, ScaleRef sc, double* ws,
                          int q) {
+  const float zscale = sc.z(ws);
   double acc = 0.0;
   FX_FOR_CELLS_N(n) acc += double(r[i]) * double(to_f(z[i]) * zscale);
   block_partial(acc, ws, q);
@@ -150,7 +178,8 @@ __global__ void k_dot_rz(int64_t n, const float* __restrict__ r, const T* __rest
 // beta = max(0, (rz_new - rz_cross) / rz_old); p = z*zscale + beta p
 template <typename T>
 __global__ void k_pcg_direction(int64_t n, const double* __restrict__ ws, int q_new, int q_cross, int q_old,
-                                const T* __restrict__ z, float zscale, float* __restrict__ p, int first) {
+                                const T* __restrict__ z, ScaleRef sc, float* __restrict__ p, int first) {
+  const float zscale = sc.z(ws);
   float beta = 0.0f;
   if (!first) {
     double den = ws[q_old];
@@ -232,8 +261,8 @@ __global__ void k_pcg_apply_q(LevelDev L, const float* __restrict__ p, float* __
 template <typename T>
 __global__ void k_pcg_update_q(int nq, double* ws, int q_rz, int q_pap, float* __restrict__ x,
                                const float* __restrict__ p, float* __restrict__ r, const float* __restrict__ Ap,
-                               const T* __restrict__ zold, float zscale, T* __restrict__ rT, float inv_s, int q_rr,
-                               int q_cross) {
+                               const T* __restrict__ zold, ScaleRef sc, T* __restrict__ rT, int q_rr, int q_cross) {
+  const float zscale = sc.z(ws), inv_s = sc.inv_next(ws);
   double pap = ws[q_pap];
   float alpha = pap > 0.0 ? float(ws[q_rz] / pap) : 0.0f;
   double rr = 0.0, rzo = 0.0;
@@ -257,8 +286,9 @@ __global__ void k_pcg_update_q(int nq, double* ws, int q_rz, int q_pap, float* _
 }
 
 template <typename T>
-__global__ void k_dot_rz_q(int nq, const float* __restrict__ r, const T* __restrict__ z, float zscale, double* ws,
+__global__ void k_dot_rz_q(int nq, const float* __restrict__ r, const T* __restrict__ z, ScaleRef sc, double* ws,
                            int q) {
+  const float zscale = sc.z(ws);
   double acc = 0.0;
   FX_FOR_QUADS(nq) {
     const int64_t i = int64_t(q_) * 4;
@@ -271,7 +301,8 @@ __global__ void k_dot_rz_q(int nq, const float* __restrict__ r, const T* __restr
 
 template <typename T>
 __global__ void k_pcg_direction_q(int nq, const double* __restrict__ ws, int q_new, int q_cross, int q_old,
-                                  const T* __restrict__ z, float zscale, float* __restrict__ p, int first) {
+                                  const T* __restrict__ z, ScaleRef sc, float* __restrict__ p, int first) {
+  const float zscale = sc.z(ws);
   float beta = 0.0f;
   if (!first) {
     double den = ws[q_old];
@@ -367,12 +398,18 @@ int pcg_loop(const fx_mg_level& L0, int storage, void* rT, void** z_slot, const 
   for (int restart = 0; restart < 4 && !done && it < prm->max_iters; ++restart) {
     measured = true;
     int q_rz = 4, q_next = 5;
-    // 16-bit residual scale: unit RMS (FP16 range); fp32/bf16 keep scale 1
+    // 16-bit residual scale: unit RMS (FP16 range); fp32/bf16 keep scale 1.  In
+    // FP16 the scale of later iterations is kept on the device in the
+    // alternating slots 12/13 (ScaleRef), so no iteration waits for a read-back.
     double scale = 1.0;
+    int sc_cur = -1, sc_next = -1;
     if (fp16) {
       if (int e = resnorm_q(L0, v.x, b, ws, 0, 15, s)) return e;
       if (int e = read_residual(ws, ref_q, &rr, &ref2, s)) return e;
       scale = fmax(sqrt(rr) * rms_div, 1e-30);
+      sc_cur = 12;
+      sc_next = 13;
+      hipLaunchKernelGGL(k_set_ws, dim3(1), dim3(64), 0, s, ws, sc_cur, scale);
     }
     if (quad) {
       FX_DISPATCH_T(storage, {
@@ -401,27 +438,30 @@ int pcg_loop(const fx_mg_level& L0, int storage, void* rT, void** z_slot, const 
     }
     if (int e = precond(s)) return e;
     float zscale = float(-scale);
+    // the current scale for the z consumers: the host value, or ws[sc_cur] (fp16)
+    auto cur_scale = [&]() { return ScaleRef{zscale, float(1.0 / scale), sc_cur, sc_next, rms_div}; };
     // direction p = z*zscale + beta*p from slots (q_new, q_cross, q_old)
     int dir_first = 1, dq_new = q_rz, dq_cross = q_rz, dq_old = q_rz;
     auto dot_rz = [&](int q) -> int {
+      const ScaleRef sc = cur_scale();
       FX_DISPATCH_T(storage, {
         if (quad)
-          hipLaunchKernelGGL((k_dot_rz_q<TV>), kRedGrid, kRedBlock, 0, s, nq, v.r, (const TV*)*z_slot, zscale, ws,
-                             q);
+          hipLaunchKernelGGL((k_dot_rz_q<TV>), kRedGrid, kRedBlock, 0, s, nq, v.r, (const TV*)*z_slot, sc, ws, q);
         else
-          hipLaunchKernelGGL((k_dot_rz<TV>), kRedGrid, kRedBlock, 0, s, n, v.r, (const TV*)*z_slot, zscale, ws, q);
+          hipLaunchKernelGGL((k_dot_rz<TV>), kRedGrid, kRedBlock, 0, s, n, v.r, (const TV*)*z_slot, sc, ws, q);
       });
       ws_finalize(ws, q, 1, s);
       return FX_OK;
     };
     auto direction = [&]() -> int {
+      const ScaleRef sc = cur_scale();
       FX_DISPATCH_T(storage, {
         if (quad)
           hipLaunchKernelGGL((k_pcg_direction_q<TV>), kRedGrid, kRedBlock, 0, s, nq, ws, dq_new, dq_cross, dq_old,
-                             (const TV*)*z_slot, zscale, v.p, dir_first);
+                             (const TV*)*z_slot, sc, v.p, dir_first);
         else
           hipLaunchKernelGGL((k_pcg_direction<TV>), kRedGrid, kRedBlock, 0, s, n, ws, dq_new, dq_cross, dq_old,
-                             (const TV*)*z_slot, zscale, v.p, dir_first);
+                             (const TV*)*z_slot, sc, v.p, dir_first);
       });
       return FX_OK;
     };
@@ -433,9 +473,9 @@ int pcg_loop(const fx_mg_level& L0, int storage, void* rT, void** z_slot, const 
       void* t = v.zold;
       v.zold = *z_slot;
       *z_slot = t;
-      float zscale_old = zscale;
-      double next_scale = scale;
-      if (fp16) next_scale = fmax(sqrt(rr) * rms_div * 0.3, 1e-30);
+      // fp32/bf16: scale 1 throughout; fp16: the update kernel derives the next
+      // scale from the last finalised ||r||^2 and stores it in ws[sc_next]
+      const ScaleRef upd{zscale, float(1.0 / scale), sc_cur, sc_next, rms_div};
       if (quad) {
         if (L0.op == FX_OP_FLAGS)
           hipLaunchKernelGGL((k_pcg_apply_q<FX_OP_FLAGS>), kRedGrid, kRedBlock, 0, s, D, v.p, v.Ap, ws, 3, nq);
@@ -450,23 +490,22 @@ int pcg_loop(const fx_mg_level& L0, int storage, void* rT, void** z_slot, const 
       FX_DISPATCH_T(storage, {
         if (quad)
           hipLaunchKernelGGL((k_pcg_update_q<TV>), kRedGrid, kRedBlock, 0, s, nq, ws, q_rz, 3, v.x, v.p, v.r, v.Ap,
-                             (const TV*)v.zold, zscale_old, (TV*)rT, float(1.0 / next_scale), 0, 2);
+                             (const TV*)v.zold, upd, (TV*)rT, 0, 2);
         else
           hipLaunchKernelGGL((k_pcg_update<TV>), kRedGrid, kRedBlock, 0, s, n, ws, q_rz, 3, v.x, v.p, v.r, v.Ap,
-                             (const TV*)v.zold, zscale_old, (TV*)rT, float(1.0 / next_scale), 0, 2);
+                             (const TV*)v.zold, upd, (TV*)rT, 0, 2);
       });
       FX_LAUNCHED("pcg update");
       ws_finalize_mask(ws, (1u << 0) | (1u << 2), s);
       ++it;
-      if (fp16 || it >= prm->check_from) {
+      if (it >= prm->check_from) {
         if (int e = read_residual(ws, ref_q, &rr, &ref2, s)) return e;
         FX_REQUIRE(std::isfinite(rr), FX_ESTATE, "non-finite residual after %d PCG iterations", it);
         if (rr <= tol2 * ref2) break;
       }
       if (it >= prm->max_iters) break;
-      scale = next_scale;
+      if (fp16) std::swap(sc_cur, sc_next);
       if (int e = precond(s)) return e;
-      zscale = float(-scale);
       if (int e = dot_rz(q_next)) return e;
       dir_first = 0;
       dq_new = q_next;

@@ -146,15 +146,16 @@ def evaluate_cold(device, n, seeds, weights=None, log=None):
     return out
 
 
-def evaluate_in_situ(device, n, seeds, warmup=4, steps=6, log=None):
-    """Per-seed mean iterations and ms per step of the full step in each mode."""
+def evaluate_in_situ(device, n, seeds, warmup=4, steps=6, weights=None, log=None):
+    """Per-seed mean iterations and ms per step of the full step in each mode
+    (weights: the level-1 network's .npz, default the committed one)."""
     from . import training
     from .backend.projected import DenseProjectedSmoke
     from .physics.config import GridSpec
     from .learned import UNetPreconditioner
     torch = device.torch
     net = UNetPreconditioner(device)
-    net.load(_weights(WEIGHTS_L1))
+    net.load(weights or _weights(WEIGHTS_L1))
     modes = {
         "mixed": _settings(pressure_precision="BF16", pressure_krylov="PCG"),
         "learned once (level 1)": _settings(pressure_precision="BF16", pressure_krylov="LEARNED",
@@ -233,6 +234,7 @@ def main(argv=None):
     ap.add_argument("--seeds", default="20000:20256")
     ap.add_argument("--in-situ", action="store_true")
     ap.add_argument("--out", default=None)
+    ap.add_argument("--weights", default=None, help="network .npz (default: the committed one for the mode)")
     args = ap.parse_args(argv)
     from .backend.device import HipDevice
     lo, hi = (int(v) for v in args.seeds.split(":"))
@@ -244,10 +246,10 @@ def main(argv=None):
 
     t0 = time.time()
     if args.in_situ:
-        res = evaluate_in_situ(device, args.n, seeds, log=log)
+        res = evaluate_in_situ(device, args.n, seeds, weights=args.weights, log=log)
         title = f"In-situ smoke steps at {args.n}^3, held-out seeds {lo}-{hi - 1}, 1e-5 relative residual"
     else:
-        res = evaluate_cold(device, args.n, seeds, log=log)
+        res = evaluate_cold(device, args.n, seeds, weights=args.weights, log=log)
         title = f"Cold solves at {args.n}^3, held-out seeds {lo}-{hi - 1}, to 1e-5 relative residual"
     text = table(res, title) + f"\n({time.time() - t0:.0f} s)\n"
     print(text, flush=True)
