@@ -176,6 +176,19 @@ def advance(sim, dt):
     sim.step(dt)
 
 
+def active_bricks(sim, side=8, threshold=1e-4):
+    """8^3 bricks of the fire grid that hold density or fuel above 1e-4 (the
+    sparse engine's activity rule, here a diagnostic of the dense step): one
+    support scan per field on the device, the flags OR-ed."""
+    from fluxfx_amd.native import BrickPool
+    nx, ny, nz = sim.grid.shape
+    flags = BrickPool.support(sim.density.tensor.view(nz, ny, nx), side, threshold)
+    if sim.combustion:
+        flags |= BrickPool.support(sim.combustion.fuel.tensor.view(nz, ny, nx), side, threshold)
+    active, total = int(flags.sum().item()), flags.numel()
+    return {"side": side, "threshold": threshold, "active": active, "total": total, "fraction": active / total}
+
+
 def timed_run(device, sim, dt, steps, warmup):
     """W untimed steps, then K steps each bracketed by HIP events on the stream
     the step's kernels run on; returns the per-step event times (ms), the wall
@@ -536,6 +549,8 @@ def main():
                          "wall_steps_per_s": args.steps / wall,
                          "mean_iterations": float(np.mean(cycles)), "relative_residual": m.get("relative_residual"),
                          "converged": m.get("converged"), "fallback": m.get("fallback")}
+        if fire:
+            results[mode]["active_bricks"] = active_bricks(sim)
         print(f"# {mode}: {1e3 / med:.2f} steps/s (median of {args.steps} event-timed steps {med:.3f} ms, wall "
               f"{1e3 * wall / args.steps:.3f} ms/step), iterations {np.mean(cycles):.2f}, residual "
               f"{m.get('relative_residual')}", file=sys.stderr, flush=True)

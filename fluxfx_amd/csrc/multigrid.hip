@@ -529,10 +529,12 @@ __global__ __launch_bounds__(kCgMaxCells) void k_coarse_cg(LevelDev L, const T* 
   if (i < m) {
     const int z = i / sz, y = (i - z * sz) / sy, x = i - z * sz - y * sy;
     if (op_has_edges(OP)) {
+      // the +x/+y/+z conductances of the level's last cells are wall faces
+      // (0); tested anyway, so no neighbour index leaves the level
       const float4 c = edge_at<OP>(L, i);
-      w[1] = c.x;
-      w[3] = c.y;
-      w[5] = c.z;
+      w[1] = x + 1 < L.nx ? c.x : 0.0f;
+      w[3] = y + 1 < L.ny ? c.y : 0.0f;
+      w[5] = z + 1 < L.nz ? c.z : 0.0f;
       w[0] = x > 0 ? edge_x<OP>(L, i - 1) : 0.0f;
       w[2] = y > 0 ? edge_y<OP>(L, i - sy) : 0.0f;
       w[4] = z > 0 ? edge_z<OP>(L, i - sz) : 0.0f;
