@@ -27,11 +27,18 @@ figures) and its ratio to plain are in the JSON line.  ``roofline`` is the
 level-0 smoother of the BF16 V-cycle (the kernel with the largest share of
 the learned+mixed step), timed with HIP events over 60 back-to-back sweeps.
 
+The scalar predictor, scalar transport and combustion run over the 8^3
+bricks that can hold nonzero scalars (exact zero tracking: the values of the
+whole-grid passes; ``modes.<mode>.scalar_bricks`` gives the shares of bricks
+the last step's passes computed); ``--dense-scalars`` runs them over the
+whole grid instead.
+
 ``--workload bricks`` times the sparse brick pool at 256^3 instead (support
 scan, gather and scatter of the reference's 112^3-box occupancy case).
 
 ``--workload fire`` runs a 384^3 dense fire instead (fuel and heat emitter,
-combustion, vorticity confinement, a static sphere collider; no CPU leg).
+combustion, vorticity confinement, a sphere collider moving 0.5 cells per step;
+no CPU leg).
 
 Prints ONE JSON line.  Single GPU only.
 """
@@ -56,6 +63,8 @@ LEARNED_ITERATIONS = 1
 # damped-Jacobi sweeps after the prolonged network correction: at 256^3 they
 # buy no iterations (0/2/4/6 sweeps: 7.30/7.35/7.40/7.40 per step), so none run
 LEARNED_SWEEPS = 0
+# scalar passes over the bricks that can hold nonzero scalars (same values)
+ACTIVE_BRICKS = True
 
 
 def scene(n, seed=3):
@@ -154,11 +163,12 @@ def make_sim(device, n, mode, workload="smoke"):
     settings, dt = settings_for(n, mode, workload)
     grid = GridSpec((n, n, n), (1.0, 1.0, 1.0), max_dim=1024)
     if workload == "smoke":
-        sim = DenseProjectedSmoke(grid, settings, device, solid_mask=box_mask(n, boxes).ravel())
+        sim = DenseProjectedSmoke(grid, settings, device, solid_mask=box_mask(n, boxes).ravel(),
+                                  active_bricks=ACTIVE_BRICKS)
     else:
         # the sphere moves 0.5 cells per step: advance() moves it before each step
         path = fire_path(n)
-        sim = DenseProjectedSmoke(grid, settings, device, colliders=(path(0),))
+        sim = DenseProjectedSmoke(grid, settings, device, colliders=(path(0),), active_bricks=ACTIVE_BRICKS)
         sim.collider_path, sim.collider_step = path, 0
     sim.upload(np.zeros(n ** 3, np.float32))
     sim.upload_temperature(np.zeros(n ** 3, np.float32))
@@ -516,10 +526,13 @@ def main():
                          "confinement, a sphere collider moving 0.5 cells per step), no CPU leg; bricks: the "
                          "256^3 sparse brick pool; hbm: smoother and copy bandwidth across sizes")
     ap.add_argument("--weights", default=None, help="U-Net weights (.npz) of the learned mode")
+    ap.add_argument("--dense-scalars", action="store_true",
+                    help="scalar predictor, transport and combustion over the whole grid")
     args = ap.parse_args()
+    global WEIGHTS, ACTIVE_BRICKS
     if args.weights:
-        global WEIGHTS
         WEIGHTS = Path(args.weights).resolve()
+    ACTIVE_BRICKS = not args.dense_scalars
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: the step runs on a single MI355X; "
               "multi-GPU runs are not part of this build", file=sys.stderr)
@@ -549,6 +562,7 @@ def main():
                          "wall_steps_per_s": args.steps / wall,
                          "mean_iterations": float(np.mean(cycles)), "relative_residual": m.get("relative_residual"),
                          "converged": m.get("converged"), "fallback": m.get("fallback")}
+        results[mode]["scalar_bricks"] = sim.active.fractions() if sim.active else None
         if fire:
             results[mode]["active_bricks"] = active_bricks(sim)
         print(f"# {mode}: {1e3 / med:.2f} steps/s (median of {args.steps} event-timed steps {med:.3f} ms, wall "
@@ -579,6 +593,7 @@ def main():
         "warmup": args.warmup, "ms_per_step": 1e3 / value if value else None, "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
         "config": {"workload": workload, "grid": [n, n, n], "dt": 0.5 / n, "mode": best_mode,
+                   "scalar_passes": "active 8^3 bricks (exact)" if ACTIVE_BRICKS else "whole grid",
                    "parallelism": "single GPU"},
         "modes": results,
         "speedup_vs_plain": (value / plain) if (plain and value) else None,

@@ -233,6 +233,13 @@ SIGNATURES = {
     "fx_scalar_advect": ([C.POINTER(fx_scalar_args), P], C.c_int),
     "fx_combustion": ([I64, F32, F32, F32, F32, F32, P, P, P, P, P, P, P, P], C.c_int),
     "fx_max_abs": ([I32, C.POINTER(P), C.POINTER(I64), P, P], C.c_int),
+    "fx_active_scan": ([fx_grid, P, P, P], C.c_int),
+    "fx_active_plan": ([fx_grid, F32, P, I32, I32, C.POINTER(P), I32, C.POINTER(P), C.POINTER(fx_sources), P, P, P],
+                       C.c_int),
+    "fx_scalar_predict_bounds_active": ([fx_grid, F32, P, P, P, I32, C.POINTER(P), C.POINTER(P), C.POINTER(P),
+                                         C.POINTER(P), P, P, P, P], C.c_int),
+    "fx_scalar_advect_active": ([C.POINTER(fx_scalar_args), P, C.POINTER(P), P], C.c_int),
+    "fx_combustion_active": ([fx_grid, F32, F32, F32, F32, F32, P, P, P, P, P, P, P, P, C.POINTER(P), P], C.c_int),
 }
 
 _lib = None

@@ -63,6 +63,7 @@ class SolidMask:
     def clear_solid(self, source, target):
         """target = solid ? 0 : source (shaders/solid_clear.glsl)."""
         _lib.call("fx_solid_clear", self._g, self.solid.data_ptr(), source.ptr, target.ptr, self.device.stream)
+        target.live = None
 
     def close(self):
         self.solid = self.flags = self.clearance = None

@@ -23,12 +23,21 @@ class CombustionPasses:
         for f in (self.fuel, self.spare, self.flame):
             f.clear()
 
-    def react(self, solver, dt):
+    def react(self, solver, dt, active=None):
+        """active: the solver's ActiveBricks (backend/active.py), or None for
+        the whole-grid pass."""
         s = solver.settings
-        _lib.call("fx_combustion", self.fuel.size, float(dt), float(s.ignition_temperature), float(s.burn_rate),
-                  float(s.heat_yield), float(s.smoke_yield), self.fuel.ptr, solver._temperature.ptr,
-                  solver._front.ptr, self.spare.ptr, solver._temperature_back.ptr, solver._back.ptr,
-                  self.flame.ptr, self.device.stream)
+        rates = (float(dt), float(s.ignition_temperature), float(s.burn_rate), float(s.heat_yield),
+                 float(s.smoke_yield))
+        ins = (self.fuel, solver._temperature, solver._front)
+        outs = (self.spare, solver._temperature_back, solver._back, self.flame)
+        if active is None:
+            _lib.call("fx_combustion", self.fuel.size, *rates, *(f.ptr for f in ins), *(f.ptr for f in outs),
+                      self.device.stream)
+        else:
+            active.pointwise_plan(ins, outs)
+            _lib.call("fx_combustion_active", solver._g, *rates, *(f.ptr for f in ins), *(f.ptr for f in outs),
+                      active.react.data_ptr(), active.out_masks(outs), self.device.stream)
         self.fuel, self.spare = self.spare, self.fuel
         solver._front, solver._back = solver._back, solver._front
         solver._temperature, solver._temperature_back = solver._temperature_back, solver._temperature

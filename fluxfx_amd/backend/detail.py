@@ -69,9 +69,10 @@ class DetailPasses:
         self._predict(fields, preds, velocities, dt)
         return preds
 
-    def scalar_predictors_bounds(self, fields, velocities, dt):
+    def scalar_predictors_bounds(self, fields, velocities, dt, bricks=None):
         """Fused predictor plus the donor bounds of every channel's departure
-        sample (fx_scalar_predict_bounds), for the fused correct pass."""
+        sample (fx_scalar_predict_bounds), for the fused correct pass; with a
+        brick plan (uint8 tensor, backend/active.py) only its bricks."""
         while len(self._scalar_pred) < len(fields):
             self._scalar_pred.append(self.device.texture(self.grid.shape))
         while len(self._scalar_bounds) < len(fields):
@@ -81,9 +82,12 @@ class DetailPasses:
         preds = self._scalar_pred[:n]
         bounds = self._scalar_bounds[:n]
         arr = lambda fs: (C.c_void_p * n)(*[ptr(f) for f in fs])
-        _lib.call("fx_scalar_predict_bounds", self._g, float(dt), *(ptr(v) for v in velocities), n, arr(fields),
-                  arr(preds), arr([b[0] for b in bounds]), arr([b[1] for b in bounds]), ptr(self._solid),
-                  ptr(self._clearance), self.device.stream)
+        common = (self._g, float(dt), *(ptr(v) for v in velocities), n, arr(fields), arr(preds),
+                  arr([b[0] for b in bounds]), arr([b[1] for b in bounds]), ptr(self._solid), ptr(self._clearance))
+        if bricks is None:
+            _lib.call("fx_scalar_predict_bounds", *common, self.device.stream)
+        else:
+            _lib.call("fx_scalar_predict_bounds_active", *common, bricks.data_ptr(), self.device.stream)
         return preds, bounds
 
     def transport_velocity(self, fields, dt):

@@ -18,15 +18,22 @@ __all__ = ["HipDevice", "Field", "GPUUnavailable"]
 
 
 class Field:
-    """One device array in the reference's flat x-fast layout."""
+    """One device array in the reference's flat x-fast layout.
 
-    __slots__ = ("tensor", "shape", "channels", "device")
+    ``live``: the field's exact active-brick mask (a uint8 tensor, one byte
+    per 8^3 brick, 0 = every cell of the brick is exactly 0) kept by the
+    active-brick scalar passes, or None when unknown.  A new field starts
+    unknown and clear() keeps it exact; code that writes the tensor in place
+    by other means sets it to None."""
+
+    __slots__ = ("tensor", "shape", "channels", "device", "live")
 
     def __init__(self, tensor, shape, channels, device):
         self.tensor = tensor
         self.shape = tuple(shape)
         self.channels = channels
         self.device = device
+        self.live = None
 
     @property
     def ptr(self):
@@ -41,6 +48,10 @@ class Field:
             _lib.call("fx_fill", self.ptr, self.size, float(value[0]), self.device.stream)
         else:
             self.tensor.fill_(value[0])
+        if self.live is not None and all(v == 0 for v in value):
+            self.live.zero_()
+        else:
+            self.live = None
 
     def read(self):
         return self.device.read(self, self.shape)

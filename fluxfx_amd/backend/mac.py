@@ -52,6 +52,7 @@ class DenseAdvection:
         s = self.settings
         _lib.call("fx_seed", self._g, 0, seed_params(s.source_center, s.source_radius, scale), target.ptr,
                   self.device.stream)
+        target.live = None
 
     def reset(self, seed=True):
         self._front.clear()

@@ -23,6 +23,7 @@ from ..physics.emission import Emission, pack_sources, MAX_SOURCES
 from ..physics.interaction import validate_live_update
 from ..physics.pressure import PressureSettings
 from .collision import SolidMask
+from .active import ActiveBricks
 from .combustion import CombustionPasses
 from .detail import DetailPasses
 from .device import HipDevice, ptr
@@ -44,8 +45,15 @@ def select_projector(grid, settings, solids):
 
 
 class DenseProjectedSmoke(DenseThermalAdvection):
+    """active_bricks=True runs the scalar predictor, scalar transport and
+    combustion over the 8^3 bricks that can hold nonzero scalars (exact
+    zero tracking, backend/active.py): the same values as the whole-grid
+    passes.  Code that writes a scalar field's tensor in place outside the
+    step calls invalidate_active_bricks()."""
+
     def __init__(self, grid=None, settings=None, device=None, projector_class=None, *, colliders=(),
-                 solid_mask=None):
+                 solid_mask=None, active_bricks=False):
+        self.active = None
         self.combustion = None
         self.solids = None
         self.projector = None
@@ -83,13 +91,24 @@ class DenseProjectedSmoke(DenseThermalAdvection):
         self.detail = DetailPasses(self.grid, self.device, self.solids)
         if self.settings.combustion_enabled:
             self.combustion = CombustionPasses(self.grid, self.device)
+        if active_bricks:
+            self.active = ActiveBricks(self.grid, self.device, self._g)
         self.reset()
 
     @property
     def allocated_bytes(self):
         return (super().allocated_bytes + self.projector.allocated_bytes + self.detail.allocated_bytes +
                 (self.combustion.allocated_bytes if self.combustion else 0) +
-                (self.solids.allocated_bytes if self.solids else 0))
+                (self.solids.allocated_bytes if self.solids else 0) +
+                (self.active.allocated_bytes if self.active else 0))
+
+    def invalidate_active_bricks(self):
+        """Forget the scalar fields' live masks (rescanned before the next step)."""
+        for f in (self._front, self._back, self._temperature, self._temperature_back):
+            f.live = None
+        if self.combustion:
+            for f in (self.combustion.fuel, self.combustion.spare, self.combustion.flame):
+                f.live = None
 
     def reset(self, seed=True):
         super().reset(seed)
@@ -230,9 +249,13 @@ class DenseProjectedSmoke(DenseThermalAdvection):
             outs.append(self.combustion.spare)
             decays.append(0.0)
         corrected = s.scalar_advection == "MACCORMACK"
+        active = self.active
+        if active is not None:
+            active.transport_plan(dt, self._velocity, fields, outs, C.pointer(table))
         preds = bounds = None
         if corrected:
-            preds, bounds = self.detail.scalar_predictors_bounds(fields, self._velocity, dt)
+            preds, bounds = self.detail.scalar_predictors_bounds(fields, self._velocity, dt,
+                                                                 bricks=active.predict if active else None)
         args = _lib.fx_scalar_args()
         args.g = self._g
         args.dt = dt
@@ -251,7 +274,11 @@ class DenseProjectedSmoke(DenseThermalAdvection):
         args.solid = ptr(solid)
         args.clearance = ptr(clearance)
         args.sources = C.pointer(table)
-        _lib.call("fx_scalar_advect", C.byref(args), self.device.stream)
+        if active is not None:
+            _lib.call("fx_scalar_advect_active", C.byref(args), active.write.data_ptr(), active.out_masks(outs),
+                      self.device.stream)
+        else:
+            _lib.call("fx_scalar_advect", C.byref(args), self.device.stream)
         self._front, self._back = self._back, self._front
         self._temperature, self._temperature_back = self._temperature_back, self._temperature
         if self.combustion:
@@ -260,7 +287,7 @@ class DenseProjectedSmoke(DenseThermalAdvection):
             self.observer("transported", [self._front, self._temperature] +
                           ([self.combustion.fuel] if self.combustion else []))
         if self.combustion:
-            self.combustion.react(self, dt)
+            self.combustion.react(self, dt, active)
         self.time += dt
         self.steps += 1
 
@@ -276,6 +303,9 @@ class DenseProjectedSmoke(DenseThermalAdvection):
         return self.temperature if channel == "TEMPERATURE" else self.density
 
     def close(self):
+        if self.active is not None:
+            self.active.close()
+            self.active = None
         if self.combustion:
             self.combustion.close()
         if self.projector is not None:

@@ -970,6 +970,7 @@ struct ScalarSet {
   float* o[3];
   float* lo[3];  // donor bounds (BOUNDS variant)
   float* hi[3];
+  BrickMask bricks;  // computed bricks (the predictor has no zero-store state)
 };
 
 // BOUNDS: also store, per channel, the min/max of the eight donors of the
@@ -981,6 +982,7 @@ __global__ void k_scalar_predict(GridDev g, Vel V, float dt, int channels, Scala
   uint32_t ux, uy, uz, ue;
   if (!thread_site(g.nx, g.ny, ux, uy, uz, ue)) return;
   const int x = int(ux), y = int(uy), z = int(uz);
+  if (S_.bricks.state(x, y, z) != kBrickCompute) return;
   const uint32_t i = ue << 2;  // byte offset of this cell
   const float hx = g.hx, hy = g.hy, hz = g.hz;
   V3 p = v3((x + 0.5f) * hx, (y + 0.5f) * hy, (z + 0.5f) * hz);
@@ -1075,7 +1077,14 @@ struct ScalarArgs {
   const uint8_t* clear;
   const float* lo[3];  // donor bounds from the predictor pass (BOUNDS variant)
   const float* hi[3];
+  BrickMask bricks;    // plan: skip, compute or store 0 per brick
+  uint8_t* live[3];    // nullable: byte of the site's brick set to 1 when the channel's output is not 0
 };
+
+// the channel's output is nonzero (or NaN): mark its brick live
+__device__ __forceinline__ void mark_live(uint8_t* live, const BrickMask& m, int x, int y, int z, float v) {
+  if (live && !(v == 0.0f)) live[m.at(x, y, z)] = 1;
+}
 
 // BOUNDS (MacCormack only): the predictor pass already holds the
 // semi-Lagrangian value and the donor bounds of every channel, so only the
@@ -1086,7 +1095,13 @@ __global__ void k_scalar_advect(ScalarArgs A, fx_sources src) {
   uint32_t ux, uy, uz, ue;
   if (!thread_site(g.nx, g.ny, ux, uy, uz, ue)) return;
   const int x = int(ux), y = int(uy), z = int(uz);
+  const uint8_t plan = A.bricks.state(x, y, z);
+  if (plan == kBrickSkip) return;
   const uint32_t i = ue << 2;  // byte offset of this cell
+  if (plan == kBrickZero) {
+    for (int k = 0; k < A.channels; ++k) stf(A.out[k], i, 0.0f);
+    return;
+  }
   const float dt = A.dt;
   const float hx = g.hx, hy = g.hy, hz = g.hz;
   V3 p = v3((x + 0.5f) * hx, (y + 0.5f) * hy, (z + 0.5f) * hz);
@@ -1154,6 +1169,7 @@ __global__ void k_scalar_advect(ScalarArgs A, fx_sources src) {
       }
       if (ch == 0 && (src.multi || any_target)) value = fmaxf(value, target);
       stf(A.out[k], i, value);
+      mark_live(A.live[k], A.bricks, x, y, z, value);
     }
     return;
   }
@@ -1212,6 +1228,7 @@ __global__ void k_scalar_advect(ScalarArgs A, fx_sources src) {
     }
     if (ch == 0 && (src.multi || any_target)) value = fmaxf(value, target);
     stf(A.out[k], i, value);
+    mark_live(A.live[k], A.bricks, x, y, z, value);
   }
 }
 
@@ -1229,6 +1246,36 @@ __global__ void k_combustion(int64_t n, float dt, float ign, float k, float heat
   To[i] = t + dt * heat * fl;
   ro[i] = rho[i] + dt * smoke * fl;
   flame[i] = fl;
+}
+
+// k_combustion over the bricks of a plan (skipped bricks untouched, zero
+// bricks stored 0); live[0..3]: fuel, temperature, density and flame outputs
+__global__ void k_combustion_active(int nx, int ny, int n, float dt, float ign, float k, float heat, float smoke,
+                                    const float* __restrict__ fuel, const float* __restrict__ T,
+                                    const float* __restrict__ rho, float* __restrict__ fo, float* __restrict__ To,
+                                    float* __restrict__ ro, float* __restrict__ flame, BrickMask m,
+                                    uint8_t* lf, uint8_t* lt, uint8_t* lr, uint8_t* ll) {
+  const int i = int(blockIdx.x) * int(blockDim.x) + int(threadIdx.x);
+  if (i >= n) return;
+  const int row = i / nx, x = i - row * nx, y = row % ny, z = row / ny;
+  const uint8_t plan = m.state(x, y, z);
+  if (plan == kBrickSkip) return;
+  if (plan == kBrickZero) {
+    fo[i] = To[i] = ro[i] = flame[i] = 0.0f;
+    return;
+  }
+  float f = fuel[i], t = T[i];
+  float burned = t >= ign ? fmaxf(f, 0.0f) * -expm1f(-k * dt) : 0.0f;
+  float fl = burned / dt;
+  const float a = fmaxf(f - dt * fl, 0.0f), b = t + dt * heat * fl, c = rho[i] + dt * smoke * fl;
+  fo[i] = a;
+  To[i] = b;
+  ro[i] = c;
+  flame[i] = fl;
+  mark_live(lf, m, x, y, z, a);
+  mark_live(lt, m, x, y, z, b);
+  mark_live(lr, m, x, y, z, c);
+  mark_live(ll, m, x, y, z, fl);
 }
 
 struct MaxArgs {
@@ -1450,9 +1497,14 @@ int fx_scalar_predict(fx_grid g, float dt, const float* u, const float* v, const
   return FX_OK;
 }
 
-int fx_scalar_predict_bounds(fx_grid g, float dt, const float* u, const float* v, const float* w, int32_t channels,
-                             const float* const* fields, float* const* preds, float* const* lo, float* const* hi,
-                             const uint8_t* solid, const uint8_t* clearance, void* stream) {
+}  // extern "C"
+
+namespace fx {
+
+static int scalar_predict_bounds(fx_grid g, float dt, const float* u, const float* v, const float* w,
+                                 int32_t channels, const float* const* fields, float* const* preds,
+                                 float* const* lo, float* const* hi, const uint8_t* solid, const uint8_t* clearance,
+                                 const uint8_t* bricks, void* stream) {
   if (int e = check_transport_grid(g)) return e;
   FX_REQUIRE(channels >= 1 && channels <= 3, FX_EINVAL, "channels must be 1..3");
   FX_REQUIRE(u && v && w && fields && preds && lo && hi, FX_EINVAL, "null array");
@@ -1468,6 +1520,7 @@ int fx_scalar_predict_bounds(fx_grid g, float dt, const float* u, const float* v
     set.lo[k] = lo[k];
     set.hi[k] = hi[k];
   }
+  set.bricks = BrickMask{bricks, active_bricks_along(g.nx), active_bricks_along(g.ny)};
   dim3 gr = site_grid(g.nx, g.ny, g.nz);
   const dim3 kBlock = kSiteBlock;
   Vel V{u, v, w};
@@ -1481,7 +1534,7 @@ int fx_scalar_predict_bounds(fx_grid g, float dt, const float* u, const float* v
   return FX_OK;
 }
 
-int fx_scalar_advect(const fx_scalar_args* a, void* stream) {
+static int scalar_advect(const fx_scalar_args* a, const uint8_t* bricks, uint8_t* const* live, void* stream) {
   FX_REQUIRE(a, FX_EINVAL, "null arguments");
   if (int e = check_transport_grid(a->g)) return e;
   if (int e = check_sources(a->sources)) return e;
@@ -1510,7 +1563,9 @@ int fx_scalar_advect(const fx_scalar_args* a, void* stream) {
     A.out[k] = a->out[k];
     A.keep[k] = expf(-a->decay[k] * a->dt);
     A.chan[k] = a->channel_id[k];
+    A.live[k] = live ? live[k] : nullptr;
   }
+  A.bricks = BrickMask{bricks, active_bricks_along(a->g.nx), active_bricks_along(a->g.ny)};
   bool bounds = a->maccormack != 0 && a->boundary == 0;
   for (int k = 0; k < a->channels; ++k) bounds = bounds && a->lo[k] && a->hi[k];
   for (int k = 0; bounds && k < a->channels; ++k) {
@@ -1536,6 +1591,34 @@ int fx_scalar_advect(const fx_scalar_args* a, void* stream) {
   return FX_OK;
 }
 
+}  // namespace fx
+
+extern "C" {
+
+int fx_scalar_predict_bounds(fx_grid g, float dt, const float* u, const float* v, const float* w, int32_t channels,
+                             const float* const* fields, float* const* preds, float* const* lo, float* const* hi,
+                             const uint8_t* solid, const uint8_t* clearance, void* stream) {
+  return scalar_predict_bounds(g, dt, u, v, w, channels, fields, preds, lo, hi, solid, clearance, nullptr, stream);
+}
+
+int fx_scalar_predict_bounds_active(fx_grid g, float dt, const float* u, const float* v, const float* w,
+                                    int32_t channels, const float* const* fields, float* const* preds,
+                                    float* const* lo, float* const* hi, const uint8_t* solid,
+                                    const uint8_t* clearance, const uint8_t* bricks, void* stream) {
+  FX_REQUIRE(bricks, FX_EINVAL, "active predictor: null brick mask");
+  return scalar_predict_bounds(g, dt, u, v, w, channels, fields, preds, lo, hi, solid, clearance, bricks, stream);
+}
+
+int fx_scalar_advect(const fx_scalar_args* a, void* stream) { return scalar_advect(a, nullptr, nullptr, stream); }
+
+int fx_scalar_advect_active(const fx_scalar_args* a, const uint8_t* bricks, uint8_t* const* live, void* stream) {
+  FX_REQUIRE(bricks && live, FX_EINVAL, "active transport: null brick mask or live masks");
+  FX_REQUIRE(a, FX_EINVAL, "null arguments");
+  FX_REQUIRE(a->channels >= 1 && a->channels <= 3, FX_EINVAL, "channels must be 1..3");
+  for (int k = 0; k < a->channels; ++k) FX_REQUIRE(live[k], FX_EINVAL, "active transport: null live mask %d", k);
+  return scalar_advect(a, bricks, live, stream);
+}
+
 int fx_combustion(int64_t n, float dt, float ignition, float burn_rate, float heat_yield, float smoke_yield,
                   const float* fuel, const float* temperature, const float* density, float* fuel_out,
                   float* temperature_out, float* density_out, float* flame, void* stream) {
@@ -1549,6 +1632,27 @@ int fx_combustion(int64_t n, float dt, float ignition, float burn_rate, float he
                      heat_yield, smoke_yield, fuel, temperature, density, fuel_out, temperature_out, density_out,
                      flame);
   FX_LAUNCHED("fx_combustion");
+  return FX_OK;
+}
+
+int fx_combustion_active(fx_grid g, float dt, float ignition, float burn_rate, float heat_yield, float smoke_yield,
+                         const float* fuel, const float* temperature, const float* density, float* fuel_out,
+                         float* temperature_out, float* density_out, float* flame, const uint8_t* bricks,
+                         uint8_t* const* live, void* stream) {
+  if (int e = check_transport_grid(g)) return e;
+  FX_REQUIRE(fuel && temperature && density && fuel_out && temperature_out && density_out && flame, FX_EINVAL,
+             "combustion: null array");
+  FX_REQUIRE(bricks && live && live[0] && live[1] && live[2] && live[3], FX_EINVAL,
+             "active combustion: null brick mask or live masks");
+  FX_REQUIRE(dt > 0.0f, FX_EINVAL, "dt must be positive");
+  FX_REQUIRE(fuel_out != fuel && temperature_out != temperature && density_out != density, FX_EINVAL,
+             "Advection input/output textures must be distinct");
+  const int n = g.nx * g.ny * g.nz;
+  const BrickMask m{bricks, active_bricks_along(g.nx), active_bricks_along(g.ny)};
+  hipLaunchKernelGGL(k_combustion_active, dim3((n + 255) / 256), dim3(256), 0, S(stream), g.nx, g.ny, n, dt,
+                     ignition, burn_rate, heat_yield, smoke_yield, fuel, temperature, density, fuel_out,
+                     temperature_out, density_out, flame, m, live[0], live[1], live[2], live[3]);
+  FX_LAUNCHED("fx_combustion_active");
   return FX_OK;
 }
 

@@ -181,6 +181,23 @@ __device__ __forceinline__ void block_partial(double v, double* ws, int q) {
 // clearances above this many cells are stored as 255 (far)
 constexpr int kClearanceRadius = 15;
 
+// Active-brick masks of the scalar passes: one byte per 8^3 brick of the cell
+// grid, x fastest (bricks at the high ends may be partial).  A plan byte says
+// what a pass does with the brick's cells: kBrickSkip leaves them, kBrickCompute
+// computes them, kBrickZero stores 0 (outputs known to be exactly zero there).
+constexpr int kActiveShift = 3;
+constexpr uint8_t kBrickSkip = 0, kBrickCompute = 1, kBrickZero = 2;
+struct BrickMask {
+  const uint8_t* on;  // null: every brick computed
+  int bx, by;         // bricks along x and y
+  __device__ __forceinline__ uint32_t at(int x, int y, int z) const {
+    return uint32_t(x >> kActiveShift) +
+           uint32_t(bx) * (uint32_t(y >> kActiveShift) + uint32_t(by) * uint32_t(z >> kActiveShift));
+  }
+  __device__ __forceinline__ uint8_t state(int x, int y, int z) const { return on ? on[at(x, y, z)] : kBrickCompute; }
+};
+inline int active_bricks_along(int n) { return (n + (1 << kActiveShift) - 1) >> kActiveShift; }
+
 // box-restricted mask passes of the moving colliders (advect.hip, pressure.hip);
 // b = {x0, y0, z0, x1, y1, z1}, half-open cell ranges inside the grid
 int launch_clearance_box(const fx_grid& g, const int* b, const uint8_t* solid, uint8_t* clearance, uint8_t* scratch,

@@ -479,6 +479,51 @@ int fx_combustion(int64_t n, float dt, float ignition, float burn_rate, float he
                   const float* fuel, const float* temperature, const float* density, float* fuel_out,
                   float* temperature_out, float* density_out, float* flame, void* stream);
 
+/* Exact active bricks of the scalar passes.  No reference counterpart: the
+ * reference's sparse engine drops density below 1e-4 (native/projection.mm);
+ * these masks track exact zeros instead, so the passes below give the same
+ * values as their whole-grid forms.  A mask holds one byte per 8^3 brick of
+ * the cell grid (ceil(n/8) bricks per axis, x fastest).  A field's live mask
+ * byte 0 guarantees every cell of that brick of the field is exactly 0.
+ *
+ * fx_active_scan: the live mask of a field (1 where any cell is not exactly
+ * 0, NaN included). */
+int fx_active_scan(fx_grid g, const float* field, uint8_t* live, void* stream);
+
+/* Plan of the next scalar pass from the live masks of its inputs (nin) and
+ * of the buffers it writes (nout, each 1..4).  Plan bytes: 0 skip the brick,
+ * 1 compute it, 2 store 0 (the outputs are exactly zero there but a buffer may
+ * hold other values).  transport = 1 (fx_scalar_advect_active): computed are
+ * the bricks within one brick of a live input or of an emitter's reach
+ * (sources: the step's source table), and predict_bricks receives the bricks
+ * within one brick of those (fx_scalar_predict_bounds_active); vmax is the
+ * device max |U|, |V|, |W| of the transporting velocity (fx_max_abs), and a
+ * plan with dt * max|u| / h above 2.5 cells on any axis computes every brick.
+ * transport = 0 (fx_combustion_active): computed are the live inputs'
+ * bricks.  The output masks are cleared on every brick the pass writes; the
+ * pass sets them again where it stores nonzero values. */
+int fx_active_plan(fx_grid g, float dt, const float* vmax, int32_t transport, int32_t nin,
+                   const uint8_t* const* live_in, int32_t nout, uint8_t* const* live_out, const fx_sources* sources,
+                   uint8_t* predict_bricks, uint8_t* write_bricks, void* stream);
+
+/* fx_scalar_predict_bounds on the bricks predict_bricks computes (other
+ * cells of the outputs untouched). */
+int fx_scalar_predict_bounds_active(fx_grid g, float dt, const float* u, const float* v, const float* w,
+                                    int32_t channels, const float* const* fields, float* const* preds,
+                                    float* const* lo, float* const* hi, const uint8_t* solid,
+                                    const uint8_t* clearance, const uint8_t* bricks, void* stream);
+
+/* fx_scalar_advect under a transport plan; live[k] (the output's live mask)
+ * gets 1 for every brick where channel k stores a nonzero value. */
+int fx_scalar_advect_active(const fx_scalar_args* a, const uint8_t* bricks, uint8_t* const* live, void* stream);
+
+/* fx_combustion under a pointwise plan; live[0..3]: the live masks of the
+ * fuel, temperature, density and flame outputs. */
+int fx_combustion_active(fx_grid g, float dt, float ignition, float burn_rate, float heat_yield, float smoke_yield,
+                         const float* fuel, const float* temperature, const float* density, float* fuel_out,
+                         float* temperature_out, float* density_out, float* flame, const uint8_t* bricks,
+                         uint8_t* const* live, void* stream);
+
 /* max |x| over `count` arrays into out[i] (device floats), NaN/Inf -> 3.4e38
  * (AdaptiveTimestep.bounds, backend/timestep.py:24-33, shaders/max_reduce.glsl). */
 int fx_max_abs(int32_t count, const float* const* arrays, const int64_t* sizes, float* out, void* stream);

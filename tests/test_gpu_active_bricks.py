@@ -1,0 +1,129 @@
+"""Exact active bricks of the scalar passes (no reference counterpart).
+
+A step with active_bricks=True runs the scalar predictor, transport and
+combustion only on the 8^3 bricks a plan selects from the fields' exact-zero
+masks.  Its fields must equal those of the same step over the whole grid, value
+for value, after every step: plumes with box solids and tolerance multigrid,
+the fire workload with a moving sphere, semi-Lagrangian scalars from several
+emitters on a grid that is not a multiple of 8, a velocity too fast for a
+plan (every brick computed), and a field replaced between steps.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _fields(sim):
+    out = {"density": sim.density.tensor, "temperature": sim.temperature.tensor,
+           **{f"velocity{a}": f.tensor for a, f in enumerate(sim._velocity)},
+           "pressure": sim.projector.pressure.tensor}
+    if sim.combustion:
+        out.update(fuel=sim.combustion.fuel.tensor, flame=sim.combustion.flame.tensor)
+    return out
+
+
+def _assert_same(a, b, label):
+    fa, fb = _fields(a), _fields(b)
+    for k in fa:
+        assert bool((fa[k] == fb[k]).all()), f"{label}: {k} differs in " \
+            f"{int((fa[k] != fb[k]).sum())} cells"
+
+
+def _pair(device, grid, settings, **kw):
+    from fluxfx_amd.backend.projected import DenseProjectedSmoke
+    return (DenseProjectedSmoke(grid, settings, device, **kw),
+            DenseProjectedSmoke(grid, settings, device, active_bricks=True, **kw))
+
+
+def _run(pair, steps, dt, before=None, sources=None, label=""):
+    fractions = []
+    for k in range(steps):
+        for sim in pair:
+            if before:
+                before(sim, k)
+            sim.step(dt, sources=sources)
+        _assert_same(*pair, f"{label} step {k + 1}")
+        fractions.append(pair[1].active.fractions())
+    return fractions
+
+
+def _smoke_settings(n, **kw):
+    from fluxfx_amd.physics.pressure import PressureSettings
+    dt = 0.5 / n
+    base = dict(source_center=(.5, .5, .16), source_radius=.09, source_rate=1 / dt, heat_source_rate=1 / dt,
+                dissipation=0.0, cooling=0.05, thermal_lift=1.0, density_weight=0.0, source_profile="SOLID",
+                initial_temperature=0.0, velocity_advection="MACCORMACK", scalar_advection="MACCORMACK",
+                pressure_tolerance=1e-5, mg_pre=2, mg_post=2, mg_coarsest=8)
+    base.update(kw)
+    return PressureSettings(**base), dt
+
+
+@pytest.mark.parametrize("precision", ["FP32", "BF16"])
+def test_plume_with_boxes_equals_whole_grid(device, precision):
+    import bench
+    from fluxfx_amd.physics.config import GridSpec
+    n = 64
+    print(f"active bricks: plume with boxes at {n}^3, {precision}", flush=True)
+    s, dt = _smoke_settings(n, pressure_precision=precision, pressure_krylov="PCG" if precision != "FP32" else "NONE")
+    _, _, boxes = bench.scene(n)
+    mask = bench.box_mask(n, boxes).ravel()
+    pair = _pair(device, GridSpec((n,) * 3), s, solid_mask=mask)
+    fr = _run(pair, 10, dt, label="plume")
+    # the plume covers a small part of the grid: most bricks are skipped
+    assert fr[-1]["transport"] < 0.5 and fr[-1]["predict"] < 0.7, fr[-1]
+
+
+def test_fire_with_moving_sphere_equals_whole_grid(device):
+    from fluxfx_amd.physics.config import GridSpec
+    from tests.test_gpu_moving_tolerance import _fire_scene
+    n = 48
+    print(f"active bricks: fire with a moving sphere at {n}^3", flush=True)
+    s, _, sphere, dt = _fire_scene(n, {"pressure_precision": "BF16", "pressure_krylov": "PCG"})
+    pair = _pair(device, GridSpec((n,) * 3), s, colliders=(sphere(0),))
+
+    def move(sim, k):
+        sim.move_colliders((sphere(k + 1),), dt)
+
+    fr = _run(pair, 10, dt, before=move, label="fire")
+    assert fr[-1]["combustion"] < 0.5 and fr[-1]["transport"] < 0.7, fr[-1]
+
+
+def test_semi_lagrangian_sources_on_a_ragged_grid(device):
+    """40 x 36 x 44 cells (partial bricks at the high ends), semi-Lagrangian
+    scalars, two moving emitters of the multi-source table, Jacobi projection."""
+    from fluxfx_amd.physics.config import GridSpec
+    from fluxfx_amd.physics.emission import Emission, Source
+    from fluxfx_amd.physics.pressure import PressureSettings
+    shape = (40, 36, 44)
+    print(f"active bricks: semi-Lagrangian emitters on {shape}", flush=True)
+    dt = 0.5 / 44
+    s = PressureSettings(dissipation=0.1, cooling=0.0, thermal_lift=1.0, density_weight=0.2, initial_temperature=0.0,
+                         scalar_advection="SEMI_LAGRANGIAN", velocity_advection="MACCORMACK",
+                         pressure_solver="JACOBI", pressure_iterations=40)
+    pair = _pair(device, GridSpec(shape), s)
+    srcs = [Source((.3, .4, .2), .08, 20.0, 10.0, Emission((.3, .4, .2), (0.0, 0.0, 0.5), 4.0), "RATE", "SOFT"),
+            Source((.7, .6, .5), .06, 10.0, 5.0, Emission((.65, .6, .45)), "TARGET", "SOLID")]
+    _run(pair, 8, dt, sources=srcs, label="ragged")
+
+
+def test_fast_velocity_plans_every_brick_and_replaced_fields(device):
+    """A velocity moving more than 2.5 cells per step: the plans compute every
+    brick (still equal); then a density uploaded between steps is rescanned."""
+    from fluxfx_amd.physics.config import GridSpec
+    n = 32
+    print(f"active bricks: fast velocity at {n}^3", flush=True)
+    s, dt = _smoke_settings(n, pressure_tolerance=0.0, pressure_solver="JACOBI", pressure_iterations=30)
+    pair = _pair(device, GridSpec((n,) * 3), s)
+    rng = np.random.default_rng(11)
+    vel = [rng.uniform(-20.0, 20.0, int(np.prod(sh))) for sh in pair[0].grid.face_shapes]
+    for sim in pair:
+        sim.upload_velocity(vel)
+    fr = _run(pair, 2, dt, label="fast")
+    assert fr[0]["transport"] == 1.0 and fr[0]["predict"] == 1.0, fr[0]
+    blob = np.zeros((n, n, n))
+    blob[20:26, 3:9, 10:14] = 1.0
+    for sim in pair:
+        sim.upload(blob.ravel())
+        sim.upload_velocity([np.zeros(int(np.prod(sh))) for sh in sim.grid.face_shapes])
+    _run(pair, 3, dt, label="replaced")
