@@ -71,21 +71,35 @@ struct PlanArgs {
   int box[kMaxEmitterBoxes][6];  // emitter bricks, inclusive [x0, x1] x [y0, y1] x [z0, z1]
 };
 
-// One thread per brick.  The thread also clears the live bytes of the output
+// A block owns an 8 x 8 x 8 tile of bricks, one thread per brick, and stages
+// the union of the input masks over the tile and two bricks around it in LDS
+// (0 outside the grid).  The thread also clears the live bytes of the output
 // buffers of every brick the pass writes (the pass sets them again where it
 // stores nonzero values), so each output's mask stays exact: a skipped brick
 // keeps its zeros, a written one is rewritten.
-__global__ void k_active_plan(PlanArgs A) {
-  const int nb = A.bx * A.by * A.bz;
-  const int b = int(blockIdx.x) * int(blockDim.x) + int(threadIdx.x);
-  if (b >= nb) return;
-  const int cx = b % A.bx, cy = (b / A.bx) % A.by, cz = b / (A.bx * A.by);
-  auto live = [&](int x, int y, int z) {
-    if (x < 0 || y < 0 || z < 0 || x >= A.bx || y >= A.by || z >= A.bz) return false;
-    const int k = x + A.bx * (y + A.by * z);
-    bool any = false;
-    for (int c = 0; c < A.nin; ++c) any = any || A.in[c][k] != 0;
-    return any;
+constexpr int kPlanTile = 8, kPlanHalo = 2, kPlanSpan = kPlanTile + 2 * kPlanHalo;
+
+__global__ __launch_bounds__(kPlanTile * kPlanTile * kPlanTile) void k_active_plan(PlanArgs A) {
+  __shared__ uint8_t U[kPlanSpan][kPlanSpan][kPlanSpan];
+  const int t = threadIdx.x;
+  const int ox = blockIdx.x * kPlanTile, oy = blockIdx.y * kPlanTile, oz = blockIdx.z * kPlanTile;
+  for (int k = t; k < kPlanSpan * kPlanSpan * kPlanSpan; k += blockDim.x) {
+    const int hx = k % kPlanSpan, hy = (k / kPlanSpan) % kPlanSpan, hz = k / (kPlanSpan * kPlanSpan);
+    const int x = ox + hx - kPlanHalo, y = oy + hy - kPlanHalo, z = oz + hz - kPlanHalo;
+    uint8_t any = 0;
+    if (x >= 0 && y >= 0 && z >= 0 && x < A.bx && y < A.by && z < A.bz) {
+      const int b = x + A.bx * (y + A.by * z);
+      for (int c = 0; c < A.nin; ++c) any |= A.in[c][b];
+    }
+    U[hz][hy][hx] = any;
+  }
+  __syncthreads();
+  const int tx = t % kPlanTile, ty = (t / kPlanTile) % kPlanTile, tz = t / (kPlanTile * kPlanTile);
+  const int cx = ox + tx, cy = oy + ty, cz = oz + tz;
+  if (cx >= A.bx || cy >= A.by || cz >= A.bz) return;
+  const int b = cx + A.bx * (cy + A.by * cz);
+  auto live = [&](int dx, int dy, int dz) {
+    return U[tz + kPlanHalo + dz][ty + kPlanHalo + dy][tx + kPlanHalo + dx] != 0;
   };
   bool n = false, p = false;
   if (A.transport) {
@@ -96,7 +110,7 @@ __global__ void k_active_plan(PlanArgs A) {
       for (int dz = -2; dz <= 2 && !n; ++dz)
         for (int dy = -2; dy <= 2 && !n; ++dy)
           for (int dx = -2; dx <= 2 && !n; ++dx) {
-            if (!live(cx + dx, cy + dy, cz + dz)) continue;
+            if (!live(dx, dy, dz)) continue;
             p = true;
             n = abs(dx) <= 1 && abs(dy) <= 1 && abs(dz) <= 1;
           }
@@ -111,7 +125,7 @@ __global__ void k_active_plan(PlanArgs A) {
       }
     }
   } else {
-    n = live(cx, cy, cz);
+    n = live(0, 0, 0);
   }
   bool stale = false;
   for (int c = 0; c < A.nout; ++c) stale = stale || A.out[c][b] != 0;
@@ -209,8 +223,9 @@ int fx_active_plan(fx_grid g, float dt, const float* vmax, int32_t transport, in
       ++A.nbox;
     }
   }
-  const int nb = A.bx * A.by * A.bz;
-  hipLaunchKernelGGL(k_active_plan, dim3((nb + 255) / 256), dim3(256), 0, S(stream), A);
+  const dim3 grid((A.bx + kPlanTile - 1) / kPlanTile, (A.by + kPlanTile - 1) / kPlanTile,
+                  (A.bz + kPlanTile - 1) / kPlanTile);
+  hipLaunchKernelGGL(k_active_plan, grid, dim3(kPlanTile * kPlanTile * kPlanTile), 0, S(stream), A);
   FX_LAUNCHED("fx_active_plan");
   return FX_OK;
 }

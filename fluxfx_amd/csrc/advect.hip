@@ -1287,15 +1287,34 @@ __global__ void k_max_init(float* out, int count) {
   if (int(threadIdx.x) < count) out[threadIdx.x] = 0.0f;
 }
 
+// 16-byte loads, four in flight per thread, on a 16-byte aligned array (the
+// tail and unaligned arrays element by element); the maximum does not depend
+// on the order
 __global__ void k_max_abs(MaxArgs A, float* out) {
-  int k = blockIdx.y;
+  const int k = blockIdx.y;
   const float* a = A.a[k];
-  int64_t n = A.n[k];
+  const int64_t n = A.n[k];
   float m = 0.0f;
-  for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += int64_t(gridDim.x) * blockDim.x) {
-    float v = a[i];
-    m = fmaxf(m, isfinite(v) ? fabsf(v) : 3.4e38f);
+  auto acc = [&](float v) { m = fmaxf(m, isfinite(v) ? fabsf(v) : 3.4e38f); };
+  auto acc4 = [&](const float4& v) {
+    acc(v.x);
+    acc(v.y);
+    acc(v.z);
+    acc(v.w);
+  };
+  const int64_t nq = (reinterpret_cast<uintptr_t>(a) & 15u) ? 0 : n / 4;
+  const float4* a4 = reinterpret_cast<const float4*>(a);
+  const int64_t stride = int64_t(gridDim.x) * blockDim.x;
+  int64_t q = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  for (; q + 3 * stride < nq; q += 4 * stride) {
+    const float4 v0 = a4[q], v1 = a4[q + stride], v2 = a4[q + 2 * stride], v3 = a4[q + 3 * stride];
+    acc4(v0);
+    acc4(v1);
+    acc4(v2);
+    acc4(v3);
   }
+  for (; q < nq; q += stride) acc4(a4[q]);
+  for (int64_t i = 4 * nq + int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride) acc(a[i]);
   m = wave_max(m);
   __shared__ float part[4];
   int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
@@ -1683,7 +1702,7 @@ int fx_max_abs(int32_t count, const float* const* arrays, const int64_t* sizes, 
   }
   hipStream_t s = S(stream);
   hipLaunchKernelGGL(k_max_init, dim3(1), dim3(64), 0, s, out, count);
-  hipLaunchKernelGGL(k_max_abs, dim3(512, count), dim3(256), 0, s, A, out);
+  hipLaunchKernelGGL(k_max_abs, dim3(1024, count), dim3(256), 0, s, A, out);
   FX_LAUNCHED("fx_max_abs");
   return FX_OK;
 }
