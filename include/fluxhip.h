@@ -490,8 +490,8 @@ int fx_combustion(int64_t n, float dt, float ignition, float burn_rate, float he
  * 0, NaN included). */
 int fx_active_scan(fx_grid g, const float* field, uint8_t* live, void* stream);
 
-/* Plan of the next scalar pass from the live masks of its inputs (nin) and
- * of the buffers it writes (nout, each 1..4).  Plan bytes: 0 skip the brick,
+/* Plan of the next scalar pass from the live masks of its inputs (nin, 1..4)
+ * and of the buffers it writes (nout, 0..4).  Plan bytes: 0 skip the brick,
  * 1 compute it, 2 store 0 (the outputs are exactly zero there but a buffer may
  * hold other values).  transport = 1 (fx_scalar_advect_active): computed are
  * the bricks within one brick of a live input or of an emitter's reach
