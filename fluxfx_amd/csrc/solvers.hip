@@ -442,6 +442,9 @@ int pcg_loop(const fx_mg_level& L0, int storage, void* rT, void** z_slot, const 
     auto cur_scale = [&]() { return ScaleRef{zscale, float(1.0 / scale), sc_cur, sc_next, rms_div}; };
     // direction p = z*zscale + beta*p from slots (q_new, q_cross, q_old)
     int dir_first = 1, dq_new = q_rz, dq_cross = q_rz, dq_old = q_rz;
+    // quantities of the last update (||r||^2, r.z_old) not needed before the
+    // next r.z: finalised in the same launch as r.z
+    uint32_t pending = 0;
     auto dot_rz = [&](int q) -> int {
       const ScaleRef sc = cur_scale();
       FX_DISPATCH_T(storage, {
@@ -450,7 +453,8 @@ int pcg_loop(const fx_mg_level& L0, int storage, void* rT, void** z_slot, const 
         else
           hipLaunchKernelGGL((k_dot_rz<TV>), kRedGrid, kRedBlock, 0, s, n, v.r, (const TV*)*z_slot, sc, ws, q);
       });
-      ws_finalize(ws, q, 1, s);
+      ws_finalize_mask(ws, pending | (1u << q), s);
+      pending = 0;
       return FX_OK;
     };
     auto direction = [&]() -> int {
@@ -496,8 +500,15 @@ int pcg_loop(const fx_mg_level& L0, int storage, void* rT, void** z_slot, const 
                              (const TV*)v.zold, upd, (TV*)rT, 0, 2);
       });
       FX_LAUNCHED("pcg update");
-      ws_finalize_mask(ws, (1u << 0) | (1u << 2), s);
       ++it;
+      const uint32_t updated = (1u << 0) | (1u << 2);
+      if (it < prm->check_from && it < prm->max_iters) {
+        // read next by the direction (r.z_old) and the next update's fp16
+        // scale (||r||^2), both after the next r.z
+        pending = updated;
+      } else {
+        ws_finalize_mask(ws, updated, s);
+      }
       if (it >= prm->check_from) {
         if (int e = read_residual(ws, ref_q, &rr, &ref2, s)) return e;
         FX_REQUIRE(std::isfinite(rr), FX_ESTATE, "non-finite residual after %d PCG iterations", it);
