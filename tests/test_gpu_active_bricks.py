@@ -127,3 +127,8 @@ def test_fast_velocity_plans_every_brick_and_replaced_fields(device):
         sim.upload(blob.ravel())
         sim.upload_velocity([np.zeros(int(np.prod(sh))) for sh in sim.grid.face_shapes])
     _run(pair, 3, dt, label="replaced")
+    # a write into the tensor in place, announced with invalidate_active_bricks()
+    for sim in pair:
+        sim.temperature.tensor.view(n, n, n)[2:5, 25:29, 4:8] = 0.75
+    pair[1].invalidate_active_bricks()
+    _run(pair, 3, dt, label="written in place")
