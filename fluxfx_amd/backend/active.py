@@ -32,7 +32,9 @@ class ActiveBricks:
 
     @property
     def allocated_bytes(self):
-        return 3 * self.count + 12
+        # three plans, the velocity bound and the live masks of up to seven
+        # scalar fields (density, temperature and fuel pairs, flame)
+        return 10 * self.count + 12
 
     def live(self, field):
         """The field's live mask, scanned from its values when unknown."""
