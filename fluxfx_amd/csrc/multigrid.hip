@@ -272,7 +272,8 @@ __device__ __forceinline__ void ld_span<_Float16, 4>(const _Float16* __restrict_
 
 template <typename T, int FOP, int MODE, int W>
 __global__ __launch_bounds__(256) void k_prolong_q(LevelDev F, LevelDev C, const uint8_t* __restrict__ cact,
-                                                   T* __restrict__ p, const T* __restrict__ coarse, int nu) {
+                                                   T* __restrict__ p, const T* __restrict__ coarse, int nu,
+                                                   int zero_in) {
   constexpr int CW = W / 2 + 2;
   static_assert(9 * CW <= 64, "activity window must fit one 64-bit mask");
   const int u = xcd_tile(blockIdx.x, gridDim.x) * 256 + threadIdx.x;
@@ -341,7 +342,8 @@ __global__ __launch_bounds__(256) void k_prolong_q(LevelDev F, LevelDev C, const
       const int64_t i = idx3(x0, y, z, F.nx, F.ny);
       uint32_t fl[W];
       vec_flags<W>(F, FOP, x0, y, z, i, fl);
-      FV<W> r = VecIO<T, W>::ld(p, i);
+      FV<W> r{};
+      if (!zero_in) r = VecIO<T, W>::ld(p, i);  // zero_in: p starts from 0 (not read)
       float corr[W];
       if (MODE == FX_PROLONG_CONSTANT) {
 #pragma unroll
@@ -476,7 +478,7 @@ __global__ void k_prolong(LevelDev F, LevelDev C, const T* in, const T* __restri
   int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y * blockDim.y + threadIdx.y, z = blockIdx.z;
   if (x >= F.nx || y >= F.ny) return;
   int64_t i = idx3(x, y, z, F.nx, F.ny);
-  out[i] = from_f<T>(prolong_cell<T, FOP, COP, MODE>(F, C, x, y, z, i, to_f(in[i]), coarse));
+  out[i] = from_f<T>(prolong_cell<T, FOP, COP, MODE>(F, C, x, y, z, i, in ? to_f(in[i]) : 0.0f, coarse));
 }
 
 // p = scale * M rhs, one wavefront per row, FP64 accumulation
@@ -1123,8 +1125,9 @@ static int launch_prolong(const fx_mg_config* c, int l, const void* in, const vo
   const int W = storage_width(c->storage);
   // the vector kernel reads W/2 coarse values at a time: an 8-byte aligned coarse array
   const bool coarse_aligned = (reinterpret_cast<uintptr_t>(coarse) & 7u) == 0;
-  if (in == out && fcells >= kQuadMinCells && coarse_known && C.op != FX_OP_FLAGS && coarse_aligned &&
+  if ((in == out || !in) && fcells >= kQuadMinCells && coarse_known && C.op != FX_OP_FLAGS && coarse_aligned &&
       vec_ok(F, out, nullptr, nullptr, W)) {
+    const int zero_in = in ? 0 : 1;
     const int nu = int(int64_t(F.nx / W) * C.ny * C.nz);
     dim3 grid((nu + 255) / 256);
     const uint8_t* cact = op_has_edges(C.op) ? C.active : nullptr;
@@ -1132,7 +1135,7 @@ static int launch_prolong(const fx_mg_config* c, int l, const void* in, const vo
   FX_DISPATCH_T(c->storage, {                                                                           \
     constexpr int WV = vec_width<TV>();                                                                 \
     hipLaunchKernelGGL((k_prolong_q<TV, FO, MO, WV>), grid, dim3(256), 0, s, DF, DC, cact, (TV*)out,    \
-                       (const TV*)coarse, nu);                                                          \
+                       (const TV*)coarse, nu, zero_in);                                                 \
   })
     if (F.op == FX_OP_FLAGS) {
       if (mode == FX_PROLONG_CONSTANT) { FX_PROQ(FX_OP_FLAGS, FX_PROLONG_CONSTANT); }
@@ -1412,8 +1415,7 @@ int fx_mg_restrict(const fx_mg_config* cfg, int32_t index, const void* fine, voi
 int fx_mg_prolong(const fx_mg_config* cfg, int32_t index, const void* fine_in, const void* coarse, void* fine_out,
                   void* stream) {
   if (int e = check_config(cfg)) return e;
-  FX_REQUIRE(index >= 0 && index + 1 < cfg->levels && fine_in && coarse && fine_out, FX_EINVAL,
-             "bad level or arrays");
+  FX_REQUIRE(index >= 0 && index + 1 < cfg->levels && coarse && fine_out, FX_EINVAL, "bad level or arrays");
   return launch_prolong(cfg, index, fine_in, coarse, fine_out, S(stream));
 }
 

@@ -607,8 +607,7 @@ int fx_learned_pcg_solve(const fx_unet* net, fx_mg_config* mg, float* x, const f
       sm.level[0].rhs = rT;
       if (int e = fx_mg_restrict(&sm, 0, rT, rc, st)) return e;
       if (int e = unet_forward(net, rc, solid, ws, post_scale, 1, zc, st)) return e;
-      if (int e = fx_fill((float*)zs, n0, 0.0f, st)) return e;
-      if (int e = fx_mg_prolong(&sm, 0, zs, zc, zs, st)) return e;
+      if (int e = fx_mg_prolong(&sm, 0, nullptr, zc, zs, st)) return e;
     } else {
       if (int e = unet_forward(net, rT, solid, ws, post_scale, 1, (float*)zs, st)) return e;
     }

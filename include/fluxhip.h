@@ -275,7 +275,8 @@ int fx_mg_residual_restrict(fx_mg_config* cfg, int32_t index, float inv_dt, void
 int fx_mg_restrict(const fx_mg_config* cfg, int32_t index, const void* fine, void* coarse, void* stream);
 
 /* fine out = fine in + prolongation(coarse) (shaders/mg_prolong.glsl:2-15,
- * shaders/solid_mg_prolong.glsl:1-7). */
+ * shaders/solid_mg_prolong.glsl:1-7); fine_in NULL: a zero fine input (out
+ * = the prolonged correction, inactive cells 0). */
 int fx_mg_prolong(const fx_mg_config* cfg, int32_t index, const void* fine_in, const void* coarse,
                   void* fine_out, void* stream);
 
