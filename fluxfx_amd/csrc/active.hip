@@ -99,7 +99,7 @@ __global__ __launch_bounds__(kPlanTile * kPlanTile * kPlanTile) void k_active_pl
   if (cx >= A.bx || cy >= A.by || cz >= A.bz) return;
   const int b = cx + A.bx * (cy + A.by * cz);
   auto live = [&](int dx, int dy, int dz) {
-    return U[tz + kPlanHalo + dz][ty + kPlanHalo + dy][tx + kPlanHalo + dx] != 0;
+    return U[tz + kPlanHalo + dz][ty + kPlanHalo + dy][tx + kPlanHalo + dx];
   };
   bool n = false, p = false;
   if (A.transport) {
@@ -107,13 +107,20 @@ __global__ __launch_bounds__(kPlanTile * kPlanTile * kPlanTile) void k_active_pl
     if (!(d <= 2.5f)) {
       n = p = true;
     } else {
-      for (int dz = -2; dz <= 2 && !n; ++dz)
-        for (int dy = -2; dy <= 2 && !n; ++dy)
-          for (int dx = -2; dx <= 2 && !n; ++dx) {
-            if (!live(dx, dy, dz)) continue;
-            p = true;
-            n = abs(dx) <= 1 && abs(dy) <= 1 && abs(dz) <= 1;
+      // all 125 reads issued together (no early exit): OR within one and two bricks
+      uint32_t near = 0, far = 0;
+#pragma unroll
+      for (int dz = -2; dz <= 2; ++dz)
+#pragma unroll
+        for (int dy = -2; dy <= 2; ++dy)
+#pragma unroll
+          for (int dx = -2; dx <= 2; ++dx) {
+            const uint32_t v = live(dx, dy, dz);
+            far |= v;
+            if (abs(dx) <= 1 && abs(dy) <= 1 && abs(dz) <= 1) near |= v;
           }
+      n = near != 0;
+      p = far != 0;
       for (int s = 0; s < A.nbox; ++s) {
         const int* e = A.box[s];
         const int ox = cx < e[0] ? e[0] - cx : (cx > e[1] ? cx - e[1] : 0);
@@ -125,7 +132,7 @@ __global__ __launch_bounds__(kPlanTile * kPlanTile * kPlanTile) void k_active_pl
       }
     }
   } else {
-    n = live(0, 0, 0);
+    n = live(0, 0, 0) != 0;
   }
   bool stale = false;
   for (int c = 0; c < A.nout; ++c) stale = stale || A.out[c][b] != 0;
