@@ -71,13 +71,13 @@ struct PlanArgs {
   int box[kMaxEmitterBoxes][6];  // emitter bricks, inclusive [x0, x1] x [y0, y1] x [z0, z1]
 };
 
-// A block owns an 8 x 8 x 8 tile of bricks, one thread per brick, and stages
+// A block owns a 4 x 4 x 4 tile of bricks, one thread per brick, and stages
 // the union of the input masks over the tile and two bricks around it in LDS
 // (0 outside the grid).  The thread also clears the live bytes of the output
 // buffers of every brick the pass writes (the pass sets them again where it
 // stores nonzero values), so each output's mask stays exact: a skipped brick
 // keeps its zeros, a written one is rewritten.
-constexpr int kPlanTile = 8, kPlanHalo = 2, kPlanSpan = kPlanTile + 2 * kPlanHalo;
+constexpr int kPlanTile = 4, kPlanHalo = 2, kPlanSpan = kPlanTile + 2 * kPlanHalo;
 
 __global__ __launch_bounds__(kPlanTile * kPlanTile * kPlanTile) void k_active_plan(PlanArgs A) {
   __shared__ uint8_t U[kPlanSpan][kPlanSpan][kPlanSpan];
