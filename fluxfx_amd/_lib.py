@@ -234,6 +234,7 @@ SIGNATURES = {
     "fx_combustion": ([I64, F32, F32, F32, F32, F32, P, P, P, P, P, P, P, P], C.c_int),
     "fx_max_abs": ([I32, C.POINTER(P), C.POINTER(I64), P, P], C.c_int),
     "fx_active_scan": ([fx_grid, P, P, P], C.c_int),
+    "fx_divergence_bound": ([fx_grid, P, P, P, P, P, P, P, P, P, P, P], C.c_int),
     "fx_active_plan": ([fx_grid, F32, P, I32, I32, C.POINTER(P), I32, C.POINTER(P), C.POINTER(fx_sources), P, P, P],
                        C.c_int),
     "fx_scalar_predict_bounds_active": ([fx_grid, F32, P, P, P, I32, C.POINTER(P), C.POINTER(P), C.POINTER(P),

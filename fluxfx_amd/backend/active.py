@@ -28,13 +28,17 @@ class ActiveBricks:
         self.write = device.zeros(self.count, "uint8")
         self.react = device.zeros(self.count, "uint8")
         self.vmax = device.zeros(3)
+        # the projector's divergence of the projected velocity also writes
+        # vmax (fx_divergence_bound) and sets `bound_ready`; else fx_max_abs
+        self.partials = device.zeros(3 * -(-prod(grid.shape) // 1024))
+        self.bound_ready = False
         self._sizes = (C.c_int64 * 3)(*[prod(s) for s in grid.face_shapes])
 
     @property
     def allocated_bytes(self):
-        # three plans, the velocity bound and the live masks of up to seven
-        # scalar fields (density, temperature and fuel pairs, flame)
-        return 10 * self.count + 12
+        # three plans, the velocity bound and its partials, and the live masks
+        # of up to seven scalar fields (density, temperature and fuel pairs, flame)
+        return 10 * self.count + 12 + 4 * self.partials.numel()
 
     def live(self, field):
         """The field's live mask, scanned from its values when unknown."""
@@ -50,8 +54,10 @@ class ActiveBricks:
     def transport_plan(self, dt, velocity, inputs, outputs, sources):
         """Plan of the scalar predictor (self.predict) and transport
         (self.write) of `inputs` into `outputs` by `velocity` over dt."""
-        arrays = (C.c_void_p * 3)(*[ptr(f) for f in velocity])
-        _lib.call("fx_max_abs", 3, arrays, self._sizes, self.vmax.data_ptr(), self.device.stream)
+        if not self.bound_ready:
+            arrays = (C.c_void_p * 3)(*[ptr(f) for f in velocity])
+            _lib.call("fx_max_abs", 3, arrays, self._sizes, self.vmax.data_ptr(), self.device.stream)
+        self.bound_ready = False
         _lib.call("fx_active_plan", self._g, float(dt), self.vmax.data_ptr(), 1, len(inputs), self._masks(inputs),
                   len(outputs), self._masks(outputs), sources, self.predict.data_ptr(), self.write.data_ptr(),
                   self.device.stream)
@@ -72,4 +78,4 @@ class ActiveBricks:
                 "transport_zero": share(self.write, 2), "combustion": share(self.react, 1)}
 
     def close(self):
-        self.predict = self.write = self.react = self.vmax = None
+        self.predict = self.write = self.react = self.vmax = self.partials = None

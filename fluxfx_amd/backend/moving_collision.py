@@ -127,7 +127,12 @@ class MovingPressureProjector(SolidPressureProjector):
         _lib.call("fx_enforce_walls_moving", self._g, self.flags_ptr, *(ptr(f) for f in fields), *self._walls(),
                   *(ptr(f) for f in outputs), self.device.stream)
 
-    def compute_divergence(self, fields, target):
+    def compute_divergence(self, fields, target, bound=None):
+        if bound is not None:
+            _lib.call("fx_divergence_bound", self._g, self.flags_ptr, *(ptr(f) for f in fields), *self._walls(),
+                      ptr(target), bound.partials.data_ptr(), bound.vmax.data_ptr(), self.device.stream)
+            bound.bound_ready = True
+            return
         _lib.call("fx_divergence_moving", self._g, self.flags_ptr, *(ptr(f) for f in fields), *self._walls(),
                   ptr(target), self.device.stream)
 
@@ -189,7 +194,12 @@ class MovingToleranceProjector(ToleranceProjector):
         _lib.call("fx_enforce_walls_moving", self._g, self.flags_ptr, *(ptr(f) for f in fields), *self._walls(),
                   *(ptr(f) for f in outputs), self.device.stream)
 
-    def compute_divergence(self, fields, target):
+    def compute_divergence(self, fields, target, bound=None):
+        if bound is not None:
+            _lib.call("fx_divergence_bound", self._g, self.flags_ptr, *(ptr(f) for f in fields), *self._walls(),
+                      ptr(target), bound.partials.data_ptr(), bound.vmax.data_ptr(), self.device.stream)
+            bound.bound_ready = True
+            return
         _lib.call("fx_divergence_moving", self._g, self.flags_ptr, *(ptr(f) for f in fields), *self._walls(),
                   ptr(target), self.device.stream)
 

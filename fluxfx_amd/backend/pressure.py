@@ -18,6 +18,9 @@ from .engine import MGLevels
 class PressureProjector:
     solver_name = "JACOBI"
 
+    # ActiveBricks of the owning step: project() then writes its velocity bound
+    velocity_bound = None
+
     def __init__(self, grid, settings=None, device=None, *, solids=None):
         self.grid = grid
         self.settings = settings or PressureSettings()
@@ -74,7 +77,13 @@ class PressureProjector:
         _lib.call("fx_enforce_walls", self._g, self.flags_ptr, *(ptr(f) for f in fields),
                   *(ptr(f) for f in outputs), self.device.stream)
 
-    def compute_divergence(self, fields, target):
+    def compute_divergence(self, fields, target, bound=None):
+        """bound: an ActiveBricks whose velocity bound the same pass writes."""
+        if bound is not None:
+            _lib.call("fx_divergence_bound", self._g, self.flags_ptr, *(ptr(f) for f in fields), None, None, None,
+                      ptr(target), bound.partials.data_ptr(), bound.vmax.data_ptr(), self.device.stream)
+            bound.bound_ready = True
+            return
         _lib.call("fx_divergence", self._g, self.flags_ptr, *(ptr(f) for f in fields), ptr(target),
                   self.device.stream)
 
@@ -99,7 +108,9 @@ class PressureProjector:
             self.pressure, self.spare = self.spare, self.pressure
         self.solve_pressure(solve_dt, previous_solve_dt)
         self.apply_gradient(fields, outputs, solve_dt)
-        self.compute_divergence(outputs, self.after)
+        # the projected velocity's divergence (and, for the active-brick scalar
+        # passes, its bound max |U|, |V|, |W| from the same loads)
+        self.compute_divergence(outputs, self.after, bound=self.velocity_bound)
         self.last_dt = dt
         self.ready = True
 

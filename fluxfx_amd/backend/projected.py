@@ -93,6 +93,7 @@ class DenseProjectedSmoke(DenseThermalAdvection):
             self.combustion = CombustionPasses(self.grid, self.device)
         if active_bricks:
             self.active = ActiveBricks(self.grid, self.device, self._g)
+            self.projector.velocity_bound = self.active
         self.reset()
 
     @property

@@ -241,37 +241,98 @@ __device__ __forceinline__ uint32_t row_flags(const uint8_t* __restrict__ flags,
   return w;
 }
 
-// bu/bv/bw (nullable): the wall velocity closed faces read (moving colliders)
+__device__ __forceinline__ float bound_abs(float v) { return isfinite(v) ? fabsf(v) : 3.4e38f; }
+
+// bu/bv/bw (nullable): the wall velocity closed faces read (moving colliders).
+// part (nullable): the block's max |U|, |V|, |W| over the face values its
+// lanes load, which over all lanes are every face of the three arrays (the
+// lanes' U rows end at the next lane's first face, the V and W loads take
+// the faces above the lane's cells as well), at part[axis * blocks + block]
+// (non-finite values count as 3.4e38, as in fx_max_abs).
 __global__ __launch_bounds__(256) void k_divergence_q(int nx, int ny, int nz, float ix, float iy, float iz,
                                                       const uint8_t* __restrict__ flags, const float* __restrict__ u,
                                                       const float* __restrict__ v, const float* __restrict__ w,
                                                       const float* __restrict__ bu, const float* __restrict__ bv,
-                                                      const float* __restrict__ bw, float* __restrict__ div, int nq) {
-  const int q = xcd_row_tile(blockIdx.x, gridDim.x) * 256 + threadIdx.x;
-  if (q >= nq) return;
-  const int qx = nx >> 2, row = q / qx, x0 = (q - row * qx) << 2, z = row / ny, y = row - z * ny;
-  const int c = row * nx + x0, iu = row * (nx + 1) + x0, iv = (y + (ny + 1) * z) * nx + x0, sz = nx * ny;
-  const uint32_t fw = row_flags(flags, x0, y, z, nx, ny, nz, c);
-  float uu[5];
-#pragma unroll
-  for (int k = 0; k < 5; ++k) uu[k] = u[iu + k];
-  const float4 v0 = ld4f(v, iv), v1 = ld4f(v, iv + nx), w0 = ld4f(w, c), w1 = ld4f(w, c + sz);
-  float d[4];
-#pragma unroll
-  for (int k = 0; k < 4; ++k) {
-    const uint8_t f = uint8_t(fw >> (8 * k));
-    d[k] = 0.0f;
-    if (f & FLAG_FLUID) {
-      const float a0 = (f & FLAG_XLO) ? uu[k] : closed_value(bu, iu + k),
-                  a1 = (f & FLAG_XHI) ? uu[k + 1] : closed_value(bu, iu + k + 1);
-      const float b0 = (f & FLAG_YLO) ? comp(v0, k) : closed_value(bv, iv + k),
-                  b1 = (f & FLAG_YHI) ? comp(v1, k) : closed_value(bv, iv + nx + k);
-      const float c0 = (f & FLAG_ZLO) ? comp(w0, k) : closed_value(bw, c + k),
-                  c1 = (f & FLAG_ZHI) ? comp(w1, k) : closed_value(bw, c + sz + k);
-      d[k] = (a1 - a0) * ix + (b1 - b0) * iy + (c1 - c0) * iz;
+                                                      const float* __restrict__ bw, float* __restrict__ div, int nq,
+                                                      float* __restrict__ part) {
+  const int tile = xcd_row_tile(blockIdx.x, gridDim.x);
+  const int q = tile * 256 + threadIdx.x;
+  float mu = 0.0f, mv = 0.0f, mw = 0.0f;
+  if (q < nq) {
+    const int qx = nx >> 2, row = q / qx, x0 = (q - row * qx) << 2, z = row / ny, y = row - z * ny;
+    const int c = row * nx + x0, iu = row * (nx + 1) + x0, iv = (y + (ny + 1) * z) * nx + x0, sz = nx * ny;
+    const uint32_t fw = row_flags(flags, x0, y, z, nx, ny, nz, c);
+    float uu[5];
+  #pragma unroll
+    for (int k = 0; k < 5; ++k) uu[k] = u[iu + k];
+    const float4 v0 = ld4f(v, iv), v1 = ld4f(v, iv + nx), w0 = ld4f(w, c), w1 = ld4f(w, c + sz);
+    float d[4];
+  #pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const uint8_t f = uint8_t(fw >> (8 * k));
+      d[k] = 0.0f;
+      if (f & FLAG_FLUID) {
+        const float a0 = (f & FLAG_XLO) ? uu[k] : closed_value(bu, iu + k),
+                    a1 = (f & FLAG_XHI) ? uu[k + 1] : closed_value(bu, iu + k + 1);
+        const float b0 = (f & FLAG_YLO) ? comp(v0, k) : closed_value(bv, iv + k),
+                    b1 = (f & FLAG_YHI) ? comp(v1, k) : closed_value(bv, iv + nx + k);
+        const float c0 = (f & FLAG_ZLO) ? comp(w0, k) : closed_value(bw, c + k),
+                    c1 = (f & FLAG_ZHI) ? comp(w1, k) : closed_value(bw, c + sz + k);
+        d[k] = (a1 - a0) * ix + (b1 - b0) * iy + (c1 - c0) * iz;
+      }
+    }
+    st4f(div, c, make_float4(d[0], d[1], d[2], d[3]));
+    if (part) {
+  #pragma unroll
+      for (int k = 0; k < 5; ++k) mu = fmaxf(mu, bound_abs(uu[k]));
+  #pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        mv = fmaxf(mv, fmaxf(bound_abs(comp(v0, k)), bound_abs(comp(v1, k))));
+        mw = fmaxf(mw, fmaxf(bound_abs(comp(w0, k)), bound_abs(comp(w1, k))));
+      }
     }
   }
-  st4f(div, c, make_float4(d[0], d[1], d[2], d[3]));
+  if (part) {  // uniform over the grid
+    __shared__ float m[3][4];
+    mu = wave_max(mu);
+    mv = wave_max(mv);
+    mw = wave_max(mw);
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    if (lane == 0) {
+      m[0][wv] = mu;
+      m[1][wv] = mv;
+      m[2][wv] = mw;
+    }
+    __syncthreads();
+    if (threadIdx.x < 3) {
+      const int a = threadIdx.x;
+      part[a * gridDim.x + blockIdx.x] = fmaxf(fmaxf(m[a][0], m[a][1]), fmaxf(m[a][2], m[a][3]));
+    }
+  }
+}
+
+// vmax[axis] = max over the `blocks` block partials of that axis (one block of
+// 1024 threads per axis, four loads in flight per thread)
+__global__ __launch_bounds__(1024) void k_bound_finish(const float* __restrict__ part, int blocks,
+                                                       float* __restrict__ vmax) {
+  __shared__ float m[16];
+  const float* p = part + int64_t(blockIdx.x) * blocks;
+  float r0 = 0.0f, r1 = 0.0f, r2 = 0.0f, r3 = 0.0f;
+  int b = threadIdx.x;
+  for (; b + 3 * 1024 < blocks; b += 4 * 1024) {
+    r0 = fmaxf(r0, p[b]);
+    r1 = fmaxf(r1, p[b + 1024]);
+    r2 = fmaxf(r2, p[b + 2048]);
+    r3 = fmaxf(r3, p[b + 3072]);
+  }
+  for (; b < blocks; b += 1024) r0 = fmaxf(r0, p[b]);
+  float r = wave_max(fmaxf(fmaxf(r0, r1), fmaxf(r2, r3)));
+  if ((threadIdx.x & 63) == 0) m[threadIdx.x >> 6] = r;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int k = 1; k < 16; ++k) r = fmaxf(r, m[k]);
+    vmax[blockIdx.x] = fmaxf(r, m[0]);
+  }
 }
 
 // Gradient correction of the faces at the low sides of the lane's cells (and
@@ -518,7 +579,7 @@ int fx_divergence(fx_grid g, const uint8_t* flags, const float* u, const float* 
     const int nq = g.nx / 4 * g.ny * g.nz;
     hipLaunchKernelGGL(k_divergence_q, dim3((nq + 255) / 256), dim3(256), 0, S(stream), g.nx, g.ny, g.nz,
                        g.nx / g.ex, g.ny / g.ey, g.nz / g.ez, flags, u, v, w, (const float*)nullptr,
-                       (const float*)nullptr, (const float*)nullptr, div, nq);
+                       (const float*)nullptr, (const float*)nullptr, div, nq, (float*)nullptr);
   } else {
     dim3 gr = grid3(g.nx, g.ny, g.nz, kBlock);
     hipLaunchKernelGGL(k_divergence, gr, kBlock, 0, S(stream), g.nx, g.ny, g.nz, g.nx / g.ex, g.ny / g.ey,
@@ -529,6 +590,33 @@ int fx_divergence(fx_grid g, const uint8_t* flags, const float* u, const float* 
   return FX_OK;
 }
 
+int fx_divergence_bound(fx_grid g, const uint8_t* flags, const float* u, const float* v, const float* w,
+                        const float* bu, const float* bv, const float* bw, float* div, float* partials, float* vmax,
+                        void* stream) {
+  if (int e = check_grid(g)) return e;
+  FX_REQUIRE(u && v && w && div && partials && vmax, FX_EINVAL, "null array");
+  FX_REQUIRE((bu && bv && bw) || (!bu && !bv && !bw), FX_EINVAL, "divergence bound: all or none of the walls");
+  FX_REQUIRE(!bu || flags, FX_EINVAL, "divergence bound: walls need the flags");
+  hipStream_t s = S(stream);
+  if (rows_vec_ok(g, flags, v, w, div)) {
+    const int nq = g.nx / 4 * g.ny * g.nz, blocks = (nq + 255) / 256;
+    hipLaunchKernelGGL(k_divergence_q, dim3(blocks), dim3(256), 0, s, g.nx, g.ny, g.nz, g.nx / g.ex, g.ny / g.ey,
+                       g.nz / g.ez, flags, u, v, w, bu, bv, bw, div, nq, partials);
+    hipLaunchKernelGGL(k_bound_finish, dim3(3), dim3(1024), 0, s, (const float*)partials, blocks, vmax);
+    FX_LAUNCHED("fx_divergence_bound");
+    return FX_OK;
+  }
+  // one-cell form: the divergence, then the bound from the arrays
+  dim3 gr = grid3(g.nx, g.ny, g.nz, kBlock);
+  hipLaunchKernelGGL(k_divergence, gr, kBlock, 0, s, g.nx, g.ny, g.nz, g.nx / g.ex, g.ny / g.ey, g.nz / g.ez, flags,
+                     u, v, w, bu, bv, bw, div);
+  FX_LAUNCHED("fx_divergence_bound (one-cell)");
+  const float* arrays[3] = {u, v, w};
+  const int64_t sizes[3] = {int64_t(g.nx + 1) * g.ny * g.nz, int64_t(g.nx) * (g.ny + 1) * g.nz,
+                            int64_t(g.nx) * g.ny * (g.nz + 1)};
+  return fx_max_abs(3, arrays, sizes, vmax, stream);
+}
+
 int fx_divergence_moving(fx_grid g, const uint8_t* flags, const float* u, const float* v, const float* w,
                          const float* bu, const float* bv, const float* bw, float* div, void* stream) {
   if (int e = check_grid(g)) return e;
@@ -536,7 +624,7 @@ int fx_divergence_moving(fx_grid g, const uint8_t* flags, const float* u, const 
   if (rows_vec_ok(g, flags, v, w, div)) {
     const int nq = g.nx / 4 * g.ny * g.nz;
     hipLaunchKernelGGL(k_divergence_q, dim3((nq + 255) / 256), dim3(256), 0, S(stream), g.nx, g.ny, g.nz,
-                       g.nx / g.ex, g.ny / g.ey, g.nz / g.ez, flags, u, v, w, bu, bv, bw, div, nq);
+                       g.nx / g.ex, g.ny / g.ey, g.nz / g.ez, flags, u, v, w, bu, bv, bw, div, nq, (float*)nullptr);
   } else {
     dim3 gr = grid3(g.nx, g.ny, g.nz, kBlock);
     hipLaunchKernelGGL(k_divergence, gr, kBlock, 0, S(stream), g.nx, g.ny, g.nz, g.nx / g.ex, g.ny / g.ey,

@@ -141,6 +141,16 @@ int fx_enforce_walls_moving(fx_grid g, const uint8_t* flags, const float* u, con
                             void* stream);
 int fx_divergence_moving(fx_grid g, const uint8_t* flags, const float* u, const float* v, const float* w,
                          const float* bu, const float* bv, const float* bw, float* div, void* stream);
+
+/* fx_divergence (bu/bv/bw NULL) or fx_divergence_moving, plus vmax[0..2] =
+ * max |U|, |V|, |W| over every face of the three arrays (device floats,
+ * non-finite values as 3.4e38, as fx_max_abs): the velocity bound of the
+ * active-brick plans taken from the loads the divergence makes anyway (no
+ * reference counterpart).  partials: 3 * ceil(nx*ny*nz / 1024) floats of
+ * scratch. */
+int fx_divergence_bound(fx_grid g, const uint8_t* flags, const float* u, const float* v, const float* w,
+                        const float* bu, const float* bv, const float* bw, float* div, float* partials, float* vmax,
+                        void* stream);
 int fx_gradient_moving(fx_grid g, const uint8_t* flags, const float* p, const float* u, const float* v,
                        const float* w, const float* bu, const float* bv, const float* bw, float* uo, float* vo,
                        float* wo, float scale, void* stream);

@@ -132,3 +132,49 @@ def test_fast_velocity_plans_every_brick_and_replaced_fields(device):
         sim.temperature.tensor.view(n, n, n)[2:5, 25:29, 4:8] = 0.75
     pair[1].invalidate_active_bricks()
     _run(pair, 3, dt, label="written in place")
+
+
+@pytest.mark.parametrize("shape", [(48, 32, 40), (30, 20, 18)])
+@pytest.mark.parametrize("walls", [False, True])
+def test_divergence_bound_equals_divergence_and_max_abs(device, shape, walls):
+    """fx_divergence_bound: the divergence of fx_divergence(_moving) bit for bit
+    and max |U|, |V|, |W| over the whole arrays (vector path; a grid with
+    nx % 4 != 0 takes the one-cell path)."""
+    import ctypes as C
+    from fluxfx_amd import _lib
+    from fluxfx_amd.backend.device import grid_struct
+    from fluxfx_amd.physics.config import GridSpec
+    torch = device.torch
+    print(f"divergence bound: {shape}, walls={walls}", flush=True)
+    grid = GridSpec(shape, (1.0, 0.8, 1.2))
+    g = grid_struct(grid)
+    nx, ny, nz = shape
+    rng = np.random.default_rng(3)
+    solid = (rng.random((nz, ny, nx)) < 0.1).astype(np.uint8)
+    solid_t = torch.from_numpy(solid.ravel()).to(device.device)
+    flags = torch.zeros(nx * ny * nz, dtype=torch.uint8, device=device.device)
+    _lib.call("fx_build_flags", g, solid_t.data_ptr(), flags.data_ptr(), device.stream)
+    vel = [torch.from_numpy(rng.uniform(-3, 3, int(np.prod(s))).astype(np.float32)).to(device.device)
+           for s in grid.face_shapes]
+    vel[1][17] = -7.5  # a known maximum on V
+    wall = [torch.from_numpy(rng.uniform(-1, 1, int(np.prod(s))).astype(np.float32)).to(device.device)
+            for s in grid.face_shapes] if walls else [None, None, None]
+    wp = [w.data_ptr() if w is not None else None for w in wall]
+    n = nx * ny * nz
+    div_ref = torch.empty(n, device=device.device)
+    div = torch.empty(n, device=device.device)
+    part = torch.zeros(3 * -(-n // 1024), device=device.device)
+    vmax = torch.zeros(3, device=device.device)
+    if walls:
+        _lib.call("fx_divergence_moving", g, flags.data_ptr(), *(t.data_ptr() for t in vel), *wp,
+                  div_ref.data_ptr(), device.stream)
+    else:
+        _lib.call("fx_divergence", g, flags.data_ptr(), *(t.data_ptr() for t in vel), div_ref.data_ptr(),
+                  device.stream)
+    _lib.call("fx_divergence_bound", g, flags.data_ptr(), *(t.data_ptr() for t in vel), *wp, div.data_ptr(),
+              part.data_ptr(), vmax.data_ptr(), device.stream)
+    torch.cuda.synchronize()
+    assert torch.equal(div, div_ref)
+    want = torch.stack([t.abs().max() for t in vel])
+    assert torch.equal(vmax, want), (vmax.tolist(), want.tolist())
+    assert float(vmax[1]) == 7.5
